@@ -1,0 +1,207 @@
+// Shared device/host helpers of the gfx950 kernel library (libgkomi.so).
+//
+// Every kernel in this library is written for CDNA4 (gfx950): wave64,
+// 256-thread workgroups (4 waves) unless stated otherwise, DPP / ds_swizzle
+// cross-lane reductions, and single-pass cross-workgroup combines through a
+// ticket counter (the last workgroup to finish reduces the per-workgroup
+// partials in a fixed order, so every reduction is deterministic).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "gkomi.h"
+
+namespace gkomi {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+// ---------------------------------------------------------------- errors ----
+
+void set_error(const char* fmt, ...);
+int fail(int code, const char* what);
+int hip_fail(hipError_t err, const char* what);
+
+#define GKOMI_CHECK_HIP(expr)                                   \
+    do {                                                        \
+        hipError_t gkomi_err_ = (expr);                         \
+        if (gkomi_err_ != hipSuccess) {                         \
+            return ::gkomi::hip_fail(gkomi_err_, #expr);        \
+        }                                                       \
+    } while (0)
+
+#define GKOMI_CHECK_LAUNCH(what)                                \
+    do {                                                        \
+        hipError_t gkomi_err_ = hipGetLastError();              \
+        if (gkomi_err_ != hipSuccess) {                         \
+            return ::gkomi::hip_fail(gkomi_err_, what);         \
+        }                                                       \
+    } while (0)
+
+#define GKOMI_REQUIRE(cond, code, msg)                          \
+    do {                                                        \
+        if (!(cond)) {                                          \
+            return ::gkomi::fail(code, msg);                    \
+        }                                                       \
+    } while (0)
+
+inline hipStream_t stream_of(const gkomi_ctx* ctx)
+{
+    return ctx ? static_cast<hipStream_t>(ctx->stream) : nullptr;
+}
+
+// Sets the context's device current; returns non-zero on failure.
+int enter(const gkomi_ctx* ctx);
+
+// Number of compute units of the current device (cached per device).
+int num_cus();
+
+__host__ __device__ inline int64_t ceildiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Grid for `items` work items of `per_block` each, capped so that a
+// grid-stride loop covers the rest (>= 64 workgroups per CU is plenty).
+inline unsigned grid_for(int64_t items, int64_t per_block, int64_t cap_per_cu = 64)
+{
+    int64_t g = ceildiv(items, per_block);
+    int64_t cap = static_cast<int64_t>(num_cus()) * cap_per_cu;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return static_cast<unsigned>(g);
+}
+
+// --------------------------------------------------------- value traits ----
+
+template <typename T>
+struct vt;
+template <>
+struct vt<double> {
+    static constexpr int bytes = 8;
+};
+template <>
+struct vt<float> {
+    static constexpr int bytes = 4;
+};
+
+// ------------------------------------------------ cross-lane reductions ----
+// DPP controls (GFX9 encoding): quad_perm [1,0,3,2] = 0xB1, [2,3,0,1] = 0x4E,
+// row_half_mirror = 0x141, row_mirror = 0x140.  ds_swizzle bit-mask mode
+// with xor_mask 0x10 (offset 0x401F) swaps the 16-lane halves of each
+// 32-lane group.  Each step leaves every lane of the group holding the
+// group's sum, so the reductions below are "all-reduce" within the group.
+
+template <int Ctrl>
+__device__ __forceinline__ int dpp_i32(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, Ctrl, 0xF, 0xF, false);
+}
+
+template <int Ctrl>
+__device__ __forceinline__ float dpp(float v)
+{
+    return __int_as_float(dpp_i32<Ctrl>(__float_as_int(v)));
+}
+
+template <int Ctrl>
+__device__ __forceinline__ double dpp(double v)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = dpp_i32<Ctrl>(static_cast<int>(bits & 0xffffffffll));
+    const int hi = dpp_i32<Ctrl>(static_cast<int>(bits >> 32));
+    return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                                (static_cast<unsigned int>(lo)));
+}
+
+__device__ __forceinline__ float swz16(float v)
+{
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
+}
+
+__device__ __forceinline__ double swz16(double v)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_swizzle(static_cast<int>(bits & 0xffffffffll), 0x401F);
+    const int hi = __builtin_amdgcn_ds_swizzle(static_cast<int>(bits >> 32), 0x401F);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                                (static_cast<unsigned int>(lo)));
+}
+
+// Sum over aligned groups of G consecutive lanes (G a power of two <= 64);
+// every lane of a group receives the group's sum.
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v)
+{
+    if constexpr (G >= 2) v += dpp<0xB1>(v);
+    if constexpr (G >= 4) v += dpp<0x4E>(v);
+    if constexpr (G >= 8) v += dpp<0x141>(v);
+    if constexpr (G >= 16) v += dpp<0x140>(v);
+    if constexpr (G >= 32) v += swz16(v);
+    if constexpr (G >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+    return group_sum<64>(v);
+}
+
+// Workgroup sum over blockDim.x == 256 threads; result valid in all threads.
+// `lds` must hold kWavesPerBlock values.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* lds)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    T total = lds[0];
+#pragma unroll
+    for (int w = 1; w < kWavesPerBlock; ++w) total += lds[w];
+    __syncthreads();
+    return total;
+}
+
+// XCD-aware workgroup index: consecutive workgroups are dispatched round-robin
+// over the 8 XCDs; this remaps them so that each XCD walks one contiguous
+// range of the logical index space (better L2 reuse of gathered vectors).
+__device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblocks)
+{
+    constexpr unsigned kXcd = 8;
+    const unsigned per = nblocks / kXcd;
+    const unsigned rem = nblocks % kXcd;
+    const unsigned xcd = bid % kXcd;
+    const unsigned pos = bid / kXcd;
+    // the first `rem` XCDs own one extra block each
+    const unsigned base = xcd * per + (xcd < rem ? xcd : rem);
+    return base + pos;
+}
+
+// -------------------------------------------- single-pass combine helper ----
+// Partials of `count` values per workgroup are written to `partials`
+// [gridDim.x][count]; the last workgroup (ticket) sees them all.  The ticket
+// counter is reset by the last workgroup, so the workspace stays reusable
+// without a memset (also under graph capture).
+__device__ __forceinline__ bool last_block_ticket(unsigned* ticket, int* lds_flag)
+{
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned t = atomicAdd(ticket, 1u);
+        *lds_flag = (t == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    const bool last = *lds_flag != 0;
+    if (last) {
+        __threadfence();
+    }
+    return last;
+}
+
+}  // namespace gkomi

@@ -1,0 +1,486 @@
+// CSR SpMV for gfx950 plus the CSR-side index work (load-balance partition,
+// longest row) and the scalar Jacobi generate / apply.
+//
+// Kernels:
+//  * csr_spmv_subwave<SW>: SW lanes of a wave64 per row, DPP / ds_swizzle
+//    group reduction; for rows of (nearly) uniform short length such as
+//    stencils.  Rows are walked in XCD-contiguous ranges.
+//  * csr_spmv_balanced: workgroups own contiguous nnz ranges of the
+//    partition `srow`; the block's row pointers and the products a_ij * x_j
+//    are staged in LDS, then rows are reduced thread-per-row (short) or
+//    wave-per-row (long).  Blocks that do not fit LDS walk their rows
+//    wave-per-row from global memory.  Fixed summation order: deterministic.
+//  * csr_spmm_rowwave: one wave per row, lanes over right-hand sides (the
+//    many-RHS case).
+#include "common.hpp"
+#include "spmv_common.hpp"
+
+namespace gkomi {
+
+template <typename T, int SW, bool Adv>
+__global__ void __launch_bounds__(kBlock)
+    csr_spmv_subwave(int64_t n_rows, const int32_t* __restrict__ rp,
+                     const int32_t* __restrict__ ci, const T* __restrict__ v,
+                     const T* __restrict__ b, int64_t bs, T* __restrict__ c, int64_t cs,
+                     const T* __restrict__ alpha, const T* __restrict__ beta)
+{
+    constexpr int kRowsPerBlock = kBlock / SW;
+    const int g = threadIdx.x / SW;
+    const int l = threadIdx.x % SW;
+    const int64_t nb = static_cast<int64_t>(gridDim.x);
+    const int64_t row_blocks = ceildiv(n_rows, kRowsPerBlock);
+    for (int64_t blk = xcd_block(blockIdx.x, gridDim.x); blk < row_blocks; blk += nb) {
+        const int64_t row = blk * kRowsPerBlock + g;
+        T sum = 0;
+        if (row < n_rows) {
+            const int32_t end = rp[row + 1];
+            for (int32_t k = rp[row] + l; k < end; k += SW) {
+                sum += v[k] * b[static_cast<int64_t>(ci[k]) * bs];
+            }
+        }
+        sum = group_sum<SW>(sum);
+        if (row < n_rows && l == 0) {
+            store_result<Adv>(c + row * cs, sum, alpha, beta);
+        }
+    }
+}
+
+template <typename T, bool Adv>
+__global__ void __launch_bounds__(kBlock)
+    csr_spmv_balanced(int64_t n_rows, const int32_t* __restrict__ rp,
+                      const int32_t* __restrict__ ci, const T* __restrict__ v,
+                      const T* __restrict__ b, int64_t bs, T* __restrict__ c, int64_t cs,
+                      const int32_t* __restrict__ srow, const T* __restrict__ alpha,
+                      const T* __restrict__ beta)
+{
+    __shared__ T prod[kLbLdsNnz];
+    __shared__ int32_t lrp[kLbLdsRows + 1];
+    __shared__ int32_t long_rows[kLbLdsRows];
+    __shared__ int32_t num_long;
+    const int64_t k = xcd_block(blockIdx.x, gridDim.x);
+    const int32_t rs = srow[k];
+    const int32_t re = srow[k + 1];
+    if (rs >= re) {
+        return;
+    }
+    const int32_t ps = rp[rs];
+    const int32_t pe = rp[re];
+    const int32_t nrows = re - rs;
+    const int32_t nnzb = pe - ps;
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    if (nnzb <= kLbLdsNnz && nrows <= kLbLdsRows) {
+        if (threadIdx.x == 0) num_long = 0;
+        for (int i = threadIdx.x; i <= nrows; i += kBlock) {
+            lrp[i] = rp[rs + i] - ps;
+        }
+        for (int i = threadIdx.x; i < nnzb; i += kBlock) {
+            prod[i] = v[ps + i] * b[static_cast<int64_t>(ci[ps + i]) * bs];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nrows; i += kBlock) {
+            const int32_t s = lrp[i];
+            const int32_t e = lrp[i + 1];
+            if (e - s <= kLbShortRow) {
+                T sum = 0;
+                for (int32_t j = s; j < e; ++j) sum += prod[j];
+                store_result<Adv>(c + static_cast<int64_t>(rs + i) * cs, sum, alpha, beta);
+            } else {
+                long_rows[atomicAdd(&num_long, 1)] = i;
+            }
+        }
+        __syncthreads();
+        const int nl = num_long;
+        for (int t = wave; t < nl; t += kWavesPerBlock) {
+            const int i = long_rows[t];
+            const int32_t s = lrp[i];
+            const int32_t e = lrp[i + 1];
+            T sum = 0;
+            for (int32_t j = s + lane; j < e; j += kWave) sum += prod[j];
+            sum = wave_sum(sum);
+            if (lane == 0) {
+                store_result<Adv>(c + static_cast<int64_t>(rs + i) * cs, sum, alpha, beta);
+            }
+        }
+    } else {
+        // rows that do not fit LDS: one wave per row straight from memory
+        for (int32_t i = wave; i < nrows; i += kWavesPerBlock) {
+            const int64_t row = rs + i;
+            const int32_t e = rp[row + 1];
+            T sum = 0;
+            for (int32_t j = rp[row] + lane; j < e; j += kWave) {
+                sum += v[j] * b[static_cast<int64_t>(ci[j]) * bs];
+            }
+            sum = wave_sum(sum);
+            if (lane == 0) {
+                store_result<Adv>(c + row * cs, sum, alpha, beta);
+            }
+        }
+    }
+}
+
+template <typename T, bool Adv>
+__global__ void __launch_bounds__(kBlock)
+    csr_spmm_rowwave(int64_t n_rows, int64_t nrhs, const int32_t* __restrict__ rp,
+                     const int32_t* __restrict__ ci, const T* __restrict__ v,
+                     const T* __restrict__ b, int64_t bs, T* __restrict__ c, int64_t cs,
+                     const T* __restrict__ alpha, const T* __restrict__ beta)
+{
+    const int lane = threadIdx.x % kWave;
+    const int64_t wave_id =
+        static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kWavesPerBlock +
+        threadIdx.x / kWave;
+    const int64_t num_waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+    for (int64_t row = wave_id; row < n_rows; row += num_waves) {
+        const int32_t s = rp[row];
+        const int32_t e = rp[row + 1];
+        for (int64_t j0 = 0; j0 < nrhs; j0 += kWave) {
+            const int64_t j = j0 + lane;
+            if (j < nrhs) {
+                T sum = 0;
+                for (int32_t k = s; k < e; ++k) {
+                    sum += v[k] * b[static_cast<int64_t>(ci[k]) * bs + j];
+                }
+                store_result<Adv>(c + row * cs + j, sum, alpha, beta);
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    lb_partition_kernel(int64_t n_rows, const int32_t* __restrict__ rp, int64_t chunk,
+                        int32_t* __restrict__ srow, int64_t num_blocks)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t k = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; k <= num_blocks;
+         k += stride) {
+        if (k == num_blocks) {
+            srow[k] = static_cast<int32_t>(n_rows);
+            continue;
+        }
+        // first row r in [0, n_rows] with rp[r] >= k * chunk (r = n_rows if none)
+        const int64_t target = k * chunk;
+        int64_t lo = 0;
+        int64_t hi = n_rows;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) / 2;
+            if (rp[mid] >= target) {
+                hi = mid;
+            } else {
+                lo = mid + 1;
+            }
+        }
+        srow[k] = static_cast<int32_t>(lo);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    max_row_nnz_kernel(int64_t n_rows, const int32_t* __restrict__ rp,
+                       unsigned long long* __restrict__ result)
+{
+    __shared__ int64_t lds[kWavesPerBlock];
+    int64_t m = 0;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n_rows;
+         i += stride) {
+        m = max(m, static_cast<int64_t>(rp[i + 1] - rp[i]));
+    }
+    for (int off = 32; off >= 1; off /= 2) m = max(m, __shfl_xor(m, off, 64));
+    if (threadIdx.x % kWave == 0) lds[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kWavesPerBlock; ++w) m = max(m, lds[w]);
+        atomicMax(result, static_cast<unsigned long long>(m));
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    jacobi_generate_kernel(int64_t n_rows, const int32_t* __restrict__ rp,
+                           const int32_t* __restrict__ ci, const T* __restrict__ v,
+                           T* __restrict__ inv_diag)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t row = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; row < n_rows;
+         row += stride) {
+        T d = 0;
+        for (int32_t k = rp[row]; k < rp[row + 1]; ++k) {
+            if (ci[k] == row) {
+                d = v[k];
+                break;
+            }
+        }
+        inv_diag[row] = (d == T(0)) ? T(1) : T(1) / d;
+    }
+}
+
+template <typename T, bool Adv>
+__global__ void __launch_bounds__(kBlock)
+    jacobi_apply_kernel(int64_t n, int64_t nrhs, const T* __restrict__ inv_diag,
+                        const T* __restrict__ alpha, const T* __restrict__ b, int64_t bs,
+                        const T* __restrict__ beta, T* __restrict__ x, int64_t xs)
+{
+    const int64_t total = n * nrhs;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; e < total;
+         e += stride) {
+        const int64_t i = e / nrhs;
+        const int64_t j = e - i * nrhs;
+        if constexpr (Adv) {
+            x[i * xs + j] = beta[0] * x[i * xs + j] + alpha[0] * b[i * bs + j] * inv_diag[i];
+        } else {
+            x[i * xs + j] = b[i * bs + j] * inv_diag[i];
+        }
+    }
+}
+
+// -------------------------------------------------------------- drivers ----
+
+int check_csr(const gkomi_csr* a)
+{
+    GKOMI_REQUIRE(a != nullptr, GKOMI_INVALID_ARGUMENT, "null CSR descriptor");
+    GKOMI_REQUIRE(a->n_rows >= 0 && a->n_cols >= 0 && a->nnz >= 0, GKOMI_INVALID_ARGUMENT,
+                  "CSR: negative size");
+    GKOMI_REQUIRE(a->nnz < (int64_t{1} << 31) && a->n_rows < (int64_t{1} << 31),
+                  GKOMI_INVALID_ARGUMENT, "CSR: sizes exceed the int32 index range");
+    GKOMI_REQUIRE(a->n_rows == 0 || a->row_ptrs != nullptr, GKOMI_INVALID_ARGUMENT,
+                  "CSR: null row pointers");
+    GKOMI_REQUIRE(a->nnz == 0 || (a->col_idxs != nullptr && a->values != nullptr),
+                  GKOMI_INVALID_ARGUMENT, "CSR: null column indices or values");
+    GKOMI_REQUIRE(a->strategy == GKOMI_CSR_CLASSICAL || a->strategy == GKOMI_CSR_LOAD_BALANCE,
+                  GKOMI_INVALID_ARGUMENT, "CSR: unknown strategy");
+    GKOMI_REQUIRE(a->strategy != GKOMI_CSR_LOAD_BALANCE ||
+                      (a->srow != nullptr && a->srow_blocks >= 1),
+                  GKOMI_INVALID_ARGUMENT, "CSR: load_balance strategy without partition");
+    return GKOMI_SUCCESS;
+}
+
+int pick_subwave(const gkomi_csr* a)
+{
+    if (a->subwave > 0) {
+        return a->subwave;
+    }
+    const double mean = a->n_rows > 0 ? static_cast<double>(a->nnz) / a->n_rows : 1.0;
+    int sw = 1;
+    while (sw < 64 && sw < mean) sw *= 2;
+    return sw;
+}
+
+template <typename T, bool Adv>
+int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const T* b, int64_t bs,
+                    T* c, int64_t cs, const T* alpha, const T* beta)
+{
+    const T* v = static_cast<const T*>(a->values);
+    if (nrhs > 1) {
+        const unsigned g = grid_for(a->n_rows, kWavesPerBlock, 32);
+        csr_spmm_rowwave<T, Adv><<<g, kBlock, 0, stream>>>(a->n_rows, nrhs, a->row_ptrs,
+                                                          a->col_idxs, v, b, bs, c, cs, alpha,
+                                                          beta);
+        GKOMI_CHECK_LAUNCH("csr_spmm_rowwave");
+        return GKOMI_SUCCESS;
+    }
+    if (a->strategy == GKOMI_CSR_LOAD_BALANCE) {
+        GKOMI_REQUIRE(a->srow_blocks < (int64_t{1} << 31), GKOMI_INVALID_ARGUMENT,
+                      "CSR: partition too large");
+        csr_spmv_balanced<T, Adv>
+            <<<static_cast<unsigned>(a->srow_blocks), kBlock, 0, stream>>>(
+                a->n_rows, a->row_ptrs, a->col_idxs, v, b, bs, c, cs, a->srow, alpha, beta);
+        GKOMI_CHECK_LAUNCH("csr_spmv_balanced");
+        return GKOMI_SUCCESS;
+    }
+    const int sw = pick_subwave(a);
+    GKOMI_REQUIRE(sw == 1 || sw == 2 || sw == 4 || sw == 8 || sw == 16 || sw == 32 || sw == 64,
+                  GKOMI_INVALID_ARGUMENT, "CSR: subwave must be a power of two <= 64");
+    const int64_t rows_per_block = kBlock / sw;
+    const unsigned g = grid_for(a->n_rows, rows_per_block, 64);
+#define GKOMI_SUBWAVE_CASE(S)                                                                 \
+    case S:                                                                                   \
+        csr_spmv_subwave<T, S, Adv><<<g, kBlock, 0, stream>>>(a->n_rows, a->row_ptrs,         \
+                                                              a->col_idxs, v, b, bs, c, cs,   \
+                                                              alpha, beta);                   \
+        break;
+    switch (sw) {
+        GKOMI_SUBWAVE_CASE(1)
+        GKOMI_SUBWAVE_CASE(2)
+        GKOMI_SUBWAVE_CASE(4)
+        GKOMI_SUBWAVE_CASE(8)
+        GKOMI_SUBWAVE_CASE(16)
+        GKOMI_SUBWAVE_CASE(32)
+        GKOMI_SUBWAVE_CASE(64)
+    }
+#undef GKOMI_SUBWAVE_CASE
+    GKOMI_CHECK_LAUNCH("csr_spmv_subwave");
+    return GKOMI_SUCCESS;
+}
+
+template int csr_spmv_launch<double, false>(hipStream_t, const gkomi_csr*, int64_t,
+                                            const double*, int64_t, double*, int64_t,
+                                            const double*, const double*);
+template int csr_spmv_launch<double, true>(hipStream_t, const gkomi_csr*, int64_t,
+                                           const double*, int64_t, double*, int64_t,
+                                           const double*, const double*);
+template int csr_spmv_launch<float, false>(hipStream_t, const gkomi_csr*, int64_t,
+                                           const float*, int64_t, float*, int64_t,
+                                           const float*, const float*);
+template int csr_spmv_launch<float, true>(hipStream_t, const gkomi_csr*, int64_t, const float*,
+                                          int64_t, float*, int64_t, const float*,
+                                          const float*);
+
+template <typename T, bool Adv>
+int csr_spmv_impl(const gkomi_ctx* ctx, const T* alpha, const gkomi_csr* a, int64_t nrhs,
+                  const T* b, int64_t bs, const T* beta, T* c, int64_t cs)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_csr(a)) return rc;
+    GKOMI_REQUIRE(nrhs >= 0, GKOMI_INVALID_ARGUMENT, "spmv: negative number of columns");
+    if (a->n_rows == 0 || nrhs == 0) return GKOMI_SUCCESS;
+    GKOMI_REQUIRE(c != nullptr && (a->nnz == 0 || b != nullptr), GKOMI_INVALID_ARGUMENT,
+                  "spmv: null vector");
+    GKOMI_REQUIRE(bs >= nrhs && cs >= nrhs, GKOMI_INVALID_ARGUMENT,
+                  "spmv: stride smaller than the number of columns");
+    GKOMI_REQUIRE(!Adv || (alpha != nullptr && beta != nullptr), GKOMI_INVALID_ARGUMENT,
+                  "advanced spmv: null alpha or beta");
+    return csr_spmv_launch<T, Adv>(stream_of(ctx), a, nrhs, b, bs, c, cs, alpha, beta);
+}
+
+template <typename T>
+int jacobi_generate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, T* inv_diag)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_csr(a)) return rc;
+    GKOMI_REQUIRE(a->n_rows == a->n_cols, GKOMI_DIMENSION_MISMATCH,
+                  "Jacobi: system matrix is not square");
+    if (a->n_rows == 0) return GKOMI_SUCCESS;
+    GKOMI_REQUIRE(inv_diag != nullptr, GKOMI_INVALID_ARGUMENT, "Jacobi: null output");
+    jacobi_generate_kernel<T><<<grid_for(a->n_rows, kBlock), kBlock, 0, stream_of(ctx)>>>(
+        a->n_rows, a->row_ptrs, a->col_idxs, static_cast<const T*>(a->values), inv_diag);
+    GKOMI_CHECK_LAUNCH("jacobi_generate_kernel");
+    return GKOMI_SUCCESS;
+}
+
+template <typename T, bool Adv>
+int jacobi_apply_impl(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const T* inv_diag,
+                      const T* alpha, const T* b, int64_t bs, const T* beta, T* x, int64_t xs)
+{
+    if (int rc = enter(ctx)) return rc;
+    GKOMI_REQUIRE(n >= 0 && nrhs >= 0, GKOMI_INVALID_ARGUMENT, "Jacobi apply: negative size");
+    if (n == 0 || nrhs == 0) return GKOMI_SUCCESS;
+    GKOMI_REQUIRE(inv_diag && b && x, GKOMI_INVALID_ARGUMENT, "Jacobi apply: null pointer");
+    GKOMI_REQUIRE(bs >= nrhs && xs >= nrhs, GKOMI_INVALID_ARGUMENT,
+                  "Jacobi apply: stride smaller than the number of columns");
+    GKOMI_REQUIRE(!Adv || (alpha && beta), GKOMI_INVALID_ARGUMENT,
+                  "Jacobi apply: null alpha or beta");
+    jacobi_apply_kernel<T, Adv><<<grid_for(n * nrhs, kBlock * 4), kBlock, 0, stream_of(ctx)>>>(
+        n, nrhs, inv_diag, alpha, b, bs, beta, x, xs);
+    GKOMI_CHECK_LAUNCH("jacobi_apply_kernel");
+    return GKOMI_SUCCESS;
+}
+
+}  // namespace gkomi
+
+using namespace gkomi;
+
+extern "C" {
+
+int64_t gkomi_csr_lb_chunk(int32_t value_bytes)
+{
+    (void)value_bytes;
+    return kLbChunk;
+}
+
+int gkomi_csr_lb_partition(const gkomi_ctx* ctx, int64_t n_rows, const int32_t* row_ptrs,
+                           int64_t nnz, int64_t chunk, int32_t* srow, int64_t num_blocks)
+{
+    if (int rc = enter(ctx)) return rc;
+    GKOMI_REQUIRE(n_rows >= 0 && nnz >= 0 && chunk > 0, GKOMI_INVALID_ARGUMENT,
+                  "partition: bad size");
+    GKOMI_REQUIRE(num_blocks == std::max<int64_t>(1, ceildiv(nnz, chunk)),
+                  GKOMI_DIMENSION_MISMATCH, "partition: num_blocks != max(1, ceil(nnz/chunk))");
+    GKOMI_REQUIRE(srow != nullptr && (n_rows == 0 || row_ptrs != nullptr),
+                  GKOMI_INVALID_ARGUMENT, "partition: null pointer");
+    lb_partition_kernel<<<grid_for(num_blocks + 1, kBlock), kBlock, 0, stream_of(ctx)>>>(
+        n_rows, row_ptrs, chunk, srow, num_blocks);
+    GKOMI_CHECK_LAUNCH("lb_partition_kernel");
+    return GKOMI_SUCCESS;
+}
+
+int gkomi_csr_max_row_nnz(const gkomi_ctx* ctx, int64_t n_rows, const int32_t* row_ptrs,
+                          int64_t* result)
+{
+    if (int rc = enter(ctx)) return rc;
+    GKOMI_REQUIRE(result != nullptr, GKOMI_INVALID_ARGUMENT, "max_row_nnz: null result");
+    // zero the result with a kernel (graph-capture safe), then reduce into it
+    if (int rc = gkomi_fill_zero(ctx, result, sizeof(int64_t))) return rc;
+    if (n_rows == 0) return GKOMI_SUCCESS;
+    max_row_nnz_kernel<<<grid_for(n_rows, kBlock * 8, 4), kBlock, 0, stream_of(ctx)>>>(
+        n_rows, row_ptrs, reinterpret_cast<unsigned long long*>(result));
+    GKOMI_CHECK_LAUNCH("max_row_nnz_kernel");
+    return GKOMI_SUCCESS;
+}
+
+int gkomi_csr_spmv_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t nrhs, const double* b,
+                       int64_t bs, double* c, int64_t cs)
+{
+    return csr_spmv_impl<double, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
+}
+
+int gkomi_csr_spmv_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t nrhs, const float* b,
+                       int64_t bs, float* c, int64_t cs)
+{
+    return csr_spmv_impl<float, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
+}
+
+int gkomi_csr_advanced_spmv_f64(const gkomi_ctx* ctx, const double* alpha, const gkomi_csr* a,
+                                int64_t nrhs, const double* b, int64_t bs, const double* beta,
+                                double* c, int64_t cs)
+{
+    return csr_spmv_impl<double, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
+}
+
+int gkomi_csr_advanced_spmv_f32(const gkomi_ctx* ctx, const float* alpha, const gkomi_csr* a,
+                                int64_t nrhs, const float* b, int64_t bs, const float* beta,
+                                float* c, int64_t cs)
+{
+    return csr_spmv_impl<float, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
+}
+
+int gkomi_jacobi_generate_f64(const gkomi_ctx* ctx, const gkomi_csr* a, double* inv_diag)
+{
+    return jacobi_generate_impl(ctx, a, inv_diag);
+}
+
+int gkomi_jacobi_generate_f32(const gkomi_ctx* ctx, const gkomi_csr* a, float* inv_diag)
+{
+    return jacobi_generate_impl(ctx, a, inv_diag);
+}
+
+int gkomi_jacobi_apply_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const double* inv_diag,
+                           const double* b, int64_t bs, double* x, int64_t xs)
+{
+    return jacobi_apply_impl<double, false>(ctx, n, nrhs, inv_diag, nullptr, b, bs, nullptr, x,
+                                            xs);
+}
+
+int gkomi_jacobi_apply_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const float* inv_diag,
+                           const float* b, int64_t bs, float* x, int64_t xs)
+{
+    return jacobi_apply_impl<float, false>(ctx, n, nrhs, inv_diag, nullptr, b, bs, nullptr, x,
+                                           xs);
+}
+
+int gkomi_jacobi_advanced_apply_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs,
+                                    const double* inv_diag, const double* alpha,
+                                    const double* b, int64_t bs, const double* beta, double* x,
+                                    int64_t xs)
+{
+    return jacobi_apply_impl<double, true>(ctx, n, nrhs, inv_diag, alpha, b, bs, beta, x, xs);
+}
+
+int gkomi_jacobi_advanced_apply_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs,
+                                    const float* inv_diag, const float* alpha, const float* b,
+                                    int64_t bs, const float* beta, float* x, int64_t xs)
+{
+    return jacobi_apply_impl<float, true>(ctx, n, nrhs, inv_diag, alpha, b, bs, beta, x, xs);
+}
+
+}  // extern "C"

@@ -1,0 +1,609 @@
+// SELL-P and ELL storage (conversion from CSR, SpMV) and the CSR transpose.
+//
+// SELL-P with slice_size 64 maps one slice onto one wave: lane = row in the
+// slice, so every load of the column-major slice storage is a fully
+// coalesced 64-wide access.  ELL is column-major with stride n_rows, one lane
+// per row.  The transpose is a counting sort: column histogram, exclusive
+// scan, scatter, then a per-column rank sort by original position that makes
+// the order identical to the reference's sequential stable scatter.
+#include "common.hpp"
+#include "spmv_common.hpp"
+
+namespace gkomi {
+
+// ------------------------------------------------------- exclusive scan ----
+// out[0..n] = exclusive prefix sum of in[0..n) with out[n] = total.
+// Three phases: per-tile sums, scan of tile sums in one workgroup, per-tile
+// scan plus offset.  Integer only, hence exact.
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+template <typename I>
+__device__ I block_exclusive_scan(I v, I* lds, I* total)
+{
+    // Hillis-Steele over the workgroup in LDS (integers)
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off *= 2) {
+        I t = threadIdx.x >= off ? lds[threadIdx.x - off] : I(0);
+        __syncthreads();
+        lds[threadIdx.x] += t;
+        __syncthreads();
+    }
+    const I incl = lds[threadIdx.x];
+    *total = lds[kBlock - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+template <typename I>
+__global__ void __launch_bounds__(kBlock)
+    scan_tile_sums(int64_t n, const I* __restrict__ in, I* __restrict__ tile_sums)
+{
+    __shared__ I lds[kBlock];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+    I acc = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + static_cast<int64_t>(k) * kBlock + threadIdx.x;
+        if (i < n) acc += in[i];
+    }
+    I total;
+    block_exclusive_scan(acc, lds, &total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+template <typename I>
+__global__ void __launch_bounds__(kBlock) scan_tile_offsets(int64_t num_tiles, I* __restrict__ tile_sums)
+{
+    __shared__ I lds[kBlock];
+    I carry = 0;
+    for (int64_t t0 = 0; t0 < num_tiles; t0 += kBlock) {
+        const int64_t t = t0 + threadIdx.x;
+        const I v = t < num_tiles ? tile_sums[t] : I(0);
+        I total;
+        const I ex = block_exclusive_scan(v, lds, &total);
+        if (t < num_tiles) tile_sums[t] = carry + ex;
+        carry += total;
+    }
+}
+
+template <typename I>
+__global__ void __launch_bounds__(kBlock)
+    scan_apply(int64_t n, const I* __restrict__ in, const I* __restrict__ tile_offsets,
+               I* __restrict__ out)
+{
+    __shared__ I lds[kBlock];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+    // each thread owns kScanItems consecutive elements
+    const int64_t first = base + static_cast<int64_t>(threadIdx.x) * kScanItems;
+    I vals[kScanItems];
+    I local = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = first + k;
+        vals[k] = i < n ? in[i] : I(0);
+        local += vals[k];
+    }
+    I total;
+    I run = block_exclusive_scan(local, lds, &total) + tile_offsets[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = first + k;
+        if (i < n) out[i] = run;
+        run += vals[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) {
+        out[n] = run;
+    }
+}
+
+size_t scan_ws_bytes(int64_t n, size_t elem)
+{
+    return elem * static_cast<size_t>(std::max<int64_t>(1, ceildiv(n, kScanTile))) + 256;
+}
+
+// `in` and `out` may alias (in-place scan): each tile reads its inputs into
+// registers before writing.  Wait -- scan_apply's tiles are independent, and
+// phase 1 completes before phase 3, so aliasing is safe.
+template <typename I>
+int exclusive_scan(hipStream_t stream, int64_t n, const I* in, I* out, void* ws)
+{
+    const int64_t tiles = std::max<int64_t>(1, ceildiv(n, kScanTile));
+    I* tile = static_cast<I*>(ws);
+    scan_tile_sums<I><<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(n, in, tile);
+    scan_tile_offsets<I><<<1, kBlock, 0, stream>>>(tiles, tile);
+    scan_apply<I><<<static_cast<unsigned>(tiles), kBlock, 0, stream>>>(n, in, tile, out);
+    GKOMI_CHECK_LAUNCH("exclusive_scan");
+    return GKOMI_SUCCESS;
+}
+
+// -------------------------------------------------------------- SELL-P ----
+
+__global__ void __launch_bounds__(kBlock)
+    slice_lengths_kernel(int64_t n_rows, const int32_t* __restrict__ rp, int64_t slice_size,
+                         int64_t stride_factor, int64_t num_slices,
+                         int64_t* __restrict__ slice_lengths)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t s = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; s < num_slices;
+         s += stride) {
+        int64_t len = 0;
+        const int64_t r0 = s * slice_size;
+        const int64_t r1 = std::min(n_rows, r0 + slice_size);
+        for (int64_t r = r0; r < r1; ++r) {
+            const int64_t l = rp[r + 1] - rp[r];
+            len = max(len, ceildiv(l, stride_factor) * stride_factor);
+        }
+        slice_lengths[s] = len;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    csr_to_sellp_kernel(int64_t n_rows, const int32_t* __restrict__ rp,
+                        const int32_t* __restrict__ ci, const T* __restrict__ v,
+                        int64_t slice_size, int64_t num_slices,
+                        const int64_t* __restrict__ slice_sets,
+                        const int64_t* __restrict__ slice_lengths, int32_t* __restrict__ s_cols,
+                        T* __restrict__ s_vals)
+{
+    const int64_t total_rows = num_slices * slice_size;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; r < total_rows;
+         r += stride) {
+        const int64_t s = r / slice_size;
+        const int64_t lr = r - s * slice_size;
+        const int64_t base = slice_sets[s] * slice_size + lr;
+        const int64_t slen = slice_lengths[s];
+        int64_t k = 0;
+        if (r < n_rows) {
+            const int32_t b = rp[r];
+            const int32_t len = rp[r + 1] - b;
+            for (; k < len; ++k) {
+                s_cols[base + k * slice_size] = ci[b + k];
+                s_vals[base + k * slice_size] = v[b + k];
+            }
+        }
+        for (; k < slen; ++k) {
+            s_cols[base + k * slice_size] = -1;
+            s_vals[base + k * slice_size] = T(0);
+        }
+    }
+}
+
+template <typename T, bool Adv>
+__global__ void __launch_bounds__(kBlock)
+    sellp_spmv_kernel(int64_t n_rows, int64_t nrhs, int64_t slice_size,
+                      const int64_t* __restrict__ slice_sets,
+                      const int64_t* __restrict__ slice_lengths,
+                      const int32_t* __restrict__ s_cols, const T* __restrict__ s_vals,
+                      const T* __restrict__ b, int64_t bs, T* __restrict__ c, int64_t cs,
+                      const T* __restrict__ alpha, const T* __restrict__ beta)
+{
+    const int64_t nb = gridDim.x;
+    const int64_t row_blocks = ceildiv(n_rows, kBlock);
+    for (int64_t blk = xcd_block(blockIdx.x, gridDim.x); blk < row_blocks; blk += nb) {
+        const int64_t r = blk * kBlock + threadIdx.x;
+        if (r >= n_rows) continue;
+        const int64_t s = r / slice_size;
+        const int64_t base = slice_sets[s] * slice_size + (r - s * slice_size);
+        const int64_t slen = slice_lengths[s];
+        for (int64_t j = 0; j < nrhs; ++j) {
+            T sum = 0;
+            for (int64_t k = 0; k < slen; ++k) {
+                const int64_t idx = base + k * slice_size;
+                const int32_t col = s_cols[idx];
+                if (col != -1) {
+                    sum += s_vals[idx] * b[static_cast<int64_t>(col) * bs + j];
+                }
+            }
+            store_result<Adv>(c + r * cs + j, sum, alpha, beta);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ ELL ----
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    csr_to_ell_kernel(int64_t n_rows, const int32_t* __restrict__ rp,
+                      const int32_t* __restrict__ ci, const T* __restrict__ v, int64_t width,
+                      int64_t stride_e, int32_t* __restrict__ e_cols, T* __restrict__ e_vals)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; r < stride_e;
+         r += stride) {
+        int64_t k = 0;
+        if (r < n_rows) {
+            const int32_t b = rp[r];
+            const int64_t len = std::min<int64_t>(rp[r + 1] - b, width);
+            for (; k < len; ++k) {
+                e_cols[r + k * stride_e] = ci[b + k];
+                e_vals[r + k * stride_e] = v[b + k];
+            }
+        }
+        for (; k < width; ++k) {
+            e_cols[r + k * stride_e] = -1;
+            e_vals[r + k * stride_e] = T(0);
+        }
+    }
+}
+
+template <typename T, bool Adv>
+__global__ void __launch_bounds__(kBlock)
+    ell_spmv_kernel(int64_t n_rows, int64_t nrhs, int64_t width, int64_t stride_e,
+                    const int32_t* __restrict__ e_cols, const T* __restrict__ e_vals,
+                    const T* __restrict__ b, int64_t bs, T* __restrict__ c, int64_t cs,
+                    const T* __restrict__ alpha, const T* __restrict__ beta)
+{
+    const int64_t nb = gridDim.x;
+    const int64_t row_blocks = ceildiv(n_rows, kBlock);
+    for (int64_t blk = xcd_block(blockIdx.x, gridDim.x); blk < row_blocks; blk += nb) {
+        const int64_t r = blk * kBlock + threadIdx.x;
+        if (r >= n_rows) continue;
+        for (int64_t j = 0; j < nrhs; ++j) {
+            T sum = 0;
+            for (int64_t k = 0; k < width; ++k) {
+                const int32_t col = e_cols[r + k * stride_e];
+                if (col != -1) {
+                    sum += e_vals[r + k * stride_e] * b[static_cast<int64_t>(col) * bs + j];
+                }
+            }
+            store_result<Adv>(c + r * cs + j, sum, alpha, beta);
+        }
+    }
+}
+
+// ------------------------------------------------------------ transpose ----
+
+__global__ void __launch_bounds__(kBlock)
+    column_histogram(int64_t nnz, const int32_t* __restrict__ ci, int32_t* __restrict__ counts)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < nnz;
+         i += stride) {
+        atomicAdd(&counts[ci[i]], 1);
+    }
+}
+
+// scatter (original index, row) into the column segments; order inside a
+// segment is arbitrary here and fixed by the rank sort below
+__global__ void __launch_bounds__(kBlock)
+    transpose_scatter(int64_t n_rows, const int32_t* __restrict__ rp,
+                      const int32_t* __restrict__ ci, int32_t* __restrict__ cursor,
+                      int32_t* __restrict__ key, int32_t* __restrict__ row_of)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; r < n_rows;
+         r += stride) {
+        for (int32_t i = rp[r]; i < rp[r + 1]; ++i) {
+            const int32_t slot = atomicAdd(&cursor[ci[i]], 1);
+            key[slot] = i;
+            row_of[slot] = static_cast<int32_t>(r);
+        }
+    }
+}
+
+// rank sort of each column segment by original position (keys are unique):
+// short segments by one lane, long ones by the whole wave
+constexpr int kShortSegment = 16;
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    transpose_finish(int64_t n_cols, const int32_t* __restrict__ t_rp,
+                     const int32_t* __restrict__ key, const int32_t* __restrict__ row_of,
+                     const T* __restrict__ v, int32_t* __restrict__ t_ci, T* __restrict__ t_v)
+{
+    const int lane = threadIdx.x % kWave;
+    const int64_t wave_id = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+    const int64_t num_waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+    for (int64_t c0 = wave_id * kWave; c0 < n_cols; c0 += num_waves * kWave) {
+        const int64_t c = c0 + lane;
+        int32_t s = 0, e = 0;
+        if (c < n_cols) {
+            s = t_rp[c];
+            e = t_rp[c + 1];
+        }
+        const bool is_long = (e - s) > kShortSegment;
+        if (c < n_cols && !is_long) {
+            for (int32_t a = s; a < e; ++a) {
+                const int32_t ka = key[a];
+                int32_t rank = 0;
+                for (int32_t q = s; q < e; ++q) rank += key[q] < ka ? 1 : 0;
+                t_ci[s + rank] = row_of[a];
+                t_v[s + rank] = v[ka];
+            }
+        }
+        uint64_t long_mask = __ballot(is_long);
+        while (long_mask) {
+            const int l = __ffsll(static_cast<long long>(long_mask)) - 1;
+            long_mask &= long_mask - 1;
+            const int64_t cc = c0 + l;
+            const int32_t ls = t_rp[cc];
+            const int32_t le = t_rp[cc + 1];
+            for (int32_t a = ls + lane; a < le; a += kWave) {
+                const int32_t ka = key[a];
+                int32_t rank = 0;
+                for (int32_t q = ls; q < le; ++q) rank += key[q] < ka ? 1 : 0;
+                t_ci[ls + rank] = row_of[a];
+                t_v[ls + rank] = v[ka];
+            }
+        }
+    }
+}
+
+struct transpose_ws {
+    int32_t* cursor;  // n_cols + 1
+    int32_t* key;     // nnz
+    int32_t* row_of;  // nnz
+    void* scan;
+};
+
+size_t transpose_ws_layout(int64_t n_rows, int64_t n_cols, int64_t nnz, void* base,
+                           transpose_ws* out)
+{
+    (void)n_rows;
+    auto align = [](size_t x) { return (x + 255) / 256 * 256; };
+    size_t off = 0;
+    const size_t cur = align(sizeof(int32_t) * static_cast<size_t>(n_cols + 1));
+    const size_t kb = align(sizeof(int32_t) * static_cast<size_t>(std::max<int64_t>(nnz, 1)));
+    const size_t sc = align(scan_ws_bytes(n_cols + 1, sizeof(int32_t)));
+    if (out) {
+        char* p = static_cast<char*>(base);
+        out->cursor = reinterpret_cast<int32_t*>(p + off);
+        out->key = reinterpret_cast<int32_t*>(p + off + cur);
+        out->row_of = reinterpret_cast<int32_t*>(p + off + cur + kb);
+        out->scan = p + off + cur + 2 * kb;
+    }
+    return cur + 2 * kb + sc;
+}
+
+// --------------------------------------------------------------- drivers ----
+
+int check_sellp(const gkomi_sellp* a)
+{
+    GKOMI_REQUIRE(a != nullptr, GKOMI_INVALID_ARGUMENT, "null SELL-P descriptor");
+    GKOMI_REQUIRE(a->n_rows >= 0 && a->slice_size > 0 && a->stride_factor > 0,
+                  GKOMI_INVALID_ARGUMENT, "SELL-P: bad sizes");
+    GKOMI_REQUIRE(a->n_rows == 0 || (a->slice_sets && a->slice_lengths), GKOMI_INVALID_ARGUMENT,
+                  "SELL-P: null slice arrays");
+    return GKOMI_SUCCESS;
+}
+
+template <typename T, bool Adv>
+int sellp_spmv_impl(const gkomi_ctx* ctx, const T* alpha, const gkomi_sellp* a, int64_t nrhs,
+                    const T* b, int64_t bs, const T* beta, T* c, int64_t cs)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_sellp(a)) return rc;
+    if (a->n_rows == 0 || nrhs == 0) return GKOMI_SUCCESS;
+    GKOMI_REQUIRE(nrhs > 0 && bs >= nrhs && cs >= nrhs, GKOMI_INVALID_ARGUMENT,
+                  "SELL-P spmv: bad strides");
+    GKOMI_REQUIRE(b && c, GKOMI_INVALID_ARGUMENT, "SELL-P spmv: null vector");
+    GKOMI_REQUIRE(!Adv || (alpha && beta), GKOMI_INVALID_ARGUMENT, "SELL-P: null alpha/beta");
+    sellp_spmv_kernel<T, Adv><<<grid_for(a->n_rows, kBlock, 32), kBlock, 0, stream_of(ctx)>>>(
+        a->n_rows, nrhs, a->slice_size, a->slice_sets, a->slice_lengths, a->col_idxs,
+        static_cast<const T*>(a->values), b, bs, c, cs, alpha, beta);
+    GKOMI_CHECK_LAUNCH("sellp_spmv_kernel");
+    return GKOMI_SUCCESS;
+}
+
+template <typename T>
+int csr_to_sellp_impl(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t slice_size,
+                      const int64_t* slice_sets, const int64_t* slice_lengths, int32_t* s_cols,
+                      T* s_vals)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_csr(a)) return rc;
+    GKOMI_REQUIRE(slice_size > 0, GKOMI_INVALID_ARGUMENT, "SELL-P: slice size must be > 0");
+    const int64_t num_slices = ceildiv(a->n_rows, slice_size);
+    if (num_slices == 0) return GKOMI_SUCCESS;
+    GKOMI_REQUIRE(slice_sets && slice_lengths, GKOMI_INVALID_ARGUMENT,
+                  "SELL-P: null slice arrays");
+    csr_to_sellp_kernel<T><<<grid_for(num_slices * slice_size, kBlock), kBlock, 0,
+                             stream_of(ctx)>>>(a->n_rows, a->row_ptrs, a->col_idxs,
+                                               static_cast<const T*>(a->values), slice_size,
+                                               num_slices, slice_sets, slice_lengths, s_cols,
+                                               s_vals);
+    GKOMI_CHECK_LAUNCH("csr_to_sellp_kernel");
+    return GKOMI_SUCCESS;
+}
+
+template <typename T>
+int csr_to_ell_impl(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t width, int64_t stride,
+                    int32_t* e_cols, T* e_vals)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_csr(a)) return rc;
+    GKOMI_REQUIRE(width >= 0 && stride >= a->n_rows, GKOMI_INVALID_ARGUMENT,
+                  "ELL: stride must be >= number of rows");
+    if (stride == 0 || width == 0) return GKOMI_SUCCESS;
+    GKOMI_REQUIRE(e_cols && e_vals, GKOMI_INVALID_ARGUMENT, "ELL: null output");
+    csr_to_ell_kernel<T><<<grid_for(stride, kBlock), kBlock, 0, stream_of(ctx)>>>(
+        a->n_rows, a->row_ptrs, a->col_idxs, static_cast<const T*>(a->values), width, stride,
+        e_cols, e_vals);
+    GKOMI_CHECK_LAUNCH("csr_to_ell_kernel");
+    return GKOMI_SUCCESS;
+}
+
+template <typename T, bool Adv>
+int ell_spmv_impl(const gkomi_ctx* ctx, const T* alpha, const gkomi_ell* a, int64_t nrhs,
+                  const T* b, int64_t bs, const T* beta, T* c, int64_t cs)
+{
+    if (int rc = enter(ctx)) return rc;
+    GKOMI_REQUIRE(a != nullptr && a->n_rows >= 0 && a->width >= 0 && a->stride >= a->n_rows,
+                  GKOMI_INVALID_ARGUMENT, "ELL: bad descriptor");
+    if (a->n_rows == 0 || nrhs == 0) return GKOMI_SUCCESS;
+    GKOMI_REQUIRE(nrhs > 0 && bs >= nrhs && cs >= nrhs, GKOMI_INVALID_ARGUMENT,
+                  "ELL spmv: bad strides");
+    GKOMI_REQUIRE(c && (a->width == 0 || (b && a->col_idxs && a->values)),
+                  GKOMI_INVALID_ARGUMENT, "ELL spmv: null pointer");
+    GKOMI_REQUIRE(!Adv || (alpha && beta), GKOMI_INVALID_ARGUMENT, "ELL: null alpha/beta");
+    ell_spmv_kernel<T, Adv><<<grid_for(a->n_rows, kBlock, 32), kBlock, 0, stream_of(ctx)>>>(
+        a->n_rows, nrhs, a->width, a->stride, a->col_idxs, static_cast<const T*>(a->values), b,
+        bs, c, cs, alpha, beta);
+    GKOMI_CHECK_LAUNCH("ell_spmv_kernel");
+    return GKOMI_SUCCESS;
+}
+
+template <typename T>
+int transpose_impl(const gkomi_ctx* ctx, const gkomi_csr* a, int32_t* t_rp, int32_t* t_ci,
+                   T* t_v, void* ws, size_t wsb)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_csr(a)) return rc;
+    GKOMI_REQUIRE(t_rp != nullptr, GKOMI_INVALID_ARGUMENT, "transpose: null row pointers");
+    GKOMI_REQUIRE(a->nnz == 0 || (t_ci && t_v), GKOMI_INVALID_ARGUMENT,
+                  "transpose: null output");
+    const size_t need = transpose_ws_layout(a->n_rows, a->n_cols, a->nnz, nullptr, nullptr);
+    GKOMI_REQUIRE(ws != nullptr && wsb >= need, GKOMI_INVALID_ARGUMENT,
+                  "transpose: workspace too small");
+    transpose_ws w;
+    transpose_ws_layout(a->n_rows, a->n_cols, a->nnz, ws, &w);
+    hipStream_t st = stream_of(ctx);
+    // counts -> exclusive scan -> t_rp; cursor = copy of t_rp
+    if (int rc = gkomi_fill_zero(ctx, w.cursor, sizeof(int32_t) * (a->n_cols + 1))) return rc;
+    if (a->nnz > 0) {
+        column_histogram<<<grid_for(a->nnz, kBlock * 4), kBlock, 0, st>>>(a->nnz, a->col_idxs,
+                                                                          w.cursor);
+        GKOMI_CHECK_LAUNCH("column_histogram");
+    }
+    if (int rc = exclusive_scan<int32_t>(st, a->n_cols, w.cursor, t_rp, w.scan)) return rc;
+    if (a->nnz == 0) return GKOMI_SUCCESS;
+    GKOMI_CHECK_HIP(hipMemcpyAsync(w.cursor, t_rp, sizeof(int32_t) * (a->n_cols + 1),
+                                   hipMemcpyDeviceToDevice, st));
+    transpose_scatter<<<grid_for(a->n_rows, kBlock), kBlock, 0, st>>>(
+        a->n_rows, a->row_ptrs, a->col_idxs, w.cursor, w.key, w.row_of);
+    GKOMI_CHECK_LAUNCH("transpose_scatter");
+    transpose_finish<T><<<grid_for(ceildiv(a->n_cols, kWave), kWavesPerBlock), kBlock, 0, st>>>(
+        a->n_cols, t_rp, w.key, w.row_of, static_cast<const T*>(a->values), t_ci, t_v);
+    GKOMI_CHECK_LAUNCH("transpose_finish");
+    return GKOMI_SUCCESS;
+}
+
+}  // namespace gkomi
+
+using namespace gkomi;
+
+extern "C" {
+
+size_t gkomi_scan_workspace_bytes(int64_t n) { return scan_ws_bytes(n, sizeof(int64_t)); }
+
+int gkomi_sellp_compute_slice_sets(const gkomi_ctx* ctx, int64_t n_rows, const int32_t* row_ptrs,
+                                   int64_t slice_size, int64_t stride_factor,
+                                   int64_t* slice_sets, int64_t* slice_lengths, void* ws,
+                                   size_t wsb)
+{
+    if (int rc = enter(ctx)) return rc;
+    GKOMI_REQUIRE(n_rows >= 0 && slice_size > 0 && stride_factor > 0, GKOMI_INVALID_ARGUMENT,
+                  "slice sets: bad sizes");
+    const int64_t num_slices = ceildiv(n_rows, slice_size);
+    GKOMI_REQUIRE(slice_sets != nullptr, GKOMI_INVALID_ARGUMENT, "slice sets: null output");
+    GKOMI_REQUIRE(ws != nullptr && wsb >= scan_ws_bytes(num_slices, sizeof(int64_t)),
+                  GKOMI_INVALID_ARGUMENT, "slice sets: workspace too small");
+    if (num_slices > 0) {
+        GKOMI_REQUIRE(slice_lengths && row_ptrs, GKOMI_INVALID_ARGUMENT,
+                      "slice sets: null pointer");
+        slice_lengths_kernel<<<grid_for(num_slices, kBlock), kBlock, 0, stream_of(ctx)>>>(
+            n_rows, row_ptrs, slice_size, stride_factor, num_slices, slice_lengths);
+        GKOMI_CHECK_LAUNCH("slice_lengths_kernel");
+    }
+    return exclusive_scan<int64_t>(stream_of(ctx), num_slices, slice_lengths, slice_sets, ws);
+}
+
+int gkomi_csr_to_sellp_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t slice_size,
+                           const int64_t* slice_sets, const int64_t* slice_lengths,
+                           int32_t* s_cols, double* s_vals)
+{
+    return csr_to_sellp_impl(ctx, a, slice_size, slice_sets, slice_lengths, s_cols, s_vals);
+}
+
+int gkomi_csr_to_sellp_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t slice_size,
+                           const int64_t* slice_sets, const int64_t* slice_lengths,
+                           int32_t* s_cols, float* s_vals)
+{
+    return csr_to_sellp_impl(ctx, a, slice_size, slice_sets, slice_lengths, s_cols, s_vals);
+}
+
+int gkomi_sellp_spmv_f64(const gkomi_ctx* ctx, const gkomi_sellp* a, int64_t nrhs,
+                         const double* b, int64_t bs, double* c, int64_t cs)
+{
+    return sellp_spmv_impl<double, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
+}
+
+int gkomi_sellp_spmv_f32(const gkomi_ctx* ctx, const gkomi_sellp* a, int64_t nrhs,
+                         const float* b, int64_t bs, float* c, int64_t cs)
+{
+    return sellp_spmv_impl<float, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
+}
+
+int gkomi_sellp_advanced_spmv_f64(const gkomi_ctx* ctx, const double* alpha,
+                                  const gkomi_sellp* a, int64_t nrhs, const double* b,
+                                  int64_t bs, const double* beta, double* c, int64_t cs)
+{
+    return sellp_spmv_impl<double, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
+}
+
+int gkomi_sellp_advanced_spmv_f32(const gkomi_ctx* ctx, const float* alpha,
+                                  const gkomi_sellp* a, int64_t nrhs, const float* b,
+                                  int64_t bs, const float* beta, float* c, int64_t cs)
+{
+    return sellp_spmv_impl<float, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
+}
+
+int gkomi_csr_to_ell_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t width,
+                         int64_t stride, int32_t* e_cols, double* e_vals)
+{
+    return csr_to_ell_impl(ctx, a, width, stride, e_cols, e_vals);
+}
+
+int gkomi_csr_to_ell_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t width,
+                         int64_t stride, int32_t* e_cols, float* e_vals)
+{
+    return csr_to_ell_impl(ctx, a, width, stride, e_cols, e_vals);
+}
+
+int gkomi_ell_spmv_f64(const gkomi_ctx* ctx, const gkomi_ell* a, int64_t nrhs, const double* b,
+                       int64_t bs, double* c, int64_t cs)
+{
+    return ell_spmv_impl<double, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
+}
+
+int gkomi_ell_spmv_f32(const gkomi_ctx* ctx, const gkomi_ell* a, int64_t nrhs, const float* b,
+                       int64_t bs, float* c, int64_t cs)
+{
+    return ell_spmv_impl<float, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
+}
+
+int gkomi_ell_advanced_spmv_f64(const gkomi_ctx* ctx, const double* alpha, const gkomi_ell* a,
+                                int64_t nrhs, const double* b, int64_t bs, const double* beta,
+                                double* c, int64_t cs)
+{
+    return ell_spmv_impl<double, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
+}
+
+int gkomi_ell_advanced_spmv_f32(const gkomi_ctx* ctx, const float* alpha, const gkomi_ell* a,
+                                int64_t nrhs, const float* b, int64_t bs, const float* beta,
+                                float* c, int64_t cs)
+{
+    return ell_spmv_impl<float, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
+}
+
+size_t gkomi_csr_transpose_workspace_bytes(int64_t n_rows, int64_t n_cols, int64_t nnz)
+{
+    return transpose_ws_layout(n_rows, n_cols, nnz, nullptr, nullptr);
+}
+
+int gkomi_csr_transpose_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int32_t* t_rp,
+                            int32_t* t_ci, double* t_v, void* ws, size_t wsb)
+{
+    return transpose_impl(ctx, a, t_rp, t_ci, t_v, ws, wsb);
+}
+
+int gkomi_csr_transpose_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int32_t* t_rp,
+                            int32_t* t_ci, float* t_v, void* ws, size_t wsb)
+{
+    return transpose_impl(ctx, a, t_rp, t_ci, t_v, ws, wsb);
+}
+
+}  // extern "C"

@@ -1,0 +1,782 @@
+// CG and BiCGSTAB kernels for gfx950.
+//
+// Two families:
+//  * reference-semantics step kernels (cg::initialize/step_1/step_2,
+//    bicgstab::initialize/step_1/2/3/finalize) over n x nrhs blocks with
+//    per-column stopping status, restated from
+//    reference/solver/{cg,bicgstab}_kernels.cpp;
+//  * the fused single-RHS CG iteration: two kernels per iteration that fold
+//    the axpy updates, the Jacobi scaling, both dots and ||r|| into the SpMV
+//    and update passes, and evaluate stop::Combined on the device so the host
+//    never waits inside the loop.
+#include "common.hpp"
+#include "reduce.hpp"
+#include "spmv_common.hpp"
+#include "stop.hpp"
+
+namespace gkomi {
+
+// ------------------------------------------------ reference-semantics ----
+
+#define GKOMI_ELEMENTS(total)                                                                \
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; e < (total);    \
+         e += static_cast<int64_t>(gridDim.x) * kBlock)
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    cg_initialize_kernel(int64_t n, int64_t nrhs, const T* __restrict__ b, int64_t bs,
+                         T* __restrict__ r, T* __restrict__ z, T* __restrict__ p,
+                         T* __restrict__ q, int64_t vs, T* __restrict__ prev_rho,
+                         T* __restrict__ rho, uint8_t* __restrict__ status)
+{
+    GKOMI_ELEMENTS(n * nrhs)
+    {
+        const int64_t i = e / nrhs;
+        const int64_t j = e - i * nrhs;
+        r[i * vs + j] = b[i * bs + j];
+        z[i * vs + j] = T(0);
+        p[i * vs + j] = T(0);
+        q[i * vs + j] = T(0);
+        if (i == 0) {
+            rho[j] = T(0);
+            prev_rho[j] = T(1);
+            status[j] = 0;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    cg_step_1_kernel(int64_t n, int64_t nrhs, T* __restrict__ p, const T* __restrict__ z,
+                     int64_t vs, const T* __restrict__ rho, const T* __restrict__ prev_rho,
+                     const uint8_t* __restrict__ status)
+{
+    GKOMI_ELEMENTS(n * nrhs)
+    {
+        const int64_t i = e / nrhs;
+        const int64_t j = e - i * nrhs;
+        if (has_stopped(status[j])) continue;
+        if (prev_rho[j] == T(0)) {
+            p[i * vs + j] = z[i * vs + j];
+        } else {
+            const T tmp = rho[j] / prev_rho[j];
+            p[i * vs + j] = z[i * vs + j] + tmp * p[i * vs + j];
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    cg_step_2_kernel(int64_t n, int64_t nrhs, T* __restrict__ x, int64_t xs, T* __restrict__ r,
+                     const T* __restrict__ p, const T* __restrict__ q, int64_t vs,
+                     const T* __restrict__ beta, const T* __restrict__ rho,
+                     const uint8_t* __restrict__ status)
+{
+    GKOMI_ELEMENTS(n * nrhs)
+    {
+        const int64_t i = e / nrhs;
+        const int64_t j = e - i * nrhs;
+        if (has_stopped(status[j])) continue;
+        if (beta[j] != T(0)) {
+            const T tmp = rho[j] / beta[j];
+            x[i * xs + j] += tmp * p[i * vs + j];
+            r[i * vs + j] -= tmp * q[i * vs + j];
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    bicgstab_initialize_kernel(int64_t n, int64_t nrhs, const T* __restrict__ b, int64_t bs,
+                               T* r, T* rr, T* y, T* s, T* t, T* z, T* v, T* p, int64_t vs,
+                               T* prev_rho, T* rho, T* alpha, T* beta, T* gamma, T* omega,
+                               uint8_t* status)
+{
+    GKOMI_ELEMENTS(n * nrhs)
+    {
+        const int64_t i = e / nrhs;
+        const int64_t j = e - i * nrhs;
+        const int64_t o = i * vs + j;
+        r[o] = b[i * bs + j];
+        rr[o] = z[o] = v[o] = s[o] = t[o] = y[o] = p[o] = T(0);
+        if (i == 0) {
+            rho[j] = prev_rho[j] = alpha[j] = beta[j] = gamma[j] = omega[j] = T(1);
+            status[j] = 0;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    bicgstab_step_1_kernel(int64_t n, int64_t nrhs, const T* __restrict__ r, T* __restrict__ p,
+                           const T* __restrict__ v, int64_t vs, const T* __restrict__ rho,
+                           const T* __restrict__ prev_rho, const T* __restrict__ alpha,
+                           const T* __restrict__ omega, const uint8_t* __restrict__ status)
+{
+    GKOMI_ELEMENTS(n * nrhs)
+    {
+        const int64_t i = e / nrhs;
+        const int64_t j = e - i * nrhs;
+        if (has_stopped(status[j])) continue;
+        const int64_t o = i * vs + j;
+        if (prev_rho[j] * omega[j] != T(0)) {
+            const T tmp = rho[j] / prev_rho[j] * alpha[j] / omega[j];
+            p[o] = r[o] + tmp * (p[o] - omega[j] * v[o]);
+        } else {
+            p[o] = r[o];
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    bicgstab_step_2_kernel(int64_t n, int64_t nrhs, const T* __restrict__ r, T* __restrict__ s,
+                           const T* __restrict__ v, int64_t vs, const T* __restrict__ rho,
+                           T* __restrict__ alpha, const T* __restrict__ beta,
+                           const uint8_t* __restrict__ status)
+{
+    GKOMI_ELEMENTS(n * nrhs)
+    {
+        const int64_t i = e / nrhs;
+        const int64_t j = e - i * nrhs;
+        if (has_stopped(status[j])) continue;
+        const int64_t o = i * vs + j;
+        if (beta[j] != T(0)) {
+            const T a = rho[j] / beta[j];
+            s[o] = r[o] - a * v[o];
+            if (i == 0) alpha[j] = a;
+        } else {
+            s[o] = r[o];
+            if (i == 0) alpha[j] = T(0);
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    bicgstab_step_3_kernel(int64_t n, int64_t nrhs, T* __restrict__ x, int64_t xs,
+                           T* __restrict__ r, const T* __restrict__ s, const T* __restrict__ t,
+                           const T* __restrict__ y, const T* __restrict__ z, int64_t vs,
+                           const T* __restrict__ alpha, const T* __restrict__ beta,
+                           const T* __restrict__ gamma, T* __restrict__ omega,
+                           const uint8_t* __restrict__ status)
+{
+    GKOMI_ELEMENTS(n * nrhs)
+    {
+        const int64_t i = e / nrhs;
+        const int64_t j = e - i * nrhs;
+        if (has_stopped(status[j])) continue;
+        const T w = beta[j] != T(0) ? gamma[j] / beta[j] : T(0);
+        const int64_t o = i * vs + j;
+        x[i * xs + j] += alpha[j] * y[o] + w * z[o];
+        r[o] = s[o] - w * t[o];
+    }
+}
+
+// omega is written by a separate single-pass kernel so that every element
+// kernel above reads the old values consistently
+template <typename T>
+__global__ void bicgstab_omega_kernel(int64_t nrhs, const T* __restrict__ beta,
+                                      const T* __restrict__ gamma, T* __restrict__ omega,
+                                      const uint8_t* __restrict__ status)
+{
+    for (int64_t j = threadIdx.x; j < nrhs; j += blockDim.x) {
+        if (has_stopped(status[j])) continue;
+        omega[j] = beta[j] != T(0) ? gamma[j] / beta[j] : T(0);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    bicgstab_finalize_kernel(int64_t n, int64_t nrhs, T* __restrict__ x, int64_t xs,
+                             const T* __restrict__ y, int64_t vs, const T* __restrict__ alpha,
+                             const uint8_t* __restrict__ status)
+{
+    GKOMI_ELEMENTS(n * nrhs)
+    {
+        const int64_t i = e / nrhs;
+        const int64_t j = e - i * nrhs;
+        if (has_stopped(status[j]) && !(status[j] & GKOMI_STATUS_FINALIZED)) {
+            x[i * xs + j] += alpha[j] * y[i * vs + j];
+        }
+    }
+}
+
+__global__ void status_finalize_kernel(int64_t nrhs, uint8_t* status)
+{
+    for (int64_t j = threadIdx.x; j < nrhs; j += blockDim.x) {
+        if (has_stopped(status[j])) status[j] |= GKOMI_STATUS_FINALIZED;
+    }
+}
+
+template <typename T>
+__global__ void criteria_check_kernel(gkomi_criteria crit, int64_t nrhs, const T* tau,
+                                      const T* implicit_sq, const T* orig, int set_finalized,
+                                      uint8_t* status, gkomi_stop_flags* flags)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        int all = 0, changed = 0;
+        combined_check<T>(crit, flags->iteration, nrhs, tau, implicit_sq, orig,
+                          set_finalized != 0, status, &all, &changed);
+        flags->all_stopped = all;
+        flags->one_changed = changed;
+    }
+}
+
+// ---------------------------------------------------------- fused CG ----
+// scalars layout (value type): [0] rho, [1] prev_rho, [2] beta, [3] tau
+
+constexpr int kCgMaxGrid = 2048;
+
+struct cg_work {
+    unsigned* tickets;   // 4 tickets, 64 B apart
+    double* partials;    // kCgMaxGrid * 2 doubles per phase
+};
+
+__host__ __device__ inline cg_work cg_work_layout(void* base)
+{
+    cg_work w;
+    w.tickets = static_cast<unsigned*>(base);
+    w.partials = reinterpret_cast<double*>(static_cast<char*>(base) + kTicketBytes);
+    return w;
+}
+
+__device__ __forceinline__ void block_range(int64_t total, int64_t* begin, int64_t* end)
+{
+    const int64_t g = gridDim.x;
+    const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t per = ceildiv(total, g);
+    *begin = min(total, lb * per);
+    *end = min(total, *begin + per);
+}
+
+// Two-value block reduction into partials[blockIdx.x * 2 + {0,1}] and a
+// last-block combine; returns true in the last block with sums in out0/out1.
+template <typename T>
+__device__ bool reduce_pair(T a, T b, unsigned* ticket, double* partials, double* out0,
+                            double* out1)
+{
+    __shared__ T lds_t[2][kWavesPerBlock];
+    __shared__ double lds_d[2][kWavesPerBlock];
+    __shared__ int flag;
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    if (lane == 0) {
+        lds_t[0][wave] = a;
+        lds_t[1][wave] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T s0 = lds_t[0][0], s1 = lds_t[1][0];
+        for (int w = 1; w < kWavesPerBlock; ++w) {
+            s0 += lds_t[0][w];
+            s1 += lds_t[1][w];
+        }
+        partials[2 * blockIdx.x] = static_cast<double>(s0);
+        partials[2 * blockIdx.x + 1] = static_cast<double>(s1);
+    }
+    if (!last_block_ticket(ticket, &flag)) {
+        return false;
+    }
+    double c0 = 0, c1 = 0;
+    for (unsigned k = threadIdx.x; k < gridDim.x; k += kBlock) {
+        c0 += partials[2 * k];
+        c1 += partials[2 * k + 1];
+    }
+    c0 = wave_sum(c0);
+    c1 = wave_sum(c1);
+    if (lane == 0) {
+        lds_d[0][wave] = c0;
+        lds_d[1][wave] = c1;
+    }
+    __syncthreads();
+    double s0 = lds_d[0][0], s1 = lds_d[1][0];
+    for (int w = 1; w < kWavesPerBlock; ++w) {
+        s0 += lds_d[0][w];
+        s1 += lds_d[1][w];
+    }
+    *out0 = s0;
+    *out1 = s1;
+    if (threadIdx.x == 0) {
+        *ticket = 0;
+    }
+    return true;
+}
+
+// Start: r = b - A x (sub-wave SpMV), zero p_a / p_b / q, and the baselines
+// ||b|| and ||r0|| of every residual criterion; resets status and flags.
+template <typename T, int SW>
+__global__ void __launch_bounds__(kBlock)
+    cg_fused_init_kernel(int64_t n, const int32_t* __restrict__ rp,
+                         const int32_t* __restrict__ ci, const T* __restrict__ v,
+                         const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ r,
+                         T* __restrict__ pa, T* __restrict__ pb, T* __restrict__ q,
+                         gkomi_criteria crit, T* __restrict__ scalars, T* __restrict__ orig,
+                         uint8_t* __restrict__ status, gkomi_stop_flags* __restrict__ flags,
+                         unsigned* ticket, double* partials)
+{
+    constexpr int kRows = kBlock / SW;
+    const int g = threadIdx.x / SW;
+    const int l = threadIdx.x % SW;
+    int64_t rb0, rb1;
+    block_range(ceildiv(n, kRows), &rb0, &rb1);
+    T bb = 0, rr = 0;
+    for (int64_t blk = rb0; blk < rb1; ++blk) {
+        const int64_t row = blk * kRows + g;
+        T sum = 0;
+        if (row < n) {
+            const int32_t end = rp[row + 1];
+            for (int32_t k = rp[row] + l; k < end; k += SW) sum += v[k] * x[ci[k]];
+        }
+        sum = group_sum<SW>(sum);
+        if (row < n && l == 0) {
+            const T bi = b[row];
+            const T ri = bi - sum;
+            r[row] = ri;
+            pa[row] = T(0);
+            pb[row] = T(0);
+            q[row] = T(0);
+            bb += bi * bi;
+            rr += ri * ri;
+        }
+    }
+    double sbb, srr;
+    if (reduce_pair(bb, rr, ticket, partials, &sbb, &srr) && threadIdx.x == 0) {
+        for (int k = 0; k < crit.num; ++k) {
+            const int base = crit.c[k].baseline;
+            orig[k] = base == GKOMI_BASELINE_RHS_NORM
+                          ? static_cast<T>(sqrt(sbb))
+                          : (base == GKOMI_BASELINE_INITIAL_RESNORM ? static_cast<T>(sqrt(srr))
+                                                                    : T(1));
+        }
+        scalars[0] = T(0);  // rho
+        scalars[1] = T(1);  // prev_rho
+        scalars[2] = T(0);  // beta
+        status[0] = 0;
+        flags->iteration = -1;
+        flags->all_stopped = 0;
+        flags->one_changed = 0;
+    }
+}
+
+// Kernel 1 of an iteration: p_out = z + rho/prev_rho p_in (p_out = z when
+// prev_rho == 0, reference/solver/cg_kernels.cpp:64-70), q = A p_out with
+// p_out recomputed at the gathered columns, and beta = p.q.
+template <typename T, int SW>
+__global__ void __launch_bounds__(kBlock)
+    cg_fused_direction_kernel(int64_t n, const int32_t* __restrict__ rp,
+                              const int32_t* __restrict__ ci, const T* __restrict__ v,
+                              const T* __restrict__ z, const T* __restrict__ p_in,
+                              T* __restrict__ p_out, T* __restrict__ q, T* __restrict__ scalars,
+                              const gkomi_stop_flags* __restrict__ flags, unsigned* ticket,
+                              double* partials)
+{
+    if (flags->all_stopped) return;
+    const T rho = scalars[0];
+    const T prev = scalars[1];
+    const bool restart = prev == T(0);
+    const T tmp = restart ? T(0) : rho / prev;
+    constexpr int kRows = kBlock / SW;
+    const int g = threadIdx.x / SW;
+    const int l = threadIdx.x % SW;
+    int64_t rb0, rb1;
+    block_range(ceildiv(n, kRows), &rb0, &rb1);
+    T pq = 0;
+    for (int64_t blk = rb0; blk < rb1; ++blk) {
+        const int64_t row = blk * kRows + g;
+        T sum = 0;
+        if (row < n) {
+            const int32_t end = rp[row + 1];
+            for (int32_t k = rp[row] + l; k < end; k += SW) {
+                const int32_t c = ci[k];
+                const T pc = restart ? z[c] : fma(tmp, p_in[c], z[c]);
+                sum += v[k] * pc;
+            }
+        }
+        sum = group_sum<SW>(sum);
+        if (row < n && l == 0) {
+            const T pi = restart ? z[row] : fma(tmp, p_in[row], z[row]);
+            p_out[row] = pi;
+            q[row] = sum;
+            pq += pi * sum;
+        }
+    }
+    double spq, unused;
+    if (reduce_pair(pq, T(0), ticket, partials, &spq, &unused) && threadIdx.x == 0) {
+        scalars[2] = static_cast<T>(spq);
+    }
+}
+
+// Kernel 2 of an iteration (and, with Update = false, the tail of the start):
+// x += rho/beta p, r -= rho/beta q (skipped when beta == 0,
+// reference/solver/cg_kernels.cpp:89-95), z = M r, rho_new = r.z, ||r||,
+// prev_rho = rho, rho = rho_new, iteration + 1, Combined check.
+template <typename T, bool Update, bool Jacobi>
+__global__ void __launch_bounds__(kBlock)
+    cg_fused_update_kernel(int64_t n, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                           const T* __restrict__ p, const T* __restrict__ q,
+                           const T* __restrict__ inv_diag, gkomi_criteria crit,
+                           T* __restrict__ scalars, const T* __restrict__ orig,
+                           uint8_t* __restrict__ status, gkomi_stop_flags* __restrict__ flags,
+                           unsigned* ticket, double* partials)
+{
+    if (flags->all_stopped) return;
+    const T beta = scalars[2];
+    const bool do_update = Update && beta != T(0);
+    const T alpha = do_update ? scalars[0] / beta : T(0);
+    int64_t i0, i1;
+    block_range(ceildiv(n, kBlock), &i0, &i1);
+    T rz = 0, rr = 0;
+    for (int64_t blk = i0; blk < i1; ++blk) {
+        const int64_t i = blk * kBlock + threadIdx.x;
+        if (i >= n) continue;
+        T ri = r[i];
+        if (do_update) {
+            x[i] = fma(alpha, p[i], x[i]);
+            ri = fma(-alpha, q[i], ri);
+            r[i] = ri;
+        }
+        const T zi = Jacobi ? ri * inv_diag[i] : ri;
+        z[i] = zi;
+        rz += ri * zi;
+        rr += ri * ri;
+    }
+    double srz, srr;
+    if (reduce_pair(rz, rr, ticket, partials, &srz, &srr) && threadIdx.x == 0) {
+        const T rho_new = static_cast<T>(srz);
+        const T tau = static_cast<T>(sqrt(srr));
+        if (Update) {
+            scalars[1] = scalars[0];
+        }
+        scalars[0] = rho_new;
+        scalars[3] = tau;
+        const int64_t iter = flags->iteration + 1;
+        int all = 0, changed = 0;
+        combined_check<T>(crit, iter, 1, &tau, &rho_new, orig, true, status, &all, &changed);
+        flags->iteration = iter;
+        flags->all_stopped = all;
+        flags->one_changed = changed;
+    }
+}
+
+int fused_grid(int64_t items_per_row_block_count)
+{
+    int64_t g = std::min<int64_t>(items_per_row_block_count, static_cast<int64_t>(num_cus()) * 8);
+    g = std::min<int64_t>(g, kCgMaxGrid);
+    return static_cast<int>(std::max<int64_t>(g, 1));
+}
+
+int check_crit(const gkomi_criteria* crit)
+{
+    GKOMI_REQUIRE(crit != nullptr && crit->num >= 1 && crit->num <= GKOMI_MAX_CRITERIA,
+                  GKOMI_NOT_SUPPORTED, "criteria: between 1 and 4 criteria are supported");
+    for (int k = 0; k < crit->num; ++k) {
+        const int kind = crit->c[k].kind;
+        GKOMI_REQUIRE(kind == GKOMI_CRIT_ITERATION || kind == GKOMI_CRIT_RESIDUAL_NORM ||
+                          kind == GKOMI_CRIT_IMPLICIT_RESNORM,
+                      GKOMI_NOT_SUPPORTED, "criteria: unknown kind");
+        GKOMI_REQUIRE(kind == GKOMI_CRIT_ITERATION ||
+                          (crit->c[k].baseline >= 0 && crit->c[k].baseline <= 2),
+                      GKOMI_NOT_SUPPORTED, "criteria: unknown baseline");
+    }
+    return GKOMI_SUCCESS;
+}
+
+int check_cg_state(const gkomi_cg_state* st)
+{
+    GKOMI_REQUIRE(st != nullptr, GKOMI_INVALID_ARGUMENT, "null CG state");
+    GKOMI_REQUIRE(st->n >= 0, GKOMI_INVALID_ARGUMENT, "CG state: negative size");
+    GKOMI_REQUIRE(st->x && st->b && st->r && st->z && st->p_a && st->p_b && st->q &&
+                      st->scalars && st->orig_tau && st->status && st->flags && st->work,
+                  GKOMI_INVALID_ARGUMENT, "CG state: null pointer");
+    return GKOMI_SUCCESS;
+}
+
+template <typename T>
+int cg_fused_start_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_criteria* crit,
+                        gkomi_cg_state* st)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_csr(a)) return rc;
+    if (int rc = check_crit(crit)) return rc;
+    if (int rc = check_cg_state(st)) return rc;
+    GKOMI_REQUIRE(a->n_rows == a->n_cols && a->n_rows == st->n, GKOMI_DIMENSION_MISMATCH,
+                  "fused CG: matrix and vectors do not conform");
+    GKOMI_REQUIRE(a->strategy == GKOMI_CSR_CLASSICAL, GKOMI_NOT_SUPPORTED,
+                  "fused CG: needs the classical (sub-wave) CSR strategy");
+    const int sw = pick_subwave(a);
+    hipStream_t s = stream_of(ctx);
+    cg_work w = cg_work_layout(st->work);
+    const T* v = static_cast<const T*>(a->values);
+    T* sc = static_cast<T*>(st->scalars);
+    const int64_t n = st->n;
+    const int grid = fused_grid(ceildiv(std::max<int64_t>(n, 1), kBlock / sw));
+#define GKOMI_INIT_CASE(S)                                                                   \
+    case S:                                                                                  \
+        cg_fused_init_kernel<T, S><<<grid, kBlock, 0, s>>>(                                  \
+            n, a->row_ptrs, a->col_idxs, v, static_cast<const T*>(st->b),                    \
+            static_cast<const T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->p_a),  \
+            static_cast<T*>(st->p_b), static_cast<T*>(st->q), *crit, sc,                     \
+            static_cast<T*>(st->orig_tau), st->status, st->flags, w.tickets, w.partials);    \
+        break;
+    switch (sw) {
+        GKOMI_INIT_CASE(1)
+        GKOMI_INIT_CASE(2)
+        GKOMI_INIT_CASE(4)
+        GKOMI_INIT_CASE(8)
+        GKOMI_INIT_CASE(16)
+        GKOMI_INIT_CASE(32)
+        GKOMI_INIT_CASE(64)
+        default:
+            return fail(GKOMI_INVALID_ARGUMENT, "fused CG: bad subwave");
+    }
+#undef GKOMI_INIT_CASE
+    GKOMI_CHECK_LAUNCH("cg_fused_init_kernel");
+    const int ugrid = fused_grid(ceildiv(std::max<int64_t>(n, 1), kBlock));
+    unsigned* t2 = w.tickets + 16;
+    double* p2 = w.partials + 2 * kCgMaxGrid;
+    if (st->inv_diag) {
+        cg_fused_update_kernel<T, false, true><<<ugrid, kBlock, 0, s>>>(
+            n, static_cast<T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->z),
+            static_cast<const T*>(st->p_a), static_cast<const T*>(st->q),
+            static_cast<const T*>(st->inv_diag), *crit, sc, static_cast<const T*>(st->orig_tau),
+            st->status, st->flags, t2, p2);
+    } else {
+        cg_fused_update_kernel<T, false, false><<<ugrid, kBlock, 0, s>>>(
+            n, static_cast<T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->z),
+            static_cast<const T*>(st->p_a), static_cast<const T*>(st->q), nullptr, *crit, sc,
+            static_cast<const T*>(st->orig_tau), st->status, st->flags, t2, p2);
+    }
+    GKOMI_CHECK_LAUNCH("cg_fused_update_kernel");
+    return GKOMI_SUCCESS;
+}
+
+template <typename T>
+int cg_fused_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_criteria* crit,
+                          gkomi_cg_state* st, int32_t count)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_csr(a)) return rc;
+    if (int rc = check_crit(crit)) return rc;
+    if (int rc = check_cg_state(st)) return rc;
+    GKOMI_REQUIRE(a->n_rows == a->n_cols && a->n_rows == st->n, GKOMI_DIMENSION_MISMATCH,
+                  "fused CG: matrix and vectors do not conform");
+    GKOMI_REQUIRE(a->strategy == GKOMI_CSR_CLASSICAL, GKOMI_NOT_SUPPORTED,
+                  "fused CG: needs the classical (sub-wave) CSR strategy");
+    GKOMI_REQUIRE(count >= 0, GKOMI_INVALID_ARGUMENT, "fused CG: negative count");
+    const int sw = pick_subwave(a);
+    hipStream_t s = stream_of(ctx);
+    cg_work w = cg_work_layout(st->work);
+    const T* v = static_cast<const T*>(a->values);
+    T* sc = static_cast<T*>(st->scalars);
+    const int64_t n = st->n;
+    const int dgrid = fused_grid(ceildiv(std::max<int64_t>(n, 1), kBlock / sw));
+    const int ugrid = fused_grid(ceildiv(std::max<int64_t>(n, 1), kBlock));
+    unsigned* t1 = w.tickets;
+    unsigned* t2 = w.tickets + 16;
+    double* p1 = w.partials;
+    double* p2 = w.partials + 2 * kCgMaxGrid;
+    for (int32_t it = 0; it < count; ++it) {
+        T* pin = static_cast<T*>(st->p_a);
+        T* pout = static_cast<T*>(st->p_b);
+#define GKOMI_DIR_CASE(S)                                                                  \
+    case S:                                                                                \
+        cg_fused_direction_kernel<T, S><<<dgrid, kBlock, 0, s>>>(                          \
+            n, a->row_ptrs, a->col_idxs, v, static_cast<const T*>(st->z), pin, pout,       \
+            static_cast<T*>(st->q), sc, st->flags, t1, p1);                                \
+        break;
+        switch (sw) {
+            GKOMI_DIR_CASE(1)
+            GKOMI_DIR_CASE(2)
+            GKOMI_DIR_CASE(4)
+            GKOMI_DIR_CASE(8)
+            GKOMI_DIR_CASE(16)
+            GKOMI_DIR_CASE(32)
+            GKOMI_DIR_CASE(64)
+            default:
+                return fail(GKOMI_INVALID_ARGUMENT, "fused CG: bad subwave");
+        }
+#undef GKOMI_DIR_CASE
+        GKOMI_CHECK_LAUNCH("cg_fused_direction_kernel");
+        if (st->inv_diag) {
+            cg_fused_update_kernel<T, true, true><<<ugrid, kBlock, 0, s>>>(
+                n, static_cast<T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->z), pout,
+                static_cast<const T*>(st->q), static_cast<const T*>(st->inv_diag), *crit, sc,
+                static_cast<const T*>(st->orig_tau), st->status, st->flags, t2, p2);
+        } else {
+            cg_fused_update_kernel<T, true, false><<<ugrid, kBlock, 0, s>>>(
+                n, static_cast<T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->z), pout,
+                static_cast<const T*>(st->q), nullptr, *crit, sc,
+                static_cast<const T*>(st->orig_tau), st->status, st->flags, t2, p2);
+        }
+        GKOMI_CHECK_LAUNCH("cg_fused_update_kernel");
+        std::swap(st->p_a, st->p_b);
+    }
+    return GKOMI_SUCCESS;
+}
+
+// ------------------------------------------------- step-kernel drivers ----
+
+inline unsigned egrid(int64_t total) { return grid_for(total, kBlock * 4); }
+
+int check_block(int64_t n, int64_t nrhs, int64_t vs)
+{
+    GKOMI_REQUIRE(n >= 0 && nrhs >= 0, GKOMI_INVALID_ARGUMENT, "negative size");
+    GKOMI_REQUIRE(vs >= nrhs, GKOMI_INVALID_ARGUMENT, "stride smaller than the columns");
+    return GKOMI_SUCCESS;
+}
+
+}  // namespace gkomi
+
+using namespace gkomi;
+
+#define GKOMI_BEGIN(n, nrhs, vs)                            \
+    if (int rc_ = enter(ctx)) return rc_;                   \
+    if (int rc_ = check_block(n, nrhs, vs)) return rc_;     \
+    if ((n) * (nrhs) == 0) return GKOMI_SUCCESS;            \
+    hipStream_t s_ = stream_of(ctx);
+
+extern "C" {
+
+#define GKOMI_CG_INIT(S, T)                                                                    \
+    int gkomi_cg_initialize_##S(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const T* b,     \
+                                int64_t bs, T* r, T* z, T* p, T* q, int64_t vs, T* prev_rho,   \
+                                T* rho, uint8_t* status)                                       \
+    {                                                                                          \
+        GKOMI_BEGIN(n, nrhs, vs)                                                               \
+        cg_initialize_kernel<T><<<egrid(n * nrhs), kBlock, 0, s_>>>(n, nrhs, b, bs, r, z, p, q, \
+                                                                   vs, prev_rho, rho, status); \
+        GKOMI_CHECK_LAUNCH("cg_initialize");                                                   \
+        return GKOMI_SUCCESS;                                                                  \
+    }                                                                                          \
+    int gkomi_cg_step_1_##S(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, T* p, const T* z,   \
+                            int64_t vs, const T* rho, const T* prev_rho,                       \
+                            const uint8_t* status)                                             \
+    {                                                                                          \
+        GKOMI_BEGIN(n, nrhs, vs)                                                               \
+        cg_step_1_kernel<T><<<egrid(n * nrhs), kBlock, 0, s_>>>(n, nrhs, p, z, vs, rho,        \
+                                                               prev_rho, status);              \
+        GKOMI_CHECK_LAUNCH("cg_step_1");                                                       \
+        return GKOMI_SUCCESS;                                                                  \
+    }                                                                                          \
+    int gkomi_cg_step_2_##S(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, T* x, int64_t xs,   \
+                            T* r, const T* p, const T* q, int64_t vs, const T* beta,           \
+                            const T* rho, const uint8_t* status)                               \
+    {                                                                                          \
+        GKOMI_BEGIN(n, nrhs, vs)                                                               \
+        GKOMI_REQUIRE(xs >= nrhs, GKOMI_INVALID_ARGUMENT, "x stride");                         \
+        cg_step_2_kernel<T><<<egrid(n * nrhs), kBlock, 0, s_>>>(n, nrhs, x, xs, r, p, q, vs,   \
+                                                               beta, rho, status);             \
+        GKOMI_CHECK_LAUNCH("cg_step_2");                                                       \
+        return GKOMI_SUCCESS;                                                                  \
+    }                                                                                          \
+    int gkomi_bicgstab_initialize_##S(const gkomi_ctx* ctx, int64_t n, int64_t nrhs,          \
+                                      const T* b, int64_t bs, T* r, T* rr, T* y, T* s, T* t,   \
+                                      T* z, T* v, T* p, int64_t vs, T* prev_rho, T* rho,       \
+                                      T* alpha, T* beta, T* gamma, T* omega, uint8_t* status)  \
+    {                                                                                          \
+        GKOMI_BEGIN(n, nrhs, vs)                                                               \
+        bicgstab_initialize_kernel<T><<<egrid(n * nrhs), kBlock, 0, s_>>>(                     \
+            n, nrhs, b, bs, r, rr, y, s, t, z, v, p, vs, prev_rho, rho, alpha, beta, gamma,    \
+            omega, status);                                                                    \
+        GKOMI_CHECK_LAUNCH("bicgstab_initialize");                                             \
+        return GKOMI_SUCCESS;                                                                  \
+    }                                                                                          \
+    int gkomi_bicgstab_step_1_##S(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const T* r,  \
+                                  T* p, const T* v, int64_t vs, const T* rho,                  \
+                                  const T* prev_rho, const T* alpha, const T* omega,           \
+                                  const uint8_t* status)                                       \
+    {                                                                                          \
+        GKOMI_BEGIN(n, nrhs, vs)                                                               \
+        bicgstab_step_1_kernel<T><<<egrid(n * nrhs), kBlock, 0, s_>>>(                         \
+            n, nrhs, r, p, v, vs, rho, prev_rho, alpha, omega, status);                        \
+        GKOMI_CHECK_LAUNCH("bicgstab_step_1");                                                 \
+        return GKOMI_SUCCESS;                                                                  \
+    }                                                                                          \
+    int gkomi_bicgstab_step_2_##S(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const T* r,  \
+                                  T* s, const T* v, int64_t vs, const T* rho, T* alpha,        \
+                                  const T* beta, const uint8_t* status)                        \
+    {                                                                                          \
+        GKOMI_BEGIN(n, nrhs, vs)                                                               \
+        bicgstab_step_2_kernel<T><<<egrid(n * nrhs), kBlock, 0, s_>>>(n, nrhs, r, s, v, vs,   \
+                                                                     rho, alpha, beta, status); \
+        GKOMI_CHECK_LAUNCH("bicgstab_step_2");                                                 \
+        return GKOMI_SUCCESS;                                                                  \
+    }                                                                                          \
+    int gkomi_bicgstab_step_3_##S(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, T* x,        \
+                                  int64_t xs, T* r, const T* s, const T* t, const T* y,        \
+                                  const T* z, int64_t vs, const T* alpha, const T* beta,       \
+                                  const T* gamma, T* omega, const uint8_t* status)             \
+    {                                                                                          \
+        GKOMI_BEGIN(n, nrhs, vs)                                                               \
+        GKOMI_REQUIRE(xs >= nrhs, GKOMI_INVALID_ARGUMENT, "x stride");                         \
+        bicgstab_step_3_kernel<T><<<egrid(n * nrhs), kBlock, 0, s_>>>(                         \
+            n, nrhs, x, xs, r, s, t, y, z, vs, alpha, beta, gamma, omega, status);             \
+        bicgstab_omega_kernel<T><<<1, kBlock, 0, s_>>>(nrhs, beta, gamma, omega, status);      \
+        GKOMI_CHECK_LAUNCH("bicgstab_step_3");                                                 \
+        return GKOMI_SUCCESS;                                                                  \
+    }                                                                                          \
+    int gkomi_bicgstab_finalize_##S(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, T* x,      \
+                                    int64_t xs, const T* y, int64_t vs, const T* alpha,        \
+                                    uint8_t* status)                                           \
+    {                                                                                          \
+        if (int rc_ = enter(ctx)) return rc_;                                                  \
+        if (int rc_ = check_block(n, nrhs, vs)) return rc_;                                    \
+        hipStream_t s_ = stream_of(ctx);                                                       \
+        if (n * nrhs > 0) {                                                                    \
+            bicgstab_finalize_kernel<T><<<egrid(n * nrhs), kBlock, 0, s_>>>(n, nrhs, x, xs, y, \
+                                                                           vs, alpha, status); \
+        }                                                                                      \
+        if (n * nrhs > 0) status_finalize_kernel<<<1, kBlock, 0, s_>>>(nrhs, status);          \
+        GKOMI_CHECK_LAUNCH("bicgstab_finalize");                                               \
+        return GKOMI_SUCCESS;                                                                  \
+    }                                                                                          \
+    int gkomi_criteria_check_##S(const gkomi_ctx* ctx, const gkomi_criteria* crit,             \
+                                 int64_t nrhs, const T* tau, const T* implicit_sq,             \
+                                 const T* orig_tau, int32_t set_finalized, uint8_t* status,    \
+                                 gkomi_stop_flags* flags)                                      \
+    {                                                                                          \
+        if (int rc_ = enter(ctx)) return rc_;                                                  \
+        if (int rc_ = check_crit(crit)) return rc_;                                            \
+        GKOMI_REQUIRE(status && flags && nrhs >= 0, GKOMI_INVALID_ARGUMENT,                    \
+                      "criteria check: null status/flags");                                   \
+        for (int k = 0; k < crit->num; ++k) {                                                  \
+            if (crit->c[k].kind == GKOMI_CRIT_IMPLICIT_RESNORM) {                              \
+                GKOMI_REQUIRE(implicit_sq && orig_tau, GKOMI_NOT_SUPPORTED,                    \
+                              "ImplicitResidualNorm needs the implicit squared norm");        \
+            }                                                                                  \
+            if (crit->c[k].kind == GKOMI_CRIT_RESIDUAL_NORM) {                                 \
+                GKOMI_REQUIRE(orig_tau, GKOMI_INVALID_ARGUMENT, "null baselines");            \
+            }                                                                                  \
+        }                                                                                      \
+        criteria_check_kernel<T><<<1, 64, 0, stream_of(ctx)>>>(                                \
+            *crit, nrhs, tau, implicit_sq, orig_tau, set_finalized, status, flags);            \
+        GKOMI_CHECK_LAUNCH("criteria_check");                                                  \
+        return GKOMI_SUCCESS;                                                                  \
+    }                                                                                          \
+    int gkomi_cg_fused_start_##S(const gkomi_ctx* ctx, const gkomi_csr* a,                     \
+                                 const gkomi_criteria* crit, gkomi_cg_state* st)               \
+    {                                                                                          \
+        return cg_fused_start_impl<T>(ctx, a, crit, st);                                       \
+    }                                                                                          \
+    int gkomi_cg_fused_iterate_##S(const gkomi_ctx* ctx, const gkomi_csr* a,                   \
+                                   const gkomi_criteria* crit, gkomi_cg_state* st,             \
+                                   int32_t count)                                              \
+    {                                                                                          \
+        return cg_fused_iterate_impl<T>(ctx, a, crit, st, count);                              \
+    }
+
+GKOMI_CG_INIT(f64, double)
+GKOMI_CG_INIT(f32, float)
+
+size_t gkomi_cg_work_bytes(int64_t n)
+{
+    (void)n;
+    return kTicketBytes + sizeof(double) * 4 * static_cast<size_t>(kCgMaxGrid);
+}
+
+void* gkomi_cg_current_p(const gkomi_cg_state* st) { return st ? st->p_a : nullptr; }
+
+}  // extern "C"

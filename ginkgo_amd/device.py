@@ -1,0 +1,330 @@
+"""Thin PyTorch-ROCm plumbing over the C ABI (device memory and streams).
+
+torch provides device allocations and the stream; every computation is a
+call into libgkomi.so.  Matrices are held as torch tensors plus the plain C
+descriptor the C ABI takes.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _capi as C
+
+_VT = {torch.float64: "f64", torch.float32: "f32"}
+
+
+def suffix(dtype):
+    try:
+        return _VT[dtype]
+    except KeyError:
+        raise C.NotSupported(f"value type {dtype} is not supported (float64/float32 only)")
+
+
+def ctx(device=None, stream=None):
+    """gkomi_ctx for torch's current (or given) device and stream."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    s = torch.cuda.current_stream(dev) if stream is None else stream
+    c = C.Ctx()
+    c.device = dev.index if dev.index is not None else torch.cuda.current_device()
+    c.stream = ctypes.c_void_p(s.cuda_stream)
+    return c
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() > 0 else ctypes.c_void_p(0)
+
+
+def workspace(nbytes, device):
+    """Zero-filled scratch (the reductions require a zeroed ticket)."""
+    return torch.zeros(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+def scalar(value, dtype, device):
+    return torch.full((1,), float(value), dtype=dtype, device=device)
+
+
+class DeviceCsr:
+    """Csr<V, int32> on the device with its SpMV strategy.
+
+    strategy: 'classical', 'load_balance' or 'automatical' (picks classical
+    for short, uniform rows and load_balance otherwise, as the reference's
+    automatical::process does from row statistics, csr.hpp:606-673)."""
+
+    def __init__(self, row_ptrs, col_idxs, values, n_cols=None, strategy="automatical",
+                 subwave=0, device="cuda"):
+        dev = torch.device(device)
+        self.row_ptrs = torch.as_tensor(np.asarray(row_ptrs, dtype=np.int32)).to(dev) \
+            if not torch.is_tensor(row_ptrs) else row_ptrs.to(dev, torch.int32)
+        self.col_idxs = torch.as_tensor(np.asarray(col_idxs, dtype=np.int32)).to(dev) \
+            if not torch.is_tensor(col_idxs) else col_idxs.to(dev, torch.int32)
+        self.values = torch.as_tensor(np.asarray(values)).to(dev) \
+            if not torch.is_tensor(values) else values.to(dev)
+        self.dtype = self.values.dtype
+        suffix(self.dtype)
+        self.n_rows = self.row_ptrs.numel() - 1
+        self.n_cols = self.n_rows if n_cols is None else int(n_cols)
+        self.nnz = self.col_idxs.numel()
+        self.device = dev
+        rp = self.row_ptrs.cpu().numpy() if strategy == "automatical" else None
+        if strategy == "automatical":
+            lens = np.diff(rp) if self.n_rows > 0 else np.zeros(1)
+            mx = int(lens.max()) if lens.size else 0
+            mean = float(lens.mean()) if lens.size else 0.0
+            strategy = "classical" if (mx <= 64 and mx <= 4 * mean + 4) else "load_balance"
+        self.strategy = strategy
+        self.subwave = int(subwave)
+        self.srow = None
+        self.srow_blocks = 0
+        if strategy == "load_balance":
+            chunk = int(C.lib().gkomi_csr_lb_chunk(self.values.element_size()))
+            self.srow_blocks = max(1, -(-self.nnz // chunk))
+            self.srow = torch.empty(self.srow_blocks + 1, dtype=torch.int32, device=dev)
+            c = ctx(dev)
+            C.call("gkomi_csr_lb_partition", ctypes.byref(c), self.n_rows, ptr(self.row_ptrs),
+                   self.nnz, chunk, ptr(self.srow), self.srow_blocks)
+        elif strategy != "classical":
+            raise C.InvalidArgument(f"unknown strategy {strategy}")
+        self.desc = self._desc()
+
+    def _desc(self):
+        d = C.Csr()
+        d.n_rows, d.n_cols, d.nnz = self.n_rows, self.n_cols, self.nnz
+        d.row_ptrs = ptr(self.row_ptrs).value
+        d.col_idxs = ptr(self.col_idxs).value
+        d.values = ptr(self.values).value
+        d.strategy = C.CSR_CLASSICAL if self.strategy == "classical" else C.CSR_LOAD_BALANCE
+        d.subwave = self.subwave
+        d.srow = ptr(self.srow).value if self.srow is not None else None
+        d.srow_blocks = self.srow_blocks
+        return d
+
+    def spmv(self, b, out=None, stream=None):
+        b2 = b.reshape(b.shape[0], -1)
+        if out is None:
+            out = torch.empty((self.n_rows, b2.shape[1]), dtype=self.dtype, device=self.device)
+        o2 = out.reshape(out.shape[0], -1)
+        if b2.shape[0] != self.n_cols or o2.shape[0] != self.n_rows or o2.shape[1] != b2.shape[1]:
+            raise C.DimensionMismatch("spmv: operand sizes do not conform")
+        c = ctx(self.device, stream)
+        C.call(f"gkomi_csr_spmv_{suffix(self.dtype)}", ctypes.byref(c), ctypes.byref(self.desc),
+               b2.shape[1], ptr(b2), b2.stride(0), ptr(o2), o2.stride(0))
+        return out
+
+    def advanced_spmv(self, alpha, b, beta, c_vec, stream=None):
+        b2 = b.reshape(b.shape[0], -1)
+        c2 = c_vec.reshape(c_vec.shape[0], -1)
+        a = alpha if torch.is_tensor(alpha) else scalar(alpha, self.dtype, self.device)
+        bt = beta if torch.is_tensor(beta) else scalar(beta, self.dtype, self.device)
+        c = ctx(self.device, stream)
+        C.call(f"gkomi_csr_advanced_spmv_{suffix(self.dtype)}", ctypes.byref(c), ptr(a),
+               ctypes.byref(self.desc), b2.shape[1], ptr(b2), b2.stride(0), ptr(bt), ptr(c2),
+               c2.stride(0))
+        return c_vec
+
+    def max_row_nnz(self):
+        out = torch.zeros(1, dtype=torch.int64, device=self.device)
+        c = ctx(self.device)
+        C.call("gkomi_csr_max_row_nnz", ctypes.byref(c), self.n_rows, ptr(self.row_ptrs),
+               ptr(out))
+        return int(out.item())
+
+    def transpose(self):
+        t_rp = torch.empty(self.n_cols + 1, dtype=torch.int32, device=self.device)
+        t_ci = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=self.device)
+        t_v = torch.empty(max(self.nnz, 1), dtype=self.dtype, device=self.device)
+        wsb = C.lib().gkomi_csr_transpose_workspace_bytes(self.n_rows, self.n_cols, self.nnz)
+        ws = workspace(wsb, self.device)
+        c = ctx(self.device)
+        C.call(f"gkomi_csr_transpose_{suffix(self.dtype)}", ctypes.byref(c),
+               ctypes.byref(self.desc), ptr(t_rp), ptr(t_ci), ptr(t_v), ptr(ws), wsb)
+        return t_rp, t_ci[: self.nnz], t_v[: self.nnz]
+
+    def jacobi(self):
+        inv = torch.empty(self.n_rows, dtype=self.dtype, device=self.device)
+        c = ctx(self.device)
+        C.call(f"gkomi_jacobi_generate_{suffix(self.dtype)}", ctypes.byref(c),
+               ctypes.byref(self.desc), ptr(inv))
+        return inv
+
+
+class DeviceSellp:
+    """Sellp<V, int32> built on the device from a DeviceCsr."""
+
+    def __init__(self, csr, slice_size=64, stride_factor=1):
+        self.n_rows, self.n_cols = csr.n_rows, csr.n_cols
+        self.dtype, self.device = csr.dtype, csr.device
+        self.slice_size, self.stride_factor = int(slice_size), int(stride_factor)
+        ns = -(-self.n_rows // self.slice_size)
+        self.num_slices = ns
+        self.slice_sets = torch.empty(ns + 1, dtype=torch.int64, device=self.device)
+        self.slice_lengths = torch.empty(max(ns, 1), dtype=torch.int64, device=self.device)
+        wsb = C.lib().gkomi_scan_workspace_bytes(ns)
+        ws = workspace(wsb, self.device)
+        c = ctx(self.device)
+        C.call("gkomi_sellp_compute_slice_sets", ctypes.byref(c), self.n_rows,
+               ptr(csr.row_ptrs), self.slice_size, self.stride_factor, ptr(self.slice_sets),
+               ptr(self.slice_lengths), ptr(ws), wsb)
+        total = int(self.slice_sets[-1].item()) * self.slice_size
+        self.col_idxs = torch.empty(max(total, 1), dtype=torch.int32, device=self.device)
+        self.values = torch.empty(max(total, 1), dtype=self.dtype, device=self.device)
+        C.call(f"gkomi_csr_to_sellp_{suffix(self.dtype)}", ctypes.byref(c),
+               ctypes.byref(csr.desc), self.slice_size, ptr(self.slice_sets),
+               ptr(self.slice_lengths), ptr(self.col_idxs), ptr(self.values))
+        self.total = total
+        d = C.Sellp()
+        d.n_rows, d.n_cols = self.n_rows, self.n_cols
+        d.slice_size, d.stride_factor = self.slice_size, self.stride_factor
+        d.slice_sets = ptr(self.slice_sets).value
+        d.slice_lengths = ptr(self.slice_lengths).value
+        d.col_idxs = ptr(self.col_idxs).value
+        d.values = ptr(self.values).value
+        self.desc = d
+
+    def spmv(self, b, out=None, stream=None):
+        b2 = b.reshape(b.shape[0], -1)
+        if out is None:
+            out = torch.empty((self.n_rows, b2.shape[1]), dtype=self.dtype, device=self.device)
+        o2 = out.reshape(out.shape[0], -1)
+        c = ctx(self.device, stream)
+        C.call(f"gkomi_sellp_spmv_{suffix(self.dtype)}", ctypes.byref(c), ctypes.byref(self.desc),
+               b2.shape[1], ptr(b2), b2.stride(0), ptr(o2), o2.stride(0))
+        return out
+
+    def advanced_spmv(self, alpha, b, beta, c_vec):
+        b2 = b.reshape(b.shape[0], -1)
+        c2 = c_vec.reshape(c_vec.shape[0], -1)
+        a = scalar(alpha, self.dtype, self.device)
+        bt = scalar(beta, self.dtype, self.device)
+        c = ctx(self.device)
+        C.call(f"gkomi_sellp_advanced_spmv_{suffix(self.dtype)}", ctypes.byref(c), ptr(a),
+               ctypes.byref(self.desc), b2.shape[1], ptr(b2), b2.stride(0), ptr(bt), ptr(c2),
+               c2.stride(0))
+        return c_vec
+
+
+class DeviceEll:
+    """Ell<V, int32> (column-major, stride = n_rows) built on the device."""
+
+    def __init__(self, csr, width=None, stride=None):
+        self.n_rows, self.n_cols = csr.n_rows, csr.n_cols
+        self.dtype, self.device = csr.dtype, csr.device
+        self.width = csr.max_row_nnz() if width is None else int(width)
+        self.stride = self.n_rows if stride is None else int(stride)
+        n = max(self.width * self.stride, 1)
+        self.col_idxs = torch.empty(n, dtype=torch.int32, device=self.device)
+        self.values = torch.empty(n, dtype=self.dtype, device=self.device)
+        c = ctx(self.device)
+        C.call(f"gkomi_csr_to_ell_{suffix(self.dtype)}", ctypes.byref(c), ctypes.byref(csr.desc),
+               self.width, self.stride, ptr(self.col_idxs), ptr(self.values))
+        d = C.Ell()
+        d.n_rows, d.n_cols, d.width, d.stride = self.n_rows, self.n_cols, self.width, self.stride
+        d.col_idxs = ptr(self.col_idxs).value
+        d.values = ptr(self.values).value
+        self.desc = d
+
+    def spmv(self, b, out=None, stream=None):
+        b2 = b.reshape(b.shape[0], -1)
+        if out is None:
+            out = torch.empty((self.n_rows, b2.shape[1]), dtype=self.dtype, device=self.device)
+        o2 = out.reshape(out.shape[0], -1)
+        c = ctx(self.device, stream)
+        C.call(f"gkomi_ell_spmv_{suffix(self.dtype)}", ctypes.byref(c), ctypes.byref(self.desc),
+               b2.shape[1], ptr(b2), b2.stride(0), ptr(o2), o2.stride(0))
+        return out
+
+    def advanced_spmv(self, alpha, b, beta, c_vec):
+        b2 = b.reshape(b.shape[0], -1)
+        c2 = c_vec.reshape(c_vec.shape[0], -1)
+        a = scalar(alpha, self.dtype, self.device)
+        bt = scalar(beta, self.dtype, self.device)
+        c = ctx(self.device)
+        C.call(f"gkomi_ell_advanced_spmv_{suffix(self.dtype)}", ctypes.byref(c), ptr(a),
+               ctypes.byref(self.desc), b2.shape[1], ptr(b2), b2.stride(0), ptr(bt), ptr(c2),
+               c2.stride(0))
+        return c_vec
+
+
+def dot(x, y):
+    x2 = x.reshape(x.shape[0], -1)
+    y2 = y.reshape(y.shape[0], -1)
+    out = torch.empty(x2.shape[1], dtype=x.dtype, device=x.device)
+    wsb = C.lib().gkomi_reduction_workspace_bytes(x2.shape[0], x2.shape[1])
+    ws = workspace(wsb, x.device)
+    c = ctx(x.device)
+    C.call(f"gkomi_dense_dot_{suffix(x.dtype)}", ctypes.byref(c), x2.shape[0], x2.shape[1],
+           ptr(x2), x2.stride(0), ptr(y2), y2.stride(0), ptr(out), ptr(ws), wsb)
+    return out
+
+
+def norm2(x):
+    x2 = x.reshape(x.shape[0], -1)
+    out = torch.empty(x2.shape[1], dtype=x.dtype, device=x.device)
+    wsb = C.lib().gkomi_reduction_workspace_bytes(x2.shape[0], x2.shape[1])
+    ws = workspace(wsb, x.device)
+    c = ctx(x.device)
+    C.call(f"gkomi_dense_norm2_{suffix(x.dtype)}", ctypes.byref(c), x2.shape[0], x2.shape[1],
+           ptr(x2), x2.stride(0), ptr(out), ptr(ws), wsb)
+    return out
+
+
+class FusedCg:
+    """Single-RHS CG on the fused gfx950 kernels with the convergence test
+    on the device; the host enqueues `chunk` iterations ahead and polls the
+    stop flags once per chunk."""
+
+    def __init__(self, csr, criteria_spec, jacobi=False, chunk=16):
+        if csr.strategy != "classical":
+            raise C.NotSupported("fused CG needs the classical CSR strategy")
+        self.A = csr
+        self.crit = C.criteria(criteria_spec)
+        self.inv_diag = csr.jacobi() if jacobi else None
+        self.chunk = int(chunk)
+        dev, dt, n = csr.device, csr.dtype, csr.n_rows
+        self.r = torch.empty(n, dtype=dt, device=dev)
+        self.z = torch.empty(n, dtype=dt, device=dev)
+        self.pa = torch.empty(n, dtype=dt, device=dev)
+        self.pb = torch.empty(n, dtype=dt, device=dev)
+        self.q = torch.empty(n, dtype=dt, device=dev)
+        self.scalars = torch.zeros(8, dtype=dt, device=dev)
+        self.orig = torch.zeros(C.MAX_CRITERIA, dtype=dt, device=dev)
+        self.status = torch.zeros(1, dtype=torch.uint8, device=dev)
+        self.flags = torch.zeros(2, dtype=torch.int64, device=dev)
+        wsb = C.lib().gkomi_cg_work_bytes(n)
+        self.work = workspace(wsb, dev)
+        self.flags_host = torch.zeros(2, dtype=torch.int64).pin_memory()
+
+    def _state(self, b, x):
+        st = C.CgState()
+        st.n = self.A.n_rows
+        st.x, st.b = ptr(x).value, ptr(b).value
+        st.r, st.z = ptr(self.r).value, ptr(self.z).value
+        st.p_a, st.p_b, st.q = ptr(self.pa).value, ptr(self.pb).value, ptr(self.q).value
+        st.inv_diag = ptr(self.inv_diag).value if self.inv_diag is not None else None
+        st.scalars, st.orig_tau = ptr(self.scalars).value, ptr(self.orig).value
+        st.status, st.flags, st.work = (ptr(self.status).value, ptr(self.flags).value,
+                                        ptr(self.work).value)
+        return st
+
+    def apply(self, b, x, max_chunks=1 << 30):
+        """Solve A x = b in place (x is the initial guess).  Returns the
+        iteration count of the reference's Convergence logger."""
+        st = self._state(b, x)
+        c = ctx(self.A.device)
+        sfx = suffix(self.A.dtype)
+        C.call(f"gkomi_cg_fused_start_{sfx}", ctypes.byref(c), ctypes.byref(self.A.desc),
+               ctypes.byref(self.crit), ctypes.byref(st))
+        done = False
+        chunks = 0
+        while not done and chunks < max_chunks:
+            C.call(f"gkomi_cg_fused_iterate_{sfx}", ctypes.byref(c), ctypes.byref(self.A.desc),
+                   ctypes.byref(self.crit), ctypes.byref(st), self.chunk)
+            self.flags_host.copy_(self.flags, non_blocking=True)
+            torch.cuda.current_stream(self.A.device).synchronize()
+            done = (int(self.flags_host[1].item()) & 0xffffffff) != 0
+            chunks += 1
+        return int(self.flags_host[0].item())
+
+    @property
+    def iteration(self):
+        return int(self.flags[0].item())

@@ -1,0 +1,483 @@
+/*
+ * gkomi.h -- C ABI of the MI355X (gfx950) kernel library libgkomi.so.
+ *
+ * This is the drop-in boundary of the sparse solver hot path: the kernel
+ * namespace that Ginkgo reaches through GKO_REGISTER_OPERATION
+ * (include/ginkgo/core/base/executor.hpp:419-474), one `extern "C"` symbol
+ * per (operation, value type), with index type int32.  Arguments are plain
+ * device pointers, sizes and strides; dense operands follow
+ * matrix::view::dense (include/ginkgo/core/matrix/device_views.hpp:24-45):
+ * element (i, j) of an n x k block is at ptr[i * stride + j].  Scalars that
+ * the reference passes as 1x1 / 1xk Dense views (alpha, beta, rho, ...) are
+ * device pointers here too.
+ *
+ * Every entry point returns a gkomi_status (0 = success).  On failure
+ * gkomi_last_error() describes the problem.  Work is enqueued on the context's
+ * stream and runs asynchronously, as the reference's HIP executor does; the
+ * library never frees caller memory and never allocates inside a kernel
+ * entry point: scratch comes from caller-provided workspace.
+ *
+ * Value-type suffixes: _f64 = double, _f32 = float.
+ */
+#ifndef GKOMI_H_
+#define GKOMI_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ basics */
+
+/* Error codes; the C++ layer maps them to the reference's exception types
+ * (include/ginkgo/core/base/exception.hpp: HipError, NotSupported,
+ * DimensionMismatch, AllocationError). */
+typedef enum {
+    GKOMI_SUCCESS = 0,
+    GKOMI_INVALID_ARGUMENT = 1,
+    GKOMI_NOT_SUPPORTED = 2,
+    GKOMI_ALLOCATION_ERROR = 3,
+    GKOMI_HIP_ERROR = 4,
+    GKOMI_DIMENSION_MISMATCH = 5
+} gkomi_status;
+
+/* Execution context: a device ordinal and the HIP stream (as an opaque
+ * pointer; NULL = the device's null stream) that orders all work.  Mirrors
+ * HipExecutor's device id + stream (include/ginkgo/core/base/executor.hpp:
+ * 1786-1988, hip/base/executor.hip.cpp:215-245). */
+typedef struct {
+    int32_t device;
+    int32_t reserved;
+    void* stream;
+} gkomi_ctx;
+
+/* Thread-local description of the last failure. */
+const char* gkomi_last_error(void);
+/* Library version string. */
+const char* gkomi_version(void);
+
+/* ------------------------------------------- executor / memory (A1 row) */
+
+typedef struct {
+    char name[64];
+    char arch[32];              /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+    int32_t num_cus;            /* multiProcessorCount */
+    int32_t warp_size;          /* 64 on CDNA */
+    int64_t max_lds_per_block;  /* sharedMemPerBlock */
+    int64_t total_global_mem;
+    int32_t clock_khz;
+    int32_t memory_clock_khz;
+    int32_t memory_bus_width;
+    int32_t reserved;
+} gkomi_device_info;
+
+/* Executor::get_num_devices / HipExecutor::get_device_id & properties
+ * (hip/base/executor.hip.cpp:215-245, include/ginkgo/core/base/executor.hpp:1786-1988). */
+int gkomi_get_device_count(int32_t* count);
+int gkomi_get_device_info(int32_t device, gkomi_device_info* info);
+/* hip/base/stream.hip.cpp (hip_stream): create / destroy a default-priority stream. */
+int gkomi_stream_create(int32_t device, void** stream);
+int gkomi_stream_destroy(int32_t device, void* stream);
+/* Executor::synchronize (include/ginkgo/core/base/executor.hpp:824-831). */
+int gkomi_synchronize(const gkomi_ctx* ctx);
+/* HipAllocator::allocate / deallocate (hip/base/memory.hip.cpp:56-111). */
+int gkomi_malloc(const gkomi_ctx* ctx, size_t bytes, void** ptr);
+int gkomi_free(const gkomi_ctx* ctx, void* ptr);
+/* Pinned host memory (HipHostAllocator, hip/base/memory.hip.cpp:113-175). */
+int gkomi_host_malloc(size_t bytes, void** ptr);
+int gkomi_host_free(void* ptr);
+/* Executor::copy_from / raw_copy_to (executor.hpp:745-815): stream-ordered copy
+ * between any two of host, pinned host and device memory. */
+int gkomi_memcpy_async(const gkomi_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Executor::copy_val_to_host (executor.hpp:809-815): blocking copy to host. */
+int gkomi_memcpy_to_host(const gkomi_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Timer (hip/base/timer.hip.cpp:25-57): HIP events on the context stream. */
+int gkomi_event_create(int32_t device, void** event);
+int gkomi_event_destroy(int32_t device, void* event);
+int gkomi_event_record(const gkomi_ctx* ctx, void* event);
+int gkomi_event_synchronize(void* event);
+int gkomi_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ------------------------------------------------------------- components */
+
+/* components::fill_array (core/components/fill_array_kernels.hpp); a kernel,
+ * safe under graph capture. */
+int gkomi_fill_zero(const gkomi_ctx* ctx, void* ptr, size_t bytes);
+int gkomi_fill_f64(const gkomi_ctx* ctx, int64_t n, double value, double* ptr);
+int gkomi_fill_f32(const gkomi_ctx* ctx, int64_t n, float value, float* ptr);
+/* components::convert_precision (core/components/precision_conversion_kernels.hpp). */
+int gkomi_convert_f64_to_f32(const gkomi_ctx* ctx, int64_t n, const double* in, float* out);
+int gkomi_convert_f32_to_f64(const gkomi_ctx* ctx, int64_t n, const float* in, double* out);
+
+/* Bytes of scratch the reductions below need for an n x nrhs operand. */
+size_t gkomi_reduction_workspace_bytes(int64_t n, int64_t nrhs);
+
+/* Dense::compute_conj_dot / compute_norm2 (core/matrix/dense.cpp:400-466;
+ * reference/matrix/dense_kernels.cpp:299-352): result[j] per column, on the
+ * device.  Single-pass: per-workgroup partials plus a last-workgroup combine
+ * in fixed order.  The workspace must be zero-filled before its first use;
+ * every call leaves it zero-filled again. */
+int gkomi_dense_dot_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const double* x,
+                        int64_t x_stride, const double* y, int64_t y_stride,
+                        double* result, void* workspace, size_t workspace_bytes);
+int gkomi_dense_dot_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const float* x,
+                        int64_t x_stride, const float* y, int64_t y_stride, float* result,
+                        void* workspace, size_t workspace_bytes);
+int gkomi_dense_norm2_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const double* x,
+                          int64_t x_stride, double* result, void* workspace,
+                          size_t workspace_bytes);
+int gkomi_dense_norm2_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const float* x,
+                          int64_t x_stride, float* result, void* workspace,
+                          size_t workspace_bytes);
+/* Dense::add_scaled (common/unified/matrix/dense_kernels.template.cpp:225-260):
+ * y += alpha[0 or j] * x; alpha has 1 or nrhs entries (alpha_len). */
+int gkomi_dense_add_scaled_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs,
+                               const double* alpha, int64_t alpha_len, const double* x,
+                               int64_t x_stride, double* y, int64_t y_stride);
+int gkomi_dense_add_scaled_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs,
+                               const float* alpha, int64_t alpha_len, const float* x,
+                               int64_t x_stride, float* y, int64_t y_stride);
+/* Dense::scale: y *= alpha[0 or j]. */
+int gkomi_dense_scale_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const double* alpha,
+                          int64_t alpha_len, double* y, int64_t y_stride);
+int gkomi_dense_scale_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const float* alpha,
+                          int64_t alpha_len, float* y, int64_t y_stride);
+/* Dense::copy_from with strides (dense_kernels.template.cpp: copy). */
+int gkomi_dense_copy_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const double* x,
+                         int64_t x_stride, double* y, int64_t y_stride);
+int gkomi_dense_copy_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const float* x,
+                         int64_t x_stride, float* y, int64_t y_stride);
+
+/* ------------------------------------------------------------------ CSR */
+
+/* csr::strategy_type (include/ginkgo/core/matrix/csr.hpp:176-699).
+ * CLASSICAL: a sub-wavefront of `subwave` lanes per row (0 = choose from the
+ * mean row length).  LOAD_BALANCE: nnz-balanced workgroups over the
+ * partition `srow` built by gkomi_csr_lb_partition. */
+enum { GKOMI_CSR_CLASSICAL = 0, GKOMI_CSR_LOAD_BALANCE = 1 };
+
+/* Matrix::Csr<V, int32> storage (csr.hpp:104-1690) plus its strategy data.
+ * `values` has the value type of the entry point it is passed to. */
+typedef struct {
+    int64_t n_rows;
+    int64_t n_cols;
+    int64_t nnz;
+    const int32_t* row_ptrs;
+    const int32_t* col_idxs;
+    const void* values;
+    int32_t strategy;
+    int32_t subwave;
+    const int32_t* srow;     /* LOAD_BALANCE: srow_blocks + 1 row starts */
+    int64_t srow_blocks;
+} gkomi_csr;
+
+/* nnz handled by one workgroup of the LOAD_BALANCE kernel for value type V. */
+int64_t gkomi_csr_lb_chunk(int32_t value_bytes);
+/* Partition for LOAD_BALANCE (the role of load_balance::process / srow,
+ * csr.hpp:417-468): srow[k] = first row whose start is >= k * chunk,
+ * srow[num_blocks] = n_rows, num_blocks = max(1, ceil(nnz / chunk)).
+ * Integer work, deterministic. */
+int gkomi_csr_lb_partition(const gkomi_ctx* ctx, int64_t n_rows, const int32_t* row_ptrs,
+                           int64_t nnz, int64_t chunk, int32_t* srow, int64_t num_blocks);
+/* Longest row (csr::compute_max_row_nnz, common/unified/matrix/csr_kernels.cpp);
+ * result is written to device memory. */
+int gkomi_csr_max_row_nnz(const gkomi_ctx* ctx, int64_t n_rows, const int32_t* row_ptrs,
+                          int64_t* result);
+
+/* csr::spmv (core/matrix/csr_kernels.hpp:29-36; reference/matrix/csr_kernels.cpp:47-78):
+ * c = A b. */
+int gkomi_csr_spmv_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t nrhs,
+                       const double* b, int64_t b_stride, double* c, int64_t c_stride);
+int gkomi_csr_spmv_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t nrhs,
+                       const float* b, int64_t b_stride, float* c, int64_t c_stride);
+/* csr::advanced_spmv (csr_kernels.hpp:38-43; reference/matrix/csr_kernels.cpp:84-121):
+ * c = alpha A b + beta c; beta == 0 ignores c (NaN included). */
+int gkomi_csr_advanced_spmv_f64(const gkomi_ctx* ctx, const double* alpha, const gkomi_csr* a,
+                                int64_t nrhs, const double* b, int64_t b_stride,
+                                const double* beta, double* c, int64_t c_stride);
+int gkomi_csr_advanced_spmv_f32(const gkomi_ctx* ctx, const float* alpha, const gkomi_csr* a,
+                                int64_t nrhs, const float* b, int64_t b_stride,
+                                const float* beta, float* c, int64_t c_stride);
+
+/* csr::transpose (reference/matrix/csr_kernels.cpp:676-731): the CSR of A^T
+ * with each row sorted by column, entries in the reference's stable order.
+ * Bit-exact.  t_row_ptrs has n_cols + 1 entries. */
+size_t gkomi_csr_transpose_workspace_bytes(int64_t n_rows, int64_t n_cols, int64_t nnz);
+int gkomi_csr_transpose_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int32_t* t_row_ptrs,
+                            int32_t* t_col_idxs, double* t_values, void* workspace,
+                            size_t workspace_bytes);
+int gkomi_csr_transpose_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int32_t* t_row_ptrs,
+                            int32_t* t_col_idxs, float* t_values, void* workspace,
+                            size_t workspace_bytes);
+
+/* --------------------------------------------------------- SELL-P, ELL */
+
+/* matrix::Sellp (include/ginkgo/core/matrix/sellp.hpp:17-18, 379-383): entry k of
+ * row r lives at (slice_sets[r / slice_size] + k) * slice_size + r % slice_size. */
+typedef struct {
+    int64_t n_rows;
+    int64_t n_cols;
+    int64_t slice_size;
+    int64_t stride_factor;
+    const int64_t* slice_sets;    /* num_slices + 1 */
+    const int64_t* slice_lengths; /* num_slices */
+    const int32_t* col_idxs;      /* padding = -1 (invalid_index) */
+    const void* values;           /* padding = 0 */
+} gkomi_sellp;
+
+/* matrix::Ell (include/ginkgo/core/matrix/ell.hpp:52, 385-388): entry k of row r
+ * at r + k * stride. */
+typedef struct {
+    int64_t n_rows;
+    int64_t n_cols;
+    int64_t width;
+    int64_t stride;
+    const int32_t* col_idxs;
+    const void* values;
+} gkomi_ell;
+
+/* sellp::compute_slice_sets (reference/matrix/sellp_kernels.cpp:109-132):
+ * slice_lengths[s] = max over the slice of ceil(len / stride_factor) * stride_factor,
+ * slice_sets = exclusive prefix sum with num_slices + 1 entries.  Bit-exact. */
+int gkomi_sellp_compute_slice_sets(const gkomi_ctx* ctx, int64_t n_rows,
+                                   const int32_t* row_ptrs, int64_t slice_size,
+                                   int64_t stride_factor, int64_t* slice_sets,
+                                   int64_t* slice_lengths, void* workspace,
+                                   size_t workspace_bytes);
+size_t gkomi_scan_workspace_bytes(int64_t n);
+/* csr::convert_to_sellp (reference/matrix/csr_kernels.cpp:528-569).  Storage past
+ * the last row of the last slice is written as padding too. */
+int gkomi_csr_to_sellp_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t slice_size,
+                           const int64_t* slice_sets, const int64_t* slice_lengths,
+                           int32_t* s_cols, double* s_vals);
+int gkomi_csr_to_sellp_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t slice_size,
+                           const int64_t* slice_sets, const int64_t* slice_lengths,
+                           int32_t* s_cols, float* s_vals);
+/* sellp::spmv / advanced_spmv (reference/matrix/sellp_kernels.cpp:26-103). */
+int gkomi_sellp_spmv_f64(const gkomi_ctx* ctx, const gkomi_sellp* a, int64_t nrhs,
+                         const double* b, int64_t b_stride, double* c, int64_t c_stride);
+int gkomi_sellp_spmv_f32(const gkomi_ctx* ctx, const gkomi_sellp* a, int64_t nrhs,
+                         const float* b, int64_t b_stride, float* c, int64_t c_stride);
+int gkomi_sellp_advanced_spmv_f64(const gkomi_ctx* ctx, const double* alpha,
+                                  const gkomi_sellp* a, int64_t nrhs, const double* b,
+                                  int64_t b_stride, const double* beta, double* c,
+                                  int64_t c_stride);
+int gkomi_sellp_advanced_spmv_f32(const gkomi_ctx* ctx, const float* alpha,
+                                  const gkomi_sellp* a, int64_t nrhs, const float* b,
+                                  int64_t b_stride, const float* beta, float* c,
+                                  int64_t c_stride);
+/* csr::convert_to_ell (reference/matrix/csr_kernels.cpp:575-600). */
+int gkomi_csr_to_ell_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t width,
+                         int64_t stride, int32_t* e_cols, double* e_vals);
+int gkomi_csr_to_ell_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t width,
+                         int64_t stride, int32_t* e_cols, float* e_vals);
+/* ell::spmv / advanced_spmv (reference/matrix/ell_kernels.cpp:29-110). */
+int gkomi_ell_spmv_f64(const gkomi_ctx* ctx, const gkomi_ell* a, int64_t nrhs, const double* b,
+                       int64_t b_stride, double* c, int64_t c_stride);
+int gkomi_ell_spmv_f32(const gkomi_ctx* ctx, const gkomi_ell* a, int64_t nrhs, const float* b,
+                       int64_t b_stride, float* c, int64_t c_stride);
+int gkomi_ell_advanced_spmv_f64(const gkomi_ctx* ctx, const double* alpha, const gkomi_ell* a,
+                                int64_t nrhs, const double* b, int64_t b_stride,
+                                const double* beta, double* c, int64_t c_stride);
+int gkomi_ell_advanced_spmv_f32(const gkomi_ctx* ctx, const float* alpha, const gkomi_ell* a,
+                                int64_t nrhs, const float* b, int64_t b_stride,
+                                const float* beta, float* c, int64_t c_stride);
+
+/* ------------------------------------------------------- scalar Jacobi */
+
+/* Scalar Jacobi generate (core/preconditioner/jacobi.cpp:338-370): extract the
+ * diagonal (reference/matrix/csr_kernels.cpp:1318-1336, 0 when absent) and
+ * invert it (reference/preconditioner/jacobi_kernels.cpp:582-591, 0 -> 1). */
+int gkomi_jacobi_generate_f64(const gkomi_ctx* ctx, const gkomi_csr* a, double* inv_diag);
+int gkomi_jacobi_generate_f32(const gkomi_ctx* ctx, const gkomi_csr* a, float* inv_diag);
+/* jacobi::simple_scalar_apply (jacobi_kernels.cpp:554-566): x = inv_diag .* b. */
+int gkomi_jacobi_apply_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const double* inv_diag,
+                           const double* b, int64_t b_stride, double* x, int64_t x_stride);
+int gkomi_jacobi_apply_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const float* inv_diag,
+                           const float* b, int64_t b_stride, float* x, int64_t x_stride);
+/* jacobi::scalar_apply (jacobi_kernels.cpp:534-550): x = beta x + alpha inv_diag .* b. */
+int gkomi_jacobi_advanced_apply_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs,
+                                    const double* inv_diag, const double* alpha,
+                                    const double* b, int64_t b_stride, const double* beta,
+                                    double* x, int64_t x_stride);
+int gkomi_jacobi_advanced_apply_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs,
+                                    const float* inv_diag, const float* alpha, const float* b,
+                                    int64_t b_stride, const float* beta, float* x,
+                                    int64_t x_stride);
+
+/* ------------------------------------------------ stopping criteria */
+
+/* stopping_status bits (include/ginkgo/core/stop/stopping_status.hpp:115-118). */
+#define GKOMI_STATUS_CONVERGED 0x80u
+#define GKOMI_STATUS_FINALIZED 0x40u
+#define GKOMI_STATUS_ID_MASK 0x3fu
+
+enum {
+    GKOMI_CRIT_ITERATION = 1,       /* stop::Iteration (core/stop/iteration.cpp:14-24) */
+    GKOMI_CRIT_RESIDUAL_NORM = 2,   /* stop::ResidualNorm (core/stop/residual_norm.cpp:165-207) */
+    GKOMI_CRIT_IMPLICIT_RESNORM = 3 /* stop::ImplicitResidualNorm (residual_norm.cpp:210-226) */
+};
+/* stop::mode (include/ginkgo/core/stop/residual_norm.hpp:37). */
+enum { GKOMI_BASELINE_RHS_NORM = 0, GKOMI_BASELINE_INITIAL_RESNORM = 1, GKOMI_BASELINE_ABSOLUTE = 2 };
+
+typedef struct {
+    int32_t kind;
+    int32_t baseline;
+    int64_t max_iters;
+    double reduction_factor;
+} gkomi_criterion;
+
+/* stop::Combined (core/stop/combined.cpp:40-60): criteria in order, the k-th has
+ * stopping id k + 1. */
+#define GKOMI_MAX_CRITERIA 4
+typedef struct {
+    int32_t num;
+    int32_t reserved;
+    gkomi_criterion c[GKOMI_MAX_CRITERIA];
+} gkomi_criteria;
+
+/* Device-resident loop state of a Krylov solve.  `iteration` is the
+ * reference's `iter` (core/solver/cg.cpp:145-158); all_stopped / one_changed
+ * are the results of the last Combined::check. */
+typedef struct {
+    int64_t iteration;
+    int32_t all_stopped;
+    int32_t one_changed;
+} gkomi_stop_flags;
+
+/* residual_norm::residual_norm (reference/stop/residual_norm_kernels.cpp:26-56)
+ * generalised to the whole Combined list and evaluated on the device:
+ * `tau` = explicit residual norms (NULL if not available), `implicit_sq` =
+ * implicit squared norms (NULL if not available), orig_tau[k * nrhs + j] =
+ * baseline of criterion k.  Reads flags->iteration, updates status and flags. */
+int gkomi_criteria_check_f64(const gkomi_ctx* ctx, const gkomi_criteria* crit, int64_t nrhs,
+                             const double* tau, const double* implicit_sq,
+                             const double* orig_tau, int32_t set_finalized,
+                             uint8_t* status, gkomi_stop_flags* flags);
+int gkomi_criteria_check_f32(const gkomi_ctx* ctx, const gkomi_criteria* crit, int64_t nrhs,
+                             const float* tau, const float* implicit_sq,
+                             const float* orig_tau, int32_t set_finalized, uint8_t* status,
+                             gkomi_stop_flags* flags);
+
+/* -------------------------------------------- CG (reference semantics) */
+
+/* cg::initialize / step_1 / step_2 (core/solver/cg_kernels.hpp:20-68;
+ * reference/solver/cg_kernels.cpp:23-101).  All vectors share stride vs except
+ * x (x_stride) and b (b_stride). */
+int gkomi_cg_initialize_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const double* b,
+                            int64_t b_stride, double* r, double* z, double* p, double* q,
+                            int64_t vs, double* prev_rho, double* rho, uint8_t* status);
+int gkomi_cg_initialize_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const float* b,
+                            int64_t b_stride, float* r, float* z, float* p, float* q,
+                            int64_t vs, float* prev_rho, float* rho, uint8_t* status);
+int gkomi_cg_step_1_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, double* p,
+                        const double* z, int64_t vs, const double* rho,
+                        const double* prev_rho, const uint8_t* status);
+int gkomi_cg_step_1_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, float* p,
+                        const float* z, int64_t vs, const float* rho, const float* prev_rho,
+                        const uint8_t* status);
+int gkomi_cg_step_2_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, double* x,
+                        int64_t x_stride, double* r, const double* p, const double* q,
+                        int64_t vs, const double* beta, const double* rho,
+                        const uint8_t* status);
+int gkomi_cg_step_2_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, float* x,
+                        int64_t x_stride, float* r, const float* p, const float* q,
+                        int64_t vs, const float* beta, const float* rho,
+                        const uint8_t* status);
+
+/* ------------------------------------------------ CG (fused, gfx950) */
+
+/* Device state of a fused CG solve with one right-hand side.  All pointers
+ * are device memory of the entry point's value type (V); `work` holds the
+ * reduction partials and must be zero-filled before the first call
+ * (gkomi_cg_work_bytes bytes).  p_a / p_b are used alternately. */
+typedef struct {
+    int64_t n;
+    void* x;        /* solution (in: initial guess) */
+    const void* b;  /* right-hand side */
+    void* r;
+    void* z;
+    void* p_a;
+    void* p_b;
+    void* q;
+    const void* inv_diag; /* scalar Jacobi, or NULL for the identity */
+    void* scalars;  /* 8 values: rho, prev_rho, beta, tau, rho_new ... (library-owned layout) */
+    void* orig_tau; /* GKOMI_MAX_CRITERIA values */
+    uint8_t* status;
+    gkomi_stop_flags* flags;
+    void* work;
+} gkomi_cg_state;
+
+size_t gkomi_cg_work_bytes(int64_t n);
+/* Start of Cg::apply_dense_impl (core/solver/cg.cpp:93-158): initialize,
+ * r = b - A x, the criteria baselines, z = M r, rho = r.z, and the check of
+ * iteration 0 -- all on the device. */
+int gkomi_cg_fused_start_f64(const gkomi_ctx* ctx, const gkomi_csr* a,
+                             const gkomi_criteria* crit, gkomi_cg_state* st);
+int gkomi_cg_fused_start_f32(const gkomi_ctx* ctx, const gkomi_csr* a,
+                             const gkomi_criteria* crit, gkomi_cg_state* st);
+/* `count` CG iterations (core/solver/cg.cpp:160-192 then :145-158): per
+ * iteration two kernels -- (1) p = z + rho/prev_rho p fused into q = A p and
+ * beta = p.q, (2) x += rho/beta p, r -= rho/beta q, z = M r, rho = r.z,
+ * ||r||, and the Combined check.  Once flags->all_stopped is set the
+ * remaining kernels return immediately, so the host may enqueue ahead. */
+int gkomi_cg_fused_iterate_f64(const gkomi_ctx* ctx, const gkomi_csr* a,
+                               const gkomi_criteria* crit, gkomi_cg_state* st, int32_t count);
+int gkomi_cg_fused_iterate_f32(const gkomi_ctx* ctx, const gkomi_csr* a,
+                               const gkomi_criteria* crit, gkomi_cg_state* st, int32_t count);
+/* Pointer to the p buffer that holds the current search direction. */
+void* gkomi_cg_current_p(const gkomi_cg_state* st);
+
+/* ------------------------------------------- BiCGSTAB (reference semantics) */
+
+/* bicgstab::initialize / step_1 / step_2 / step_3 / finalize
+ * (core/solver/bicgstab_kernels.hpp:42-104; reference/solver/bicgstab_kernels.cpp:26-185). */
+int gkomi_bicgstab_initialize_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs,
+                                  const double* b, int64_t b_stride, double* r, double* rr,
+                                  double* y, double* s, double* t, double* z, double* v,
+                                  double* p, int64_t vs, double* prev_rho, double* rho,
+                                  double* alpha, double* beta, double* gamma, double* omega,
+                                  uint8_t* status);
+int gkomi_bicgstab_initialize_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const float* b,
+                                  int64_t b_stride, float* r, float* rr, float* y, float* s,
+                                  float* t, float* z, float* v, float* p, int64_t vs,
+                                  float* prev_rho, float* rho, float* alpha, float* beta,
+                                  float* gamma, float* omega, uint8_t* status);
+int gkomi_bicgstab_step_1_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const double* r,
+                              double* p, const double* v, int64_t vs, const double* rho,
+                              const double* prev_rho, const double* alpha,
+                              const double* omega, const uint8_t* status);
+int gkomi_bicgstab_step_1_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const float* r,
+                              float* p, const float* v, int64_t vs, const float* rho,
+                              const float* prev_rho, const float* alpha, const float* omega,
+                              const uint8_t* status);
+int gkomi_bicgstab_step_2_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const double* r,
+                              double* s, const double* v, int64_t vs, const double* rho,
+                              double* alpha, const double* beta, const uint8_t* status);
+int gkomi_bicgstab_step_2_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const float* r,
+                              float* s, const float* v, int64_t vs, const float* rho,
+                              float* alpha, const float* beta, const uint8_t* status);
+int gkomi_bicgstab_step_3_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, double* x,
+                              int64_t x_stride, double* r, const double* s, const double* t,
+                              const double* y, const double* z, int64_t vs,
+                              const double* alpha, const double* beta, const double* gamma,
+                              double* omega, const uint8_t* status);
+int gkomi_bicgstab_step_3_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, float* x,
+                              int64_t x_stride, float* r, const float* s, const float* t,
+                              const float* y, const float* z, int64_t vs, const float* alpha,
+                              const float* beta, const float* gamma, float* omega,
+                              const uint8_t* status);
+int gkomi_bicgstab_finalize_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, double* x,
+                                int64_t x_stride, const double* y, int64_t vs,
+                                const double* alpha, uint8_t* status);
+int gkomi_bicgstab_finalize_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, float* x,
+                                int64_t x_stride, const float* y, int64_t vs,
+                                const float* alpha, uint8_t* status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GKOMI_H_ */
