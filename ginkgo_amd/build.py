@@ -116,10 +116,53 @@ def build_host(force=False, jobs=None):
     return lib
 
 
+def build_python(force=False):
+    """pybind11 module ginkgo_amd/_gko*.so over libginkgo_amd.so."""
+    import sysconfig
+    import pybind11
+    src = os.path.join(CORE, "python", "bindings.cpp")
+    if not os.path.exists(src):
+        return None
+    suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    out = os.path.join(PKG, "_gko" + suffix)
+    hdrs = _headers(os.path.join(CORE, "include")) + _headers(INCLUDE)
+    host = os.path.join(LIBDIR, "libginkgo_amd.so")
+    if force or _newer([src, host] + hdrs, out):
+        tmp = out + ".tmp"
+        _run([CXX, "-O1", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
+              "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
+              "-I", INCLUDE, "-I", os.path.join(CORE, "include"), src, "-o", tmp,
+              "-L", LIBDIR, "-lginkgo_amd", "-lgkomi", "-Wl,-rpath,$ORIGIN/lib"])
+        os.replace(tmp, out)
+    return out
+
+
+def build_examples(force=False):
+    """C++ programs under examples/ linked against the host library."""
+    exdir = os.path.join(ROOT, "examples")
+    if not os.path.isdir(exdir):
+        return []
+    outs = []
+    hdrs = _headers(os.path.join(CORE, "include")) + _headers(INCLUDE)
+    for f in sorted(os.listdir(exdir)):
+        if not f.endswith(".cpp"):
+            continue
+        src = os.path.join(exdir, f)
+        out = os.path.join(LIBDIR, os.path.splitext(f)[0])
+        if force or _newer([src, os.path.join(LIBDIR, "libginkgo_amd.so")] + hdrs, out):
+            _run([CXX, "-O2", "-std=c++17", "-I", INCLUDE, "-I", os.path.join(CORE, "include"),
+                  src, "-o", out, "-L", LIBDIR, "-lginkgo_amd", "-lgkomi",
+                  "-Wl,-rpath,$ORIGIN"])
+        outs.append(out)
+    return outs
+
+
 def build_all(force=False):
     k = build_kernels(force)
     h = build_host(force)
-    return k, h
+    p = build_python(force)
+    e = build_examples(force)
+    return [k, h, p] + e
 
 
 if __name__ == "__main__":

@@ -1,0 +1,350 @@
+// pybind11 binding of the gko:: host library (module ginkgo_amd._gko).
+// Python reaches the solvers through the same C++ classes a C++ user
+// would: Cg<V>::build().with_criteria(...).on(exec)->generate(A)->apply(b, x).
+// Device memory handed in from Python (torch tensors) is wrapped in
+// non-owning views; the caller keeps it alive.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "ginkgo_amd/ginkgo.hpp"
+
+namespace py = pybind11;
+using namespace gko;
+
+namespace {
+
+using crit_param = deferred_factory_parameter<const stop::CriterionFactory>;
+using op_param = deferred_factory_parameter<const LinOpFactory>;
+
+crit_param to_crit(const py::handle& o)
+{
+    if (py::isinstance<stop::Iteration::parameters_type>(o))
+        return crit_param(o.cast<stop::Iteration::parameters_type>());
+    if (py::isinstance<stop::ResidualNorm<double>::parameters_type>(o))
+        return crit_param(o.cast<stop::ResidualNorm<double>::parameters_type>());
+    if (py::isinstance<stop::ResidualNorm<float>::parameters_type>(o))
+        return crit_param(o.cast<stop::ResidualNorm<float>::parameters_type>());
+    if (py::isinstance<stop::ImplicitResidualNorm<double>::parameters_type>(o))
+        return crit_param(o.cast<stop::ImplicitResidualNorm<double>::parameters_type>());
+    if (py::isinstance<stop::ImplicitResidualNorm<float>::parameters_type>(o))
+        return crit_param(o.cast<stop::ImplicitResidualNorm<float>::parameters_type>());
+    if (py::isinstance<stop::CriterionFactory>(o))
+        return crit_param(o.cast<std::shared_ptr<const stop::CriterionFactory>>());
+    throw py::type_error("not a stopping criterion");
+}
+
+op_param to_op(const py::handle& o)
+{
+    if (o.is_none()) return op_param();
+    if (py::isinstance<preconditioner::Jacobi<double>::parameters_type>(o))
+        return op_param(o.cast<preconditioner::Jacobi<double>::parameters_type>());
+    if (py::isinstance<preconditioner::Jacobi<float>::parameters_type>(o))
+        return op_param(o.cast<preconditioner::Jacobi<float>::parameters_type>());
+    if (py::isinstance<solver::Cg<double>::parameters_type>(o))
+        return op_param(o.cast<solver::Cg<double>::parameters_type>());
+    if (py::isinstance<solver::Cg<float>::parameters_type>(o))
+        return op_param(o.cast<solver::Cg<float>::parameters_type>());
+    if (py::isinstance<solver::Bicgstab<double>::parameters_type>(o))
+        return op_param(o.cast<solver::Bicgstab<double>::parameters_type>());
+    if (py::isinstance<solver::Bicgstab<float>::parameters_type>(o))
+        return op_param(o.cast<solver::Bicgstab<float>::parameters_type>());
+    if (py::isinstance<LinOpFactory>(o)) return op_param(o.cast<std::shared_ptr<const LinOpFactory>>());
+    throw py::type_error("not a LinOp factory or its parameters");
+}
+
+template <typename V>
+std::shared_ptr<typename matrix::Csr<V>::strategy_type> strategy(const std::string& s, int sw)
+{
+    using C = matrix::Csr<V>;
+    if (s == "classical") return std::make_shared<typename C::classical>(sw);
+    if (s == "load_balance") return std::make_shared<typename C::load_balance>();
+    if (s == "automatical") return std::make_shared<typename C::automatical>();
+    throw py::value_error("strategy must be classical, load_balance or automatical");
+}
+
+template <typename T>
+std::vector<T> vec(py::array_t<T, py::array::c_style | py::array::forcecast> a)
+{
+    return std::vector<T>(a.data(), a.data() + a.size());
+}
+
+template <typename T>
+py::array_t<T> arr(const std::vector<T>& v)
+{
+    py::array_t<T> out(v.size());
+    std::copy(v.begin(), v.end(), out.mutable_data());
+    return out;
+}
+
+template <typename V>
+void bind_value_type(py::module_& m, const std::string& sfx)
+{
+    using Dense = matrix::Dense<V>;
+    using Csr = matrix::Csr<V>;
+    using Sellp = matrix::Sellp<V>;
+    using Ell = matrix::Ell<V>;
+
+    py::class_<Dense, LinOp, std::shared_ptr<Dense>>(m, ("Dense" + sfx).c_str())
+        .def_static("create",
+                    [](std::shared_ptr<const Executor> e, size_type r, size_type c) {
+                        return std::shared_ptr<Dense>(Dense::create(e, dim<2>{r, c}));
+                    })
+        .def_static("create_view",
+                    [](std::shared_ptr<const Executor> e, size_type r, size_type c,
+                       std::uintptr_t ptr, size_type stride) {
+                        return std::shared_ptr<Dense>(Dense::create_view(
+                            e, dim<2>{r, c}, reinterpret_cast<V*>(ptr), stride));
+                    })
+        .def_static("from_numpy",
+                    [](std::shared_ptr<const Executor> e,
+                       py::array_t<V, py::array::c_style | py::array::forcecast> a) {
+                        const size_type r = a.ndim() > 0 ? a.shape(0) : 1;
+                        const size_type c = a.ndim() > 1 ? a.shape(1) : 1;
+                        return std::shared_ptr<Dense>(
+                            Dense::create_from_host(e, dim<2>{r, c}, vec<V>(a)));
+                    })
+        .def("to_numpy",
+             [](const Dense& d) {
+                 auto h = d.to_host();
+                 py::array_t<V> out({d.get_size().rows, d.get_size().cols});
+                 std::copy(h.begin(), h.end(), out.mutable_data());
+                 return out;
+             })
+        .def("fill", &Dense::fill)
+        .def("scale", &Dense::scale)
+        .def("add_scaled", &Dense::add_scaled)
+        .def("compute_dot", &Dense::compute_dot)
+        .def("compute_norm2", &Dense::compute_norm2)
+        .def("copy_from", [](Dense& d, const Dense& o) { d.copy_from(&o); })
+        .def("copy_from", [](Dense& d, const matrix::Dense<std::conditional_t<
+                                          std::is_same<V, double>::value, float, double>>& o) {
+            d.copy_from(&o);
+        })
+        .def_property_readonly("stride", &Dense::get_stride)
+        .def_property_readonly("data_ptr", [](Dense& d) {
+            return reinterpret_cast<std::uintptr_t>(d.get_values());
+        });
+
+    py::class_<Csr, LinOp, std::shared_ptr<Csr>>(m, ("Csr" + sfx).c_str())
+        .def_static(
+            "create_from_host",
+            [](std::shared_ptr<const Executor> e, size_type r, size_type c,
+               py::array_t<int32> rp, py::array_t<int32> ci, py::array_t<V> v,
+               const std::string& s, int sw) {
+                return std::shared_ptr<Csr>(Csr::create_from_host(
+                    e, dim<2>{r, c}, vec<int32>(rp), vec<int32>(ci), vec<V>(v), strategy<V>(s, sw)));
+            },
+            py::arg("exec"), py::arg("rows"), py::arg("cols"), py::arg("row_ptrs"),
+            py::arg("col_idxs"), py::arg("values"), py::arg("strategy") = "automatical",
+            py::arg("subwave") = 0)
+        .def_static(
+            "create_view",
+            [](std::shared_ptr<const Executor> e, size_type r, size_type c, size_type nnz,
+               std::uintptr_t v, std::uintptr_t ci, std::uintptr_t rp, const std::string& s,
+               int sw) {
+                return std::shared_ptr<Csr>(Csr::create_view(
+                    e, dim<2>{r, c}, nnz, reinterpret_cast<V*>(v), reinterpret_cast<int32*>(ci),
+                    reinterpret_cast<int32*>(rp), strategy<V>(s, sw)));
+            },
+            py::arg("exec"), py::arg("rows"), py::arg("cols"), py::arg("nnz"), py::arg("values"),
+            py::arg("col_idxs"), py::arg("row_ptrs"), py::arg("strategy") = "automatical",
+            py::arg("subwave") = 0)
+        .def("transpose", [](const Csr& a) { return std::shared_ptr<Csr>(a.transpose()); })
+        .def("to_sellp",
+             [](const Csr& a, size_type ss, size_type sf) {
+                 return std::shared_ptr<Sellp>(a.to_sellp(ss, sf));
+             },
+             py::arg("slice_size") = 64, py::arg("stride_factor") = 1)
+        .def("to_ell", [](const Csr& a) { return std::shared_ptr<Ell>(a.to_ell()); })
+        .def("to_f32", [](const Csr& a) {
+            return std::shared_ptr<matrix::Csr<float>>(a.template convert_precision<float>());
+        })
+        .def("to_f64", [](const Csr& a) {
+            return std::shared_ptr<matrix::Csr<double>>(a.template convert_precision<double>());
+        })
+        .def("max_row_nnz", &Csr::max_row_nnz)
+        .def_property_readonly("effective_strategy", &Csr::get_effective_strategy)
+        .def_property_readonly("nnz", &Csr::get_num_stored_elements)
+        .def("row_ptrs", [](const Csr& a) { return arr(a.row_ptrs_to_host()); })
+        .def("col_idxs", [](const Csr& a) { return arr(a.col_idxs_to_host()); })
+        .def("values", [](const Csr& a) { return arr(a.values_to_host()); });
+
+    py::class_<Sellp, LinOp, std::shared_ptr<Sellp>>(m, ("Sellp" + sfx).c_str())
+        .def_property_readonly("slice_size", &Sellp::get_slice_size)
+        .def_property_readonly("total_cols", &Sellp::get_total_cols)
+        .def("slice_sets", [](const Sellp& a) { return arr(a.slice_sets_to_host()); })
+        .def("col_idxs", [](const Sellp& a) { return arr(a.col_idxs_to_host()); })
+        .def("values", [](const Sellp& a) { return arr(a.values_to_host()); });
+
+    py::class_<Ell, LinOp, std::shared_ptr<Ell>>(m, ("Ell" + sfx).c_str())
+        .def_property_readonly("width", &Ell::get_num_stored_elements_per_row)
+        .def_property_readonly("stride", &Ell::get_stride)
+        .def("col_idxs", [](const Ell& a) { return arr(a.col_idxs_to_host()); })
+        .def("values", [](const Ell& a) { return arr(a.values_to_host()); });
+
+    // criteria parameters
+    using RN = stop::ResidualNorm<V>;
+    py::class_<typename RN::parameters_type>(m, ("ResidualNorm" + sfx).c_str())
+        .def(py::init<>())
+        .def("with_reduction_factor",
+             [](typename RN::parameters_type p, double r) { return p.with_reduction_factor(r); })
+        .def("with_baseline",
+             [](typename RN::parameters_type p, stop::mode b) { return p.with_baseline(b); })
+        .def_readonly("reduction_factor", &RN::parameters_type::reduction_factor);
+    using IRN = stop::ImplicitResidualNorm<V>;
+    py::class_<typename IRN::parameters_type>(m, ("ImplicitResidualNorm" + sfx).c_str())
+        .def(py::init<>())
+        .def("with_reduction_factor",
+             [](typename IRN::parameters_type p, double r) { return p.with_reduction_factor(r); })
+        .def("with_baseline",
+             [](typename IRN::parameters_type p, stop::mode b) { return p.with_baseline(b); });
+
+    // Jacobi
+    using J = preconditioner::Jacobi<V>;
+    py::class_<typename J::parameters_type>(m, ("JacobiParams" + sfx).c_str())
+        .def(py::init<>())
+        .def("with_max_block_size",
+             [](typename J::parameters_type p, uint32_t b) { return p.with_max_block_size(b); })
+        .def("on", [](const typename J::parameters_type& p, std::shared_ptr<const Executor> e) {
+            return std::shared_ptr<LinOpFactory>(p.on(e));
+        });
+    py::class_<J, LinOp, std::shared_ptr<J>>(m, ("Jacobi" + sfx).c_str())
+        .def("inverse_diagonal", [](const J& j) { return arr(j.inverse_diagonal_to_host()); });
+
+    // Krylov solvers
+    auto bind_krylov = [&](auto tag, const std::string& name) {
+        using S = typename decltype(tag)::type;
+        using P = typename S::parameters_type;
+        py::class_<P>(m, (name + "Params" + sfx).c_str())
+            .def(py::init<>())
+            .def("with_criteria",
+                 [](P p, py::args a) {
+                     std::vector<crit_param> l;
+                     for (auto o : a) l.push_back(to_crit(o));
+                     p.with_criteria_list(l);
+                     return p;
+                 })
+            .def("with_preconditioner",
+                 [](P p, py::object o) { return p.with_preconditioner(to_op(o)); })
+            .def("with_generated_preconditioner",
+                 [](P p, std::shared_ptr<const LinOp> o) {
+                     return p.with_generated_preconditioner(o);
+                 })
+            .def("on", [](const P& p, std::shared_ptr<const Executor> e) {
+                return std::shared_ptr<LinOpFactory>(p.on(e));
+            });
+        py::class_<S, LinOp, std::shared_ptr<S>>(m, (name + sfx).c_str())
+            .def("set_poll_chunk", &S::set_poll_chunk)
+            .def_property_readonly("last_iterations",
+                                   [](const S& s) { return s.get_last_stats().iterations; })
+            .def_property_readonly("last_used_fused_path",
+                                   [](const S& s) { return s.get_last_stats().used_fused_path; })
+            .def_property_readonly("last_status", [](const S& s) {
+                std::vector<int> out;
+                for (auto st : s.get_last_stats().status) out.push_back(st.raw());
+                return out;
+            });
+    };
+    struct cg_tag { using type = solver::Cg<V>; };
+    struct bicg_tag { using type = solver::Bicgstab<V>; };
+    bind_krylov(cg_tag{}, "Cg");
+    bind_krylov(bicg_tag{}, "Bicgstab");
+
+    using I = solver::Ir<V>;
+    using IP = typename I::parameters_type;
+    py::class_<IP>(m, ("IrParams" + sfx).c_str())
+        .def(py::init<>())
+        .def("with_criteria",
+             [](IP p, py::args a) {
+                 p.criteria.list.clear();
+                 for (auto o : a) p.criteria.list.push_back(to_crit(o));
+                 return p;
+             })
+        .def("with_solver", [](IP p, py::object o) { return p.with_solver(to_op(o)); })
+        .def("with_relaxation_factor", [](IP p, double w) { return p.with_relaxation_factor(w); })
+        .def("on", [](const IP& p, std::shared_ptr<const Executor> e) {
+            return std::shared_ptr<LinOpFactory>(p.on(e));
+        });
+    py::class_<I, LinOp, std::shared_ptr<I>>(m, ("Ir" + sfx).c_str())
+        .def_property_readonly("last_iterations",
+                               [](const I& s) { return s.get_last_stats().iterations; })
+        .def_property_readonly("last_status", [](const I& s) {
+            std::vector<int> out;
+            for (auto st : s.get_last_stats().status) out.push_back(st.raw());
+            return out;
+        });
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_gko, m)
+{
+    m.doc() = "gko:: host library (MI355X executor) -- pybind11 binding";
+
+    // translators are tried last-registered first: the base class goes first
+    auto base = py::register_exception<Error>(m, "GkoError");
+    py::register_exception<NotSupported>(m, "NotSupported", base.ptr());
+    py::register_exception<DimensionMismatch>(m, "DimensionMismatch", base.ptr());
+    py::register_exception<NotImplemented>(m, "NotImplemented", base.ptr());
+    py::register_exception<AllocationError>(m, "AllocationError", base.ptr());
+    py::register_exception<HipError>(m, "HipError", base.ptr());
+
+    py::class_<Executor, std::shared_ptr<Executor>>(m, "Executor")
+        .def("synchronize", &Executor::synchronize)
+        .def_property_readonly("is_host", &Executor::is_host);
+    py::class_<HostExecutor, Executor, std::shared_ptr<HostExecutor>>(m, "HostExecutor")
+        .def_static("create", &HostExecutor::create);
+    py::class_<Mi355xExecutor, Executor, std::shared_ptr<Mi355xExecutor>>(m, "Mi355xExecutor")
+        .def_static(
+            "create",
+            [](int dev, std::uintptr_t stream, bool caching) {
+                return Mi355xExecutor::create(dev, reinterpret_cast<void*>(stream), caching);
+            },
+            py::arg("device") = 0, py::arg("stream") = 0, py::arg("caching") = true)
+        .def_static("get_num_devices", &Mi355xExecutor::get_num_devices)
+        .def_property_readonly("device_id", &Mi355xExecutor::get_device_id)
+        .def_property_readonly("num_multiprocessor", &Mi355xExecutor::get_num_multiprocessor)
+        .def_property_readonly("stream", [](const Mi355xExecutor& e) {
+            return reinterpret_cast<std::uintptr_t>(e.get_stream());
+        })
+        .def_property_readonly("pool_bytes", &Mi355xExecutor::get_pool_bytes)
+        .def("release_pool", &Mi355xExecutor::release_pool);
+
+    py::class_<LinOp, std::shared_ptr<LinOp>>(m, "LinOp")
+        .def("apply", [](const LinOp& op, const LinOp* b, LinOp* x) { op.apply(b, x); })
+        .def("apply", [](const LinOp& op, const LinOp* a, const LinOp* b, const LinOp* beta,
+                         LinOp* x) { op.apply(a, b, beta, x); })
+        .def_property_readonly("size",
+                               [](const LinOp& op) {
+                                   return py::make_tuple(op.get_size().rows, op.get_size().cols);
+                               })
+        .def("add_logger", [](const LinOp& op, std::shared_ptr<const log::Logger> l) {
+            op.add_logger(l);
+        });
+    py::class_<LinOpFactory, std::shared_ptr<LinOpFactory>>(m, "LinOpFactory")
+        .def("generate", [](const LinOpFactory& f, std::shared_ptr<const LinOp> a) {
+            return std::shared_ptr<LinOp>(f.generate(a));
+        });
+
+    py::enum_<stop::mode>(m, "mode")
+        .value("absolute", stop::mode::absolute)
+        .value("initial_resnorm", stop::mode::initial_resnorm)
+        .value("rhs_norm", stop::mode::rhs_norm);
+    py::class_<stop::CriterionFactory, std::shared_ptr<stop::CriterionFactory>>(m,
+                                                                                "CriterionFactory");
+    py::class_<stop::Iteration::parameters_type>(m, "Iteration")
+        .def(py::init<>())
+        .def("with_max_iters",
+             [](stop::Iteration::parameters_type p, size_type n) { return p.with_max_iters(n); });
+
+    py::class_<log::Logger, std::shared_ptr<log::Logger>>(m, "Logger");
+    py::class_<log::Convergence, log::Logger, std::shared_ptr<log::Convergence>>(m, "Convergence")
+        .def_static("create", &log::Convergence::create)
+        .def_property_readonly("num_iterations", &log::Convergence::get_num_iterations)
+        .def_property_readonly("has_converged", &log::Convergence::has_converged)
+        .def_property_readonly("residual_norm", &log::Convergence::get_residual_norm);
+
+    bind_value_type<double>(m, "_f64");
+    bind_value_type<float>(m, "_f32");
+}

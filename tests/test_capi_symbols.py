@@ -36,3 +36,22 @@ def test_kernel_library_reports_errors_without_gpu():
     # a null descriptor is rejected with a typed error, before any device call
     with pytest.raises(_capi.InvalidArgument):
         _capi.call("gkomi_csr_spmv_f64", None, None, 1, None, 1, None, 1)
+
+
+def test_host_library_has_no_cpu_compute_path():
+    """The gko:: host library loads without a GPU; LinOps on the host executor
+    hold memory only and refuse to compute (no CPU fallback)."""
+    import numpy as np
+    import ginkgo_amd.gko as gko
+    host = gko.HostExecutor.create()
+    A = gko.csr_from_host(host, np.array([0, 1, 2], np.int32), np.array([0, 1], np.int32),
+                          np.array([2.0, 3.0]))
+    b = gko.matrix.Dense("f64").from_numpy(host, np.ones((2, 1)))
+    x = gko.matrix.Dense("f64").create(host, 2, 1)
+    with pytest.raises(gko.NotSupported):
+        A.apply(b, x)
+    # factories resolve their parameters without touching a device
+    params = (gko.solver.Cg.build("f64")
+              .with_criteria(gko.stop.Iteration.build().with_max_iters(5),
+                             gko.stop.ResidualNorm.build("f64").with_reduction_factor(1e-8)))
+    assert params is not None
