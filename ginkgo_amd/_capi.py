@@ -125,6 +125,34 @@ class CgState(ctypes.Structure):
                 ("flags", ctypes.c_void_p), ("work", ctypes.c_void_p)]
 
 
+BATCH_MAX_BINS = 8
+
+
+class BatchCsr(ctypes.Structure):
+    _fields_ = [("num_systems", ctypes.c_int64), ("max_rows", ctypes.c_int64),
+                ("max_nnz", ctypes.c_int64), ("row_offsets", ctypes.c_void_p),
+                ("row_ptrs", ctypes.c_void_p), ("col_idxs", ctypes.c_void_p),
+                ("values", ctypes.c_void_p), ("uniform_rows", ctypes.c_int64),
+                ("uniform_nnz", ctypes.c_int64)]
+
+
+class BatchSettings(ctypes.Structure):
+    _fields_ = [("max_iterations", ctypes.c_int32), ("tolerance_type", ctypes.c_int32),
+                ("tolerance", ctypes.c_double), ("preconditioner", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class BatchPlan(ctypes.Structure):
+    _fields_ = [("num_bins", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("order", ctypes.c_void_p),
+                ("bin_begin", ctypes.c_int64 * (BATCH_MAX_BINS + 1)),
+                ("bin_max_rows", ctypes.c_int64 * BATCH_MAX_BINS),
+                ("bin_max_nnz", ctypes.c_int64 * BATCH_MAX_BINS),
+                ("block_size", ctypes.c_int32 * BATCH_MAX_BINS),
+                ("matrix_in_lds", ctypes.c_int32 * BATCH_MAX_BINS),
+                ("blocks_per_cu", ctypes.c_int32 * BATCH_MAX_BINS)]
+
+
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _I32 = ctypes.c_int32
@@ -168,6 +196,8 @@ _SIGS = {
                                                       _SZ]),
     "gkomi_cg_work_bytes": (_SZ, [_I64]),
     "gkomi_cg_current_p": (_P, [ctypes.POINTER(CgState)]),
+    "gkomi_batch_workspace_bytes": (_SZ, [ctypes.POINTER(BatchCsr)]),
+    "gkomi_batch_lds_bytes": (_SZ, [_I32, _I64, _I64, _I32]),
 }
 
 for _s in ("f64", "f32"):
@@ -224,6 +254,14 @@ for _s in ("f64", "f32"):
                                                        _P, _P, _I64, _P, _P, _P, _P, _P]),
         f"gkomi_bicgstab_finalize_{_s}": (ctypes.c_int, [_CTX, _I64, _I64, _P, _I64, _P, _I64,
                                                          _P, _P]),
+        f"gkomi_batch_bicgstab_solve_{_s}": (ctypes.c_int, [_CTX, ctypes.POINTER(BatchCsr),
+                                                           ctypes.POINTER(BatchSettings),
+                                                           ctypes.POINTER(BatchPlan), _P, _P, _P,
+                                                           _P, _P, _SZ]),
+        f"gkomi_batch_bicgstab_tune_{_s}": (ctypes.c_int, [_CTX, ctypes.POINTER(BatchCsr),
+                                                          ctypes.POINTER(BatchSettings),
+                                                          ctypes.POINTER(BatchPlan), _P, _P, _P,
+                                                          _P, _P, _P, _SZ]),
     })
 
 _lib = None

@@ -1,0 +1,568 @@
+// Batched BiCGSTAB for gfx950: thousands of small independent systems.
+//
+// One workgroup solves one system at a time and pulls the next system from
+// a work queue (an atomic ticket over a size-sorted order, largest first),
+// so uneven sizes balance themselves.  The system's matrix (values, 16-bit
+// local column indices, row pointers) and all nine work vectors live in LDS
+// when they fit; otherwise the matrix is read from global memory (L2
+// resident for the working set of the active workgroups).  The whole
+// iteration runs inside the kernel (reference/solver/batch_bicgstab_kernels.hpp
+// :180-354): no host round trips, no breakdown guards, the same half-step
+// convergence exit and the same logging of iterations and residual norm.
+//
+// Two storage formats:
+//  * uneven: a block-diagonal CSR, system k owning global rows
+//    [row_off[k], row_off[k+1]) with column indices local to the system;
+//  * uniform: batch::matrix::Csr of the reference (include/ginkgo/core/matrix/
+//    batch_csr.hpp:29-31): one shared pattern of n rows, values per item.
+#include "common.hpp"
+
+namespace gkomi {
+
+namespace {
+
+// all values to every thread; lds must hold 2 * kMaxWaves values
+template <typename T, int NThreads>
+__device__ __forceinline__ void block_sum2(T& a, T& b, T* lds)
+{
+    constexpr int kW = NThreads / kWave;
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) {
+        lds[wave] = a;
+        lds[kW + wave] = b;
+    }
+    __syncthreads();
+    T s0 = lds[0], s1 = lds[kW];
+#pragma unroll
+    for (int w = 1; w < kW; ++w) {
+        s0 += lds[w];
+        s1 += lds[kW + w];
+    }
+    a = s0;
+    b = s1;
+}
+
+template <typename T, int NThreads>
+__device__ __forceinline__ T block_sum1(T a, T* lds)
+{
+    T dummy = 0;
+    block_sum2<T, NThreads>(a, dummy, lds);
+    return a;
+}
+
+struct sys_view {
+    int32_t n;
+    int32_t nnz;
+    int64_t row0;        // first global row (vector offset)
+    int64_t nnz0;        // first value of the system
+    const int32_t* rp;   // global row pointers of the system (uneven) or shared
+    const int32_t* ci;   // column indices (local)
+    int32_t rp_base;     // subtracted from rp entries to get local offsets
+};
+
+}  // namespace
+
+// LDS layout per system slot (n = lds_rows, z = lds_nnz):
+//   T vec[10][n]   r, r_hat, p, p_hat, v, s, s_hat, t, x, prec
+//   T vals[z]      (MatLds)
+//   int32 rp[n+1]  (MatLds)
+//   uint16 ci[z]   (MatLds)
+template <typename T, int NThreads, bool MatLds, bool Uniform>
+__global__ void __launch_bounds__(NThreads)
+    batch_bicgstab_kernel(int64_t num_systems, const int32_t* __restrict__ order,
+                          const int64_t* __restrict__ row_off, const int32_t* __restrict__ rp,
+                          const int32_t* __restrict__ ci, const T* __restrict__ vals,
+                          int64_t uni_n, int64_t uni_nnz, const T* __restrict__ b,
+                          T* __restrict__ x, int32_t max_iters, int32_t relative, double tol_d,
+                          int32_t jacobi, int32_t* __restrict__ iters_out,
+                          T* __restrict__ res_out, unsigned* __restrict__ queue,
+                          int64_t lds_rows, int64_t lds_nnz)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ T red[2 * (NThreads / kWave)];
+    __shared__ int64_t slot;
+    const int tid = threadIdx.x;
+    T* vec = reinterpret_cast<T*>(smem);
+    const int64_t L = lds_rows;
+    T* r = vec;
+    T* rh = vec + L;
+    T* p = vec + 2 * L;
+    T* ph = vec + 3 * L;
+    T* v = vec + 4 * L;
+    T* s = vec + 5 * L;
+    T* sh = vec + 6 * L;
+    T* t = vec + 7 * L;
+    T* xl = vec + 8 * L;
+    T* prec = vec + 9 * L;
+    T* lv = vec + 10 * L;
+    int32_t* lrp = reinterpret_cast<int32_t*>(lv + (MatLds ? lds_nnz : 0));
+    uint16_t* lci = reinterpret_cast<uint16_t*>(lrp + (MatLds ? L + 1 : 0));
+    const T tol = static_cast<T>(tol_d);
+
+    while (true) {
+        __syncthreads();
+        if (tid == 0) slot = static_cast<int64_t>(atomicAdd(queue, 1u));
+        __syncthreads();
+        const int64_t k = slot;
+        if (k >= num_systems) break;
+        const int64_t sys = order ? order[k] : k;
+        sys_view sv;
+        if constexpr (Uniform) {
+            sv.n = static_cast<int32_t>(uni_n);
+            sv.nnz = static_cast<int32_t>(uni_nnz);
+            sv.row0 = sys * uni_n;
+            sv.nnz0 = sys * uni_nnz;
+            sv.rp = rp;
+            sv.ci = ci;
+            sv.rp_base = 0;
+        } else {
+            sv.row0 = row_off[sys];
+            sv.n = static_cast<int32_t>(row_off[sys + 1] - sv.row0);
+            sv.rp = rp + sv.row0;
+            sv.rp_base = sv.rp[0];
+            sv.nnz = sv.rp[sv.n] - sv.rp_base;
+            sv.nnz0 = sv.rp_base;
+            sv.ci = ci + sv.nnz0;
+        }
+        const int32_t n = sv.n;
+        const T* gv = vals + sv.nnz0;
+        const T* gb = b + sv.row0;
+        T* gx = x + sv.row0;
+        // stage the matrix
+        if constexpr (MatLds) {
+            for (int i = tid; i < sv.nnz; i += NThreads) {
+                lv[i] = gv[i];
+                lci[i] = static_cast<uint16_t>(sv.ci[i]);
+            }
+            for (int i = tid; i <= n; i += NThreads) lrp[i] = sv.rp[i] - sv.rp_base;
+        }
+        __syncthreads();
+        auto row_begin = [&](int row) {
+            if constexpr (MatLds) return lrp[row];
+            return sv.rp[row] - sv.rp_base;
+        };
+        auto row_end = [&](int row) {
+            if constexpr (MatLds) return lrp[row + 1];
+            return sv.rp[row + 1] - sv.rp_base;
+        };
+        auto val = [&](int32_t e) {
+            if constexpr (MatLds) return lv[e];
+            return gv[e];
+        };
+        auto col = [&](int32_t e) -> int32_t {
+            if constexpr (MatLds) return lci[e];
+            return sv.ci[e];
+        };
+        // out = A in (thread per row)
+        auto spmv = [&](const T* in, T* out) {
+            for (int row = tid; row < n; row += NThreads) {
+                T sum = 0;
+                const int32_t e1 = row_end(row);
+                for (int32_t e = row_begin(row); e < e1; ++e) sum += val(e) * in[col(e)];
+                out[row] = sum;
+            }
+        };
+        // prec.generate: batch scalar Jacobi (reference/preconditioner/
+        // batch_scalar_jacobi.hpp:79-92) or identity
+        T bb = 0, unused = 0;
+        for (int row = tid; row < n; row += NThreads) {
+            T d = T(1);
+            if (jacobi) {
+                const int32_t e1 = row_end(row);
+                for (int32_t e = row_begin(row); e < e1; ++e) {
+                    if (col(e) == row) {
+                        if (val(e) != T(0)) d = T(1) / val(e);
+                        break;
+                    }
+                }
+            }
+            prec[row] = d;
+            const T bi = gb[row];
+            xl[row] = gx[row];
+            r[row] = bi;
+            bb += bi * bi;
+        }
+        block_sum2<T, NThreads>(bb, unused, red);
+        const T rhs_norm = sqrt(bb);
+        __syncthreads();
+        // r = b - A x, ||r||, r_hat = r, p = p_hat = v = 0
+        T rr = 0;
+        for (int row = tid; row < n; row += NThreads) {
+            T sum = 0;
+            const int32_t e1 = row_end(row);
+            for (int32_t e = row_begin(row); e < e1; ++e) sum += val(e) * xl[col(e)];
+            const T ri = r[row] - sum;
+            r[row] = ri;
+            rh[row] = ri;
+            p[row] = T(0);
+            ph[row] = T(0);
+            v[row] = T(0);
+            rr += ri * ri;
+        }
+        T rho_new = 0;  // r_hat . r, fused with ||r||
+        for (int row = tid; row < n; row += NThreads) rho_new += r[row] * r[row];
+        block_sum2<T, NThreads>(rr, rho_new, red);
+        T res = sqrt(rr);
+        const T limit = relative ? tol * rhs_norm : tol;
+        T rho_old = 1, omega = 1, alpha = 1;
+        int32_t iter = 0;
+        for (; iter < max_iters; ++iter) {
+            if (res <= limit) break;
+            // p = r + beta (p - omega v), p_hat = M p
+            const T beta = (rho_new / rho_old) * (alpha / omega);
+            for (int row = tid; row < n; row += NThreads) {
+                const T pi = r[row] + beta * (p[row] - omega * v[row]);
+                p[row] = pi;
+                ph[row] = prec[row] * pi;
+            }
+            __syncthreads();
+            spmv(ph, v);
+            __syncthreads();
+            T rv = 0;
+            for (int row = tid; row < n; row += NThreads) rv += rh[row] * v[row];
+            rv = block_sum1<T, NThreads>(rv, red);
+            alpha = rho_new / rv;
+            T ss = 0;
+            for (int row = tid; row < n; row += NThreads) {
+                const T si = r[row] - alpha * v[row];
+                s[row] = si;
+                sh[row] = prec[row] * si;
+                ss += si * si;
+            }
+            ss = block_sum1<T, NThreads>(ss, red);
+            res = sqrt(ss);
+            if (res <= limit) {
+                for (int row = tid; row < n; row += NThreads) xl[row] += alpha * ph[row];
+                break;
+            }
+            spmv(sh, t);
+            __syncthreads();
+            T ts = 0, tt = 0;
+            for (int row = tid; row < n; row += NThreads) {
+                ts += t[row] * s[row];
+                tt += t[row] * t[row];
+            }
+            block_sum2<T, NThreads>(ts, tt, red);
+            omega = ts / tt;
+            T r2 = 0, rhr = 0;
+            for (int row = tid; row < n; row += NThreads) {
+                xl[row] += alpha * ph[row] + omega * sh[row];
+                const T ri = s[row] - omega * t[row];
+                r[row] = ri;
+                r2 += ri * ri;
+                rhr += rh[row] * ri;
+            }
+            block_sum2<T, NThreads>(r2, rhr, red);
+            res = sqrt(r2);
+            rho_old = rho_new;
+            rho_new = rhr;
+        }
+        __syncthreads();
+        for (int row = tid; row < n; row += NThreads) gx[row] = xl[row];
+        if (tid == 0) {
+            iters_out[sys] = iter;
+            res_out[sys] = res;
+        }
+    }
+    // the last workgroup out resets the queue for the next launch
+    if (tid == 0) {
+        const unsigned d = atomicAdd(queue + 1, 1u);
+        if (d == gridDim.x - 1) {
+            queue[0] = 0;
+            queue[1] = 0;
+        }
+    }
+}
+
+// ------------------------------------------------------------ launcher ----
+
+constexpr int kBatchMaxBins = GKOMI_BATCH_MAX_BINS;
+constexpr size_t kBatchQueueBytes = 64;  // per bin: queue + done counters
+
+size_t batch_lds_bytes(int vbytes, int64_t rows, int64_t nnz, bool mat_lds)
+{
+    size_t b = static_cast<size_t>(vbytes) * 10 * rows;
+    if (mat_lds) {
+        b += static_cast<size_t>(vbytes) * nnz;
+        b += 4 * (rows + 1);
+        b += 2 * nnz;
+    }
+    return (b + 15) / 16 * 16;
+}
+
+template <typename T, int NT, bool MatLds, bool Uniform>
+int launch_batch(hipStream_t stream, int grid, size_t lds, int64_t count, const int32_t* order,
+                 const gkomi_batch_csr* a, const T* b, T* x, const gkomi_batch_settings* st,
+                 int32_t* iters, T* res, unsigned* queue, int64_t lds_rows, int64_t lds_nnz)
+{
+    auto kern = batch_bicgstab_kernel<T, NT, MatLds, Uniform>;
+    if (lds > 64 * 1024) {
+        GKOMI_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            static_cast<int>(lds)));
+    }
+    kern<<<grid, NT, lds, stream>>>(count, order, a->row_offsets, a->row_ptrs, a->col_idxs,
+                                    static_cast<const T*>(a->values), a->uniform_rows,
+                                    a->uniform_nnz, b, x, st->max_iterations, st->tolerance_type,
+                                    st->tolerance, st->preconditioner, iters, res, queue,
+                                    lds_rows, lds_nnz);
+    GKOMI_CHECK_LAUNCH("batch_bicgstab_kernel");
+    return GKOMI_SUCCESS;
+}
+
+template <typename T, bool MatLds, bool Uniform>
+int launch_batch_nt(int nt, hipStream_t stream, int grid, size_t lds, int64_t count,
+                    const int32_t* order, const gkomi_batch_csr* a, const T* b, T* x,
+                    const gkomi_batch_settings* st, int32_t* iters, T* res, unsigned* queue,
+                    int64_t lds_rows, int64_t lds_nnz)
+{
+    switch (nt) {
+    case 64:
+        return launch_batch<T, 64, MatLds, Uniform>(stream, grid, lds, count, order, a, b, x, st,
+                                                    iters, res, queue, lds_rows, lds_nnz);
+    case 128:
+        return launch_batch<T, 128, MatLds, Uniform>(stream, grid, lds, count, order, a, b, x,
+                                                     st, iters, res, queue, lds_rows, lds_nnz);
+    case 256:
+        return launch_batch<T, 256, MatLds, Uniform>(stream, grid, lds, count, order, a, b, x,
+                                                     st, iters, res, queue, lds_rows, lds_nnz);
+    case 512:
+        return launch_batch<T, 512, MatLds, Uniform>(stream, grid, lds, count, order, a, b, x,
+                                                     st, iters, res, queue, lds_rows, lds_nnz);
+    default:
+        return fail(GKOMI_INVALID_ARGUMENT, "batch: block size must be 64, 128, 256 or 512");
+    }
+}
+
+int device_lds_limit()
+{
+    int dev = 0;
+    int v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        v <= 0) {
+        return 64 * 1024;
+    }
+    return v;
+}
+
+int check_batch(const gkomi_batch_csr* a, const gkomi_batch_settings* st)
+{
+    GKOMI_REQUIRE(a && st, GKOMI_INVALID_ARGUMENT, "batch: null descriptor or settings");
+    GKOMI_REQUIRE(a->num_systems >= 0 && a->max_rows >= 0 && a->max_nnz >= 0,
+                  GKOMI_INVALID_ARGUMENT, "batch: negative sizes");
+    const bool uniform = a->row_offsets == nullptr;
+    GKOMI_REQUIRE(!uniform || (a->uniform_rows == a->max_rows && a->uniform_nnz == a->max_nnz),
+                  GKOMI_INVALID_ARGUMENT, "batch: uniform batch needs max_rows/max_nnz = n/nnz");
+    GKOMI_REQUIRE(a->max_rows <= 65535, GKOMI_NOT_SUPPORTED,
+                  "batch: systems are limited to 65535 rows (16-bit local indices)");
+    GKOMI_REQUIRE(a->num_systems == 0 || (a->row_ptrs && a->col_idxs && a->values),
+                  GKOMI_INVALID_ARGUMENT, "batch: null matrix arrays");
+    GKOMI_REQUIRE(st->tolerance_type == 0 || st->tolerance_type == 1, GKOMI_INVALID_ARGUMENT,
+                  "batch: tolerance_type must be absolute (0) or relative (1)");
+    GKOMI_REQUIRE(st->preconditioner == 0 || st->preconditioner == 1, GKOMI_NOT_SUPPORTED,
+                  "batch: preconditioner must be identity (0) or scalar Jacobi (1)");
+    GKOMI_REQUIRE(st->max_iterations >= 0, GKOMI_INVALID_ARGUMENT, "batch: negative max_iterations");
+    return GKOMI_SUCCESS;
+}
+
+template <typename T>
+int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                     const gkomi_batch_settings* st, const gkomi_batch_plan* plan, const T* b,
+                     T* x, int32_t* iters, T* res, void* ws, size_t wsb)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_batch(a, st)) return rc;
+    if (a->num_systems == 0) return GKOMI_SUCCESS;
+    GKOMI_REQUIRE(b && x && iters && res, GKOMI_INVALID_ARGUMENT, "batch: null vectors");
+    GKOMI_REQUIRE(ws && wsb >= gkomi_batch_workspace_bytes(a), GKOMI_INVALID_ARGUMENT,
+                  "batch: workspace too small (zero-filled, gkomi_batch_workspace_bytes)");
+    gkomi_batch_plan single{};
+    if (plan == nullptr) {
+        single.num_bins = 1;
+        single.bin_begin[0] = 0;
+        single.bin_begin[1] = a->num_systems;
+        single.bin_max_rows[0] = a->max_rows;
+        single.bin_max_nnz[0] = a->max_nnz;
+        single.block_size[0] = a->max_rows > 256 ? 256 : (a->max_rows > 64 ? 128 : 64);
+        single.matrix_in_lds[0] = -1;
+        single.blocks_per_cu[0] = 0;
+        plan = &single;
+    }
+    GKOMI_REQUIRE(plan->num_bins >= 1 && plan->num_bins <= GKOMI_BATCH_MAX_BINS,
+                  GKOMI_INVALID_ARGUMENT, "batch: bad number of bins");
+    GKOMI_REQUIRE(plan->bin_begin[0] == 0 && plan->bin_begin[plan->num_bins] == a->num_systems,
+                  GKOMI_INVALID_ARGUMENT, "batch: bins must cover all systems");
+    const bool uniform = a->row_offsets == nullptr;
+    const int lds_limit = device_lds_limit();
+    hipStream_t s = stream_of(ctx);
+    unsigned* queues = static_cast<unsigned*>(ws);
+    for (int bin = 0; bin < plan->num_bins; ++bin) {
+        const int64_t beg = plan->bin_begin[bin];
+        const int64_t cnt = plan->bin_begin[bin + 1] - beg;
+        if (cnt <= 0) continue;
+        const int64_t rows = plan->bin_max_rows[bin];
+        const int64_t nnz = plan->bin_max_nnz[bin];
+        GKOMI_REQUIRE(rows <= a->max_rows && nnz <= a->max_nnz, GKOMI_INVALID_ARGUMENT,
+                      "batch: bin maxima exceed the batch maxima");
+        int mat = plan->matrix_in_lds[bin];
+        if (mat < 0) {
+            mat = batch_lds_bytes(sizeof(T), rows, nnz, true) <= static_cast<size_t>(lds_limit);
+        }
+        const size_t lds = batch_lds_bytes(sizeof(T), rows, nnz, mat != 0);
+        GKOMI_REQUIRE(lds <= static_cast<size_t>(lds_limit), GKOMI_NOT_SUPPORTED,
+                      "batch: the work vectors of the largest system exceed LDS");
+        int bpc = plan->blocks_per_cu[bin];
+        if (bpc <= 0) {
+            bpc = std::max(1, std::min(8, static_cast<int>((160 * 1024) / std::max<size_t>(lds, 1))));
+        }
+        const int grid = static_cast<int>(
+            std::max<int64_t>(1, std::min<int64_t>(cnt, static_cast<int64_t>(num_cus()) * bpc)));
+        const int32_t* order = plan->order ? plan->order + beg : nullptr;
+        GKOMI_REQUIRE(plan->order != nullptr || plan->num_bins == 1, GKOMI_INVALID_ARGUMENT,
+                      "batch: several bins need an order");
+        unsigned* q = queues + bin * (kBatchQueueBytes / sizeof(unsigned));
+        const int nt = plan->block_size[bin];
+        int rc;
+        if (uniform) {
+            rc = mat ? launch_batch_nt<T, true, true>(nt, s, grid, lds, cnt, order, a, b, x, st,
+                                                     iters, res, q, rows, nnz)
+                     : launch_batch_nt<T, false, true>(nt, s, grid, lds, cnt, order, a, b, x, st,
+                                                      iters, res, q, rows, nnz);
+        } else {
+            rc = mat ? launch_batch_nt<T, true, false>(nt, s, grid, lds, cnt, order, a, b, x, st,
+                                                      iters, res, q, rows, nnz)
+                     : launch_batch_nt<T, false, false>(nt, s, grid, lds, cnt, order, a, b, x,
+                                                       st, iters, res, q, rows, nnz);
+        }
+        if (rc) return rc;
+    }
+    return GKOMI_SUCCESS;
+}
+
+
+template <typename T>
+int batch_tune_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                    const gkomi_batch_settings* st, gkomi_batch_plan* plan, const T* b,
+                    const T* x, T* xs, int32_t* iters, T* res, void* ws, size_t wsb)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_batch(a, st)) return rc;
+    GKOMI_REQUIRE(plan && x && xs, GKOMI_INVALID_ARGUMENT, "batch tune: null plan or vectors");
+    if (a->num_systems == 0) return GKOMI_SUCCESS;
+    const int64_t total_rows =
+        a->row_offsets ? -1 : a->num_systems * a->uniform_rows;  // uneven: read below
+    int64_t rows_all = total_rows;
+    if (rows_all < 0) {
+        GKOMI_CHECK_HIP(hipMemcpy(&rows_all, a->row_offsets + a->num_systems, sizeof(int64_t),
+                                  hipMemcpyDeviceToHost));
+    }
+    const int lds_limit = device_lds_limit();
+    hipStream_t s = stream_of(ctx);
+    hipEvent_t e0, e1;
+    GKOMI_CHECK_HIP(hipEventCreate(&e0));
+    GKOMI_CHECK_HIP(hipEventCreate(&e1));
+    int rc = GKOMI_SUCCESS;
+    for (int bin = 0; bin < plan->num_bins && rc == GKOMI_SUCCESS; ++bin) {
+        gkomi_batch_plan one = *plan;
+        one.num_bins = 1;
+        one.order = plan->order ? plan->order + plan->bin_begin[bin] : nullptr;
+        const int64_t cnt = plan->bin_begin[bin + 1] - plan->bin_begin[bin];
+        if (cnt <= 0) continue;
+        one.bin_begin[0] = 0;
+        one.bin_begin[1] = cnt;
+        one.bin_max_rows[0] = plan->bin_max_rows[bin];
+        one.bin_max_nnz[0] = plan->bin_max_nnz[bin];
+        one.blocks_per_cu[0] = plan->blocks_per_cu[bin];
+        gkomi_batch_csr sub = *a;
+        sub.num_systems = cnt;
+        float best = 1e30f;
+        int best_nt = 256, best_mat = 0;
+        const bool fits = batch_lds_bytes(sizeof(T), one.bin_max_rows[0], one.bin_max_nnz[0],
+                                          true) <= static_cast<size_t>(lds_limit);
+        for (int nt : {64, 128, 256, 512}) {
+            for (int mat = fits ? 1 : 0; mat >= 0; --mat) {
+                one.block_size[0] = nt;
+                one.matrix_in_lds[0] = mat;
+                // sub-batch of this bin only: order is rebased, so the
+                // descriptor keeps the full arrays but counts cnt systems
+                GKOMI_CHECK_HIP(hipMemcpyAsync(xs, x, sizeof(T) * rows_all,
+                                               hipMemcpyDeviceToDevice, s));
+                GKOMI_CHECK_HIP(hipEventRecord(e0, s));
+                rc = batch_solve_impl<T>(ctx, &sub, st, &one, b, xs, iters, res, ws, wsb);
+                if (rc) break;
+                GKOMI_CHECK_HIP(hipEventRecord(e1, s));
+                GKOMI_CHECK_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                GKOMI_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    best_nt = nt;
+                    best_mat = mat;
+                }
+            }
+            if (rc) break;
+        }
+        plan->block_size[bin] = best_nt;
+        plan->matrix_in_lds[bin] = best_mat;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+}  // namespace gkomi
+
+using namespace gkomi;
+
+extern "C" {
+
+size_t gkomi_batch_workspace_bytes(const gkomi_batch_csr* a)
+{
+    (void)a;
+    return kBatchQueueBytes * kBatchMaxBins;
+}
+
+size_t gkomi_batch_lds_bytes(int32_t value_bytes, int64_t rows, int64_t nnz,
+                             int32_t matrix_in_lds)
+{
+    return batch_lds_bytes(value_bytes, rows, nnz, matrix_in_lds != 0);
+}
+
+int gkomi_batch_bicgstab_solve_f64(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                                   const gkomi_batch_settings* st, const gkomi_batch_plan* plan,
+                                   const double* b, double* x, int32_t* iters, double* res,
+                                   void* ws, size_t wsb)
+{
+    return batch_solve_impl(ctx, a, st, plan, b, x, iters, res, ws, wsb);
+}
+
+int gkomi_batch_bicgstab_solve_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                                   const gkomi_batch_settings* st, const gkomi_batch_plan* plan,
+                                   const float* b, float* x, int32_t* iters, float* res,
+                                   void* ws, size_t wsb)
+{
+    return batch_solve_impl(ctx, a, st, plan, b, x, iters, res, ws, wsb);
+}
+
+int gkomi_batch_bicgstab_tune_f64(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                                  const gkomi_batch_settings* st, gkomi_batch_plan* plan,
+                                  const double* b, const double* x, double* xs, int32_t* iters,
+                                  double* res, void* ws, size_t wsb)
+{
+    return batch_tune_impl(ctx, a, st, plan, b, x, xs, iters, res, ws, wsb);
+}
+
+int gkomi_batch_bicgstab_tune_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                                  const gkomi_batch_settings* st, gkomi_batch_plan* plan,
+                                  const float* b, const float* x, float* xs, int32_t* iters,
+                                  float* res, void* ws, size_t wsb)
+{
+    return batch_tune_impl(ctx, a, st, plan, b, x, xs, iters, res, ws, wsb);
+}
+
+}  // extern "C"

@@ -328,3 +328,135 @@ class FusedCg:
     @property
     def iteration(self):
         return int(self.flags[0].item())
+
+
+class DeviceBatchCsr:
+    """A batch of independent systems on the device.
+
+    Uneven (row_off given): block-diagonal CSR with column indices local to
+    each system.  Uniform (`DeviceBatchCsr.uniform`): the reference's
+    batch::matrix::Csr with one shared pattern and values per system."""
+
+    def __init__(self, row_off, row_ptrs, col_idxs, values, device="cuda"):
+        dev = torch.device(device)
+        self.device = dev
+        row_off = np.asarray(row_off, dtype=np.int64)
+        rp = np.asarray(row_ptrs, dtype=np.int32)
+        sizes = np.diff(row_off)
+        nnzs = rp[row_off[1:]] - rp[row_off[:-1]]
+        self.sizes, self.nnzs = sizes, nnzs
+        self.num_systems = len(sizes)
+        self.total_rows = int(row_off[-1])
+        self.row_off = torch.as_tensor(row_off).to(dev)
+        self.row_ptrs = torch.as_tensor(rp).to(dev)
+        self.col_idxs = torch.as_tensor(np.asarray(col_idxs, dtype=np.int32)).to(dev)
+        self.values = torch.as_tensor(np.asarray(values)).to(dev) \
+            if not torch.is_tensor(values) else values.to(dev)
+        self.dtype = self.values.dtype
+        self.uniform = False
+        self.desc = self._desc()
+
+    @classmethod
+    def uniform(cls, num_systems, row_ptrs, col_idxs, values, device="cuda"):
+        self = cls.__new__(cls)
+        dev = torch.device(device)
+        self.device = dev
+        rp = np.asarray(row_ptrs, dtype=np.int32)
+        n = len(rp) - 1
+        nnz = int(rp[-1])
+        self.sizes = np.full(num_systems, n, dtype=np.int64)
+        self.nnzs = np.full(num_systems, nnz, dtype=np.int64)
+        self.num_systems = num_systems
+        self.total_rows = num_systems * n
+        self.row_off = None
+        self.row_ptrs = torch.as_tensor(rp).to(dev)
+        self.col_idxs = torch.as_tensor(np.asarray(col_idxs, dtype=np.int32)).to(dev)
+        self.values = torch.as_tensor(np.asarray(values)).to(dev) \
+            if not torch.is_tensor(values) else values.to(dev)
+        self.dtype = self.values.dtype
+        self.uniform = True
+        self.desc = self._desc()
+        return self
+
+    def _desc(self):
+        d = C.BatchCsr()
+        d.num_systems = self.num_systems
+        d.max_rows = int(self.sizes.max()) if self.num_systems else 0
+        d.max_nnz = int(self.nnzs.max()) if self.num_systems else 0
+        d.row_offsets = ptr(self.row_off).value if self.row_off is not None else None
+        d.row_ptrs = ptr(self.row_ptrs).value
+        d.col_idxs = ptr(self.col_idxs).value
+        d.values = ptr(self.values).value
+        if self.uniform:
+            d.uniform_rows = int(self.sizes[0]) if self.num_systems else 0
+            d.uniform_nnz = int(self.nnzs[0]) if self.num_systems else 0
+        return d
+
+    def plan(self, edges=(1 << 16, 512, 256, 128, 64, 0), block_size=None):
+        """Size-ordered work queue split into bins (n in (edges[i+1], edges[i]])."""
+        order = np.argsort(-self.sizes, kind="stable").astype(np.int32)
+        ss = self.sizes[order]
+        nz = self.nnzs[order]
+        self._order = torch.as_tensor(order).to(self.device)
+        p = C.BatchPlan()
+        p.order = ptr(self._order).value
+        begin = 0
+        nb = 0
+        for hi, lo in zip(edges[:-1], edges[1:]):
+            end = int(np.searchsorted(-ss, -lo, side="left"))
+            if end > begin:
+                p.bin_begin[nb] = begin
+                p.bin_max_rows[nb] = int(ss[begin:end].max())
+                p.bin_max_nnz[nb] = int(nz[begin:end].max())
+                r = p.bin_max_rows[nb]
+                p.block_size[nb] = block_size or (256 if r > 256 else (128 if r > 64 else 64))
+                p.matrix_in_lds[nb] = -1
+                p.blocks_per_cu[nb] = 0
+                nb += 1
+                begin = end
+        p.num_bins = max(nb, 1)
+        p.bin_begin[p.num_bins] = self.num_systems
+        if nb == 0:
+            p.bin_max_rows[0] = 0
+            p.bin_max_nnz[0] = 0
+            p.block_size[0] = 64
+        return p
+
+    def workspace(self):
+        if getattr(self, "_ws", None) is None:
+            self._ws = workspace(C.lib().gkomi_batch_workspace_bytes(ctypes.byref(self.desc)),
+                                 self.device)
+        return self._ws
+
+    def settings(self, tol=1e-10, relative=True, max_iters=100, jacobi=True):
+        s = C.BatchSettings()
+        s.max_iterations = int(max_iters)
+        s.tolerance_type = 1 if relative else 0
+        s.tolerance = float(tol)
+        s.preconditioner = 1 if jacobi else 0
+        return s
+
+    def solve(self, b, x, settings, plan=None, stream=None):
+        """BiCGSTAB on every system; x holds the initial guess and gets the
+        solution.  Returns (iterations, residual norms) as device tensors."""
+        iters = torch.empty(self.num_systems, dtype=torch.int32, device=self.device)
+        res = torch.empty(self.num_systems, dtype=self.dtype, device=self.device)
+        ws = self.workspace()
+        c = ctx(self.device, stream)
+        C.call(f"gkomi_batch_bicgstab_solve_{suffix(self.dtype)}", ctypes.byref(c),
+               ctypes.byref(self.desc), ctypes.byref(settings),
+               ctypes.byref(plan) if plan is not None else None, ptr(b), ptr(x), ptr(iters),
+               ptr(res), ptr(ws), ws.numel())
+        return iters, res
+
+    def tune(self, b, x, settings, plan):
+        """Autotune block size and LDS residency per bin (writes into plan)."""
+        xs = torch.empty_like(x)
+        iters = torch.empty(self.num_systems, dtype=torch.int32, device=self.device)
+        res = torch.empty(self.num_systems, dtype=self.dtype, device=self.device)
+        ws = self.workspace()
+        c = ctx(self.device)
+        C.call(f"gkomi_batch_bicgstab_tune_{suffix(self.dtype)}", ctypes.byref(c),
+               ctypes.byref(self.desc), ctypes.byref(settings), ctypes.byref(plan), ptr(b),
+               ptr(x), ptr(xs), ptr(iters), ptr(res), ptr(ws), ws.numel())
+        return plan

@@ -284,3 +284,41 @@ def dense_to_csr(d, keep_zeros=False):
     row_ptrs = np.zeros(d.shape[0] + 1, dtype=np.int32)
     np.cumsum(counts, out=row_ptrs[1:])
     return row_ptrs, cols.astype(np.int32), d[rows, cols]
+
+
+def pad_to_uniform(row_off, row_ptrs, col_idxs, vals, b, n_pad=None):
+    """The reference-compatible uniform batch of an uneven one: every system
+    padded to n_pad rows with one shared pattern, the union of all systems'
+    patterns plus the diagonal (batch::matrix::Csr needs one pattern,
+    include/ginkgo/core/matrix/batch_csr.hpp:29-31).  Padded rows carry a
+    unit diagonal and a zero right-hand side, so the padded unknowns are 0.
+    Returns (row_ptrs[n_pad+1], col_idxs, values[S * nnz], b[S * n_pad])."""
+    row_off = np.asarray(row_off, dtype=np.int64)
+    sizes = np.diff(row_off)
+    S = len(sizes)
+    n_pad = int(sizes.max()) if n_pad is None else int(n_pad)
+    total = int(row_off[-1])
+    sys_of_row = np.repeat(np.arange(S), sizes)
+    local_row = np.arange(total) - row_off[sys_of_row]
+    counts = np.diff(row_ptrs)
+    e_row = np.repeat(local_row, counts)
+    e_sys = np.repeat(sys_of_row, counts)
+    mask = np.zeros((n_pad, n_pad), dtype=bool)
+    mask[e_row, col_idxs] = True
+    mask[np.arange(n_pad), np.arange(n_pad)] = True
+    pr, pc = np.nonzero(mask)
+    nnz = pr.shape[0]
+    index = np.full((n_pad, n_pad), -1, dtype=np.int64)
+    index[pr, pc] = np.arange(nnz)
+    rp = np.zeros(n_pad + 1, dtype=np.int32)
+    np.cumsum(np.bincount(pr, minlength=n_pad), out=rp[1:])
+    values = np.zeros((S, nnz), dtype=vals.dtype)
+    values[e_sys, index[e_row, col_idxs]] = vals
+    # unit diagonal on the padded rows of each system
+    diag_pos = index[np.arange(n_pad), np.arange(n_pad)]
+    pad_mask = np.arange(n_pad)[None, :] >= sizes[:, None]
+    rows_s, rows_i = np.nonzero(pad_mask)
+    values[rows_s, diag_pos[rows_i]] = 1.0
+    bp = np.zeros((S, n_pad), dtype=b.dtype)
+    bp[sys_of_row, local_row] = b
+    return rp, pc.astype(np.int32), values.reshape(-1), bp.reshape(-1)

@@ -476,6 +476,93 @@ int gkomi_bicgstab_finalize_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, f
                                 int64_t x_stride, const float* y, int64_t vs,
                                 const float* alpha, uint8_t* status);
 
+
+/* ----------------------------------------------------- batched BiCGSTAB */
+
+/* A batch of independent sparse systems (batch::matrix::Csr,
+ * include/ginkgo/core/matrix/batch_csr.hpp:29-31, generalised to uneven
+ * sizes).  Uneven: system k owns global rows [row_offsets[k], row_offsets[k+1])
+ * of one block-diagonal CSR whose column indices are local to the system.
+ * Uniform (row_offsets == NULL): every system has uniform_rows rows and the
+ * shared pattern row_ptrs[uniform_rows + 1] / col_idxs[uniform_nnz]; system k's
+ * values start at k * uniform_nnz.  Vectors are concatenated per system. */
+#define GKOMI_BATCH_MAX_BINS 8
+typedef struct {
+    int64_t num_systems;
+    int64_t max_rows;
+    int64_t max_nnz;
+    const int64_t* row_offsets;
+    const int32_t* row_ptrs;
+    const int32_t* col_idxs;
+    const void* values;
+    int64_t uniform_rows;
+    int64_t uniform_nnz;
+} gkomi_batch_csr;
+
+/* kernels::batch_bicgstab::settings (core/solver/batch_bicgstab_kernels.hpp):
+ * tolerance_type 0 = absolute, 1 = relative to ||b||
+ * (reference/stop/batch_criteria.hpp); preconditioner 0 = identity,
+ * 1 = scalar Jacobi (reference/preconditioner/batch_scalar_jacobi.hpp). */
+typedef struct {
+    int32_t max_iterations;
+    int32_t tolerance_type;
+    double tolerance;
+    int32_t preconditioner;
+    int32_t reserved;
+} gkomi_batch_settings;
+
+/* Launch plan: systems in `order` (device, largest first) split into bins of
+ * similar size; each bin is one persistent launch with its own workgroup
+ * size, LDS residency of the matrix (1 = LDS, 0 = global, -1 = LDS if it
+ * fits) and workgroups per CU (0 = from the LDS footprint).  No reference
+ * counterpart: the reference launches one uniform kernel per batch
+ * (hip/solver/batch_bicgstab_kernels.hip.cpp). */
+typedef struct {
+    int32_t num_bins;
+    int32_t reserved;
+    const int32_t* order;
+    int64_t bin_begin[GKOMI_BATCH_MAX_BINS + 1];
+    int64_t bin_max_rows[GKOMI_BATCH_MAX_BINS];
+    int64_t bin_max_nnz[GKOMI_BATCH_MAX_BINS];
+    int32_t block_size[GKOMI_BATCH_MAX_BINS];
+    int32_t matrix_in_lds[GKOMI_BATCH_MAX_BINS];
+    int32_t blocks_per_cu[GKOMI_BATCH_MAX_BINS];
+} gkomi_batch_plan;
+
+/* Zero-filled scratch for the work queues; left zero-filled after every call. */
+size_t gkomi_batch_workspace_bytes(const gkomi_batch_csr* a);
+/* LDS bytes one workgroup needs for a system of `rows` rows and `nnz` entries. */
+size_t gkomi_batch_lds_bytes(int32_t value_bytes, int64_t rows, int64_t nnz,
+                             int32_t matrix_in_lds);
+/* batch_bicgstab::apply (core/solver/batch_bicgstab_kernels.hpp:177-198;
+ * reference/solver/batch_bicgstab_kernels.hpp:180-354): x (initial guess in,
+ * solution out), per-system iteration counts and final residual norms (the
+ * batch log_data).  plan == NULL: one bin, natural order. */
+int gkomi_batch_bicgstab_solve_f64(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                                   const gkomi_batch_settings* settings,
+                                   const gkomi_batch_plan* plan, const double* b, double* x,
+                                   int32_t* iterations, double* residual_norms,
+                                   void* workspace, size_t workspace_bytes);
+int gkomi_batch_bicgstab_solve_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                                   const gkomi_batch_settings* settings,
+                                   const gkomi_batch_plan* plan, const float* b, float* x,
+                                   int32_t* iterations, float* residual_norms, void* workspace,
+                                   size_t workspace_bytes);
+/* Autotuner: for every bin of `plan`, times the candidate workgroup sizes
+ * (64..512) and LDS / global matrix residency on that bin's systems (solving
+ * into x_scratch from the initial guess x, which is not modified) and
+ * writes the fastest choice into the plan.  No reference counterpart. */
+int gkomi_batch_bicgstab_tune_f64(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                                  const gkomi_batch_settings* settings, gkomi_batch_plan* plan,
+                                  const double* b, const double* x, double* x_scratch,
+                                  int32_t* iterations, double* residual_norms, void* workspace,
+                                  size_t workspace_bytes);
+int gkomi_batch_bicgstab_tune_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                                  const gkomi_batch_settings* settings, gkomi_batch_plan* plan,
+                                  const float* b, const float* x, float* x_scratch,
+                                  int32_t* iterations, float* residual_norms, void* workspace,
+                                  size_t workspace_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -305,8 +305,15 @@ def test_cg_and_bicgstab_step_kats(dev, dtype):
     dt = TD[dtype]
     s = "f64" if dtype == np.float64 else "f32"
     c = D().ctx(dev)
-    P = D().ptr
-    st = lambda v: torch.tensor(v, dtype=torch.uint8, device=dev)
+    keep = []  # every tensor handed to the C ABI stays alive until the call returns
+
+    def P(t):
+        keep.append(t)
+        return D().ptr(t)
+
+    def st(v):
+        return torch.tensor(v, dtype=torch.uint8, device=dev)
+
     k = KATS["cg_kernels"]["step_1"]
     p, z = _full(k["p_fill"], dev, dt), _full(k["z_fill"], dev, dt)
     C.call(f"gkomi_cg_step_1_{s}", ctypes.byref(c), 2, 2, P(p), P(z), 2, P(_vec(k["rho"], dev, dt)),
