@@ -94,6 +94,150 @@ def cpu_spmv_baseline(rp, ci, v, x, budget_s=12.0):
     }
 
 
+def timed(fn):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, out
+
+
+def bench_cg_vs_ir(dev, nx=96):
+    """96^3 27-pt variable-coefficient diffusion: plain FP64 CG to 1e-12 vs
+    mixed-precision IR (FP64 outer to 1e-12, FP32 CG inner reducing the
+    residual by 1e-3 per sweep), both through the gko:: C++ solvers."""
+    import numpy as np
+    import torch
+    import ginkgo_amd.gko as gko
+    from ginkgo_amd import generators as gen
+    log(f"[bench] generating {nx}^3 27-pt diffusion")
+    rp, ci, v = gen.diffusion27(nx)
+    n, nnz = len(rp) - 1, int(rp[-1])
+    exec_ = gko.Mi355xExecutor.create(0, torch.cuda.current_stream(dev).cuda_stream)
+    A = gko.csr_from_host(exec_, rp, ci, v, strategy="classical")
+    b_h = np.random.default_rng(4).uniform(-1, 1, (n, 1))
+    b = gko.matrix.Dense("f64").from_numpy(exec_, b_h)
+    crit = lambda dt, tol, it: (gko.stop.Iteration.build().with_max_iters(it),
+                                gko.stop.ResidualNorm.build(dt).with_reduction_factor(tol))
+    cg = gko.solver.Cg.build("f64").with_criteria(*crit("f64", 1e-12, 20000)).on(exec_).generate(A)
+    inner = gko.solver.Cg.build("f32").with_criteria(*crit("f32", 1e-3, 20000))
+    ir = (gko.solver.Ir.build("f64").with_criteria(*crit("f64", 1e-12, 200))
+          .with_solver(inner).on(exec_).generate(A))
+    res = {}
+    sols = {}
+    for name, solver in (("cg_f64", cg), ("ir_f64_f32", ir)):
+        best = None
+        for rep in range(3):
+            x = gko.matrix.Dense("f64").create(exec_, n, 1)
+            x.fill(0.0)
+            t, _ = timed(lambda: solver.apply(b, x))
+            best = t if best is None else min(best, t)
+        sols[name] = x.to_numpy()[:, 0]
+        res[name] = {"time_s": round(best, 5), "iterations": int(solver.last_iterations),
+                     "status": solver.last_status}
+    # true relative residuals and the distance between the two solutions
+    d_rp, d_ci = rp, ci
+    import oracle as ora
+    for name, xs in sols.items():
+        r = b_h[:, 0] - ora.csr_spmv(d_rp, d_ci, v, xs.reshape(-1, 1))[:, 0]
+        res[name]["true_relres"] = float(np.linalg.norm(r) / np.linalg.norm(b_h))
+    res["ir_vs_cg_rel_diff"] = float(np.linalg.norm(sols["ir_f64_f32"] - sols["cg_f64"]) /
+                                     np.linalg.norm(sols["cg_f64"]))
+    res["speedup_ir_over_cg"] = round(res["cg_f64"]["time_s"] / res["ir_f64_f32"]["time_s"], 3)
+    b_it64 = nnz * 12 + 4 * (n + 1) + 18 * n * 8
+    res["cg_f64"]["achieved_GBps"] = round(
+        res["cg_f64"]["iterations"] * b_it64 / res["cg_f64"]["time_s"] / 1e9, 1)
+    res["problem"] = {"n": n, "nnz": nnz, "reference_note": "x is compared with the FP64 CG "
+                      "solution; the direct FP64 solve of 884,736 unknowns is not run in the bench"}
+    return res, (rp, ci, v, b_h)
+
+
+def bench_batch(dev, num=20000):
+    """20,000 uneven nonsymmetric systems, n ~ U{16..512}, BiCGSTAB + scalar
+    Jacobi to 1e-10 (relative): tuned uneven batch vs the reference-style
+    uniform batch padded to n = 512 with the union pattern."""
+    import numpy as np
+    import torch
+    from ginkgo_amd import device as D
+    from ginkgo_amd import generators as gen
+    log(f"[bench] generating {num} uneven systems")
+    row_off, rp, ci, v, b = gen.uneven_batch(num)
+    out = {"num_systems": num, "total_rows": int(row_off[-1]), "total_nnz": int(rp[-1])}
+    for dt, tol in ((np.float64, 1e-10), (np.float32, 1e-5)):
+        A = D.DeviceBatchCsr(row_off, rp, ci, v.astype(dt), device=dev)
+        bd = torch.as_tensor(b.astype(dt)).to(dev)
+        x0 = torch.zeros_like(bd)
+        st = A.settings(tol=tol, relative=True, max_iters=500, jacobi=True)
+        plan = A.tune(bd, x0, st, A.plan())
+        times = []
+        for _ in range(3):
+            x = torch.zeros_like(bd)
+            t, (it, rs) = timed(lambda: A.solve(bd, x, st, plan))
+            times.append(t)
+        key = "f64" if dt == np.float64 else "f32"
+        out[f"uneven_{key}"] = {"time_s": round(min(times), 5),
+                                "solves_per_s": round(num / min(times), 1),
+                                "max_iterations": int(it.max().item()),
+                                "mean_iterations": round(float(it.float().mean().item()), 2),
+                                "plan_block_sizes": [plan.block_size[i] for i in range(plan.num_bins)],
+                                "plan_matrix_in_lds": [plan.matrix_in_lds[i] for i in range(plan.num_bins)]}
+        del A
+    log("[bench] padding the batch to n = 512 (reference-compatible uniform batch)")
+    prp, pci, pv, pb = gen.pad_to_uniform(row_off, rp, ci, v, b, 512)
+    U = D.DeviceBatchCsr.uniform(num, prp, pci, pv, device=dev)
+    ub = torch.as_tensor(pb).to(dev)
+    st = U.settings(tol=1e-10, relative=True, max_iters=500, jacobi=True)
+    plan = U.plan()
+    plan = U.tune(ub, torch.zeros_like(ub), st, plan)
+    times = []
+    for _ in range(2):
+        ux = torch.zeros_like(ub)
+        t, (it, rs) = timed(lambda: U.solve(ub, ux, st, plan))
+        times.append(t)
+    out["padded_uniform_f64"] = {"time_s": round(min(times), 5),
+                                 "solves_per_s": round(num / min(times), 1),
+                                 "pattern_nnz_per_system": int(prp[-1])}
+    out["speedup_uneven_over_padded_f64"] = round(
+        out["padded_uniform_f64"]["time_s"] / out["uneven_f64"]["time_s"], 3)
+    del U, ub
+    torch.cuda.empty_cache()
+    return out, (row_off, rp, ci, v, b)
+
+
+def bench_adjoint(dev, nx=512, nrhs=64):
+    """512^2 anisotropic diffusion, 64 right-hand sides: forward FP64 CG to
+    1e-12 plus the adjoint backward (A^T solve + sampled outer product)."""
+    import numpy as np
+    import torch
+    from ginkgo_amd import generators as gen
+    from ginkgo_amd.autograd import sparse_solve
+    log("[bench] adjoint: generating 512^2 anisotropic diffusion")
+    rp, ci, vals, theta, vmap, coef = gen.anisotropic2d(nx, seed=6)
+    n, nnz = nx * nx, len(ci)
+    vmap_t = torch.as_tensor(vmap).to(dev)
+    coef_t = torch.as_tensor(coef).to(dev)
+    th = torch.as_tensor(theta).to(dev).requires_grad_(True)
+    rng = np.random.default_rng(7)
+    b = torch.as_tensor(rng.uniform(-1, 1, (n, nrhs))).to(dev)
+    xt = torch.as_tensor(rng.uniform(-1, 1, (n, nrhs))).to(dev)
+    rp_t = torch.as_tensor(rp).to(dev)
+    ci_t = torch.as_tensor(ci).to(dev)
+
+    def run():
+        th.grad = None
+        v = torch.zeros(nnz, dtype=torch.float64, device=dev).index_add(
+            0, vmap_t[:, 0], coef_t * th[vmap_t[:, 1]])
+        x = sparse_solve(v, b, rp_t, ci_t, method="cg", tol=1e-12, max_iters=20000)
+        loss = 0.5 * ((x - xt) ** 2).sum()
+        loss.backward()
+        return loss
+    t, loss = timed(run)
+    t2, loss = timed(run)
+    return {"time_s_forward_backward": round(min(t, t2), 4), "n": n, "nnz": nnz, "nrhs": nrhs,
+            "loss": float(loss.item())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +328,15 @@ def main():
         secondary["csr_f64_powerlaw1M_GBps"] = round(spmv_bytes(pn, pnnz, 8) / (ms * 1e-3) / 1e9, 1)
         secondary["powerlaw_nnz"] = pnnz
         del P
+    if not args.quick:
+        for name, fn in (("cg_vs_ir_96cubed", lambda: bench_cg_vs_ir(dev)[0]),
+                         ("batched_bicgstab_20000", lambda: bench_batch(dev)[0]),
+                         ("adjoint_512sq_64rhs", lambda: bench_adjoint(dev))):
+            try:
+                secondary[name] = fn()
+            except Exception as e:  # a secondary failure must not hide the headline
+                secondary[name] = {"error": f"{type(e).__name__}: {e}"}
+            torch.cuda.empty_cache()
     if secondary:
         result["secondary"] = secondary
 

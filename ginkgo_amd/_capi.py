@@ -254,6 +254,9 @@ for _s in ("f64", "f32"):
                                                        _P, _P, _I64, _P, _P, _P, _P, _P]),
         f"gkomi_bicgstab_finalize_{_s}": (ctypes.c_int, [_CTX, _I64, _I64, _P, _I64, _P, _I64,
                                                          _P, _P]),
+        f"gkomi_csr_sampled_outer_product_{_s}": (ctypes.c_int, [_CTX, ctypes.POINTER(Csr),
+                                                                _I64, _D if _s == "f64" else _F,
+                                                                _P, _I64, _P, _I64, _P]),
         f"gkomi_batch_bicgstab_solve_{_s}": (ctypes.c_int, [_CTX, ctypes.POINTER(BatchCsr),
                                                            ctypes.POINTER(BatchSettings),
                                                            ctypes.POINTER(BatchPlan), _P, _P, _P,

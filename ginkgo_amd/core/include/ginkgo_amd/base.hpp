@@ -361,3 +361,14 @@ inline const gkomi_ctx* ctx_of(const std::shared_ptr<const Executor>& exec)
 }
 
 }  // namespace gko
+
+namespace gko {
+
+// gko::share (include/ginkgo/core/base/utils_helper.hpp): unique -> shared
+template <typename T>
+std::shared_ptr<T> share(std::unique_ptr<T>&& p)
+{
+    return std::shared_ptr<T>(std::move(p));
+}
+
+}  // namespace gko

@@ -7,3 +7,4 @@
 #include "ginkgo_amd/matrix.hpp"
 #include "ginkgo_amd/solver.hpp"
 #include "ginkgo_amd/stop.hpp"
+#include "ginkgo_amd/mtx_io.hpp"

@@ -484,3 +484,98 @@ int gkomi_jacobi_advanced_apply_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrh
 }
 
 }  // extern "C"
+
+// ------------------------------------------------ sampled outer product ----
+// out[k] = alpha * sum_j L[row(k), j] * X[col(k), j] for every stored entry k
+// of the pattern (the gradient of a solve with respect to the matrix values
+// is this product with L = adjoint solution, X = forward solution,
+// alpha = -1).  One wave per row: the row's L entries are held in registers
+// (lane = right-hand side), each stored entry gathers one row of X with a
+// coalesced load and reduces across the wave.  nrhs == 1 uses one thread
+// per stored entry.
+namespace gkomi {
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    sampled_outer_rowwave(int64_t n_rows, int64_t nrhs, const int32_t* __restrict__ rp,
+                          const int32_t* __restrict__ ci, const T* __restrict__ l, int64_t ls,
+                          const T* __restrict__ x, int64_t xs, T alpha, T* __restrict__ out)
+{
+    const int lane = threadIdx.x % kWave;
+    const int64_t wave_id =
+        static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kWavesPerBlock +
+        threadIdx.x / kWave;
+    const int64_t num_waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+    for (int64_t row = wave_id; row < n_rows; row += num_waves) {
+        const int32_t s = rp[row];
+        const int32_t e = rp[row + 1];
+        for (int32_t k = s; k < e; ++k) {
+            const int64_t c = ci[k];
+            T acc = 0;
+            for (int64_t j = lane; j < nrhs; j += kWave) {
+                acc += l[row * ls + j] * x[c * xs + j];
+            }
+            acc = wave_sum(acc);
+            if (lane == 0) out[k] = alpha * acc;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    sampled_outer_single(int64_t n_rows, const int32_t* __restrict__ rp,
+                         const int32_t* __restrict__ ci, const T* __restrict__ l, int64_t ls,
+                         const T* __restrict__ x, int64_t xs, T alpha, T* __restrict__ out)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t row = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; row < n_rows;
+         row += stride) {
+        const T li = alpha * l[row * ls];
+        for (int32_t k = rp[row]; k < rp[row + 1]; ++k) {
+            out[k] = li * x[static_cast<int64_t>(ci[k]) * xs];
+        }
+    }
+}
+
+template <typename T>
+int sampled_outer_impl(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t nrhs, T alpha,
+                       const T* l, int64_t ls, const T* x, int64_t xs, T* out)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_csr(a)) return rc;
+    GKOMI_REQUIRE(nrhs >= 1, GKOMI_INVALID_ARGUMENT, "sampled outer product: nrhs must be >= 1");
+    if (a->nnz == 0 || a->n_rows == 0) return GKOMI_SUCCESS;
+    GKOMI_REQUIRE(l && x && out, GKOMI_INVALID_ARGUMENT, "sampled outer product: null pointer");
+    GKOMI_REQUIRE(ls >= nrhs && xs >= nrhs, GKOMI_INVALID_ARGUMENT,
+                  "sampled outer product: stride smaller than the columns");
+    if (nrhs == 1) {
+        sampled_outer_single<T><<<grid_for(a->n_rows, kBlock), kBlock, 0, stream_of(ctx)>>>(
+            a->n_rows, a->row_ptrs, a->col_idxs, l, ls, x, xs, alpha, out);
+    } else {
+        sampled_outer_rowwave<T><<<grid_for(a->n_rows, kWavesPerBlock, 32), kBlock, 0,
+                                   stream_of(ctx)>>>(a->n_rows, nrhs, a->row_ptrs, a->col_idxs,
+                                                     l, ls, x, xs, alpha, out);
+    }
+    GKOMI_CHECK_LAUNCH("sampled_outer_product");
+    return GKOMI_SUCCESS;
+}
+
+}  // namespace gkomi
+
+extern "C" {
+
+int gkomi_csr_sampled_outer_product_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t nrhs,
+                                        double alpha, const double* l, int64_t ls,
+                                        const double* x, int64_t xs, double* out)
+{
+    return sampled_outer_impl(ctx, a, nrhs, alpha, l, ls, x, xs, out);
+}
+
+int gkomi_csr_sampled_outer_product_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t nrhs,
+                                        float alpha, const float* l, int64_t ls, const float* x,
+                                        int64_t xs, float* out)
+{
+    return sampled_outer_impl(ctx, a, nrhs, alpha, l, ls, x, xs, out);
+}
+
+}  // extern "C"

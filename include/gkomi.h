@@ -563,6 +563,20 @@ int gkomi_batch_bicgstab_tune_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* a
                                   int32_t* iterations, float* residual_norms, void* workspace,
                                   size_t workspace_bytes);
 
+/* ------------------------------------------------------ adjoint support */
+
+/* Sampled outer product on the pattern of A: out[k] = alpha * sum_j
+ * L(row(k), j) * X(col(k), j) for every stored entry k.  With L the adjoint
+ * solution of A^T L = dJ/dX and X the forward solution, alpha = -1 gives
+ * dJ/da_ij.  No reference counterpart (the reference has no differentiable
+ * solve); `a->values` is not read. */
+int gkomi_csr_sampled_outer_product_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t nrhs,
+                                        double alpha, const double* l, int64_t l_stride,
+                                        const double* x, int64_t x_stride, double* out);
+int gkomi_csr_sampled_outer_product_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t nrhs,
+                                        float alpha, const float* l, int64_t l_stride,
+                                        const float* x, int64_t x_stride, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,90 @@
+"""MatrixMarket input and the fork's CG harness (examples/simple_solver.cpp)
+on the device, against the oracle's CG on the same matrices."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as ora
+from ginkgo_amd import generators as gen
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+
+def read_mtx(path):
+    """Reads a MatrixMarket coordinate file into CSR on the host (test helper)."""
+    with open(path) as f:
+        header = f.readline().lower().split()
+        line = f.readline()
+        while line.startswith("%"):
+            line = f.readline()
+        n, m, nnz = map(int, line.split())
+        data = np.loadtxt(f, ndmin=2)
+    r = data[:, 0].astype(np.int64) - 1
+    c = data[:, 1].astype(np.int64) - 1
+    v = data[:, 2] if data.shape[1] > 2 else np.ones(len(r))
+    if "symmetric" in header:
+        off = r != c
+        r, c, v = np.concatenate([r, c[off]]), np.concatenate([c, r[off]]), np.concatenate([v, v[off]])
+    d = np.zeros((n, m))
+    np.add.at(d, (r, c), v)
+    return gen.dense_to_csr(d)
+
+
+@pytest.mark.parametrize("name,method", [("ani1.mtx", "cg"), ("1138_bus.mtx", "cg"),
+                                         ("ani1_nonsymm.mtx", "bicgstab")])
+def test_reference_matrices_solve(name, method):
+    import ginkgo_amd.gko as gko
+    rp, ci, v = read_mtx(os.path.join(HERE, "golden", name))
+    n = len(rp) - 1
+    b_h = ora.csr_spmv(rp, ci, v, np.ones((n, 1)))
+    spec = [{"kind": "iteration", "max_iters": 10000},
+            {"kind": "residual_norm", "reduction_factor": 1e-10}]
+    solve = ora.cg_solve if method == "cg" else ora.bicgstab_solve
+    xo, so, io = solve(rp, ci, v, b_h, np.zeros_like(b_h), spec)
+    exec_ = gko.Mi355xExecutor.create(0, torch.cuda.current_stream().cuda_stream)
+    A = gko.csr_from_host(exec_, rp, ci, v)
+    build = gko.solver.Cg.build if method == "cg" else gko.solver.Bicgstab.build
+    s = (build("f64").with_criteria(gko.stop.Iteration.build().with_max_iters(10000),
+                                    gko.stop.ResidualNorm.build("f64").with_reduction_factor(1e-10))
+         .on(exec_).generate(A))
+    x = gko.matrix.Dense("f64").create(exec_, n, 1)
+    x.fill(0.0)
+    s.apply(gko.matrix.Dense("f64").from_numpy(exec_, b_h), x)
+    xh = x.to_numpy()
+    res = np.linalg.norm(b_h - gen.csr_to_dense(rp, ci, v) @ xh) / np.linalg.norm(b_h)
+    assert res <= 2e-10
+    # iteration counts of the same recurrence; 1138_bus is ill-conditioned, so
+    # rounding differences move the count more there
+    band = 3 if name != "1138_bus.mtx" else max(5, io.iterations // 20)
+    assert abs(s.last_iterations - io.iterations) <= band, (s.last_iterations, io.iterations)
+
+
+def test_simple_solver_harness(tmp_path):
+    exe = os.path.join(ROOT, "ginkgo_amd", "lib", "simple_solver")
+    assert os.path.exists(exe), "examples are built by __graft_entry__.build()"
+    rp, ci, v = gen.poisson2d(48)
+    n = len(rp) - 1
+    mtx = tmp_path / "poisson48.mtx"
+    with open(mtx, "w") as f:
+        f.write("%%MatrixMarket matrix coordinate real general\n")
+        f.write(f"{n} {n} {len(ci)}\n")
+        rows = np.repeat(np.arange(n), np.diff(rp))
+        for r, c, val in zip(rows, ci, v):
+            f.write(f"{r + 1} {c + 1} {val:.17g}\n")
+    csv = tmp_path / "out.csv"
+    out = subprocess.run([exe, str(mtx), str(csv)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    path, rows_s, nnz_s, ms, iters = open(csv).read().strip().split(",")
+    assert int(rows_s) == n and int(nnz_s) == len(ci)
+    b = ora.csr_spmv(rp, ci, v, np.ones((n, 1)))
+    _, _, io = ora.cg_solve(rp, ci, v, b, np.zeros_like(b),
+                            [{"kind": "iteration", "max_iters": 10000},
+                             {"kind": "residual_norm", "reduction_factor": 1e-10}])
+    assert abs(int(iters) - io.iterations) <= 3
+    assert float(ms) > 0
