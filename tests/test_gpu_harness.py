@@ -58,7 +58,9 @@ def test_reference_matrices_solve(name, method):
     s.apply(gko.matrix.Dense("f64").from_numpy(exec_, b_h), x)
     xh = x.to_numpy()
     res = np.linalg.norm(b_h - gen.csr_to_dense(rp, ci, v) @ xh) / np.linalg.norm(b_h)
-    assert res <= 2e-10
+    # the stopping test reads the recurrence residual; on 1138_bus (cond ~ 8.6e6)
+    # the true residual drifts away from it by rounding
+    assert res <= (2e-10 if name != "1138_bus.mtx" else 1e-8)
     # iteration counts of the same recurrence; 1138_bus is ill-conditioned, so
     # rounding differences move the count more there
     band = 3 if name != "1138_bus.mtx" else max(5, io.iterations // 20)
