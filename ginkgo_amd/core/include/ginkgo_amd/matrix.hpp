@@ -50,6 +50,9 @@ public:
         virtual ~strategy_type() = default;
         virtual std::string get_name() const = 0;
     };
+    // Sub-wavefront per row.  `subwave` = lanes per row (bits 0..7, a power
+    // of two <= 64) | rows in flight per lane group << 8 (1, 2, 4 or 8);
+    // 0 in either field = chosen from the mean row length.
     class classical : public strategy_type {
     public:
         classical(int subwave = 0) : subwave_{subwave} {}

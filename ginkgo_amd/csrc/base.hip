@@ -145,6 +145,28 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// Contiguous copy in 16-byte units, four independent units in flight per
+// thread (the Dense::copy_from case with packed rows, and the stream-copy
+// bandwidth probe).  `n16` units; the tail below 16 bytes is copied by the
+// caller's element kernel.
+__global__ void __launch_bounds__(kBlock)
+    copy16_kernel(int64_t n16, const uint4* __restrict__ x, uint4* __restrict__ y)
+{
+    constexpr int kUnroll = 4;
+    const int64_t threads = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    const int64_t t = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x +
+                      threadIdx.x;
+    int64_t i = t;
+    for (; i + (kUnroll - 1) * threads < n16; i += kUnroll * threads) {
+        uint4 v[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) v[u] = x[i + u * threads];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) y[i + u * threads] = v[u];
+    }
+    for (; i < n16; i += threads) y[i] = x[i];
+}
+
 template <typename T>
 int fill_impl(const gkomi_ctx* ctx, int64_t n, T value, T* ptr)
 {
@@ -182,6 +204,23 @@ int dense_ew_impl(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const T* alpha,
                   "dense op: null pointer");
     GKOMI_REQUIRE(Op == ew::copy || (alpha != nullptr && (alpha_len == 1 || alpha_len == nrhs)),
                   GKOMI_DIMENSION_MISMATCH, "dense op: alpha must have 1 or nrhs entries");
+    if constexpr (Op == ew::copy) {
+        const int64_t bytes = n * nrhs * static_cast<int64_t>(sizeof(T));
+        const bool packed = xs == nrhs && ys == nrhs;
+        const bool aligned = (reinterpret_cast<uintptr_t>(x) % 16 == 0) &&
+                             (reinterpret_cast<uintptr_t>(y) % 16 == 0);
+        if (packed && aligned && bytes >= 16) {
+            const int64_t n16 = bytes / 16;
+            copy16_kernel<<<grid_for(n16, kBlock * 4, 16), kBlock, 0, stream_of(ctx)>>>(
+                n16, reinterpret_cast<const uint4*>(x), reinterpret_cast<uint4*>(y));
+            GKOMI_CHECK_LAUNCH("copy16_kernel");
+            const int64_t done = n16 * 16 / static_cast<int64_t>(sizeof(T));
+            if (done == n * nrhs) return GKOMI_SUCCESS;
+            // remaining elements (fewer than 16 bytes) as one packed row
+            return dense_ew_impl<Op, T>(ctx, n * nrhs - done, 1, nullptr, 0, x + done, 1,
+                                        y + done, 1);
+        }
+    }
     dense_ew_kernel<Op, T><<<grid_for(n * nrhs, kBlock * 4), kBlock, 0, stream_of(ctx)>>>(
         n, nrhs, alpha, alpha_len, x, xs, y, ys);
     GKOMI_CHECK_LAUNCH("dense_ew_kernel");

@@ -2,9 +2,10 @@
 // longest row) and the scalar Jacobi generate / apply.
 //
 // Kernels:
-//  * csr_spmv_subwave<SW>: SW lanes of a wave64 per row, DPP / ds_swizzle
-//    group reduction; for rows of (nearly) uniform short length such as
-//    stencils.  Rows are walked in XCD-contiguous ranges.
+//  * csr_spmv_subwave<SW, RPG>: SW lanes of a wave64 per row, each lane group
+//    holding RPG rows in flight, DPP / ds_swizzle group reduction; for rows
+//    of (nearly) uniform short length such as stencils.  Rows are walked in
+//    XCD-contiguous ranges.
 //  * csr_spmv_balanced: workgroups own contiguous nnz ranges of the
 //    partition `srow`; the block's row pointers and the products a_ij * x_j
 //    are staged in LDS, then rows are reduced thread-per-row (short) or
@@ -12,36 +13,107 @@
 //    wave-per-row from global memory.  Fixed summation order: deterministic.
 //  * csr_spmm_rowwave: one wave per row, lanes over right-hand sides (the
 //    many-RHS case).
+#include <algorithm>
+#include <cmath>
+
 #include "common.hpp"
 #include "spmv_common.hpp"
 
 namespace gkomi {
 
-template <typename T, int SW, bool Adv>
+// Each group of SW lanes owns RPG rows at once (rows g, g + 64/SW, ... of the
+// wave's contiguous range of 64/SW * RPG rows).  The body is branch-free:
+// lanes past the end of a row load entry 0 (valid whenever a row is active)
+// and add a zero product, so the compiler issues the loads of all RPG rows
+// back to back and waits once -- RPG times the bytes in flight per wave,
+// which is what a short-row SpMV needs to cover HBM latency.  Per row the
+// summation order is fixed (lane l adds entries l, l + SW, ... by fused
+// multiply-add; then the group tree), so results do not depend on RPG or
+// the launch.
+template <typename T, int SW, int RPG, bool Adv, bool Strided>
+__device__ __forceinline__ void csr_subwave_body(int64_t n_rows, const int32_t* __restrict__ rp,
+                                                 const int32_t* __restrict__ ci,
+                                                 const T* __restrict__ v,
+                                                 const T* __restrict__ b, int64_t bs,
+                                                 T* __restrict__ c, int64_t cs,
+                                                 const T* __restrict__ alpha,
+                                                 const T* __restrict__ beta)
+{
+    constexpr int kGroups = kWave / SW;
+    constexpr int kRowsPerWave = kGroups * RPG;
+    constexpr int64_t kRowsPerBlock = int64_t{kRowsPerWave} * kWavesPerBlock;
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    const int g = lane / SW;
+    const int l = lane % SW;
+    const int64_t nb = static_cast<int64_t>(gridDim.x);
+    const int64_t row_blocks = ceildiv(n_rows, kRowsPerBlock);
+    for (int64_t blk = xcd_block(blockIdx.x, gridDim.x); blk < row_blocks; blk += nb) {
+        const int64_t row0 = blk * kRowsPerBlock + wave * kRowsPerWave + g;
+        int32_t pos[RPG];
+        int32_t end[RPG];
+        T sum[RPG];
+        int32_t iters = 0;
+        // rows past the end read rp[n_rows] twice: empty
+#pragma unroll
+        for (int k = 0; k < RPG; ++k) {
+            const int64_t row = row0 + k * kGroups;
+            const int64_t rc = row < n_rows ? row : n_rows;
+            pos[k] = rp[rc];
+            end[k] = rp[row < n_rows ? row + 1 : n_rows];
+        }
+#pragma unroll
+        for (int k = 0; k < RPG; ++k) {
+            pos[k] += l;
+            sum[k] = 0;
+            const int32_t len = end[k] - pos[k];
+            const int32_t it = len > 0 ? (len + SW - 1) / SW : 0;
+            iters = it > iters ? it : iters;
+        }
+        for (int32_t j = 0; j < iters; ++j) {
+            int32_t col[RPG];
+            T val[RPG];
+            bool act[RPG];
+#pragma unroll
+            for (int k = 0; k < RPG; ++k) {
+                act[k] = pos[k] < end[k];
+                const int32_t p = act[k] ? pos[k] : 0;
+                col[k] = ci[p];
+                val[k] = v[p];
+                pos[k] = act[k] ? pos[k] + SW : pos[k];
+            }
+            T xv[RPG];
+#pragma unroll
+            for (int k = 0; k < RPG; ++k) {
+                xv[k] = Strided ? b[static_cast<int64_t>(col[k]) * bs] : b[col[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < RPG; ++k) {
+                sum[k] = fma(act[k] ? val[k] : T(0), act[k] ? xv[k] : T(0), sum[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RPG; ++k) {
+            sum[k] = group_sum<SW>(sum[k]);
+            const int64_t row = row0 + k * kGroups;
+            if (row < n_rows && l == 0) {
+                store_result<Adv>(c + row * cs, sum[k], alpha, beta);
+            }
+        }
+    }
+}
+
+template <typename T, int SW, int RPG, bool Adv>
 __global__ void __launch_bounds__(kBlock)
     csr_spmv_subwave(int64_t n_rows, const int32_t* __restrict__ rp,
                      const int32_t* __restrict__ ci, const T* __restrict__ v,
                      const T* __restrict__ b, int64_t bs, T* __restrict__ c, int64_t cs,
                      const T* __restrict__ alpha, const T* __restrict__ beta)
 {
-    constexpr int kRowsPerBlock = kBlock / SW;
-    const int g = threadIdx.x / SW;
-    const int l = threadIdx.x % SW;
-    const int64_t nb = static_cast<int64_t>(gridDim.x);
-    const int64_t row_blocks = ceildiv(n_rows, kRowsPerBlock);
-    for (int64_t blk = xcd_block(blockIdx.x, gridDim.x); blk < row_blocks; blk += nb) {
-        const int64_t row = blk * kRowsPerBlock + g;
-        T sum = 0;
-        if (row < n_rows) {
-            const int32_t end = rp[row + 1];
-            for (int32_t k = rp[row] + l; k < end; k += SW) {
-                sum += v[k] * b[static_cast<int64_t>(ci[k]) * bs];
-            }
-        }
-        sum = group_sum<SW>(sum);
-        if (row < n_rows && l == 0) {
-            store_result<Adv>(c + row * cs, sum, alpha, beta);
-        }
+    if (bs == 1) {
+        csr_subwave_body<T, SW, RPG, Adv, false>(n_rows, rp, ci, v, b, bs, c, cs, alpha, beta);
+    } else {
+        csr_subwave_body<T, SW, RPG, Adv, true>(n_rows, rp, ci, v, b, bs, c, cs, alpha, beta);
     }
 }
 
@@ -255,15 +327,24 @@ int check_csr(const gkomi_csr* a)
     return GKOMI_SUCCESS;
 }
 
-int pick_subwave(const gkomi_csr* a)
+subwave_shape pick_subwave(const gkomi_csr* a, int value_bytes)
 {
-    if (a->subwave > 0) {
-        return a->subwave;
-    }
+    subwave_shape sh{a->subwave & 0xff, (a->subwave >> 8) & 0xff};
     const double mean = a->n_rows > 0 ? static_cast<double>(a->nnz) / a->n_rows : 1.0;
-    int sw = 1;
-    while (sw < 64 && sw < mean) sw *= 2;
-    return sw;
+    if (sh.sw == 0) {
+        // about two entries per lane and row
+        sh.sw = 1;
+        while (sh.sw < 64 && 2 * sh.sw < mean) sh.sw *= 2;
+    }
+    if (sh.rpg == 0) {
+        // four FP64 (eight FP32) entries in flight per lane; measured best
+        // on the 7-point Laplacian 128^3
+        const int target = value_bytes >= 8 ? 4 : 8;
+        const int iters = std::max(1, static_cast<int>(std::ceil(mean / sh.sw)));
+        sh.rpg = 1;
+        while (sh.rpg < 8 && 2 * sh.rpg * iters <= target) sh.rpg *= 2;
+    }
+    return sh;
 }
 
 template <typename T, bool Adv>
@@ -288,26 +369,29 @@ int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const 
         GKOMI_CHECK_LAUNCH("csr_spmv_balanced");
         return GKOMI_SUCCESS;
     }
-    const int sw = pick_subwave(a);
-    GKOMI_REQUIRE(sw == 1 || sw == 2 || sw == 4 || sw == 8 || sw == 16 || sw == 32 || sw == 64,
-                  GKOMI_INVALID_ARGUMENT, "CSR: subwave must be a power of two <= 64");
-    const int64_t rows_per_block = kBlock / sw;
+    const subwave_shape sh = pick_subwave(a, sizeof(T));
+    GKOMI_REQUIRE(valid_subwave(sh), GKOMI_INVALID_ARGUMENT,
+                  "CSR: subwave must be a power of two <= 64 and rows per group 1, 2, 4 or 8");
+    const int64_t rows_per_block = int64_t{kBlock} / sh.sw * sh.rpg;
     const unsigned g = grid_for(a->n_rows, rows_per_block, 64);
-#define GKOMI_SUBWAVE_CASE(S)                                                                 \
-    case S:                                                                                   \
-        csr_spmv_subwave<T, S, Adv><<<g, kBlock, 0, stream>>>(a->n_rows, a->row_ptrs,         \
-                                                              a->col_idxs, v, b, bs, c, cs,   \
-                                                              alpha, beta);                   \
+#define GKOMI_SUBWAVE_CASE(S, R)                                                              \
+    case S * 16 + R:                                                                          \
+        csr_spmv_subwave<T, S, R, Adv><<<g, kBlock, 0, stream>>>(a->n_rows, a->row_ptrs,      \
+                                                                 a->col_idxs, v, b, bs, c,    \
+                                                                 cs, alpha, beta);            \
         break;
-    switch (sw) {
-        GKOMI_SUBWAVE_CASE(1)
-        GKOMI_SUBWAVE_CASE(2)
-        GKOMI_SUBWAVE_CASE(4)
-        GKOMI_SUBWAVE_CASE(8)
-        GKOMI_SUBWAVE_CASE(16)
-        GKOMI_SUBWAVE_CASE(32)
-        GKOMI_SUBWAVE_CASE(64)
+#define GKOMI_SUBWAVE_ROW(S) \
+    GKOMI_SUBWAVE_CASE(S, 1) GKOMI_SUBWAVE_CASE(S, 2) GKOMI_SUBWAVE_CASE(S, 4) GKOMI_SUBWAVE_CASE(S, 8)
+    switch (sh.sw * 16 + sh.rpg) {
+        GKOMI_SUBWAVE_ROW(1)
+        GKOMI_SUBWAVE_ROW(2)
+        GKOMI_SUBWAVE_ROW(4)
+        GKOMI_SUBWAVE_ROW(8)
+        GKOMI_SUBWAVE_ROW(16)
+        GKOMI_SUBWAVE_ROW(32)
+        GKOMI_SUBWAVE_ROW(64)
     }
+#undef GKOMI_SUBWAVE_ROW
 #undef GKOMI_SUBWAVE_CASE
     GKOMI_CHECK_LAUNCH("csr_spmv_subwave");
     return GKOMI_SUCCESS;

@@ -506,7 +506,7 @@ int cg_fused_start_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_cr
                   "fused CG: matrix and vectors do not conform");
     GKOMI_REQUIRE(a->strategy == GKOMI_CSR_CLASSICAL, GKOMI_NOT_SUPPORTED,
                   "fused CG: needs the classical (sub-wave) CSR strategy");
-    const int sw = pick_subwave(a);
+    const int sw = pick_subwave(a, sizeof(T)).sw;
     hipStream_t s = stream_of(ctx);
     cg_work w = cg_work_layout(st->work);
     const T* v = static_cast<const T*>(a->values);
@@ -566,7 +566,7 @@ int cg_fused_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_
     GKOMI_REQUIRE(a->strategy == GKOMI_CSR_CLASSICAL, GKOMI_NOT_SUPPORTED,
                   "fused CG: needs the classical (sub-wave) CSR strategy");
     GKOMI_REQUIRE(count >= 0, GKOMI_INVALID_ARGUMENT, "fused CG: negative count");
-    const int sw = pick_subwave(a);
+    const int sw = pick_subwave(a, sizeof(T)).sw;
     hipStream_t s = stream_of(ctx);
     cg_work w = cg_work_layout(st->work);
     const T* v = static_cast<const T*>(a->values);

@@ -28,9 +28,21 @@ __device__ __forceinline__ void store_result(T* dst, T sum, const T* alpha, cons
 }
 
 int check_csr(const gkomi_csr* a);
-// lanes per row of the classical kernel: the descriptor's value, or the
-// smallest power of two >= the mean row length (capped at 64)
-int pick_subwave(const gkomi_csr* a);
+// Shape of the classical kernel: `sw` lanes per row, `rpg` rows in flight per
+// lane group.  gkomi_csr::subwave holds sw in bits 0..7 and rpg in bits 8..15
+// (0 = automatic).
+struct subwave_shape {
+    int sw;
+    int rpg;
+};
+
+subwave_shape pick_subwave(const gkomi_csr* a, int value_bytes);
+
+inline bool valid_subwave(subwave_shape sh)
+{
+    return sh.sw >= 1 && sh.sw <= 64 && (sh.sw & (sh.sw - 1)) == 0 &&
+           (sh.rpg == 1 || sh.rpg == 2 || sh.rpg == 4 || sh.rpg == 8);
+}
 
 template <typename T, bool Adv>
 int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const T* b, int64_t bs,

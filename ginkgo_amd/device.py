@@ -257,6 +257,16 @@ def dot(x, y):
     return out
 
 
+def copy(x, y):
+    """y = x for device tensors of equal shape (Dense::copy_from)."""
+    x2 = x.reshape(x.shape[0], -1)
+    y2 = y.reshape(y.shape[0], -1)
+    c = ctx(x.device)
+    C.call(f"gkomi_dense_copy_{suffix(x.dtype)}", ctypes.byref(c), x2.shape[0], x2.shape[1],
+           ptr(x2), x2.stride(0), ptr(y2), y2.stride(0))
+    return y
+
+
 def norm2(x):
     x2 = x.reshape(x.shape[0], -1)
     out = torch.empty(x2.shape[1], dtype=x.dtype, device=x.device)

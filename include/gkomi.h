@@ -153,8 +153,9 @@ int gkomi_dense_copy_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const fl
 /* ------------------------------------------------------------------ CSR */
 
 /* csr::strategy_type (include/ginkgo/core/matrix/csr.hpp:176-699).
- * CLASSICAL: a sub-wavefront of `subwave` lanes per row (0 = choose from the
- * mean row length).  LOAD_BALANCE: nnz-balanced workgroups over the
+ * CLASSICAL: a sub-wavefront of S lanes per row, each lane group keeping R
+ * rows in flight; `subwave` = S | (R << 8), S a power of two <= 64, R one of
+ * 1, 2, 4, 8, either 0 = chosen from the mean row length.  LOAD_BALANCE: nnz-balanced workgroups over the
  * partition `srow` built by gkomi_csr_lb_partition. */
 enum { GKOMI_CSR_CLASSICAL = 0, GKOMI_CSR_LOAD_BALANCE = 1 };
 

@@ -1,6 +1,6 @@
 """Differentiable solve: forward solve, adjoint backward on the device-built
 transpose, sampled outer product -- checked against central finite
-differences (h = 1e-6) and against a dense FP64 torch.linalg.solve gradient."""
+differences (h = 1e-4) and against a dense FP64 torch.linalg.solve gradient."""
 import numpy as np
 import pytest
 
@@ -69,10 +69,13 @@ def test_gradients_match_finite_differences_and_dense(dev, method, convection):
     np.testing.assert_allclose(x.detach().cpu().numpy(), xd.detach().numpy(), rtol=0, atol=1e-10)
     assert np.max(np.abs(g - gd)) <= 1e-8 * np.max(np.abs(gd))
 
-    # central finite differences on random theta entries, h = 1e-6
+    # central finite differences on random theta entries.  h = 1e-4 balances
+    # the O(h^2) truncation error (about 3e-8 relative here, measured with a
+    # dense solve) against the rounding noise of L divided by 2h, which at
+    # h = 1e-6 already reaches 4e-6 relative with an exact dense solve.
     rng = np.random.default_rng(8)
     idx = rng.choice(theta0.numel(), 12, replace=False)
-    h = 1e-6
+    h = 1e-4
     for i in idx:
         tp = theta0.clone()
         tm = theta0.clone()

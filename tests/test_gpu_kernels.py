@@ -96,6 +96,31 @@ def test_spmv_classical_every_subwave(dev, dtype, subwave):
     A = D().DeviceCsr(rp, ci, v, strategy="classical", subwave=subwave, device=dev)
     y = A.spmv(to_dev(x, dev)).cpu().numpy()
     check_spmv(rp, ci, v, x, y, dtype)
+    # rows in flight per lane group (subwave bits 8..15) change the schedule,
+    # never the per-row summation order: bit-identical results
+    for rpg in (1, 2, 4, 8):
+        A2 = D().DeviceCsr(rp, ci, v, strategy="classical", subwave=subwave | (rpg << 8),
+                           device=dev)
+        y2 = A2.spmv(to_dev(x, dev)).cpu().numpy()
+        np.testing.assert_array_equal(y2, y)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 1000, 4099, 100003])
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_dense_copy(dev, dtype, n):
+    # packed (16-byte vector path plus element tail), offset by one element
+    # (element path), and strided
+    src = torch.as_tensor(np.random.default_rng(n).uniform(-1, 1, n + 3).astype(dtype)).to(dev)
+    for off in (0, 1):
+        dst = torch.full((n + 3,), 7.0, dtype=TD[dtype], device=dev)
+        D().copy(src[off:off + n], dst[off:off + n])
+        assert torch.equal(dst[off:off + n], src[off:off + n])
+        assert torch.all(dst[:off] == 7.0) and torch.all(dst[off + n:] == 7.0)
+    if n >= 2:
+        a = src[: (n // 2) * 2].reshape(n // 2, 2)
+        b = torch.zeros(n // 2, 3, dtype=TD[dtype], device=dev)
+        D().copy(a, b[:, :2])
+        assert torch.equal(b[:, :2], a) and torch.all(b[:, 2] == 0)
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
