@@ -58,18 +58,22 @@ def time_kernel_loop(fn, steps, warmup, stream):
 
 
 def stream_copy_gbps(dev):
+    """Device copy of 1 GiB by the library's 16-byte vector copy kernel
+    (read + write counted), and torch's copy for comparison."""
     import torch
+    from ginkgo_amd import device as D
     n = 1 << 27  # 1 GiB of doubles
     a = torch.empty(n, dtype=torch.float64, device=dev)
     a.fill_(1.0)
     b = torch.empty_like(a)
     s = torch.cuda.current_stream(dev)
-    _, ms = time_kernel_loop(lambda: b.copy_(a), 20, 5, s)
+    _, ms_own = time_kernel_loop(lambda: D.copy(a, b), 20, 5, s)
+    _, ms_torch = time_kernel_loop(lambda: b.copy_(a), 20, 5, s)
     del a, b
-    return 2 * n * 8 / (ms * 1e-3) / 1e9
+    return 2 * n * 8 / (ms_own * 1e-3) / 1e9, 2 * n * 8 / (ms_torch * 1e-3) / 1e9
 
 
-def cpu_spmv_baseline(rp, ci, v, x, budget_s=12.0):
+def cpu_spmv_baseline(rp, ci, v, x, budget_s=10.0):
     """Oracle OpenMP SpMV on the host (the CPU baseline, not a target)."""
     import numpy as np
     import oracle as ora
@@ -80,7 +84,7 @@ def cpu_spmv_baseline(rp, ci, v, x, budget_s=12.0):
         ora.csr_spmv(rp, ci, v, x, omp=True)
         reps += 1
         el = time.perf_counter() - t0
-        if el > budget_s or reps >= 200:
+        if el > budget_s:
             break
     per = el / reps
     n = len(rp) - 1
@@ -205,11 +209,20 @@ def bench_batch(dev, num=20000):
     return out, (row_off, rp, ci, v, b)
 
 
-def bench_adjoint(dev, nx=512, nrhs=64):
+def bench_adjoint(dev, nx=512, nrhs=64, fd_entries=32, h=1e-6):
     """512^2 anisotropic diffusion, 64 right-hand sides: forward FP64 CG to
-    1e-12 plus the adjoint backward (A^T solve + sampled outer product)."""
+    1e-12 plus the adjoint backward (A^T solve + sampled outer product),
+    timed; then the gradient checked against central differences of the loss
+    on `fd_entries` random conductivities with step h.
+
+    The central difference [L(th + h e_i) - L(th - h e_i)] / 2h is evaluated
+    without cancellation: with x0 the base solution and A_pm = A(th +- h e_i),
+    x(th +- h e_i) = x0 + d_pm where A_pm d_pm = -(A_pm - A_0) x0 (exact), and
+    L_+ - L_- = 0.5 sum (d_+ - d_-)(2 x0 + d_+ + d_- - 2 x_t).  Subtracting two
+    separately solved losses instead loses ~eps |L| / h to rounding."""
     import numpy as np
     import torch
+    from ginkgo_amd import device as D
     from ginkgo_amd import generators as gen
     from ginkgo_amd.autograd import sparse_solve
     log("[bench] adjoint: generating 512^2 anisotropic diffusion")
@@ -217,25 +230,60 @@ def bench_adjoint(dev, nx=512, nrhs=64):
     n, nnz = nx * nx, len(ci)
     vmap_t = torch.as_tensor(vmap).to(dev)
     coef_t = torch.as_tensor(coef).to(dev)
-    th = torch.as_tensor(theta).to(dev).requires_grad_(True)
+    th0 = torch.as_tensor(theta).to(dev)
+    th = th0.clone().requires_grad_(True)
     rng = np.random.default_rng(7)
     b = torch.as_tensor(rng.uniform(-1, 1, (n, nrhs))).to(dev)
     xt = torch.as_tensor(rng.uniform(-1, 1, (n, nrhs))).to(dev)
     rp_t = torch.as_tensor(rp).to(dev)
     ci_t = torch.as_tensor(ci).to(dev)
 
+    def values_of(t):
+        return torch.zeros(nnz, dtype=torch.float64, device=dev).index_add(
+            0, vmap_t[:, 0], coef_t * t[vmap_t[:, 1]])
+
     def run():
         th.grad = None
-        v = torch.zeros(nnz, dtype=torch.float64, device=dev).index_add(
-            0, vmap_t[:, 0], coef_t * th[vmap_t[:, 1]])
-        x = sparse_solve(v, b, rp_t, ci_t, method="cg", tol=1e-12, max_iters=20000)
+        x = sparse_solve(values_of(th), b, rp_t, ci_t, method="cg", tol=1e-12, max_iters=20000)
         loss = 0.5 * ((x - xt) ** 2).sum()
         loss.backward()
-        return loss
-    t, loss = timed(run)
-    t2, loss = timed(run)
-    return {"time_s_forward_backward": round(min(t, t2), 4), "n": n, "nnz": nnz, "nrhs": nrhs,
-            "loss": float(loss.item())}
+        return loss, x
+    t, _ = timed(run)
+    log(f"[bench] adjoint: first forward + backward {t:.2f} s")
+    t2, (loss, x0) = timed(run)
+    g = th.grad.detach().clone()
+    out = {"time_s_forward_backward": round(min(t, t2), 4), "n": n, "nnz": nnz, "nrhs": nrhs,
+           "loss": float(loss.item())}
+    if fd_entries > 0:
+        log(f"[bench] adjoint: central differences on {fd_entries} entries, h = {h}")
+        x0 = x0.detach()
+        v0 = values_of(th0)
+        idx = np.random.default_rng(11).choice(th0.numel(), fd_entries, replace=False)
+        t0 = time.perf_counter()
+        fd = []
+        for i in idx:
+            ds = []
+            for sgn in (1.0, -1.0):
+                tp = th0.clone()
+                tp[i] += sgn * h
+                vp = values_of(tp)
+                dA = D.DeviceCsr(rp_t, ci_t, vp - v0, strategy="classical", device=dev)
+                rhs = -dA.spmv(x0)
+                ds.append(sparse_solve(vp, rhs, rp_t, ci_t, method="cg", tol=1e-12,
+                                       max_iters=20000))
+            dp, dm = ds
+            fd.append((0.5 * ((dp - dm) * (2 * x0 + dp + dm - 2 * xt)).sum() / (2 * h)).item())
+            log(f"[bench] adjoint: fd entry {len(fd)}/{fd_entries} "
+                f"({time.perf_counter() - t0:.1f} s)")
+        fd = np.array(fd)
+        gi = g[torch.as_tensor(idx, device=dev)].cpu().numpy()
+        gmax = float(g.abs().max().item())
+        out["fd_check"] = {
+            "entries": int(fd_entries), "h": h,
+            "max_rel_err": float(np.max(np.abs(fd - gi) / np.abs(gi))),
+            "max_err_over_max_grad": float(np.max(np.abs(fd - gi)) / gmax),
+            "time_s": round(time.perf_counter() - t0, 2)}
+    return out
 
 
 def main():
@@ -245,6 +293,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="skip secondary measurements")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--only", action="append", default=None,
+                    choices=["spmv", "cg_ir", "batch", "adjoint"],
+                    help="run only these secondary sections (repeatable)")
+    ap.add_argument("--fd-entries", type=int, default=32,
+                    help="conductivities checked by central differences in the adjoint section")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1 "
@@ -283,7 +336,7 @@ def main():
     value = b64 / (wall_ms * 1e-3) / 1e9
     achieved = b64 / (ev_ms * 1e-3) / 1e9
 
-    stream_bw = stream_copy_gbps(dev)
+    stream_bw, torch_copy_bw = stream_copy_gbps(dev)
     result = {
         "metric": "SpMV HBM GB/s (FP64 CSR, 3D 7-pt Laplacian 128^3)",
         "value": round(value, 2),
@@ -304,11 +357,13 @@ def main():
                      "traffic": None, "kernel_avg_us": round(ev_ms * 1e3, 3),
                      "algorithmic_bytes": b64},
         "stream_copy_GBps": round(stream_bw, 1),
+        "torch_copy_GBps": round(torch_copy_bw, 1),
         "frac_of_stream_copy": round(achieved / stream_bw, 4),
     }
 
     secondary = {}
-    if not args.quick:
+    want = (lambda name: not args.quick and (args.only is None or name in args.only))
+    if want("spmv"):
         log("[bench] secondary: FP32 CSR, SELL-P, power-law")
         A32 = D.DeviceCsr(rp, ci, v.astype(np.float32), strategy="classical", device=dev)
         x32 = x.float()
@@ -328,15 +383,20 @@ def main():
         secondary["csr_f64_powerlaw1M_GBps"] = round(spmv_bytes(pn, pnnz, 8) / (ms * 1e-3) / 1e9, 1)
         secondary["powerlaw_nnz"] = pnnz
         del P
-    if not args.quick:
-        for name, fn in (("cg_vs_ir_96cubed", lambda: bench_cg_vs_ir(dev)[0]),
-                         ("batched_bicgstab_20000", lambda: bench_batch(dev)[0]),
-                         ("adjoint_512sq_64rhs", lambda: bench_adjoint(dev))):
-            try:
-                secondary[name] = fn()
-            except Exception as e:  # a secondary failure must not hide the headline
-                secondary[name] = {"error": f"{type(e).__name__}: {e}"}
-            torch.cuda.empty_cache()
+    for key, name, fn in (
+            ("cg_ir", "cg_vs_ir_96cubed", lambda: bench_cg_vs_ir(dev)[0]),
+            ("batch", "batched_bicgstab_20000", lambda: bench_batch(dev)[0]),
+            ("adjoint", "adjoint_512sq_64rhs",
+             lambda: bench_adjoint(dev, fd_entries=args.fd_entries))):
+        if not want(key):
+            continue
+        t0 = time.perf_counter()
+        try:
+            secondary[name] = fn()
+        except Exception as e:  # a secondary failure must not hide the headline
+            secondary[name] = {"error": f"{type(e).__name__}: {e}"}
+        log(f"[bench] {name}: {time.perf_counter() - t0:.1f} s")
+        torch.cuda.empty_cache()
     if secondary:
         result["secondary"] = secondary
 

@@ -122,7 +122,8 @@ class CgState(ctypes.Structure):
                 ("p_b", ctypes.c_void_p), ("q", ctypes.c_void_p),
                 ("inv_diag", ctypes.c_void_p), ("scalars", ctypes.c_void_p),
                 ("orig_tau", ctypes.c_void_p), ("status", ctypes.c_void_p),
-                ("flags", ctypes.c_void_p), ("work", ctypes.c_void_p)]
+                ("flags", ctypes.c_void_p), ("work", ctypes.c_void_p),
+                ("issued", ctypes.c_int64)]
 
 
 BATCH_MAX_BINS = 8

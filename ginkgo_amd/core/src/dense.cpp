@@ -34,10 +34,9 @@ struct ops<float> {
 template <typename V>
 char* Dense<V>::reduction_workspace(size_type bytes) const
 {
-    // zero-filled once; the reduction kernels leave it zeroed after each use
+    // partials of the reduction kernels; kept per vector to avoid reallocation
     if (workspace_.get_size() < bytes) {
         workspace_ = array<char>(this->get_executor(), bytes);
-        GKO_CALL(gkomi_fill_zero(ctx_of(this->get_executor()), workspace_.get_data(), bytes));
     }
     return workspace_.get_data();
 }

@@ -299,7 +299,6 @@ bool Cg<V>::try_fused(const Vec<V>* b, Vec<V>* x) const
     array<char> flags(exec, sizeof(gkomi_stop_flags));
     const size_type wb = gkomi_cg_work_bytes(n);
     array<char> work(exec, wb);
-    GKO_CALL(gkomi_fill_zero(ctx_of(exec), work.get_data(), wb));
     gkomi_cg_state st{};
     st.n = static_cast<int64_t>(n);
     st.x = x->get_values();

@@ -241,10 +241,9 @@ int reduce_impl(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const T* x, int64
                   "reduction: stride smaller than the number of columns");
     GKOMI_REQUIRE(ws != nullptr && ws_bytes >= gkomi_reduction_workspace_bytes(n, nrhs),
                   GKOMI_INVALID_ARGUMENT, "reduction: workspace too small");
-    return launch_column_reduction<T, Norm>(stream_of(ctx), n, nrhs, x, xs, y, ys, result,
-                                            static_cast<unsigned*>(ws),
-                                            reinterpret_cast<double*>(
-                                                static_cast<char*>(ws) + kTicketBytes));
+    return launch_column_reduction<T, Norm>(
+        stream_of(ctx), n, nrhs, x, xs, y, ys, result,
+        reinterpret_cast<double*>(static_cast<char*>(ws) + kTicketBytes));
 }
 
 }  // namespace gkomi

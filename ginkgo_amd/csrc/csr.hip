@@ -21,15 +21,6 @@
 
 namespace gkomi {
 
-// Each group of SW lanes owns RPG rows at once (rows g, g + 64/SW, ... of the
-// wave's contiguous range of 64/SW * RPG rows).  The body is branch-free:
-// lanes past the end of a row load entry 0 (valid whenever a row is active)
-// and add a zero product, so the compiler issues the loads of all RPG rows
-// back to back and waits once -- RPG times the bytes in flight per wave,
-// which is what a short-row SpMV needs to cover HBM latency.  Per row the
-// summation order is fixed (lane l adds entries l, l + SW, ... by fused
-// multiply-add; then the group tree), so results do not depend on RPG or
-// the launch.
 template <typename T, int SW, int RPG, bool Adv, bool Strided>
 __device__ __forceinline__ void csr_subwave_body(int64_t n_rows, const int32_t* __restrict__ rp,
                                                  const int32_t* __restrict__ ci,
@@ -39,67 +30,13 @@ __device__ __forceinline__ void csr_subwave_body(int64_t n_rows, const int32_t* 
                                                  const T* __restrict__ alpha,
                                                  const T* __restrict__ beta)
 {
-    constexpr int kGroups = kWave / SW;
-    constexpr int kRowsPerWave = kGroups * RPG;
-    constexpr int64_t kRowsPerBlock = int64_t{kRowsPerWave} * kWavesPerBlock;
-    const int lane = threadIdx.x % kWave;
-    const int wave = threadIdx.x / kWave;
-    const int g = lane / SW;
-    const int l = lane % SW;
     const int64_t nb = static_cast<int64_t>(gridDim.x);
-    const int64_t row_blocks = ceildiv(n_rows, kRowsPerBlock);
-    for (int64_t blk = xcd_block(blockIdx.x, gridDim.x); blk < row_blocks; blk += nb) {
-        const int64_t row0 = blk * kRowsPerBlock + wave * kRowsPerWave + g;
-        int32_t pos[RPG];
-        int32_t end[RPG];
-        T sum[RPG];
-        int32_t iters = 0;
-        // rows past the end read rp[n_rows] twice: empty
-#pragma unroll
-        for (int k = 0; k < RPG; ++k) {
-            const int64_t row = row0 + k * kGroups;
-            const int64_t rc = row < n_rows ? row : n_rows;
-            pos[k] = rp[rc];
-            end[k] = rp[row < n_rows ? row + 1 : n_rows];
-        }
-#pragma unroll
-        for (int k = 0; k < RPG; ++k) {
-            pos[k] += l;
-            sum[k] = 0;
-            const int32_t len = end[k] - pos[k];
-            const int32_t it = len > 0 ? (len + SW - 1) / SW : 0;
-            iters = it > iters ? it : iters;
-        }
-        for (int32_t j = 0; j < iters; ++j) {
-            int32_t col[RPG];
-            T val[RPG];
-            bool act[RPG];
-#pragma unroll
-            for (int k = 0; k < RPG; ++k) {
-                act[k] = pos[k] < end[k];
-                const int32_t p = act[k] ? pos[k] : 0;
-                col[k] = ci[p];
-                val[k] = v[p];
-                pos[k] = act[k] ? pos[k] + SW : pos[k];
-            }
-            T xv[RPG];
-#pragma unroll
-            for (int k = 0; k < RPG; ++k) {
-                xv[k] = Strided ? b[static_cast<int64_t>(col[k]) * bs] : b[col[k]];
-            }
-#pragma unroll
-            for (int k = 0; k < RPG; ++k) {
-                sum[k] = fma(act[k] ? val[k] : T(0), act[k] ? xv[k] : T(0), sum[k]);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < RPG; ++k) {
-            sum[k] = group_sum<SW>(sum[k]);
-            const int64_t row = row0 + k * kGroups;
-            if (row < n_rows && l == 0) {
-                store_result<Adv>(c + row * cs, sum[k], alpha, beta);
-            }
-        }
+    const int64_t tiles = ceildiv(n_rows, subwave_tile_rows<SW, RPG>);
+    for (int64_t t = xcd_block(blockIdx.x, gridDim.x); t < tiles; t += nb) {
+        subwave_tile<T, SW, RPG>(
+            n_rows, t, rp, ci, v,
+            [&](int32_t col) { return Strided ? b[static_cast<int64_t>(col) * bs] : b[col]; },
+            [&](int64_t row, T sum) { store_result<Adv>(c + row * cs, sum, alpha, beta); });
     }
 }
 
@@ -337,9 +274,10 @@ subwave_shape pick_subwave(const gkomi_csr* a, int value_bytes)
         while (sh.sw < 64 && 2 * sh.sw < mean) sh.sw *= 2;
     }
     if (sh.rpg == 0) {
-        // four FP64 (eight FP32) entries in flight per lane; measured best
-        // on the 7-point Laplacian 128^3
-        const int target = value_bytes >= 8 ? 4 : 8;
+        // entries in flight per lane: eight, except four for FP64 rows of at
+        // most eight entries (measured best on the 7-point Laplacian 128^3
+        // and the 27-point diffusion 96^3, FP64 and FP32)
+        const int target = (value_bytes >= 8 && sh.sw <= 4) ? 4 : 8;
         const int iters = std::max(1, static_cast<int>(std::ceil(mean / sh.sw)));
         sh.rpg = 1;
         while (sh.rpg < 8 && 2 * sh.rpg * iters <= target) sh.rpg *= 2;
@@ -370,29 +308,14 @@ int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const 
         return GKOMI_SUCCESS;
     }
     const subwave_shape sh = pick_subwave(a, sizeof(T));
-    GKOMI_REQUIRE(valid_subwave(sh), GKOMI_INVALID_ARGUMENT,
-                  "CSR: subwave must be a power of two <= 64 and rows per group 1, 2, 4 or 8");
-    const int64_t rows_per_block = int64_t{kBlock} / sh.sw * sh.rpg;
-    const unsigned g = grid_for(a->n_rows, rows_per_block, 64);
-#define GKOMI_SUBWAVE_CASE(S, R)                                                              \
-    case S * 16 + R:                                                                          \
-        csr_spmv_subwave<T, S, R, Adv><<<g, kBlock, 0, stream>>>(a->n_rows, a->row_ptrs,      \
-                                                                 a->col_idxs, v, b, bs, c,    \
-                                                                 cs, alpha, beta);            \
-        break;
-#define GKOMI_SUBWAVE_ROW(S) \
-    GKOMI_SUBWAVE_CASE(S, 1) GKOMI_SUBWAVE_CASE(S, 2) GKOMI_SUBWAVE_CASE(S, 4) GKOMI_SUBWAVE_CASE(S, 8)
-    switch (sh.sw * 16 + sh.rpg) {
-        GKOMI_SUBWAVE_ROW(1)
-        GKOMI_SUBWAVE_ROW(2)
-        GKOMI_SUBWAVE_ROW(4)
-        GKOMI_SUBWAVE_ROW(8)
-        GKOMI_SUBWAVE_ROW(16)
-        GKOMI_SUBWAVE_ROW(32)
-        GKOMI_SUBWAVE_ROW(64)
-    }
-#undef GKOMI_SUBWAVE_ROW
-#undef GKOMI_SUBWAVE_CASE
+    const int64_t tile_rows = int64_t{kBlock} / sh.sw * sh.rpg;
+    const unsigned g = grid_for(a->n_rows, tile_rows, 64);
+#define GKOMI_SPMV_CASE(S, R)                                                                 \
+    csr_spmv_subwave<T, S, R, Adv><<<g, kBlock, 0, stream>>>(                                 \
+        a->n_rows, a->row_ptrs, a->col_idxs, v, b, bs, c, cs, alpha, beta);                   \
+    break;
+    GKOMI_SUBWAVE_SWITCH(sh, GKOMI_SPMV_CASE)
+#undef GKOMI_SPMV_CASE
     GKOMI_CHECK_LAUNCH("csr_spmv_subwave");
     return GKOMI_SUCCESS;
 }

@@ -1,27 +1,28 @@
-// Deterministic single-pass column reductions (dot / squared norm).
+// Deterministic column reductions (dot / squared norm) in two kernels.
 //
 // Per-thread and per-wave accumulation happens in the value type (FP32 for
 // float operands), the per-workgroup partials and the final combine in FP64.
-// The last workgroup to finish (ticket counter) adds the partials in
-// workgroup order, so results do not depend on scheduling.
+// A second one-workgroup kernel adds the partials in workgroup order, so
+// results do not depend on scheduling.  No device-scope fence is used: on
+// gfx950 a release fence writes back the XCD's L2, and thousands of them
+// cost more than the reduction; the kernel boundary orders the two steps.
 #pragma once
 
 #include "common.hpp"
 
 namespace gkomi {
 
+// bytes at the start of a reduction workspace that are not used any more
+// (kept so that workspace sizes stay as documented)
 constexpr size_t kTicketBytes = 256;
 constexpr int kRedMaxBlocks = 2048;
 
 template <typename T, bool Norm>
 __global__ void __launch_bounds__(kBlock)
     reduce_one_column_kernel(int64_t n, const T* __restrict__ x, int64_t xs,
-                             const T* __restrict__ y, int64_t ys, T* __restrict__ result,
-                             unsigned* __restrict__ ticket, double* __restrict__ partials)
+                             const T* __restrict__ y, int64_t ys, double* __restrict__ partials)
 {
     __shared__ T lds_t[kWavesPerBlock];
-    __shared__ double lds_d[kWavesPerBlock];
-    __shared__ int flag;
     T acc = 0;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
@@ -33,27 +34,14 @@ __global__ void __launch_bounds__(kBlock)
     if (threadIdx.x == 0) {
         partials[blockIdx.x] = static_cast<double>(bsum);
     }
-    if (last_block_ticket(ticket, &flag)) {
-        double a = 0;
-        for (unsigned k = threadIdx.x; k < gridDim.x; k += kBlock) {
-            a += partials[k];
-        }
-        const double total = block_sum(a, lds_d);
-        if (threadIdx.x == 0) {
-            result[0] = static_cast<T>(Norm ? sqrt(total) : total);
-            *ticket = 0;
-        }
-    }
 }
 
 template <typename T, bool Norm>
 __global__ void __launch_bounds__(kBlock)
     reduce_columns_kernel(int64_t n, int64_t nrhs, const T* __restrict__ x, int64_t xs,
-                          const T* __restrict__ y, int64_t ys, T* __restrict__ result,
-                          unsigned* __restrict__ ticket, double* __restrict__ partials)
+                          const T* __restrict__ y, int64_t ys, double* __restrict__ partials)
 {
     __shared__ double lds[kWavesPerBlock][kWave];
-    __shared__ int flag;
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
     const int64_t rstride = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
@@ -77,50 +65,70 @@ __global__ void __launch_bounds__(kBlock)
         }
         __syncthreads();
     }
-    if (last_block_ticket(ticket, &flag)) {
-        for (int64_t c0 = 0; c0 < nrhs; c0 += kWave) {
-            const int64_t c = c0 + lane;
-            double acc = 0;
-            if (c < nrhs) {
-                for (unsigned k = wave; k < gridDim.x; k += kWavesPerBlock) {
-                    acc += partials[static_cast<int64_t>(k) * nrhs + c];
-                }
-            }
-            lds[wave][lane] = acc;
-            __syncthreads();
-            if (wave == 0 && c < nrhs) {
-                double s = 0;
-#pragma unroll
-                for (int w = 0; w < kWavesPerBlock; ++w) s += lds[w][lane];
-                result[c] = static_cast<T>(Norm ? sqrt(s) : s);
-            }
-            __syncthreads();
-        }
+}
+
+// One workgroup: result[c] = sum over `blocks` partials of column c, in
+// workgroup order per wave, then the waves in order (sqrt for norms).
+template <typename T, bool Norm>
+__global__ void __launch_bounds__(kBlock)
+    reduce_combine_kernel(int64_t blocks, int64_t nrhs, const double* __restrict__ partials,
+                          T* __restrict__ result)
+{
+    __shared__ double lds[kWavesPerBlock][kWave];
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    if (nrhs == 1) {
+        double acc = 0;
+        for (int64_t k = threadIdx.x; k < blocks; k += kBlock) acc += partials[k];
+        acc = wave_sum(acc);
+        if (lane == 0) lds[wave][0] = acc;
+        __syncthreads();
         if (threadIdx.x == 0) {
-            *ticket = 0;
+            double s = lds[0][0];
+            for (int w = 1; w < kWavesPerBlock; ++w) s += lds[w][0];
+            result[0] = static_cast<T>(Norm ? sqrt(s) : s);
         }
+        return;
+    }
+    for (int64_t c0 = 0; c0 < nrhs; c0 += kWave) {
+        const int64_t c = c0 + lane;
+        double acc = 0;
+        if (c < nrhs) {
+            for (int64_t k = wave; k < blocks; k += kWavesPerBlock) acc += partials[k * nrhs + c];
+        }
+        lds[wave][lane] = acc;
+        __syncthreads();
+        if (wave == 0 && c < nrhs) {
+            double s = 0;
+#pragma unroll
+            for (int w = 0; w < kWavesPerBlock; ++w) s += lds[w][lane];
+            result[c] = static_cast<T>(Norm ? sqrt(s) : s);
+        }
+        __syncthreads();
     }
 }
 
 template <typename T, bool Norm>
 int launch_column_reduction(hipStream_t stream, int64_t n, int64_t nrhs, const T* x,
-                            int64_t xs, const T* y, int64_t ys, T* result, unsigned* ticket,
-                            double* partials)
+                            int64_t xs, const T* y, int64_t ys, T* result, double* partials)
 {
+    int64_t g;
     if (nrhs == 1) {
-        int64_t g = ceildiv(n, kBlock * 8);
+        g = ceildiv(n, kBlock * 8);
         g = std::max<int64_t>(1, std::min<int64_t>(g, std::min<int64_t>(kRedMaxBlocks,
                                                                          num_cus() * 4)));
         reduce_one_column_kernel<T, Norm><<<static_cast<unsigned>(g), kBlock, 0, stream>>>(
-            n, x, xs, y, ys, result, ticket, partials);
+            n, x, xs, y, ys, partials);
     } else {
-        int64_t g = ceildiv(n, kWavesPerBlock * 16);
+        g = ceildiv(n, kWavesPerBlock * 16);
         g = std::max<int64_t>(1, std::min<int64_t>(g, std::min<int64_t>(kRedMaxBlocks,
                                                                          num_cus() * 4)));
         reduce_columns_kernel<T, Norm><<<static_cast<unsigned>(g), kBlock, 0, stream>>>(
-            n, nrhs, x, xs, y, ys, result, ticket, partials);
+            n, nrhs, x, xs, y, ys, partials);
     }
     GKOMI_CHECK_LAUNCH("column reduction");
+    reduce_combine_kernel<T, Norm><<<1, kBlock, 0, stream>>>(g, nrhs, partials, result);
+    GKOMI_CHECK_LAUNCH("column reduction combine");
     return GKOMI_SUCCESS;
 }
 

@@ -224,20 +224,37 @@ __global__ void criteria_check_kernel(gkomi_criteria crit, int64_t nrhs, const T
 }
 
 // ---------------------------------------------------------- fused CG ----
-// scalars layout (value type): [0] rho, [1] prev_rho, [2] beta, [3] tau
+// Two kernels per iteration.  Cross-workgroup sums never use a device-scope
+// fence (on gfx950 a release fence writes back the XCD's whole L2, which at
+// thousands of workgroups costs more than the iteration itself): each
+// producer writes one FP64 partial per workgroup, and every workgroup of the
+// consumer kernel adds all partials in the same fixed order, so all
+// workgroups reach bit-identical scalars and the same stop decision.  The
+// kernel boundary orders producer and consumer.  Values that a kernel both
+// reads and publishes (rho, tau, the stopping status) live in two slots
+// selected by the iteration's parity, so no workgroup reads a slot that
+// another workgroup of the same kernel writes.
+//
+// scalars (value type): [0], [1] rho of even / odd iterations (slot 1 starts
+// as prev_rho = 1), [2] beta, [3], [4] tau of even / odd iterations.
 
 constexpr int kCgMaxGrid = 2048;
 
 struct cg_work {
-    unsigned* tickets;   // 4 tickets, 64 B apart
-    double* partials;    // kCgMaxGrid * 2 doubles per phase
+    double* part_init;  // [kCgMaxGrid][2]: (b.b, r0.r0) of the start
+    double* part_upd;   // [kCgMaxGrid][2]: (r.z, r.r) of the last update kernel
+    double* part_dir;   // [kCgMaxGrid]: p.q of the last direction kernel
+    uint8_t* status;    // [2]: stopping status of even / odd iterations
 };
 
 __host__ __device__ inline cg_work cg_work_layout(void* base)
 {
     cg_work w;
-    w.tickets = static_cast<unsigned*>(base);
-    w.partials = reinterpret_cast<double*>(static_cast<char*>(base) + kTicketBytes);
+    double* d = static_cast<double*>(base);
+    w.part_init = d;
+    w.part_upd = d + 2 * kCgMaxGrid;
+    w.part_dir = d + 4 * kCgMaxGrid;
+    w.status = reinterpret_cast<uint8_t*>(d + 5 * kCgMaxGrid);
     return w;
 }
 
@@ -250,185 +267,149 @@ __device__ __forceinline__ void block_range(int64_t total, int64_t* begin, int64
     *end = min(total, *begin + per);
 }
 
-// Two-value block reduction into partials[blockIdx.x * 2 + {0,1}] and a
-// last-block combine; returns true in the last block with sums in out0/out1.
-template <typename T>
-__device__ bool reduce_pair(T a, T b, unsigned* ticket, double* partials, double* out0,
-                            double* out1)
+// Workgroup sums of (a, b) in the value type, converted to FP64 and stored
+// as this workgroup's partials.
+template <typename T, int Count>
+__device__ __forceinline__ void store_partials(const T (&vals)[Count], double* partials)
 {
-    __shared__ T lds_t[2][kWavesPerBlock];
-    __shared__ double lds_d[2][kWavesPerBlock];
-    __shared__ int flag;
-    a = wave_sum(a);
-    b = wave_sum(b);
+    __shared__ T lds[Count][kWavesPerBlock];
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
-    if (lane == 0) {
-        lds_t[0][wave] = a;
-        lds_t[1][wave] = b;
+#pragma unroll
+    for (int c = 0; c < Count; ++c) {
+        const T s = wave_sum(vals[c]);
+        if (lane == 0) lds[c][wave] = s;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        T s0 = lds_t[0][0], s1 = lds_t[1][0];
-        for (int w = 1; w < kWavesPerBlock; ++w) {
-            s0 += lds_t[0][w];
-            s1 += lds_t[1][w];
+#pragma unroll
+        for (int c = 0; c < Count; ++c) {
+            T s = lds[c][0];
+            for (int w = 1; w < kWavesPerBlock; ++w) s += lds[c][w];
+            partials[static_cast<int64_t>(blockIdx.x) * Count + c] = static_cast<double>(s);
         }
-        partials[2 * blockIdx.x] = static_cast<double>(s0);
-        partials[2 * blockIdx.x + 1] = static_cast<double>(s1);
     }
-    if (!last_block_ticket(ticket, &flag)) {
-        return false;
-    }
-    double c0 = 0, c1 = 0;
-    for (unsigned k = threadIdx.x; k < gridDim.x; k += kBlock) {
-        c0 += partials[2 * k];
-        c1 += partials[2 * k + 1];
-    }
-    c0 = wave_sum(c0);
-    c1 = wave_sum(c1);
-    if (lane == 0) {
-        lds_d[0][wave] = c0;
-        lds_d[1][wave] = c1;
-    }
-    __syncthreads();
-    double s0 = lds_d[0][0], s1 = lds_d[1][0];
-    for (int w = 1; w < kWavesPerBlock; ++w) {
-        s0 += lds_d[0][w];
-        s1 += lds_d[1][w];
-    }
-    *out0 = s0;
-    *out1 = s1;
-    if (threadIdx.x == 0) {
-        *ticket = 0;
-    }
-    return true;
 }
 
-// Start: r = b - A x (sub-wave SpMV), zero p_a / p_b / q, and the baselines
-// ||b|| and ||r0|| of every residual criterion; resets status and flags.
-template <typename T, int SW>
+// Sum of `count` workgroup partials (Count values each) in a fixed order:
+// thread t adds partials t, t + kBlock, ...; then the wave tree and the
+// waves in order.  Every workgroup obtains the same bits.
+template <int Count>
+__device__ __forceinline__ void combine_partials(const double* __restrict__ partials, int count,
+                                                 double (&out)[Count])
+{
+    __shared__ double lds[Count][kWavesPerBlock];
+    double acc[Count];
+#pragma unroll
+    for (int c = 0; c < Count; ++c) acc[c] = 0;
+    for (int k = threadIdx.x; k < count; k += kBlock) {
+#pragma unroll
+        for (int c = 0; c < Count; ++c) acc[c] += partials[static_cast<int64_t>(k) * Count + c];
+    }
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int c = 0; c < Count; ++c) {
+        const double s = wave_sum(acc[c]);
+        if (lane == 0) lds[c][wave] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < Count; ++c) {
+        double s = lds[c][0];
+        for (int w = 1; w < kWavesPerBlock; ++w) s += lds[c][w];
+        out[c] = s;
+    }
+    __syncthreads();
+}
+
+// Start, kernel 1: r = b - A x (sub-wave SpMV), p_a = p_b = q = 0, and
+// partials of b.b and r.r; resets the flags, the status and prev_rho = 1
+// (reference/solver/cg_kernels.cpp:26-50).
+template <typename T, int SW, int RPG>
 __global__ void __launch_bounds__(kBlock)
     cg_fused_init_kernel(int64_t n, const int32_t* __restrict__ rp,
                          const int32_t* __restrict__ ci, const T* __restrict__ v,
                          const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ r,
                          T* __restrict__ pa, T* __restrict__ pb, T* __restrict__ q,
-                         gkomi_criteria crit, T* __restrict__ scalars, T* __restrict__ orig,
-                         uint8_t* __restrict__ status, gkomi_stop_flags* __restrict__ flags,
-                         unsigned* ticket, double* partials)
+                         T* __restrict__ scalars, uint8_t* __restrict__ status,
+                         gkomi_stop_flags* __restrict__ flags, cg_work w)
 {
-    constexpr int kRows = kBlock / SW;
-    const int g = threadIdx.x / SW;
-    const int l = threadIdx.x % SW;
     int64_t rb0, rb1;
-    block_range(ceildiv(n, kRows), &rb0, &rb1);
-    T bb = 0, rr = 0;
+    block_range(ceildiv(n, subwave_tile_rows<SW, RPG>), &rb0, &rb1);
+    T acc[2] = {T(0), T(0)};
     for (int64_t blk = rb0; blk < rb1; ++blk) {
-        const int64_t row = blk * kRows + g;
-        T sum = 0;
-        if (row < n) {
-            const int32_t end = rp[row + 1];
-            for (int32_t k = rp[row] + l; k < end; k += SW) sum += v[k] * x[ci[k]];
-        }
-        sum = group_sum<SW>(sum);
-        if (row < n && l == 0) {
-            const T bi = b[row];
-            const T ri = bi - sum;
-            r[row] = ri;
-            pa[row] = T(0);
-            pb[row] = T(0);
-            q[row] = T(0);
-            bb += bi * bi;
-            rr += ri * ri;
-        }
+        subwave_tile<T, SW, RPG>(
+            n, blk, rp, ci, v, [&](int32_t c) { return x[c]; },
+            [&](int64_t row, T sum) {
+                const T bi = b[row];
+                const T ri = bi - sum;
+                r[row] = ri;
+                pa[row] = T(0);
+                pb[row] = T(0);
+                q[row] = T(0);
+                acc[0] += bi * bi;
+                acc[1] += ri * ri;
+            });
     }
-    double sbb, srr;
-    if (reduce_pair(bb, rr, ticket, partials, &sbb, &srr) && threadIdx.x == 0) {
-        for (int k = 0; k < crit.num; ++k) {
-            const int base = crit.c[k].baseline;
-            orig[k] = base == GKOMI_BASELINE_RHS_NORM
-                          ? static_cast<T>(sqrt(sbb))
-                          : (base == GKOMI_BASELINE_INITIAL_RESNORM ? static_cast<T>(sqrt(srr))
-                                                                    : T(1));
-        }
-        scalars[0] = T(0);  // rho
-        scalars[1] = T(1);  // prev_rho
-        scalars[2] = T(0);  // beta
+    store_partials<T, 2>(acc, w.part_init);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scalars[0] = T(0);
+        scalars[1] = T(1);  // prev_rho of iteration 0
+        scalars[2] = T(0);
+        scalars[3] = T(0);
+        scalars[4] = T(0);
         status[0] = 0;
+        w.status[0] = 0;
+        w.status[1] = 0;
         flags->iteration = -1;
         flags->all_stopped = 0;
         flags->one_changed = 0;
     }
 }
 
-// Kernel 1 of an iteration: p_out = z + rho/prev_rho p_in (p_out = z when
-// prev_rho == 0, reference/solver/cg_kernels.cpp:64-70), q = A p_out with
-// p_out recomputed at the gathered columns, and beta = p.q.
-template <typename T, int SW>
-__global__ void __launch_bounds__(kBlock)
-    cg_fused_direction_kernel(int64_t n, const int32_t* __restrict__ rp,
-                              const int32_t* __restrict__ ci, const T* __restrict__ v,
-                              const T* __restrict__ z, const T* __restrict__ p_in,
-                              T* __restrict__ p_out, T* __restrict__ q, T* __restrict__ scalars,
-                              const gkomi_stop_flags* __restrict__ flags, unsigned* ticket,
-                              double* partials)
-{
-    if (flags->all_stopped) return;
-    const T rho = scalars[0];
-    const T prev = scalars[1];
-    const bool restart = prev == T(0);
-    const T tmp = restart ? T(0) : rho / prev;
-    constexpr int kRows = kBlock / SW;
-    const int g = threadIdx.x / SW;
-    const int l = threadIdx.x % SW;
-    int64_t rb0, rb1;
-    block_range(ceildiv(n, kRows), &rb0, &rb1);
-    T pq = 0;
-    for (int64_t blk = rb0; blk < rb1; ++blk) {
-        const int64_t row = blk * kRows + g;
-        T sum = 0;
-        if (row < n) {
-            const int32_t end = rp[row + 1];
-            for (int32_t k = rp[row] + l; k < end; k += SW) {
-                const int32_t c = ci[k];
-                const T pc = restart ? z[c] : fma(tmp, p_in[c], z[c]);
-                sum += v[k] * pc;
-            }
-        }
-        sum = group_sum<SW>(sum);
-        if (row < n && l == 0) {
-            const T pi = restart ? z[row] : fma(tmp, p_in[row], z[row]);
-            p_out[row] = pi;
-            q[row] = sum;
-            pq += pi * sum;
-        }
-    }
-    double spq, unused;
-    if (reduce_pair(pq, T(0), ticket, partials, &spq, &unused) && threadIdx.x == 0) {
-        scalars[2] = static_cast<T>(spq);
-    }
-}
-
-// Kernel 2 of an iteration (and, with Update = false, the tail of the start):
-// x += rho/beta p, r -= rho/beta q (skipped when beta == 0,
-// reference/solver/cg_kernels.cpp:89-95), z = M r, rho_new = r.z, ||r||,
-// prev_rho = rho, rho = rho_new, iteration + 1, Combined check.
+// Kernel 2 of an iteration (Update) and of the start (!Update):
+//   Update: beta = p.q from the direction partials; x += rho/beta p,
+//           r -= rho/beta q (skipped when beta == 0,
+//           reference/solver/cg_kernels.cpp:89-95);
+//   start:  the criteria baselines ||b|| / ||r0|| from the init partials;
+// then z = M r and partials of r.z and r.r for the next direction kernel.
 template <typename T, bool Update, bool Jacobi>
 __global__ void __launch_bounds__(kBlock)
     cg_fused_update_kernel(int64_t n, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
                            const T* __restrict__ p, const T* __restrict__ q,
                            const T* __restrict__ inv_diag, gkomi_criteria crit,
-                           T* __restrict__ scalars, const T* __restrict__ orig,
-                           uint8_t* __restrict__ status, gkomi_stop_flags* __restrict__ flags,
-                           unsigned* ticket, double* partials)
+                           T* __restrict__ scalars, T* __restrict__ orig,
+                           const gkomi_stop_flags* __restrict__ flags, cg_work w, int count_in,
+                           int par)
 {
-    if (flags->all_stopped) return;
-    const T beta = scalars[2];
-    const bool do_update = Update && beta != T(0);
-    const T alpha = do_update ? scalars[0] / beta : T(0);
+    T alpha = T(0);
+    bool do_update = false;
+    if constexpr (Update) {
+        if (flags->all_stopped) return;
+        double pq[1];
+        combine_partials<1>(w.part_dir, count_in, pq);
+        const T beta = static_cast<T>(pq[0]);
+        do_update = beta != T(0);
+        alpha = do_update ? scalars[par] / beta : T(0);
+        if (blockIdx.x == 0 && threadIdx.x == 0) scalars[2] = beta;
+    } else {
+        double s[2];
+        combine_partials<2>(w.part_init, count_in, s);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            for (int k = 0; k < crit.num; ++k) {
+                const int base = crit.c[k].baseline;
+                orig[k] = base == GKOMI_BASELINE_RHS_NORM
+                              ? static_cast<T>(sqrt(s[0]))
+                              : (base == GKOMI_BASELINE_INITIAL_RESNORM
+                                     ? static_cast<T>(sqrt(s[1]))
+                                     : T(1));
+            }
+        }
+    }
     int64_t i0, i1;
     block_range(ceildiv(n, kBlock), &i0, &i1);
-    T rz = 0, rr = 0;
+    T acc[2] = {T(0), T(0)};
     for (int64_t blk = i0; blk < i1; ++blk) {
         const int64_t i = blk * kBlock + threadIdx.x;
         if (i >= n) continue;
@@ -440,30 +421,74 @@ __global__ void __launch_bounds__(kBlock)
         }
         const T zi = Jacobi ? ri * inv_diag[i] : ri;
         z[i] = zi;
-        rz += ri * zi;
-        rr += ri * ri;
+        acc[0] += ri * zi;
+        acc[1] += ri * ri;
     }
-    double srz, srr;
-    if (reduce_pair(rz, rr, ticket, partials, &srz, &srr) && threadIdx.x == 0) {
-        const T rho_new = static_cast<T>(srz);
-        const T tau = static_cast<T>(sqrt(srr));
-        if (Update) {
-            scalars[1] = scalars[0];
-        }
-        scalars[0] = rho_new;
-        scalars[3] = tau;
-        const int64_t iter = flags->iteration + 1;
-        int all = 0, changed = 0;
-        combined_check<T>(crit, iter, 1, &tau, &rho_new, orig, true, status, &all, &changed);
+    store_partials<T, 2>(acc, w.part_upd);
+}
+
+// Kernel 1 of iteration `iter` (parity par = iter % 2): rho = r.z and
+// tau = ||r|| from the update partials, the Combined check of this
+// iteration (core/solver/cg.cpp:145-158) in every workgroup; unless stopped,
+// p_out = z + rho/prev_rho p_in (p_out = z when prev_rho == 0,
+// reference/solver/cg_kernels.cpp:64-70), q = A p_out with p_out recomputed
+// at the gathered columns, and partials of p.q.
+template <typename T, int SW, int RPG>
+__global__ void __launch_bounds__(kBlock)
+    cg_fused_direction_kernel(int64_t n, const int32_t* __restrict__ rp,
+                              const int32_t* __restrict__ ci, const T* __restrict__ v,
+                              const T* __restrict__ z, const T* __restrict__ p_in,
+                              T* __restrict__ p_out, T* __restrict__ q, gkomi_criteria crit,
+                              T* __restrict__ scalars, const T* __restrict__ orig,
+                              uint8_t* __restrict__ status, gkomi_stop_flags* __restrict__ flags,
+                              cg_work w, int count_in, int64_t iter, int par)
+{
+    // workgroup 0 may set the flag during this kernel (when this iteration
+    // stops): read it once per workgroup so the early exit is uniform
+    __shared__ int stopped;
+    if (threadIdx.x == 0) stopped = flags->all_stopped;
+    __syncthreads();
+    if (stopped) return;
+    double s[2];
+    combine_partials<2>(w.part_upd, count_in, s);
+    const T rho = static_cast<T>(s[0]);
+    const T tau = static_cast<T>(sqrt(s[1]));
+    const T prev = scalars[par ^ 1];
+    uint8_t st = w.status[par ^ 1];
+    int all = 0, changed = 0;
+    combined_check<T>(crit, iter, 1, &tau, &rho, orig, true, &st, &all, &changed);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scalars[par] = rho;
+        scalars[3 + par] = tau;
+        w.status[par] = st;
+        status[0] = st;
         flags->iteration = iter;
         flags->all_stopped = all;
         flags->one_changed = changed;
     }
+    if (all) return;
+    const bool restart = prev == T(0);
+    const T tmp = restart ? T(0) : rho / prev;
+    int64_t rb0, rb1;
+    block_range(ceildiv(n, subwave_tile_rows<SW, RPG>), &rb0, &rb1);
+    T acc[1] = {T(0)};
+    for (int64_t blk = rb0; blk < rb1; ++blk) {
+        subwave_tile<T, SW, RPG>(
+            n, blk, rp, ci, v,
+            [&](int32_t c) { return restart ? z[c] : fma(tmp, p_in[c], z[c]); },
+            [&](int64_t row, T sum) {
+                const T pi = restart ? z[row] : fma(tmp, p_in[row], z[row]);
+                p_out[row] = pi;
+                q[row] = sum;
+                acc[0] += pi * sum;
+            });
+    }
+    store_partials<T, 1>(acc, w.part_dir);
 }
 
-int fused_grid(int64_t items_per_row_block_count)
+int fused_grid(int64_t tiles)
 {
-    int64_t g = std::min<int64_t>(items_per_row_block_count, static_cast<int64_t>(num_cus()) * 8);
+    int64_t g = std::min<int64_t>(tiles, static_cast<int64_t>(num_cus()) * 8);
     g = std::min<int64_t>(g, kCgMaxGrid);
     return static_cast<int>(std::max<int64_t>(g, 1));
 }
@@ -494,6 +519,25 @@ int check_cg_state(const gkomi_cg_state* st)
     return GKOMI_SUCCESS;
 }
 
+// Launch shapes shared by start and iterate (the consumer of a partial
+// array must know how many workgroups produced it).
+struct cg_grids {
+    subwave_shape sh;
+    int spmv;    // init and direction kernels
+    int update;  // update kernels
+};
+
+template <typename T>
+cg_grids cg_grids_for(const gkomi_csr* a, int64_t n)
+{
+    cg_grids g;
+    g.sh = pick_subwave(a, sizeof(T));
+    const int64_t tile_rows = int64_t{kBlock} / g.sh.sw * g.sh.rpg;
+    g.spmv = fused_grid(ceildiv(std::max<int64_t>(n, 1), tile_rows));
+    g.update = fused_grid(ceildiv(std::max<int64_t>(n, 1), kBlock));
+    return g;
+}
+
 template <typename T>
 int cg_fused_start_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_criteria* crit,
                         gkomi_cg_state* st)
@@ -506,50 +550,35 @@ int cg_fused_start_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_cr
                   "fused CG: matrix and vectors do not conform");
     GKOMI_REQUIRE(a->strategy == GKOMI_CSR_CLASSICAL, GKOMI_NOT_SUPPORTED,
                   "fused CG: needs the classical (sub-wave) CSR strategy");
-    const int sw = pick_subwave(a, sizeof(T)).sw;
     hipStream_t s = stream_of(ctx);
-    cg_work w = cg_work_layout(st->work);
+    const cg_work w = cg_work_layout(st->work);
     const T* v = static_cast<const T*>(a->values);
     T* sc = static_cast<T*>(st->scalars);
     const int64_t n = st->n;
-    const int grid = fused_grid(ceildiv(std::max<int64_t>(n, 1), kBlock / sw));
-#define GKOMI_INIT_CASE(S)                                                                   \
-    case S:                                                                                  \
-        cg_fused_init_kernel<T, S><<<grid, kBlock, 0, s>>>(                                  \
-            n, a->row_ptrs, a->col_idxs, v, static_cast<const T*>(st->b),                    \
-            static_cast<const T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->p_a),  \
-            static_cast<T*>(st->p_b), static_cast<T*>(st->q), *crit, sc,                     \
-            static_cast<T*>(st->orig_tau), st->status, st->flags, w.tickets, w.partials);    \
-        break;
-    switch (sw) {
-        GKOMI_INIT_CASE(1)
-        GKOMI_INIT_CASE(2)
-        GKOMI_INIT_CASE(4)
-        GKOMI_INIT_CASE(8)
-        GKOMI_INIT_CASE(16)
-        GKOMI_INIT_CASE(32)
-        GKOMI_INIT_CASE(64)
-        default:
-            return fail(GKOMI_INVALID_ARGUMENT, "fused CG: bad subwave");
-    }
+    const cg_grids g = cg_grids_for<T>(a, n);
+#define GKOMI_INIT_CASE(S, R)                                                                \
+    cg_fused_init_kernel<T, S, R><<<g.spmv, kBlock, 0, s>>>(                                 \
+        n, a->row_ptrs, a->col_idxs, v, static_cast<const T*>(st->b),                        \
+        static_cast<const T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->p_a),      \
+        static_cast<T*>(st->p_b), static_cast<T*>(st->q), sc, st->status, st->flags, w);     \
+    break;
+    GKOMI_SUBWAVE_SWITCH(g.sh, GKOMI_INIT_CASE)
 #undef GKOMI_INIT_CASE
     GKOMI_CHECK_LAUNCH("cg_fused_init_kernel");
-    const int ugrid = fused_grid(ceildiv(std::max<int64_t>(n, 1), kBlock));
-    unsigned* t2 = w.tickets + 16;
-    double* p2 = w.partials + 2 * kCgMaxGrid;
     if (st->inv_diag) {
-        cg_fused_update_kernel<T, false, true><<<ugrid, kBlock, 0, s>>>(
+        cg_fused_update_kernel<T, false, true><<<g.update, kBlock, 0, s>>>(
             n, static_cast<T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->z),
             static_cast<const T*>(st->p_a), static_cast<const T*>(st->q),
-            static_cast<const T*>(st->inv_diag), *crit, sc, static_cast<const T*>(st->orig_tau),
-            st->status, st->flags, t2, p2);
+            static_cast<const T*>(st->inv_diag), *crit, sc, static_cast<T*>(st->orig_tau),
+            st->flags, w, g.spmv, 0);
     } else {
-        cg_fused_update_kernel<T, false, false><<<ugrid, kBlock, 0, s>>>(
+        cg_fused_update_kernel<T, false, false><<<g.update, kBlock, 0, s>>>(
             n, static_cast<T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->z),
             static_cast<const T*>(st->p_a), static_cast<const T*>(st->q), nullptr, *crit, sc,
-            static_cast<const T*>(st->orig_tau), st->status, st->flags, t2, p2);
+            static_cast<T*>(st->orig_tau), st->flags, w, g.spmv, 0);
     }
     GKOMI_CHECK_LAUNCH("cg_fused_update_kernel");
+    st->issued = 0;
     return GKOMI_SUCCESS;
 }
 
@@ -566,53 +595,42 @@ int cg_fused_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_
     GKOMI_REQUIRE(a->strategy == GKOMI_CSR_CLASSICAL, GKOMI_NOT_SUPPORTED,
                   "fused CG: needs the classical (sub-wave) CSR strategy");
     GKOMI_REQUIRE(count >= 0, GKOMI_INVALID_ARGUMENT, "fused CG: negative count");
-    const int sw = pick_subwave(a, sizeof(T)).sw;
+    GKOMI_REQUIRE(st->issued >= 0, GKOMI_INVALID_ARGUMENT,
+                  "fused CG: state not started (issued < 0)");
     hipStream_t s = stream_of(ctx);
-    cg_work w = cg_work_layout(st->work);
+    const cg_work w = cg_work_layout(st->work);
     const T* v = static_cast<const T*>(a->values);
     T* sc = static_cast<T*>(st->scalars);
     const int64_t n = st->n;
-    const int dgrid = fused_grid(ceildiv(std::max<int64_t>(n, 1), kBlock / sw));
-    const int ugrid = fused_grid(ceildiv(std::max<int64_t>(n, 1), kBlock));
-    unsigned* t1 = w.tickets;
-    unsigned* t2 = w.tickets + 16;
-    double* p1 = w.partials;
-    double* p2 = w.partials + 2 * kCgMaxGrid;
+    const cg_grids g = cg_grids_for<T>(a, n);
     for (int32_t it = 0; it < count; ++it) {
         T* pin = static_cast<T*>(st->p_a);
         T* pout = static_cast<T*>(st->p_b);
-#define GKOMI_DIR_CASE(S)                                                                  \
-    case S:                                                                                \
-        cg_fused_direction_kernel<T, S><<<dgrid, kBlock, 0, s>>>(                          \
-            n, a->row_ptrs, a->col_idxs, v, static_cast<const T*>(st->z), pin, pout,       \
-            static_cast<T*>(st->q), sc, st->flags, t1, p1);                                \
-        break;
-        switch (sw) {
-            GKOMI_DIR_CASE(1)
-            GKOMI_DIR_CASE(2)
-            GKOMI_DIR_CASE(4)
-            GKOMI_DIR_CASE(8)
-            GKOMI_DIR_CASE(16)
-            GKOMI_DIR_CASE(32)
-            GKOMI_DIR_CASE(64)
-            default:
-                return fail(GKOMI_INVALID_ARGUMENT, "fused CG: bad subwave");
-        }
+        const int64_t iter = st->issued;
+        const int par = static_cast<int>(iter & 1);
+#define GKOMI_DIR_CASE(S, R)                                                               \
+    cg_fused_direction_kernel<T, S, R><<<g.spmv, kBlock, 0, s>>>(                          \
+        n, a->row_ptrs, a->col_idxs, v, static_cast<const T*>(st->z), pin, pout,           \
+        static_cast<T*>(st->q), *crit, sc, static_cast<const T*>(st->orig_tau), st->status, \
+        st->flags, w, g.update, iter, par);                                                \
+    break;
+        GKOMI_SUBWAVE_SWITCH(g.sh, GKOMI_DIR_CASE)
 #undef GKOMI_DIR_CASE
         GKOMI_CHECK_LAUNCH("cg_fused_direction_kernel");
         if (st->inv_diag) {
-            cg_fused_update_kernel<T, true, true><<<ugrid, kBlock, 0, s>>>(
+            cg_fused_update_kernel<T, true, true><<<g.update, kBlock, 0, s>>>(
                 n, static_cast<T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->z), pout,
                 static_cast<const T*>(st->q), static_cast<const T*>(st->inv_diag), *crit, sc,
-                static_cast<const T*>(st->orig_tau), st->status, st->flags, t2, p2);
+                static_cast<T*>(st->orig_tau), st->flags, w, g.spmv, par);
         } else {
-            cg_fused_update_kernel<T, true, false><<<ugrid, kBlock, 0, s>>>(
+            cg_fused_update_kernel<T, true, false><<<g.update, kBlock, 0, s>>>(
                 n, static_cast<T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->z), pout,
-                static_cast<const T*>(st->q), nullptr, *crit, sc,
-                static_cast<const T*>(st->orig_tau), st->status, st->flags, t2, p2);
+                static_cast<const T*>(st->q), nullptr, *crit, sc, static_cast<T*>(st->orig_tau),
+                st->flags, w, g.spmv, par);
         }
         GKOMI_CHECK_LAUNCH("cg_fused_update_kernel");
         std::swap(st->p_a, st->p_b);
+        ++st->issued;
     }
     return GKOMI_SUCCESS;
 }
@@ -774,7 +792,7 @@ GKOMI_CG_INIT(f32, float)
 size_t gkomi_cg_work_bytes(int64_t n)
 {
     (void)n;
-    return kTicketBytes + sizeof(double) * 4 * static_cast<size_t>(kCgMaxGrid);
+    return sizeof(double) * 5 * static_cast<size_t>(kCgMaxGrid) + 64;
 }
 
 void* gkomi_cg_current_p(const gkomi_cg_state* st) { return st ? st->p_a : nullptr; }

@@ -27,6 +27,99 @@ __device__ __forceinline__ void store_result(T* dst, T sum, const T* alpha, cons
     }
 }
 
+// Rows per workgroup tile of the classical (sub-wave) kernels.
+template <int SW, int RPG>
+constexpr int64_t subwave_tile_rows = int64_t{kWave / SW} * RPG * kWavesPerBlock;
+
+// One workgroup tile of the classical CSR product: rows
+// [tile * subwave_tile_rows, ...).  Each group of SW lanes owns RPG rows at
+// once (rows g, g + 64/SW, ... of the wave's contiguous range of 64/SW * RPG
+// rows).  The body is branch-free: lanes past the end of a row load entry 0
+// (valid whenever a row is active) and add a zero product, so the compiler
+// issues the loads of all RPG rows back to back and waits once -- RPG times
+// the bytes in flight per wave, which is what a short-row SpMV needs to cover
+// HBM latency.  Per row the summation order is fixed (lane l adds entries l,
+// l + SW, ... by fused multiply-add; then the group tree), independent of
+// RPG and of the launch.  `gather(col)` supplies the vector entry of a
+// column; `row_op(row, sum)` runs in lane 0 of the group for every row below
+// n_rows.
+template <typename T, int SW, int RPG, typename Gather, typename RowOp>
+__device__ __forceinline__ void subwave_tile(int64_t n_rows, int64_t tile,
+                                             const int32_t* __restrict__ rp,
+                                             const int32_t* __restrict__ ci,
+                                             const T* __restrict__ v, Gather gather,
+                                             RowOp row_op)
+{
+    constexpr int kGroups = kWave / SW;
+    constexpr int kRowsPerWave = kGroups * RPG;
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    const int g = lane / SW;
+    const int l = lane % SW;
+    const int64_t row0 = tile * subwave_tile_rows<SW, RPG> + wave * kRowsPerWave + g;
+    int32_t pos[RPG];
+    int32_t end[RPG];
+    T sum[RPG];
+    int32_t iters = 0;
+    // rows past the end read rp[n_rows] twice: empty
+#pragma unroll
+    for (int k = 0; k < RPG; ++k) {
+        const int64_t row = row0 + k * kGroups;
+        pos[k] = rp[row < n_rows ? row : n_rows];
+        end[k] = rp[row < n_rows ? row + 1 : n_rows];
+    }
+#pragma unroll
+    for (int k = 0; k < RPG; ++k) {
+        pos[k] += l;
+        sum[k] = 0;
+        const int32_t len = end[k] - pos[k];
+        const int32_t it = len > 0 ? (len + SW - 1) / SW : 0;
+        iters = it > iters ? it : iters;
+    }
+    for (int32_t j = 0; j < iters; ++j) {
+        int32_t col[RPG];
+        T val[RPG];
+        bool act[RPG];
+#pragma unroll
+        for (int k = 0; k < RPG; ++k) {
+            act[k] = pos[k] < end[k];
+            const int32_t p = act[k] ? pos[k] : 0;
+            col[k] = ci[p];
+            val[k] = v[p];
+            pos[k] = act[k] ? pos[k] + SW : pos[k];
+        }
+        T xv[RPG];
+#pragma unroll
+        for (int k = 0; k < RPG; ++k) xv[k] = gather(col[k]);
+#pragma unroll
+        for (int k = 0; k < RPG; ++k) {
+            sum[k] = fma(act[k] ? val[k] : T(0), act[k] ? xv[k] : T(0), sum[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RPG; ++k) {
+        sum[k] = group_sum<SW>(sum[k]);
+        const int64_t row = row0 + k * kGroups;
+        if (row < n_rows && l == 0) row_op(row, sum[k]);
+    }
+}
+
+// Dispatch over the classical shapes: sw in {1..64}, rpg in {1, 2, 4, 8};
+// CASE(S, R) is expanded for each and must end in `break` or `return`.
+#define GKOMI_SUBWAVE_SWITCH(sh, CASE)                                                  \
+    switch ((sh).sw * 16 + (sh).rpg) {                                                  \
+        GKOMI_SUBWAVE_ROW_(1, CASE) GKOMI_SUBWAVE_ROW_(2, CASE) GKOMI_SUBWAVE_ROW_(4, CASE) \
+        GKOMI_SUBWAVE_ROW_(8, CASE) GKOMI_SUBWAVE_ROW_(16, CASE)                        \
+        GKOMI_SUBWAVE_ROW_(32, CASE) GKOMI_SUBWAVE_ROW_(64, CASE)                       \
+        default:                                                                        \
+            return fail(GKOMI_INVALID_ARGUMENT,                                         \
+                        "CSR: subwave must be a power of two <= 64 and rows per group "  \
+                        "1, 2, 4 or 8");                                                \
+    }
+#define GKOMI_SUBWAVE_ROW_(S, CASE) \
+    case S * 16 + 1: CASE(S, 1) case S * 16 + 2: CASE(S, 2) case S * 16 + 4: CASE(S, 4) \
+    case S * 16 + 8: CASE(S, 8)
+
 int check_csr(const gkomi_csr* a);
 // Shape of the classical kernel: `sw` lanes per row, `rpg` rows in flight per
 // lane group.  gkomi_csr::subwave holds sw in bits 0..7 and rpg in bits 8..15
@@ -38,11 +131,6 @@ struct subwave_shape {
 
 subwave_shape pick_subwave(const gkomi_csr* a, int value_bytes);
 
-inline bool valid_subwave(subwave_shape sh)
-{
-    return sh.sw >= 1 && sh.sw <= 64 && (sh.sw & (sh.sw - 1)) == 0 &&
-           (sh.rpg == 1 || sh.rpg == 2 || sh.rpg == 4 || sh.rpg == 8);
-}
 
 template <typename T, bool Adv>
 int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const T* b, int64_t bs,

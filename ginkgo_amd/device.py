@@ -36,7 +36,7 @@ def ptr(t):
 
 
 def workspace(nbytes, device):
-    """Zero-filled scratch (the reductions require a zeroed ticket)."""
+    """Zero-filled device scratch of at least one byte."""
     return torch.zeros(max(int(nbytes), 1), dtype=torch.uint8, device=device)
 
 

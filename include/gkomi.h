@@ -116,9 +116,8 @@ size_t gkomi_reduction_workspace_bytes(int64_t n, int64_t nrhs);
 
 /* Dense::compute_conj_dot / compute_norm2 (core/matrix/dense.cpp:400-466;
  * reference/matrix/dense_kernels.cpp:299-352): result[j] per column, on the
- * device.  Single-pass: per-workgroup partials plus a last-workgroup combine
- * in fixed order.  The workspace must be zero-filled before its first use;
- * every call leaves it zero-filled again. */
+ * device.  Per-workgroup FP64 partials, then a one-workgroup combine in
+ * fixed order (deterministic).  The workspace needs no initialisation. */
 int gkomi_dense_dot_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const double* x,
                         int64_t x_stride, const double* y, int64_t y_stride,
                         double* result, void* workspace, size_t workspace_bytes);
@@ -392,8 +391,11 @@ int gkomi_cg_step_2_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, float* x,
 
 /* Device state of a fused CG solve with one right-hand side.  All pointers
  * are device memory of the entry point's value type (V); `work` holds the
- * reduction partials and must be zero-filled before the first call
- * (gkomi_cg_work_bytes bytes).  p_a / p_b are used alternately. */
+ * per-workgroup reduction partials (gkomi_cg_work_bytes bytes, no
+ * initialisation needed).  p_a / p_b are used alternately.  `issued` counts
+ * the iterations enqueued since the start (set by gkomi_cg_fused_start,
+ * advanced by gkomi_cg_fused_iterate); the state is host memory that both
+ * calls update. */
 typedef struct {
     int64_t n;
     void* x;        /* solution (in: initial guess) */
@@ -404,26 +406,28 @@ typedef struct {
     void* p_b;
     void* q;
     const void* inv_diag; /* scalar Jacobi, or NULL for the identity */
-    void* scalars;  /* 8 values: rho, prev_rho, beta, tau, rho_new ... (library-owned layout) */
+    void* scalars;  /* 8 values (library-owned layout) */
     void* orig_tau; /* GKOMI_MAX_CRITERIA values */
     uint8_t* status;
     gkomi_stop_flags* flags;
     void* work;
+    int64_t issued;
 } gkomi_cg_state;
 
 size_t gkomi_cg_work_bytes(int64_t n);
-/* Start of Cg::apply_dense_impl (core/solver/cg.cpp:93-158): initialize,
- * r = b - A x, the criteria baselines, z = M r, rho = r.z, and the check of
- * iteration 0 -- all on the device. */
+/* Start of Cg::apply_dense_impl (core/solver/cg.cpp:93-144): initialize,
+ * r = b - A x, the criteria baselines, z = M r and the partial sums of
+ * rho = r.z -- all on the device; the check of iteration 0 runs in the first
+ * iterate kernel. */
 int gkomi_cg_fused_start_f64(const gkomi_ctx* ctx, const gkomi_csr* a,
                              const gkomi_criteria* crit, gkomi_cg_state* st);
 int gkomi_cg_fused_start_f32(const gkomi_ctx* ctx, const gkomi_csr* a,
                              const gkomi_criteria* crit, gkomi_cg_state* st);
-/* `count` CG iterations (core/solver/cg.cpp:160-192 then :145-158): per
- * iteration two kernels -- (1) p = z + rho/prev_rho p fused into q = A p and
- * beta = p.q, (2) x += rho/beta p, r -= rho/beta q, z = M r, rho = r.z,
- * ||r||, and the Combined check.  Once flags->all_stopped is set the
- * remaining kernels return immediately, so the host may enqueue ahead. */
+/* `count` CG iterations (core/solver/cg.cpp:145-192): per iteration two
+ * kernels -- (1) rho = r.z, ||r|| and the Combined check of the iteration,
+ * then p = z + rho/prev_rho p fused into q = A p and beta = p.q, (2)
+ * x += rho/beta p, r -= rho/beta q, z = M r.  Once flags->all_stopped is set
+ * the remaining kernels return immediately, so the host may enqueue ahead. */
 int gkomi_cg_fused_iterate_f64(const gkomi_ctx* ctx, const gkomi_csr* a,
                                const gkomi_criteria* crit, gkomi_cg_state* st, int32_t count);
 int gkomi_cg_fused_iterate_f32(const gkomi_ctx* ctx, const gkomi_csr* a,
