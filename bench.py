@@ -337,6 +337,15 @@ def main():
     achieved = b64 / (ev_ms * 1e-3) / 1e9
 
     stream_bw, torch_copy_bw = stream_copy_gbps(dev)
+    traffic, traffic_src = None, None
+    prof = os.path.join(ROOT, "profiles", "csr_spmv_f64_laplace3d_128.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            p = json.load(f)
+        if p.get("algorithmic_bytes") == b64:
+            traffic = p["traffic_bytes"]
+            traffic_src = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of " + p["kernel"] +
+                           ", copy-kernel calibrated (profiles/csr_spmv_f64_laplace3d_128.md)")
     result = {
         "metric": "SpMV HBM GB/s (FP64 CSR, 3D 7-pt Laplacian 128^3)",
         "value": round(value, 2),
@@ -354,7 +363,8 @@ def main():
                    "index": "int32", "strategy": A.strategy, "parallelism": "single_gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": None, "kernel_avg_us": round(ev_ms * 1e3, 3),
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel_avg_us": round(ev_ms * 1e3, 3),
                      "algorithmic_bytes": b64},
         "stream_copy_GBps": round(stream_bw, 1),
         "torch_copy_GBps": round(torch_copy_bw, 1),
