@@ -224,6 +224,7 @@ protected:
 private:
     void apply_dense(const matrix::Dense<ValueType>* b, matrix::Dense<ValueType>* x) const;
     bool try_fused(const matrix::Dense<ValueType>* b, matrix::Dense<ValueType>* x) const;
+    bool try_fused_block(const matrix::Dense<ValueType>* b, matrix::Dense<ValueType>* x) const;
 };
 
 template <typename ValueType = double>

@@ -1,13 +1,13 @@
 // Jacobi, Cg, Bicgstab and Ir on the Mi355xExecutor.
 //
-// Cg has two paths.  The fused path (one right-hand side, classical CSR,
+// Cg has three paths.  The fused path (one right-hand side, classical CSR,
 // identity or scalar Jacobi preconditioner) runs two kernels per iteration
 // and evaluates the stopping criteria on the device; the host enqueues
 // iterations ahead in chunks and polls the device flags through pinned
 // memory one chunk behind, so it never stalls the stream inside the loop.
-// The generic path follows core/solver/cg.cpp step by step with the
-// reference-semantics kernels and handles any LinOp system matrix and any
-// number of right-hand sides.
+// The fused block path does the same for several right-hand sides over any
+// Csr.  The generic path follows core/solver/cg.cpp step by step with the
+// reference-semantics kernels and handles any LinOp system matrix.
 #include <cmath>
 
 #include "ginkgo_amd/dispatch.hpp"
@@ -30,6 +30,8 @@ struct sk<double> {
     static constexpr auto cg_step_2 = gkomi_cg_step_2_f64;
     static constexpr auto cg_fused_start = gkomi_cg_fused_start_f64;
     static constexpr auto cg_fused_iterate = gkomi_cg_fused_iterate_f64;
+    static constexpr auto block_cg_start = gkomi_block_cg_start_f64;
+    static constexpr auto block_cg_iterate = gkomi_block_cg_iterate_f64;
     static constexpr auto bicg_init = gkomi_bicgstab_initialize_f64;
     static constexpr auto bicg_step_1 = gkomi_bicgstab_step_1_f64;
     static constexpr auto bicg_step_2 = gkomi_bicgstab_step_2_f64;
@@ -49,6 +51,8 @@ struct sk<float> {
     static constexpr auto cg_step_2 = gkomi_cg_step_2_f32;
     static constexpr auto cg_fused_start = gkomi_cg_fused_start_f32;
     static constexpr auto cg_fused_iterate = gkomi_cg_fused_iterate_f32;
+    static constexpr auto block_cg_start = gkomi_block_cg_start_f32;
+    static constexpr auto block_cg_iterate = gkomi_block_cg_iterate_f32;
     static constexpr auto bicg_init = gkomi_bicgstab_initialize_f32;
     static constexpr auto bicg_step_1 = gkomi_bicgstab_step_1_f32;
     static constexpr auto bicg_step_2 = gkomi_bicgstab_step_2_f32;
@@ -349,13 +353,75 @@ bool Cg<V>::try_fused(const Vec<V>* b, Vec<V>* x) const
     return true;
 }
 
+// Several right-hand sides: the fused k-column CG (four kernels per
+// iteration, per-column scalars and status), with the same chunked lagged
+// polling.  Any Csr strategy works (one lane group per row).
+template <typename V>
+bool Cg<V>::try_fused_block(const Vec<V>* b, Vec<V>* x) const
+{
+    auto csr = dynamic_cast<const matrix::Csr<V, int32>*>(this->system_matrix_.get());
+    const size_type k = b->get_size().cols;
+    if (!csr || k < 2) return false;
+    if (this->precond_ && !this->inv_diag_) return false;
+    auto exec = this->get_executor();
+    const size_type n = b->get_size().rows;
+    array<V> r(exec, n * k), pa(exec, n * k), pb(exec, n * k), q(exec, n * k);
+    array<V> z(exec, this->inv_diag_ ? n * k : 0);
+    array<V> scalars(exec, 5 * k), orig(exec, GKOMI_MAX_CRITERIA * k);
+    array<uint8> status(exec, k);
+    array<char> flags(exec, sizeof(gkomi_stop_flags));
+    array<char> work(exec, gkomi_block_cg_work_bytes(n, k));
+    gkomi_block_cg_state st{};
+    st.n = static_cast<int64_t>(n);
+    st.nrhs = static_cast<int64_t>(k);
+    st.x = x->get_values();
+    st.x_stride = static_cast<int64_t>(x->get_stride());
+    st.b = b->get_const_values();
+    st.b_stride = static_cast<int64_t>(b->get_stride());
+    st.r = r.get_data();
+    st.z = this->inv_diag_ ? z.get_data() : r.get_data();
+    st.p_a = pa.get_data();
+    st.p_b = pb.get_data();
+    st.q = q.get_data();
+    st.inv_diag = this->inv_diag_;
+    st.scalars = scalars.get_data();
+    st.orig_tau = orig.get_data();
+    st.status = status.get_data();
+    st.flags = reinterpret_cast<gkomi_stop_flags*>(flags.get_data());
+    st.work = work.get_data();
+    const auto d = csr->descriptor();
+    const gkomi_ctx* ctx = ctx_of(exec);
+    GKO_CALL(sk<V>::block_cg_start(ctx, &d, &this->crit_, &st));
+    pinned_flags slots(2);
+    const int dev = ctx->device;
+    event e0(dev), e1(dev);
+    event* evs[2] = {&e0, &e1};
+    auto enqueue = [&](int slot) {
+        GKO_CALL(sk<V>::block_cg_iterate(ctx, &d, &this->crit_, &st, this->chunk_));
+        GKO_CALL(gkomi_memcpy_async(ctx, &slots.host[slot], st.flags, sizeof(gkomi_stop_flags)));
+        GKO_CALL(gkomi_event_record(ctx, evs[slot]->ev));
+    };
+    enqueue(0);
+    for (int64 i = 0;; ++i) {
+        enqueue(static_cast<int>((i + 1) % 2));
+        GKO_CALL(gkomi_event_synchronize(evs[i % 2]->ev));
+        if (slots.host[i % 2].all_stopped) break;
+    }
+    exec->synchronize();
+    const auto f = read_flags(exec, st.flags);
+    this->stats_.iterations = f.iteration;
+    this->stats_.status = host_status(status);
+    this->stats_.used_fused_path = true;
+    return true;
+}
+
 // Generic path: core/solver/cg.cpp:93-196, with the check evaluated on the
 // device and its result read back once per iteration.
 template <typename V>
 void Cg<V>::apply_dense(const Vec<V>* b, Vec<V>* x) const
 {
     if (!this->system_matrix_) return;
-    if (!try_fused(b, x)) {
+    if (!try_fused(b, x) && !try_fused_block(b, x)) {
         auto exec = this->get_executor();
         const auto ctx = ctx_of(exec);
         const size_type n = b->get_size().rows, k = b->get_size().cols;

@@ -183,25 +183,16 @@ __device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblocks)
     return base + pos;
 }
 
-// -------------------------------------------- single-pass combine helper ----
-// Partials of `count` values per workgroup are written to `partials`
-// [gridDim.x][count]; the last workgroup (ticket) sees them all.  The ticket
-// counter is reset by the last workgroup, so the workspace stays reusable
-// without a memset (also under graph capture).
-__device__ __forceinline__ bool last_block_ticket(unsigned* ticket, int* lds_flag)
+// Contiguous share [begin, end) of `total` items for this workgroup, with the
+// XCD-aware workgroup order (neighbouring shares on one XCD).
+__device__ __forceinline__ void block_range(int64_t total, int64_t* begin, int64_t* end)
 {
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned t = atomicAdd(ticket, 1u);
-        *lds_flag = (t == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    const bool last = *lds_flag != 0;
-    if (last) {
-        __threadfence();
-    }
-    return last;
+    const int64_t g = gridDim.x;
+    const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t per = ceildiv(total, g);
+    *begin = min(total, lb * per);
+    *end = min(total, *begin + per);
 }
+
 
 }  // namespace gkomi

@@ -258,15 +258,6 @@ __host__ __device__ inline cg_work cg_work_layout(void* base)
     return w;
 }
 
-__device__ __forceinline__ void block_range(int64_t total, int64_t* begin, int64_t* end)
-{
-    const int64_t g = gridDim.x;
-    const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
-    const int64_t per = ceildiv(total, g);
-    *begin = min(total, lb * per);
-    *end = min(total, *begin + per);
-}
-
 // Workgroup sums of (a, b) in the value type, converted to FP64 and stored
 // as this workgroup's partials.
 template <typename T, int Count>

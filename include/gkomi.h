@@ -435,6 +435,58 @@ int gkomi_cg_fused_iterate_f32(const gkomi_ctx* ctx, const gkomi_csr* a,
 /* Pointer to the p buffer that holds the current search direction. */
 void* gkomi_cg_current_p(const gkomi_cg_state* st);
 
+/* ------------------------------------------ CG, k right-hand sides (fused) */
+
+/* Device state of a fused CG over k = nrhs right-hand sides sharing one
+ * matrix: Cg::apply_dense_impl (core/solver/cg.cpp:93-196) with per-column
+ * scalars and stopping status, a stopped column keeping its p, x and r
+ * (reference/solver/cg_kernels.cpp:55-95).  x and b are n x nrhs with their
+ * own strides; r, z, p_a, p_b, q are n x nrhs with stride nrhs.  Without a
+ * preconditioner (inv_diag NULL) z must be the same buffer as r.
+ * `scalars` holds 5 * nrhs values, `orig_tau` GKOMI_MAX_CRITERIA * nrhs,
+ * `status` nrhs bytes; `work` gkomi_block_cg_work_bytes(n, nrhs) bytes (no
+ * initialisation needed); `issued` as in gkomi_cg_state. */
+typedef struct {
+    int64_t n;
+    int64_t nrhs;
+    void* x;
+    int64_t x_stride;
+    const void* b;
+    int64_t b_stride;
+    void* r;
+    void* z;
+    void* p_a;
+    void* p_b;
+    void* q;
+    const void* inv_diag;
+    void* scalars;
+    void* orig_tau;
+    uint8_t* status;
+    gkomi_stop_flags* flags;
+    void* work;
+    int64_t issued;
+} gkomi_block_cg_state;
+
+size_t gkomi_block_cg_work_bytes(int64_t n, int64_t nrhs);
+/* Start: r = b - A x, the baselines per column, z = M r and the partials of
+ * r.z (cg.cpp:93-144 with nrhs columns; reference/solver/cg_kernels.cpp:23-50). */
+int gkomi_block_cg_start_f64(const gkomi_ctx* ctx, const gkomi_csr* a,
+                             const gkomi_criteria* crit, gkomi_block_cg_state* st);
+int gkomi_block_cg_start_f32(const gkomi_ctx* ctx, const gkomi_csr* a,
+                             const gkomi_criteria* crit, gkomi_block_cg_state* st);
+/* `count` iterations (cg.cpp:145-192), four kernels each: the Combined check
+ * over all columns, p = z + rho/prev_rho p with q = A p, beta = p.q, and the
+ * x / r update with z = M r.  Once flags->all_stopped is set the remaining
+ * kernels return immediately. */
+int gkomi_block_cg_iterate_f64(const gkomi_ctx* ctx, const gkomi_csr* a,
+                               const gkomi_criteria* crit, gkomi_block_cg_state* st,
+                               int32_t count);
+int gkomi_block_cg_iterate_f32(const gkomi_ctx* ctx, const gkomi_csr* a,
+                               const gkomi_criteria* crit, gkomi_block_cg_state* st,
+                               int32_t count);
+/* Pointer to the p buffer that holds the current search directions. */
+void* gkomi_block_cg_current_p(const gkomi_block_cg_state* st);
+
 /* ------------------------------------------- BiCGSTAB (reference semantics) */
 
 /* bicgstab::initialize / step_1 / step_2 / step_3 / finalize

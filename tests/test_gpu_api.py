@@ -208,7 +208,71 @@ def test_config1_poisson_cg_jacobi(gko, exec_, dt):
     assert res <= tol * (2 if dt == "f64" else 20)
 
 
+@pytest.mark.parametrize("k,jacobi,dt", [(2, False, "f64"), (5, True, "f64"), (64, False, "f64"),
+                                          (70, True, "f64"), (7, False, "f32")])
+def test_cg_many_rhs_block_path(gko, exec_, k, jacobi, dt):
+    """k right-hand sides on the fused block path: per-column scalars and
+    stopping status against the oracle's restatement of cg.cpp."""
+    rp, ci, v = gen.poisson2d(40)
+    v = v.astype(NP[dt])
+    n = 1600
+    rng = np.random.default_rng(k)
+    b_h = rng.uniform(-1, 1, (n, k)).astype(NP[dt])
+    tol = 1e-10 if dt == "f64" else 1e-5
+    spec = [{"kind": "iteration", "max_iters": 1000},
+            {"kind": "residual_norm", "reduction_factor": tol}]
+    inv = ora.jacobi_inverse_diagonal(rp, ci, v) if jacobi else None
+    xo, so, io = ora.cg_solve(rp, ci, v, b_h, np.zeros_like(b_h), spec, inv_diag=inv)
+    A = gko.csr_from_host(exec_, rp, ci, v)
+    params = gko.solver.Cg.build(dt).with_criteria(*crits(gko, spec, dt))
+    if jacobi:
+        params = params.with_preconditioner(
+            gko.preconditioner.Jacobi.build(dt).with_max_block_size(1))
+    s = params.on(exec_).generate(A)
+    x = gko.matrix.Dense(dt).create(exec_, n, k)
+    x.fill(0.0)
+    s.apply(dense(gko, exec_, b_h, dt), x)
+    assert s.last_used_fused_path
+    band = 3 if dt == "f64" else max(3, io.iterations // 10)
+    assert abs(s.last_iterations - io.iterations) <= band
+    if dt == "f64":
+        assert s.last_status == [int(c) for c in so]
+        np.testing.assert_allclose(x.to_numpy(), xo, atol=1e-7)
+    d = gen.csr_to_dense(rp, ci, v.astype(np.float64))
+    xh = x.to_numpy().astype(np.float64)
+    res = np.linalg.norm(b_h - d @ xh, axis=0) / np.linalg.norm(b_h, axis=0)
+    assert np.all(res <= tol * (2 if dt == "f64" else 20))
+
+
+def test_cg_block_per_column_stopping(gko, exec_):
+    """Columns that stop at different iterations keep their p, x and r
+    (reference/solver/cg_kernels.cpp:55-95): a zero column stops at
+    iteration 0, an eigenvector column at iteration 1, the rest later."""
+    nx = 30
+    rp, ci, v = gen.poisson2d(nx)
+    n = nx * nx
+    i = np.arange(nx) + 1
+    eig = np.outer(np.sin(np.pi * i / (nx + 1)), np.sin(2 * np.pi * i / (nx + 1))).reshape(-1)
+    rng = np.random.default_rng(5)
+    b_h = np.stack([np.zeros(n), eig, rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)], axis=1)
+    spec = [{"kind": "iteration", "max_iters": 500},
+            {"kind": "residual_norm", "reduction_factor": 1e-10}]
+    xo, so, io = ora.cg_solve(rp, ci, v, b_h, np.zeros_like(b_h), spec)
+    A = gko.csr_from_host(exec_, rp, ci, v)
+    s = gko.solver.Cg.build("f64").with_criteria(*crits(gko, spec, "f64")).on(exec_).generate(A)
+    x = gko.matrix.Dense("f64").create(exec_, n, 4)
+    x.fill(0.0)
+    s.apply(dense(gko, exec_, b_h, "f64"), x)
+    assert s.last_used_fused_path
+    assert s.last_status == [int(c) for c in so]
+    assert abs(s.last_iterations - io.iterations) <= 3
+    xh = x.to_numpy()
+    assert np.all(xh[:, 0] == 0.0)
+    np.testing.assert_allclose(xh, xo, atol=1e-9)
+
+
 def test_cg_many_rhs_generic_path(gko, exec_):
+    """A non-Csr system matrix (Ell) takes the generic reference-order loop."""
     rp, ci, v = gen.poisson2d(48)
     n = 48 * 48
     rng = np.random.default_rng(3)
@@ -216,7 +280,7 @@ def test_cg_many_rhs_generic_path(gko, exec_):
     spec = [{"kind": "iteration", "max_iters": 1000},
             {"kind": "residual_norm", "reduction_factor": 1e-10}]
     xo, so, io = ora.cg_solve(rp, ci, v, b_h, np.zeros_like(b_h), spec)
-    A = gko.csr_from_host(exec_, rp, ci, v)
+    A = gko.csr_from_host(exec_, rp, ci, v).to_ell()
     s = gko.solver.Cg.build("f64").with_criteria(*crits(gko, spec, "f64")).on(exec_).generate(A)
     x = gko.matrix.Dense("f64").create(exec_, n, 5)
     x.fill(0.0)
