@@ -180,6 +180,15 @@ def bench_batch(dev, num=20000):
             t, (it, rs) = timed(lambda: A.solve(bd, x, st, plan))
             times.append(t)
         key = "f64" if dt == np.float64 else "f32"
+        # the same solve as one captured HIP graph (x reset inside the graph)
+        xg = torch.zeros_like(bd)
+        graph, _, _ = A.capture(bd, xg, st, plan)
+        gtimes = []
+        for _ in range(3):
+            t, _ = timed(graph.replay)
+            gtimes.append(t)
+        out[f"uneven_{key}_graph"] = {"time_s": round(min(gtimes), 5),
+                                      "solves_per_s": round(num / min(gtimes), 1)}
         out[f"uneven_{key}"] = {"time_s": round(min(times), 5),
                                 "solves_per_s": round(num / min(times), 1),
                                 "max_iterations": int(it.max().item()),

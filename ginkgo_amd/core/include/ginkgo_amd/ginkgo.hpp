@@ -8,3 +8,4 @@
 #include "ginkgo_amd/solver.hpp"
 #include "ginkgo_amd/stop.hpp"
 #include "ginkgo_amd/mtx_io.hpp"
+#include "ginkgo_amd/batch.hpp"

@@ -77,6 +77,98 @@ py::array_t<T> arr(const std::vector<T>& v)
     return out;
 }
 
+// The batch module (batch::MultiVector / matrix::Csr / matrix::BlockDiagCsr /
+// solver::Bicgstab / log::BatchConvergence).
+template <typename V>
+void bind_batch(py::module_& m, const std::string& sfx)
+{
+    using MV = batch::MultiVector<V>;
+    using BCsr = batch::matrix::Csr<V, int32>;
+    using BDiag = batch::matrix::BlockDiagCsr<V, int32>;
+    using BJ = batch::preconditioner::Jacobi<V>;
+    using BS = batch::solver::Bicgstab<V>;
+    using BL = batch::log::BatchConvergence<V>;
+    py::class_<MV, batch::BatchLinOp, std::shared_ptr<MV>>(m, ("BatchMultiVector" + sfx).c_str())
+        .def_static("create",
+                    [](std::shared_ptr<const Executor> e, size_type num, size_type n) {
+                        return std::shared_ptr<MV>(
+                            MV::create(e, batch_dim<2>{num, dim<2>{n, 1}}));
+                    })
+        .def_static("from_numpy",
+                    [](std::shared_ptr<const Executor> e, size_type num, size_type n,
+                       py::array_t<V, py::array::c_style | py::array::forcecast> a) {
+                        return std::shared_ptr<MV>(MV::create_from_host(e, num, n, vec<V>(a)));
+                    })
+        .def_static("uneven",
+                    [](std::shared_ptr<const Executor> e,
+                       py::array_t<int64, py::array::c_style | py::array::forcecast> off,
+                       py::object values) {
+                        const auto o = vec<int64>(off);
+                        if (values.is_none()) {
+                            return std::shared_ptr<MV>(MV::create_uneven(e, o));
+                        }
+                        const auto v = vec<V>(values.cast<py::array_t<
+                            V, py::array::c_style | py::array::forcecast>>());
+                        return std::shared_ptr<MV>(MV::create_uneven(e, o, &v));
+                    },
+                    py::arg("exec"), py::arg("row_offsets"), py::arg("values") = py::none())
+        .def("to_numpy", [](const MV& x) { return arr(x.to_host()); })
+        .def("fill", &MV::fill)
+        .def_property_readonly("total_rows", &MV::get_total_rows);
+    py::class_<BCsr, batch::BatchLinOp, std::shared_ptr<BCsr>>(m, ("BatchCsr" + sfx).c_str())
+        .def_static("from_numpy",
+                    [](std::shared_ptr<const Executor> e, size_type num, size_type n,
+                       py::array_t<int32, py::array::c_style | py::array::forcecast> rp,
+                       py::array_t<int32, py::array::c_style | py::array::forcecast> ci,
+                       py::array_t<V, py::array::c_style | py::array::forcecast> v) {
+                        return std::shared_ptr<BCsr>(BCsr::create_from_host(
+                            e, num, n, vec<int32>(rp), vec<int32>(ci), vec<V>(v)));
+                    });
+    py::class_<BDiag, batch::BatchLinOp, std::shared_ptr<BDiag>>(
+        m, ("BatchBlockDiagCsr" + sfx).c_str())
+        .def_static("from_numpy",
+                    [](std::shared_ptr<const Executor> e,
+                       py::array_t<int64, py::array::c_style | py::array::forcecast> off,
+                       py::array_t<int32, py::array::c_style | py::array::forcecast> rp,
+                       py::array_t<int32, py::array::c_style | py::array::forcecast> ci,
+                       py::array_t<V, py::array::c_style | py::array::forcecast> v) {
+                        return std::shared_ptr<BDiag>(BDiag::create_from_host(
+                            e, vec<int64>(off), vec<int32>(rp), vec<int32>(ci), vec<V>(v)));
+                    });
+    py::class_<typename BJ::parameters_type>(m, ("BatchJacobiParams" + sfx).c_str())
+        .def(py::init<>())
+        .def("with_max_block_size", [](typename BJ::parameters_type p, unsigned b) {
+            return p.with_max_block_size(b);
+        });
+    py::class_<BL, std::shared_ptr<BL>>(m, ("BatchConvergence" + sfx).c_str())
+        .def_static("create", &BL::create)
+        .def_property_readonly("num_iterations",
+                               [](const BL& l) { return arr(l.get_num_iterations()); })
+        .def_property_readonly("residual_norm",
+                               [](const BL& l) { return arr(l.get_residual_norm()); });
+    py::class_<BS, std::shared_ptr<BS>>(m, ("BatchBicgstab" + sfx).c_str())
+        .def("apply", [](const BS& s, const MV* b, MV* x) { s.apply(b, x); })
+        .def("tune", [](BS& s, const MV* b, const MV* x) { s.tune(b, x); })
+        .def("add_logger", [](BS& s, std::shared_ptr<const BL> l) { s.add_logger(l); })
+        .def_property_readonly("block_sizes", &BS::get_block_sizes)
+        .def_property_readonly("matrix_in_lds", &BS::get_matrix_in_lds);
+    py::class_<typename BS::Factory, std::shared_ptr<typename BS::Factory>>(
+        m, ("BatchBicgstabFactory" + sfx).c_str())
+        .def("generate", [](const typename BS::Factory& f, std::shared_ptr<const batch::BatchLinOp> a) {
+            return std::shared_ptr<BS>(f.generate(std::move(a)));
+        });
+    using BP = typename BS::parameters_type;
+    py::class_<BP>(m, ("BatchBicgstabParams" + sfx).c_str())
+        .def(py::init<>())
+        .def("with_max_iterations", [](BP p, int v) { return p.with_max_iterations(v); })
+        .def("with_tolerance", [](BP p, double v) { return p.with_tolerance(v); })
+        .def("with_tolerance_type",
+             [](BP p, batch::stop::tolerance_type t) { return p.with_tolerance_type(t); })
+        .def("with_preconditioner",
+             [](BP p, const typename BJ::parameters_type& j) { return p.with_preconditioner(j); })
+        .def("on", [](const BP& p, std::shared_ptr<const Executor> e) { return p.on(e); });
+}
+
 template <typename V>
 void bind_value_type(py::module_& m, const std::string& sfx)
 {
@@ -347,4 +439,11 @@ PYBIND11_MODULE(_gko, m)
 
     bind_value_type<double>(m, "_f64");
     bind_value_type<float>(m, "_f32");
+    py::enum_<batch::stop::tolerance_type>(m, "batch_tolerance_type")
+        .value("absolute", batch::stop::tolerance_type::absolute)
+        .value("relative", batch::stop::tolerance_type::relative);
+    py::class_<batch::BatchLinOp, std::shared_ptr<batch::BatchLinOp>>(m, "BatchLinOp")
+        .def_property_readonly("num_batch_items", &batch::BatchLinOp::get_num_batch_items);
+    bind_batch<double>(m, "_f64");
+    bind_batch<float>(m, "_f32");
 }

@@ -1,0 +1,349 @@
+// The batch module on the Mi355xExecutor: storage, the size-binned launch
+// plan, and batch::solver::Bicgstab over gkomi_batch_bicgstab_{solve,tune}.
+#include "ginkgo_amd/batch.hpp"
+
+#include <algorithm>
+#include <numeric>
+
+namespace gko {
+namespace batch {
+
+// ---------------------------------------------------------- MultiVector ----
+
+template <typename V>
+std::unique_ptr<MultiVector<V>> MultiVector<V>::create(std::shared_ptr<const Executor> exec,
+                                                       batch_dim<2> size)
+{
+    if (size.common_size.cols != 1) {
+        GKO_NOT_SUPPORTED_MSG("batch::MultiVector: one right-hand side per item");
+    }
+    const size_type total = size.num_batch_items * size.common_size.rows;
+    return std::unique_ptr<MultiVector>(new MultiVector(exec, size, total));
+}
+
+template <typename V>
+std::unique_ptr<MultiVector<V>> MultiVector<V>::create_from_host(
+    std::shared_ptr<const Executor> exec, size_type num_items, size_type n,
+    const std::vector<V>& values)
+{
+    if (values.size() != num_items * n) {
+        GKO_DIMENSION_MISMATCH_MSG("batch::MultiVector: values do not match num_items * n");
+    }
+    auto mv = create(exec, batch_dim<2>{num_items, dim<2>{n, 1}});
+    if (!values.empty()) {
+        exec->copy_from(exec->get_master().get(), values.size(), values.data(), mv->get_values());
+    }
+    return mv;
+}
+
+template <typename V>
+std::unique_ptr<MultiVector<V>> MultiVector<V>::create_uneven(
+    std::shared_ptr<const Executor> exec, const std::vector<int64>& row_offsets,
+    const std::vector<V>* values)
+{
+    if (row_offsets.empty() || row_offsets.front() != 0) {
+        GKO_DIMENSION_MISMATCH_MSG("batch::MultiVector: row offsets must start at 0");
+    }
+    size_type max_rows = 0;
+    for (size_type k = 0; k + 1 < row_offsets.size(); ++k) {
+        if (row_offsets[k + 1] < row_offsets[k]) {
+            GKO_DIMENSION_MISMATCH_MSG("batch::MultiVector: decreasing row offsets");
+        }
+        max_rows = std::max<size_type>(max_rows, row_offsets[k + 1] - row_offsets[k]);
+    }
+    const size_type total = static_cast<size_type>(row_offsets.back());
+    std::unique_ptr<MultiVector> mv(new MultiVector(
+        exec, batch_dim<2>{row_offsets.size() - 1, dim<2>{max_rows, 1}}, total));
+    if (values) {
+        if (values->size() != total) {
+            GKO_DIMENSION_MISMATCH_MSG("batch::MultiVector: values do not match the offsets");
+        }
+        if (total) {
+            exec->copy_from(exec->get_master().get(), total, values->data(), mv->get_values());
+        }
+    }
+    return mv;
+}
+
+namespace matrix {
+
+// ------------------------------------------------------------------ Csr ----
+
+template <typename V, typename I>
+std::unique_ptr<Csr<V, I>> Csr<V, I>::create_from_host(std::shared_ptr<const Executor> exec,
+                                                       size_type num_items, size_type n,
+                                                       const std::vector<I>& row_ptrs,
+                                                       const std::vector<I>& col_idxs,
+                                                       const std::vector<V>& values)
+{
+    if (row_ptrs.size() != n + 1 || static_cast<size_type>(row_ptrs.back()) != col_idxs.size() ||
+        values.size() != num_items * col_idxs.size()) {
+        GKO_DIMENSION_MISMATCH_MSG("batch::matrix::Csr: inconsistent pattern or values");
+    }
+    std::unique_ptr<Csr> m(new Csr(exec, batch_dim<2>{num_items, dim<2>{n, n}}));
+    m->row_ptrs_ = array<I>(exec, row_ptrs);
+    m->col_idxs_ = array<I>(exec, col_idxs);
+    m->values_ = array<V>(exec, values);
+    return m;
+}
+
+template <typename V, typename I>
+gkomi_batch_csr Csr<V, I>::descriptor() const
+{
+    gkomi_batch_csr d{};
+    d.num_systems = static_cast<int64_t>(size_.num_batch_items);
+    d.max_rows = static_cast<int64_t>(size_.common_size.rows);
+    d.max_nnz = static_cast<int64_t>(col_idxs_.get_size());
+    d.row_offsets = nullptr;
+    d.row_ptrs = row_ptrs_.get_const_data();
+    d.col_idxs = col_idxs_.get_const_data();
+    d.values = values_.get_const_data();
+    d.uniform_rows = d.max_rows;
+    d.uniform_nnz = d.max_nnz;
+    return d;
+}
+
+// --------------------------------------------------------- BlockDiagCsr ----
+
+template <typename V, typename I>
+std::unique_ptr<BlockDiagCsr<V, I>> BlockDiagCsr<V, I>::create_from_host(
+    std::shared_ptr<const Executor> exec, const std::vector<int64>& row_offsets,
+    const std::vector<I>& row_ptrs, const std::vector<I>& col_idxs, const std::vector<V>& values)
+{
+    if (row_offsets.empty() || row_offsets.front() != 0 ||
+        row_ptrs.size() != static_cast<size_type>(row_offsets.back()) + 1 ||
+        static_cast<size_type>(row_ptrs.back()) != col_idxs.size() ||
+        values.size() != col_idxs.size()) {
+        GKO_DIMENSION_MISMATCH_MSG("batch::matrix::BlockDiagCsr: inconsistent arrays");
+    }
+    const size_type num = row_offsets.size() - 1;
+    std::vector<int64> rows(num), nnz(num);
+    int64 max_rows = 0;
+    for (size_type k = 0; k < num; ++k) {
+        rows[k] = row_offsets[k + 1] - row_offsets[k];
+        if (rows[k] < 0) GKO_DIMENSION_MISMATCH_MSG("BlockDiagCsr: decreasing row offsets");
+        nnz[k] = row_ptrs[row_offsets[k + 1]] - row_ptrs[row_offsets[k]];
+        for (int64 r = row_offsets[k]; r < row_offsets[k + 1]; ++r) {
+            for (I p = row_ptrs[r]; p < row_ptrs[r + 1]; ++p) {
+                if (col_idxs[p] < 0 || col_idxs[p] >= rows[k]) {
+                    GKO_DIMENSION_MISMATCH_MSG(
+                        "BlockDiagCsr: column index outside its item (indices are local)");
+                }
+            }
+        }
+        max_rows = std::max(max_rows, rows[k]);
+    }
+    std::unique_ptr<BlockDiagCsr> m(new BlockDiagCsr(
+        exec, batch_dim<2>{num, dim<2>{static_cast<size_type>(max_rows),
+                                       static_cast<size_type>(max_rows)}}));
+    m->row_offsets_ = array<int64>(exec, row_offsets);
+    m->row_ptrs_ = array<I>(exec, row_ptrs);
+    m->col_idxs_ = array<I>(exec, col_idxs);
+    m->values_ = array<V>(exec, values);
+    m->row_offsets_host_ = row_offsets;
+    m->item_rows_ = std::move(rows);
+    m->item_nnz_ = std::move(nnz);
+    return m;
+}
+
+template <typename V, typename I>
+gkomi_batch_csr BlockDiagCsr<V, I>::descriptor() const
+{
+    gkomi_batch_csr d{};
+    d.num_systems = static_cast<int64_t>(size_.num_batch_items);
+    d.max_rows = static_cast<int64_t>(size_.common_size.rows);
+    d.max_nnz = item_nnz_.empty() ? 0 : *std::max_element(item_nnz_.begin(), item_nnz_.end());
+    d.row_offsets = row_offsets_.get_const_data();
+    d.row_ptrs = row_ptrs_.get_const_data();
+    d.col_idxs = col_idxs_.get_const_data();
+    d.values = values_.get_const_data();
+    d.uniform_rows = 0;
+    d.uniform_nnz = 0;
+    return d;
+}
+
+}  // namespace matrix
+
+// ------------------------------------------------------------- Bicgstab ----
+
+namespace solver {
+
+namespace {
+
+template <typename V>
+struct bk;
+template <>
+struct bk<double> {
+    static constexpr auto solve = gkomi_batch_bicgstab_solve_f64;
+    static constexpr auto tune = gkomi_batch_bicgstab_tune_f64;
+};
+template <>
+struct bk<float> {
+    static constexpr auto solve = gkomi_batch_bicgstab_solve_f32;
+    static constexpr auto tune = gkomi_batch_bicgstab_tune_f32;
+};
+
+// Size bins (rows in (lo, hi]) of the launch plan, largest first.
+constexpr int64 kBinEdges[] = {int64{1} << 31, 512, 256, 128, 64, 0};
+
+}  // namespace
+
+template <typename V>
+std::unique_ptr<Bicgstab<V>> Bicgstab<V>::Factory::generate(
+    std::shared_ptr<const BatchLinOp> a) const
+{
+    return std::unique_ptr<Bicgstab>(new Bicgstab(exec_, params_, std::move(a)));
+}
+
+template <typename V>
+Bicgstab<V>::Bicgstab(std::shared_ptr<const Executor> exec, parameters_type params,
+                      std::shared_ptr<const BatchLinOp> a)
+    : exec_{std::move(exec)},
+      params_{std::move(params)},
+      system_{std::move(a)},
+      order_(exec_),
+      workspace_(exec_),
+      iterations_(exec_),
+      residuals_(exec_)
+{
+    if (!system_) GKO_NOT_SUPPORTED_MSG("batch::solver::Bicgstab: no system matrix");
+    if (params_.max_iterations < 0 || params_.tolerance < 0) {
+        throw InvalidStateError(__FILE__, __LINE__,
+                                "batch::solver::Bicgstab: negative tolerance or iterations");
+    }
+    if (system_->get_common_size().rows != system_->get_common_size().cols) {
+        GKO_DIMENSION_MISMATCH_MSG("batch::solver::Bicgstab: items must be square");
+    }
+    std::vector<int64> rows, nnz;
+    const size_type num = system_->get_num_batch_items();
+    if (auto u = dynamic_cast<const matrix::Csr<V, int32>*>(system_.get())) {
+        desc_ = u->descriptor();
+        rows.assign(num, desc_.uniform_rows);
+        nnz.assign(num, desc_.uniform_nnz);
+        total_rows_ = num * static_cast<size_type>(desc_.uniform_rows);
+    } else if (auto b = dynamic_cast<const matrix::BlockDiagCsr<V, int32>*>(system_.get())) {
+        desc_ = b->descriptor();
+        rows = b->get_item_rows();
+        nnz = b->get_item_nnz();
+        total_rows_ = static_cast<size_type>(b->get_row_offsets().back());
+    } else {
+        GKO_NOT_SUPPORTED_MSG(
+            "batch::solver::Bicgstab: system must be batch::matrix::Csr or BlockDiagCsr");
+    }
+    // plan: items sorted by size (largest first), split into bins
+    std::vector<int32> order(num);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32 x, int32 y) { return rows[x] > rows[y]; });
+    order_ = array<int32>(exec_, order);
+    plan_ = gkomi_batch_plan{};
+    plan_.order = order_.get_const_data();
+    size_type begin = 0;
+    int nb = 0;
+    for (int e = 0; e + 1 < static_cast<int>(sizeof(kBinEdges) / sizeof(kBinEdges[0])); ++e) {
+        size_type end = begin;
+        while (end < num && rows[order[end]] > kBinEdges[e + 1]) ++end;
+        if (end > begin) {
+            int64 mr = 0, mz = 0;
+            for (size_type k = begin; k < end; ++k) {
+                mr = std::max(mr, rows[order[k]]);
+                mz = std::max(mz, nnz[order[k]]);
+            }
+            plan_.bin_begin[nb] = static_cast<int64_t>(begin);
+            plan_.bin_max_rows[nb] = mr;
+            plan_.bin_max_nnz[nb] = mz;
+            plan_.block_size[nb] = mr > 256 ? 256 : (mr > 64 ? 128 : 64);
+            plan_.matrix_in_lds[nb] = -1;
+            plan_.blocks_per_cu[nb] = 0;
+            ++nb;
+            begin = end;
+        }
+    }
+    if (nb == 0) {
+        plan_.bin_max_rows[0] = 0;
+        plan_.bin_max_nnz[0] = 0;
+        plan_.block_size[0] = 64;
+        nb = 1;
+    }
+    plan_.num_bins = nb;
+    plan_.bin_begin[nb] = static_cast<int64_t>(num);
+    workspace_ = array<char>(exec_, gkomi_batch_workspace_bytes(&desc_));
+    workspace_.fill(0);
+    iterations_ = array<int32>(exec_, num);
+    residuals_ = array<V>(exec_, num);
+}
+
+template <typename V>
+gkomi_batch_settings Bicgstab<V>::settings() const
+{
+    gkomi_batch_settings s{};
+    s.max_iterations = params_.max_iterations;
+    s.tolerance_type = params_.tolerance_type == stop::tolerance_type::relative ? 1 : 0;
+    s.tolerance = params_.tolerance;
+    s.preconditioner = params_.preconditioner ? 1 : 0;
+    return s;
+}
+
+template <typename V>
+void Bicgstab<V>::check_vectors(const MultiVector<V>* b, const MultiVector<V>* x) const
+{
+    if (!b || !x) GKO_NOT_SUPPORTED_MSG("batch::solver::Bicgstab: null vector");
+    if (b->get_num_batch_items() != system_->get_num_batch_items() ||
+        x->get_num_batch_items() != system_->get_num_batch_items() ||
+        b->get_total_rows() != total_rows_ || x->get_total_rows() != total_rows_) {
+        GKO_DIMENSION_MISMATCH_MSG("batch::solver::Bicgstab: vectors do not match the system");
+    }
+}
+
+template <typename V>
+void Bicgstab<V>::apply(const MultiVector<V>* b, MultiVector<V>* x) const
+{
+    check_vectors(b, x);
+    const auto s = settings();
+    GKO_CALL(bk<V>::solve(ctx_of(exec_), &desc_, &s, &plan_, b->get_const_values(),
+                          x->get_values(), iterations_.get_data(), residuals_.get_data(),
+                          workspace_.get_data(), workspace_.get_size()));
+    if (!loggers_.empty()) {
+        auto it = iterations_.to_host();
+        auto res = residuals_.to_host();
+        for (auto& l : loggers_) l->record(it, res);
+    }
+}
+
+template <typename V>
+void Bicgstab<V>::tune(const MultiVector<V>* b, const MultiVector<V>* x)
+{
+    check_vectors(b, x);
+    const auto s = settings();
+    array<V> scratch(exec_, total_rows_);
+    GKO_CALL(bk<V>::tune(ctx_of(exec_), &desc_, &s, &plan_, b->get_const_values(),
+                         x->get_const_values(), scratch.get_data(), iterations_.get_data(),
+                         residuals_.get_data(), workspace_.get_data(), workspace_.get_size()));
+}
+
+template <typename V>
+std::vector<int> Bicgstab<V>::get_block_sizes() const
+{
+    return std::vector<int>(plan_.block_size, plan_.block_size + plan_.num_bins);
+}
+
+template <typename V>
+std::vector<int> Bicgstab<V>::get_matrix_in_lds() const
+{
+    return std::vector<int>(plan_.matrix_in_lds, plan_.matrix_in_lds + plan_.num_bins);
+}
+
+template class Bicgstab<double>;
+template class Bicgstab<float>;
+
+}  // namespace solver
+
+template class MultiVector<double>;
+template class MultiVector<float>;
+template class matrix::Csr<double, int32>;
+template class matrix::Csr<float, int32>;
+template class matrix::BlockDiagCsr<double, int32>;
+template class matrix::BlockDiagCsr<float, int32>;
+
+}  // namespace batch
+}  // namespace gko

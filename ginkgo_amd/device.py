@@ -459,6 +459,33 @@ class DeviceBatchCsr:
                ptr(res), ptr(ws), ws.numel())
         return iters, res
 
+    def capture(self, b, x, settings, plan=None, x0=None):
+        """The whole batched solve -- x = x0 (zero when x0 is None), then every
+        bin's persistent launch -- captured in one HIP graph on a side
+        stream.  Returns (graph, iterations, residual norms); graph.replay()
+        re-runs it on the same buffers.  The work-queue counters are reset by
+        the kernels themselves, and x is set by a fill/copy kernel, so the
+        graph holds no memset node."""
+        iters = torch.empty(self.num_systems, dtype=torch.int32, device=self.device)
+        res = torch.empty(self.num_systems, dtype=self.dtype, device=self.device)
+        ws = self.workspace()
+        sfx = suffix(self.dtype)
+        side = torch.cuda.Stream(device=self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=side):
+            if x0 is None:
+                x.zero_()
+            else:
+                x.copy_(x0)
+            c = ctx(self.device, side)
+            C.call(f"gkomi_batch_bicgstab_solve_{sfx}", ctypes.byref(c),
+                   ctypes.byref(self.desc), ctypes.byref(settings),
+                   ctypes.byref(plan) if plan is not None else None, ptr(b), ptr(x),
+                   ptr(iters), ptr(res), ptr(ws), ws.numel())
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        return graph, iters, res
+
     def tune(self, b, x, settings, plan):
         """Autotune block size and LDS residency per bin (writes into plan)."""
         xs = torch.empty_like(x)

@@ -65,6 +65,20 @@ solver = types.SimpleNamespace(
 
 log = types.SimpleNamespace(Convergence=_g.Convergence)
 
+# batch module: batch.matrix.Csr (shared pattern), batch.matrix.BlockDiagCsr
+# (items of different sizes), batch.MultiVector, batch.solver.Bicgstab,
+# batch.preconditioner.Jacobi, batch.log.BatchConvergence, batch.stop.tolerance_type
+batch = types.SimpleNamespace(
+    MultiVector=_typed("BatchMultiVector"),
+    matrix=types.SimpleNamespace(Csr=_typed("BatchCsr"), BlockDiagCsr=_typed("BatchBlockDiagCsr")),
+    stop=types.SimpleNamespace(tolerance_type=_g.batch_tolerance_type),
+    preconditioner=types.SimpleNamespace(
+        Jacobi=types.SimpleNamespace(build=lambda dtype="f64": _typed("BatchJacobiParams")(dtype)())),
+    solver=types.SimpleNamespace(
+        Bicgstab=types.SimpleNamespace(build=lambda dtype="f64": _typed("BatchBicgstabParams")(dtype)())),
+    log=types.SimpleNamespace(BatchConvergence=_typed("BatchConvergence")),
+)
+
 
 def dense_view(exec, t):
     """Non-owning Dense view of a 1-D or 2-D device tensor (kept alive by the caller)."""

@@ -55,3 +55,20 @@ def test_host_library_has_no_cpu_compute_path():
               .with_criteria(gko.stop.Iteration.build().with_max_iters(5),
                              gko.stop.ResidualNorm.build("f64").with_reduction_factor(1e-8)))
     assert params is not None
+
+
+def test_host_batch_api_has_no_cpu_compute_path():
+    """batch::solver::Bicgstab on the host executor: storage and the launch
+    plan are built, the solve refuses (no CPU fallback)."""
+    import numpy as np
+    import ginkgo_amd.gko as gko
+    host = gko.HostExecutor.create()
+    rp = np.array([0, 1, 2], np.int32)
+    A = gko.batch.matrix.Csr("f64").from_numpy(host, 2, 2, rp, np.array([0, 1], np.int32),
+                                               np.ones(4))
+    s = gko.batch.solver.Bicgstab.build("f64").with_tolerance(1e-8).on(host).generate(A)
+    assert s.block_sizes == [64]
+    b = gko.batch.MultiVector("f64").from_numpy(host, 2, 2, np.ones(4))
+    x = gko.batch.MultiVector("f64").create(host, 2, 2)
+    with pytest.raises(gko.NotSupported):
+        s.apply(b, x)

@@ -145,3 +145,116 @@ def test_tuned_plan_and_padded_uniform_agree(dev):
     for s in range(200):
         np.testing.assert_allclose(uxh[s, :sizes[s]], xh[row_off[s]:row_off[s + 1]], atol=1e-8)
         assert np.all(uxh[s, sizes[s]:] == 0)
+
+
+# ------------------------------------------------- the gko:: batch API ----
+# batch::solver::Bicgstab<V>::build().with_*(...).on(exec)->generate(A)->apply(b, x)
+# through the C++ library, as reference/test/solver/batch_bicgstab_kernels.cpp
+# drives it.
+
+@pytest.fixture(scope="module")
+def gko_exec(dev):
+    import ginkgo_amd.gko as gko
+    return gko, gko.Mi355xExecutor.create(0, torch.cuda.current_stream().cuda_stream)
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+def test_api_reference_stencil_kat(gko_exec, dt):
+    gko, ex = gko_exec
+    k = KATS["batch_bicgstab"][0]
+    row_off, rp, ci, v, b, xs, n = stencil_batch(k)
+    npdt = np.float64 if dt == "f64" else np.float32
+    nb = k["num_batch_items"]
+    A = gko.batch.matrix.Csr(dt).from_numpy(ex, nb, n, rp[: n + 1], ci[: rp[n]],
+                                            v.astype(npdt))
+    solver = (gko.batch.solver.Bicgstab.build(dt)
+              .with_max_iterations(k["max_iterations"])
+              .with_tolerance(k["tolerance"])
+              .with_tolerance_type(gko.batch.stop.tolerance_type.relative)
+              .on(ex).generate(A))
+    logger = gko.batch.log.BatchConvergence(dt).create()
+    solver.add_logger(logger)
+    bb = gko.batch.MultiVector(dt).from_numpy(ex, nb, n, b.astype(npdt))
+    x = gko.batch.MultiVector(dt).create(ex, nb, n)
+    x.fill(0.0)
+    solver.apply(bb, x)
+    xh = x.to_numpy().astype(np.float64)
+    d = gen.csr_to_dense(rp[: n + 1], ci[: rp[n]], v[: rp[n]])
+    for s in range(nb):
+        bs = b[s * n:(s + 1) * n]
+        assert np.linalg.norm(bs - d @ xh[s * n:(s + 1) * n]) / np.linalg.norm(bs) <= \
+            2 * k["tolerance"]
+        assert np.linalg.norm(xh[s * n:(s + 1) * n] - xs) / np.linalg.norm(xs) <= \
+            10 * k["tolerance"]
+    assert len(logger.num_iterations) == nb and np.all(logger.num_iterations > 0)
+    assert np.all(logger.residual_norm / np.linalg.norm(b[:n]) <= k["tolerance"] * 1.0001)
+
+
+def test_api_uneven_block_diagonal_against_oracle(gko_exec):
+    gko, ex = gko_exec
+    row_off, rp, ci, v, b = gen.uneven_batch(150, n_min=16, n_max=300, seed=9)
+    A = gko.batch.matrix.BlockDiagCsr("f64").from_numpy(ex, row_off, rp, ci, v)
+    solver = (gko.batch.solver.Bicgstab.build("f64").with_max_iterations(500)
+              .with_tolerance(1e-10)
+              .with_tolerance_type(gko.batch.stop.tolerance_type.relative)
+              .with_preconditioner(gko.batch.preconditioner.Jacobi.build("f64"))
+              .on(ex).generate(A))
+    logger = gko.batch.log.BatchConvergence("f64").create()
+    solver.add_logger(logger)
+    bb = gko.batch.MultiVector("f64").uneven(ex, row_off, b)
+    x = gko.batch.MultiVector("f64").uneven(ex, row_off, np.zeros_like(b))
+    solver.tune(bb, x)
+    assert all(bs in (64, 128, 256, 512) for bs in solver.block_sizes)
+    solver.apply(bb, x)
+    xo, io, ro = ora.batch_bicgstab_solve(row_off, rp, ci, v, b, np.zeros_like(b),
+                                          use_jacobi=True, tol=1e-10, max_iters=500)
+    xh = x.to_numpy()
+    for s in range(150):
+        r0, r1 = row_off[s], row_off[s + 1]
+        bs = b[r0:r1]
+        d = gen.csr_to_dense(rp[r0:r1 + 1] - rp[r0], ci[rp[r0]:rp[r1]], v[rp[r0]:rp[r1]])
+        assert np.linalg.norm(bs - d @ xh[r0:r1]) / np.linalg.norm(bs) <= 2e-10
+    assert np.max(np.abs(logger.num_iterations - io)) <= 2
+
+
+def test_api_errors(gko_exec):
+    gko, ex = gko_exec
+    with pytest.raises(gko.NotSupported):
+        gko.batch.solver.Bicgstab.build("f64").with_preconditioner(
+            gko.batch.preconditioner.Jacobi.build("f64").with_max_block_size(4))
+    rp = np.array([0, 1, 3], dtype=np.int32)
+    with pytest.raises(gko.DimensionMismatch):   # column 2 is outside a 2-row item
+        gko.batch.matrix.BlockDiagCsr("f64").from_numpy(
+            ex, np.array([0, 2]), rp, np.array([0, 0, 2], dtype=np.int32), np.ones(3))
+    A = gko.batch.matrix.Csr("f64").from_numpy(ex, 3, 2, rp, np.array([0, 0, 1], np.int32),
+                                               np.ones(9))
+    s = gko.batch.solver.Bicgstab.build("f64").on(ex).generate(A)
+    with pytest.raises(gko.DimensionMismatch):
+        s.apply(gko.batch.MultiVector("f64").create(ex, 2, 2),
+                gko.batch.MultiVector("f64").create(ex, 2, 2))
+
+
+def test_graph_captured_batch_solve_replays(dev):
+    """The whole uneven batched solve captured in one HIP graph: every replay
+    (with eager work and synchronisation between replays) reproduces the
+    eager solve bit for bit -- each system is solved by one workgroup in a
+    fixed order, and the queue counters are reset by the kernels."""
+    row_off, rp, ci, v, b = gen.uneven_batch(300, n_min=16, n_max=400, seed=21)
+    A = D().DeviceBatchCsr(row_off, rp, ci, v, device=dev)
+    bd = torch.as_tensor(b).to(dev)
+    st = A.settings(tol=1e-10, relative=True, max_iters=500, jacobi=True)
+    plan = A.plan()
+    x_eager = torch.zeros_like(bd)
+    it_eager, res_eager = A.solve(bd, x_eager, st, plan)
+    x = torch.full_like(bd, 7.0)
+    graph, it_g, res_g = A.capture(bd, x, st, plan)
+    for rep in range(3):
+        # unrelated eager work on the same buffers' neighbours between replays
+        scratch = torch.randn(1 << 16, dtype=torch.float64, device=dev)
+        scratch.mul_(2.0).sum()
+        x.fill_(-3.0)
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(x, x_eager), rep
+        assert torch.equal(it_g, it_eager), rep
+        assert torch.equal(res_g, res_eager), rep
