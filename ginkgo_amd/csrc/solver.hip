@@ -477,6 +477,76 @@ __global__ void __launch_bounds__(kBlock)
     store_partials<T, 1>(acc, w.part_dir);
 }
 
+// Split form of kernel 1 for rows with many entries, where recomputing p at
+// every gathered column (two gathers per entry) costs more than one extra
+// pass over p.  (a) the check of iteration `iter` exactly as in the fused
+// direction kernel, then p_out = z + rho/prev_rho p_in elementwise;
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    cg_split_p_kernel(int64_t n, const T* __restrict__ z, const T* __restrict__ p_in,
+                      T* __restrict__ p_out, gkomi_criteria crit, T* __restrict__ scalars,
+                      const T* __restrict__ orig, uint8_t* __restrict__ status,
+                      gkomi_stop_flags* __restrict__ flags, cg_work w, int count_in,
+                      int64_t iter, int par)
+{
+    __shared__ int stopped;
+    if (threadIdx.x == 0) stopped = flags->all_stopped;
+    __syncthreads();
+    if (stopped) return;
+    double s[2];
+    combine_partials<2>(w.part_upd, count_in, s);
+    const T rho = static_cast<T>(s[0]);
+    const T tau = static_cast<T>(sqrt(s[1]));
+    const T prev = scalars[par ^ 1];
+    uint8_t st = w.status[par ^ 1];
+    int all = 0, changed = 0;
+    combined_check<T>(crit, iter, 1, &tau, &rho, orig, true, &st, &all, &changed);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scalars[par] = rho;
+        scalars[3 + par] = tau;
+        w.status[par] = st;
+        status[0] = st;
+        flags->iteration = iter;
+        flags->all_stopped = all;
+        flags->one_changed = changed;
+    }
+    if (all) return;
+    const bool restart = prev == T(0);
+    const T tmp = restart ? T(0) : rho / prev;
+    int64_t i0, i1;
+    block_range(ceildiv(n, kBlock), &i0, &i1);
+    for (int64_t blk = i0; blk < i1; ++blk) {
+        const int64_t i = blk * kBlock + threadIdx.x;
+        if (i < n) p_out[i] = restart ? z[i] : fma(tmp, p_in[i], z[i]);
+    }
+}
+
+// (b) q = A p (one gather per entry) and the partials of p.q.
+template <typename T, int SW, int RPG>
+__global__ void __launch_bounds__(kBlock)
+    cg_split_spmv_kernel(int64_t n, const int32_t* __restrict__ rp,
+                         const int32_t* __restrict__ ci, const T* __restrict__ v,
+                         const T* __restrict__ p, T* __restrict__ q,
+                         const gkomi_stop_flags* __restrict__ flags, cg_work w)
+{
+    if (flags->all_stopped) return;
+    int64_t rb0, rb1;
+    block_range(ceildiv(n, subwave_tile_rows<SW, RPG>), &rb0, &rb1);
+    T acc[1] = {T(0)};
+    for (int64_t blk = rb0; blk < rb1; ++blk) {
+        subwave_tile<T, SW, RPG>(
+            n, blk, rp, ci, v, [&](int32_t c) { return p[c]; },
+            [&](int64_t row, T sum) {
+                q[row] = sum;
+                acc[0] += p[row] * sum;
+            });
+    }
+    store_partials<T, 1>(acc, w.part_dir);
+}
+
+// Rows of at least this mean length take the split form of kernel 1.
+constexpr double kCgSplitMeanRow = 16.0;
+
 int fused_grid(int64_t tiles)
 {
     int64_t g = std::min<int64_t>(tiles, static_cast<int64_t>(num_cus()) * 8);
@@ -594,20 +664,37 @@ int cg_fused_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_
     T* sc = static_cast<T*>(st->scalars);
     const int64_t n = st->n;
     const cg_grids g = cg_grids_for<T>(a, n);
+    const bool split =
+        n > 0 && static_cast<double>(a->nnz) / static_cast<double>(n) >= kCgSplitMeanRow;
     for (int32_t it = 0; it < count; ++it) {
         T* pin = static_cast<T*>(st->p_a);
         T* pout = static_cast<T*>(st->p_b);
         const int64_t iter = st->issued;
         const int par = static_cast<int>(iter & 1);
+        if (split) {
+            cg_split_p_kernel<T><<<g.update, kBlock, 0, s>>>(
+                n, static_cast<const T*>(st->z), pin, pout, *crit, sc,
+                static_cast<const T*>(st->orig_tau), st->status, st->flags, w, g.update, iter,
+                par);
+            GKOMI_CHECK_LAUNCH("cg_split_p_kernel");
+#define GKOMI_SPLIT_CASE(S, R)                                                             \
+    cg_split_spmv_kernel<T, S, R><<<g.spmv, kBlock, 0, s>>>(                               \
+        n, a->row_ptrs, a->col_idxs, v, pout, static_cast<T*>(st->q), st->flags, w);       \
+    break;
+            GKOMI_SUBWAVE_SWITCH(g.sh, GKOMI_SPLIT_CASE)
+#undef GKOMI_SPLIT_CASE
+            GKOMI_CHECK_LAUNCH("cg_split_spmv_kernel");
+        } else {
 #define GKOMI_DIR_CASE(S, R)                                                               \
     cg_fused_direction_kernel<T, S, R><<<g.spmv, kBlock, 0, s>>>(                          \
         n, a->row_ptrs, a->col_idxs, v, static_cast<const T*>(st->z), pin, pout,           \
         static_cast<T*>(st->q), *crit, sc, static_cast<const T*>(st->orig_tau), st->status, \
         st->flags, w, g.update, iter, par);                                                \
     break;
-        GKOMI_SUBWAVE_SWITCH(g.sh, GKOMI_DIR_CASE)
+            GKOMI_SUBWAVE_SWITCH(g.sh, GKOMI_DIR_CASE)
 #undef GKOMI_DIR_CASE
-        GKOMI_CHECK_LAUNCH("cg_fused_direction_kernel");
+            GKOMI_CHECK_LAUNCH("cg_fused_direction_kernel");
+        }
         if (st->inv_diag) {
             cg_fused_update_kernel<T, true, true><<<g.update, kBlock, 0, s>>>(
                 n, static_cast<T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->z), pout,

@@ -426,8 +426,10 @@ int gkomi_cg_fused_start_f32(const gkomi_ctx* ctx, const gkomi_csr* a,
 /* `count` CG iterations (core/solver/cg.cpp:145-192): per iteration two
  * kernels -- (1) rho = r.z, ||r|| and the Combined check of the iteration,
  * then p = z + rho/prev_rho p fused into q = A p and beta = p.q, (2)
- * x += rho/beta p, r -= rho/beta q, z = M r.  Once flags->all_stopped is set
- * the remaining kernels return immediately, so the host may enqueue ahead. */
+ * x += rho/beta p, r -= rho/beta q, z = M r.  For rows of 16 or more entries
+ * on average kernel 1 is split in two (the p update, then q = A p with one
+ * gather per entry).  Once flags->all_stopped is set the remaining kernels
+ * return immediately, so the host may enqueue ahead. */
 int gkomi_cg_fused_iterate_f64(const gkomi_ctx* ctx, const gkomi_csr* a,
                                const gkomi_criteria* crit, gkomi_cg_state* st, int32_t count);
 int gkomi_cg_fused_iterate_f32(const gkomi_ctx* ctx, const gkomi_csr* a,

@@ -449,6 +449,34 @@ def test_fused_cg_poisson(dev, dtype, tol, jacobi):
     assert err <= 2 * kappa * max(res, np.finfo(dtype).eps)
 
 
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 1e-5)])
+@pytest.mark.parametrize("jacobi", [True, False])
+def test_fused_cg_split_form_27pt(dev, dtype, tol, jacobi):
+    """Rows of 27 entries take the split form of kernel 1 (p update, then
+    q = A p with one gather per entry): same recurrence as the oracle."""
+    rp, ci, v = gen.diffusion27(16)
+    v = v.astype(dtype)
+    n = 16 ** 3
+    b = np.random.default_rng(3).uniform(-1, 1, n).astype(dtype)
+    crit = [{"kind": "iteration", "max_iters": 3000},
+            {"kind": "residual_norm", "reduction_factor": tol}]
+    inv = ora.jacobi_inverse_diagonal(rp, ci, v) if jacobi else None
+    xo, so, io = ora.cg_solve(rp, ci, v, b.reshape(-1, 1), np.zeros((n, 1), dtype), crit, inv)
+    A = D().DeviceCsr(rp, ci, v, strategy="classical", device=dev)
+    solver = D().FusedCg(A, crit, jacobi=jacobi, chunk=8)
+    x = torch.zeros(n, dtype=TD[dtype], device=dev)
+    iters = solver.apply(to_dev(b, dev), x)
+    band = 3 if dtype == np.float64 else max(3, int(0.1 * io.iterations))
+    assert abs(iters - io.iterations) <= band, (iters, io.iterations)
+    assert solver.status.item() == so[0]
+    xh = x.cpu().numpy().astype(np.float64)
+    dense = gen.csr_to_dense(rp, ci, v.astype(np.float64))
+    res = np.linalg.norm(b - dense @ xh) / np.linalg.norm(b)
+    assert res <= tol * (2 if dtype == np.float64 else 20)
+    if dtype == np.float64:
+        np.testing.assert_allclose(xh, xo[:, 0], rtol=0, atol=1e-8 * np.abs(xo).max())
+
+
 def test_fused_cg_iteration_limit_and_restart(dev):
     rp, ci, v = gen.poisson2d(32)
     n = 1024
