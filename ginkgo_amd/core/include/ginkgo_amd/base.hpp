@@ -19,6 +19,7 @@
 namespace gko {
 
 using size_type = std::size_t;
+using int16 = std::int16_t;
 using int32 = std::int32_t;
 using int64 = std::int64_t;
 using uint8 = std::uint8_t;

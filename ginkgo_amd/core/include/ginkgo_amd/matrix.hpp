@@ -135,6 +135,14 @@ public:
     std::vector<IndexType> col_idxs_to_host() const { return col_idxs_.to_host(); }
     std::vector<ValueType> values_to_host() const { return values_.to_host(); }
 
+    // 2-byte column offsets (col - row), read instead of the 4-byte indices
+    // by the classical single right-hand-side SpMV and the fused CG; built
+    // only when every offset fits int16 (returns whether it did).  Results
+    // are bit-identical either way.  No reference counterpart.  Values or
+    // indices changed later through a view require calling it again.
+    bool compact_indices();
+    bool has_compact_indices() const noexcept { return compact_; }
+
     // the plain descriptor the C ABI takes
     gkomi_csr descriptor() const;
     // longest row (cached after the first query)
@@ -148,6 +156,7 @@ protected:
           col_idxs_(exec, nnz),
           row_ptrs_(exec, size.rows + 1),
           srow_(exec),
+          col_offsets_(exec),
           strategy_{strategy ? std::move(strategy) : std::make_shared<automatical>()}
     {
         if (size.rows == 0 && nnz == 0) {
@@ -164,6 +173,8 @@ private:
     array<IndexType> col_idxs_;
     array<IndexType> row_ptrs_;
     array<IndexType> srow_;
+    array<int16> col_offsets_;
+    bool compact_{false};
     std::shared_ptr<strategy_type> strategy_;
     std::string effective_{"classical"};
     int subwave_{0};

@@ -256,6 +256,8 @@ void bind_value_type(py::module_& m, const std::string& sfx)
             return std::shared_ptr<matrix::Csr<double>>(a.template convert_precision<double>());
         })
         .def("max_row_nnz", &Csr::max_row_nnz)
+        .def("compact_indices", &Csr::compact_indices)
+        .def_property_readonly("has_compact_indices", &Csr::has_compact_indices)
         .def_property_readonly("effective_strategy", &Csr::get_effective_strategy)
         .def_property_readonly("nnz", &Csr::get_num_stored_elements)
         .def("row_ptrs", [](const Csr& a) { return arr(a.row_ptrs_to_host()); })

@@ -224,6 +224,7 @@ void array<T>::fill(T value)
 
 template class array<double>;
 template class array<float>;
+template class array<int16>;
 template class array<int32>;
 template class array<int64>;
 template class array<uint8>;

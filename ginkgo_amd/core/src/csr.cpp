@@ -199,7 +199,26 @@ gkomi_csr Csr<V, I>::descriptor() const
     d.subwave = subwave_;
     d.srow = srow_.get_const_data();
     d.srow_blocks = srow_.get_size() ? static_cast<int64>(srow_.get_size()) - 1 : 0;
+    d.col_offsets = compact_ ? col_offsets_.get_const_data() : nullptr;
     return d;
+}
+
+template <typename V, typename I>
+bool Csr<V, I>::compact_indices()
+{
+    auto exec = this->get_executor();
+    const auto ctx = ctx_of(exec);
+    const size_type nnz = values_.get_size();
+    if (nnz == 0) return false;
+    array<int16> off(exec, nnz);
+    array<int32> fits(exec, 1);
+    auto d = descriptor();
+    d.col_offsets = nullptr;
+    GKO_CALL(gkomi_csr_col_offsets(ctx, &d, off.get_data(), fits.get_data()));
+    if (exec->copy_val_to_host(fits.get_const_data()) == 0) return false;
+    col_offsets_ = std::move(off);
+    compact_ = true;
+    return true;
 }
 
 // Csr::apply_impl (core/matrix/csr.cpp:257-315) -> csr::spmv / advanced_spmv

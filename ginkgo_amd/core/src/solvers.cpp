@@ -591,7 +591,12 @@ Ir<V>::Ir(std::shared_ptr<const Executor> exec, const parameters_type& p,
                               dynamic_cast<const typename Bicgstab<Other>::Factory*>(probe);
         if (other_cg) {
             if (auto csr = dynamic_cast<const matrix::Csr<V, int32>*>(system_matrix_.get())) {
-                inner_a = std::shared_ptr<const LinOp>(csr->template convert_precision<Other>());
+                // the inner copy is the library's own: it also gets the 2-byte
+                // column offsets when they fit (6 instead of 8 bytes per FP32
+                // entry, bit-identical results)
+                auto lower = csr->template convert_precision<Other>();
+                lower->compact_indices();
+                inner_a = std::shared_ptr<const LinOp>(std::move(lower));
             }
         }
         inner_ = factory->generate(inner_a);

@@ -21,9 +21,10 @@
 
 namespace gkomi {
 
-template <typename T, int SW, int RPG, bool Adv, bool Strided>
+template <typename T, int SW, int RPG, bool Adv, bool Strided, bool Compact>
 __device__ __forceinline__ void csr_subwave_body(int64_t n_rows, const int32_t* __restrict__ rp,
                                                  const int32_t* __restrict__ ci,
+                                                 const int16_t* __restrict__ co,
                                                  const T* __restrict__ v,
                                                  const T* __restrict__ b, int64_t bs,
                                                  T* __restrict__ c, int64_t cs,
@@ -33,8 +34,8 @@ __device__ __forceinline__ void csr_subwave_body(int64_t n_rows, const int32_t* 
     const int64_t nb = static_cast<int64_t>(gridDim.x);
     const int64_t tiles = ceildiv(n_rows, subwave_tile_rows<SW, RPG>);
     for (int64_t t = xcd_block(blockIdx.x, gridDim.x); t < tiles; t += nb) {
-        subwave_tile<T, SW, RPG>(
-            n_rows, t, rp, ci, v,
+        subwave_tile<T, SW, RPG, Compact>(
+            n_rows, t, rp, ci, co, v,
             [&](int32_t col) { return Strided ? b[static_cast<int64_t>(col) * bs] : b[col]; },
             [&](int64_t row, T sum) { store_result<Adv>(c + row * cs, sum, alpha, beta); });
     }
@@ -43,14 +44,22 @@ __device__ __forceinline__ void csr_subwave_body(int64_t n_rows, const int32_t* 
 template <typename T, int SW, int RPG, bool Adv>
 __global__ void __launch_bounds__(kBlock)
     csr_spmv_subwave(int64_t n_rows, const int32_t* __restrict__ rp,
-                     const int32_t* __restrict__ ci, const T* __restrict__ v,
-                     const T* __restrict__ b, int64_t bs, T* __restrict__ c, int64_t cs,
-                     const T* __restrict__ alpha, const T* __restrict__ beta)
+                     const int32_t* __restrict__ ci, const int16_t* __restrict__ co,
+                     const T* __restrict__ v, const T* __restrict__ b, int64_t bs,
+                     T* __restrict__ c, int64_t cs, const T* __restrict__ alpha,
+                     const T* __restrict__ beta)
 {
-    if (bs == 1) {
-        csr_subwave_body<T, SW, RPG, Adv, false>(n_rows, rp, ci, v, b, bs, c, cs, alpha, beta);
+    // the compact offsets serve the unit-stride case; strided vectors read
+    // the full indices
+    if (bs == 1 && co != nullptr) {
+        csr_subwave_body<T, SW, RPG, Adv, false, true>(n_rows, rp, ci, co, v, b, bs, c, cs,
+                                                       alpha, beta);
+    } else if (bs == 1) {
+        csr_subwave_body<T, SW, RPG, Adv, false, false>(n_rows, rp, ci, co, v, b, bs, c, cs,
+                                                        alpha, beta);
     } else {
-        csr_subwave_body<T, SW, RPG, Adv, true>(n_rows, rp, ci, v, b, bs, c, cs, alpha, beta);
+        csr_subwave_body<T, SW, RPG, Adv, true, false>(n_rows, rp, ci, co, v, b, bs, c, cs,
+                                                       alpha, beta);
     }
 }
 
@@ -245,6 +254,26 @@ __global__ void __launch_bounds__(kBlock)
 
 // -------------------------------------------------------------- drivers ----
 
+// col - row per entry as int16; a row that needs a wider offset clears *fits
+__global__ void __launch_bounds__(kBlock)
+    col_offsets_kernel(int64_t n_rows, const int32_t* __restrict__ rp,
+                       const int32_t* __restrict__ ci, int16_t* __restrict__ out,
+                       int32_t* __restrict__ fits)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    bool ok = true;
+    for (int64_t row = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; row < n_rows;
+         row += stride) {
+        const int32_t e = rp[row + 1];
+        for (int32_t p = rp[row]; p < e; ++p) {
+            const int64_t d = static_cast<int64_t>(ci[p]) - row;
+            ok = ok && d >= -32768 && d <= 32767;
+            out[p] = static_cast<int16_t>(d);
+        }
+    }
+    if (!ok) atomicMin(fits, 0);
+}
+
 int check_csr(const gkomi_csr* a)
 {
     GKOMI_REQUIRE(a != nullptr, GKOMI_INVALID_ARGUMENT, "null CSR descriptor");
@@ -313,7 +342,7 @@ int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const 
     const unsigned g = grid_for(a->n_rows, tile_rows, 64);
 #define GKOMI_SPMV_CASE(S, R)                                                                 \
     csr_spmv_subwave<T, S, R, Adv><<<g, kBlock, 0, stream>>>(                                 \
-        a->n_rows, a->row_ptrs, a->col_idxs, v, b, bs, c, cs, alpha, beta);                   \
+        a->n_rows, a->row_ptrs, a->col_idxs, a->col_offsets, v, b, bs, c, cs, alpha, beta);   \
     break;
     GKOMI_SUBWAVE_SWITCH(sh, GKOMI_SPMV_CASE)
 #undef GKOMI_SPMV_CASE
@@ -423,6 +452,26 @@ int gkomi_csr_max_row_nnz(const gkomi_ctx* ctx, int64_t n_rows, const int32_t* r
     max_row_nnz_kernel<<<grid_for(n_rows, kBlock * 8, 4), kBlock, 0, stream_of(ctx)>>>(
         n_rows, row_ptrs, reinterpret_cast<unsigned long long*>(result));
     GKOMI_CHECK_LAUNCH("max_row_nnz_kernel");
+    return GKOMI_SUCCESS;
+}
+
+int gkomi_csr_col_offsets(const gkomi_ctx* ctx, const gkomi_csr* a, int16_t* out,
+                          int32_t* fits)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_csr(a)) return rc;
+    GKOMI_REQUIRE(fits != nullptr && (a->nnz == 0 || out != nullptr), GKOMI_INVALID_ARGUMENT,
+                  "col_offsets: null output");
+    // *fits = 1 by a kernel (graph-capture safe), cleared by rows that overflow
+    static_assert(sizeof(int32_t) == sizeof(float), "fill by bit pattern");
+    if (int rc = gkomi_fill_f32(ctx, 1, __builtin_bit_cast(float, int32_t{1}),
+                                reinterpret_cast<float*>(fits))) {
+        return rc;
+    }
+    if (a->n_rows == 0) return GKOMI_SUCCESS;
+    col_offsets_kernel<<<grid_for(a->n_rows, kBlock * 4, 16), kBlock, 0, stream_of(ctx)>>>(
+        a->n_rows, a->row_ptrs, a->col_idxs, out, fits);
+    GKOMI_CHECK_LAUNCH("col_offsets_kernel");
     return GKOMI_SUCCESS;
 }
 

@@ -258,12 +258,13 @@ __host__ __device__ inline cg_work cg_work_layout(void* base)
     return w;
 }
 
-// Workgroup sums of (a, b) in the value type, converted to FP64 and stored
-// as this workgroup's partials.
-template <typename T, int Count>
+// Workgroup sums of Count values in the value type, converted to FP64 and
+// stored as this workgroup's partials (NT threads per workgroup).
+template <typename T, int Count, int NT = kBlock>
 __device__ __forceinline__ void store_partials(const T (&vals)[Count], double* partials)
 {
-    __shared__ T lds[Count][kWavesPerBlock];
+    constexpr int kW = NT / kWave;
+    __shared__ T lds[Count][kW];
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
 #pragma unroll
@@ -276,24 +277,25 @@ __device__ __forceinline__ void store_partials(const T (&vals)[Count], double* p
 #pragma unroll
         for (int c = 0; c < Count; ++c) {
             T s = lds[c][0];
-            for (int w = 1; w < kWavesPerBlock; ++w) s += lds[c][w];
+            for (int w = 1; w < kW; ++w) s += lds[c][w];
             partials[static_cast<int64_t>(blockIdx.x) * Count + c] = static_cast<double>(s);
         }
     }
 }
 
 // Sum of `count` workgroup partials (Count values each) in a fixed order:
-// thread t adds partials t, t + kBlock, ...; then the wave tree and the
-// waves in order.  Every workgroup obtains the same bits.
-template <int Count>
+// thread t adds partials t, t + NT, ...; then the wave tree and the waves in
+// order.  Every workgroup obtains the same bits.
+template <int Count, int NT = kBlock>
 __device__ __forceinline__ void combine_partials(const double* __restrict__ partials, int count,
                                                  double (&out)[Count])
 {
-    __shared__ double lds[Count][kWavesPerBlock];
+    constexpr int kW = NT / kWave;
+    __shared__ double lds[Count][kW];
     double acc[Count];
 #pragma unroll
     for (int c = 0; c < Count; ++c) acc[c] = 0;
-    for (int k = threadIdx.x; k < count; k += kBlock) {
+    for (int k = threadIdx.x; k < count; k += NT) {
 #pragma unroll
         for (int c = 0; c < Count; ++c) acc[c] += partials[static_cast<int64_t>(k) * Count + c];
     }
@@ -308,19 +310,26 @@ __device__ __forceinline__ void combine_partials(const double* __restrict__ part
 #pragma unroll
     for (int c = 0; c < Count; ++c) {
         double s = lds[c][0];
-        for (int w = 1; w < kWavesPerBlock; ++w) s += lds[c][w];
+        for (int w = 1; w < kW; ++w) s += lds[c][w];
         out[c] = s;
     }
     __syncthreads();
 }
 
+// The elementwise kernels (update, split p) run 1024-thread workgroups on a
+// grid of at most kCgUpdGrid: few partials to combine in their consumers,
+// full occupancy for the streaming work.
+constexpr int kUpdBlock = 1024;
+constexpr int kCgUpdGrid = 512;
+
 // Start, kernel 1: r = b - A x (sub-wave SpMV), p_a = p_b = q = 0, and
 // partials of b.b and r.r; resets the flags, the status and prev_rho = 1
 // (reference/solver/cg_kernels.cpp:26-50).
-template <typename T, int SW, int RPG>
+template <typename T, int SW, int RPG, bool Compact>
 __global__ void __launch_bounds__(kBlock)
     cg_fused_init_kernel(int64_t n, const int32_t* __restrict__ rp,
-                         const int32_t* __restrict__ ci, const T* __restrict__ v,
+                         const int32_t* __restrict__ ci, const int16_t* __restrict__ co,
+                         const T* __restrict__ v,
                          const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ r,
                          T* __restrict__ pa, T* __restrict__ pb, T* __restrict__ q,
                          T* __restrict__ scalars, uint8_t* __restrict__ status,
@@ -330,8 +339,8 @@ __global__ void __launch_bounds__(kBlock)
     block_range(ceildiv(n, subwave_tile_rows<SW, RPG>), &rb0, &rb1);
     T acc[2] = {T(0), T(0)};
     for (int64_t blk = rb0; blk < rb1; ++blk) {
-        subwave_tile<T, SW, RPG>(
-            n, blk, rp, ci, v, [&](int32_t c) { return x[c]; },
+        subwave_tile<T, SW, RPG, Compact>(
+            n, blk, rp, ci, co, v, [&](int32_t c) { return x[c]; },
             [&](int64_t row, T sum) {
                 const T bi = b[row];
                 const T ri = bi - sum;
@@ -366,7 +375,7 @@ __global__ void __launch_bounds__(kBlock)
 //   start:  the criteria baselines ||b|| / ||r0|| from the init partials;
 // then z = M r and partials of r.z and r.r for the next direction kernel.
 template <typename T, bool Update, bool Jacobi>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kUpdBlock)
     cg_fused_update_kernel(int64_t n, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
                            const T* __restrict__ p, const T* __restrict__ q,
                            const T* __restrict__ inv_diag, gkomi_criteria crit,
@@ -379,14 +388,14 @@ __global__ void __launch_bounds__(kBlock)
     if constexpr (Update) {
         if (flags->all_stopped) return;
         double pq[1];
-        combine_partials<1>(w.part_dir, count_in, pq);
+        combine_partials<1, kUpdBlock>(w.part_dir, count_in, pq);
         const T beta = static_cast<T>(pq[0]);
         do_update = beta != T(0);
         alpha = do_update ? scalars[par] / beta : T(0);
         if (blockIdx.x == 0 && threadIdx.x == 0) scalars[2] = beta;
     } else {
         double s[2];
-        combine_partials<2>(w.part_init, count_in, s);
+        combine_partials<2, kUpdBlock>(w.part_init, count_in, s);
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             for (int k = 0; k < crit.num; ++k) {
                 const int base = crit.c[k].baseline;
@@ -399,10 +408,10 @@ __global__ void __launch_bounds__(kBlock)
         }
     }
     int64_t i0, i1;
-    block_range(ceildiv(n, kBlock), &i0, &i1);
+    block_range(ceildiv(n, kUpdBlock), &i0, &i1);
     T acc[2] = {T(0), T(0)};
     for (int64_t blk = i0; blk < i1; ++blk) {
-        const int64_t i = blk * kBlock + threadIdx.x;
+        const int64_t i = blk * kUpdBlock + threadIdx.x;
         if (i >= n) continue;
         T ri = r[i];
         if (do_update) {
@@ -415,7 +424,7 @@ __global__ void __launch_bounds__(kBlock)
         acc[0] += ri * zi;
         acc[1] += ri * ri;
     }
-    store_partials<T, 2>(acc, w.part_upd);
+    store_partials<T, 2, kUpdBlock>(acc, w.part_upd);
 }
 
 // Kernel 1 of iteration `iter` (parity par = iter % 2): rho = r.z and
@@ -424,10 +433,11 @@ __global__ void __launch_bounds__(kBlock)
 // p_out = z + rho/prev_rho p_in (p_out = z when prev_rho == 0,
 // reference/solver/cg_kernels.cpp:64-70), q = A p_out with p_out recomputed
 // at the gathered columns, and partials of p.q.
-template <typename T, int SW, int RPG>
+template <typename T, int SW, int RPG, bool Compact>
 __global__ void __launch_bounds__(kBlock)
     cg_fused_direction_kernel(int64_t n, const int32_t* __restrict__ rp,
-                              const int32_t* __restrict__ ci, const T* __restrict__ v,
+                              const int32_t* __restrict__ ci, const int16_t* __restrict__ co,
+                              const T* __restrict__ v,
                               const T* __restrict__ z, const T* __restrict__ p_in,
                               T* __restrict__ p_out, T* __restrict__ q, gkomi_criteria crit,
                               T* __restrict__ scalars, const T* __restrict__ orig,
@@ -464,8 +474,8 @@ __global__ void __launch_bounds__(kBlock)
     block_range(ceildiv(n, subwave_tile_rows<SW, RPG>), &rb0, &rb1);
     T acc[1] = {T(0)};
     for (int64_t blk = rb0; blk < rb1; ++blk) {
-        subwave_tile<T, SW, RPG>(
-            n, blk, rp, ci, v,
+        subwave_tile<T, SW, RPG, Compact>(
+            n, blk, rp, ci, co, v,
             [&](int32_t c) { return restart ? z[c] : fma(tmp, p_in[c], z[c]); },
             [&](int64_t row, T sum) {
                 const T pi = restart ? z[row] : fma(tmp, p_in[row], z[row]);
@@ -482,7 +492,7 @@ __global__ void __launch_bounds__(kBlock)
 // pass over p.  (a) the check of iteration `iter` exactly as in the fused
 // direction kernel, then p_out = z + rho/prev_rho p_in elementwise;
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kUpdBlock)
     cg_split_p_kernel(int64_t n, const T* __restrict__ z, const T* __restrict__ p_in,
                       T* __restrict__ p_out, gkomi_criteria crit, T* __restrict__ scalars,
                       const T* __restrict__ orig, uint8_t* __restrict__ status,
@@ -494,7 +504,7 @@ __global__ void __launch_bounds__(kBlock)
     __syncthreads();
     if (stopped) return;
     double s[2];
-    combine_partials<2>(w.part_upd, count_in, s);
+    combine_partials<2, kUpdBlock>(w.part_upd, count_in, s);
     const T rho = static_cast<T>(s[0]);
     const T tau = static_cast<T>(sqrt(s[1]));
     const T prev = scalars[par ^ 1];
@@ -514,18 +524,19 @@ __global__ void __launch_bounds__(kBlock)
     const bool restart = prev == T(0);
     const T tmp = restart ? T(0) : rho / prev;
     int64_t i0, i1;
-    block_range(ceildiv(n, kBlock), &i0, &i1);
+    block_range(ceildiv(n, kUpdBlock), &i0, &i1);
     for (int64_t blk = i0; blk < i1; ++blk) {
-        const int64_t i = blk * kBlock + threadIdx.x;
+        const int64_t i = blk * kUpdBlock + threadIdx.x;
         if (i < n) p_out[i] = restart ? z[i] : fma(tmp, p_in[i], z[i]);
     }
 }
 
 // (b) q = A p (one gather per entry) and the partials of p.q.
-template <typename T, int SW, int RPG>
+template <typename T, int SW, int RPG, bool Compact>
 __global__ void __launch_bounds__(kBlock)
     cg_split_spmv_kernel(int64_t n, const int32_t* __restrict__ rp,
-                         const int32_t* __restrict__ ci, const T* __restrict__ v,
+                         const int32_t* __restrict__ ci, const int16_t* __restrict__ co,
+                         const T* __restrict__ v,
                          const T* __restrict__ p, T* __restrict__ q,
                          const gkomi_stop_flags* __restrict__ flags, cg_work w)
 {
@@ -534,8 +545,8 @@ __global__ void __launch_bounds__(kBlock)
     block_range(ceildiv(n, subwave_tile_rows<SW, RPG>), &rb0, &rb1);
     T acc[1] = {T(0)};
     for (int64_t blk = rb0; blk < rb1; ++blk) {
-        subwave_tile<T, SW, RPG>(
-            n, blk, rp, ci, v, [&](int32_t c) { return p[c]; },
+        subwave_tile<T, SW, RPG, Compact>(
+            n, blk, rp, ci, co, v, [&](int32_t c) { return p[c]; },
             [&](int64_t row, T sum) {
                 q[row] = sum;
                 acc[0] += p[row] * sum;
@@ -595,7 +606,9 @@ cg_grids cg_grids_for(const gkomi_csr* a, int64_t n)
     g.sh = pick_subwave(a, sizeof(T));
     const int64_t tile_rows = int64_t{kBlock} / g.sh.sw * g.sh.rpg;
     g.spmv = fused_grid(ceildiv(std::max<int64_t>(n, 1), tile_rows));
-    g.update = fused_grid(ceildiv(std::max<int64_t>(n, 1), kBlock));
+    g.update = static_cast<int>(
+        std::max<int64_t>(1, std::min<int64_t>(ceildiv(std::max<int64_t>(n, 1), kUpdBlock),
+                                               kCgUpdGrid)));
     return g;
 }
 
@@ -617,23 +630,31 @@ int cg_fused_start_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_cr
     T* sc = static_cast<T*>(st->scalars);
     const int64_t n = st->n;
     const cg_grids g = cg_grids_for<T>(a, n);
-#define GKOMI_INIT_CASE(S, R)                                                                \
-    cg_fused_init_kernel<T, S, R><<<g.spmv, kBlock, 0, s>>>(                                 \
-        n, a->row_ptrs, a->col_idxs, v, static_cast<const T*>(st->b),                        \
+#define GKOMI_INIT_CASE_C(S, R, C)                                                           \
+    cg_fused_init_kernel<T, S, R, C><<<g.spmv, kBlock, 0, s>>>(                              \
+        n, a->row_ptrs, a->col_idxs, a->col_offsets, v, static_cast<const T*>(st->b),        \
         static_cast<const T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->p_a),      \
         static_cast<T*>(st->p_b), static_cast<T*>(st->q), sc, st->status, st->flags, w);     \
     break;
-    GKOMI_SUBWAVE_SWITCH(g.sh, GKOMI_INIT_CASE)
+#define GKOMI_INIT_CASE(S, R) GKOMI_INIT_CASE_C(S, R, false)
+#define GKOMI_INIT_CASE_COMPACT(S, R) GKOMI_INIT_CASE_C(S, R, true)
+    if (a->col_offsets) {
+        GKOMI_SUBWAVE_SWITCH(g.sh, GKOMI_INIT_CASE_COMPACT)
+    } else {
+        GKOMI_SUBWAVE_SWITCH(g.sh, GKOMI_INIT_CASE)
+    }
 #undef GKOMI_INIT_CASE
+#undef GKOMI_INIT_CASE_COMPACT
+#undef GKOMI_INIT_CASE_C
     GKOMI_CHECK_LAUNCH("cg_fused_init_kernel");
     if (st->inv_diag) {
-        cg_fused_update_kernel<T, false, true><<<g.update, kBlock, 0, s>>>(
+        cg_fused_update_kernel<T, false, true><<<g.update, kUpdBlock, 0, s>>>(
             n, static_cast<T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->z),
             static_cast<const T*>(st->p_a), static_cast<const T*>(st->q),
             static_cast<const T*>(st->inv_diag), *crit, sc, static_cast<T*>(st->orig_tau),
             st->flags, w, g.spmv, 0);
     } else {
-        cg_fused_update_kernel<T, false, false><<<g.update, kBlock, 0, s>>>(
+        cg_fused_update_kernel<T, false, false><<<g.update, kUpdBlock, 0, s>>>(
             n, static_cast<T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->z),
             static_cast<const T*>(st->p_a), static_cast<const T*>(st->q), nullptr, *crit, sc,
             static_cast<T*>(st->orig_tau), st->flags, w, g.spmv, 0);
@@ -672,36 +693,53 @@ int cg_fused_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_
         const int64_t iter = st->issued;
         const int par = static_cast<int>(iter & 1);
         if (split) {
-            cg_split_p_kernel<T><<<g.update, kBlock, 0, s>>>(
+            cg_split_p_kernel<T><<<g.update, kUpdBlock, 0, s>>>(
                 n, static_cast<const T*>(st->z), pin, pout, *crit, sc,
                 static_cast<const T*>(st->orig_tau), st->status, st->flags, w, g.update, iter,
                 par);
             GKOMI_CHECK_LAUNCH("cg_split_p_kernel");
-#define GKOMI_SPLIT_CASE(S, R)                                                             \
-    cg_split_spmv_kernel<T, S, R><<<g.spmv, kBlock, 0, s>>>(                               \
-        n, a->row_ptrs, a->col_idxs, v, pout, static_cast<T*>(st->q), st->flags, w);       \
+#define GKOMI_SPLIT_CASE_C(S, R, C)                                                        \
+    cg_split_spmv_kernel<T, S, R, C><<<g.spmv, kBlock, 0, s>>>(                            \
+        n, a->row_ptrs, a->col_idxs, a->col_offsets, v, pout, static_cast<T*>(st->q),      \
+        st->flags, w);                                                                     \
     break;
-            GKOMI_SUBWAVE_SWITCH(g.sh, GKOMI_SPLIT_CASE)
+#define GKOMI_SPLIT_CASE(S, R) GKOMI_SPLIT_CASE_C(S, R, false)
+#define GKOMI_SPLIT_CASE_COMPACT(S, R) GKOMI_SPLIT_CASE_C(S, R, true)
+            if (a->col_offsets) {
+                GKOMI_SUBWAVE_SWITCH(g.sh, GKOMI_SPLIT_CASE_COMPACT)
+            } else {
+                GKOMI_SUBWAVE_SWITCH(g.sh, GKOMI_SPLIT_CASE)
+            }
 #undef GKOMI_SPLIT_CASE
+#undef GKOMI_SPLIT_CASE_COMPACT
+#undef GKOMI_SPLIT_CASE_C
             GKOMI_CHECK_LAUNCH("cg_split_spmv_kernel");
         } else {
-#define GKOMI_DIR_CASE(S, R)                                                               \
-    cg_fused_direction_kernel<T, S, R><<<g.spmv, kBlock, 0, s>>>(                          \
-        n, a->row_ptrs, a->col_idxs, v, static_cast<const T*>(st->z), pin, pout,           \
-        static_cast<T*>(st->q), *crit, sc, static_cast<const T*>(st->orig_tau), st->status, \
-        st->flags, w, g.update, iter, par);                                                \
+#define GKOMI_DIR_CASE_C(S, R, C)                                                          \
+    cg_fused_direction_kernel<T, S, R, C><<<g.spmv, kBlock, 0, s>>>(                       \
+        n, a->row_ptrs, a->col_idxs, a->col_offsets, v, static_cast<const T*>(st->z), pin, \
+        pout, static_cast<T*>(st->q), *crit, sc, static_cast<const T*>(st->orig_tau),      \
+        st->status, st->flags, w, g.update, iter, par);                                    \
     break;
-            GKOMI_SUBWAVE_SWITCH(g.sh, GKOMI_DIR_CASE)
+#define GKOMI_DIR_CASE(S, R) GKOMI_DIR_CASE_C(S, R, false)
+#define GKOMI_DIR_CASE_COMPACT(S, R) GKOMI_DIR_CASE_C(S, R, true)
+            if (a->col_offsets) {
+                GKOMI_SUBWAVE_SWITCH(g.sh, GKOMI_DIR_CASE_COMPACT)
+            } else {
+                GKOMI_SUBWAVE_SWITCH(g.sh, GKOMI_DIR_CASE)
+            }
 #undef GKOMI_DIR_CASE
+#undef GKOMI_DIR_CASE_COMPACT
+#undef GKOMI_DIR_CASE_C
             GKOMI_CHECK_LAUNCH("cg_fused_direction_kernel");
         }
         if (st->inv_diag) {
-            cg_fused_update_kernel<T, true, true><<<g.update, kBlock, 0, s>>>(
+            cg_fused_update_kernel<T, true, true><<<g.update, kUpdBlock, 0, s>>>(
                 n, static_cast<T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->z), pout,
                 static_cast<const T*>(st->q), static_cast<const T*>(st->inv_diag), *crit, sc,
                 static_cast<T*>(st->orig_tau), st->flags, w, g.spmv, par);
         } else {
-            cg_fused_update_kernel<T, true, false><<<g.update, kBlock, 0, s>>>(
+            cg_fused_update_kernel<T, true, false><<<g.update, kUpdBlock, 0, s>>>(
                 n, static_cast<T*>(st->x), static_cast<T*>(st->r), static_cast<T*>(st->z), pout,
                 static_cast<const T*>(st->q), nullptr, *crit, sc, static_cast<T*>(st->orig_tau),
                 st->flags, w, g.spmv, par);

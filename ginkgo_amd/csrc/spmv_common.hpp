@@ -42,11 +42,13 @@ constexpr int64_t subwave_tile_rows = int64_t{kWave / SW} * RPG * kWavesPerBlock
 // l + SW, ... by fused multiply-add; then the group tree), independent of
 // RPG and of the launch.  `gather(col)` supplies the vector entry of a
 // column; `row_op(row, sum)` runs in lane 0 of the group for every row below
-// n_rows.
-template <typename T, int SW, int RPG, typename Gather, typename RowOp>
+// n_rows.  With Compact the columns come from the 2-byte offsets `co`
+// (column = row + co[p]) instead of `ci`.
+template <typename T, int SW, int RPG, bool Compact = false, typename Gather, typename RowOp>
 __device__ __forceinline__ void subwave_tile(int64_t n_rows, int64_t tile,
                                              const int32_t* __restrict__ rp,
                                              const int32_t* __restrict__ ci,
+                                             const int16_t* __restrict__ co,
                                              const T* __restrict__ v, Gather gather,
                                              RowOp row_op)
 {
@@ -84,7 +86,13 @@ __device__ __forceinline__ void subwave_tile(int64_t n_rows, int64_t tile,
         for (int k = 0; k < RPG; ++k) {
             act[k] = pos[k] < end[k];
             const int32_t p = act[k] ? pos[k] : 0;
-            col[k] = ci[p];
+            if constexpr (Compact) {
+                // an inactive slot gathers column 0 (always valid)
+                const int32_t c = static_cast<int32_t>(row0 + k * kGroups) + co[p];
+                col[k] = act[k] ? c : 0;
+            } else {
+                col[k] = ci[p];
+            }
             val[k] = v[p];
             pos[k] = act[k] ? pos[k] + SW : pos[k];
         }

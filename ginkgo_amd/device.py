@@ -85,7 +85,24 @@ class DeviceCsr:
                    self.nnz, chunk, ptr(self.srow), self.srow_blocks)
         elif strategy != "classical":
             raise C.InvalidArgument(f"unknown strategy {strategy}")
+        self.col_offsets = None
         self.desc = self._desc()
+
+    def compact(self):
+        """Builds the 2-byte column offsets (col - row) the classical SpMV and
+        the fused CG read instead of the 4-byte indices; returns whether every
+        offset fits int16 (else nothing changes).  Results are bit-identical."""
+        off = torch.empty(max(self.nnz, 1), dtype=torch.int16, device=self.device)
+        fits = torch.zeros(1, dtype=torch.int32, device=self.device)
+        c = ctx(self.device)
+        d = self._desc()
+        d.col_offsets = None
+        C.call("gkomi_csr_col_offsets", ctypes.byref(c), ctypes.byref(d), ptr(off), ptr(fits))
+        if self.nnz > 0 and int(fits.item()) == 1:
+            self.col_offsets = off
+            self.desc = self._desc()
+            return True
+        return False
 
     def _desc(self):
         d = C.Csr()
@@ -97,6 +114,8 @@ class DeviceCsr:
         d.subwave = self.subwave
         d.srow = ptr(self.srow).value if self.srow is not None else None
         d.srow_blocks = self.srow_blocks
+        d.col_offsets = ptr(self.col_offsets).value if getattr(self, "col_offsets", None) \
+            is not None else None
         return d
 
     def spmv(self, b, out=None, stream=None):

@@ -171,6 +171,11 @@ typedef struct {
     int32_t subwave;
     const int32_t* srow;     /* LOAD_BALANCE: srow_blocks + 1 row starts */
     int64_t srow_blocks;
+    /* Optional (NULL = absent, no reference counterpart): col_idxs[p] - row
+     * as int16 for every entry, built by gkomi_csr_col_offsets.  col_idxs
+     * stays required; the CLASSICAL single right-hand-side SpMV and the fused
+     * CG read these 2-byte offsets instead of the 4-byte indices. */
+    const int16_t* col_offsets;
 } gkomi_csr;
 
 /* nnz handled by one workgroup of the LOAD_BALANCE kernel for value type V. */
@@ -181,6 +186,11 @@ int64_t gkomi_csr_lb_chunk(int32_t value_bytes);
  * Integer work, deterministic. */
 int gkomi_csr_lb_partition(const gkomi_ctx* ctx, int64_t n_rows, const int32_t* row_ptrs,
                            int64_t nnz, int64_t chunk, int32_t* srow, int64_t num_blocks);
+/* Compact column indices (no reference counterpart): out[p] = col_idxs[p] -
+ * row(p) as int16 for every entry.  *fits (device int32) becomes 1 when every
+ * offset lies in [-32768, 32767], else 0 (then `out` must not be used). */
+int gkomi_csr_col_offsets(const gkomi_ctx* ctx, const gkomi_csr* a, int16_t* out,
+                          int32_t* fits);
 /* Longest row (csr::compute_max_row_nnz, common/unified/matrix/csr_kernels.cpp);
  * result is written to device memory. */
 int gkomi_csr_max_row_nnz(const gkomi_ctx* ctx, int64_t n_rows, const int32_t* row_ptrs,

@@ -103,6 +103,26 @@ def test_spmv_classical_every_subwave(dev, dtype, subwave):
                            device=dev)
         y2 = A2.spmv(to_dev(x, dev)).cpu().numpy()
         np.testing.assert_array_equal(y2, y)
+        # 2-byte column offsets instead of the indices: the same bits
+        assert A2.compact()
+        y3 = A2.spmv(to_dev(x, dev)).cpu().numpy()
+        np.testing.assert_array_equal(y3, y)
+
+
+def test_compact_offsets_refuse_wide_rows(dev):
+    n = 40000
+    rp = np.arange(n + 1, dtype=np.int32)
+    ci = np.arange(n, dtype=np.int32)
+    ci[5] = 39000                        # offset 38995 does not fit int16
+    A = D().DeviceCsr(rp, ci, np.ones(n), strategy="classical", device=dev)
+    assert not A.compact()
+    assert A.col_offsets is None
+    ci[5] = 5 + 32767                    # the widest offset that fits
+    B = D().DeviceCsr(rp, ci, np.ones(n), strategy="classical", device=dev)
+    assert B.compact()
+    x = torch.arange(n, dtype=torch.float64, device=dev)
+    y = B.spmv(x).cpu().numpy()
+    assert y[5] == 5 + 32767 and y[6] == 6
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 1000, 4099, 100003])
@@ -475,6 +495,27 @@ def test_fused_cg_split_form_27pt(dev, dtype, tol, jacobi):
     assert res <= tol * (2 if dtype == np.float64 else 20)
     if dtype == np.float64:
         np.testing.assert_allclose(xh, xo[:, 0], rtol=0, atol=1e-8 * np.abs(xo).max())
+
+
+@pytest.mark.parametrize("maker", ["poisson", "d27"])
+def test_fused_cg_compact_offsets_bit_identical(dev, maker):
+    """The fused CG over the 2-byte column offsets (both kernel-1 forms)
+    reproduces the 4-byte-index run bit for bit."""
+    rp, ci, v = gen.poisson2d(48) if maker == "poisson" else gen.diffusion27(14)
+    n = len(rp) - 1
+    b = to_dev(np.random.default_rng(2).uniform(-1, 1, n), dev)
+    crit = [{"kind": "iteration", "max_iters": 2000},
+            {"kind": "residual_norm", "reduction_factor": 1e-10}]
+    out = []
+    for compact in (False, True):
+        A = D().DeviceCsr(rp, ci, v, strategy="classical", device=dev)
+        if compact:
+            assert A.compact()
+        x = torch.zeros(n, dtype=torch.float64, device=dev)
+        it = D().FusedCg(A, crit, jacobi=True, chunk=8).apply(b, x)
+        out.append((it, x.cpu().numpy()))
+    assert out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
 def test_fused_cg_iteration_limit_and_restart(dev):
