@@ -303,11 +303,12 @@ subwave_shape pick_subwave(const gkomi_csr* a, int value_bytes)
         while (sh.sw < 64 && 2 * sh.sw < mean) sh.sw *= 2;
     }
     if (sh.rpg == 0) {
-        // about eight entries in flight per lane (measured over sw 1..32 and
-        // rpg 1..8 on the 5-point 1448^2, 7-point 128^3 and 27-point 96^3
-        // matrices, FP64 and FP32: within 2% of the best shape on each)
-        (void)value_bytes;
-        const int target = 8;
+        // entries in flight per lane: eight, except four for FP64 rows that
+        // take at most four lanes (measured over sw 1..32 and rpg 1..8 on the
+        // 5-point 1448^2, 7-point 128^3 and 27-point 96^3 matrices, FP64 and
+        // FP32; on the 7-point FP64 case four also reads the least: 1.01x the
+        // algorithmic bytes against 1.06x with eight)
+        const int target = (value_bytes >= 8 && sh.sw <= 4) ? 4 : 8;
         const int iters = std::max(1, static_cast<int>(std::ceil(mean / sh.sw)));
         sh.rpg = 1;
         while (sh.rpg < 8 && 2 * sh.rpg * iters <= target) sh.rpg *= 2;
