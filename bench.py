@@ -350,8 +350,8 @@ def main():
                     help="conductivities checked by central differences in the adjoint section")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1 "
-                                   "(multi-GPU sharding is a design note, not built)"}))
+        print(json.dumps({"error": "the sparse solver path runs on one GPU; run with --gpus 1 "
+                                   "(sharding across GPUs is a design note, not implemented)"}))
         return 2
 
     import numpy as np
@@ -441,8 +441,22 @@ def main():
         py = torch.empty(pn, dtype=torch.float64, device=dev)
         _, ms = time_kernel_loop(lambda: P.spmv(px, py), args.steps, args.warmup, stream)
         secondary["csr_f64_powerlaw1M_GBps"] = round(spmv_bytes(pn, pnnz, 8) / (ms * 1e-3) / 1e9, 1)
+        P32 = D.DeviceCsr(prp, pci, pv.astype(np.float32), strategy="load_balance", device=dev)
+        px32, py32 = px.float(), torch.empty(pn, dtype=torch.float32, device=dev)
+        _, ms = time_kernel_loop(lambda: P32.spmv(px32, py32), args.steps, args.warmup, stream)
+        secondary["csr_f32_powerlaw1M_GBps"] = round(spmv_bytes(pn, pnnz, 4) / (ms * 1e-3) / 1e9, 1)
         secondary["powerlaw_nnz"] = pnnz
-        del P
+        del P, P32
+        # the 27-point matrix of the CG / IR comparison (SpMV alone)
+        drp, dci, dv = gen.diffusion27(96)
+        dn, dnnz = len(drp) - 1, int(drp[-1])
+        Dm = D.DeviceCsr(drp, dci, dv, strategy="classical", device=dev)
+        dx = torch.as_tensor(np.random.default_rng(4).uniform(-1, 1, dn)).to(dev)
+        dy = torch.empty(dn, dtype=torch.float64, device=dev)
+        _, ms = time_kernel_loop(lambda: Dm.spmv(dx, dy), args.steps, args.warmup, stream)
+        secondary["csr_f64_diffusion27_96_GBps"] = round(
+            spmv_bytes(dn, dnnz, 8) / (ms * 1e-3) / 1e9, 1)
+        del Dm
     for key, name, fn in (
             ("cg_ir", "cg_vs_ir_96cubed", lambda: bench_cg_vs_ir(dev)[0]),
             ("batch", "batched_bicgstab_20000", lambda: bench_batch(dev)[0]),
