@@ -22,6 +22,7 @@ import torch
 
 from . import _capi as C
 from . import device as D
+from .device import as_2d
 
 _EXECS = {}
 
@@ -58,8 +59,8 @@ def _solve(gko, exec_, A, rhs, method, tol, max_iters, jacobi):
 
 def sampled_outer_product(row_ptrs, col_idxs, values, lam, x, alpha=-1.0):
     """out[k] = alpha * sum_j lam[row(k), j] * x[col(k), j] on the CSR pattern."""
-    lam2 = lam.reshape(lam.shape[0], -1).contiguous()
-    x2 = x.reshape(x.shape[0], -1).contiguous()
+    lam2 = as_2d(lam).contiguous()
+    x2 = as_2d(x).contiguous()
     out = torch.empty_like(values)
     d = C.Csr()
     d.n_rows, d.n_cols, d.nnz = row_ptrs.numel() - 1, x2.shape[0], values.numel()

@@ -31,6 +31,11 @@ def ctx(device=None, stream=None):
     return c
 
 
+def as_2d(t):
+    """A 1-D vector as one column, a 2-D block unchanged (also when empty)."""
+    return t.reshape(t.shape[0], 1) if t.dim() == 1 else t.reshape(t.shape[0], t.shape[1])
+
+
 def ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() > 0 else ctypes.c_void_p(0)
 
@@ -119,10 +124,10 @@ class DeviceCsr:
         return d
 
     def spmv(self, b, out=None, stream=None):
-        b2 = b.reshape(b.shape[0], -1)
+        b2 = as_2d(b)
         if out is None:
             out = torch.empty((self.n_rows, b2.shape[1]), dtype=self.dtype, device=self.device)
-        o2 = out.reshape(out.shape[0], -1)
+        o2 = as_2d(out)
         if b2.shape[0] != self.n_cols or o2.shape[0] != self.n_rows or o2.shape[1] != b2.shape[1]:
             raise C.DimensionMismatch("spmv: operand sizes do not conform")
         c = ctx(self.device, stream)
@@ -131,8 +136,8 @@ class DeviceCsr:
         return out
 
     def advanced_spmv(self, alpha, b, beta, c_vec, stream=None):
-        b2 = b.reshape(b.shape[0], -1)
-        c2 = c_vec.reshape(c_vec.shape[0], -1)
+        b2 = as_2d(b)
+        c2 = as_2d(c_vec)
         a = alpha if torch.is_tensor(alpha) else scalar(alpha, self.dtype, self.device)
         bt = beta if torch.is_tensor(beta) else scalar(beta, self.dtype, self.device)
         c = ctx(self.device, stream)
@@ -201,18 +206,18 @@ class DeviceSellp:
         self.desc = d
 
     def spmv(self, b, out=None, stream=None):
-        b2 = b.reshape(b.shape[0], -1)
+        b2 = as_2d(b)
         if out is None:
             out = torch.empty((self.n_rows, b2.shape[1]), dtype=self.dtype, device=self.device)
-        o2 = out.reshape(out.shape[0], -1)
+        o2 = as_2d(out)
         c = ctx(self.device, stream)
         C.call(f"gkomi_sellp_spmv_{suffix(self.dtype)}", ctypes.byref(c), ctypes.byref(self.desc),
                b2.shape[1], ptr(b2), b2.stride(0), ptr(o2), o2.stride(0))
         return out
 
     def advanced_spmv(self, alpha, b, beta, c_vec):
-        b2 = b.reshape(b.shape[0], -1)
-        c2 = c_vec.reshape(c_vec.shape[0], -1)
+        b2 = as_2d(b)
+        c2 = as_2d(c_vec)
         a = scalar(alpha, self.dtype, self.device)
         bt = scalar(beta, self.dtype, self.device)
         c = ctx(self.device)
@@ -243,18 +248,18 @@ class DeviceEll:
         self.desc = d
 
     def spmv(self, b, out=None, stream=None):
-        b2 = b.reshape(b.shape[0], -1)
+        b2 = as_2d(b)
         if out is None:
             out = torch.empty((self.n_rows, b2.shape[1]), dtype=self.dtype, device=self.device)
-        o2 = out.reshape(out.shape[0], -1)
+        o2 = as_2d(out)
         c = ctx(self.device, stream)
         C.call(f"gkomi_ell_spmv_{suffix(self.dtype)}", ctypes.byref(c), ctypes.byref(self.desc),
                b2.shape[1], ptr(b2), b2.stride(0), ptr(o2), o2.stride(0))
         return out
 
     def advanced_spmv(self, alpha, b, beta, c_vec):
-        b2 = b.reshape(b.shape[0], -1)
-        c2 = c_vec.reshape(c_vec.shape[0], -1)
+        b2 = as_2d(b)
+        c2 = as_2d(c_vec)
         a = scalar(alpha, self.dtype, self.device)
         bt = scalar(beta, self.dtype, self.device)
         c = ctx(self.device)
@@ -265,8 +270,8 @@ class DeviceEll:
 
 
 def dot(x, y):
-    x2 = x.reshape(x.shape[0], -1)
-    y2 = y.reshape(y.shape[0], -1)
+    x2 = as_2d(x)
+    y2 = as_2d(y)
     out = torch.empty(x2.shape[1], dtype=x.dtype, device=x.device)
     wsb = C.lib().gkomi_reduction_workspace_bytes(x2.shape[0], x2.shape[1])
     ws = workspace(wsb, x.device)
@@ -278,8 +283,8 @@ def dot(x, y):
 
 def copy(x, y):
     """y = x for device tensors of equal shape (Dense::copy_from)."""
-    x2 = x.reshape(x.shape[0], -1)
-    y2 = y.reshape(y.shape[0], -1)
+    x2 = as_2d(x)
+    y2 = as_2d(y)
     c = ctx(x.device)
     C.call(f"gkomi_dense_copy_{suffix(x.dtype)}", ctypes.byref(c), x2.shape[0], x2.shape[1],
            ptr(x2), x2.stride(0), ptr(y2), y2.stride(0))
@@ -287,7 +292,7 @@ def copy(x, y):
 
 
 def norm2(x):
-    x2 = x.reshape(x.shape[0], -1)
+    x2 = as_2d(x)
     out = torch.empty(x2.shape[1], dtype=x.dtype, device=x.device)
     wsb = C.lib().gkomi_reduction_workspace_bytes(x2.shape[0], x2.shape[1])
     ws = workspace(wsb, x.device)

@@ -82,7 +82,7 @@ batch = types.SimpleNamespace(
 
 def dense_view(exec, t):
     """Non-owning Dense view of a 1-D or 2-D device tensor (kept alive by the caller)."""
-    t2 = t.reshape(t.shape[0], -1) if t.dim() != 2 else t
+    t2 = t.reshape(t.shape[0], 1) if t.dim() == 1 else t
     if t2.stride(-1) != 1:
         raise NotSupported("Dense views need unit column stride")
     cls = getattr(_g, "Dense" + _sfx(t.dtype))
