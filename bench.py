@@ -14,7 +14,9 @@ The line also carries
   * stream_copy_GBps: a device-to-device copy measured in the same run;
   * cpu_baseline: the oracle's OpenMP CSR SpMV (restated from
     omp/matrix/csr_kernels.cpp) on the host cores, same matrix;
-  * secondary: further measurements of the path (FP32, SELL-P, power-law).
+  * secondary: further measurements of the path (FP32, SELL-P, power-law and
+    27-point SpMV; CG + Jacobi on the 256^2 Poisson problem beside the host
+    solves; FP64 CG vs mixed IR; the uneven batch; the adjoint solves).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -105,6 +107,103 @@ def timed(fn):
     out = fn()
     torch.cuda.synchronize()
     return time.perf_counter() - t0, out
+
+
+def bench_poisson_cg(dev, nx=256):
+    """2-D 5-point Poisson 256^2 (diagonal 4, off-diagonals -1), x_true ~
+    U[-1,1] (seed 1), b = A x_true: CG with scalar Jacobi to 1e-10 relative
+    residual (rhs_norm), at most 2000 iterations.  The device solve goes
+    through the gko:: C++ solver; the same solve runs on the host through the
+    oracle's sequential (reference executor) and OpenMP restatements."""
+    import numpy as np
+    import torch
+    import ginkgo_amd.gko as gko
+    import oracle as ora
+    from ginkgo_amd import generators as gen
+    rp, ci, v = gen.poisson2d(nx)
+    n, nnz = len(rp) - 1, int(rp[-1])
+    x_true = np.random.default_rng(1).uniform(-1, 1, (n, 1))
+    b_h = ora.csr_spmv(rp, ci, v, x_true)
+    exec_ = gko.Mi355xExecutor.create(0, torch.cuda.current_stream(dev).cuda_stream)
+    A = gko.csr_from_host(exec_, rp, ci, v, strategy="classical")
+    b = gko.matrix.Dense("f64").from_numpy(exec_, b_h)
+    s = (gko.solver.Cg.build("f64")
+         .with_criteria(gko.stop.Iteration.build().with_max_iters(2000),
+                        gko.stop.ResidualNorm.build("f64").with_reduction_factor(1e-10))
+         .with_preconditioner(gko.preconditioner.Jacobi.build("f64").with_max_block_size(1))
+         .on(exec_).generate(A))
+    best = None
+    for _ in range(5):
+        x = gko.matrix.Dense("f64").create(exec_, n, 1)
+        x.fill(0.0)
+        t, _ = timed(lambda: s.apply(b, x))
+        best = t if best is None else min(best, t)
+    xs = x.to_numpy()
+    r = b_h - ora.csr_spmv(rp, ci, v, xs)
+    its = int(s.last_iterations)
+    b_it = nnz * 12 + 4 * (n + 1) + 18 * n * 8 + n * 8
+    out = {"n": n, "nnz": nnz,
+           "gpu": {"time_s": round(best, 6), "iterations": its,
+                   "true_relres": float(np.linalg.norm(r) / np.linalg.norm(b_h)),
+                   "max_abs_err_vs_x_true": float(np.max(np.abs(xs - x_true))),
+                   "GBps_per_iteration_model": round(its * b_it / best / 1e9, 1)}}
+    crit = [{"kind": "iteration", "max_iters": 2000},
+            {"kind": "residual_norm", "reduction_factor": 1e-10}]
+    inv_diag = ora.jacobi_inverse_diagonal(rp, ci, v)
+    for name, omp in (("cpu_reference_sequential", False), ("cpu_omp", True)):
+        t0 = time.perf_counter()
+        xo, so, io = ora.cg_solve(rp, ci, v, b_h, np.zeros_like(b_h), crit, inv_diag=inv_diag,
+                                  omp=omp)
+        t = time.perf_counter() - t0
+        out[name] = {"time_s": round(t, 4), "iterations": int(io.iterations),
+                     "cores": ora.omp_threads() if omp else 1,
+                     "max_abs_err_vs_x_true": float(np.max(np.abs(xo - x_true)))}
+    out["speedup_gpu_over_cpu_omp"] = round(out["cpu_omp"]["time_s"] / best, 2)
+    return out
+
+
+def bench_bicgstab(dev, nx=512):
+    """One right-hand side of the nonsymmetric 512^2 convection-diffusion
+    problem (upwind convection 10 in +x), BiCGSTAB + scalar Jacobi to 1e-12:
+    the fused single-RHS iteration (classical Csr) against the generic,
+    reference-ordered path (the same matrix with the load_balance strategy),
+    both through gko::solver::Bicgstab."""
+    import numpy as np
+    import torch
+    import ginkgo_amd.gko as gko
+    import oracle as ora
+    from ginkgo_amd import generators as gen
+    rp, ci, v, *_ = gen.anisotropic2d(nx, seed=6, convection=10.0)
+    n, nnz = len(rp) - 1, int(rp[-1])
+    b_h = np.random.default_rng(7).uniform(-1, 1, (n, 1))
+    exec_ = gko.Mi355xExecutor.create(0, torch.cuda.current_stream(dev).cuda_stream)
+    b = gko.matrix.Dense("f64").from_numpy(exec_, b_h)
+    # reference byte model per iteration (core/solver/bicgstab.cpp:154-164)
+    b_it = 2 * (nnz * 12 + 4 * (n + 1) + n * 8) + 31 * n * 8
+    out = {"n": n, "nnz": nnz}
+    for name, strategy in (("fused", "classical"), ("generic", "load_balance")):
+        A = gko.csr_from_host(exec_, rp, ci, v, strategy=strategy)
+        s = (gko.solver.Bicgstab.build("f64")
+             .with_criteria(gko.stop.Iteration.build().with_max_iters(20000),
+                            gko.stop.ResidualNorm.build("f64").with_reduction_factor(1e-12))
+             .with_preconditioner(gko.preconditioner.Jacobi.build("f64").with_max_block_size(1))
+             .on(exec_).generate(A))
+        best = None
+        for _ in range(3):
+            x = gko.matrix.Dense("f64").create(exec_, n, 1)
+            x.fill(0.0)
+            t, _ = timed(lambda: s.apply(b, x))
+            best = t if best is None else min(best, t)
+        xs = x.to_numpy()
+        r = b_h - ora.csr_spmv(rp, ci, v, xs)
+        its = int(s.last_iterations)
+        out[name] = {"time_s": round(best, 5), "iterations": its,
+                     "used_fused_path": bool(s.last_used_fused_path),
+                     "true_relres": float(np.linalg.norm(r) / np.linalg.norm(b_h)),
+                     "GBps_reference_model": round(its * b_it / best / 1e9, 1)}
+    out["speedup_fused_over_generic"] = round(out["generic"]["time_s"] / out["fused"]["time_s"],
+                                              2)
+    return out
 
 
 def bench_cg_vs_ir(dev, nx=96):
@@ -246,18 +345,22 @@ def edge_rank_one(rows_of, vmap, coef, e):
     return [a, b], [1.0, -1.0], cval
 
 
-def bench_adjoint(dev, nx=512, nrhs=64, fd_entries=32, h=1e-6):
-    """512^2 anisotropic diffusion, 64 right-hand sides: forward FP64 CG to
-    1e-12 plus the adjoint backward (A^T solve + sampled outer product),
-    timed; then the gradient checked against central differences of the loss
-    on `fd_entries` random conductivities with step h.
+def bench_adjoint(dev, nx=512, nrhs=64, fd_entries=32, h=1e-6, convection=0.0, method="cg"):
+    """512^2 anisotropic diffusion, 64 right-hand sides: forward FP64 solve to
+    1e-12 (CG; BiCGSTAB for the nonsymmetric convection-diffusion variant,
+    upwind convection of strength `convection` in +x, cell Peclet number
+    convection / theta ~ 10 at convection = 10) plus the adjoint backward
+    (A^T solve with the same method + sampled outer product), timed; then the
+    gradient checked against central differences of the loss on `fd_entries`
+    random conductivities with step h.
 
     Each conductivity enters A as c u u^T (u = e_a - e_b, or e_a at the
     boundary), so A(theta +- h e_i) = A0 +- h c u u^T and, by the
     Sherman-Morrison identity, x(theta +- h e_i) = x0 - (+-hc)(u^T x0) w /
     (1 +- hc s) with w = A0^{-1} u, s = u^T w.  The central difference
     [L(theta + h e_i) - L(theta - h e_i)] / 2h is evaluated from these exact
-    expressions:
+    expressions (the identity needs no symmetry, so it also holds with the
+    constant convection part in A0):
         (L+ - L-) / 2h = -(c/D) sum_k (u^T x0_k) [w.(x0_k - xt_k)
                                                  + (hc)^2 s (u^T x0_k) w.w / D],
         D = 1 - (hcs)^2.
@@ -269,8 +372,8 @@ def bench_adjoint(dev, nx=512, nrhs=64, fd_entries=32, h=1e-6):
     import torch
     from ginkgo_amd import generators as gen
     from ginkgo_amd.autograd import sparse_solve
-    log("[bench] adjoint: generating 512^2 anisotropic diffusion")
-    rp, ci, vals, theta, vmap, coef = gen.anisotropic2d(nx, seed=6)
+    log(f"[bench] adjoint: generating 512^2 anisotropic diffusion, convection {convection}")
+    rp, ci, vals, theta, vmap, coef = gen.anisotropic2d(nx, seed=6, convection=convection)
     n, nnz = nx * nx, len(ci)
     vmap_t = torch.as_tensor(vmap).to(dev)
     coef_t = torch.as_tensor(coef).to(dev)
@@ -282,13 +385,17 @@ def bench_adjoint(dev, nx=512, nrhs=64, fd_entries=32, h=1e-6):
     rp_t = torch.as_tensor(rp).to(dev)
     ci_t = torch.as_tensor(ci).to(dev)
 
-    def values_of(t):
-        return torch.zeros(nnz, dtype=torch.float64, device=dev).index_add(
-            0, vmap_t[:, 0], coef_t * t[vmap_t[:, 1]])
+    def values_of(t, const=None):
+        base = torch.zeros(nnz, dtype=torch.float64, device=dev) if const is None else const
+        return base.index_add(0, vmap_t[:, 0], coef_t * t[vmap_t[:, 1]])
+    # the part of the values that does not depend on theta (the convection)
+    const = torch.as_tensor(vals).to(dev) - values_of(th0)
+    values_of_theta = lambda t: values_of(t, const)
 
     def run():
         th.grad = None
-        x = sparse_solve(values_of(th), b, rp_t, ci_t, method="cg", tol=1e-12, max_iters=20000)
+        x = sparse_solve(values_of_theta(th), b, rp_t, ci_t, method=method, tol=1e-12,
+                         max_iters=20000)
         loss = 0.5 * ((x - xt) ** 2).sum()
         loss.backward()
         return loss, x
@@ -297,7 +404,8 @@ def bench_adjoint(dev, nx=512, nrhs=64, fd_entries=32, h=1e-6):
     t2, (loss, x0) = timed(run)
     g = th.grad.detach().clone()
     out = {"time_s_forward_backward": round(min(t, t2), 4), "n": n, "nnz": nnz, "nrhs": nrhs,
-           "loss": float(loss.item())}
+           "method": method, "convection": convection, "loss": float(loss.item()),
+           "symmetric": bool(torch.all(const == 0).item())}
     if fd_entries > 0:
         log(f"[bench] adjoint: central differences on {fd_entries} entries, h = {h}")
         t0 = time.perf_counter()
@@ -312,7 +420,7 @@ def bench_adjoint(dev, nx=512, nrhs=64, fd_entries=32, h=1e-6):
             for r_, s_ in zip(sup, sgn):
                 U[r_, m] = s_
             shapes.append((sup, sgn, c))
-        W = sparse_solve(values_of(th0), U, rp_t, ci_t, method="cg", tol=1e-12,
+        W = sparse_solve(values_of_theta(th0), U, rp_t, ci_t, method=method, tol=1e-12,
                          max_iters=20000)
         fd = []
         for m, (sup, sgn, c) in enumerate(shapes):
@@ -344,7 +452,8 @@ def main():
     ap.add_argument("--quick", action="store_true", help="skip secondary measurements")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--only", action="append", default=None,
-                    choices=["spmv", "cg_ir", "batch", "adjoint"],
+                    choices=["spmv", "poisson", "bicgstab", "cg_ir", "batch", "adjoint",
+                             "adjoint_convection"],
                     help="run only these secondary sections (repeatable)")
     ap.add_argument("--fd-entries", type=int, default=32,
                     help="conductivities checked by central differences in the adjoint section")
@@ -458,10 +567,15 @@ def main():
             spmv_bytes(dn, dnnz, 8) / (ms * 1e-3) / 1e9, 1)
         del Dm
     for key, name, fn in (
+            ("poisson", "cg_jacobi_poisson256sq", lambda: bench_poisson_cg(dev)),
+            ("bicgstab", "bicgstab_jacobi_convection512sq", lambda: bench_bicgstab(dev)),
             ("cg_ir", "cg_vs_ir_96cubed", lambda: bench_cg_vs_ir(dev)[0]),
             ("batch", "batched_bicgstab_20000", lambda: bench_batch(dev)[0]),
             ("adjoint", "adjoint_512sq_64rhs",
-             lambda: bench_adjoint(dev, fd_entries=args.fd_entries))):
+             lambda: bench_adjoint(dev, fd_entries=args.fd_entries)),
+            ("adjoint_convection", "adjoint_convection_512sq_64rhs",
+             lambda: bench_adjoint(dev, fd_entries=args.fd_entries, convection=10.0,
+                                   method="bicgstab"))):
         if not want(key):
             continue
         t0 = time.perf_counter()

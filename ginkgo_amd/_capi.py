@@ -138,6 +138,17 @@ class BlockCgState(ctypes.Structure):
                 ("issued", ctypes.c_int64)]
 
 
+class BicgstabState(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("x", ctypes.c_void_p), ("b", ctypes.c_void_p),
+                ("r", ctypes.c_void_p), ("rr", ctypes.c_void_p), ("p", ctypes.c_void_p),
+                ("v", ctypes.c_void_p), ("s", ctypes.c_void_p), ("t", ctypes.c_void_p),
+                ("y", ctypes.c_void_p), ("z", ctypes.c_void_p),
+                ("inv_diag", ctypes.c_void_p), ("scalars", ctypes.c_void_p),
+                ("orig_tau", ctypes.c_void_p), ("status", ctypes.c_void_p),
+                ("flags", ctypes.c_void_p), ("work", ctypes.c_void_p),
+                ("issued", ctypes.c_int64)]
+
+
 BATCH_MAX_BINS = 8
 
 
@@ -212,6 +223,7 @@ _SIGS = {
     "gkomi_cg_current_p": (_P, [ctypes.POINTER(CgState)]),
     "gkomi_block_cg_work_bytes": (_SZ, [_I64, _I64]),
     "gkomi_block_cg_current_p": (_P, [ctypes.POINTER(BlockCgState)]),
+    "gkomi_bicgstab_work_bytes": (_SZ, [_I64]),
     "gkomi_batch_workspace_bytes": (_SZ, [ctypes.POINTER(BatchCsr)]),
     "gkomi_batch_lds_bytes": (_SZ, [_I32, _I64, _I64, _I32]),
 }
@@ -266,6 +278,13 @@ for _s in ("f64", "f32"):
         f"gkomi_block_cg_iterate_{_s}": (ctypes.c_int, [_CTX, ctypes.POINTER(Csr),
                                                         ctypes.POINTER(Criteria),
                                                         ctypes.POINTER(BlockCgState), _I32]),
+        f"gkomi_bicgstab_fused_start_{_s}": (ctypes.c_int, [_CTX, ctypes.POINTER(Csr),
+                                                            ctypes.POINTER(Criteria),
+                                                            ctypes.POINTER(BicgstabState)]),
+        f"gkomi_bicgstab_fused_iterate_{_s}": (ctypes.c_int, [_CTX, ctypes.POINTER(Csr),
+                                                              ctypes.POINTER(Criteria),
+                                                              ctypes.POINTER(BicgstabState),
+                                                              _I32]),
         f"gkomi_bicgstab_initialize_{_s}": (ctypes.c_int, [_CTX, _I64, _I64, _P, _I64] +
                                             [_P] * 8 + [_I64] + [_P] * 7),
         f"gkomi_bicgstab_step_1_{_s}": (ctypes.c_int, [_CTX, _I64, _I64, _P, _P, _P, _I64, _P,

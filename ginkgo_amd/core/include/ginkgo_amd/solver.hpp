@@ -265,6 +265,7 @@ protected:
 
 private:
     void apply_dense(const matrix::Dense<ValueType>* b, matrix::Dense<ValueType>* x) const;
+    bool try_fused(const matrix::Dense<ValueType>* b, matrix::Dense<ValueType>* x) const;
 };
 
 // solver::Ir (core/solver/ir.cpp:193-257): x += w * inner_solve(b - A x).

@@ -8,6 +8,8 @@
 // The fused block path does the same for several right-hand sides over any
 // Csr.  The generic path follows core/solver/cg.cpp step by step with the
 // reference-semantics kernels and handles any LinOp system matrix.
+// Bicgstab has the fused path for one right-hand side (five kernels per
+// iteration, both checks on the device) and the generic path otherwise.
 #include <cmath>
 
 #include "ginkgo_amd/dispatch.hpp"
@@ -37,6 +39,8 @@ struct sk<double> {
     static constexpr auto bicg_step_2 = gkomi_bicgstab_step_2_f64;
     static constexpr auto bicg_step_3 = gkomi_bicgstab_step_3_f64;
     static constexpr auto bicg_finalize = gkomi_bicgstab_finalize_f64;
+    static constexpr auto bicg_fused_start = gkomi_bicgstab_fused_start_f64;
+    static constexpr auto bicg_fused_iterate = gkomi_bicgstab_fused_iterate_f64;
     static constexpr auto check = gkomi_criteria_check_f64;
     static constexpr auto fill = gkomi_fill_f64;
 };
@@ -58,6 +62,8 @@ struct sk<float> {
     static constexpr auto bicg_step_2 = gkomi_bicgstab_step_2_f32;
     static constexpr auto bicg_step_3 = gkomi_bicgstab_step_3_f32;
     static constexpr auto bicg_finalize = gkomi_bicgstab_finalize_f32;
+    static constexpr auto bicg_fused_start = gkomi_bicgstab_fused_start_f32;
+    static constexpr auto bicg_fused_iterate = gkomi_bicgstab_fused_iterate_f32;
     static constexpr auto check = gkomi_criteria_check_f32;
     static constexpr auto fill = gkomi_fill_f32;
 };
@@ -157,6 +163,28 @@ gkomi_stop_flags read_flags(const std::shared_ptr<const Executor>& exec,
                             const gkomi_stop_flags* dflags)
 {
     return exec->copy_val_to_host(dflags);
+}
+
+// Lagged polling of a fused solve: `iterate()` enqueues one chunk of
+// iterations; two chunks stay in flight, and the host waits only for the
+// older chunk's copy of the device flags before deciding to go on.
+template <typename Iterate>
+void run_until_stopped(const gkomi_ctx* ctx, const gkomi_stop_flags* dflags, Iterate iterate)
+{
+    pinned_flags slots(2);
+    event e0(ctx->device), e1(ctx->device);
+    event* evs[2] = {&e0, &e1};
+    auto enqueue = [&](int slot) {
+        iterate();
+        GKO_CALL(gkomi_memcpy_async(ctx, &slots.host[slot], dflags, sizeof(gkomi_stop_flags)));
+        GKO_CALL(gkomi_event_record(ctx, evs[slot]->ev));
+    };
+    enqueue(0);
+    for (int64 i = 0;; ++i) {
+        enqueue(static_cast<int>((i + 1) % 2));
+        GKO_CALL(gkomi_event_synchronize(evs[i % 2]->ev));
+        if (slots.host[i % 2].all_stopped) break;
+    }
 }
 
 }  // namespace
@@ -486,11 +514,67 @@ void Bicgstab<V>::apply_impl(const LinOp* b, LinOp* x) const
     precision_dispatch<V>([this](auto db, auto dx) { this->apply_dense(db, dx); }, b, x);
 }
 
+// One right-hand side over a classical Csr with the identity or scalar
+// Jacobi: the fused iteration (five kernels, checks on the device).
+template <typename V>
+bool Bicgstab<V>::try_fused(const Vec<V>* b, Vec<V>* x) const
+{
+    auto csr = dynamic_cast<const matrix::Csr<V, int32>*>(this->system_matrix_.get());
+    if (!csr || b->get_size().cols != 1 || b->get_stride() != 1 || x->get_stride() != 1) {
+        return false;
+    }
+    if (csr->get_effective_strategy() != "classical") return false;
+    if (this->precond_ && !this->inv_diag_) return false;
+    auto exec = this->get_executor();
+    const size_type n = b->get_size().rows;
+    const size_type nm = this->inv_diag_ ? n : 0;
+    array<V> r(exec, n), rr(exec, n), p(exec, n), v(exec, n), s(exec, n), t(exec, n);
+    array<V> y(exec, nm), z(exec, nm);
+    array<V> scalars(exec, 8), orig(exec, GKOMI_MAX_CRITERIA);
+    array<uint8> status(exec, 1);
+    array<char> flags(exec, sizeof(gkomi_stop_flags));
+    array<char> work(exec, gkomi_bicgstab_work_bytes(static_cast<int64_t>(n)));
+    gkomi_bicgstab_state st{};
+    st.n = static_cast<int64_t>(n);
+    st.x = x->get_values();
+    st.b = b->get_const_values();
+    st.r = r.get_data();
+    st.rr = rr.get_data();
+    st.p = p.get_data();
+    st.v = v.get_data();
+    st.s = s.get_data();
+    st.t = t.get_data();
+    st.y = nm ? y.get_data() : nullptr;
+    st.z = nm ? z.get_data() : nullptr;
+    st.inv_diag = this->inv_diag_;
+    st.scalars = scalars.get_data();
+    st.orig_tau = orig.get_data();
+    st.status = status.get_data();
+    st.flags = reinterpret_cast<gkomi_stop_flags*>(flags.get_data());
+    st.work = work.get_data();
+    const auto d = csr->descriptor();
+    const gkomi_ctx* ctx = ctx_of(exec);
+    GKO_CALL(sk<V>::bicg_fused_start(ctx, &d, &this->crit_, &st));
+    run_until_stopped(ctx, st.flags, [&] {
+        GKO_CALL(sk<V>::bicg_fused_iterate(ctx, &d, &this->crit_, &st, this->chunk_));
+    });
+    exec->synchronize();
+    const auto f = read_flags(exec, st.flags);
+    this->stats_.iterations = f.iteration;
+    this->stats_.status = host_status(status);
+    this->stats_.used_fused_path = true;
+    return true;
+}
+
 // core/solver/bicgstab.cpp:95-255 with device-side checks.
 template <typename V>
 void Bicgstab<V>::apply_dense(const Vec<V>* b, Vec<V>* x) const
 {
     if (!this->system_matrix_) return;
+    if (try_fused(b, x)) {
+        this->log_result(b, x);
+        return;
+    }
     auto exec = this->get_executor();
     const auto ctx = ctx_of(exec);
     const size_type n = b->get_size().rows, k = b->get_size().cols;

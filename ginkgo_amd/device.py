@@ -364,6 +364,60 @@ class FusedCg:
         return int(self.flags[0].item())
 
 
+class FusedBicgstab:
+    """Single-RHS BiCGSTAB on the fused gfx950 kernels (five per iteration,
+    both convergence checks on the device), polled once per chunk."""
+
+    def __init__(self, csr, criteria_spec, jacobi=False, chunk=16):
+        if csr.strategy != "classical":
+            raise C.NotSupported("fused BiCGSTAB needs the classical CSR strategy")
+        self.A = csr
+        self.crit = C.criteria(criteria_spec)
+        self.inv_diag = csr.jacobi() if jacobi else None
+        self.chunk = int(chunk)
+        dev, dt, n = csr.device, csr.dtype, csr.n_rows
+        names = ("r", "rr", "p", "v", "s", "t") + (("y", "z") if jacobi else ())
+        self.vec = {k: torch.empty(n, dtype=dt, device=dev) for k in names}
+        self.scalars = torch.zeros(8, dtype=dt, device=dev)
+        self.orig = torch.zeros(C.MAX_CRITERIA, dtype=dt, device=dev)
+        self.status = torch.zeros(1, dtype=torch.uint8, device=dev)
+        self.flags = torch.zeros(2, dtype=torch.int64, device=dev)
+        self.work = workspace(C.lib().gkomi_bicgstab_work_bytes(n), dev)
+        self.flags_host = torch.zeros(2, dtype=torch.int64).pin_memory()
+
+    def _state(self, b, x):
+        st = C.BicgstabState()
+        st.n = self.A.n_rows
+        st.x, st.b = ptr(x).value, ptr(b).value
+        for k in ("r", "rr", "p", "v", "s", "t", "y", "z"):
+            setattr(st, k, ptr(self.vec[k]).value if k in self.vec else None)
+        st.inv_diag = ptr(self.inv_diag).value if self.inv_diag is not None else None
+        st.scalars, st.orig_tau = ptr(self.scalars).value, ptr(self.orig).value
+        st.status, st.flags, st.work = (ptr(self.status).value, ptr(self.flags).value,
+                                        ptr(self.work).value)
+        return st
+
+    def apply(self, b, x, max_chunks=1 << 30):
+        """Solve A x = b in place (x is the initial guess).  Returns the
+        iteration count of the reference's loop (core/solver/bicgstab.cpp)."""
+        st = self._state(b, x)
+        c = ctx(self.A.device)
+        sfx = suffix(self.A.dtype)
+        C.call(f"gkomi_bicgstab_fused_start_{sfx}", ctypes.byref(c), ctypes.byref(self.A.desc),
+               ctypes.byref(self.crit), ctypes.byref(st))
+        done = False
+        chunks = 0
+        while not done and chunks < max_chunks:
+            C.call(f"gkomi_bicgstab_fused_iterate_{sfx}", ctypes.byref(c),
+                   ctypes.byref(self.A.desc), ctypes.byref(self.crit), ctypes.byref(st),
+                   self.chunk)
+            self.flags_host.copy_(self.flags, non_blocking=True)
+            torch.cuda.current_stream(self.A.device).synchronize()
+            done = (int(self.flags_host[1].item()) & 0xffffffff) != 0
+            chunks += 1
+        return int(self.flags_host[0].item())
+
+
 class DeviceBatchCsr:
     """A batch of independent systems on the device.
 

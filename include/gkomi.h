@@ -499,6 +499,58 @@ int gkomi_block_cg_iterate_f32(const gkomi_ctx* ctx, const gkomi_csr* a,
 /* Pointer to the p buffer that holds the current search directions. */
 void* gkomi_block_cg_current_p(const gkomi_block_cg_state* st);
 
+/* ------------------------------------------ BiCGSTAB (fused, gfx950) */
+
+/* Device state of a fused BiCGSTAB solve with one right-hand side
+ * (core/solver/bicgstab.cpp:95-255).  All vectors are device memory of n
+ * values of the entry point's type.  y = M p and z = M s are only needed
+ * with a preconditioner (inv_diag non-NULL); without one they may be NULL
+ * (p and s are used, as the reference's identity copy would give).
+ * `scalars` holds 8 values, `orig_tau` GKOMI_MAX_CRITERIA; `work`
+ * gkomi_bicgstab_work_bytes(n) bytes (no initialisation needed); `issued`
+ * as in gkomi_cg_state. */
+typedef struct {
+    int64_t n;
+    void* x;        /* solution (in: initial guess) */
+    const void* b;  /* right-hand side */
+    void* r;
+    void* rr;       /* the shadow residual r0 */
+    void* p;
+    void* v;
+    void* s;
+    void* t;
+    void* y;
+    void* z;
+    const void* inv_diag;
+    void* scalars;
+    void* orig_tau;
+    uint8_t* status;
+    gkomi_stop_flags* flags;
+    void* work;
+    int64_t issued;
+} gkomi_bicgstab_state;
+
+size_t gkomi_bicgstab_work_bytes(int64_t n);
+/* Start (bicgstab.cpp:103-146; initialize,
+ * reference/solver/bicgstab_kernels.cpp:26-58): r = b - A x, rr = r,
+ * p = v = 0, scalars = 1, the criteria baselines -- on the device. */
+int gkomi_bicgstab_fused_start_f64(const gkomi_ctx* ctx, const gkomi_csr* a,
+                                   const gkomi_criteria* crit, gkomi_bicgstab_state* st);
+int gkomi_bicgstab_fused_start_f32(const gkomi_ctx* ctx, const gkomi_csr* a,
+                                   const gkomi_criteria* crit, gkomi_bicgstab_state* st);
+/* `count` iterations (bicgstab.cpp:158-253), five kernels each: rho = rr.r
+ * with the Combined check and step_1 (p, y = M p); v = A y with rr.v;
+ * step_2 (alpha, s, z = M s) with s.s; the half-step check with t = A z,
+ * s.t and t.t; step_3 (omega, x, r) or, after a half-step stop, finalize.
+ * Once flags->all_stopped is set the remaining kernels return immediately,
+ * so the host may enqueue ahead. */
+int gkomi_bicgstab_fused_iterate_f64(const gkomi_ctx* ctx, const gkomi_csr* a,
+                                     const gkomi_criteria* crit, gkomi_bicgstab_state* st,
+                                     int32_t count);
+int gkomi_bicgstab_fused_iterate_f32(const gkomi_ctx* ctx, const gkomi_csr* a,
+                                     const gkomi_criteria* crit, gkomi_bicgstab_state* st,
+                                     int32_t count);
+
 /* ------------------------------------------- BiCGSTAB (reference semantics) */
 
 /* bicgstab::initialize / step_1 / step_2 / step_3 / finalize

@@ -61,10 +61,21 @@ def test_reference_matrices_solve(name, method):
     # the stopping test reads the recurrence residual; on 1138_bus (cond ~ 8.6e6)
     # the true residual drifts away from it by rounding
     assert res <= (2e-10 if name != "1138_bus.mtx" else 1e-8)
-    # iteration counts of the same recurrence; 1138_bus is ill-conditioned, so
-    # rounding differences move the count more there
-    band = 3 if name != "1138_bus.mtx" else max(5, io.iterations // 20)
-    assert abs(s.last_iterations - io.iterations) <= band, (s.last_iterations, io.iterations)
+    # iteration counts of the same recurrence.  How far rounding alone moves
+    # the count depends on the matrix (BiCGSTAB on the 36-row ani1_nonsymm
+    # needs more iterations than rows, and relative perturbations of b of
+    # 1e-15 move the oracle's own count between 59 and 69), so the band is
+    # the oracle's range over such perturbations, widened by 3 (by 5% on the
+    # ill-conditioned 1138_bus, cond ~ 8.6e6, where a different summation
+    # order perturbs more than a last-digit change of b).
+    rng = np.random.default_rng(0)
+    counts = [io.iterations]
+    for _ in range(10):
+        bp = b_h * (1 + 1e-15 * rng.standard_normal(b_h.shape))
+        counts.append(solve(rp, ci, v, bp, np.zeros_like(b_h), spec)[2].iterations)
+    widen = 3 if name != "1138_bus.mtx" else max(5, io.iterations // 20)
+    lo, hi = min(counts) - widen, max(counts) + widen
+    assert lo <= s.last_iterations <= hi, (s.last_iterations, counts)
 
 
 def test_simple_solver_harness(tmp_path):
