@@ -226,6 +226,7 @@ _SIGS = {
     "gkomi_bicgstab_work_bytes": (_SZ, [_I64]),
     "gkomi_batch_workspace_bytes": (_SZ, [ctypes.POINTER(BatchCsr)]),
     "gkomi_batch_lds_bytes": (_SZ, [_I32, _I64, _I64, _I32]),
+    "gkomi_batch_cg_lds_bytes": (_SZ, [_I32, _I64, _I64, _I32]),
 }
 
 for _s in ("f64", "f32"):
@@ -306,6 +307,14 @@ for _s in ("f64", "f32"):
                                                           ctypes.POINTER(BatchSettings),
                                                           ctypes.POINTER(BatchPlan), _P, _P, _P,
                                                           _P, _P, _P, _SZ]),
+        f"gkomi_batch_cg_solve_{_s}": (ctypes.c_int, [_CTX, ctypes.POINTER(BatchCsr),
+                                                     ctypes.POINTER(BatchSettings),
+                                                     ctypes.POINTER(BatchPlan), _P, _P, _P, _P,
+                                                     _P, _SZ]),
+        f"gkomi_batch_cg_tune_{_s}": (ctypes.c_int, [_CTX, ctypes.POINTER(BatchCsr),
+                                                    ctypes.POINTER(BatchSettings),
+                                                    ctypes.POINTER(BatchPlan), _P, _P, _P, _P,
+                                                    _P, _P, _SZ]),
     })
 
 _lib = None

@@ -6,8 +6,10 @@
 //   batch::stop::tolerance_type include/ginkgo/core/stop/batch_stop_enum.hpp
 //   batch::preconditioner::Jacobi (scalar) include/ginkgo/core/preconditioner/batch_jacobi.hpp
 //   batch::solver::Bicgstab     include/ginkgo/core/solver/batch_bicgstab.hpp,
-//                               batch_solver_base.hpp (defaults: 100 iterations,
-//                               tolerance 1e-11, absolute)
+//   batch::solver::Cg           include/ginkgo/core/solver/batch_cg.hpp,
+//                               both with batch_solver_base.hpp:161-178
+//                               (defaults: 100 iterations, tolerance 1e-11,
+//                               absolute)
 //   batch::log::BatchConvergence include/ginkgo/core/log/batch_logger.hpp
 // plus one extension with no reference counterpart: batch::matrix::BlockDiagCsr,
 // systems of different sizes in one block-diagonal CSR (the reference's batch
@@ -207,11 +209,18 @@ private:
 
 namespace solver {
 
+// The iterative batch solvers share their parameters, factory, plan and
+// launch machinery; they differ in the per-system iteration.
+enum class batch_method { bicgstab, cg };
+
 // batch::solver::Bicgstab (include/ginkgo/core/solver/batch_bicgstab.hpp):
 // an independent BiCGSTAB per item, no breakdown guards, the half-step exit
 // (reference/solver/batch_bicgstab_kernels.hpp:180-354).
-template <typename ValueType>
-class Bicgstab {
+// batch::solver::Cg (include/ginkgo/core/solver/batch_cg.hpp): an
+// independent preconditioned CG per item, stopped on the implicit norm
+// sqrt|r.z| (reference/solver/batch_cg_kernels.hpp:111-221).
+template <typename ValueType, batch_method Method>
+class BatchKrylov {
 public:
     using value_type = ValueType;
     using precond_params = typename preconditioner::Jacobi<ValueType>::parameters_type;
@@ -250,7 +259,7 @@ public:
 
     class Factory {
     public:
-        std::unique_ptr<Bicgstab> generate(std::shared_ptr<const BatchLinOp> a) const;
+        std::unique_ptr<BatchKrylov> generate(std::shared_ptr<const BatchLinOp> a) const;
         const parameters_type& get_parameters() const { return params_; }
 
     private:
@@ -279,8 +288,8 @@ public:
     const parameters_type& get_parameters() const { return params_; }
 
 private:
-    Bicgstab(std::shared_ptr<const Executor> exec, parameters_type params,
-             std::shared_ptr<const BatchLinOp> a);
+    BatchKrylov(std::shared_ptr<const Executor> exec, parameters_type params,
+                std::shared_ptr<const BatchLinOp> a);
     gkomi_batch_settings settings() const;
     void check_vectors(const MultiVector<ValueType>* b, const MultiVector<ValueType>* x) const;
 
@@ -296,6 +305,11 @@ private:
     mutable array<ValueType> residuals_;
     std::vector<std::shared_ptr<const log::BatchConvergence<ValueType>>> loggers_;
 };
+
+template <typename ValueType>
+using Bicgstab = BatchKrylov<ValueType, batch_method::bicgstab>;
+template <typename ValueType>
+using Cg = BatchKrylov<ValueType, batch_method::cg>;
 
 }  // namespace solver
 }  // namespace batch

@@ -77,8 +77,40 @@ py::array_t<T> arr(const std::vector<T>& v)
     return out;
 }
 
+// One batch solver class with its factory and parameters (batch::solver::
+// Bicgstab and Cg share the surface).
+template <typename BS>
+void bind_batch_solver(py::module_& m, const std::string& name, const std::string& sfx)
+{
+    using V = typename BS::value_type;
+    using MV = batch::MultiVector<V>;
+    using BJ = batch::preconditioner::Jacobi<V>;
+    using BL = batch::log::BatchConvergence<V>;
+    py::class_<BS, std::shared_ptr<BS>>(m, (name + sfx).c_str())
+        .def("apply", [](const BS& s, const MV* b, MV* x) { s.apply(b, x); })
+        .def("tune", [](BS& s, const MV* b, const MV* x) { s.tune(b, x); })
+        .def("add_logger", [](BS& s, std::shared_ptr<const BL> l) { s.add_logger(l); })
+        .def_property_readonly("block_sizes", &BS::get_block_sizes)
+        .def_property_readonly("matrix_in_lds", &BS::get_matrix_in_lds);
+    py::class_<typename BS::Factory, std::shared_ptr<typename BS::Factory>>(
+        m, (name + "Factory" + sfx).c_str())
+        .def("generate", [](const typename BS::Factory& f, std::shared_ptr<const batch::BatchLinOp> a) {
+            return std::shared_ptr<BS>(f.generate(std::move(a)));
+        });
+    using BP = typename BS::parameters_type;
+    py::class_<BP>(m, (name + "Params" + sfx).c_str())
+        .def(py::init<>())
+        .def("with_max_iterations", [](BP p, int v) { return p.with_max_iterations(v); })
+        .def("with_tolerance", [](BP p, double v) { return p.with_tolerance(v); })
+        .def("with_tolerance_type",
+             [](BP p, batch::stop::tolerance_type t) { return p.with_tolerance_type(t); })
+        .def("with_preconditioner",
+             [](BP p, const typename BJ::parameters_type& j) { return p.with_preconditioner(j); })
+        .def("on", [](const BP& p, std::shared_ptr<const Executor> e) { return p.on(e); });
+}
+
 // The batch module (batch::MultiVector / matrix::Csr / matrix::BlockDiagCsr /
-// solver::Bicgstab / log::BatchConvergence).
+// solver::Bicgstab / solver::Cg / log::BatchConvergence).
 template <typename V>
 void bind_batch(py::module_& m, const std::string& sfx)
 {
@@ -86,7 +118,6 @@ void bind_batch(py::module_& m, const std::string& sfx)
     using BCsr = batch::matrix::Csr<V, int32>;
     using BDiag = batch::matrix::BlockDiagCsr<V, int32>;
     using BJ = batch::preconditioner::Jacobi<V>;
-    using BS = batch::solver::Bicgstab<V>;
     using BL = batch::log::BatchConvergence<V>;
     py::class_<MV, batch::BatchLinOp, std::shared_ptr<MV>>(m, ("BatchMultiVector" + sfx).c_str())
         .def_static("create",
@@ -146,27 +177,8 @@ void bind_batch(py::module_& m, const std::string& sfx)
                                [](const BL& l) { return arr(l.get_num_iterations()); })
         .def_property_readonly("residual_norm",
                                [](const BL& l) { return arr(l.get_residual_norm()); });
-    py::class_<BS, std::shared_ptr<BS>>(m, ("BatchBicgstab" + sfx).c_str())
-        .def("apply", [](const BS& s, const MV* b, MV* x) { s.apply(b, x); })
-        .def("tune", [](BS& s, const MV* b, const MV* x) { s.tune(b, x); })
-        .def("add_logger", [](BS& s, std::shared_ptr<const BL> l) { s.add_logger(l); })
-        .def_property_readonly("block_sizes", &BS::get_block_sizes)
-        .def_property_readonly("matrix_in_lds", &BS::get_matrix_in_lds);
-    py::class_<typename BS::Factory, std::shared_ptr<typename BS::Factory>>(
-        m, ("BatchBicgstabFactory" + sfx).c_str())
-        .def("generate", [](const typename BS::Factory& f, std::shared_ptr<const batch::BatchLinOp> a) {
-            return std::shared_ptr<BS>(f.generate(std::move(a)));
-        });
-    using BP = typename BS::parameters_type;
-    py::class_<BP>(m, ("BatchBicgstabParams" + sfx).c_str())
-        .def(py::init<>())
-        .def("with_max_iterations", [](BP p, int v) { return p.with_max_iterations(v); })
-        .def("with_tolerance", [](BP p, double v) { return p.with_tolerance(v); })
-        .def("with_tolerance_type",
-             [](BP p, batch::stop::tolerance_type t) { return p.with_tolerance_type(t); })
-        .def("with_preconditioner",
-             [](BP p, const typename BJ::parameters_type& j) { return p.with_preconditioner(j); })
-        .def("on", [](const BP& p, std::shared_ptr<const Executor> e) { return p.on(e); });
+    bind_batch_solver<batch::solver::Bicgstab<V>>(m, "BatchBicgstab", sfx);
+    bind_batch_solver<batch::solver::Cg<V>>(m, "BatchCg", sfx);
 }
 
 template <typename V>

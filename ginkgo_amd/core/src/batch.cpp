@@ -1,9 +1,10 @@
 // The batch module on the Mi355xExecutor: storage, the size-binned launch
-// plan, and batch::solver::Bicgstab over gkomi_batch_bicgstab_{solve,tune}.
+// plan, and batch::solver::{Bicgstab, Cg} over gkomi_batch_{bicgstab,cg}_{solve,tune}.
 #include "ginkgo_amd/batch.hpp"
 
 #include <algorithm>
 #include <numeric>
+#include <string>
 
 namespace gko {
 namespace batch {
@@ -164,40 +165,59 @@ gkomi_batch_csr BlockDiagCsr<V, I>::descriptor() const
 
 }  // namespace matrix
 
-// ------------------------------------------------------------- Bicgstab ----
+// ------------------------------------------------------ Bicgstab and Cg ----
 
 namespace solver {
 
 namespace {
 
-template <typename V>
+template <typename V, batch_method M>
 struct bk;
 template <>
-struct bk<double> {
+struct bk<double, batch_method::bicgstab> {
     static constexpr auto solve = gkomi_batch_bicgstab_solve_f64;
     static constexpr auto tune = gkomi_batch_bicgstab_tune_f64;
+    static constexpr const char* name = "batch::solver::Bicgstab";
 };
 template <>
-struct bk<float> {
+struct bk<float, batch_method::bicgstab> {
     static constexpr auto solve = gkomi_batch_bicgstab_solve_f32;
     static constexpr auto tune = gkomi_batch_bicgstab_tune_f32;
+    static constexpr const char* name = "batch::solver::Bicgstab";
 };
+template <>
+struct bk<double, batch_method::cg> {
+    static constexpr auto solve = gkomi_batch_cg_solve_f64;
+    static constexpr auto tune = gkomi_batch_cg_tune_f64;
+    static constexpr const char* name = "batch::solver::Cg";
+};
+template <>
+struct bk<float, batch_method::cg> {
+    static constexpr auto solve = gkomi_batch_cg_solve_f32;
+    static constexpr auto tune = gkomi_batch_cg_tune_f32;
+    static constexpr const char* name = "batch::solver::Cg";
+};
+
+std::string what_of(const char* who, const char* what)
+{
+    return std::string(who) + ": " + what;
+}
 
 // Size bins (rows in (lo, hi]) of the launch plan, largest first.
 constexpr int64 kBinEdges[] = {int64{1} << 31, 512, 256, 128, 64, 0};
 
 }  // namespace
 
-template <typename V>
-std::unique_ptr<Bicgstab<V>> Bicgstab<V>::Factory::generate(
+template <typename V, batch_method M>
+std::unique_ptr<BatchKrylov<V, M>> BatchKrylov<V, M>::Factory::generate(
     std::shared_ptr<const BatchLinOp> a) const
 {
-    return std::unique_ptr<Bicgstab>(new Bicgstab(exec_, params_, std::move(a)));
+    return std::unique_ptr<BatchKrylov>(new BatchKrylov(exec_, params_, std::move(a)));
 }
 
-template <typename V>
-Bicgstab<V>::Bicgstab(std::shared_ptr<const Executor> exec, parameters_type params,
-                      std::shared_ptr<const BatchLinOp> a)
+template <typename V, batch_method M>
+BatchKrylov<V, M>::BatchKrylov(std::shared_ptr<const Executor> exec, parameters_type params,
+                               std::shared_ptr<const BatchLinOp> a)
     : exec_{std::move(exec)},
       params_{std::move(params)},
       system_{std::move(a)},
@@ -206,13 +226,14 @@ Bicgstab<V>::Bicgstab(std::shared_ptr<const Executor> exec, parameters_type para
       iterations_(exec_),
       residuals_(exec_)
 {
-    if (!system_) GKO_NOT_SUPPORTED_MSG("batch::solver::Bicgstab: no system matrix");
+    const char* who = bk<V, M>::name;
+    if (!system_) GKO_NOT_SUPPORTED_MSG(what_of(who, "no system matrix"));
     if (params_.max_iterations < 0 || params_.tolerance < 0) {
         throw InvalidStateError(__FILE__, __LINE__,
-                                "batch::solver::Bicgstab: negative tolerance or iterations");
+                                what_of(who, "negative tolerance or iterations"));
     }
     if (system_->get_common_size().rows != system_->get_common_size().cols) {
-        GKO_DIMENSION_MISMATCH_MSG("batch::solver::Bicgstab: items must be square");
+        GKO_DIMENSION_MISMATCH_MSG(what_of(who, "items must be square"));
     }
     std::vector<int64> rows, nnz;
     const size_type num = system_->get_num_batch_items();
@@ -227,8 +248,7 @@ Bicgstab<V>::Bicgstab(std::shared_ptr<const Executor> exec, parameters_type para
         nnz = b->get_item_nnz();
         total_rows_ = static_cast<size_type>(b->get_row_offsets().back());
     } else {
-        GKO_NOT_SUPPORTED_MSG(
-            "batch::solver::Bicgstab: system must be batch::matrix::Csr or BlockDiagCsr");
+        GKO_NOT_SUPPORTED_MSG(what_of(who, "system must be batch::matrix::Csr or BlockDiagCsr"));
     }
     // plan: items sorted by size (largest first), split into bins
     std::vector<int32> order(num);
@@ -273,8 +293,8 @@ Bicgstab<V>::Bicgstab(std::shared_ptr<const Executor> exec, parameters_type para
     residuals_ = array<V>(exec_, num);
 }
 
-template <typename V>
-gkomi_batch_settings Bicgstab<V>::settings() const
+template <typename V, batch_method M>
+gkomi_batch_settings BatchKrylov<V, M>::settings() const
 {
     gkomi_batch_settings s{};
     s.max_iterations = params_.max_iterations;
@@ -284,25 +304,27 @@ gkomi_batch_settings Bicgstab<V>::settings() const
     return s;
 }
 
-template <typename V>
-void Bicgstab<V>::check_vectors(const MultiVector<V>* b, const MultiVector<V>* x) const
+template <typename V, batch_method M>
+void BatchKrylov<V, M>::check_vectors(const MultiVector<V>* b, const MultiVector<V>* x) const
 {
-    if (!b || !x) GKO_NOT_SUPPORTED_MSG("batch::solver::Bicgstab: null vector");
+    const char* who = bk<V, M>::name;
+    if (!b || !x) GKO_NOT_SUPPORTED_MSG(what_of(who, "null vector"));
     if (b->get_num_batch_items() != system_->get_num_batch_items() ||
         x->get_num_batch_items() != system_->get_num_batch_items() ||
         b->get_total_rows() != total_rows_ || x->get_total_rows() != total_rows_) {
-        GKO_DIMENSION_MISMATCH_MSG("batch::solver::Bicgstab: vectors do not match the system");
+        GKO_DIMENSION_MISMATCH_MSG(what_of(who, "vectors do not match the system"));
     }
 }
 
-template <typename V>
-void Bicgstab<V>::apply(const MultiVector<V>* b, MultiVector<V>* x) const
+template <typename V, batch_method M>
+void BatchKrylov<V, M>::apply(const MultiVector<V>* b, MultiVector<V>* x) const
 {
     check_vectors(b, x);
     const auto s = settings();
-    GKO_CALL(bk<V>::solve(ctx_of(exec_), &desc_, &s, &plan_, b->get_const_values(),
-                          x->get_values(), iterations_.get_data(), residuals_.get_data(),
-                          workspace_.get_data(), workspace_.get_size()));
+    using K = bk<V, M>;
+    GKO_CALL(K::solve(ctx_of(exec_), &desc_, &s, &plan_, b->get_const_values(), x->get_values(),
+                      iterations_.get_data(), residuals_.get_data(), workspace_.get_data(),
+                      workspace_.get_size()));
     if (!loggers_.empty()) {
         auto it = iterations_.to_host();
         auto res = residuals_.to_host();
@@ -310,31 +332,34 @@ void Bicgstab<V>::apply(const MultiVector<V>* b, MultiVector<V>* x) const
     }
 }
 
-template <typename V>
-void Bicgstab<V>::tune(const MultiVector<V>* b, const MultiVector<V>* x)
+template <typename V, batch_method M>
+void BatchKrylov<V, M>::tune(const MultiVector<V>* b, const MultiVector<V>* x)
 {
     check_vectors(b, x);
     const auto s = settings();
     array<V> scratch(exec_, total_rows_);
-    GKO_CALL(bk<V>::tune(ctx_of(exec_), &desc_, &s, &plan_, b->get_const_values(),
-                         x->get_const_values(), scratch.get_data(), iterations_.get_data(),
-                         residuals_.get_data(), workspace_.get_data(), workspace_.get_size()));
+    using K = bk<V, M>;
+    GKO_CALL(K::tune(ctx_of(exec_), &desc_, &s, &plan_, b->get_const_values(),
+                     x->get_const_values(), scratch.get_data(), iterations_.get_data(),
+                     residuals_.get_data(), workspace_.get_data(), workspace_.get_size()));
 }
 
-template <typename V>
-std::vector<int> Bicgstab<V>::get_block_sizes() const
+template <typename V, batch_method M>
+std::vector<int> BatchKrylov<V, M>::get_block_sizes() const
 {
     return std::vector<int>(plan_.block_size, plan_.block_size + plan_.num_bins);
 }
 
-template <typename V>
-std::vector<int> Bicgstab<V>::get_matrix_in_lds() const
+template <typename V, batch_method M>
+std::vector<int> BatchKrylov<V, M>::get_matrix_in_lds() const
 {
     return std::vector<int>(plan_.matrix_in_lds, plan_.matrix_in_lds + plan_.num_bins);
 }
 
-template class Bicgstab<double>;
-template class Bicgstab<float>;
+template class BatchKrylov<double, batch_method::bicgstab>;
+template class BatchKrylov<float, batch_method::bicgstab>;
+template class BatchKrylov<double, batch_method::cg>;
+template class BatchKrylov<float, batch_method::cg>;
 
 }  // namespace solver
 

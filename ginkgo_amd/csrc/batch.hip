@@ -1,14 +1,16 @@
-// Batched BiCGSTAB for gfx950: thousands of small independent systems.
+// Batched BiCGSTAB and CG for gfx950: thousands of small independent systems.
 //
 // One workgroup solves one system at a time and pulls the next system from
 // a work queue (an atomic ticket over a size-sorted order, largest first),
 // so uneven sizes balance themselves.  The system's matrix (values, 16-bit
-// local column indices, row pointers) and all nine work vectors live in LDS
+// local column indices, row pointers) and all work vectors live in LDS
 // when they fit; otherwise the matrix is read from global memory (L2
 // resident for the working set of the active workgroups).  The whole
-// iteration runs inside the kernel (reference/solver/batch_bicgstab_kernels.hpp
-// :180-354): no host round trips, no breakdown guards, the same half-step
-// convergence exit and the same logging of iterations and residual norm.
+// iteration runs inside the kernel, with the reference's per-system
+// semantics and logging of iterations and residual norm: BiCGSTAB
+// (reference/solver/batch_bicgstab_kernels.hpp:180-354: no breakdown guards,
+// the half-step convergence exit) or CG (reference/solver/
+// batch_cg_kernels.hpp:31-221: the implicit residual norm sqrt|r.z|).
 //
 // Two storage formats:
 //  * uneven: a block-diagonal CSR, system k owning global rows
@@ -66,14 +68,24 @@ struct sys_view {
 
 }  // namespace
 
-// LDS layout per system slot (n = lds_rows, z = lds_nnz):
-//   T vec[10][n]   r, r_hat, p, p_hat, v, s, s_hat, t, x, prec
+// The solver run per system.
+enum batch_method { kBatchBicgstab = 0, kBatchCg = 1 };
+
+// Work vectors per system: BiCGSTAB r, r_hat, p, p_hat, v, s, s_hat, t, x,
+// prec; CG r, z, p, Ap, x, prec.
+__host__ __device__ constexpr int batch_vectors(int method)
+{
+    return method == kBatchCg ? 6 : 10;
+}
+
+// LDS layout per system slot (n = lds_rows, z = lds_nnz, V = batch_vectors):
+//   T vec[V][n]    the work vectors, x and prec last
 //   T vals[z]      (MatLds)
 //   int32 rp[n+1]  (MatLds)
 //   uint16 ci[z]   (MatLds)
-template <typename T, int NThreads, bool MatLds, bool Uniform>
+template <typename T, int NThreads, bool MatLds, bool Uniform, int Method>
 __global__ void __launch_bounds__(NThreads)
-    batch_bicgstab_kernel(int64_t num_systems, const int32_t* __restrict__ order,
+    batch_solver_kernel(int64_t num_systems, const int32_t* __restrict__ order,
                           const int64_t* __restrict__ row_off, const int32_t* __restrict__ rp,
                           const int32_t* __restrict__ ci, const T* __restrict__ vals,
                           int64_t uni_n, int64_t uni_nnz, const T* __restrict__ b,
@@ -88,17 +100,11 @@ __global__ void __launch_bounds__(NThreads)
     const int tid = threadIdx.x;
     T* vec = reinterpret_cast<T*>(smem);
     const int64_t L = lds_rows;
+    constexpr int kVec = batch_vectors(Method);
     T* r = vec;
-    T* rh = vec + L;
-    T* p = vec + 2 * L;
-    T* ph = vec + 3 * L;
-    T* v = vec + 4 * L;
-    T* s = vec + 5 * L;
-    T* sh = vec + 6 * L;
-    T* t = vec + 7 * L;
-    T* xl = vec + 8 * L;
-    T* prec = vec + 9 * L;
-    T* lv = vec + 10 * L;
+    T* xl = vec + (kVec - 2) * L;
+    T* prec = vec + (kVec - 1) * L;
+    T* lv = vec + kVec * L;
     int32_t* lrp = reinterpret_cast<int32_t*>(lv + (MatLds ? lds_nnz : 0));
     uint16_t* lci = reinterpret_cast<uint16_t*>(lrp + (MatLds ? L + 1 : 0));
     const T tol = static_cast<T>(tol_d);
@@ -189,6 +195,66 @@ __global__ void __launch_bounds__(NThreads)
         block_sum2<T, NThreads>(bb, unused, red);
         const T rhs_norm = sqrt(bb);
         __syncthreads();
+        const T limit = relative ? tol * rhs_norm : tol;
+        int32_t iter = 0;
+        T res = 0;
+        if constexpr (Method == kBatchCg) {
+            // reference/solver/batch_cg_kernels.hpp:31-221 (batch_entry_cg_impl):
+            // r = b - A x, p = z = Ap = 0, rho_old = 1; per iteration
+            // z = M r, rho = r.z, the implicit norm sqrt|rho| against the
+            // tolerance (iterations counted from 1), p = z + rho/rho_old p
+            // (p = z when rho_old == 0), Ap = A p, x += rho/(p.Ap) p,
+            // r -= rho/(p.Ap) Ap.
+            T* z = vec + L;
+            T* p = vec + 2 * L;
+            T* ap = vec + 3 * L;
+            for (int row = tid; row < n; row += NThreads) {
+                T sum = 0;
+                const int32_t e1 = row_end(row);
+                for (int32_t e = row_begin(row); e < e1; ++e) sum += val(e) * xl[col(e)];
+                r[row] = r[row] - sum;
+                p[row] = T(0);
+                z[row] = T(0);
+                ap[row] = T(0);
+            }
+            T rho_old = 1;
+            while (true) {
+                T rz = 0;
+                for (int row = tid; row < n; row += NThreads) {
+                    const T zi = prec[row] * r[row];
+                    z[row] = zi;
+                    rz += r[row] * zi;
+                }
+                const T rho = block_sum1<T, NThreads>(rz, red);
+                ++iter;
+                res = sqrt(rho < T(0) ? -rho : rho);
+                if (iter >= max_iters || res <= limit) break;
+                const bool restart = rho_old == T(0);
+                const T beta = restart ? T(0) : rho / rho_old;
+                for (int row = tid; row < n; row += NThreads) {
+                    p[row] = restart ? z[row] : z[row] + beta * p[row];
+                }
+                __syncthreads();
+                spmv(p, ap);
+                __syncthreads();
+                T pap = 0;
+                for (int row = tid; row < n; row += NThreads) pap += p[row] * ap[row];
+                pap = block_sum1<T, NThreads>(pap, red);
+                const T temp = rho / pap;
+                for (int row = tid; row < n; row += NThreads) {
+                    xl[row] += temp * p[row];
+                    r[row] -= temp * ap[row];
+                }
+                rho_old = rho;
+            }
+        } else {
+        T* rh = vec + L;
+        T* p = vec + 2 * L;
+        T* ph = vec + 3 * L;
+        T* v = vec + 4 * L;
+        T* s = vec + 5 * L;
+        T* sh = vec + 6 * L;
+        T* t = vec + 7 * L;
         // r = b - A x, ||r||, r_hat = r, p = p_hat = v = 0
         T rr = 0;
         for (int row = tid; row < n; row += NThreads) {
@@ -206,10 +272,8 @@ __global__ void __launch_bounds__(NThreads)
         T rho_new = 0;  // r_hat . r, fused with ||r||
         for (int row = tid; row < n; row += NThreads) rho_new += r[row] * r[row];
         block_sum2<T, NThreads>(rr, rho_new, red);
-        T res = sqrt(rr);
-        const T limit = relative ? tol * rhs_norm : tol;
+        res = sqrt(rr);
         T rho_old = 1, omega = 1, alpha = 1;
-        int32_t iter = 0;
         for (; iter < max_iters; ++iter) {
             if (res <= limit) break;
             // p = r + beta (p - omega v), p_hat = M p
@@ -261,6 +325,7 @@ __global__ void __launch_bounds__(NThreads)
             rho_old = rho_new;
             rho_new = rhr;
         }
+        }
         __syncthreads();
         for (int row = tid; row < n; row += NThreads) gx[row] = xl[row];
         if (tid == 0) {
@@ -283,9 +348,10 @@ __global__ void __launch_bounds__(NThreads)
 constexpr int kBatchMaxBins = GKOMI_BATCH_MAX_BINS;
 constexpr size_t kBatchQueueBytes = 64;  // per bin: queue + done counters
 
-size_t batch_lds_bytes(int vbytes, int64_t rows, int64_t nnz, bool mat_lds)
+size_t batch_lds_bytes(int vbytes, int64_t rows, int64_t nnz, bool mat_lds,
+                       int method = kBatchBicgstab)
 {
-    size_t b = static_cast<size_t>(vbytes) * 10 * rows;
+    size_t b = static_cast<size_t>(vbytes) * batch_vectors(method) * rows;
     if (mat_lds) {
         b += static_cast<size_t>(vbytes) * nnz;
         b += 4 * (rows + 1);
@@ -294,12 +360,12 @@ size_t batch_lds_bytes(int vbytes, int64_t rows, int64_t nnz, bool mat_lds)
     return (b + 15) / 16 * 16;
 }
 
-template <typename T, int NT, bool MatLds, bool Uniform>
+template <typename T, int NT, bool MatLds, bool Uniform, int Method>
 int launch_batch(hipStream_t stream, int grid, size_t lds, int64_t count, const int32_t* order,
                  const gkomi_batch_csr* a, const T* b, T* x, const gkomi_batch_settings* st,
                  int32_t* iters, T* res, unsigned* queue, int64_t lds_rows, int64_t lds_nnz)
 {
-    auto kern = batch_bicgstab_kernel<T, NT, MatLds, Uniform>;
+    auto kern = batch_solver_kernel<T, NT, MatLds, Uniform, Method>;
     if (lds > 64 * 1024) {
         GKOMI_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                             hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -310,11 +376,11 @@ int launch_batch(hipStream_t stream, int grid, size_t lds, int64_t count, const 
                                     a->uniform_nnz, b, x, st->max_iterations, st->tolerance_type,
                                     st->tolerance, st->preconditioner, iters, res, queue,
                                     lds_rows, lds_nnz);
-    GKOMI_CHECK_LAUNCH("batch_bicgstab_kernel");
+    GKOMI_CHECK_LAUNCH("batch_solver_kernel");
     return GKOMI_SUCCESS;
 }
 
-template <typename T, bool MatLds, bool Uniform>
+template <typename T, bool MatLds, bool Uniform, int Method>
 int launch_batch_nt(int nt, hipStream_t stream, int grid, size_t lds, int64_t count,
                     const int32_t* order, const gkomi_batch_csr* a, const T* b, T* x,
                     const gkomi_batch_settings* st, int32_t* iters, T* res, unsigned* queue,
@@ -322,16 +388,16 @@ int launch_batch_nt(int nt, hipStream_t stream, int grid, size_t lds, int64_t co
 {
     switch (nt) {
     case 64:
-        return launch_batch<T, 64, MatLds, Uniform>(stream, grid, lds, count, order, a, b, x, st,
+        return launch_batch<T, 64, MatLds, Uniform, Method>(stream, grid, lds, count, order, a, b, x, st,
                                                     iters, res, queue, lds_rows, lds_nnz);
     case 128:
-        return launch_batch<T, 128, MatLds, Uniform>(stream, grid, lds, count, order, a, b, x,
+        return launch_batch<T, 128, MatLds, Uniform, Method>(stream, grid, lds, count, order, a, b, x,
                                                      st, iters, res, queue, lds_rows, lds_nnz);
     case 256:
-        return launch_batch<T, 256, MatLds, Uniform>(stream, grid, lds, count, order, a, b, x,
+        return launch_batch<T, 256, MatLds, Uniform, Method>(stream, grid, lds, count, order, a, b, x,
                                                      st, iters, res, queue, lds_rows, lds_nnz);
     case 512:
-        return launch_batch<T, 512, MatLds, Uniform>(stream, grid, lds, count, order, a, b, x,
+        return launch_batch<T, 512, MatLds, Uniform, Method>(stream, grid, lds, count, order, a, b, x,
                                                      st, iters, res, queue, lds_rows, lds_nnz);
     default:
         return fail(GKOMI_INVALID_ARGUMENT, "batch: block size must be 64, 128, 256 or 512");
@@ -370,7 +436,7 @@ int check_batch(const gkomi_batch_csr* a, const gkomi_batch_settings* st)
     return GKOMI_SUCCESS;
 }
 
-template <typename T>
+template <typename T, int Method>
 int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
                      const gkomi_batch_settings* st, const gkomi_batch_plan* plan, const T* b,
                      T* x, int32_t* iters, T* res, void* ws, size_t wsb)
@@ -411,9 +477,10 @@ int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
                       "batch: bin maxima exceed the batch maxima");
         int mat = plan->matrix_in_lds[bin];
         if (mat < 0) {
-            mat = batch_lds_bytes(sizeof(T), rows, nnz, true) <= static_cast<size_t>(lds_limit);
+            mat = batch_lds_bytes(sizeof(T), rows, nnz, true, Method) <=
+                  static_cast<size_t>(lds_limit);
         }
-        const size_t lds = batch_lds_bytes(sizeof(T), rows, nnz, mat != 0);
+        const size_t lds = batch_lds_bytes(sizeof(T), rows, nnz, mat != 0, Method);
         GKOMI_REQUIRE(lds <= static_cast<size_t>(lds_limit), GKOMI_NOT_SUPPORTED,
                       "batch: the work vectors of the largest system exceed LDS");
         int bpc = plan->blocks_per_cu[bin];
@@ -429,14 +496,14 @@ int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
         const int nt = plan->block_size[bin];
         int rc;
         if (uniform) {
-            rc = mat ? launch_batch_nt<T, true, true>(nt, s, grid, lds, cnt, order, a, b, x, st,
+            rc = mat ? launch_batch_nt<T, true, true, Method>(nt, s, grid, lds, cnt, order, a, b, x, st,
                                                      iters, res, q, rows, nnz)
-                     : launch_batch_nt<T, false, true>(nt, s, grid, lds, cnt, order, a, b, x, st,
+                     : launch_batch_nt<T, false, true, Method>(nt, s, grid, lds, cnt, order, a, b, x, st,
                                                       iters, res, q, rows, nnz);
         } else {
-            rc = mat ? launch_batch_nt<T, true, false>(nt, s, grid, lds, cnt, order, a, b, x, st,
+            rc = mat ? launch_batch_nt<T, true, false, Method>(nt, s, grid, lds, cnt, order, a, b, x, st,
                                                       iters, res, q, rows, nnz)
-                     : launch_batch_nt<T, false, false>(nt, s, grid, lds, cnt, order, a, b, x,
+                     : launch_batch_nt<T, false, false, Method>(nt, s, grid, lds, cnt, order, a, b, x,
                                                        st, iters, res, q, rows, nnz);
         }
         if (rc) return rc;
@@ -445,7 +512,7 @@ int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
 }
 
 
-template <typename T>
+template <typename T, int Method>
 int batch_tune_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
                     const gkomi_batch_settings* st, gkomi_batch_plan* plan, const T* b,
                     const T* x, T* xs, int32_t* iters, T* res, void* ws, size_t wsb)
@@ -483,7 +550,7 @@ int batch_tune_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
         float best = 1e30f;
         int best_nt = 256, best_mat = 0;
         const bool fits = batch_lds_bytes(sizeof(T), one.bin_max_rows[0], one.bin_max_nnz[0],
-                                          true) <= static_cast<size_t>(lds_limit);
+                                          true, Method) <= static_cast<size_t>(lds_limit);
         for (int nt : {64, 128, 256, 512}) {
             for (int mat = fits ? 1 : 0; mat >= 0; --mat) {
                 one.block_size[0] = nt;
@@ -493,7 +560,8 @@ int batch_tune_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
                 GKOMI_CHECK_HIP(hipMemcpyAsync(xs, x, sizeof(T) * rows_all,
                                                hipMemcpyDeviceToDevice, s));
                 GKOMI_CHECK_HIP(hipEventRecord(e0, s));
-                rc = batch_solve_impl<T>(ctx, &sub, st, &one, b, xs, iters, res, ws, wsb);
+                rc = batch_solve_impl<T, Method>(ctx, &sub, st, &one, b, xs, iters, res, ws,
+                                                 wsb);
                 if (rc) break;
                 GKOMI_CHECK_HIP(hipEventRecord(e1, s));
                 GKOMI_CHECK_HIP(hipEventSynchronize(e1));
@@ -533,12 +601,19 @@ size_t gkomi_batch_lds_bytes(int32_t value_bytes, int64_t rows, int64_t nnz,
     return batch_lds_bytes(value_bytes, rows, nnz, matrix_in_lds != 0);
 }
 
+size_t gkomi_batch_cg_lds_bytes(int32_t value_bytes, int64_t rows, int64_t nnz,
+                                int32_t matrix_in_lds)
+{
+    return batch_lds_bytes(value_bytes, rows, nnz, matrix_in_lds != 0, kBatchCg);
+}
+
 int gkomi_batch_bicgstab_solve_f64(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
                                    const gkomi_batch_settings* st, const gkomi_batch_plan* plan,
                                    const double* b, double* x, int32_t* iters, double* res,
                                    void* ws, size_t wsb)
 {
-    return batch_solve_impl(ctx, a, st, plan, b, x, iters, res, ws, wsb);
+    return batch_solve_impl<double, kBatchBicgstab>(ctx, a, st, plan, b, x, iters, res,
+                                                            ws, wsb);
 }
 
 int gkomi_batch_bicgstab_solve_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
@@ -546,7 +621,8 @@ int gkomi_batch_bicgstab_solve_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* 
                                    const float* b, float* x, int32_t* iters, float* res,
                                    void* ws, size_t wsb)
 {
-    return batch_solve_impl(ctx, a, st, plan, b, x, iters, res, ws, wsb);
+    return batch_solve_impl<float, kBatchBicgstab>(ctx, a, st, plan, b, x, iters, res,
+                                                            ws, wsb);
 }
 
 int gkomi_batch_bicgstab_tune_f64(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
@@ -554,7 +630,8 @@ int gkomi_batch_bicgstab_tune_f64(const gkomi_ctx* ctx, const gkomi_batch_csr* a
                                   const double* b, const double* x, double* xs, int32_t* iters,
                                   double* res, void* ws, size_t wsb)
 {
-    return batch_tune_impl(ctx, a, st, plan, b, x, xs, iters, res, ws, wsb);
+    return batch_tune_impl<double, kBatchBicgstab>(ctx, a, st, plan, b, x, xs, iters,
+                                                           res, ws, wsb);
 }
 
 int gkomi_batch_bicgstab_tune_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
@@ -562,7 +639,28 @@ int gkomi_batch_bicgstab_tune_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* a
                                   const float* b, const float* x, float* xs, int32_t* iters,
                                   float* res, void* ws, size_t wsb)
 {
-    return batch_tune_impl(ctx, a, st, plan, b, x, xs, iters, res, ws, wsb);
+    return batch_tune_impl<float, kBatchBicgstab>(ctx, a, st, plan, b, x, xs, iters,
+                                                           res, ws, wsb);
 }
+
+#define GKOMI_BATCH_CG_ENTRY(T, S)                                                            \
+    int gkomi_batch_cg_solve_##S(const gkomi_ctx* ctx, const gkomi_batch_csr* a,              \
+                                 const gkomi_batch_settings* st, const gkomi_batch_plan* plan, \
+                                 const T* b, T* x, int32_t* iters, T* res, void* ws,           \
+                                 size_t wsb)                                                   \
+    {                                                                                         \
+        return batch_solve_impl<T, kBatchCg>(ctx, a, st, plan, b, x, iters, res, ws, wsb);    \
+    }                                                                                         \
+    int gkomi_batch_cg_tune_##S(const gkomi_ctx* ctx, const gkomi_batch_csr* a,               \
+                                const gkomi_batch_settings* st, gkomi_batch_plan* plan,       \
+                                const T* b, const T* x, T* xs, int32_t* iters, T* res,        \
+                                void* ws, size_t wsb)                                         \
+    {                                                                                         \
+        return batch_tune_impl<T, kBatchCg>(ctx, a, st, plan, b, x, xs, iters, res, ws, wsb); \
+    }
+
+GKOMI_BATCH_CG_ENTRY(double, f64)
+GKOMI_BATCH_CG_ENTRY(float, f32)
+#undef GKOMI_BATCH_CG_ENTRY
 
 }  // extern "C"

@@ -524,26 +524,35 @@ class DeviceBatchCsr:
         s.preconditioner = 1 if jacobi else 0
         return s
 
-    def solve(self, b, x, settings, plan=None, stream=None):
-        """BiCGSTAB on every system; x holds the initial guess and gets the
-        solution.  Returns (iterations, residual norms) as device tensors."""
+    @staticmethod
+    def _method(method):
+        if method not in ("bicgstab", "cg"):
+            raise C.NotSupported("batch method must be 'bicgstab' or 'cg'")
+        return method
+
+    def solve(self, b, x, settings, plan=None, stream=None, method="bicgstab"):
+        """BiCGSTAB (or CG, method="cg") on every system; x holds the initial
+        guess and gets the solution.  Returns (iterations, residual norms) as
+        device tensors."""
+        m = self._method(method)
         iters = torch.empty(self.num_systems, dtype=torch.int32, device=self.device)
         res = torch.empty(self.num_systems, dtype=self.dtype, device=self.device)
         ws = self.workspace()
         c = ctx(self.device, stream)
-        C.call(f"gkomi_batch_bicgstab_solve_{suffix(self.dtype)}", ctypes.byref(c),
+        C.call(f"gkomi_batch_{m}_solve_{suffix(self.dtype)}", ctypes.byref(c),
                ctypes.byref(self.desc), ctypes.byref(settings),
                ctypes.byref(plan) if plan is not None else None, ptr(b), ptr(x), ptr(iters),
                ptr(res), ptr(ws), ws.numel())
         return iters, res
 
-    def capture(self, b, x, settings, plan=None, x0=None):
+    def capture(self, b, x, settings, plan=None, x0=None, method="bicgstab"):
         """The whole batched solve -- x = x0 (zero when x0 is None), then every
         bin's persistent launch -- captured in one HIP graph on a side
         stream.  Returns (graph, iterations, residual norms); graph.replay()
         re-runs it on the same buffers.  The work-queue counters are reset by
         the kernels themselves, and x is set by a fill/copy kernel, so the
         graph holds no memset node."""
+        m = self._method(method)
         iters = torch.empty(self.num_systems, dtype=torch.int32, device=self.device)
         res = torch.empty(self.num_systems, dtype=self.dtype, device=self.device)
         ws = self.workspace()
@@ -557,21 +566,22 @@ class DeviceBatchCsr:
             else:
                 x.copy_(x0)
             c = ctx(self.device, side)
-            C.call(f"gkomi_batch_bicgstab_solve_{sfx}", ctypes.byref(c),
+            C.call(f"gkomi_batch_{m}_solve_{sfx}", ctypes.byref(c),
                    ctypes.byref(self.desc), ctypes.byref(settings),
                    ctypes.byref(plan) if plan is not None else None, ptr(b), ptr(x),
                    ptr(iters), ptr(res), ptr(ws), ws.numel())
         torch.cuda.current_stream(self.device).wait_stream(side)
         return graph, iters, res
 
-    def tune(self, b, x, settings, plan):
+    def tune(self, b, x, settings, plan, method="bicgstab"):
         """Autotune block size and LDS residency per bin (writes into plan)."""
+        m = self._method(method)
         xs = torch.empty_like(x)
         iters = torch.empty(self.num_systems, dtype=torch.int32, device=self.device)
         res = torch.empty(self.num_systems, dtype=self.dtype, device=self.device)
         ws = self.workspace()
         c = ctx(self.device)
-        C.call(f"gkomi_batch_bicgstab_tune_{suffix(self.dtype)}", ctypes.byref(c),
+        C.call(f"gkomi_batch_{m}_tune_{suffix(self.dtype)}", ctypes.byref(c),
                ctypes.byref(self.desc), ctypes.byref(settings), ctypes.byref(plan), ptr(b),
                ptr(x), ptr(xs), ptr(iters), ptr(res), ptr(ws), ws.numel())
         return plan

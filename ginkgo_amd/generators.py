@@ -148,6 +148,8 @@ def uneven_batch(num_systems=20000, n_min=16, n_max=512, k=8, band=32, seed=5,
     off-diagonal columns drawn from [i-band, i+band] minus {i}, clipped to the
     system; values U[-1,1]; diagonal = row abs sum + 1.  Returns
     (row_off[int64, S+1], row_ptrs[int32], col_idxs[int32, local], vals, b)."""
+    if n_min <= k:
+        raise ValueError("uneven_batch: every system needs more than k rows")
     rng = np.random.default_rng(seed)
     sizes = rng.integers(n_min, n_max + 1, size=num_systems).astype(np.int64)
     row_off = np.zeros(num_systems + 1, dtype=np.int64)
@@ -182,6 +184,42 @@ def uneven_batch(num_systems=20000, n_min=16, n_max=512, k=8, band=32, seed=5,
     np.cumsum(counts, out=row_ptrs[1:])
     b = uniform(rng, total, dtype)
     return row_off, row_ptrs.astype(np.int32), cols, vals, b
+
+
+def uneven_spd_batch(num_systems=20000, n_min=16, n_max=512, k=8, band=32, seed=5,
+                     dtype=np.float64):
+    """The systems of uneven_batch made symmetric positive definite (for batch
+    CG): the off-diagonal part O becomes O + O^T (entries where both exist are
+    added), and the diagonal is again the row abs sum + 1, so every system is
+    symmetric and strictly diagonally dominant with a positive diagonal.
+    Returns (row_off, row_ptrs, col_idxs, vals, b) as uneven_batch."""
+    row_off, rp, ci, v, b = uneven_batch(num_systems, n_min, n_max, k, band, seed, np.float64)
+    total = int(row_off[-1])
+    sizes = np.diff(row_off)
+    rows = np.repeat(np.arange(total, dtype=np.int64), np.diff(rp))
+    sys_of_row = np.repeat(np.arange(num_systems), sizes)
+    local = rows - row_off[sys_of_row[rows]]
+    off = ci != local
+    r_o, c_o, v_o = rows[off], ci[off].astype(np.int64), v[off]
+    g_c = c_o + row_off[sys_of_row[r_o]]          # global column of each entry
+    lr = r_o - row_off[sys_of_row[r_o]]           # local row
+    rows2 = np.concatenate([r_o, g_c])
+    cols2 = np.concatenate([c_o, lr])
+    vals2 = np.concatenate([v_o, v_o])
+    key = rows2 * (n_max + 1) + cols2
+    uniq, inv = np.unique(key, return_inverse=True)
+    sv = np.zeros(uniq.shape[0])
+    np.add.at(sv, inv, vals2)
+    ur, uc = uniq // (n_max + 1), uniq % (n_max + 1)
+    diag = np.bincount(ur, weights=np.abs(sv), minlength=total) + 1.0
+    all_r = np.concatenate([ur, np.arange(total, dtype=np.int64)])
+    all_c = np.concatenate([uc, np.arange(total, dtype=np.int64) - row_off[sys_of_row]])
+    all_v = np.concatenate([sv, diag])
+    order = np.argsort(all_r * (n_max + 1) + all_c, kind="stable")
+    all_r, all_c, all_v = all_r[order], all_c[order].astype(np.int32), all_v[order]
+    row_ptrs = np.zeros(total + 1, dtype=np.int64)
+    np.cumsum(np.bincount(all_r, minlength=total), out=row_ptrs[1:])
+    return row_off, row_ptrs.astype(np.int32), all_c, all_v.astype(dtype), b.astype(dtype)
 
 
 def stencil3pt_batch(num_items, n, diag=6.0, off=-1.0, dtype=np.float64):

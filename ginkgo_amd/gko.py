@@ -66,7 +66,7 @@ solver = types.SimpleNamespace(
 log = types.SimpleNamespace(Convergence=_g.Convergence)
 
 # batch module: batch.matrix.Csr (shared pattern), batch.matrix.BlockDiagCsr
-# (items of different sizes), batch.MultiVector, batch.solver.Bicgstab,
+# (items of different sizes), batch.MultiVector, batch.solver.Bicgstab and Cg,
 # batch.preconditioner.Jacobi, batch.log.BatchConvergence, batch.stop.tolerance_type
 batch = types.SimpleNamespace(
     MultiVector=_typed("BatchMultiVector"),
@@ -75,7 +75,8 @@ batch = types.SimpleNamespace(
     preconditioner=types.SimpleNamespace(
         Jacobi=types.SimpleNamespace(build=lambda dtype="f64": _typed("BatchJacobiParams")(dtype)())),
     solver=types.SimpleNamespace(
-        Bicgstab=types.SimpleNamespace(build=lambda dtype="f64": _typed("BatchBicgstabParams")(dtype)())),
+        Bicgstab=types.SimpleNamespace(build=lambda dtype="f64": _typed("BatchBicgstabParams")(dtype)()),
+        Cg=types.SimpleNamespace(build=lambda dtype="f64": _typed("BatchCgParams")(dtype)())),
     log=types.SimpleNamespace(BatchConvergence=_typed("BatchConvergence")),
 )
 

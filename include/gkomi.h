@@ -684,6 +684,38 @@ int gkomi_batch_bicgstab_tune_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* a
                                   int32_t* iterations, float* residual_norms, void* workspace,
                                   size_t workspace_bytes);
 
+/* -------------------------------------------------------------- batched CG */
+
+/* batch_cg::apply (core/solver/batch_cg_kernels.hpp:165-181;
+ * reference/solver/batch_cg_kernels.hpp:31-221, batch_entry_cg_impl): an
+ * independent preconditioned CG per system over the same batch formats,
+ * settings, plans and workspace as the batched BiCGSTAB.  Per the reference:
+ * the stopping test reads the implicit norm sqrt|r.z|, iterations are
+ * counted from 1, p = z when rho_old == 0.  Six work vectors per system
+ * (r, z, p, Ap, x, prec) stay in LDS, with the matrix when it fits
+ * (gkomi_batch_cg_lds_bytes). */
+size_t gkomi_batch_cg_lds_bytes(int32_t value_bytes, int64_t rows, int64_t nnz,
+                                int32_t matrix_in_lds);
+int gkomi_batch_cg_solve_f64(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                             const gkomi_batch_settings* settings, const gkomi_batch_plan* plan,
+                             const double* b, double* x, int32_t* iterations,
+                             double* residual_norms, void* workspace, size_t workspace_bytes);
+int gkomi_batch_cg_solve_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                             const gkomi_batch_settings* settings, const gkomi_batch_plan* plan,
+                             const float* b, float* x, int32_t* iterations,
+                             float* residual_norms, void* workspace, size_t workspace_bytes);
+/* Autotuner of the batched CG (as gkomi_batch_bicgstab_tune). */
+int gkomi_batch_cg_tune_f64(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                            const gkomi_batch_settings* settings, gkomi_batch_plan* plan,
+                            const double* b, const double* x, double* x_scratch,
+                            int32_t* iterations, double* residual_norms, void* workspace,
+                            size_t workspace_bytes);
+int gkomi_batch_cg_tune_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                            const gkomi_batch_settings* settings, gkomi_batch_plan* plan,
+                            const float* b, const float* x, float* x_scratch,
+                            int32_t* iterations, float* residual_norms, void* workspace,
+                            size_t workspace_bytes);
+
 /* ------------------------------------------------------ adjoint support */
 
 /* Sampled outer product on the pattern of A: out[k] = alpha * sum_j

@@ -150,7 +150,12 @@ typedef struct {
                                      const T* b, T* x, int use_jacobi,             \
                                      int relative, double tol,                     \
                                      int32_t max_iters, int use_omp,               \
-                                     int32_t* iters, T* res_norms);
+                                     int32_t* iters, T* res_norms);                \
+    int ora_batch_cg_solve_##S(int64_t num_systems, const int64_t* row_off,        \
+                               const int32_t* row_ptrs, const int32_t* col_idxs,   \
+                               const T* vals, const T* b, T* x, int use_jacobi,    \
+                               int relative, double tol, int32_t max_iters,        \
+                               int use_omp, int32_t* iters, T* res_norms);
 
 ORA_DECL_SPMV(double, d)
 ORA_DECL_SPMV(float, s)

@@ -337,6 +337,26 @@ def batch_bicgstab_solve(row_off, row_ptrs, col_idxs, vals, b, x0, use_jacobi=Tr
     return x, iters, res
 
 
+def batch_cg_solve(row_off, row_ptrs, col_idxs, vals, b, x0, use_jacobi=True, relative=True,
+                   tol=1e-10, max_iters=100, omp=False):
+    """Batch CG (reference/solver/batch_cg_kernels.hpp:31-221) over the
+    block-diagonal batch format; returns (x, iterations, implicit residual norms)."""
+    dt = vals.dtype
+    row_off = _c(row_off, np.int64)
+    ns = len(row_off) - 1
+    x = _c(x0, dt).copy()
+    iters = np.zeros(ns, dtype=np.int32)
+    res = np.zeros(ns, dtype=dt)
+    rc = getattr(lib(), f"ora_batch_cg_solve_{_suffix(dt)}")(
+        ctypes.c_int64(ns), _p(row_off), _p(_c(row_ptrs, np.int32)), _p(_c(col_idxs, np.int32)),
+        _p(_c(vals, dt)), _p(_c(b, dt)), _p(x), ctypes.c_int(1 if use_jacobi else 0),
+        ctypes.c_int(1 if relative else 0), ctypes.c_double(tol), ctypes.c_int32(max_iters),
+        ctypes.c_int(1 if omp else 0), _p(iters), _p(res))
+    if rc != 0:
+        raise MemoryError("oracle batch solve failed to allocate")
+    return x, iters, res
+
+
 def _raw(name):
     return getattr(lib(), name)
 

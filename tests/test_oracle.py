@@ -472,3 +472,73 @@ def test_batch_bicgstab_uneven_against_dense_lu():
         d = gen.csr_to_dense(sub_rp, ci[rp[r0]:rp[r1]], v[rp[r0]:rp[r1]])
         np.testing.assert_allclose(x[r0:r1], np.linalg.solve(d, b[r0:r1]), atol=1e-8)
         assert np.linalg.norm(b[r0:r1] - d @ x[r0:r1]) <= 1e-10 * np.linalg.norm(b[r0:r1]) * 1.01
+
+
+def _stencil_batch_system(k):
+    row_off, rp, ci, v = gen.stencil3pt_batch(k["num_batch_items"], k["num_rows"],
+                                               k["diag"], k["off"])
+    n = k["num_rows"]
+    d = gen.csr_to_dense(rp[: n + 1], ci[:rp[n]], v[:rp[n]])
+    xs = np.full(n, k["exact_sol_fill"])
+    b = np.tile(d @ xs, k["num_batch_items"])
+    return row_off, rp, ci, v, d, xs, b
+
+
+def test_batch_cg_reference_stencil_kats():
+    """reference/test/solver/batch_cg_kernels.cpp:81-139 restated."""
+    k = KATS["batch_cg"][0]
+    row_off, rp, ci, v, d, xs, b = _stencil_batch_system(k)
+    n = k["num_rows"]
+    x, iters, res = ora.batch_cg_solve(row_off, rp, ci, v, b, np.zeros_like(b), use_jacobi=False,
+                                       relative=True, tol=k["tolerance"],
+                                       max_iters=k["max_iterations"])
+    for s in range(k["num_batch_items"]):
+        bs, xk = b[s * n:(s + 1) * n], x[s * n:(s + 1) * n]
+        true_res = np.linalg.norm(bs - d @ xk)
+        assert true_res / np.linalg.norm(bs) <= 6 * k["tolerance"]
+        assert np.max(np.abs(xk - xs)) <= 10 * k["tolerance"]
+        # StencilSystemLoggerLogsResidual: the logged norm meets the tolerance
+        # and stays close to the true residual norm
+        assert res[s] / np.linalg.norm(bs) <= k["tolerance"]
+        assert abs(res[s] - true_res) <= 10 * k["tolerance"]
+    k3 = KATS["batch_cg"][2]
+    x, iters, res = ora.batch_cg_solve(row_off, rp, ci, v, b, np.zeros_like(b), use_jacobi=False,
+                                       relative=True, tol=k3["tolerance"],
+                                       max_iters=k3["max_iterations"])
+    assert np.all(iters == k3["check_iters_eq"])
+
+
+def test_batch_cg_spd_systems_against_dense_solve():
+    """Uneven SPD systems: the solution of a dense solve, the logged norm
+    equal to sqrt(r^T M r) of the final iterate's residual (the implicit norm
+    with Jacobi), and at most n + 1 iterations -- in exact arithmetic CG
+    ends within n steps, and the count starts at 1."""
+    row_off, rp, ci, v, b = gen.uneven_spd_batch(25, n_min=16, n_max=48, seed=9)
+    x, iters, res = ora.batch_cg_solve(row_off, rp, ci, v, b, np.zeros_like(b), use_jacobi=True,
+                                       relative=True, tol=1e-12, max_iters=500)
+    for s in range(25):
+        r0, r1 = row_off[s], row_off[s + 1]
+        d = gen.csr_to_dense(rp[r0:r1 + 1] - rp[r0], ci[rp[r0]:rp[r1]], v[rp[r0]:rp[r1]])
+        exact = np.linalg.solve(d, b[r0:r1])
+        np.testing.assert_allclose(x[r0:r1], exact, atol=1e-10 * np.abs(exact).max())
+        r = b[r0:r1] - d @ x[r0:r1]
+        implicit = np.sqrt(abs(r @ (r / np.diag(d))))
+        assert res[s] <= 1e-12 * np.linalg.norm(b[r0:r1])
+        assert abs(res[s] - implicit) <= 1e-12 * np.linalg.norm(b[r0:r1])
+        assert 1 <= iters[s] <= (r1 - r0) + 1
+
+
+def test_batch_cg_restart_and_zero_rhs():
+    """A zero right-hand side stops at the first check (iteration 1, norm 0);
+    an identity matrix converges after one step (x = b)."""
+    n = 6
+    rp = np.arange(n + 1, dtype=np.int32)
+    ci = np.array([0, 1, 2, 0, 1, 2], dtype=np.int32)  # local to each system
+    v = np.ones(n)
+    row_off = np.array([0, 3, 6], dtype=np.int64)
+    b = np.array([0.0, 0, 0, 1, 2, 3])
+    x, iters, res = ora.batch_cg_solve(row_off, rp, ci, v, b, np.zeros(n), use_jacobi=False,
+                                       relative=False, tol=1e-14, max_iters=10)
+    assert iters[0] == 1 and res[0] == 0
+    assert iters[1] == 2 and res[1] == 0
+    np.testing.assert_array_equal(x, b)
