@@ -323,6 +323,47 @@ def bench_batch(dev, num=20000):
     return out, (row_off, rp, ci, v, b)
 
 
+def bench_batch_cg(dev, num=20000):
+    """20,000 uneven SPD systems (the uneven batch's pattern made symmetric,
+    n ~ U{16..512}, diagonally dominant), batch CG + scalar Jacobi: FP64 to
+    1e-10 and FP32 to 1e-5 (relative, on the implicit norm sqrt|r.z| as the
+    reference's batch CG), tuned plan; the true residuals checked on a sample."""
+    import numpy as np
+    import torch
+    from ginkgo_amd import device as D
+    from ginkgo_amd import generators as gen
+    log(f"[bench] generating {num} uneven SPD systems")
+    row_off, rp, ci, v, b = gen.uneven_spd_batch(num)
+    out = {"num_systems": num, "total_rows": int(row_off[-1]), "total_nnz": int(rp[-1])}
+    for dt, tol in ((np.float64, 1e-10), (np.float32, 1e-5)):
+        A = D.DeviceBatchCsr(row_off, rp, ci, v.astype(dt), device=dev)
+        bd = torch.as_tensor(b.astype(dt)).to(dev)
+        st = A.settings(tol=tol, relative=True, max_iters=1000, jacobi=True)
+        plan = A.tune(bd, torch.zeros_like(bd), st, A.plan(), method="cg")
+        times = []
+        for _ in range(3):
+            x = torch.zeros_like(bd)
+            t, (it, rs) = timed(lambda: A.solve(bd, x, st, plan, method="cg"))
+            times.append(t)
+        xh = x.cpu().numpy().astype(np.float64)
+        worst = 0.0
+        for s in range(0, num, max(1, num // 40)):
+            r0, r1 = row_off[s], row_off[s + 1]
+            d = gen.csr_to_dense(rp[r0:r1 + 1] - rp[r0], ci[rp[r0]:rp[r1]], v[rp[r0]:rp[r1]])
+            bs = b[r0:r1]
+            worst = max(worst, float(np.linalg.norm(bs - d @ xh[r0:r1]) / np.linalg.norm(bs)))
+        key = "f64" if dt == np.float64 else "f32"
+        out[key] = {"time_s": round(min(times), 5), "solves_per_s": round(num / min(times), 1),
+                    "max_iterations": int(it.max().item()),
+                    "mean_iterations": round(float(it.float().mean().item()), 2),
+                    "sampled_max_true_relres": worst,
+                    "plan_block_sizes": [plan.block_size[i] for i in range(plan.num_bins)],
+                    "plan_matrix_in_lds": [plan.matrix_in_lds[i] for i in range(plan.num_bins)]}
+        del A
+    torch.cuda.empty_cache()
+    return out
+
+
 def edge_rank_one(rows_of, vmap, coef, e):
     """The conductivity theta_e enters A as c * u u^T with u = e_a - e_b
     (interior edge: entries (a,a,+c), (b,b,+c), (a,b,-c), (b,a,-c)) or
@@ -452,8 +493,8 @@ def main():
     ap.add_argument("--quick", action="store_true", help="skip secondary measurements")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--only", action="append", default=None,
-                    choices=["spmv", "poisson", "bicgstab", "cg_ir", "batch", "adjoint",
-                             "adjoint_convection"],
+                    choices=["spmv", "poisson", "bicgstab", "cg_ir", "batch", "batch_cg",
+                             "adjoint", "adjoint_convection"],
                     help="run only these secondary sections (repeatable)")
     ap.add_argument("--fd-entries", type=int, default=32,
                     help="conductivities checked by central differences in the adjoint section")
@@ -571,6 +612,7 @@ def main():
             ("bicgstab", "bicgstab_jacobi_convection512sq", lambda: bench_bicgstab(dev)),
             ("cg_ir", "cg_vs_ir_96cubed", lambda: bench_cg_vs_ir(dev)[0]),
             ("batch", "batched_bicgstab_20000", lambda: bench_batch(dev)[0]),
+            ("batch_cg", "batched_cg_20000_spd", lambda: bench_batch_cg(dev)),
             ("adjoint", "adjoint_512sq_64rhs",
              lambda: bench_adjoint(dev, fd_entries=args.fd_entries)),
             ("adjoint_convection", "adjoint_convection_512sq_64rhs",

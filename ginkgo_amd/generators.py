@@ -208,8 +208,7 @@ def uneven_spd_batch(num_systems=20000, n_min=16, n_max=512, k=8, band=32, seed=
     vals2 = np.concatenate([v_o, v_o])
     key = rows2 * (n_max + 1) + cols2
     uniq, inv = np.unique(key, return_inverse=True)
-    sv = np.zeros(uniq.shape[0])
-    np.add.at(sv, inv, vals2)
+    sv = np.bincount(inv, weights=vals2, minlength=uniq.shape[0])
     ur, uc = uniq // (n_max + 1), uniq % (n_max + 1)
     diag = np.bincount(ur, weights=np.abs(sv), minlength=total) + 1.0
     all_r = np.concatenate([ur, np.arange(total, dtype=np.int64)])
