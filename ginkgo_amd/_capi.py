@@ -20,6 +20,7 @@ DIMENSION_MISMATCH = 5
 
 CSR_CLASSICAL = 0
 CSR_LOAD_BALANCE = 1
+CSR_MERGE_PATH = 2
 
 STATUS_CONVERGED = 0x80
 STATUS_FINALIZED = 0x40
@@ -85,7 +86,8 @@ class Csr(ctypes.Structure):
                 ("row_ptrs", ctypes.c_void_p), ("col_idxs", ctypes.c_void_p),
                 ("values", ctypes.c_void_p), ("strategy", ctypes.c_int32),
                 ("subwave", ctypes.c_int32), ("srow", ctypes.c_void_p),
-                ("srow_blocks", ctypes.c_int64), ("col_offsets", ctypes.c_void_p)]
+                ("srow_blocks", ctypes.c_int64), ("col_offsets", ctypes.c_void_p),
+                ("work", ctypes.c_void_p), ("work_bytes", ctypes.c_int64)]
 
 
 class Sellp(ctypes.Structure):
@@ -214,6 +216,7 @@ _SIGS = {
     "gkomi_csr_lb_chunk": (_I64, [_I32]),
     "gkomi_csr_lb_partition": (ctypes.c_int, [_CTX, _I64, _P, _I64, _I64, _P, _I64]),
     "gkomi_csr_max_row_nnz": (ctypes.c_int, [_CTX, _I64, _P, _P]),
+    "gkomi_csr_strategy_work_bytes": (_I64, [_I32, _I64, _I64]),
     "gkomi_csr_col_offsets": (ctypes.c_int, [_CTX, ctypes.POINTER(Csr), _P, _P]),
     "gkomi_csr_transpose_workspace_bytes": (_SZ, [_I64, _I64, _I64]),
     "gkomi_scan_workspace_bytes": (_SZ, [_I64]),

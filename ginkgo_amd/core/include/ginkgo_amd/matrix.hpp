@@ -68,12 +68,27 @@ public:
         explicit load_balance(int64) {}
         std::string get_name() const override { return "load_balance"; }
     };
+    // merge-path partition of rows and entries (csr.hpp:283-301): every
+    // workgroup takes an equal share of n + nnz merge steps
+    class merge_path : public strategy_type {
+    public:
+        merge_path() = default;
+        std::string get_name() const override { return "merge_path"; }
+    };
     // picks classical for short uniform rows, load_balance otherwise
     // (automatical::process, csr.hpp:606-673, decides from row statistics)
     class automatical : public strategy_type {
     public:
         automatical() = default;
         std::string get_name() const override { return "automatical"; }
+    };
+    // "use the sparse library's own kernel" (csr.hpp:332-349: cuSPARSE /
+    // hipSPARSE there): on this executor the library is this one, so the
+    // choice is made as automatical makes it.
+    class sparselib : public strategy_type {
+    public:
+        sparselib() = default;
+        std::string get_name() const override { return "sparselib"; }
     };
 
     static std::unique_ptr<Csr> create(std::shared_ptr<const Executor> exec, dim<2> size = {},
@@ -156,6 +171,7 @@ protected:
           col_idxs_(exec, nnz),
           row_ptrs_(exec, size.rows + 1),
           srow_(exec),
+          strategy_work_(exec),
           col_offsets_(exec),
           strategy_{strategy ? std::move(strategy) : std::make_shared<automatical>()}
     {
@@ -173,6 +189,7 @@ private:
     array<IndexType> col_idxs_;
     array<IndexType> row_ptrs_;
     array<IndexType> srow_;
+    mutable array<char> strategy_work_;  // merge_path carries
     array<int16> col_offsets_;
     bool compact_{false};
     std::shared_ptr<strategy_type> strategy_;

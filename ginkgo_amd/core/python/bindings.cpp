@@ -60,7 +60,10 @@ std::shared_ptr<typename matrix::Csr<V>::strategy_type> strategy(const std::stri
     if (s == "classical") return std::make_shared<typename C::classical>(sw);
     if (s == "load_balance") return std::make_shared<typename C::load_balance>();
     if (s == "automatical") return std::make_shared<typename C::automatical>();
-    throw py::value_error("strategy must be classical, load_balance or automatical");
+    if (s == "merge_path") return std::make_shared<typename C::merge_path>();
+    if (s == "sparselib") return std::make_shared<typename C::sparselib>();
+    throw py::value_error(
+        "strategy must be classical, load_balance, merge_path, sparselib or automatical");
 }
 
 template <typename T>

@@ -138,9 +138,10 @@ void Csr<V, I>::set_strategy(std::shared_ptr<strategy_type> s)
 }
 
 // Resolves the strategy and builds the load-balance partition on the
-// device (the role of make_srow, csr.hpp:1651-1655).  automatical decides
-// from the longest and the mean row length: classical (sub-wave per row)
-// for short uniform rows, load_balance otherwise.
+// device (the role of make_srow, csr.hpp:1651-1655).  automatical (and
+// sparselib) decide from the longest and the mean row length: classical
+// (sub-wave per row) for short uniform rows, load_balance otherwise.
+// merge_path keeps a workspace for the workgroups' carries.
 template <typename V, typename I>
 void Csr<V, I>::make_srow()
 {
@@ -152,7 +153,7 @@ void Csr<V, I>::make_srow()
     subwave_ = 0;
     if (auto c = std::dynamic_pointer_cast<classical>(strategy_)) subwave_ = c->subwave();
     max_row_nnz_ = -1;
-    if (name == "automatical") {
+    if (name == "automatical" || name == "sparselib") {
         const double mean = this->get_size().rows
                                 ? double(values_.get_size()) / double(this->get_size().rows)
                                 : 0.0;
@@ -170,6 +171,14 @@ void Csr<V, I>::make_srow()
                                         srow_.get_data(), blocks));
     } else {
         srow_.clear();
+    }
+    if (name == "merge_path") {
+        const int64 wb = gkomi_csr_strategy_work_bytes(
+            GKOMI_CSR_MERGE_PATH, static_cast<int64>(this->get_size().rows),
+            static_cast<int64>(values_.get_size()));
+        strategy_work_ = array<char>(this->get_executor(), static_cast<size_type>(wb));
+    } else {
+        strategy_work_.clear();
     }
 }
 
@@ -195,11 +204,15 @@ gkomi_csr Csr<V, I>::descriptor() const
     d.row_ptrs = row_ptrs_.get_const_data();
     d.col_idxs = col_idxs_.get_const_data();
     d.values = values_.get_const_data();
-    d.strategy = effective_ == "load_balance" ? GKOMI_CSR_LOAD_BALANCE : GKOMI_CSR_CLASSICAL;
+    d.strategy = effective_ == "load_balance"
+                     ? GKOMI_CSR_LOAD_BALANCE
+                     : (effective_ == "merge_path" ? GKOMI_CSR_MERGE_PATH : GKOMI_CSR_CLASSICAL);
     d.subwave = subwave_;
     d.srow = srow_.get_const_data();
     d.srow_blocks = srow_.get_size() ? static_cast<int64>(srow_.get_size()) - 1 : 0;
     d.col_offsets = compact_ ? col_offsets_.get_const_data() : nullptr;
+    d.work = strategy_work_.get_data();
+    d.work_bytes = static_cast<int64>(strategy_work_.get_size());
     return d;
 }
 
@@ -330,6 +343,8 @@ std::unique_ptr<Csr<Other, I>> Csr<V, I>::convert_precision() const
     std::shared_ptr<typename Csr<Other, I>::strategy_type> st;
     if (effective_ == "load_balance") {
         st = std::make_shared<typename Csr<Other, I>::load_balance>();
+    } else if (effective_ == "merge_path") {
+        st = std::make_shared<typename Csr<Other, I>::merge_path>();
     } else {
         st = std::make_shared<typename Csr<Other, I>::classical>(subwave_);
     }

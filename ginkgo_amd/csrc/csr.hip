@@ -285,7 +285,8 @@ int check_csr(const gkomi_csr* a)
                   "CSR: null row pointers");
     GKOMI_REQUIRE(a->nnz == 0 || (a->col_idxs != nullptr && a->values != nullptr),
                   GKOMI_INVALID_ARGUMENT, "CSR: null column indices or values");
-    GKOMI_REQUIRE(a->strategy == GKOMI_CSR_CLASSICAL || a->strategy == GKOMI_CSR_LOAD_BALANCE,
+    GKOMI_REQUIRE(a->strategy == GKOMI_CSR_CLASSICAL || a->strategy == GKOMI_CSR_LOAD_BALANCE ||
+                      a->strategy == GKOMI_CSR_MERGE_PATH,
                   GKOMI_INVALID_ARGUMENT, "CSR: unknown strategy");
     GKOMI_REQUIRE(a->strategy != GKOMI_CSR_LOAD_BALANCE ||
                       (a->srow != nullptr && a->srow_blocks >= 1),
@@ -328,6 +329,9 @@ int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const 
                                                           beta);
         GKOMI_CHECK_LAUNCH("csr_spmm_rowwave");
         return GKOMI_SUCCESS;
+    }
+    if (a->strategy == GKOMI_CSR_MERGE_PATH) {
+        return csr_spmv_merge_launch<T, Adv>(stream, a, b, bs, c, cs, alpha, beta);
     }
     if (a->strategy == GKOMI_CSR_LOAD_BALANCE) {
         GKOMI_REQUIRE(a->srow_blocks < (int64_t{1} << 31), GKOMI_INVALID_ARGUMENT,
@@ -424,6 +428,12 @@ int64_t gkomi_csr_lb_chunk(int32_t value_bytes)
 {
     (void)value_bytes;
     return kLbChunk;
+}
+
+int64_t gkomi_csr_strategy_work_bytes(int32_t strategy, int64_t n_rows, int64_t nnz)
+{
+    if (strategy != GKOMI_CSR_MERGE_PATH || n_rows < 0 || nnz < 0) return 0;
+    return static_cast<int64_t>(merge_work_bytes(n_rows, nnz));
 }
 
 int gkomi_csr_lb_partition(const gkomi_ctx* ctx, int64_t n_rows, const int32_t* row_ptrs,

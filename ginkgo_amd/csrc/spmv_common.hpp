@@ -144,4 +144,10 @@ template <typename T, bool Adv>
 int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const T* b, int64_t bs,
                     T* c, int64_t cs, const T* alpha, const T* beta);
 
+// merge_path strategy (merge.hip): single right-hand side.
+size_t merge_work_bytes(int64_t n_rows, int64_t nnz);
+template <typename T, bool Adv>
+int csr_spmv_merge_launch(hipStream_t stream, const gkomi_csr* a, const T* b, int64_t bs, T* c,
+                          int64_t cs, const T* alpha, const T* beta);
+
 }  // namespace gkomi

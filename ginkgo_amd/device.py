@@ -52,9 +52,10 @@ def scalar(value, dtype, device):
 class DeviceCsr:
     """Csr<V, int32> on the device with its SpMV strategy.
 
-    strategy: 'classical', 'load_balance' or 'automatical' (picks classical
-    for short, uniform rows and load_balance otherwise, as the reference's
-    automatical::process does from row statistics, csr.hpp:606-673)."""
+    strategy: 'classical', 'load_balance', 'merge_path' or 'automatical'
+    (picks classical for short, uniform rows and load_balance otherwise, as
+    the reference's automatical::process does from row statistics,
+    csr.hpp:606-673)."""
 
     def __init__(self, row_ptrs, col_idxs, values, n_cols=None, strategy="automatical",
                  subwave=0, device="cuda"):
@@ -88,6 +89,10 @@ class DeviceCsr:
             c = ctx(dev)
             C.call("gkomi_csr_lb_partition", ctypes.byref(c), self.n_rows, ptr(self.row_ptrs),
                    self.nnz, chunk, ptr(self.srow), self.srow_blocks)
+        elif strategy == "merge_path":
+            wb = int(C.lib().gkomi_csr_strategy_work_bytes(C.CSR_MERGE_PATH, self.n_rows,
+                                                           self.nnz))
+            self.work = workspace(wb, dev)
         elif strategy != "classical":
             raise C.InvalidArgument(f"unknown strategy {strategy}")
         self.col_offsets = None
@@ -115,12 +120,16 @@ class DeviceCsr:
         d.row_ptrs = ptr(self.row_ptrs).value
         d.col_idxs = ptr(self.col_idxs).value
         d.values = ptr(self.values).value
-        d.strategy = C.CSR_CLASSICAL if self.strategy == "classical" else C.CSR_LOAD_BALANCE
+        d.strategy = {"classical": C.CSR_CLASSICAL, "load_balance": C.CSR_LOAD_BALANCE,
+                      "merge_path": C.CSR_MERGE_PATH}[self.strategy]
         d.subwave = self.subwave
         d.srow = ptr(self.srow).value if self.srow is not None else None
         d.srow_blocks = self.srow_blocks
         d.col_offsets = ptr(self.col_offsets).value if getattr(self, "col_offsets", None) \
             is not None else None
+        work = getattr(self, "work", None)
+        d.work = ptr(work).value if work is not None else None
+        d.work_bytes = work.numel() if work is not None else 0
         return d
 
     def spmv(self, b, out=None, stream=None):

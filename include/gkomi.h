@@ -156,7 +156,10 @@ int gkomi_dense_copy_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const fl
  * rows in flight; `subwave` = S | (R << 8), S a power of two <= 64, R one of
  * 1, 2, 4, 8, either 0 = chosen from the mean row length.  LOAD_BALANCE: nnz-balanced workgroups over the
  * partition `srow` built by gkomi_csr_lb_partition. */
-enum { GKOMI_CSR_CLASSICAL = 0, GKOMI_CSR_LOAD_BALANCE = 1 };
+/* Csr strategies (csr.hpp:233-699): classical (sub-wave rows), load_balance
+ * (nnz-balanced partition `srow`), merge_path (merge-path partition of rows
+ * and entries, csr.hpp:283-301). */
+enum { GKOMI_CSR_CLASSICAL = 0, GKOMI_CSR_LOAD_BALANCE = 1, GKOMI_CSR_MERGE_PATH = 2 };
 
 /* Matrix::Csr<V, int32> storage (csr.hpp:104-1690) plus its strategy data.
  * `values` has the value type of the entry point it is passed to. */
@@ -176,7 +179,17 @@ typedef struct {
      * stays required; the CLASSICAL single right-hand-side SpMV and the fused
      * CG read these 2-byte offsets instead of the 4-byte indices. */
     const int16_t* col_offsets;
+    /* Strategy scratch of MERGE_PATH (the workgroups' carries):
+     * gkomi_csr_strategy_work_bytes bytes of device memory kept with the
+     * matrix, like srow; no initialisation needed.  Every SpMV through the
+     * descriptor writes it, so SpMVs of one matrix must not overlap. */
+    void* work;
+    int64_t work_bytes;
 } gkomi_csr;
+
+/* Bytes of strategy scratch (gkomi_csr::work) an SpMV with `strategy` needs
+ * for an n_rows x ? matrix with nnz entries (0 for CLASSICAL and LOAD_BALANCE). */
+int64_t gkomi_csr_strategy_work_bytes(int32_t strategy, int64_t n_rows, int64_t nnz);
 
 /* nnz handled by one workgroup of the LOAD_BALANCE kernel for value type V. */
 int64_t gkomi_csr_lb_chunk(int32_t value_bytes);
