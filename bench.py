@@ -596,7 +596,15 @@ def main():
         _, ms = time_kernel_loop(lambda: P32.spmv(px32, py32), args.steps, args.warmup, stream)
         secondary["csr_f32_powerlaw1M_GBps"] = round(spmv_bytes(pn, pnnz, 4) / (ms * 1e-3) / 1e9, 1)
         secondary["powerlaw_nnz"] = pnnz
-        del P, P32
+        # the merge_path strategy on the same two matrices
+        PM = D.DeviceCsr(prp, pci, pv, strategy="merge_path", device=dev)
+        _, ms = time_kernel_loop(lambda: PM.spmv(px, py), args.steps, args.warmup, stream)
+        secondary["csr_f64_powerlaw1M_merge_path_GBps"] = round(
+            spmv_bytes(pn, pnnz, 8) / (ms * 1e-3) / 1e9, 1)
+        LM = D.DeviceCsr(rp, ci, v, strategy="merge_path", device=dev)
+        _, ms = time_kernel_loop(lambda: LM.spmv(x, y), args.steps, args.warmup, stream)
+        secondary["csr_f64_laplace128_merge_path_GBps"] = round(b64 / (ms * 1e-3) / 1e9, 1)
+        del P, P32, PM, LM
         # the 27-point matrix of the CG / IR comparison (SpMV alone)
         drp, dci, dv = gen.diffusion27(96)
         dn, dnnz = len(drp) - 1, int(drp[-1])
