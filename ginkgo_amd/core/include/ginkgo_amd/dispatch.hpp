@@ -74,6 +74,17 @@ inline void precision_dispatch(Fn fn, const LinOp* b, LinOp* x)
     dx.write_back();
 }
 
+// alpha, b -> x (the apply2 of Coo: x += alpha A b)
+template <typename V, typename Fn>
+inline void precision_dispatch(Fn fn, const LinOp* alpha, const LinOp* b, LinOp* x)
+{
+    detail::dense_in<V> da(alpha);
+    detail::dense_in<V> db(b);
+    detail::dense_out<V> dx(x);
+    fn(da.ptr, db.ptr, dx.ptr);
+    dx.write_back();
+}
+
 template <typename V, typename Fn>
 inline void precision_dispatch(Fn fn, const LinOp* alpha, const LinOp* b, const LinOp* beta, LinOp* x)
 {

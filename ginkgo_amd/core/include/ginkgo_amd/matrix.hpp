@@ -1,11 +1,14 @@
 // Sparse formats: Csr (include/ginkgo/core/matrix/csr.hpp:104-1690, with the
-// classical / load_balance / automatical strategies), Sellp (sellp.hpp) and
-// Ell (ell.hpp), their conversions and the device transpose; plus
-// matrix_data for assembling from host triplets.
+// classical / load_balance / merge_path / automatical strategies), Sellp
+// (sellp.hpp), Ell (ell.hpp), Coo (coo.hpp) and Hybrid (hybrid.hpp), their
+// conversions and the device transpose; plus matrix_data for assembling from
+// host triplets.
 #pragma once
 
 #include <algorithm>
+#include <string>
 #include <tuple>
+#include <vector>
 
 #include "ginkgo_amd/lin_op.hpp"
 
@@ -35,6 +38,10 @@ template <typename ValueType, typename IndexType>
 class Sellp;
 template <typename ValueType, typename IndexType>
 class Ell;
+template <typename ValueType, typename IndexType>
+class Coo;
+template <typename ValueType, typename IndexType>
+class Hybrid;
 
 template <typename ValueType = double, typename IndexType = int32>
 class Csr : public LinOp {
@@ -142,6 +149,8 @@ public:
     std::unique_ptr<Sellp<ValueType, IndexType>> to_sellp(size_type slice_size = 64,
                                                          size_type stride_factor = 1) const;
     std::unique_ptr<Ell<ValueType, IndexType>> to_ell() const;
+    // Csr::convert_to(Coo) (core/matrix/csr.cpp): row indices from the row pointers
+    std::unique_ptr<Coo<ValueType, IndexType>> to_coo() const;
     template <typename Other>
     std::unique_ptr<Csr<Other, IndexType>> convert_precision() const;
     std::unique_ptr<Csr> clone() const;
@@ -198,6 +207,7 @@ private:
     mutable int64 max_row_nnz_{-1};
     template <typename, typename>
     friend class Csr;
+    friend class Coo<ValueType, IndexType>;
 };
 
 template <typename ValueType = double, typename IndexType = int32>
@@ -277,6 +287,201 @@ private:
     size_type width_;
     size_type stride_;
     friend class Csr<ValueType, IndexType>;
+    friend class Hybrid<ValueType, IndexType>;
+};
+
+// matrix::Coo (include/ginkgo/core/matrix/coo.hpp): one (row, column, value)
+// triplet per entry, rows sorted (the order within a row is free, as the
+// reference's unsorted tests use).  The products read the entries through a
+// Csr view over their row-pointer index, built once (no reference
+// counterpart): 4 (n + 1) bytes of row pointers per product instead of
+// 4 nnz bytes of row indices, and the Csr strategy's kernels.
+template <typename ValueType = double, typename IndexType = int32>
+class Coo : public LinOp {
+public:
+    using value_type = ValueType;
+    using index_type = IndexType;
+    static std::unique_ptr<Coo> create(std::shared_ptr<const Executor> exec, dim<2> size = {},
+                                       size_type nnz = 0);
+    // host triplets copied to exec; throws NotSupported when rows are not sorted
+    static std::unique_ptr<Coo> create_from_host(std::shared_ptr<const Executor> exec,
+                                                 dim<2> size,
+                                                 const std::vector<IndexType>& row_idxs,
+                                                 const std::vector<IndexType>& col_idxs,
+                                                 const std::vector<ValueType>& values);
+    ValueType* get_values() noexcept { return values_.get_data(); }
+    const ValueType* get_const_values() const noexcept { return values_.get_const_data(); }
+    const IndexType* get_const_col_idxs() const noexcept { return col_idxs_.get_const_data(); }
+    const IndexType* get_const_row_idxs() const noexcept { return row_idxs_.get_const_data(); }
+    size_type get_num_stored_elements() const noexcept { return values_.get_size(); }
+    std::vector<IndexType> row_idxs_to_host() const { return row_idxs_.to_host(); }
+    std::vector<IndexType> col_idxs_to_host() const { return col_idxs_.to_host(); }
+    std::vector<ValueType> values_to_host() const { return values_.to_host(); }
+    // Coo::apply2 (coo.hpp): x += A b and x += alpha A b
+    void apply2(const LinOp* b, LinOp* x) const;
+    void apply2(const LinOp* alpha, const LinOp* b, LinOp* x) const;
+    // Coo::convert_to(Csr)
+    std::unique_ptr<Csr<ValueType, IndexType>> to_csr() const;
+    // the Csr view the products run on (shares this matrix's arrays)
+    const Csr<ValueType, IndexType>* get_index() const { return index_.get(); }
+
+protected:
+    Coo(std::shared_ptr<const Executor> exec, dim<2> size, size_type nnz)
+        : LinOp(exec, size),
+          values_(exec, nnz),
+          col_idxs_(exec, nnz),
+          row_idxs_(exec, nnz),
+          row_ptrs_(exec, size.rows + 1)
+    {}
+    void apply_impl(const LinOp* b, LinOp* x) const override;
+    void apply_impl(const LinOp* alpha, const LinOp* b, const LinOp* beta,
+                    LinOp* x) const override;
+    // row_ptrs_ from row_idxs_ (or already filled), then the Csr view
+    void build_index(bool ptrs_ready);
+
+private:
+    array<ValueType> values_;
+    array<IndexType> col_idxs_;
+    array<IndexType> row_idxs_;
+    array<IndexType> row_ptrs_;
+    std::unique_ptr<Csr<ValueType, IndexType>> index_;
+    friend class Csr<ValueType, IndexType>;
+    friend class Hybrid<ValueType, IndexType>;
+};
+
+// matrix::Hybrid (include/ginkgo/core/matrix/hybrid.hpp): an Ell part holding
+// the first K entries of every row and a Coo part with the rest; K comes from
+// the strategy and the row lengths (hybrid.hpp strategy classes, restated).
+// apply = Ell product, then the Coo part added (core/matrix/hybrid.cpp).
+template <typename ValueType = double, typename IndexType = int32>
+class Hybrid : public LinOp {
+public:
+    using value_type = ValueType;
+    using index_type = IndexType;
+
+    class strategy_type {
+    public:
+        virtual ~strategy_type() = default;
+        virtual std::string get_name() const = 0;
+        // K from the row lengths (the vector may be reordered)
+        virtual size_type compute_ell_num_stored_elements_per_row(
+            std::vector<size_type>& row_nnz) const = 0;
+    };
+    // K = num_columns
+    class column_limit : public strategy_type {
+    public:
+        explicit column_limit(size_type num_column = 0) : num_columns_{num_column} {}
+        std::string get_name() const override { return "column_limit"; }
+        size_type compute_ell_num_stored_elements_per_row(
+            std::vector<size_type>&) const override
+        {
+            return num_columns_;
+        }
+        size_type get_num_columns() const { return num_columns_; }
+
+    private:
+        size_type num_columns_;
+    };
+    // K = the row length at position floor(n * percent) of the sorted
+    // lengths (the longest when percent >= 1); percent clamped to [0, 1]
+    class imbalance_limit : public strategy_type {
+    public:
+        explicit imbalance_limit(double percent = 0.8)
+            : percent_{std::max(0.0, std::min(percent, 1.0))}
+        {}
+        std::string get_name() const override { return "imbalance_limit"; }
+        size_type compute_ell_num_stored_elements_per_row(
+            std::vector<size_type>& row_nnz) const override
+        {
+            const auto n = row_nnz.size();
+            if (n == 0) return 0;
+            std::sort(row_nnz.begin(), row_nnz.end());
+            if (percent_ < 1) return row_nnz[static_cast<size_type>(n * percent_)];
+            return row_nnz[n - 1];
+        }
+        double get_percentage() const { return percent_; }
+
+    private:
+        double percent_;
+    };
+    // imbalance_limit bounded by floor(n * ratio)
+    class imbalance_bounded_limit : public strategy_type {
+    public:
+        imbalance_bounded_limit(double percent = 0.8, double ratio = 0.0001)
+            : inner_{percent}, ratio_{ratio}
+        {}
+        std::string get_name() const override { return "imbalance_bounded_limit"; }
+        size_type compute_ell_num_stored_elements_per_row(
+            std::vector<size_type>& row_nnz) const override
+        {
+            const auto n = row_nnz.size();
+            const auto k = inner_.compute_ell_num_stored_elements_per_row(row_nnz);
+            return std::min(k, static_cast<size_type>(n * ratio_));
+        }
+        double get_percentage() const { return inner_.get_percentage(); }
+        double get_ratio() const { return ratio_; }
+
+    private:
+        imbalance_limit inner_;
+        double ratio_;
+    };
+    // imbalance_limit at sizeof(I) / (sizeof(V) + 2 sizeof(I))
+    class minimal_storage_limit : public strategy_type {
+    public:
+        minimal_storage_limit()
+            : inner_{static_cast<double>(sizeof(IndexType)) /
+                     static_cast<double>(sizeof(ValueType) + 2 * sizeof(IndexType))}
+        {}
+        std::string get_name() const override { return "minimal_storage_limit"; }
+        size_type compute_ell_num_stored_elements_per_row(
+            std::vector<size_type>& row_nnz) const override
+        {
+            return inner_.compute_ell_num_stored_elements_per_row(row_nnz);
+        }
+        double get_percentage() const { return inner_.get_percentage(); }
+
+    private:
+        imbalance_limit inner_;
+    };
+    // imbalance_bounded_limit(1/3, 0.001)
+    class automatic : public strategy_type {
+    public:
+        automatic() : inner_{1.0 / 3.0, 0.001} {}
+        std::string get_name() const override { return "automatic"; }
+        size_type compute_ell_num_stored_elements_per_row(
+            std::vector<size_type>& row_nnz) const override
+        {
+            return inner_.compute_ell_num_stored_elements_per_row(row_nnz);
+        }
+
+    private:
+        imbalance_bounded_limit inner_;
+    };
+
+    // Csr::convert_to(Hybrid) (reference/matrix/csr_kernels.cpp:911-957)
+    static std::unique_ptr<Hybrid> create_from_csr(const Csr<ValueType, IndexType>* csr,
+                                                   std::shared_ptr<strategy_type> strategy =
+                                                       nullptr);
+    const Ell<ValueType, IndexType>* get_ell() const { return ell_.get(); }
+    const Coo<ValueType, IndexType>* get_coo() const { return coo_.get(); }
+    size_type get_ell_num_stored_elements_per_row() const
+    {
+        return ell_->get_num_stored_elements_per_row();
+    }
+    size_type get_ell_stride() const { return ell_->get_stride(); }
+    size_type get_coo_num_stored_elements() const { return coo_->get_num_stored_elements(); }
+    std::shared_ptr<strategy_type> get_strategy() const { return strategy_; }
+
+protected:
+    Hybrid(std::shared_ptr<const Executor> exec, dim<2> size) : LinOp(exec, size) {}
+    void apply_impl(const LinOp* b, LinOp* x) const override;
+    void apply_impl(const LinOp* alpha, const LinOp* b, const LinOp* beta,
+                    LinOp* x) const override;
+
+private:
+    std::unique_ptr<Ell<ValueType, IndexType>> ell_;
+    std::unique_ptr<Coo<ValueType, IndexType>> coo_;
+    std::shared_ptr<strategy_type> strategy_;
 };
 
 }  // namespace matrix

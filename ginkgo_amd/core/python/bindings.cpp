@@ -191,6 +191,8 @@ void bind_value_type(py::module_& m, const std::string& sfx)
     using Csr = matrix::Csr<V>;
     using Sellp = matrix::Sellp<V>;
     using Ell = matrix::Ell<V>;
+    using Coo = matrix::Coo<V>;
+    using Hyb = matrix::Hybrid<V>;
 
     py::class_<Dense, LinOp, std::shared_ptr<Dense>>(m, ("Dense" + sfx).c_str())
         .def_static("create",
@@ -264,6 +266,24 @@ void bind_value_type(py::module_& m, const std::string& sfx)
              },
              py::arg("slice_size") = 64, py::arg("stride_factor") = 1)
         .def("to_ell", [](const Csr& a) { return std::shared_ptr<Ell>(a.to_ell()); })
+        .def("to_coo", [](const Csr& a) { return std::shared_ptr<Coo>(a.to_coo()); })
+        .def("to_hybrid",
+             [](const Csr& a, const std::string& s, double p0, double p1) {
+                 std::shared_ptr<typename Hyb::strategy_type> st;
+                 if (s == "automatic") st = std::make_shared<typename Hyb::automatic>();
+                 else if (s == "column_limit")
+                     st = std::make_shared<typename Hyb::column_limit>(static_cast<size_type>(p0));
+                 else if (s == "imbalance_limit")
+                     st = std::make_shared<typename Hyb::imbalance_limit>(p0);
+                 else if (s == "imbalance_bounded_limit")
+                     st = std::make_shared<typename Hyb::imbalance_bounded_limit>(p0, p1);
+                 else if (s == "minimal_storage_limit")
+                     st = std::make_shared<typename Hyb::minimal_storage_limit>();
+                 else
+                     throw py::value_error("unknown Hybrid strategy " + s);
+                 return std::shared_ptr<Hyb>(Hyb::create_from_csr(&a, st));
+             },
+             py::arg("strategy") = "automatic", py::arg("p0") = 0.0, py::arg("p1") = 0.0)
         .def("to_f32", [](const Csr& a) {
             return std::shared_ptr<matrix::Csr<float>>(a.template convert_precision<float>());
         })
@@ -291,6 +311,38 @@ void bind_value_type(py::module_& m, const std::string& sfx)
         .def_property_readonly("stride", &Ell::get_stride)
         .def("col_idxs", [](const Ell& a) { return arr(a.col_idxs_to_host()); })
         .def("values", [](const Ell& a) { return arr(a.values_to_host()); });
+
+    py::class_<Coo, LinOp, std::shared_ptr<Coo>>(m, ("Coo" + sfx).c_str())
+        .def_static("from_numpy",
+                    [](std::shared_ptr<const Executor> e, size_type rows, size_type cols,
+                       py::array_t<int32, py::array::c_style | py::array::forcecast> ri,
+                       py::array_t<int32, py::array::c_style | py::array::forcecast> ci,
+                       py::array_t<V, py::array::c_style | py::array::forcecast> v) {
+                        return std::shared_ptr<Coo>(Coo::create_from_host(
+                            e, dim<2>{rows, cols}, vec<int32>(ri), vec<int32>(ci), vec<V>(v)));
+                    })
+        .def("apply2", [](const Coo& a, const LinOp* b, LinOp* x) { a.apply2(b, x); })
+        .def("apply2", [](const Coo& a, const LinOp* alpha, const LinOp* b, LinOp* x) {
+            a.apply2(alpha, b, x);
+        })
+        .def("to_csr", [](const Coo& a) { return std::shared_ptr<Csr>(a.to_csr()); })
+        .def_property_readonly("nnz", &Coo::get_num_stored_elements)
+        .def_property_readonly("index_strategy",
+                               [](const Coo& a) { return a.get_index()->get_effective_strategy(); })
+        .def("row_idxs", [](const Coo& a) { return arr(a.row_idxs_to_host()); })
+        .def("col_idxs", [](const Coo& a) { return arr(a.col_idxs_to_host()); })
+        .def("values", [](const Coo& a) { return arr(a.values_to_host()); });
+
+    py::class_<Hyb, LinOp, std::shared_ptr<Hyb>>(m, ("Hybrid" + sfx).c_str())
+        .def_property_readonly("ell_width", &Hyb::get_ell_num_stored_elements_per_row)
+        .def_property_readonly("ell_stride", &Hyb::get_ell_stride)
+        .def_property_readonly("coo_nnz", &Hyb::get_coo_num_stored_elements)
+        .def_property_readonly("strategy", [](const Hyb& a) { return a.get_strategy()->get_name(); })
+        .def("ell_col_idxs", [](const Hyb& a) { return arr(a.get_ell()->col_idxs_to_host()); })
+        .def("ell_values", [](const Hyb& a) { return arr(a.get_ell()->values_to_host()); })
+        .def("coo_row_idxs", [](const Hyb& a) { return arr(a.get_coo()->row_idxs_to_host()); })
+        .def("coo_col_idxs", [](const Hyb& a) { return arr(a.get_coo()->col_idxs_to_host()); })
+        .def("coo_values", [](const Hyb& a) { return arr(a.get_coo()->values_to_host()); });
 
     // criteria parameters
     using RN = stop::ResidualNorm<V>;

@@ -21,7 +21,7 @@
 
 namespace gkomi {
 
-template <typename T, int SW, int RPG, bool Adv, bool Strided, bool Compact>
+template <typename T, int SW, int RPG, int Mode, bool Strided, bool Compact>
 __device__ __forceinline__ void csr_subwave_body(int64_t n_rows, const int32_t* __restrict__ rp,
                                                  const int32_t* __restrict__ ci,
                                                  const int16_t* __restrict__ co,
@@ -37,11 +37,11 @@ __device__ __forceinline__ void csr_subwave_body(int64_t n_rows, const int32_t* 
         subwave_tile<T, SW, RPG, Compact>(
             n_rows, t, rp, ci, co, v,
             [&](int32_t col) { return Strided ? b[static_cast<int64_t>(col) * bs] : b[col]; },
-            [&](int64_t row, T sum) { store_result<Adv>(c + row * cs, sum, alpha, beta); });
+            [&](int64_t row, T sum) { store_result<Mode>(c + row * cs, sum, alpha, beta); });
     }
 }
 
-template <typename T, int SW, int RPG, bool Adv>
+template <typename T, int SW, int RPG, int Mode>
 __global__ void __launch_bounds__(kBlock)
     csr_spmv_subwave(int64_t n_rows, const int32_t* __restrict__ rp,
                      const int32_t* __restrict__ ci, const int16_t* __restrict__ co,
@@ -52,18 +52,18 @@ __global__ void __launch_bounds__(kBlock)
     // the compact offsets serve the unit-stride case; strided vectors read
     // the full indices
     if (bs == 1 && co != nullptr) {
-        csr_subwave_body<T, SW, RPG, Adv, false, true>(n_rows, rp, ci, co, v, b, bs, c, cs,
+        csr_subwave_body<T, SW, RPG, Mode, false, true>(n_rows, rp, ci, co, v, b, bs, c, cs,
                                                        alpha, beta);
     } else if (bs == 1) {
-        csr_subwave_body<T, SW, RPG, Adv, false, false>(n_rows, rp, ci, co, v, b, bs, c, cs,
+        csr_subwave_body<T, SW, RPG, Mode, false, false>(n_rows, rp, ci, co, v, b, bs, c, cs,
                                                         alpha, beta);
     } else {
-        csr_subwave_body<T, SW, RPG, Adv, true, false>(n_rows, rp, ci, co, v, b, bs, c, cs,
+        csr_subwave_body<T, SW, RPG, Mode, true, false>(n_rows, rp, ci, co, v, b, bs, c, cs,
                                                        alpha, beta);
     }
 }
 
-template <typename T, bool Adv>
+template <typename T, int Mode>
 __global__ void __launch_bounds__(kBlock)
     csr_spmv_balanced(int64_t n_rows, const int32_t* __restrict__ rp,
                       const int32_t* __restrict__ ci, const T* __restrict__ v,
@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(kBlock)
             if (e - s <= kLbShortRow) {
                 T sum = 0;
                 for (int32_t j = s; j < e; ++j) sum += prod[j];
-                store_result<Adv>(c + static_cast<int64_t>(rs + i) * cs, sum, alpha, beta);
+                store_result<Mode>(c + static_cast<int64_t>(rs + i) * cs, sum, alpha, beta);
             } else {
                 long_rows[atomicAdd(&num_long, 1)] = i;
             }
@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(kBlock)
             for (int32_t j = s + lane; j < e; j += kWave) sum += prod[j];
             sum = wave_sum(sum);
             if (lane == 0) {
-                store_result<Adv>(c + static_cast<int64_t>(rs + i) * cs, sum, alpha, beta);
+                store_result<Mode>(c + static_cast<int64_t>(rs + i) * cs, sum, alpha, beta);
             }
         }
     } else {
@@ -131,13 +131,13 @@ __global__ void __launch_bounds__(kBlock)
             }
             sum = wave_sum(sum);
             if (lane == 0) {
-                store_result<Adv>(c + row * cs, sum, alpha, beta);
+                store_result<Mode>(c + row * cs, sum, alpha, beta);
             }
         }
     }
 }
 
-template <typename T, bool Adv>
+template <typename T, int Mode>
 __global__ void __launch_bounds__(kBlock)
     csr_spmm_rowwave(int64_t n_rows, int64_t nrhs, const int32_t* __restrict__ rp,
                      const int32_t* __restrict__ ci, const T* __restrict__ v,
@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(kBlock)
                 for (int32_t k = s; k < e; ++k) {
                     sum += v[k] * b[static_cast<int64_t>(ci[k]) * bs + j];
                 }
-                store_result<Adv>(c + row * cs + j, sum, alpha, beta);
+                store_result<Mode>(c + row * cs + j, sum, alpha, beta);
             }
         }
     }
@@ -317,26 +317,26 @@ subwave_shape pick_subwave(const gkomi_csr* a, int value_bytes)
     return sh;
 }
 
-template <typename T, bool Adv>
+template <typename T, int Mode>
 int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const T* b, int64_t bs,
                     T* c, int64_t cs, const T* alpha, const T* beta)
 {
     const T* v = static_cast<const T*>(a->values);
     if (nrhs > 1) {
         const unsigned g = grid_for(a->n_rows, kWavesPerBlock, 32);
-        csr_spmm_rowwave<T, Adv><<<g, kBlock, 0, stream>>>(a->n_rows, nrhs, a->row_ptrs,
+        csr_spmm_rowwave<T, Mode><<<g, kBlock, 0, stream>>>(a->n_rows, nrhs, a->row_ptrs,
                                                           a->col_idxs, v, b, bs, c, cs, alpha,
                                                           beta);
         GKOMI_CHECK_LAUNCH("csr_spmm_rowwave");
         return GKOMI_SUCCESS;
     }
     if (a->strategy == GKOMI_CSR_MERGE_PATH) {
-        return csr_spmv_merge_launch<T, Adv>(stream, a, b, bs, c, cs, alpha, beta);
+        return csr_spmv_merge_launch<T, Mode>(stream, a, b, bs, c, cs, alpha, beta);
     }
     if (a->strategy == GKOMI_CSR_LOAD_BALANCE) {
         GKOMI_REQUIRE(a->srow_blocks < (int64_t{1} << 31), GKOMI_INVALID_ARGUMENT,
                       "CSR: partition too large");
-        csr_spmv_balanced<T, Adv>
+        csr_spmv_balanced<T, Mode>
             <<<static_cast<unsigned>(a->srow_blocks), kBlock, 0, stream>>>(
                 a->n_rows, a->row_ptrs, a->col_idxs, v, b, bs, c, cs, a->srow, alpha, beta);
         GKOMI_CHECK_LAUNCH("csr_spmv_balanced");
@@ -346,7 +346,7 @@ int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const 
     const int64_t tile_rows = int64_t{kBlock} / sh.sw * sh.rpg;
     const unsigned g = grid_for(a->n_rows, tile_rows, 64);
 #define GKOMI_SPMV_CASE(S, R)                                                                 \
-    csr_spmv_subwave<T, S, R, Adv><<<g, kBlock, 0, stream>>>(                                 \
+    csr_spmv_subwave<T, S, R, Mode><<<g, kBlock, 0, stream>>>(                                 \
         a->n_rows, a->row_ptrs, a->col_idxs, a->col_offsets, v, b, bs, c, cs, alpha, beta);   \
     break;
     GKOMI_SUBWAVE_SWITCH(sh, GKOMI_SPMV_CASE)
@@ -355,20 +355,26 @@ int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const 
     return GKOMI_SUCCESS;
 }
 
-template int csr_spmv_launch<double, false>(hipStream_t, const gkomi_csr*, int64_t,
-                                            const double*, int64_t, double*, int64_t,
-                                            const double*, const double*);
-template int csr_spmv_launch<double, true>(hipStream_t, const gkomi_csr*, int64_t,
+template int csr_spmv_launch<double, kPlain>(hipStream_t, const gkomi_csr*, int64_t,
                                            const double*, int64_t, double*, int64_t,
                                            const double*, const double*);
-template int csr_spmv_launch<float, false>(hipStream_t, const gkomi_csr*, int64_t,
+template int csr_spmv_launch<double, kAdvanced>(hipStream_t, const gkomi_csr*, int64_t,
+                                           const double*, int64_t, double*, int64_t,
+                                           const double*, const double*);
+template int csr_spmv_launch<double, kAccumulate>(hipStream_t, const gkomi_csr*, int64_t,
+                                           const double*, int64_t, double*, int64_t,
+                                           const double*, const double*);
+template int csr_spmv_launch<float, kPlain>(hipStream_t, const gkomi_csr*, int64_t,
                                            const float*, int64_t, float*, int64_t,
                                            const float*, const float*);
-template int csr_spmv_launch<float, true>(hipStream_t, const gkomi_csr*, int64_t, const float*,
-                                          int64_t, float*, int64_t, const float*,
-                                          const float*);
+template int csr_spmv_launch<float, kAdvanced>(hipStream_t, const gkomi_csr*, int64_t,
+                                           const float*, int64_t, float*, int64_t,
+                                           const float*, const float*);
+template int csr_spmv_launch<float, kAccumulate>(hipStream_t, const gkomi_csr*, int64_t,
+                                           const float*, int64_t, float*, int64_t,
+                                           const float*, const float*);
 
-template <typename T, bool Adv>
+template <typename T, int Mode>
 int csr_spmv_impl(const gkomi_ctx* ctx, const T* alpha, const gkomi_csr* a, int64_t nrhs,
                   const T* b, int64_t bs, const T* beta, T* c, int64_t cs)
 {
@@ -380,9 +386,10 @@ int csr_spmv_impl(const gkomi_ctx* ctx, const T* alpha, const gkomi_csr* a, int6
                   "spmv: null vector");
     GKOMI_REQUIRE(bs >= nrhs && cs >= nrhs, GKOMI_INVALID_ARGUMENT,
                   "spmv: stride smaller than the number of columns");
-    GKOMI_REQUIRE(!Adv || (alpha != nullptr && beta != nullptr), GKOMI_INVALID_ARGUMENT,
+    GKOMI_REQUIRE(Mode != kAdvanced || (alpha != nullptr && beta != nullptr),
+                  GKOMI_INVALID_ARGUMENT,
                   "advanced spmv: null alpha or beta");
-    return csr_spmv_launch<T, Adv>(stream_of(ctx), a, nrhs, b, bs, c, cs, alpha, beta);
+    return csr_spmv_launch<T, Mode>(stream_of(ctx), a, nrhs, b, bs, c, cs, alpha, beta);
 }
 
 template <typename T>
@@ -489,27 +496,27 @@ int gkomi_csr_col_offsets(const gkomi_ctx* ctx, const gkomi_csr* a, int16_t* out
 int gkomi_csr_spmv_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t nrhs, const double* b,
                        int64_t bs, double* c, int64_t cs)
 {
-    return csr_spmv_impl<double, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
+    return csr_spmv_impl<double, kPlain>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
 }
 
 int gkomi_csr_spmv_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t nrhs, const float* b,
                        int64_t bs, float* c, int64_t cs)
 {
-    return csr_spmv_impl<float, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
+    return csr_spmv_impl<float, kPlain>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
 }
 
 int gkomi_csr_advanced_spmv_f64(const gkomi_ctx* ctx, const double* alpha, const gkomi_csr* a,
                                 int64_t nrhs, const double* b, int64_t bs, const double* beta,
                                 double* c, int64_t cs)
 {
-    return csr_spmv_impl<double, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
+    return csr_spmv_impl<double, kAdvanced>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
 }
 
 int gkomi_csr_advanced_spmv_f32(const gkomi_ctx* ctx, const float* alpha, const gkomi_csr* a,
                                 int64_t nrhs, const float* b, int64_t bs, const float* beta,
                                 float* c, int64_t cs)
 {
-    return csr_spmv_impl<float, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
+    return csr_spmv_impl<float, kAdvanced>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
 }
 
 int gkomi_jacobi_generate_f64(const gkomi_ctx* ctx, const gkomi_csr* a, double* inv_diag)

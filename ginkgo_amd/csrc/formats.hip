@@ -102,9 +102,9 @@ size_t scan_ws_bytes(int64_t n, size_t elem)
     return elem * static_cast<size_t>(std::max<int64_t>(1, ceildiv(n, kScanTile))) + 256;
 }
 
-// `in` and `out` may alias (in-place scan): each tile reads its inputs into
-// registers before writing.  Wait -- scan_apply's tiles are independent, and
-// phase 1 completes before phase 3, so aliasing is safe.
+// `in` and `out` may alias (in-place scan): phase 1 reads every input before
+// phase 3 starts, and each phase-3 tile reads only its own inputs before
+// writing them.
 template <typename I>
 int exclusive_scan(hipStream_t stream, int64_t n, const I* in, I* out, void* ws)
 {
@@ -220,6 +220,58 @@ __global__ void __launch_bounds__(kBlock)
             for (; k < len; ++k) {
                 e_cols[r + k * stride_e] = ci[b + k];
                 e_vals[r + k * stride_e] = v[b + k];
+            }
+        }
+        for (; k < width; ++k) {
+            e_cols[r + k * stride_e] = -1;
+            e_vals[r + k * stride_e] = T(0);
+        }
+    }
+}
+
+// Hybrid: counts[r] = entries of row r beyond the ELL width (the COO part's
+// row lengths).
+__global__ void __launch_bounds__(kBlock)
+    hybrid_coo_counts_kernel(int64_t n_rows, const int32_t* __restrict__ rp, int64_t width,
+                             int32_t* __restrict__ counts)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; r < n_rows;
+         r += stride) {
+        const int64_t len = rp[r + 1] - rp[r];
+        counts[r] = static_cast<int32_t>(len > width ? len - width : 0);
+    }
+}
+
+// csr::convert_to_hybrid (reference/matrix/csr_kernels.cpp:911-957): the
+// first `width` entries of every row into the column-major ELL part
+// (padding -1 / 0), the rest into the COO part at coo_rp[r] onwards, in row
+// order.  One thread per row: the ELL writes of a column k are coalesced.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    csr_to_hybrid_kernel(int64_t n_rows, const int32_t* __restrict__ rp,
+                         const int32_t* __restrict__ ci, const T* __restrict__ v, int64_t width,
+                         int64_t stride_e, int32_t* __restrict__ e_cols, T* __restrict__ e_vals,
+                         const int32_t* __restrict__ coo_rp, int32_t* __restrict__ coo_rows,
+                         int32_t* __restrict__ coo_cols, T* __restrict__ coo_vals)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; r < stride_e;
+         r += stride) {
+        int64_t k = 0;
+        if (r < n_rows) {
+            const int32_t b = rp[r];
+            const int64_t len = rp[r + 1] - b;
+            const int64_t in_ell = std::min<int64_t>(len, width);
+            for (; k < in_ell; ++k) {
+                e_cols[r + k * stride_e] = ci[b + k];
+                e_vals[r + k * stride_e] = v[b + k];
+            }
+            const int64_t p0 = coo_rp[r];
+            for (int64_t q = width; q < len; ++q) {
+                coo_rows[p0 + q - width] = static_cast<int32_t>(r);
+                coo_cols[p0 + q - width] = ci[b + q];
+                coo_vals[p0 + q - width] = v[b + q];
             }
         }
         for (; k < width; ++k) {
@@ -426,6 +478,57 @@ int csr_to_ell_impl(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t width, int
     return GKOMI_SUCCESS;
 }
 
+size_t hybrid_counts_bytes(int64_t n_rows)
+{
+    return (static_cast<size_t>(std::max<int64_t>(n_rows, 1)) * sizeof(int32_t) + 255) / 256 *
+           256;
+}
+
+size_t hybrid_ws_bytes(int64_t n_rows)
+{
+    return hybrid_counts_bytes(n_rows) + scan_ws_bytes(n_rows, sizeof(int64_t));
+}
+
+int hybrid_coo_row_ptrs_impl(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t width,
+                             int32_t* coo_rp, void* ws, size_t wsb)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_csr(a)) return rc;
+    GKOMI_REQUIRE(width >= 0 && coo_rp != nullptr, GKOMI_INVALID_ARGUMENT,
+                  "hybrid: negative width or null output");
+    GKOMI_REQUIRE(ws != nullptr && wsb >= hybrid_ws_bytes(a->n_rows), GKOMI_INVALID_ARGUMENT,
+                  "hybrid: workspace too small (gkomi_csr_hybrid_workspace_bytes)");
+    hipStream_t s = stream_of(ctx);
+    // the row counts, then the scan's tile sums, in the workspace
+    int32_t* counts = static_cast<int32_t*>(ws);
+    void* scan_ws = static_cast<char*>(ws) + hybrid_counts_bytes(a->n_rows);
+    if (a->n_rows > 0) {
+        hybrid_coo_counts_kernel<<<grid_for(a->n_rows, kBlock), kBlock, 0, s>>>(
+            a->n_rows, a->row_ptrs, width, counts);
+        GKOMI_CHECK_LAUNCH("hybrid_coo_counts_kernel");
+    }
+    return exclusive_scan<int32_t>(s, a->n_rows, counts, coo_rp, scan_ws);
+}
+
+template <typename T>
+int csr_to_hybrid_impl(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t width, int64_t stride,
+                       int32_t* e_cols, T* e_vals, const int32_t* coo_rp, int32_t* coo_rows,
+                       int32_t* coo_cols, T* coo_vals)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_csr(a)) return rc;
+    GKOMI_REQUIRE(width >= 0 && stride >= a->n_rows, GKOMI_INVALID_ARGUMENT,
+                  "hybrid: ELL stride must be >= number of rows");
+    if (stride == 0) return GKOMI_SUCCESS;
+    GKOMI_REQUIRE(coo_rp != nullptr && (width == 0 || (e_cols && e_vals)),
+                  GKOMI_INVALID_ARGUMENT, "hybrid: null output");
+    csr_to_hybrid_kernel<T><<<grid_for(stride, kBlock), kBlock, 0, stream_of(ctx)>>>(
+        a->n_rows, a->row_ptrs, a->col_idxs, static_cast<const T*>(a->values), width, stride,
+        e_cols, e_vals, coo_rp, coo_rows, coo_cols, coo_vals);
+    GKOMI_CHECK_LAUNCH("csr_to_hybrid_kernel");
+    return GKOMI_SUCCESS;
+}
+
 template <typename T, bool Adv>
 int ell_spmv_impl(const gkomi_ctx* ctx, const T* alpha, const gkomi_ell* a, int64_t nrhs,
                   const T* b, int64_t bs, const T* beta, T* c, int64_t cs)
@@ -488,6 +591,34 @@ using namespace gkomi;
 extern "C" {
 
 size_t gkomi_scan_workspace_bytes(int64_t n) { return scan_ws_bytes(n, sizeof(int64_t)); }
+
+size_t gkomi_csr_hybrid_workspace_bytes(int64_t n_rows) { return hybrid_ws_bytes(n_rows); }
+
+int gkomi_csr_hybrid_coo_row_ptrs(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t ell_width,
+                                  int32_t* coo_row_ptrs, void* workspace,
+                                  size_t workspace_bytes)
+{
+    return hybrid_coo_row_ptrs_impl(ctx, a, ell_width, coo_row_ptrs, workspace,
+                                    workspace_bytes);
+}
+
+int gkomi_csr_to_hybrid_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t ell_width,
+                            int64_t ell_stride, int32_t* ell_cols, double* ell_vals,
+                            const int32_t* coo_row_ptrs, int32_t* coo_row_idxs,
+                            int32_t* coo_col_idxs, double* coo_vals)
+{
+    return csr_to_hybrid_impl(ctx, a, ell_width, ell_stride, ell_cols, ell_vals, coo_row_ptrs,
+                              coo_row_idxs, coo_col_idxs, coo_vals);
+}
+
+int gkomi_csr_to_hybrid_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t ell_width,
+                            int64_t ell_stride, int32_t* ell_cols, float* ell_vals,
+                            const int32_t* coo_row_ptrs, int32_t* coo_row_idxs,
+                            int32_t* coo_col_idxs, float* coo_vals)
+{
+    return csr_to_hybrid_impl(ctx, a, ell_width, ell_stride, ell_cols, ell_vals, coo_row_ptrs,
+                              coo_row_idxs, coo_col_idxs, coo_vals);
+}
 
 int gkomi_sellp_compute_slice_sets(const gkomi_ctx* ctx, int64_t n_rows, const int32_t* row_ptrs,
                                    int64_t slice_size, int64_t stride_factor,

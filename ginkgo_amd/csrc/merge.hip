@@ -54,7 +54,7 @@ __device__ __forceinline__ int64_t merge_search(int64_t diag, int64_t n_rows, in
     return lo;
 }
 
-template <typename T, bool Adv>
+template <typename T, int Mode>
 __global__ void __launch_bounds__(kMpThreads)
     csr_spmv_merge_kernel(int64_t n_rows, int64_t nnz, const int32_t* __restrict__ rp,
                           const int32_t* __restrict__ ci, const T* __restrict__ v,
@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(kMpThreads)
                 first_row = i;
                 first_sum = sum;
             } else {
-                store_result<Adv>(c + (row0 + i) * cs, sum, alpha, beta);
+                store_result<Mode>(c + (row0 + i) * cs, sum, alpha, beta);
             }
             sum = T(0);
             ++i;
@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(kMpThreads)
         // before (thread 0's continues earlier workgroups: fix-up kernel)
         T s = first_sum;
         if (tid > 0 && crow[tid - 1] == first_row) s = cval[tid - 1] + s;
-        store_result<Adv>(c + (row0 + first_row) * cs, s, alpha, beta);
+        store_result<Mode>(c + (row0 + first_row) * cs, s, alpha, beta);
     }
     if (tid == kMpThreads - 1) {
         carry_row[blockIdx.x] = static_cast<int32_t>(row0 + crow[tid]);
@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(kMpThreads)
 
 // Adds the workgroup carries to their rows: the first workgroup of each run
 // of equal carry rows sums the run in order.
-template <typename T, bool Adv>
+template <typename T, int Mode>
 __global__ void __launch_bounds__(kBlock)
     csr_spmv_merge_fixup(int64_t groups, int64_t n_rows, const int32_t* __restrict__ carry_row,
                          const T* __restrict__ carry_val, T* __restrict__ c, int64_t cs,
@@ -161,7 +161,11 @@ __global__ void __launch_bounds__(kBlock)
     if (r >= n_rows || (w > 0 && carry_row[w - 1] == r)) return;
     T s = carry_val[w];
     for (int64_t k = w + 1; k < groups && carry_row[k] == r; ++k) s += carry_val[k];
-    if constexpr (Adv) s = alpha[0] * s;
+    if constexpr (Mode == kAdvanced) {
+        s = alpha[0] * s;
+    } else if constexpr (Mode == kAccumulate) {
+        s = alpha ? alpha[0] * s : s;
+    }
     c[static_cast<int64_t>(r) * cs] += s;
 }
 
@@ -178,7 +182,7 @@ size_t merge_work_bytes(int64_t n_rows, int64_t nnz)
     return static_cast<size_t>(g) * (sizeof(int32_t) + sizeof(double)) + 16;
 }
 
-template <typename T, bool Adv>
+template <typename T, int Mode>
 int csr_spmv_merge_launch(hipStream_t stream, const gkomi_csr* a, const T* b, int64_t bs, T* c,
                           int64_t cs, const T* alpha, const T* beta)
 {
@@ -194,27 +198,33 @@ int csr_spmv_merge_launch(hipStream_t stream, const gkomi_csr* a, const T* b, in
     T* carry_val = static_cast<T*>(a->work);
     int32_t* carry_row = reinterpret_cast<int32_t*>(static_cast<char*>(a->work) +
                                                     groups * static_cast<int64_t>(sizeof(double)));
-    csr_spmv_merge_kernel<T, Adv><<<static_cast<unsigned>(groups), kMpThreads, 0, stream>>>(
+    csr_spmv_merge_kernel<T, Mode><<<static_cast<unsigned>(groups), kMpThreads, 0, stream>>>(
         a->n_rows, a->nnz, a->row_ptrs, a->col_idxs, static_cast<const T*>(a->values), b, bs, c,
         cs, alpha, beta, carry_row, carry_val);
     GKOMI_CHECK_LAUNCH("csr_spmv_merge_kernel");
-    csr_spmv_merge_fixup<T, Adv><<<grid_for(groups, kBlock), kBlock, 0, stream>>>(
+    csr_spmv_merge_fixup<T, Mode><<<grid_for(groups, kBlock), kBlock, 0, stream>>>(
         groups, a->n_rows, carry_row, carry_val, c, cs, alpha);
     GKOMI_CHECK_LAUNCH("csr_spmv_merge_fixup");
     return GKOMI_SUCCESS;
 }
 
-template int csr_spmv_merge_launch<double, false>(hipStream_t, const gkomi_csr*, const double*,
-                                                  int64_t, double*, int64_t, const double*,
-                                                  const double*);
-template int csr_spmv_merge_launch<double, true>(hipStream_t, const gkomi_csr*, const double*,
+template int csr_spmv_merge_launch<double, kPlain>(hipStream_t, const gkomi_csr*, const double*,
                                                  int64_t, double*, int64_t, const double*,
                                                  const double*);
-template int csr_spmv_merge_launch<float, false>(hipStream_t, const gkomi_csr*, const float*,
+template int csr_spmv_merge_launch<double, kAdvanced>(hipStream_t, const gkomi_csr*, const double*,
+                                                 int64_t, double*, int64_t, const double*,
+                                                 const double*);
+template int csr_spmv_merge_launch<double, kAccumulate>(hipStream_t, const gkomi_csr*, const double*,
+                                                 int64_t, double*, int64_t, const double*,
+                                                 const double*);
+template int csr_spmv_merge_launch<float, kPlain>(hipStream_t, const gkomi_csr*, const float*,
                                                  int64_t, float*, int64_t, const float*,
                                                  const float*);
-template int csr_spmv_merge_launch<float, true>(hipStream_t, const gkomi_csr*, const float*,
-                                                int64_t, float*, int64_t, const float*,
-                                                const float*);
+template int csr_spmv_merge_launch<float, kAdvanced>(hipStream_t, const gkomi_csr*, const float*,
+                                                 int64_t, float*, int64_t, const float*,
+                                                 const float*);
+template int csr_spmv_merge_launch<float, kAccumulate>(hipStream_t, const gkomi_csr*, const float*,
+                                                 int64_t, float*, int64_t, const float*,
+                                                 const float*);
 
 }  // namespace gkomi

@@ -13,15 +13,25 @@ constexpr int kLbLdsRows = 1024;
 // rows up to this length are reduced by one thread from LDS, longer by a wave
 constexpr int kLbShortRow = 32;
 
-// c = sum (plain) or c = alpha * sum + beta * c with beta == 0 ignoring the
-// old c (reference/matrix/csr_kernels.cpp:96-117: NaN in c does not propagate).
-template <bool Adv, typename T>
+// Epilogue of a product row:
+//   kPlain      c = sum;
+//   kAdvanced   c = alpha * sum + beta * c, beta == 0 ignoring the old c
+//               (reference/matrix/csr_kernels.cpp:96-117: NaN in c does not
+//               propagate);
+//   kAccumulate c += alpha * sum, alpha == NULL meaning 1 (the apply2 of a
+//               matrix summed into another product, reference/matrix/
+//               coo_kernels.cpp:61-101).
+enum spmv_mode : int { kPlain = 0, kAdvanced = 1, kAccumulate = 2 };
+
+template <int Mode, typename T>
 __device__ __forceinline__ void store_result(T* dst, T sum, const T* alpha, const T* beta)
 {
-    if constexpr (Adv) {
+    if constexpr (Mode == kAdvanced) {
         const T bt = beta[0];
         const T r = alpha[0] * sum;
         *dst = (bt == T(0)) ? r : r + bt * *dst;
+    } else if constexpr (Mode == kAccumulate) {
+        *dst += alpha ? alpha[0] * sum : sum;
     } else {
         *dst = sum;
     }
@@ -140,13 +150,13 @@ struct subwave_shape {
 subwave_shape pick_subwave(const gkomi_csr* a, int value_bytes);
 
 
-template <typename T, bool Adv>
+template <typename T, int Mode>
 int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const T* b, int64_t bs,
                     T* c, int64_t cs, const T* alpha, const T* beta);
 
 // merge_path strategy (merge.hip): single right-hand side.
 size_t merge_work_bytes(int64_t n_rows, int64_t nnz);
-template <typename T, bool Adv>
+template <typename T, int Mode>
 int csr_spmv_merge_launch(hipStream_t stream, const gkomi_csr* a, const T* b, int64_t bs, T* c,
                           int64_t cs, const T* alpha, const T* beta);
 

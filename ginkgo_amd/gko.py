@@ -44,7 +44,8 @@ def _typed(name):
 
 
 matrix = types.SimpleNamespace(
-    Dense=_typed("Dense"), Csr=_typed("Csr"), Sellp=_typed("Sellp"), Ell=_typed("Ell"))
+    Dense=_typed("Dense"), Csr=_typed("Csr"), Sellp=_typed("Sellp"), Ell=_typed("Ell"),
+    Coo=_typed("Coo"), Hybrid=_typed("Hybrid"))
 
 stop = types.SimpleNamespace(
     Iteration=types.SimpleNamespace(build=lambda: _g.Iteration()),

@@ -308,6 +308,49 @@ int gkomi_ell_advanced_spmv_f32(const gkomi_ctx* ctx, const float* alpha, const 
                                 int64_t nrhs, const float* b, int64_t b_stride,
                                 const float* beta, float* c, int64_t c_stride);
 
+/* ---------------------------------------------------------- COO, Hybrid */
+
+/* components::convert_idxs_to_ptrs / convert_ptrs_to_idxs
+ * (core/components/format_conversion_kernels.hpp): ptrs[r] = number of
+ * entries whose row is below r (r = 0..n, idxs sorted); idxs[e] = the row of
+ * entry e.  Integer work, bit-exact. */
+int gkomi_idxs_to_ptrs(const gkomi_ctx* ctx, int64_t nnz, const int32_t* idxs, int64_t n,
+                       int32_t* ptrs);
+int gkomi_ptrs_to_idxs(const gkomi_ctx* ctx, int64_t n, const int32_t* ptrs, int64_t nnz,
+                       int32_t* idxs);
+/* coo::spmv2 / advanced_spmv2 (core/matrix/coo_kernels.hpp;
+ * reference/matrix/coo_kernels.cpp:61-101): c += alpha A b, alpha NULL
+ * meaning 1.  A matrix::Coo (include/ginkgo/core/matrix/coo.hpp, rows sorted)
+ * is passed as the gkomi_csr of its entries over their row-pointer index
+ * (gkomi_idxs_to_ptrs), so the product reads 4 (n + 1) bytes of row pointers
+ * instead of 4 nnz bytes of row indices and runs the CSR kernels of the
+ * descriptor's strategy.  coo::spmv and advanced_spmv are gkomi_csr_spmv_* and
+ * gkomi_csr_advanced_spmv_* on the same descriptor. */
+int gkomi_coo_spmv2_f64(const gkomi_ctx* ctx, const double* alpha, const gkomi_csr* a,
+                        int64_t nrhs, const double* b, int64_t b_stride, double* c,
+                        int64_t c_stride);
+int gkomi_coo_spmv2_f32(const gkomi_ctx* ctx, const float* alpha, const gkomi_csr* a,
+                        int64_t nrhs, const float* b, int64_t b_stride, float* c,
+                        int64_t c_stride);
+/* csr::convert_to_hybrid (reference/matrix/csr_kernels.cpp:911-957) in two
+ * steps.  (1) coo_row_ptrs[r] = number of entries of rows < r beyond the ELL
+ * width (n_rows + 1 values: the row-pointer index of the COO part), with
+ * gkomi_csr_hybrid_workspace_bytes(n_rows) bytes of scratch.  (2) The first
+ * ell_width entries of every row into the column-major ELL part (padding
+ * -1 / 0), the rest into the COO part in row order.  Bit-exact. */
+size_t gkomi_csr_hybrid_workspace_bytes(int64_t n_rows);
+int gkomi_csr_hybrid_coo_row_ptrs(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t ell_width,
+                                  int32_t* coo_row_ptrs, void* workspace,
+                                  size_t workspace_bytes);
+int gkomi_csr_to_hybrid_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t ell_width,
+                            int64_t ell_stride, int32_t* ell_cols, double* ell_vals,
+                            const int32_t* coo_row_ptrs, int32_t* coo_row_idxs,
+                            int32_t* coo_col_idxs, double* coo_vals);
+int gkomi_csr_to_hybrid_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t ell_width,
+                            int64_t ell_stride, int32_t* ell_cols, float* ell_vals,
+                            const int32_t* coo_row_ptrs, int32_t* coo_row_idxs,
+                            int32_t* coo_col_idxs, float* coo_vals);
+
 /* ------------------------------------------------------- scalar Jacobi */
 
 /* Scalar Jacobi generate (core/preconditioner/jacobi.cpp:338-370): extract the

@@ -151,6 +151,13 @@ typedef struct {
                                      int relative, double tol,                     \
                                      int32_t max_iters, int use_omp,               \
                                      int32_t* iters, T* res_norms);                \
+    void ora_coo_spmv2_##S(int64_t nnz, const int32_t* rows, const int32_t* cols,  \
+                           const T* vals, const T* alpha, int64_t nrhs, const T* b, \
+                           int64_t bs, T* c, int64_t cs);                         \
+    void ora_csr_to_hybrid_##S(int64_t n_rows, const int32_t* rp, const int32_t* ci,\
+                               const T* v, int64_t width, int64_t stride,          \
+                               int32_t* ell_cols, T* ell_vals, int32_t* coo_rows,  \
+                               int32_t* coo_cols, T* coo_vals);                    \
     int ora_batch_cg_solve_##S(int64_t num_systems, const int64_t* row_off,        \
                                const int32_t* row_ptrs, const int32_t* col_idxs,   \
                                const T* vals, const T* b, T* x, int use_jacobi,    \

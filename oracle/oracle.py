@@ -337,6 +337,64 @@ def batch_bicgstab_solve(row_off, row_ptrs, col_idxs, vals, b, x0, use_jacobi=Tr
     return x, iters, res
 
 
+def coo_spmv2(rows, cols, vals, b, c, alpha=None):
+    """c += (alpha *) A b entry by entry (reference/matrix/coo_kernels.cpp:61-101)."""
+    dt = vals.dtype
+    b = _c(b, dt).reshape(b.shape[0], -1)
+    c = _c(c, dt).reshape(c.shape[0], -1).copy()
+    a = np.array([alpha], dtype=dt) if alpha is not None else None
+    getattr(lib(), f"ora_coo_spmv2_{_suffix(dt)}")(
+        ctypes.c_int64(len(vals)), _p(_c(rows, np.int32)), _p(_c(cols, np.int32)), _p(_c(vals, dt)),
+        _p(a) if a is not None else None, ctypes.c_int64(b.shape[1]), _p(b),
+        ctypes.c_int64(b.shape[1]), _p(c), ctypes.c_int64(c.shape[1]))
+    return c
+
+
+def csr_to_hybrid(row_ptrs, col_idxs, vals, width, stride=None):
+    """(ell_cols, ell_vals, coo_rows, coo_cols, coo_vals) of
+    reference/matrix/csr_kernels.cpp:911-957 for ELL width `width`."""
+    dt = vals.dtype
+    n = len(row_ptrs) - 1
+    stride = n if stride is None else stride
+    lens = np.diff(np.asarray(row_ptrs, np.int64))
+    coo_nnz = int(np.maximum(lens - width, 0).sum())
+    ec = np.zeros(width * stride, np.int32)
+    ev = np.zeros(width * stride, dt)
+    cr = np.zeros(coo_nnz, np.int32)
+    cc = np.zeros(coo_nnz, np.int32)
+    cv = np.zeros(coo_nnz, dt)
+    getattr(lib(), f"ora_csr_to_hybrid_{_suffix(dt)}")(
+        ctypes.c_int64(n), _p(_c(row_ptrs, np.int32)), _p(_c(col_idxs, np.int32)), _p(_c(vals, dt)),
+        ctypes.c_int64(width), ctypes.c_int64(stride), _p(ec), _p(ev), _p(cr), _p(cc), _p(cv))
+    return ec, ev, cr, cc, cv
+
+
+def hybrid_ell_width(row_nnz, strategy, p0=0.0, p1=0.0, value_bytes=8, index_bytes=4):
+    """ELL width of the Hybrid strategies (include/ginkgo/core/matrix/hybrid.hpp:
+    column_limit, imbalance_limit, imbalance_bounded_limit,
+    minimal_storage_limit, automatic), from the row lengths."""
+    lens = np.sort(np.asarray(row_nnz, np.int64))
+    n = len(lens)
+
+    def imbalance(percent):
+        percent = min(max(percent, 0.0), 1.0)
+        if n == 0:
+            return 0
+        return int(lens[int(n * percent)]) if percent < 1 else int(lens[n - 1])
+
+    if strategy == "column_limit":
+        return int(p0)
+    if strategy == "imbalance_limit":
+        return imbalance(p0)
+    if strategy == "imbalance_bounded_limit":
+        return min(imbalance(p0), int(n * p1))
+    if strategy == "minimal_storage_limit":
+        return imbalance(index_bytes / (value_bytes + 2 * index_bytes))
+    if strategy == "automatic":
+        return min(imbalance(1.0 / 3.0), int(n * 0.001))
+    raise ValueError(strategy)
+
+
 def batch_cg_solve(row_off, row_ptrs, col_idxs, vals, b, x0, use_jacobi=True, relative=True,
                    tol=1e-10, max_iters=100, omp=False):
     """Batch CG (reference/solver/batch_cg_kernels.hpp:31-221) over the

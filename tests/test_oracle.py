@@ -542,3 +542,66 @@ def test_batch_cg_restart_and_zero_rhs():
     assert iters[0] == 1 and res[0] == 0
     assert iters[1] == 2 and res[1] == 0
     np.testing.assert_array_equal(x, b)
+
+
+def _kat_mat(a):
+    return np.array(a, dtype=np.float64)
+
+
+def test_coo_reference_kats():
+    """reference/test/matrix/coo_kernels.cpp: apply, advanced apply (beta
+    scales c, then c += alpha A b: coo_kernels.cpp:45-56), apply2 and the
+    advanced apply2, on the sorted and the column-unsorted matrix."""
+    k = KATS["coo"]
+    for cols, vals in ((k["col_idxs"], k["values"]), (k["unsorted_col_idxs"], k["unsorted_values"])):
+        rows = np.array(k["row_idxs"], np.int32)
+        cols = np.array(cols, np.int32)
+        vals = np.array(vals)
+        for case in k["cases"]:
+            x = _kat_mat(case["x"])
+            if case["op"] == "apply":
+                got = ora.coo_spmv2(rows, cols, vals, x, np.zeros((k["n_rows"], x.shape[1])))
+            elif case["op"] == "advanced":
+                c = case["beta"] * _kat_mat(case["y"])
+                got = ora.coo_spmv2(rows, cols, vals, x, c, alpha=case["alpha"])
+            else:
+                got = ora.coo_spmv2(rows, cols, vals, x, _kat_mat(case["y"]),
+                                    alpha=case.get("alpha"))
+            np.testing.assert_array_equal(got, _kat_mat(case["expected"]), err_msg=case["name"])
+
+
+def test_hybrid_reference_kats():
+    """core/test/matrix/hybrid.cpp:168-240: the ELL width each strategy picks
+    for row lengths [3, 1] and the layout of csr::convert_to_hybrid (the
+    explicit zero entry is kept)."""
+    h = KATS["hybrid"]
+    rp = np.array(h["csr"]["row_ptrs"], np.int32)
+    ci = np.array(h["csr"]["col_idxs"], np.int32)
+    v = np.array(h["csr"]["values"])
+    for lay in h["layouts"]:
+        w = ora.hybrid_ell_width(np.diff(rp), lay["strategy"], lay.get("p0", 0.0))
+        assert w == lay["ell_width"], lay["strategy"]
+        ec, ev, cr, cc, cv = ora.csr_to_hybrid(rp, ci, v, w, lay["ell_stride"])
+        assert ec.tolist() == lay["ell_col_idxs"] and ev.tolist() == lay["ell_values"]
+        assert cr.tolist() == lay["coo_row_idxs"] and cc.tolist() == lay["coo_col_idxs"]
+        assert cv.tolist() == lay["coo_values"]
+
+
+def test_hybrid_parts_sum_to_the_product():
+    """ELL part + COO part of any width reproduce A x (brute force), and the
+    COO part holds exactly the entries beyond the width."""
+    rng = np.random.default_rng(3)
+    d = rng.uniform(-1, 1, (60, 50)) * (rng.random((60, 50)) < 0.15)
+    d[7, :] = rng.uniform(-1, 1, 50)            # one full row
+    rp, ci, v = gen.dense_to_csr(d)
+    x = rng.uniform(-1, 1, 50)
+    for w in (0, 1, 3, 8, 50):
+        ec, ev, cr, cc, cv = ora.csr_to_hybrid(rp, ci, v, w, 60)
+        y = np.zeros(60)
+        for k in range(w):
+            for r in range(60):
+                if ec[r + k * 60] >= 0:
+                    y[r] += ev[r + k * 60] * x[ec[r + k * 60]]
+        y = ora.coo_spmv2(cr, cc, cv, x.reshape(-1, 1), y.reshape(-1, 1))[:, 0]
+        np.testing.assert_allclose(y, d @ x, rtol=0, atol=1e-13)
+        assert len(cv) == int(np.maximum(np.diff(rp) - w, 0).sum())
