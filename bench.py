@@ -604,7 +604,20 @@ def main():
         LM = D.DeviceCsr(rp, ci, v, strategy="merge_path", device=dev)
         _, ms = time_kernel_loop(lambda: LM.spmv(x, y), args.steps, args.warmup, stream)
         secondary["csr_f64_laplace128_merge_path_GBps"] = round(b64 / (ms * 1e-3) / 1e9, 1)
-        del P, P32, PM, LM
+        # Coo and Hybrid (automatic strategy) through the C++ surface
+        import ginkgo_amd.gko as gko
+        ex = gko.Mi355xExecutor.create(0, stream.cuda_stream)
+        G = gko.csr_from_host(ex, prp, pci, pv)
+        GC, GH = G.to_coo(), G.to_hybrid("automatic")
+        bx = gko.dense_view(ex, px)
+        by = gko.dense_view(ex, py)
+        _, ms = time_kernel_loop(lambda: GC.apply(bx, by), args.steps, args.warmup, stream)
+        secondary["coo_f64_powerlaw1M_GBps"] = round(spmv_bytes(pn, pnnz, 8) / (ms * 1e-3) / 1e9, 1)
+        _, ms = time_kernel_loop(lambda: GH.apply(bx, by), args.steps, args.warmup, stream)
+        secondary["hybrid_f64_powerlaw1M_GBps"] = round(
+            spmv_bytes(pn, pnnz, 8) / (ms * 1e-3) / 1e9, 1)
+        secondary["hybrid_powerlaw1M_ell_width_coo_nnz"] = [int(GH.ell_width), int(GH.coo_nnz)]
+        del P, P32, PM, LM, G, GC, GH
         # the 27-point matrix of the CG / IR comparison (SpMV alone)
         drp, dci, dv = gen.diffusion27(96)
         dn, dnnz = len(drp) - 1, int(drp[-1])

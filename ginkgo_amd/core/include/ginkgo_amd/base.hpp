@@ -204,6 +204,10 @@ public:
     // stream; otherwise it orders work on the given hipStream_t.
     static std::shared_ptr<Mi355xExecutor> create(int device_id = 0, void* stream = nullptr,
                                                   bool caching = true);
+    // orders work on exactly `stream`, nullptr meaning the device's null
+    // stream (the stream a framework such as PyTorch runs on by default)
+    static std::shared_ptr<Mi355xExecutor> create_on_stream(int device_id, void* stream,
+                                                            bool caching = true);
     ~Mi355xExecutor() override;
     void* raw_alloc(size_type bytes) const override;
     void raw_free(void* ptr) const noexcept override;
@@ -224,7 +228,7 @@ public:
     void release_pool() const;
 
 protected:
-    Mi355xExecutor(int device_id, void* stream, bool caching);
+    Mi355xExecutor(int device_id, void* stream, bool caching, bool own_if_null);
 
 private:
     gkomi_ctx ctx_{};

@@ -459,10 +459,14 @@ PYBIND11_MODULE(_gko, m)
     py::class_<Mi355xExecutor, Executor, std::shared_ptr<Mi355xExecutor>>(m, "Mi355xExecutor")
         .def_static(
             "create",
-            [](int dev, std::uintptr_t stream, bool caching) {
-                return Mi355xExecutor::create(dev, reinterpret_cast<void*>(stream), caching);
+            // stream None: an own stream; an integer handle (0 = the null
+            // stream, where torch runs by default): exactly that stream
+            [](int dev, py::object stream, bool caching) {
+                if (stream.is_none()) return Mi355xExecutor::create(dev, nullptr, caching);
+                return Mi355xExecutor::create_on_stream(
+                    dev, reinterpret_cast<void*>(stream.cast<std::uintptr_t>()), caching);
             },
-            py::arg("device") = 0, py::arg("stream") = 0, py::arg("caching") = true)
+            py::arg("device") = 0, py::arg("stream") = py::none(), py::arg("caching") = true)
         .def_static("get_num_devices", &Mi355xExecutor::get_num_devices)
         .def_property_readonly("device_id", &Mi355xExecutor::get_device_id)
         .def_property_readonly("num_multiprocessor", &Mi355xExecutor::get_num_multiprocessor)

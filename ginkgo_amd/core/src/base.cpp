@@ -71,15 +71,21 @@ int Mi355xExecutor::get_num_devices()
 
 std::shared_ptr<Mi355xExecutor> Mi355xExecutor::create(int device_id, void* stream, bool caching)
 {
-    return std::shared_ptr<Mi355xExecutor>(new Mi355xExecutor(device_id, stream, caching));
+    return std::shared_ptr<Mi355xExecutor>(new Mi355xExecutor(device_id, stream, caching, true));
 }
 
-Mi355xExecutor::Mi355xExecutor(int device_id, void* stream, bool caching)
+std::shared_ptr<Mi355xExecutor> Mi355xExecutor::create_on_stream(int device_id, void* stream,
+                                                                 bool caching)
+{
+    return std::shared_ptr<Mi355xExecutor>(new Mi355xExecutor(device_id, stream, caching, false));
+}
+
+Mi355xExecutor::Mi355xExecutor(int device_id, void* stream, bool caching, bool own_if_null)
     : caching_{caching}, master_{HostExecutor::create()}
 {
     ctx_.device = device_id;
     GKO_CALL(gkomi_get_device_info(device_id, &info_));
-    if (stream == nullptr) {
+    if (stream == nullptr && own_if_null) {
         GKO_CALL(gkomi_stream_create(device_id, &ctx_.stream));
         own_stream_ = true;
     } else {
