@@ -111,7 +111,7 @@ def build_host(force=False, jobs=None):
     if force or todo or _newer(objs + [os.path.join(LIBDIR, "libgkomi.so")], lib):
         tmp = lib + ".tmp"
         _run([CXX, "-shared", "-o", tmp] + objs +
-             ["-L", LIBDIR, "-lgkomi", "-Wl,-rpath,$ORIGIN"])
+             ["-L", LIBDIR, "-lgkomi", "-ldl", "-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, lib)
     return lib
 

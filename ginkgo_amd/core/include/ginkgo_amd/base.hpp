@@ -23,6 +23,7 @@ using int16 = std::int16_t;
 using int32 = std::int32_t;
 using int64 = std::int64_t;
 using uint8 = std::uint8_t;
+using uintptr = std::uintptr_t;
 
 // include/ginkgo/core/base/dim.hpp
 template <size_type Dim = 2>
@@ -141,31 +142,54 @@ static_assert(sizeof(stopping_status) == 1, "stopping_status must be one byte");
 
 class HostExecutor;
 
+namespace log {
+class Logger;
+}
+
 class Executor : public std::enable_shared_from_this<Executor> {
 public:
     virtual ~Executor() = default;
-    // Executor::alloc / free (executor.hpp:705-740)
+    // the allocation itself, without logging (Executor::raw_alloc /
+    // raw_free, executor.hpp:705-740)
     virtual void* raw_alloc(size_type bytes) const = 0;
     virtual void raw_free(void* ptr) const noexcept = 0;
+    // Executor::alloc / free: raw_alloc / raw_free between the allocation_*
+    // and free_* events of this executor's loggers
     template <typename T>
     T* alloc(size_type num) const
     {
-        return static_cast<T*>(raw_alloc(num * sizeof(T)));
+        const size_type bytes = num * sizeof(T);
+        if (loggers_.empty()) return static_cast<T*>(raw_alloc(bytes));
+        return static_cast<T*>(logged_alloc(bytes));
     }
-    void free(void* ptr) const noexcept { raw_free(ptr); }
-    // Executor::copy_from / raw_copy_to (executor.hpp:745-815)
+    void free(void* ptr) const noexcept
+    {
+        if (loggers_.empty()) {
+            raw_free(ptr);
+        } else {
+            logged_free(ptr);
+        }
+    }
+    // the copy into this executor's memory, without logging
+    // (executor.hpp:745-815)
     virtual void raw_copy_from(const Executor* src, size_type bytes, const void* src_ptr,
                                void* dst_ptr) const = 0;
+    // Executor::copy_from: raw_copy_from between the copy_* events
     template <typename T>
     void copy_from(const Executor* src, size_type num, const T* src_ptr, T* dst_ptr) const
     {
-        raw_copy_from(src, num * sizeof(T), src_ptr, dst_ptr);
+        const size_type bytes = num * sizeof(T);
+        if (loggers_.empty() && src->loggers_.empty()) {
+            raw_copy_from(src, bytes, src_ptr, dst_ptr);
+        } else {
+            logged_copy_from(src, bytes, src_ptr, dst_ptr);
+        }
     }
     template <typename T>
     T copy_val_to_host(const T* ptr) const
     {
         T out{};
-        get_master()->raw_copy_from(this, sizeof(T), ptr, &out);
+        get_master()->copy_from(this, 1, ptr, &out);
         return out;
     }
     virtual void synchronize() const = 0;
@@ -177,6 +201,28 @@ public:
     {
         return other.get() == this || (is_host() && other->is_host());
     }
+
+    // log::EnableLogging of the executor (logger.hpp:786-817) and the
+    // propagation switch (executor.hpp set_log_propagation /
+    // should_propagate_log)
+    void add_logger(std::shared_ptr<const log::Logger> logger) const;
+    void remove_logger(const log::Logger* logger) const;
+    void clear_loggers() const;
+    const std::vector<std::shared_ptr<const log::Logger>>& get_loggers() const
+    {
+        return loggers_;
+    }
+    void set_log_propagation(bool propagate) const { propagate_log_ = propagate; }
+    bool should_propagate_log() const { return propagate_log_ && !loggers_.empty(); }
+
+private:
+    void* logged_alloc(size_type bytes) const;
+    void logged_free(void* ptr) const noexcept;
+    void logged_copy_from(const Executor* src, size_type bytes, const void* src_ptr,
+                          void* dst_ptr) const;
+
+    mutable std::vector<std::shared_ptr<const log::Logger>> loggers_;
+    mutable bool propagate_log_{true};
 };
 
 class HostExecutor : public Executor {

@@ -4,6 +4,9 @@
 
 #include "ginkgo_amd/base.hpp"
 #include "ginkgo_amd/lin_op.hpp"
+#include "ginkgo_amd/log.hpp"
+#include "ginkgo_amd/profiler_hook.hpp"
+#include "ginkgo_amd/timer.hpp"
 #include "ginkgo_amd/matrix.hpp"
 #include "ginkgo_amd/solver.hpp"
 #include "ginkgo_amd/stop.hpp"

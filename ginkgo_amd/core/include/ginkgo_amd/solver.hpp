@@ -1,7 +1,7 @@
-// Preconditioner, solvers and the Convergence logger:
+// Preconditioner and solvers:
 //   preconditioner::Jacobi (core/preconditioner/jacobi.cpp:338-370, scalar case),
 //   solver::Cg (core/solver/cg.cpp), solver::Bicgstab (core/solver/bicgstab.cpp),
-//   solver::Ir (core/solver/ir.cpp), log::Convergence (core/log/convergence.cpp).
+//   solver::Ir (core/solver/ir.cpp).
 // The builder API follows abstract_factory.hpp: Cg<V>::build()
 // .with_criteria(...).with_preconditioner(...).on(exec)->generate(A)->apply(b, x).
 #pragma once
@@ -10,51 +10,6 @@
 #include "ginkgo_amd/stop.hpp"
 
 namespace gko {
-
-namespace log {
-
-// Logger base (include/ginkgo/core/log/logger.hpp:132-540): only the
-// iteration_complete event is emitted by this library's solvers.
-class Logger {
-public:
-    virtual ~Logger() = default;
-    virtual void on_iteration_complete(const LinOp* solver, int64 num_iterations,
-                                       const std::vector<double>& residual_norm,
-                                       const std::vector<stopping_status>& status,
-                                       bool all_stopped) const = 0;
-};
-
-// log::Convergence (include/ginkgo/core/log/convergence.hpp:37,
-// core/log/convergence.cpp:52-110): iteration count, convergence flag and
-// the final residual norm of each right-hand side.
-class Convergence : public Logger {
-public:
-    static std::shared_ptr<Convergence> create() { return std::make_shared<Convergence>(); }
-    void on_iteration_complete(const LinOp*, int64 num_iterations,
-                               const std::vector<double>& residual_norm,
-                               const std::vector<stopping_status>& status,
-                               bool all_stopped) const override
-    {
-        if (!all_stopped) return;
-        num_iterations_ = num_iterations;
-        residual_norm_ = residual_norm;
-        converged_ = !status.empty();
-        for (auto s : status) converged_ = converged_ && s.has_converged();
-        status_ = status;
-    }
-    int64 get_num_iterations() const { return num_iterations_; }
-    bool has_converged() const { return converged_; }
-    const std::vector<double>& get_residual_norm() const { return residual_norm_; }
-    const std::vector<stopping_status>& get_status() const { return status_; }
-
-private:
-    mutable int64 num_iterations_{0};
-    mutable bool converged_{false};
-    mutable std::vector<double> residual_norm_;
-    mutable std::vector<stopping_status> status_;
-};
-
-}  // namespace log
 
 namespace preconditioner {
 
@@ -159,7 +114,7 @@ struct solve_stats {
 };
 
 template <typename ValueType>
-class KrylovBase : public LinOp {
+class KrylovBase : public LinOp, public IterativeBase {
 public:
     using value_type = ValueType;
     bool apply_uses_initial_guess() const override { return true; }
@@ -273,7 +228,7 @@ private:
 // converted once at generate time (the explicit conversion of the
 // mixed-precision-ir example, examples/mixed-precision-ir:85-94).
 template <typename ValueType = double>
-class Ir : public LinOp {
+class Ir : public LinOp, public IterativeBase {
 public:
     using value_type = ValueType;
     class Factory;

@@ -3,6 +3,8 @@
 // would: Cg<V>::build().with_criteria(...).on(exec)->generate(A)->apply(b, x).
 // Device memory handed in from Python (torch tensors) is wrapped in
 // non-owning views; the caller keeps it alive.
+#include <sstream>
+
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
@@ -437,6 +439,214 @@ void bind_value_type(py::module_& m, const std::string& sfx)
         });
 }
 
+
+// ---------------------------------------------------- profiler bindings ----
+
+using PH = log::ProfilerHook;
+
+int64 ns_of(std::chrono::nanoseconds t) { return static_cast<int64>(t.count()); }
+
+// A Python callable kept by C++ code that may release it without the GIL.
+std::shared_ptr<py::object> keep_py(py::object o)
+{
+    return std::shared_ptr<py::object>(new py::object(std::move(o)), [](py::object* p) {
+        py::gil_scoped_acquire g;
+        delete p;
+    });
+}
+
+PH::hook_function py_hook(py::object fn)
+{
+    auto f = keep_py(std::move(fn));
+    return [f](const char* name, log::profile_event_category c) {
+        py::gil_scoped_acquire g;
+        (*f)(name, c);
+    };
+}
+
+py::list py_entries(const std::vector<PH::summary_entry>& e)
+{
+    py::list out;
+    for (const auto& x : e) out.append(x);
+    return out;
+}
+
+// Hands the summary to a Python object's write(entries, overhead_ns) /
+// write_nested(root, overhead_ns).  It runs when the hook is destroyed,
+// which may be inside a destructor: an exception is reported, not thrown.
+struct PyWriter : PH::SummaryWriter, PH::NestedSummaryWriter {
+    explicit PyWriter(py::object o) : obj{keep_py(std::move(o))} {}
+    void write(const std::vector<PH::summary_entry>& e, std::chrono::nanoseconds ov) override
+    {
+        py::gil_scoped_acquire g;
+        try {
+            obj->attr("write")(py_entries(e), ns_of(ov));
+        } catch (py::error_already_set& err) {
+            err.discard_as_unraisable("ProfilerHook summary writer");
+        }
+    }
+    void write_nested(const PH::nested_summary_entry& root, std::chrono::nanoseconds ov) override
+    {
+        py::gil_scoped_acquire g;
+        try {
+            obj->attr("write_nested")(root, ns_of(ov));
+        } catch (py::error_already_set& err) {
+            err.discard_as_unraisable("ProfilerHook summary writer");
+        }
+    }
+    std::shared_ptr<py::object> obj;
+};
+
+// TableSummaryWriter for Python: the table goes to stream.write (stream
+// None: sys.stderr)
+struct PyTableWriter {
+    py::object stream;
+    std::string header;
+    void emit(const std::string& text) const
+    {
+        py::object s = stream.is_none() ? py::module_::import("sys").attr("stderr") : stream;
+        s.attr("write")(text);
+    }
+    void write(const std::vector<PH::summary_entry>& e, int64 overhead) const
+    {
+        std::ostringstream os;
+        PH::TableSummaryWriter(os, header).write(e, std::chrono::nanoseconds{overhead});
+        emit(os.str());
+    }
+    void write_nested(const PH::nested_summary_entry& root, int64 overhead) const
+    {
+        std::ostringstream os;
+        PH::TableSummaryWriter(os, header).write_nested(root, std::chrono::nanoseconds{overhead});
+        emit(os.str());
+    }
+};
+
+// hook.user_range(name) as a context manager
+struct PyRange {
+    std::shared_ptr<const PH> hook;
+    std::string name;
+    std::unique_ptr<log::profiling_scope_guard> guard;
+};
+
+void bind_profiler(py::module_& m)
+{
+    py::enum_<log::profile_event_category>(m, "profile_event_category")
+        .value("memory", log::profile_event_category::memory)
+        .value("operation", log::profile_event_category::operation)
+        .value("object", log::profile_event_category::object)
+        .value("linop", log::profile_event_category::linop)
+        .value("factory", log::profile_event_category::factory)
+        .value("solver", log::profile_event_category::solver)
+        .value("criterion", log::profile_event_category::criterion)
+        .value("user", log::profile_event_category::user)
+        .value("internal", log::profile_event_category::internal);
+
+    py::class_<Timer, std::shared_ptr<Timer>>(m, "Timer")
+        .def_static("create_for_executor", [](std::shared_ptr<const Executor> e) {
+            return std::shared_ptr<Timer>(Timer::create_for_executor(std::move(e)));
+        });
+    py::class_<CpuTimer, Timer, std::shared_ptr<CpuTimer>>(m, "CpuTimer").def(py::init<>());
+    py::class_<Mi355xTimer, Timer, std::shared_ptr<Mi355xTimer>>(m, "Mi355xTimer")
+        .def(py::init<std::shared_ptr<const Executor>>());
+
+    py::class_<PH::summary_entry>(m, "summary_entry")
+        .def(py::init([](std::string name, int64 inclusive, int64 exclusive, int64 count) {
+                 return PH::summary_entry{std::move(name), std::chrono::nanoseconds{inclusive},
+                                          std::chrono::nanoseconds{exclusive}, count};
+             }),
+             py::arg("name"), py::arg("inclusive_ns"), py::arg("exclusive_ns"), py::arg("count"))
+        .def_readonly("name", &PH::summary_entry::name)
+        .def_property_readonly("inclusive_ns",
+                               [](const PH::summary_entry& e) { return ns_of(e.inclusive); })
+        .def_property_readonly("exclusive_ns",
+                               [](const PH::summary_entry& e) { return ns_of(e.exclusive); })
+        .def_readonly("count", &PH::summary_entry::count);
+    py::class_<PH::nested_summary_entry>(m, "nested_summary_entry")
+        .def(py::init([](std::string name, int64 elapsed, int64 count,
+                         std::vector<PH::nested_summary_entry> children) {
+                 return PH::nested_summary_entry{std::move(name), std::chrono::nanoseconds{elapsed},
+                                                 count, std::move(children)};
+             }),
+             py::arg("name"), py::arg("elapsed_ns"), py::arg("count"),
+             py::arg("children") = std::vector<PH::nested_summary_entry>{})
+        .def_readonly("name", &PH::nested_summary_entry::name)
+        .def_property_readonly("elapsed_ns",
+                               [](const PH::nested_summary_entry& e) { return ns_of(e.elapsed); })
+        .def_readonly("count", &PH::nested_summary_entry::count)
+        .def_readonly("children", &PH::nested_summary_entry::children);
+    py::class_<PyTableWriter>(m, "TableSummaryWriter")
+        .def(py::init([](py::object stream, std::string header) {
+                 return PyTableWriter{std::move(stream), std::move(header)};
+             }),
+             py::arg("stream") = py::none(), py::arg("header") = "Runtime summary")
+        .def("write", &PyTableWriter::write, py::arg("entries"), py::arg("overhead_ns"))
+        .def("write_nested", &PyTableWriter::write_nested, py::arg("root"),
+             py::arg("overhead_ns"));
+    py::class_<PyRange>(m, "profiling_scope_guard")
+        .def("__enter__",
+             [](PyRange& r) -> PyRange& {
+                 if (!r.guard) {
+                     r.guard = std::make_unique<log::profiling_scope_guard>(
+                         r.hook->user_range(r.name.c_str()));
+                 }
+                 return r;
+             },
+             py::return_value_policy::reference)
+        .def("__exit__", [](PyRange& r, py::args) {
+            r.guard.reset();
+            return false;
+        });
+
+    auto timer_or_cpu = [](py::object timer) -> std::shared_ptr<Timer> {
+        if (timer.is_none()) return std::make_shared<CpuTimer>();
+        return timer.cast<std::shared_ptr<Timer>>();
+    };
+    py::class_<PH, log::Logger, std::shared_ptr<PH>>(m, "ProfilerHook")
+        .def_static("create_custom",
+                    [](py::object begin, py::object end) {
+                        return PH::create_custom(py_hook(std::move(begin)), py_hook(std::move(end)));
+                    },
+                    py::arg("begin"), py::arg("end"))
+        .def_static("create_roctx", &PH::create_roctx)
+        .def_static("create_for_executor",
+                    [](std::shared_ptr<const Executor> e) { return PH::create_for_executor(e); })
+        .def_static(
+            "create_summary",
+            [timer_or_cpu](py::object timer, py::object writer, bool check) {
+                std::unique_ptr<PH::SummaryWriter> w;
+                if (writer.is_none()) {
+                    w = std::make_unique<PH::TableSummaryWriter>();
+                } else {
+                    w = std::make_unique<PyWriter>(std::move(writer));
+                }
+                return PH::create_summary(timer_or_cpu(std::move(timer)), std::move(w), check);
+            },
+            py::arg("timer") = py::none(), py::arg("writer") = py::none(),
+            py::arg("debug_check_nesting") = false)
+        .def_static(
+            "create_nested_summary",
+            [timer_or_cpu](py::object timer, py::object writer, bool check) {
+                std::unique_ptr<PH::NestedSummaryWriter> w;
+                if (writer.is_none()) {
+                    w = std::make_unique<PH::TableSummaryWriter>();
+                } else {
+                    w = std::make_unique<PyWriter>(std::move(writer));
+                }
+                return PH::create_nested_summary(timer_or_cpu(std::move(timer)), std::move(w),
+                                                 check);
+            },
+            py::arg("timer") = py::none(), py::arg("writer") = py::none(),
+            py::arg("debug_check_nesting") = false)
+        .def("set_object_name",
+             [](PH& h, const LinOp* o, std::string n) { h.set_object_name(o, std::move(n)); })
+        .def("set_object_name", [](PH& h, const LinOpFactory* o,
+                                   std::string n) { h.set_object_name(o, std::move(n)); })
+        .def("set_synchronization", &PH::set_synchronization)
+        .def("user_range", [](std::shared_ptr<const PH> h, std::string name) {
+            return PyRange{std::move(h), std::move(name), nullptr};
+        });
+}
+
 }  // namespace
 
 PYBIND11_MODULE(_gko, m)
@@ -453,7 +663,16 @@ PYBIND11_MODULE(_gko, m)
 
     py::class_<Executor, std::shared_ptr<Executor>>(m, "Executor")
         .def("synchronize", &Executor::synchronize)
-        .def_property_readonly("is_host", &Executor::is_host);
+        .def_property_readonly("is_host", &Executor::is_host)
+        .def("add_logger",
+             [](const Executor& e, std::shared_ptr<const log::Logger> l) { e.add_logger(l); })
+        .def("remove_logger",
+             [](const Executor& e, const log::Logger* l) { e.remove_logger(l); })
+        .def("clear_loggers", &Executor::clear_loggers)
+        .def_property_readonly("num_loggers",
+                               [](const Executor& e) { return e.get_loggers().size(); })
+        .def("set_log_propagation", &Executor::set_log_propagation)
+        .def("should_propagate_log", &Executor::should_propagate_log);
     py::class_<HostExecutor, Executor, std::shared_ptr<HostExecutor>>(m, "HostExecutor")
         .def_static("create", &HostExecutor::create);
     py::class_<Mi355xExecutor, Executor, std::shared_ptr<Mi355xExecutor>>(m, "Mi355xExecutor")
@@ -486,11 +705,17 @@ PYBIND11_MODULE(_gko, m)
                                })
         .def("add_logger", [](const LinOp& op, std::shared_ptr<const log::Logger> l) {
             op.add_logger(l);
-        });
+        })
+        .def("remove_logger", [](const LinOp& op, const log::Logger* l) { op.remove_logger(l); })
+        .def("clear_loggers", &LinOp::clear_loggers);
     py::class_<LinOpFactory, std::shared_ptr<LinOpFactory>>(m, "LinOpFactory")
         .def("generate", [](const LinOpFactory& f, std::shared_ptr<const LinOp> a) {
             return std::shared_ptr<LinOp>(f.generate(a));
-        });
+        })
+        .def("add_logger", [](const LinOpFactory& f, std::shared_ptr<const log::Logger> l) {
+            f.add_logger(l);
+        })
+        .def("clear_loggers", &LinOpFactory::clear_loggers);
 
     py::enum_<stop::mode>(m, "mode")
         .value("absolute", stop::mode::absolute)
@@ -509,6 +734,7 @@ PYBIND11_MODULE(_gko, m)
         .def_property_readonly("num_iterations", &log::Convergence::get_num_iterations)
         .def_property_readonly("has_converged", &log::Convergence::has_converged)
         .def_property_readonly("residual_norm", &log::Convergence::get_residual_norm);
+    bind_profiler(m);
 
     bind_value_type<double>(m, "_f64");
     bind_value_type<float>(m, "_f32");

@@ -4,6 +4,7 @@
 #include <sstream>
 
 #include "ginkgo_amd/base.hpp"
+#include "ginkgo_amd/log.hpp"
 
 namespace gko {
 
@@ -23,6 +24,68 @@ void throw_on_status(int status, const char* file, int line, const char* call)
     case GKOMI_INVALID_ARGUMENT:
     default:
         throw Error(file, line, msg);
+    }
+}
+
+// -------------------------------------------------------- Executor logging ----
+
+void Executor::add_logger(std::shared_ptr<const log::Logger> logger) const
+{
+    loggers_.push_back(std::move(logger));
+}
+
+void Executor::remove_logger(const log::Logger* logger) const
+{
+    for (auto it = loggers_.begin(); it != loggers_.end(); ++it) {
+        if (it->get() == logger) {
+            loggers_.erase(it);
+            return;
+        }
+    }
+    throw Error(__FILE__, __LINE__, "remove_logger: the logger is not attached");
+}
+
+void Executor::clear_loggers() const { loggers_.clear(); }
+
+void* Executor::logged_alloc(size_type bytes) const
+{
+    for (auto& l : loggers_) l->on_allocation_started(this, bytes);
+    void* p = raw_alloc(bytes);
+    const auto loc = reinterpret_cast<uintptr>(p);
+    for (auto& l : loggers_) l->on_allocation_completed(this, bytes, loc);
+    return p;
+}
+
+void Executor::logged_free(void* ptr) const noexcept
+{
+    const auto loc = reinterpret_cast<uintptr>(ptr);
+    // a logger that throws must not turn a free into a leak or a terminate
+    try {
+        for (auto& l : loggers_) l->on_free_started(this, loc);
+    } catch (...) {
+    }
+    raw_free(ptr);
+    try {
+        for (auto& l : loggers_) l->on_free_completed(this, loc);
+    } catch (...) {
+    }
+}
+
+// the copy is logged to the destination's loggers and, when it is another
+// executor, to the source's (executor.hpp:745-781)
+void Executor::logged_copy_from(const Executor* src, size_type bytes, const void* src_ptr,
+                                void* dst_ptr) const
+{
+    const auto from = reinterpret_cast<uintptr>(src_ptr);
+    const auto to = reinterpret_cast<uintptr>(dst_ptr);
+    for (auto& l : loggers_) l->on_copy_started(src, this, from, to, bytes);
+    if (src != this) {
+        for (auto& l : src->loggers_) l->on_copy_started(src, this, from, to, bytes);
+    }
+    raw_copy_from(src, bytes, src_ptr, dst_ptr);
+    for (auto& l : loggers_) l->on_copy_completed(src, this, from, to, bytes);
+    if (src != this) {
+        for (auto& l : src->loggers_) l->on_copy_completed(src, this, from, to, bytes);
     }
 }
 

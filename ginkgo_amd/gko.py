@@ -64,7 +64,13 @@ solver = types.SimpleNamespace(
     Ir=types.SimpleNamespace(build=lambda dtype="f64": _typed("IrParams")(dtype)()),
 )
 
-log = types.SimpleNamespace(Convergence=_g.Convergence)
+log = types.SimpleNamespace(
+    Convergence=_g.Convergence, ProfilerHook=_g.ProfilerHook,
+    TableSummaryWriter=_g.TableSummaryWriter, profile_event_category=_g.profile_event_category,
+    summary_entry=_g.summary_entry, nested_summary_entry=_g.nested_summary_entry)
+Timer = _g.Timer
+CpuTimer = _g.CpuTimer
+Mi355xTimer = _g.Mi355xTimer
 
 # batch module: batch.matrix.Csr (shared pattern), batch.matrix.BlockDiagCsr
 # (items of different sizes), batch.MultiVector, batch.solver.Bicgstab and Cg,
