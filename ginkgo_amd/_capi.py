@@ -159,7 +159,11 @@ class BatchCsr(ctypes.Structure):
                 ("max_nnz", ctypes.c_int64), ("row_offsets", ctypes.c_void_p),
                 ("row_ptrs", ctypes.c_void_p), ("col_idxs", ctypes.c_void_p),
                 ("values", ctypes.c_void_p), ("uniform_rows", ctypes.c_int64),
-                ("uniform_nnz", ctypes.c_int64)]
+                ("uniform_nnz", ctypes.c_int64), ("format", ctypes.c_int32),
+                ("ell_width", ctypes.c_int32), ("uniform_cols", ctypes.c_int64)]
+
+
+BATCH_CSR, BATCH_ELL, BATCH_DENSE = 0, 1, 2
 
 
 class BatchSettings(ctypes.Structure):
@@ -310,6 +314,8 @@ for _s in ("f64", "f32"):
                                                           ctypes.POINTER(BatchSettings),
                                                           ctypes.POINTER(BatchPlan), _P, _P, _P,
                                                           _P, _P, _P, _SZ]),
+        f"gkomi_batch_spmv_{_s}": (ctypes.c_int, [_CTX, ctypes.POINTER(BatchCsr), _P, _P, _P,
+                                                   _P]),
         f"gkomi_batch_cg_solve_{_s}": (ctypes.c_int, [_CTX, ctypes.POINTER(BatchCsr),
                                                      ctypes.POINTER(BatchSettings),
                                                      ctypes.POINTER(BatchPlan), _P, _P, _P, _P,

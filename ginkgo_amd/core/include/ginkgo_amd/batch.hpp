@@ -3,6 +3,9 @@
 // Mirrors the reference's batch surface:
 //   batch::MultiVector          include/ginkgo/core/base/batch_multi_vector.hpp
 //   batch::matrix::Csr          include/ginkgo/core/matrix/batch_csr.hpp (one shared pattern)
+//   batch::matrix::Ell          include/ginkgo/core/matrix/batch_ell.hpp (shared column indices)
+//   batch::matrix::Dense        include/ginkgo/core/matrix/batch_dense.hpp
+//   BatchLinOp::apply of the matrices (simple and advanced)
 //   batch::stop::tolerance_type include/ginkgo/core/stop/batch_stop_enum.hpp
 //   batch::preconditioner::Jacobi (scalar) include/ginkgo/core/preconditioner/batch_jacobi.hpp
 //   batch::solver::Bicgstab     include/ginkgo/core/solver/batch_bicgstab.hpp,
@@ -89,10 +92,49 @@ enum class tolerance_type { absolute, relative };
 
 namespace matrix {
 
+// What the batched matrices share: the descriptor the kernels read, the
+// per-item sizes the solvers plan with, and BatchLinOp::apply
+// (include/ginkgo/core/base/batch_lin_op.hpp) over gkomi_batch_spmv_*.
+template <typename ValueType>
+class BatchMatrix : public BatchLinOp {
+public:
+    using value_type = ValueType;
+    virtual gkomi_batch_csr descriptor() const = 0;
+    // rows and stored values of each item
+    virtual std::vector<int64> get_item_rows() const
+    {
+        return std::vector<int64>(size_.num_batch_items,
+                                  static_cast<int64>(size_.common_size.rows));
+    }
+    virtual std::vector<int64> get_item_nnz() const
+    {
+        return std::vector<int64>(size_.num_batch_items, descriptor().uniform_nnz);
+    }
+    virtual size_type get_total_rows() const
+    {
+        return size_.num_batch_items * size_.common_size.rows;
+    }
+    virtual size_type get_total_cols() const
+    {
+        return size_.num_batch_items * size_.common_size.cols;
+    }
+    // x_k = A_k b_k for every item k
+    void apply(const MultiVector<ValueType>* b, MultiVector<ValueType>* x) const;
+    // x_k = alpha_k A_k b_k + beta_k x_k; alpha and beta hold one value per item
+    void apply(const MultiVector<ValueType>* alpha, const MultiVector<ValueType>* b,
+               const MultiVector<ValueType>* beta, MultiVector<ValueType>* x) const;
+
+protected:
+    using BatchLinOp::BatchLinOp;
+
+private:
+    void check_apply(const MultiVector<ValueType>* b, const MultiVector<ValueType>* x) const;
+};
+
 // Reference batch::matrix::Csr: every item has the same n x n pattern
 // (row_ptrs, col_idxs shared), values stored item after item.
 template <typename ValueType, typename IndexType = int32>
-class Csr : public BatchLinOp {
+class Csr : public BatchMatrix<ValueType> {
 public:
     using value_type = ValueType;
     using index_type = IndexType;
@@ -106,14 +148,70 @@ public:
     const IndexType* get_const_col_idxs() const { return col_idxs_.get_const_data(); }
     const ValueType* get_const_values() const { return values_.get_const_data(); }
     ValueType* get_values() { return values_.get_data(); }
-    gkomi_batch_csr descriptor() const;
+    gkomi_batch_csr descriptor() const override;
 
 private:
     Csr(std::shared_ptr<const Executor> exec, batch_dim<2> size)
-        : BatchLinOp(exec, size), row_ptrs_(exec), col_idxs_(exec), values_(exec)
+        : BatchMatrix<ValueType>(exec, size), row_ptrs_(exec), col_idxs_(exec), values_(exec)
     {}
     array<IndexType> row_ptrs_;
     array<IndexType> col_idxs_;
+    array<ValueType> values_;
+};
+
+// Reference batch::matrix::Ell: the column indices of one item,
+// column-major (slot j of row i at i + j * rows), are shared by all items;
+// -1 marks the slots after a row's last entry.  The values of item k use
+// the same layout and start at k * rows * num_stored_elements_per_row.
+template <typename ValueType, typename IndexType = int32>
+class Ell : public BatchMatrix<ValueType> {
+public:
+    using value_type = ValueType;
+    using index_type = IndexType;
+    static std::unique_ptr<Ell> create_from_host(std::shared_ptr<const Executor> exec,
+                                                 size_type num_items, dim<2> size,
+                                                 size_type num_elems_per_row,
+                                                 const std::vector<IndexType>& col_idxs,
+                                                 const std::vector<ValueType>& values);
+    // From row-major dense items (num_items * rows * cols values).  The
+    // stored pattern is the union of the items' nonzeros, each row's columns
+    // ascending; an item without one of those entries stores a zero there.
+    static std::unique_ptr<Ell> create_from_dense_host(std::shared_ptr<const Executor> exec,
+                                                       size_type num_items, dim<2> size,
+                                                       const std::vector<ValueType>& dense);
+    size_type get_num_stored_elements_per_row() const { return width_; }
+    const IndexType* get_const_col_idxs() const { return col_idxs_.get_const_data(); }
+    const ValueType* get_const_values() const { return values_.get_const_data(); }
+    std::vector<IndexType> col_idxs_to_host() const { return col_idxs_.to_host(); }
+    std::vector<ValueType> values_to_host() const { return values_.to_host(); }
+    gkomi_batch_csr descriptor() const override;
+
+private:
+    Ell(std::shared_ptr<const Executor> exec, batch_dim<2> size)
+        : BatchMatrix<ValueType>(exec, size), col_idxs_(exec), values_(exec)
+    {}
+    size_type width_{};
+    array<IndexType> col_idxs_;
+    array<ValueType> values_;
+};
+
+// Reference batch::matrix::Dense: item k's values, row-major, start at
+// k * rows * cols.
+template <typename ValueType>
+class Dense : public BatchMatrix<ValueType> {
+public:
+    using value_type = ValueType;
+    static std::unique_ptr<Dense> create_from_host(std::shared_ptr<const Executor> exec,
+                                                   size_type num_items, dim<2> size,
+                                                   const std::vector<ValueType>& values);
+    const ValueType* get_const_values() const { return values_.get_const_data(); }
+    std::vector<ValueType> values_to_host() const { return values_.to_host(); }
+    gkomi_batch_csr descriptor() const override;
+
+private:
+    Dense(std::shared_ptr<const Executor> exec, batch_dim<2> size)
+        : BatchMatrix<ValueType>(exec, size), values_(exec)
+    {}
     array<ValueType> values_;
 };
 
@@ -121,7 +219,7 @@ private:
 // owns global rows [row_offsets[k], row_offsets[k+1]); row_ptrs are global
 // (size total_rows + 1), column indices local to the item.
 template <typename ValueType, typename IndexType = int32>
-class BlockDiagCsr : public BatchLinOp {
+class BlockDiagCsr : public BatchMatrix<ValueType> {
 public:
     using value_type = ValueType;
     using index_type = IndexType;
@@ -129,14 +227,19 @@ public:
         std::shared_ptr<const Executor> exec, const std::vector<int64>& row_offsets,
         const std::vector<IndexType>& row_ptrs, const std::vector<IndexType>& col_idxs,
         const std::vector<ValueType>& values);
-    const std::vector<int64>& get_item_rows() const { return item_rows_; }
-    const std::vector<int64>& get_item_nnz() const { return item_nnz_; }
+    std::vector<int64> get_item_rows() const override { return item_rows_; }
+    std::vector<int64> get_item_nnz() const override { return item_nnz_; }
+    size_type get_total_rows() const override
+    {
+        return row_offsets_host_.empty() ? 0 : static_cast<size_type>(row_offsets_host_.back());
+    }
+    size_type get_total_cols() const override { return get_total_rows(); }
     const std::vector<int64>& get_row_offsets() const { return row_offsets_host_; }
-    gkomi_batch_csr descriptor() const;
+    gkomi_batch_csr descriptor() const override;
 
 private:
     BlockDiagCsr(std::shared_ptr<const Executor> exec, batch_dim<2> size)
-        : BatchLinOp(exec, size),
+        : BatchMatrix<ValueType>(exec, size),
           row_offsets_(exec),
           row_ptrs_(exec),
           col_idxs_(exec),

@@ -120,7 +120,10 @@ template <typename V>
 void bind_batch(py::module_& m, const std::string& sfx)
 {
     using MV = batch::MultiVector<V>;
+    using BM = batch::matrix::BatchMatrix<V>;
     using BCsr = batch::matrix::Csr<V, int32>;
+    using BEll = batch::matrix::Ell<V, int32>;
+    using BDense = batch::matrix::Dense<V>;
     using BDiag = batch::matrix::BlockDiagCsr<V, int32>;
     using BJ = batch::preconditioner::Jacobi<V>;
     using BL = batch::log::BatchConvergence<V>;
@@ -151,7 +154,15 @@ void bind_batch(py::module_& m, const std::string& sfx)
         .def("to_numpy", [](const MV& x) { return arr(x.to_host()); })
         .def("fill", &MV::fill)
         .def_property_readonly("total_rows", &MV::get_total_rows);
-    py::class_<BCsr, batch::BatchLinOp, std::shared_ptr<BCsr>>(m, ("BatchCsr" + sfx).c_str())
+    py::class_<BM, batch::BatchLinOp, std::shared_ptr<BM>>(m, ("BatchMatrix" + sfx).c_str())
+        .def("apply", [](const BM& a, const MV* b, MV* x) { a.apply(b, x); })
+        .def("apply", [](const BM& a, const MV* alpha, const MV* b, const MV* beta,
+                         MV* x) { a.apply(alpha, b, beta, x); })
+        .def_property_readonly("size", [](const BM& a) {
+            return py::make_tuple(a.get_num_batch_items(), a.get_common_size().rows,
+                                  a.get_common_size().cols);
+        });
+    py::class_<BCsr, BM, std::shared_ptr<BCsr>>(m, ("BatchCsr" + sfx).c_str())
         .def_static("from_numpy",
                     [](std::shared_ptr<const Executor> e, size_type num, size_type n,
                        py::array_t<int32, py::array::c_style | py::array::forcecast> rp,
@@ -160,7 +171,46 @@ void bind_batch(py::module_& m, const std::string& sfx)
                         return std::shared_ptr<BCsr>(BCsr::create_from_host(
                             e, num, n, vec<int32>(rp), vec<int32>(ci), vec<V>(v)));
                     });
-    py::class_<BDiag, batch::BatchLinOp, std::shared_ptr<BDiag>>(
+    py::class_<BEll, BM, std::shared_ptr<BEll>>(m, ("BatchEll" + sfx).c_str())
+        .def_static("from_numpy",
+                    [](std::shared_ptr<const Executor> e, size_type num, size_type rows,
+                       size_type cols, size_type width,
+                       py::array_t<int32, py::array::c_style | py::array::forcecast> ci,
+                       py::array_t<V, py::array::c_style | py::array::forcecast> v) {
+                        return std::shared_ptr<BEll>(BEll::create_from_host(
+                            e, num, dim<2>{rows, cols}, width, vec<int32>(ci), vec<V>(v)));
+                    })
+        .def_static("from_dense",
+                    [](std::shared_ptr<const Executor> e,
+                       py::array_t<V, py::array::c_style | py::array::forcecast> d) {
+                        if (d.ndim() != 3) {
+                            throw DimensionMismatch(__FILE__, __LINE__,
+                                                    "from_dense: (items, rows, cols) array");
+                        }
+                        return std::shared_ptr<BEll>(BEll::create_from_dense_host(
+                            e, d.shape(0), dim<2>{static_cast<size_type>(d.shape(1)),
+                                                  static_cast<size_type>(d.shape(2))},
+                            vec<V>(d)));
+                    })
+        .def_property_readonly("num_stored_elements_per_row",
+                               &BEll::get_num_stored_elements_per_row)
+        .def("col_idxs", [](const BEll& a) { return arr(a.col_idxs_to_host()); })
+        .def("values", [](const BEll& a) { return arr(a.values_to_host()); });
+    py::class_<BDense, BM, std::shared_ptr<BDense>>(m, ("BatchDense" + sfx).c_str())
+        .def_static("from_numpy",
+                    [](std::shared_ptr<const Executor> e,
+                       py::array_t<V, py::array::c_style | py::array::forcecast> d) {
+                        if (d.ndim() != 3) {
+                            throw DimensionMismatch(__FILE__, __LINE__,
+                                                    "from_numpy: (items, rows, cols) array");
+                        }
+                        return std::shared_ptr<BDense>(BDense::create_from_host(
+                            e, d.shape(0), dim<2>{static_cast<size_type>(d.shape(1)),
+                                                  static_cast<size_type>(d.shape(2))},
+                            vec<V>(d)));
+                    })
+        .def("values", [](const BDense& a) { return arr(a.values_to_host()); });
+    py::class_<BDiag, BM, std::shared_ptr<BDiag>>(
         m, ("BatchBlockDiagCsr" + sfx).c_str())
         .def_static("from_numpy",
                     [](std::shared_ptr<const Executor> e,

@@ -92,8 +92,8 @@ template <typename V, typename I>
 gkomi_batch_csr Csr<V, I>::descriptor() const
 {
     gkomi_batch_csr d{};
-    d.num_systems = static_cast<int64_t>(size_.num_batch_items);
-    d.max_rows = static_cast<int64_t>(size_.common_size.rows);
+    d.num_systems = static_cast<int64_t>(this->size_.num_batch_items);
+    d.max_rows = static_cast<int64_t>(this->size_.common_size.rows);
     d.max_nnz = static_cast<int64_t>(col_idxs_.get_size());
     d.row_offsets = nullptr;
     d.row_ptrs = row_ptrs_.get_const_data();
@@ -101,6 +101,168 @@ gkomi_batch_csr Csr<V, I>::descriptor() const
     d.values = values_.get_const_data();
     d.uniform_rows = d.max_rows;
     d.uniform_nnz = d.max_nnz;
+    return d;
+}
+
+// ---------------------------------------------------- BatchMatrix apply ----
+
+namespace {
+template <typename V>
+struct bm;
+template <>
+struct bm<double> {
+    static constexpr auto spmv = gkomi_batch_spmv_f64;
+};
+template <>
+struct bm<float> {
+    static constexpr auto spmv = gkomi_batch_spmv_f32;
+};
+}  // namespace
+
+template <typename V>
+void BatchMatrix<V>::check_apply(const MultiVector<V>* b, const MultiVector<V>* x) const
+{
+    if (!b || !x) GKO_NOT_SUPPORTED_MSG("batch apply: null vector");
+    const auto num = this->get_num_batch_items();
+    if (b->get_num_batch_items() != num || x->get_num_batch_items() != num ||
+        b->get_total_rows() != get_total_cols() || x->get_total_rows() != get_total_rows()) {
+        GKO_DIMENSION_MISMATCH_MSG("batch apply: vectors do not match the matrix");
+    }
+}
+
+template <typename V>
+void BatchMatrix<V>::apply(const MultiVector<V>* b, MultiVector<V>* x) const
+{
+    check_apply(b, x);
+    const auto d = descriptor();
+    GKO_CALL(bm<V>::spmv(ctx_of(this->get_executor()), &d, nullptr, b->get_const_values(),
+                         nullptr, x->get_values()));
+}
+
+template <typename V>
+void BatchMatrix<V>::apply(const MultiVector<V>* alpha, const MultiVector<V>* b,
+                           const MultiVector<V>* beta, MultiVector<V>* x) const
+{
+    check_apply(b, x);
+    const auto num = this->get_num_batch_items();
+    if (!alpha || !beta || alpha->get_num_batch_items() != num ||
+        beta->get_num_batch_items() != num || alpha->get_total_rows() != num ||
+        beta->get_total_rows() != num) {
+        GKO_DIMENSION_MISMATCH_MSG("batch apply: alpha and beta need one value per item");
+    }
+    const auto d = descriptor();
+    GKO_CALL(bm<V>::spmv(ctx_of(this->get_executor()), &d, alpha->get_const_values(),
+                         b->get_const_values(), beta->get_const_values(), x->get_values()));
+}
+
+// ------------------------------------------------------------------ Ell ----
+
+template <typename V, typename I>
+std::unique_ptr<Ell<V, I>> Ell<V, I>::create_from_host(std::shared_ptr<const Executor> exec,
+                                                       size_type num_items, dim<2> size,
+                                                       size_type num_elems_per_row,
+                                                       const std::vector<I>& col_idxs,
+                                                       const std::vector<V>& values)
+{
+    const size_type per_item = size.rows * num_elems_per_row;
+    if (col_idxs.size() != per_item || values.size() != num_items * per_item) {
+        GKO_DIMENSION_MISMATCH_MSG(
+            "batch::matrix::Ell: col_idxs must hold rows * num_elems_per_row entries and "
+            "values num_items times that");
+    }
+    for (auto c : col_idxs) {
+        if (c < -1 || (c >= 0 && static_cast<size_type>(c) >= size.cols)) {
+            GKO_DIMENSION_MISMATCH_MSG("batch::matrix::Ell: column index out of range");
+        }
+    }
+    std::unique_ptr<Ell> m(new Ell(exec, batch_dim<2>{num_items, size}));
+    m->width_ = num_elems_per_row;
+    m->col_idxs_ = array<I>(exec, col_idxs);
+    m->values_ = array<V>(exec, values);
+    return m;
+}
+
+template <typename V, typename I>
+std::unique_ptr<Ell<V, I>> Ell<V, I>::create_from_dense_host(std::shared_ptr<const Executor> exec,
+                                                             size_type num_items, dim<2> size,
+                                                             const std::vector<V>& dense)
+{
+    const size_type rows = size.rows, cols = size.cols;
+    if (dense.size() != num_items * rows * cols) {
+        GKO_DIMENSION_MISMATCH_MSG("batch::matrix::Ell: dense data must be items * rows * cols");
+    }
+    // the union pattern, row by row
+    std::vector<std::vector<I>> pattern(rows);
+    size_type width = 0;
+    for (size_type i = 0; i < rows; ++i) {
+        for (size_type j = 0; j < cols; ++j) {
+            for (size_type k = 0; k < num_items; ++k) {
+                if (dense[(k * rows + i) * cols + j] != V{}) {
+                    pattern[i].push_back(static_cast<I>(j));
+                    break;
+                }
+            }
+        }
+        width = std::max(width, pattern[i].size());
+    }
+    std::vector<I> ci(rows * width, I{-1});
+    std::vector<V> vals(num_items * rows * width, V{});
+    for (size_type i = 0; i < rows; ++i) {
+        for (size_type s = 0; s < pattern[i].size(); ++s) {
+            const size_type j = static_cast<size_type>(pattern[i][s]);
+            ci[i + s * rows] = pattern[i][s];
+            for (size_type k = 0; k < num_items; ++k) {
+                vals[k * rows * width + i + s * rows] = dense[(k * rows + i) * cols + j];
+            }
+        }
+    }
+    return create_from_host(std::move(exec), num_items, size, width, ci, vals);
+}
+
+template <typename V, typename I>
+gkomi_batch_csr Ell<V, I>::descriptor() const
+{
+    gkomi_batch_csr d{};
+    d.num_systems = static_cast<int64_t>(this->size_.num_batch_items);
+    d.max_rows = static_cast<int64_t>(this->size_.common_size.rows);
+    d.max_nnz = static_cast<int64_t>(col_idxs_.get_size());
+    d.col_idxs = col_idxs_.get_const_data();
+    d.values = values_.get_const_data();
+    d.uniform_rows = d.max_rows;
+    d.uniform_nnz = d.max_nnz;
+    d.format = GKOMI_BATCH_ELL;
+    d.ell_width = static_cast<int32_t>(width_);
+    d.uniform_cols = static_cast<int64_t>(this->size_.common_size.cols);
+    return d;
+}
+
+// ---------------------------------------------------------------- Dense ----
+
+template <typename V>
+std::unique_ptr<Dense<V>> Dense<V>::create_from_host(std::shared_ptr<const Executor> exec,
+                                                     size_type num_items, dim<2> size,
+                                                     const std::vector<V>& values)
+{
+    if (values.size() != num_items * size.rows * size.cols) {
+        GKO_DIMENSION_MISMATCH_MSG("batch::matrix::Dense: values must be items * rows * cols");
+    }
+    std::unique_ptr<Dense> m(new Dense(exec, batch_dim<2>{num_items, size}));
+    m->values_ = array<V>(exec, values);
+    return m;
+}
+
+template <typename V>
+gkomi_batch_csr Dense<V>::descriptor() const
+{
+    gkomi_batch_csr d{};
+    d.num_systems = static_cast<int64_t>(this->size_.num_batch_items);
+    d.max_rows = static_cast<int64_t>(this->size_.common_size.rows);
+    d.max_nnz = static_cast<int64_t>(this->size_.common_size.rows * this->size_.common_size.cols);
+    d.values = values_.get_const_data();
+    d.uniform_rows = d.max_rows;
+    d.uniform_nnz = d.max_nnz;
+    d.format = GKOMI_BATCH_DENSE;
+    d.uniform_cols = static_cast<int64_t>(this->size_.common_size.cols);
     return d;
 }
 
@@ -151,8 +313,8 @@ template <typename V, typename I>
 gkomi_batch_csr BlockDiagCsr<V, I>::descriptor() const
 {
     gkomi_batch_csr d{};
-    d.num_systems = static_cast<int64_t>(size_.num_batch_items);
-    d.max_rows = static_cast<int64_t>(size_.common_size.rows);
+    d.num_systems = static_cast<int64_t>(this->size_.num_batch_items);
+    d.max_rows = static_cast<int64_t>(this->size_.common_size.rows);
     d.max_nnz = item_nnz_.empty() ? 0 : *std::max_element(item_nnz_.begin(), item_nnz_.end());
     d.row_offsets = row_offsets_.get_const_data();
     d.row_ptrs = row_ptrs_.get_const_data();
@@ -237,19 +399,15 @@ BatchKrylov<V, M>::BatchKrylov(std::shared_ptr<const Executor> exec, parameters_
     }
     std::vector<int64> rows, nnz;
     const size_type num = system_->get_num_batch_items();
-    if (auto u = dynamic_cast<const matrix::Csr<V, int32>*>(system_.get())) {
-        desc_ = u->descriptor();
-        rows.assign(num, desc_.uniform_rows);
-        nnz.assign(num, desc_.uniform_nnz);
-        total_rows_ = num * static_cast<size_type>(desc_.uniform_rows);
-    } else if (auto b = dynamic_cast<const matrix::BlockDiagCsr<V, int32>*>(system_.get())) {
-        desc_ = b->descriptor();
-        rows = b->get_item_rows();
-        nnz = b->get_item_nnz();
-        total_rows_ = static_cast<size_type>(b->get_row_offsets().back());
-    } else {
-        GKO_NOT_SUPPORTED_MSG(what_of(who, "system must be batch::matrix::Csr or BlockDiagCsr"));
+    auto mat = dynamic_cast<const matrix::BatchMatrix<V>*>(system_.get());
+    if (!mat) {
+        GKO_NOT_SUPPORTED_MSG(
+            what_of(who, "system must be batch::matrix::Csr, Ell, Dense or BlockDiagCsr"));
     }
+    desc_ = mat->descriptor();
+    rows = mat->get_item_rows();
+    nnz = mat->get_item_nnz();
+    total_rows_ = mat->get_total_rows();
     // plan: items sorted by size (largest first), split into bins
     std::vector<int32> order(num);
     std::iota(order.begin(), order.end(), 0);
@@ -369,6 +527,12 @@ template class matrix::Csr<double, int32>;
 template class matrix::Csr<float, int32>;
 template class matrix::BlockDiagCsr<double, int32>;
 template class matrix::BlockDiagCsr<float, int32>;
+template class matrix::BatchMatrix<double>;
+template class matrix::BatchMatrix<float>;
+template class matrix::Ell<double, int32>;
+template class matrix::Ell<float, int32>;
+template class matrix::Dense<double>;
+template class matrix::Dense<float>;
 
 }  // namespace batch
 }  // namespace gko

@@ -12,11 +12,16 @@
 // the half-step convergence exit) or CG (reference/solver/
 // batch_cg_kernels.hpp:31-221: the implicit residual norm sqrt|r.z|).
 //
-// Two storage formats:
+// Storage formats:
 //  * uneven: a block-diagonal CSR, system k owning global rows
 //    [row_off[k], row_off[k+1]) with column indices local to the system;
-//  * uniform: batch::matrix::Csr of the reference (include/ginkgo/core/matrix/
-//    batch_csr.hpp:29-31): one shared pattern of n rows, values per item.
+//  * uniform Csr: batch::matrix::Csr of the reference (include/ginkgo/core/
+//    matrix/batch_csr.hpp:29-31): one shared pattern of n rows, values per item;
+//  * uniform Ell: batch::matrix::Ell (batch_ell.hpp): shared column-major
+//    column indices, -1 after a row's last entry, values per item;
+//  * uniform Dense: batch::matrix::Dense (batch_dense.hpp): row-major values
+//    per item (staged transposed in LDS, so that the lanes of a wave read
+//    consecutive words).
 #include "common.hpp"
 
 namespace gkomi {
@@ -68,6 +73,9 @@ struct sys_view {
 
 }  // namespace
 
+// How the kernels read the matrix of a batch.
+enum batch_fmt : int { kFmtUneven = 0, kFmtCsr = 1, kFmtEll = 2, kFmtDense = 3 };
+
 // The solver run per system.
 enum batch_method { kBatchBicgstab = 0, kBatchCg = 1 };
 
@@ -80,20 +88,21 @@ __host__ __device__ constexpr int batch_vectors(int method)
 
 // LDS layout per system slot (n = lds_rows, z = lds_nnz, V = batch_vectors):
 //   T vec[V][n]    the work vectors, x and prec last
-//   T vals[z]      (MatLds)
-//   int32 rp[n+1]  (MatLds)
-//   uint16 ci[z]   (MatLds)
-template <typename T, int NThreads, bool MatLds, bool Uniform, int Method>
+//   T vals[z]      (MatLds; Dense transposed)
+//   int32 rp[n+1]  (MatLds, Csr formats)
+//   uint16 ci[z]   (MatLds, Csr and Ell formats; Ell padding 0xFFFF)
+template <typename T, int NThreads, bool MatLds, int Fmt, int Method>
 __global__ void __launch_bounds__(NThreads)
     batch_solver_kernel(int64_t num_systems, const int32_t* __restrict__ order,
                           const int64_t* __restrict__ row_off, const int32_t* __restrict__ rp,
                           const int32_t* __restrict__ ci, const T* __restrict__ vals,
-                          int64_t uni_n, int64_t uni_nnz, const T* __restrict__ b,
-                          T* __restrict__ x, int32_t max_iters, int32_t relative, double tol_d,
-                          int32_t jacobi, int32_t* __restrict__ iters_out,
-                          T* __restrict__ res_out, unsigned* __restrict__ queue,
-                          int64_t lds_rows, int64_t lds_nnz)
+                          int64_t uni_n, int64_t uni_nnz, int32_t ell_width,
+                          const T* __restrict__ b, T* __restrict__ x, int32_t max_iters,
+                          int32_t relative, double tol_d, int32_t jacobi,
+                          int32_t* __restrict__ iters_out, T* __restrict__ res_out,
+                          unsigned* __restrict__ queue, int64_t lds_rows, int64_t lds_nnz)
 {
+    constexpr bool kCsr = Fmt == kFmtUneven || Fmt == kFmtCsr;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ T red[2 * (NThreads / kWave)];
     __shared__ int64_t slot;
@@ -106,7 +115,7 @@ __global__ void __launch_bounds__(NThreads)
     T* prec = vec + (kVec - 1) * L;
     T* lv = vec + kVec * L;
     int32_t* lrp = reinterpret_cast<int32_t*>(lv + (MatLds ? lds_nnz : 0));
-    uint16_t* lci = reinterpret_cast<uint16_t*>(lrp + (MatLds ? L + 1 : 0));
+    uint16_t* lci = reinterpret_cast<uint16_t*>(lrp + (MatLds && kCsr ? L + 1 : 0));
     const T tol = static_cast<T>(tol_d);
 
     while (true) {
@@ -117,7 +126,7 @@ __global__ void __launch_bounds__(NThreads)
         if (k >= num_systems) break;
         const int64_t sys = order ? order[k] : k;
         sys_view sv;
-        if constexpr (Uniform) {
+        if constexpr (Fmt != kFmtUneven) {
             sv.n = static_cast<int32_t>(uni_n);
             sv.nnz = static_cast<int32_t>(uni_nnz);
             sv.row0 = sys * uni_n;
@@ -140,11 +149,26 @@ __global__ void __launch_bounds__(NThreads)
         T* gx = x + sv.row0;
         // stage the matrix
         if constexpr (MatLds) {
-            for (int i = tid; i < sv.nnz; i += NThreads) {
-                lv[i] = gv[i];
-                lci[i] = static_cast<uint16_t>(sv.ci[i]);
+            if constexpr (Fmt == kFmtDense) {
+                // transposed: entry (i, j) at j * n + i
+                for (int e = tid; e < sv.nnz; e += NThreads) {
+                    const int i = e / n;
+                    const int j = e - i * n;
+                    lv[j * n + i] = gv[e];
+                }
+            } else if constexpr (Fmt == kFmtEll) {
+                for (int e = tid; e < sv.nnz; e += NThreads) {
+                    lv[e] = gv[e];
+                    const int32_t c = sv.ci[e];
+                    lci[e] = c < 0 ? uint16_t{0xFFFF} : static_cast<uint16_t>(c);
+                }
+            } else {
+                for (int i = tid; i < sv.nnz; i += NThreads) {
+                    lv[i] = gv[i];
+                    lci[i] = static_cast<uint16_t>(sv.ci[i]);
+                }
+                for (int i = tid; i <= n; i += NThreads) lrp[i] = sv.rp[i] - sv.rp_base;
             }
-            for (int i = tid; i <= n; i += NThreads) lrp[i] = sv.rp[i] - sv.rp_base;
         }
         __syncthreads();
         auto row_begin = [&](int row) {
@@ -155,29 +179,65 @@ __global__ void __launch_bounds__(NThreads)
             if constexpr (MatLds) return lrp[row + 1];
             return sv.rp[row + 1] - sv.rp_base;
         };
-        auto val = [&](int32_t e) {
+        auto val = [&](int64_t e) {
             if constexpr (MatLds) return lv[e];
             return gv[e];
         };
-        auto col = [&](int32_t e) -> int32_t {
-            if constexpr (MatLds) return lci[e];
+        // column of stored entry e; -1 for Ell padding
+        auto col = [&](int64_t e) -> int32_t {
+            if constexpr (MatLds) {
+                const int32_t c = lci[e];
+                if constexpr (Fmt == kFmtEll) return c == 0xFFFF ? -1 : c;
+                return c;
+            }
             return sv.ci[e];
+        };
+        // sum_j a_ij in_j in the order of the reference's simple_apply of
+        // the format (common/cuda_hip/matrix/batch_{csr,ell,dense}_kernels.hpp)
+        auto row_dot = [&](int row, const T* in) -> T {
+            T sum = 0;
+            if constexpr (Fmt == kFmtDense) {
+                for (int j = 0; j < n; ++j) {
+                    const T a = MatLds ? lv[static_cast<int64_t>(j) * n + row]
+                                       : gv[static_cast<int64_t>(row) * n + j];
+                    sum += a * in[j];
+                }
+            } else if constexpr (Fmt == kFmtEll) {
+                for (int j = 0; j < ell_width; ++j) {
+                    const int64_t e = row + static_cast<int64_t>(j) * n;
+                    const int32_t c = col(e);
+                    if (c < 0) break;
+                    sum += val(e) * in[c];
+                }
+            } else {
+                const int32_t e1 = row_end(row);
+                for (int32_t e = row_begin(row); e < e1; ++e) sum += val(e) * in[col(e)];
+            }
+            return sum;
         };
         // out = A in (thread per row)
         auto spmv = [&](const T* in, T* out) {
-            for (int row = tid; row < n; row += NThreads) {
-                T sum = 0;
-                const int32_t e1 = row_end(row);
-                for (int32_t e = row_begin(row); e < e1; ++e) sum += val(e) * in[col(e)];
-                out[row] = sum;
-            }
+            for (int row = tid; row < n; row += NThreads) out[row] = row_dot(row, in);
         };
-        // prec.generate: batch scalar Jacobi (reference/preconditioner/
-        // batch_scalar_jacobi.hpp:79-92) or identity
-        T bb = 0, unused = 0;
-        for (int row = tid; row < n; row += NThreads) {
+        // the scalar Jacobi entry of a row, per format as the reference's
+        // ScalarJacobi::generate (common/cuda_hip/preconditioner/
+        // batch_scalar_jacobi.hpp): Csr and Dense 1 / a_ii with a zero or
+        // missing diagonal giving 1; Ell 1 / a_ii (no zero guard), the row
+        // ending at the first column index below its slot number
+        auto jacobi_of = [&](int row) -> T {
             T d = T(1);
-            if (jacobi) {
+            if constexpr (Fmt == kFmtDense) {
+                const T a = MatLds ? lv[static_cast<int64_t>(row) * n + row]
+                                   : gv[static_cast<int64_t>(row) * n + row];
+                if (a != T(0)) d = T(1) / a;
+            } else if constexpr (Fmt == kFmtEll) {
+                for (int j = 0; j < ell_width; ++j) {
+                    const int64_t e = row + static_cast<int64_t>(j) * n;
+                    const int32_t c = col(e);
+                    if (c < j) break;
+                    if (c == row) d = T(1) / val(e);
+                }
+            } else {
                 const int32_t e1 = row_end(row);
                 for (int32_t e = row_begin(row); e < e1; ++e) {
                     if (col(e) == row) {
@@ -186,7 +246,13 @@ __global__ void __launch_bounds__(NThreads)
                     }
                 }
             }
-            prec[row] = d;
+            return d;
+        };
+        // prec.generate: batch scalar Jacobi (reference/preconditioner/
+        // batch_scalar_jacobi.hpp:79-92) or identity
+        T bb = 0, unused = 0;
+        for (int row = tid; row < n; row += NThreads) {
+            prec[row] = jacobi ? jacobi_of(row) : T(1);
             const T bi = gb[row];
             xl[row] = gx[row];
             r[row] = bi;
@@ -209,10 +275,7 @@ __global__ void __launch_bounds__(NThreads)
             T* p = vec + 2 * L;
             T* ap = vec + 3 * L;
             for (int row = tid; row < n; row += NThreads) {
-                T sum = 0;
-                const int32_t e1 = row_end(row);
-                for (int32_t e = row_begin(row); e < e1; ++e) sum += val(e) * xl[col(e)];
-                r[row] = r[row] - sum;
+                r[row] = r[row] - row_dot(row, xl);
                 p[row] = T(0);
                 z[row] = T(0);
                 ap[row] = T(0);
@@ -258,10 +321,7 @@ __global__ void __launch_bounds__(NThreads)
         // r = b - A x, ||r||, r_hat = r, p = p_hat = v = 0
         T rr = 0;
         for (int row = tid; row < n; row += NThreads) {
-            T sum = 0;
-            const int32_t e1 = row_end(row);
-            for (int32_t e = row_begin(row); e < e1; ++e) sum += val(e) * xl[col(e)];
-            const T ri = r[row] - sum;
+            const T ri = r[row] - row_dot(row, xl);
             r[row] = ri;
             rh[row] = ri;
             p[row] = T(0);
@@ -349,23 +409,30 @@ constexpr int kBatchMaxBins = GKOMI_BATCH_MAX_BINS;
 constexpr size_t kBatchQueueBytes = 64;  // per bin: queue + done counters
 
 size_t batch_lds_bytes(int vbytes, int64_t rows, int64_t nnz, bool mat_lds,
-                       int method = kBatchBicgstab)
+                       int method = kBatchBicgstab, int fmt = kFmtCsr)
 {
     size_t b = static_cast<size_t>(vbytes) * batch_vectors(method) * rows;
     if (mat_lds) {
         b += static_cast<size_t>(vbytes) * nnz;
-        b += 4 * (rows + 1);
-        b += 2 * nnz;
+        if (fmt == kFmtUneven || fmt == kFmtCsr) b += 4 * (rows + 1);
+        if (fmt != kFmtDense) b += 2 * nnz;
     }
     return (b + 15) / 16 * 16;
 }
 
-template <typename T, int NT, bool MatLds, bool Uniform, int Method>
+int fmt_of(const gkomi_batch_csr* a)
+{
+    if (a->format == GKOMI_BATCH_ELL) return kFmtEll;
+    if (a->format == GKOMI_BATCH_DENSE) return kFmtDense;
+    return a->row_offsets ? kFmtUneven : kFmtCsr;
+}
+
+template <typename T, int NT, bool MatLds, int Fmt, int Method>
 int launch_batch(hipStream_t stream, int grid, size_t lds, int64_t count, const int32_t* order,
                  const gkomi_batch_csr* a, const T* b, T* x, const gkomi_batch_settings* st,
                  int32_t* iters, T* res, unsigned* queue, int64_t lds_rows, int64_t lds_nnz)
 {
-    auto kern = batch_solver_kernel<T, NT, MatLds, Uniform, Method>;
+    auto kern = batch_solver_kernel<T, NT, MatLds, Fmt, Method>;
     if (lds > 64 * 1024) {
         GKOMI_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                             hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -373,14 +440,15 @@ int launch_batch(hipStream_t stream, int grid, size_t lds, int64_t count, const 
     }
     kern<<<grid, NT, lds, stream>>>(count, order, a->row_offsets, a->row_ptrs, a->col_idxs,
                                     static_cast<const T*>(a->values), a->uniform_rows,
-                                    a->uniform_nnz, b, x, st->max_iterations, st->tolerance_type,
+                                    a->uniform_nnz, a->ell_width, b, x, st->max_iterations,
+                                    st->tolerance_type,
                                     st->tolerance, st->preconditioner, iters, res, queue,
                                     lds_rows, lds_nnz);
     GKOMI_CHECK_LAUNCH("batch_solver_kernel");
     return GKOMI_SUCCESS;
 }
 
-template <typename T, bool MatLds, bool Uniform, int Method>
+template <typename T, bool MatLds, int Fmt, int Method>
 int launch_batch_nt(int nt, hipStream_t stream, int grid, size_t lds, int64_t count,
                     const int32_t* order, const gkomi_batch_csr* a, const T* b, T* x,
                     const gkomi_batch_settings* st, int32_t* iters, T* res, unsigned* queue,
@@ -388,16 +456,16 @@ int launch_batch_nt(int nt, hipStream_t stream, int grid, size_t lds, int64_t co
 {
     switch (nt) {
     case 64:
-        return launch_batch<T, 64, MatLds, Uniform, Method>(stream, grid, lds, count, order, a, b, x, st,
+        return launch_batch<T, 64, MatLds, Fmt, Method>(stream, grid, lds, count, order, a, b, x, st,
                                                     iters, res, queue, lds_rows, lds_nnz);
     case 128:
-        return launch_batch<T, 128, MatLds, Uniform, Method>(stream, grid, lds, count, order, a, b, x,
+        return launch_batch<T, 128, MatLds, Fmt, Method>(stream, grid, lds, count, order, a, b, x,
                                                      st, iters, res, queue, lds_rows, lds_nnz);
     case 256:
-        return launch_batch<T, 256, MatLds, Uniform, Method>(stream, grid, lds, count, order, a, b, x,
+        return launch_batch<T, 256, MatLds, Fmt, Method>(stream, grid, lds, count, order, a, b, x,
                                                      st, iters, res, queue, lds_rows, lds_nnz);
     case 512:
-        return launch_batch<T, 512, MatLds, Uniform, Method>(stream, grid, lds, count, order, a, b, x,
+        return launch_batch<T, 512, MatLds, Fmt, Method>(stream, grid, lds, count, order, a, b, x,
                                                      st, iters, res, queue, lds_rows, lds_nnz);
     default:
         return fail(GKOMI_INVALID_ARGUMENT, "batch: block size must be 64, 128, 256 or 512");
@@ -416,24 +484,80 @@ int device_lds_limit()
     return v;
 }
 
+// The matrix part of a descriptor (solvers and products).
+int check_batch_matrix(const gkomi_batch_csr* a)
+{
+    GKOMI_REQUIRE(a, GKOMI_INVALID_ARGUMENT, "batch: null descriptor");
+    GKOMI_REQUIRE(a->num_systems >= 0 && a->max_rows >= 0 && a->max_nnz >= 0 &&
+                      a->uniform_cols >= 0,
+                  GKOMI_INVALID_ARGUMENT, "batch: negative sizes");
+    GKOMI_REQUIRE(a->format == GKOMI_BATCH_CSR || a->format == GKOMI_BATCH_ELL ||
+                      a->format == GKOMI_BATCH_DENSE,
+                  GKOMI_INVALID_ARGUMENT, "batch: format must be CSR, ELL or DENSE");
+    const bool uniform = a->row_offsets == nullptr;
+    GKOMI_REQUIRE(uniform || a->format == GKOMI_BATCH_CSR, GKOMI_NOT_SUPPORTED,
+                  "batch: Ell and Dense batches have one common size (row_offsets NULL)");
+    GKOMI_REQUIRE(!uniform || (a->uniform_rows == a->max_rows && a->uniform_nnz == a->max_nnz),
+                  GKOMI_INVALID_ARGUMENT, "batch: uniform batch needs max_rows/max_nnz = n/nnz");
+    GKOMI_REQUIRE(uniform || a->uniform_cols == 0, GKOMI_INVALID_ARGUMENT,
+                  "batch: uneven systems are square (uniform_cols must be 0)");
+    GKOMI_REQUIRE(a->max_rows <= 65535, GKOMI_NOT_SUPPORTED,
+                  "batch: systems are limited to 65535 rows (16-bit local indices)");
+    GKOMI_REQUIRE(a->max_nnz < (int64_t{1} << 31), GKOMI_NOT_SUPPORTED,
+                  "batch: at most 2^31 - 1 stored values per system");
+    const int64_t cols = a->uniform_cols ? a->uniform_cols : a->uniform_rows;
+    if (a->format == GKOMI_BATCH_ELL) {
+        GKOMI_REQUIRE(a->ell_width >= 0 && a->uniform_nnz == a->uniform_rows * a->ell_width,
+                      GKOMI_INVALID_ARGUMENT,
+                      "batch Ell: uniform_nnz must be uniform_rows * ell_width");
+        GKOMI_REQUIRE(a->num_systems == 0 || a->uniform_nnz == 0 || (a->col_idxs && a->values),
+                      GKOMI_INVALID_ARGUMENT, "batch Ell: null column indices or values");
+    } else if (a->format == GKOMI_BATCH_DENSE) {
+        GKOMI_REQUIRE(a->uniform_nnz == a->uniform_rows * cols, GKOMI_INVALID_ARGUMENT,
+                      "batch Dense: uniform_nnz must be uniform_rows * uniform_cols");
+        GKOMI_REQUIRE(a->num_systems == 0 || a->uniform_nnz == 0 || a->values,
+                      GKOMI_INVALID_ARGUMENT, "batch Dense: null values");
+    } else {
+        GKOMI_REQUIRE(a->num_systems == 0 || (a->row_ptrs && a->col_idxs && a->values),
+                      GKOMI_INVALID_ARGUMENT, "batch: null matrix arrays");
+    }
+    return GKOMI_SUCCESS;
+}
+
 int check_batch(const gkomi_batch_csr* a, const gkomi_batch_settings* st)
 {
     GKOMI_REQUIRE(a && st, GKOMI_INVALID_ARGUMENT, "batch: null descriptor or settings");
-    GKOMI_REQUIRE(a->num_systems >= 0 && a->max_rows >= 0 && a->max_nnz >= 0,
-                  GKOMI_INVALID_ARGUMENT, "batch: negative sizes");
-    const bool uniform = a->row_offsets == nullptr;
-    GKOMI_REQUIRE(!uniform || (a->uniform_rows == a->max_rows && a->uniform_nnz == a->max_nnz),
-                  GKOMI_INVALID_ARGUMENT, "batch: uniform batch needs max_rows/max_nnz = n/nnz");
-    GKOMI_REQUIRE(a->max_rows <= 65535, GKOMI_NOT_SUPPORTED,
-                  "batch: systems are limited to 65535 rows (16-bit local indices)");
-    GKOMI_REQUIRE(a->num_systems == 0 || (a->row_ptrs && a->col_idxs && a->values),
-                  GKOMI_INVALID_ARGUMENT, "batch: null matrix arrays");
+    if (int rc = check_batch_matrix(a)) return rc;
+    GKOMI_REQUIRE(a->uniform_cols == 0 || a->uniform_cols == a->uniform_rows,
+                  GKOMI_DIMENSION_MISMATCH, "batch solver: the systems must be square");
     GKOMI_REQUIRE(st->tolerance_type == 0 || st->tolerance_type == 1, GKOMI_INVALID_ARGUMENT,
                   "batch: tolerance_type must be absolute (0) or relative (1)");
     GKOMI_REQUIRE(st->preconditioner == 0 || st->preconditioner == 1, GKOMI_NOT_SUPPORTED,
                   "batch: preconditioner must be identity (0) or scalar Jacobi (1)");
     GKOMI_REQUIRE(st->max_iterations >= 0, GKOMI_INVALID_ARGUMENT, "batch: negative max_iterations");
     return GKOMI_SUCCESS;
+}
+
+template <typename T, bool MatLds, int Method>
+int launch_batch_fmt(int fmt, int nt, hipStream_t s, int grid, size_t lds, int64_t cnt,
+                     const int32_t* order, const gkomi_batch_csr* a, const T* b, T* x,
+                     const gkomi_batch_settings* st, int32_t* iters, T* res, unsigned* q,
+                     int64_t rows, int64_t nnz)
+{
+    switch (fmt) {
+    case kFmtUneven:
+        return launch_batch_nt<T, MatLds, kFmtUneven, Method>(nt, s, grid, lds, cnt, order, a, b,
+                                                              x, st, iters, res, q, rows, nnz);
+    case kFmtCsr:
+        return launch_batch_nt<T, MatLds, kFmtCsr, Method>(nt, s, grid, lds, cnt, order, a, b, x,
+                                                           st, iters, res, q, rows, nnz);
+    case kFmtEll:
+        return launch_batch_nt<T, MatLds, kFmtEll, Method>(nt, s, grid, lds, cnt, order, a, b, x,
+                                                           st, iters, res, q, rows, nnz);
+    default:
+        return launch_batch_nt<T, MatLds, kFmtDense, Method>(nt, s, grid, lds, cnt, order, a, b,
+                                                             x, st, iters, res, q, rows, nnz);
+    }
 }
 
 template <typename T, int Method>
@@ -463,7 +587,7 @@ int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
                   GKOMI_INVALID_ARGUMENT, "batch: bad number of bins");
     GKOMI_REQUIRE(plan->bin_begin[0] == 0 && plan->bin_begin[plan->num_bins] == a->num_systems,
                   GKOMI_INVALID_ARGUMENT, "batch: bins must cover all systems");
-    const bool uniform = a->row_offsets == nullptr;
+    const int fmt = fmt_of(a);
     const int lds_limit = device_lds_limit();
     hipStream_t s = stream_of(ctx);
     unsigned* queues = static_cast<unsigned*>(ws);
@@ -477,10 +601,10 @@ int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
                       "batch: bin maxima exceed the batch maxima");
         int mat = plan->matrix_in_lds[bin];
         if (mat < 0) {
-            mat = batch_lds_bytes(sizeof(T), rows, nnz, true, Method) <=
+            mat = batch_lds_bytes(sizeof(T), rows, nnz, true, Method, fmt) <=
                   static_cast<size_t>(lds_limit);
         }
-        const size_t lds = batch_lds_bytes(sizeof(T), rows, nnz, mat != 0, Method);
+        const size_t lds = batch_lds_bytes(sizeof(T), rows, nnz, mat != 0, Method, fmt);
         GKOMI_REQUIRE(lds <= static_cast<size_t>(lds_limit), GKOMI_NOT_SUPPORTED,
                       "batch: the work vectors of the largest system exceed LDS");
         int bpc = plan->blocks_per_cu[bin];
@@ -494,23 +618,146 @@ int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
                       "batch: several bins need an order");
         unsigned* q = queues + bin * (kBatchQueueBytes / sizeof(unsigned));
         const int nt = plan->block_size[bin];
-        int rc;
-        if (uniform) {
-            rc = mat ? launch_batch_nt<T, true, true, Method>(nt, s, grid, lds, cnt, order, a, b, x, st,
-                                                     iters, res, q, rows, nnz)
-                     : launch_batch_nt<T, false, true, Method>(nt, s, grid, lds, cnt, order, a, b, x, st,
-                                                      iters, res, q, rows, nnz);
-        } else {
-            rc = mat ? launch_batch_nt<T, true, false, Method>(nt, s, grid, lds, cnt, order, a, b, x, st,
-                                                      iters, res, q, rows, nnz)
-                     : launch_batch_nt<T, false, false, Method>(nt, s, grid, lds, cnt, order, a, b, x,
-                                                       st, iters, res, q, rows, nnz);
-        }
+        const int rc = mat ? launch_batch_fmt<T, true, Method>(fmt, nt, s, grid, lds, cnt, order,
+                                                               a, b, x, st, iters, res, q, rows,
+                                                               nnz)
+                           : launch_batch_fmt<T, false, Method>(fmt, nt, s, grid, lds, cnt, order,
+                                                                a, b, x, st, iters, res, q, rows,
+                                                                nnz);
         if (rc) return rc;
     }
     return GKOMI_SUCCESS;
 }
 
+
+// ------------------------------------------------------ batched product ----
+
+// x_k = A_k b_k, or with Adv x_k = sum_j (alpha_k a_ij) b_j + beta_k x_k, per
+// system k (workgroups stride over the systems).  Csr and Ell run a thread
+// per row with the reference's sequential order over the row; Dense runs a
+// wave per row with coalesced loads of the row and a wave reduction (the
+// reference reduces a warp-sized group per row as well).
+template <typename T, int Fmt, bool Adv>
+__global__ void __launch_bounds__(kBlock)
+    batch_spmv_kernel(int64_t num_systems, const int64_t* __restrict__ row_off,
+                      const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                      const T* __restrict__ vals, int64_t uni_rows, int64_t uni_cols,
+                      int64_t uni_nnz, int32_t ell_width, const T* __restrict__ alpha,
+                      const T* __restrict__ b, const T* __restrict__ beta, T* __restrict__ x)
+{
+    for (int64_t sys = blockIdx.x; sys < num_systems; sys += gridDim.x) {
+        int64_t rows, cols, row0, col0, nnz0;
+        const int32_t* irp = rp;
+        const int32_t* ici = ci;
+        int32_t base = 0;
+        if constexpr (Fmt == kFmtUneven) {
+            row0 = row_off[sys];
+            rows = row_off[sys + 1] - row0;
+            cols = rows;
+            col0 = row0;
+            irp = rp + row0;
+            base = irp[0];
+            nnz0 = base;
+            ici = ci + nnz0;
+        } else {
+            rows = uni_rows;
+            cols = uni_cols;
+            row0 = sys * uni_rows;
+            col0 = sys * uni_cols;
+            nnz0 = sys * uni_nnz;
+        }
+        const T* v = vals + nnz0;
+        const T* in = b + col0;
+        T* out = x + row0;
+        const T a = Adv ? alpha[sys] : T(1);
+        const T bt = Adv ? beta[sys] : T(0);
+        if constexpr (Fmt == kFmtDense) {
+            const int lane = threadIdx.x % kWave;
+            for (int64_t row = threadIdx.x / kWave; row < rows; row += kWavesPerBlock) {
+                T part = 0;
+                for (int64_t j = lane; j < cols; j += kWave) {
+                    if constexpr (Adv) {
+                        part += a * v[row * cols + j] * in[j];
+                    } else {
+                        part += v[row * cols + j] * in[j];
+                    }
+                }
+                part = wave_sum(part);
+                if (lane == 0) out[row] = Adv ? part + bt * out[row] : part;
+            }
+        } else {
+            for (int64_t row = threadIdx.x; row < rows; row += kBlock) {
+                T sum = 0;
+                if constexpr (Fmt == kFmtEll) {
+                    for (int j = 0; j < ell_width; ++j) {
+                        const int64_t e = row + static_cast<int64_t>(j) * rows;
+                        const int32_t c = ici[e];
+                        if (c < 0) break;
+                        if constexpr (Adv) {
+                            sum += a * v[e] * in[c];
+                        } else {
+                            sum += v[e] * in[c];
+                        }
+                    }
+                } else {
+                    const int32_t e1 = irp[row + 1] - base;
+                    for (int32_t e = irp[row] - base; e < e1; ++e) {
+                        if constexpr (Adv) {
+                            sum += a * v[e] * in[ici[e]];
+                        } else {
+                            sum += v[e] * in[ici[e]];
+                        }
+                    }
+                }
+                out[row] = Adv ? sum + bt * out[row] : sum;
+            }
+        }
+    }
+}
+
+template <typename T, int Fmt>
+int batch_spmv_launch(hipStream_t s, const gkomi_batch_csr* a, const T* alpha, const T* b,
+                      const T* beta, T* x)
+{
+    const unsigned grid =
+        static_cast<unsigned>(std::min<int64_t>(a->num_systems, int64_t{1} << 20));
+    const int64_t cols = a->uniform_cols ? a->uniform_cols : a->uniform_rows;
+    const T* v = static_cast<const T*>(a->values);
+    if (alpha) {
+        batch_spmv_kernel<T, Fmt, true><<<grid, kBlock, 0, s>>>(
+            a->num_systems, a->row_offsets, a->row_ptrs, a->col_idxs, v, a->uniform_rows, cols,
+            a->uniform_nnz, a->ell_width, alpha, b, beta, x);
+    } else {
+        batch_spmv_kernel<T, Fmt, false><<<grid, kBlock, 0, s>>>(
+            a->num_systems, a->row_offsets, a->row_ptrs, a->col_idxs, v, a->uniform_rows, cols,
+            a->uniform_nnz, a->ell_width, alpha, b, beta, x);
+    }
+    GKOMI_CHECK_LAUNCH("batch_spmv_kernel");
+    return GKOMI_SUCCESS;
+}
+
+template <typename T>
+int batch_spmv_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a, const T* alpha, const T* b,
+                    const T* beta, T* x)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_batch_matrix(a)) return rc;
+    GKOMI_REQUIRE((alpha == nullptr) == (beta == nullptr), GKOMI_INVALID_ARGUMENT,
+                  "batch spmv: alpha and beta must both be given or both be NULL");
+    if (a->num_systems == 0 || a->max_rows == 0) return GKOMI_SUCCESS;
+    GKOMI_REQUIRE(b && x, GKOMI_INVALID_ARGUMENT, "batch spmv: null vector");
+    hipStream_t s = stream_of(ctx);
+    switch (fmt_of(a)) {
+    case kFmtUneven:
+        return batch_spmv_launch<T, kFmtUneven>(s, a, alpha, b, beta, x);
+    case kFmtCsr:
+        return batch_spmv_launch<T, kFmtCsr>(s, a, alpha, b, beta, x);
+    case kFmtEll:
+        return batch_spmv_launch<T, kFmtEll>(s, a, alpha, b, beta, x);
+    default:
+        return batch_spmv_launch<T, kFmtDense>(s, a, alpha, b, beta, x);
+    }
+}
 
 template <typename T, int Method>
 int batch_tune_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
@@ -550,7 +797,8 @@ int batch_tune_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
         float best = 1e30f;
         int best_nt = 256, best_mat = 0;
         const bool fits = batch_lds_bytes(sizeof(T), one.bin_max_rows[0], one.bin_max_nnz[0],
-                                          true, Method) <= static_cast<size_t>(lds_limit);
+                                          true, Method, fmt_of(a)) <=
+                          static_cast<size_t>(lds_limit);
         for (int nt : {64, 128, 256, 512}) {
             for (int mat = fits ? 1 : 0; mat >= 0; --mat) {
                 one.block_size[0] = nt;
@@ -641,6 +889,18 @@ int gkomi_batch_bicgstab_tune_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* a
 {
     return batch_tune_impl<float, kBatchBicgstab>(ctx, a, st, plan, b, x, xs, iters,
                                                            res, ws, wsb);
+}
+
+int gkomi_batch_spmv_f64(const gkomi_ctx* ctx, const gkomi_batch_csr* a, const double* alpha,
+                         const double* b, const double* beta, double* x)
+{
+    return batch_spmv_impl(ctx, a, alpha, b, beta, x);
+}
+
+int gkomi_batch_spmv_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* a, const float* alpha,
+                         const float* b, const float* beta, float* x)
+{
+    return batch_spmv_impl(ctx, a, alpha, b, beta, x);
 }
 
 #define GKOMI_BATCH_CG_ENTRY(T, S)                                                            \

@@ -656,14 +656,30 @@ int gkomi_bicgstab_finalize_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, f
 
 /* ----------------------------------------------------- batched BiCGSTAB */
 
-/* A batch of independent sparse systems (batch::matrix::Csr,
- * include/ginkgo/core/matrix/batch_csr.hpp:29-31, generalised to uneven
- * sizes).  Uneven: system k owns global rows [row_offsets[k], row_offsets[k+1])
- * of one block-diagonal CSR whose column indices are local to the system.
- * Uniform (row_offsets == NULL): every system has uniform_rows rows and the
- * shared pattern row_ptrs[uniform_rows + 1] / col_idxs[uniform_nnz]; system k's
- * values start at k * uniform_nnz.  Vectors are concatenated per system. */
+/* A batch of independent systems: batch::matrix::Csr, Ell and Dense
+ * (include/ginkgo/core/matrix/batch_{csr,ell,dense}.hpp), the Csr form
+ * generalised to uneven sizes.  Vectors are concatenated per system.
+ *  format GKOMI_BATCH_CSR:
+ *   uneven (row_offsets != NULL): system k owns global rows
+ *   [row_offsets[k], row_offsets[k+1]) of one block-diagonal CSR whose column
+ *   indices are local to the system (square systems);
+ *   uniform (row_offsets == NULL): every system has uniform_rows rows and the
+ *   shared pattern row_ptrs[uniform_rows + 1] / col_idxs[uniform_nnz];
+ *   system k's values start at k * uniform_nnz.
+ *  format GKOMI_BATCH_ELL (uniform only; batch_ell.hpp): col_idxs[uniform_rows *
+ *   ell_width], shared, column-major (slot j of row i at i + j * uniform_rows),
+ *   -1 after the last entry of a row; system k's values, in the same layout,
+ *   start at k * uniform_nnz with uniform_nnz = uniform_rows * ell_width.
+ *  format GKOMI_BATCH_DENSE (uniform only; batch_dense.hpp): system k's
+ *   values, row-major with stride uniform_cols, start at k * uniform_nnz with
+ *   uniform_nnz = uniform_rows * uniform_cols; row_ptrs and col_idxs unused.
+ * uniform_cols == 0 means uniform_cols = uniform_rows; the solvers need square
+ * systems, gkomi_batch_spmv_* takes rectangular ones.  A descriptor with
+ * format, ell_width and uniform_cols zero is the Csr form above. */
 #define GKOMI_BATCH_MAX_BINS 8
+#define GKOMI_BATCH_CSR 0
+#define GKOMI_BATCH_ELL 1
+#define GKOMI_BATCH_DENSE 2
 typedef struct {
     int64_t num_systems;
     int64_t max_rows;
@@ -674,7 +690,23 @@ typedef struct {
     const void* values;
     int64_t uniform_rows;
     int64_t uniform_nnz;
+    int32_t format;
+    int32_t ell_width;
+    int64_t uniform_cols;
 } gkomi_batch_csr;
+
+/* BatchLinOp::apply of batch::matrix::{Csr, Ell, Dense}
+ * (common/cuda_hip/matrix/batch_{csr,ell,dense}_kernels.hpp: simple_apply,
+ * advanced_apply): per system k, x_k = A_k b_k, or with alpha and beta (one
+ * value per system, device arrays) x_k = sum_j (alpha_k a_ij) b_j +
+ * beta_k x_k, the scaled products summed as the reference sums them; beta_k
+ * multiplies x_k also when it is zero.  alpha and beta are both NULL or both
+ * set.  b has uniform_cols (uneven: the system's) rows per system, x
+ * uniform_rows. */
+int gkomi_batch_spmv_f64(const gkomi_ctx* ctx, const gkomi_batch_csr* a, const double* alpha,
+                         const double* b, const double* beta, double* x);
+int gkomi_batch_spmv_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* a, const float* alpha,
+                         const float* b, const float* beta, float* x);
 
 /* kernels::batch_bicgstab::settings (core/solver/batch_bicgstab_kernels.hpp):
  * tolerance_type 0 = absolute, 1 = relative to ||b||

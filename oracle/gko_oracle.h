@@ -162,7 +162,16 @@ typedef struct {
                                const int32_t* row_ptrs, const int32_t* col_idxs,   \
                                const T* vals, const T* b, T* x, int use_jacobi,    \
                                int relative, double tol, int32_t max_iters,        \
-                               int use_omp, int32_t* iters, T* res_norms);
+                               int use_omp, int32_t* iters, T* res_norms);        \
+    void ora_batch_csr_spmv_##S(int64_t num, int64_t rows, int64_t cols,           \
+                                const int32_t* rp, const int32_t* ci, const T* vals, \
+                                const T* alpha, const T* b, const T* beta, T* x);  \
+    void ora_batch_ell_spmv_##S(int64_t num, int64_t rows, int64_t cols,           \
+                                int64_t width, const int32_t* ci, const T* vals,   \
+                                const T* alpha, const T* b, const T* beta, T* x);  \
+    void ora_batch_dense_spmv_##S(int64_t num, int64_t rows, int64_t cols,         \
+                                  const T* vals, const T* alpha, const T* b,       \
+                                  const T* beta, T* x);
 
 ORA_DECL_SPMV(double, d)
 ORA_DECL_SPMV(float, s)

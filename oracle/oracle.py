@@ -426,3 +426,51 @@ def cg_kernel(name, dtype):
 
 def omp_threads():
     return int(lib().ora_omp_threads())
+
+
+def _batch_scalars(alpha, beta, dt):
+    if (alpha is None) != (beta is None):
+        raise ValueError("alpha and beta must both be given or both be None")
+    if alpha is None:
+        return None, None
+    return _c(np.asarray(alpha), dt), _c(np.asarray(beta), dt)
+
+
+def batch_csr_spmv(num, rows, cols, row_ptrs, col_idxs, vals, b, x=None, alpha=None, beta=None):
+    """batch::matrix::Csr apply per system (common/cuda_hip/matrix/
+    batch_csr_kernels.hpp:37-86); alpha/beta one value per system."""
+    dt = vals.dtype
+    a, bt = _batch_scalars(alpha, beta, dt)
+    out = np.zeros(num * rows, dt) if x is None else _c(x, dt).copy()
+    getattr(lib(), f"ora_batch_csr_spmv_{_suffix(dt)}")(
+        ctypes.c_int64(num), ctypes.c_int64(rows), ctypes.c_int64(cols),
+        _p(_c(row_ptrs, np.int32)), _p(_c(col_idxs, np.int32)), _p(_c(vals, dt)),
+        _p(a) if a is not None else None, _p(_c(b, dt)), _p(bt) if bt is not None else None,
+        _p(out))
+    return out
+
+
+def batch_ell_spmv(num, rows, cols, width, col_idxs, vals, b, x=None, alpha=None, beta=None):
+    """batch::matrix::Ell apply per system (common/cuda_hip/matrix/
+    batch_ell_kernels.hpp:30-100): a row ends at its first -1."""
+    dt = vals.dtype
+    a, bt = _batch_scalars(alpha, beta, dt)
+    out = np.zeros(num * rows, dt) if x is None else _c(x, dt).copy()
+    getattr(lib(), f"ora_batch_ell_spmv_{_suffix(dt)}")(
+        ctypes.c_int64(num), ctypes.c_int64(rows), ctypes.c_int64(cols), ctypes.c_int64(width),
+        _p(_c(col_idxs, np.int32)), _p(_c(vals, dt)), _p(a) if a is not None else None,
+        _p(_c(b, dt)), _p(bt) if bt is not None else None, _p(out))
+    return out
+
+
+def batch_dense_spmv(num, rows, cols, vals, b, x=None, alpha=None, beta=None):
+    """batch::matrix::Dense apply per system (common/cuda_hip/matrix/
+    batch_dense_kernels.hpp:30-110), summed in column order."""
+    dt = vals.dtype
+    a, bt = _batch_scalars(alpha, beta, dt)
+    out = np.zeros(num * rows, dt) if x is None else _c(x, dt).copy()
+    getattr(lib(), f"ora_batch_dense_spmv_{_suffix(dt)}")(
+        ctypes.c_int64(num), ctypes.c_int64(rows), ctypes.c_int64(cols), _p(_c(vals, dt)),
+        _p(a) if a is not None else None, _p(_c(b, dt)), _p(bt) if bt is not None else None,
+        _p(out))
+    return out

@@ -1,0 +1,121 @@
+"""batch::matrix::Ell and Dense without a GPU: the oracle's batched products
+against the reference's fixtures (reference/test/matrix/batch_{ell,dense}_
+kernels.cpp) and the dense product, and the Ell layout built on the host
+executor against core/test/matrix/batch_ell.cpp."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as ora
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+KAT = GOLD["batch_formats"]
+
+
+def fixture_cases():
+    """The reference's fixture split into single right-hand sides: items of
+    2 x 3, b with three columns, x with three columns."""
+    a = KAT["apply"]
+    mtx = np.array(a["mtx"])
+    b = np.array(a["b"])
+    x = np.array(a["x"])
+    for col in range(3):
+        yield mtx, b[:, :, col], x[:, :, col]
+
+
+def ell_of(mtx):
+    """Shared column-major Ell pattern of the items' union of nonzeros."""
+    num, rows, cols = mtx.shape
+    pattern = [[j for j in range(cols) if np.any(mtx[:, i, j] != 0)] for i in range(rows)]
+    width = max(len(p) for p in pattern)
+    ci = -np.ones(rows * width, np.int32)
+    vals = np.zeros((num, rows * width))
+    for i, p in enumerate(pattern):
+        for s, j in enumerate(p):
+            ci[i + s * rows] = j
+            vals[:, i + s * rows] = mtx[:, i, j]
+    return width, ci, vals.reshape(-1)
+
+
+def csr_of(mtx):
+    num, rows, cols = mtx.shape
+    rp, ci = [0], []
+    for i in range(rows):
+        for j in range(cols):
+            if np.any(mtx[:, i, j] != 0):
+                ci.append(j)
+        rp.append(len(ci))
+    vals = np.concatenate([[mtx[k, i, j] for i in range(rows) for j in ci[rp[i]:rp[i + 1]]]
+                           for k in range(num)])
+    return np.array(rp, np.int32), np.array(ci, np.int32), vals
+
+
+@pytest.mark.parametrize("fmt", ["csr", "ell", "dense"])
+@pytest.mark.parametrize("advanced", [False, True])
+def test_oracle_batch_apply_matches_dense_product(fmt, advanced):
+    """Every value of the fixture is a small binary fraction, so the dense
+    product is exact and the restatement must reproduce it bit for bit."""
+    a = KAT["apply"]
+    alpha = np.array(a["alpha"]) if advanced else None
+    beta = np.array(a["beta"]) if advanced else None
+    for mtx, b, x0 in fixture_cases():
+        num, rows, cols = mtx.shape
+        expect = np.einsum("kij,kj->ki", mtx, b)
+        if advanced:
+            expect = alpha[:, None] * expect + beta[:, None] * x0
+        kw = dict(x=x0.reshape(-1), alpha=alpha, beta=beta) if advanced else {}
+        if fmt == "dense":
+            got = ora.batch_dense_spmv(num, rows, cols, mtx.reshape(-1), b.reshape(-1), **kw)
+        elif fmt == "ell":
+            width, ci, vals = ell_of(mtx)
+            got = ora.batch_ell_spmv(num, rows, cols, width, ci, vals, b.reshape(-1), **kw)
+        else:
+            rp, ci, vals = csr_of(mtx)
+            got = ora.batch_csr_spmv(num, rows, cols, rp, ci, vals, b.reshape(-1), **kw)
+        np.testing.assert_array_equal(got.reshape(num, rows), expect)
+
+
+def test_oracle_ell_row_ends_at_first_padding():
+    """A row ends at its first -1: a value stored after it is not read."""
+    ci = np.array([0, -1, 1, 1], np.int32)   # rows 2, width 2: row 0 = {0, 1}, row 1 = {}
+    vals = np.array([2.0, 7.0, 3.0, 5.0])
+    got = ora.batch_ell_spmv(1, 2, 2, 2, ci, vals, np.array([1.0, 10.0]))
+    np.testing.assert_array_equal(got, [32.0, 0.0])
+
+
+def test_oracle_advanced_beta_zero_propagates_nan():
+    """beta * x is formed also for beta = 0 (unlike the unbatched Csr)."""
+    got = ora.batch_dense_spmv(1, 1, 1, np.array([1.0]), np.array([1.0]),
+                               x=np.array([np.nan]), alpha=np.array([1.0]),
+                               beta=np.array([0.0]))
+    assert np.isnan(got[0])
+
+
+@pytest.mark.parametrize("case", KAT["ell_layout"], ids=["sparse", "full"])
+def test_ell_layout_from_dense_items(case):
+    """batch::matrix::Ell from per-item dense data on the host executor: the
+    reference's expected column-major values and shared column indices."""
+    import ginkgo_amd.gko as gko
+    host = gko.HostExecutor.create()
+    d = np.array(case["dense"])
+    m = gko.batch.matrix.Ell("f64").from_dense(host, d)
+    assert m.num_stored_elements_per_row == case["width"]
+    np.testing.assert_array_equal(m.values(), case["values"])
+    if "col_idxs" in case:
+        np.testing.assert_array_equal(m.col_idxs(), case["col_idxs"])
+    assert m.size == (2, 2, 3)
+
+
+def test_ell_and_dense_reject_bad_input():
+    import ginkgo_amd.gko as gko
+    host = gko.HostExecutor.create()
+    with pytest.raises(gko.DimensionMismatch):
+        gko.batch.matrix.Ell("f64").from_numpy(host, 2, 2, 2, 1, np.array([0, 2], np.int32),
+                                              np.ones(4))
+    with pytest.raises(gko.DimensionMismatch):
+        gko.batch.matrix.Ell("f64").from_numpy(host, 2, 2, 2, 1, np.array([0, 1], np.int32),
+                                              np.ones(3))
+    with pytest.raises(gko.DimensionMismatch):
+        gko.batch.matrix.Dense("f64").from_numpy(host, np.ones((2, 3)))
