@@ -162,6 +162,72 @@ def bench_poisson_cg(dev, nx=256):
     return out
 
 
+# Rows of the fork's published FP64 CG table on an NVIDIA A100
+# (examples/simple-solver/ginkgo_cg_a100.csv, line numbers in the last
+# field): matrix, rows, nnz, time (ms), iterations.
+A100_CG_ROWS = [
+    ("bodyy4", 17546, 121938, 25.4507, 226, 1),
+    ("wathen100", 30401, 471601, 34.549, 313, 3),
+    ("Dubcova3", 146689, 3636649, 41.2268, 214, 11),
+    ("af_shell3", 504855, 17588875, 1356.42, 3874, 18),
+    ("crankseg_1", 52804, 10614210, 701.9, 3319, 24),
+    ("thermal2", 1228045, 8580313, 1756.9, 5386, 30),
+    ("G2_circuit", 150102, 726674, 767.267, 7061, 33),
+    ("apache2", 715176, 4817870, 1199.52, 5416, 35),
+    ("parabolic_fem", 525825, 3674625, 472.329, 2421, 38),
+    ("ecology2", 999999, 4995991, 1753.48, 6945, 39),
+]
+
+
+def bench_cg_a100_sizes(dev):
+    """The fork's A100 CG harness (examples/simple-solver/simple-solver.cpp:
+    23-127: CSR FP64, CG without preconditioner, b = A 1, x0 = 0, one warm-up
+    then one timed solve, wall clock around apply + synchronize) at the sizes
+    and iteration counts of its published table.  The SuiteSparse matrices
+    are not available here: each row runs on a synthetic SPD matrix of the
+    same size (a 2-D box stencil with the table's entries per row), and CG
+    runs exactly the table's iteration count (Iteration criterion only), so
+    that both sides do the same number of iterations over the same number of
+    rows and entries; the sparsity pattern (and so the gather locality of
+    A x) differs from the original matrix."""
+    import numpy as np
+    import torch
+    import ginkgo_amd.gko as gko
+    from ginkgo_amd import generators as gen
+    exec_ = gko.Mi355xExecutor.create(0, torch.cuda.current_stream(dev).cuda_stream)
+    rows = []
+    ratios = []
+    for name, n_ref, nnz_ref, t_ref, iters, line in A100_CG_ROWS:
+        log(f"[bench] cg at the size of {name}")
+        rp, ci, v = gen.box_grid_spd(n_ref, nnz_ref)
+        n, nnz = len(rp) - 1, int(rp[-1])
+        A = gko.csr_from_host(exec_, rp, ci, v)
+        one = gko.matrix.Dense("f64").create(exec_, n, 1)
+        one.fill(1.0)
+        b = gko.matrix.Dense("f64").create(exec_, n, 1)
+        A.apply(one, b)
+        s = (gko.solver.Cg.build("f64")
+             .with_criteria(gko.stop.Iteration.build().with_max_iters(iters))
+             .on(exec_).generate(A))
+        x = gko.matrix.Dense("f64").create(exec_, n, 1)
+        x.fill(0.0)
+        s.apply(b, x)  # warm-up
+        x.fill(0.0)
+        t, _ = timed(lambda: s.apply(b, x))
+        assert int(s.last_iterations) == iters
+        ratio = t_ref / (t * 1e3)
+        ratios.append(ratio)
+        rows.append({"matrix": name, "csv_line": line, "n": n, "nnz": nnz, "iterations": iters,
+                     "time_ms": round(t * 1e3, 3), "a100_time_ms": t_ref,
+                     "us_per_iteration": round(t * 1e6 / iters, 2),
+                     "a100_us_per_iteration": round(t_ref * 1e3 / iters, 2),
+                     "a100_time_over_mi355x_time": round(ratio, 2)})
+        del A, s, b, x, one
+    torch.cuda.empty_cache()
+    return {"rows": rows,
+            "geomean_a100_time_over_mi355x_time": round(float(np.exp(np.mean(np.log(ratios)))), 2)}
+
+
 def bench_bicgstab(dev, nx=512):
     """One right-hand side of the nonsymmetric 512^2 convection-diffusion
     problem (upwind convection 10 in +x), BiCGSTAB + scalar Jacobi to 1e-12:
@@ -494,7 +560,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--only", action="append", default=None,
                     choices=["spmv", "poisson", "bicgstab", "cg_ir", "batch", "batch_cg",
-                             "adjoint", "adjoint_convection"],
+                             "adjoint", "adjoint_convection", "cg_a100_sizes"],
                     help="run only these secondary sections (repeatable)")
     ap.add_argument("--fd-entries", type=int, default=32,
                     help="conductivities checked by central differences in the adjoint section")
@@ -630,6 +696,7 @@ def main():
         del Dm
     for key, name, fn in (
             ("poisson", "cg_jacobi_poisson256sq", lambda: bench_poisson_cg(dev)),
+            ("cg_a100_sizes", "cg_at_fork_a100_table_sizes", lambda: bench_cg_a100_sizes(dev)),
             ("bicgstab", "bicgstab_jacobi_convection512sq", lambda: bench_bicgstab(dev)),
             ("cg_ir", "cg_vs_ir_96cubed", lambda: bench_cg_vs_ir(dev)[0]),
             ("batch", "batched_bicgstab_20000", lambda: bench_batch(dev)[0]),

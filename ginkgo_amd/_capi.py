@@ -180,7 +180,8 @@ class BatchPlan(ctypes.Structure):
                 ("bin_max_nnz", ctypes.c_int64 * BATCH_MAX_BINS),
                 ("block_size", ctypes.c_int32 * BATCH_MAX_BINS),
                 ("matrix_in_lds", ctypes.c_int32 * BATCH_MAX_BINS),
-                ("blocks_per_cu", ctypes.c_int32 * BATCH_MAX_BINS)]
+                ("blocks_per_cu", ctypes.c_int32 * BATCH_MAX_BINS),
+                ("subwave", ctypes.c_int32 * BATCH_MAX_BINS)]
 
 
 _P = ctypes.c_void_p

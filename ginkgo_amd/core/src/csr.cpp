@@ -140,7 +140,10 @@ void Csr<V, I>::set_strategy(std::shared_ptr<strategy_type> s)
 // Resolves the strategy and builds the load-balance partition on the
 // device (the role of make_srow, csr.hpp:1651-1655).  automatical (and
 // sparselib) decide from the longest and the mean row length: classical
-// (sub-wave per row) for short uniform rows, load_balance otherwise.
+// (sub-wave per row) when no row is longer than the reference's AMD limit
+// of 768 entries (automatical::process, csr.hpp:524-535) and the longest row
+// is within 8x the mean plus 64 (the lanes of a sub-wave then stay busy),
+// load_balance otherwise.
 // merge_path keeps a workspace for the workgroups' carries.
 template <typename V, typename I>
 void Csr<V, I>::make_srow()
@@ -158,7 +161,7 @@ void Csr<V, I>::make_srow()
                                 ? double(values_.get_size()) / double(this->get_size().rows)
                                 : 0.0;
         const auto mx = static_cast<double>(max_row_nnz());
-        name = (mx <= 64 && mx <= 4 * mean + 4) ? "classical" : "load_balance";
+        name = (mx <= 768 && mx <= 8 * mean + 64) ? "classical" : "load_balance";
     }
     effective_ = name;
     if (name == "load_balance") {

@@ -53,6 +53,30 @@ def stencil(dims, offsets_weights, diag, dtype=np.float64):
     return _csr_from_coo(n, rows, cols, vals)
 
 
+def box_grid_spd(n, nnz, shift=1e-3, dtype=np.float64):
+    """SPD matrix of about n rows and nnz entries: a 2-D grid of
+    ceil(sqrt(n)) columns and enough rows for n points, coupled to every
+    point of a (2a+1) x (2b+1) box around it with weight -1, the box chosen
+    so that its size is closest to nnz / n; the diagonal is the number of
+    neighbours plus `shift` (weakly dominant, so CG converges slowly)."""
+    k = max(1.0, nnz / n)
+    best = None
+    for a in range(0, 40):
+        for b in range(a, 40):
+            d = abs((2 * a + 1) * (2 * b + 1) - k)
+            if best is None or d < best[0]:
+                best = (d, a, b)
+    _, a, b = best
+    nx = int(np.ceil(np.sqrt(n)))
+    ny = int(np.ceil(n / nx))
+    offs = [((i, j), -1.0) for i in range(-a, a + 1) for j in range(-b, b + 1) if (i, j) != (0, 0)]
+    rp, ci, v = stencil((ny, nx), offs, 0.0, dtype)
+    counts = np.diff(rp.astype(np.int64)) - 1
+    diag_pos = np.repeat(np.arange(len(rp) - 1), np.diff(rp)) == ci
+    v[diag_pos] = (counts + shift).astype(dtype)
+    return rp, ci, v
+
+
 def poisson2d(nx, dtype=np.float64):
     offs = [((-1, 0), -1.0), ((1, 0), -1.0), ((0, -1), -1.0), ((0, 1), -1.0)]
     return stencil((nx, nx), offs, 4.0, dtype)

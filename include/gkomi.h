@@ -723,9 +723,10 @@ typedef struct {
 /* Launch plan: systems in `order` (device, largest first) split into bins of
  * similar size; each bin is one persistent launch with its own workgroup
  * size, LDS residency of the matrix (1 = LDS, 0 = global, -1 = LDS if it
- * fits) and workgroups per CU (0 = from the LDS footprint).  No reference
- * counterpart: the reference launches one uniform kernel per batch
- * (hip/solver/batch_bicgstab_kernels.hip.cpp). */
+ * fits), workgroups per CU (0 = from the LDS footprint) and, for the Csr
+ * formats, lanes per row of the product (1, 2, 4, 8 or 16; 0 = from the
+ * bin's entries per row).  No reference counterpart: the reference launches
+ * one uniform kernel per batch (hip/solver/batch_bicgstab_kernels.hip.cpp). */
 typedef struct {
     int32_t num_bins;
     int32_t reserved;
@@ -736,6 +737,7 @@ typedef struct {
     int32_t block_size[GKOMI_BATCH_MAX_BINS];
     int32_t matrix_in_lds[GKOMI_BATCH_MAX_BINS];
     int32_t blocks_per_cu[GKOMI_BATCH_MAX_BINS];
+    int32_t subwave[GKOMI_BATCH_MAX_BINS];
 } gkomi_batch_plan;
 
 /* Zero-filled scratch for the work queues; left zero-filled after every call. */

@@ -177,10 +177,11 @@ def test_apply_uneven_block_diagonal(gko, ex):
 @pytest.mark.parametrize("fmt", ["ell", "dense"])
 @pytest.mark.parametrize("jacobi", [False, True])
 def test_solvers_match_csr_batch_and_oracle(gko, ex, method, fmt, jacobi):
-    """The same SPD batch as Csr, Ell and Dense: Ell sums every row in the
-    Csr order, so its iteration counts equal the Csr batch's; Dense adds
-    the zeros of each row (0 * x_j = 0 leaves the sums unchanged).  The
-    oracle runs the reference's per-system loop on the Csr form."""
+    """The same SPD batch as Csr, Ell and Dense (Dense adds the zeros of
+    each row, 0 * x_j = 0): the three differ only in the rounding of the
+    row sums and dot products (the Csr product may split a row over lanes),
+    so iteration counts agree to one.  The oracle runs the reference's
+    per-system loop on the Csr form."""
     num, n = 300, 48
     dense, mask = random_items(num, n, n, 4, seed=11, spd=True)
     rng = np.random.default_rng(5)
@@ -201,11 +202,8 @@ def test_solvers_match_csr_batch_and_oracle(gko, ex, method, fmt, jacobi):
 
     xf, itf = solve(make(gko, ex, fmt, dense, mask))
     xc, itc = solve(make(gko, ex, "csr", dense, mask))
-    if fmt == "ell":
-        np.testing.assert_array_equal(itf, itc)
-        np.testing.assert_allclose(xf, xc, rtol=1e-12, atol=1e-14)
-    else:
-        assert np.max(np.abs(itf - itc)) <= 1
+    assert np.max(np.abs(itf - itc)) <= 1
+    np.testing.assert_allclose(xf, xc, rtol=1e-6, atol=1e-8)
     rp, ci = csr_pattern(mask)
     row_off = np.arange(num + 1, dtype=np.int64) * n
     glob_rp = np.concatenate([rp[:-1] + k * rp[-1] for k in range(num)] + [[num * rp[-1]]])
