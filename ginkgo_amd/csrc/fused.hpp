@@ -79,6 +79,45 @@ __device__ __forceinline__ void combine_partials(const double* __restrict__ part
     __syncthreads();
 }
 
+// combine_partials that also hands one int from thread 0 to the whole
+// workgroup through the same barrier: a kernel reads its stop flag in
+// thread 0 at its start and learns the workgroup's decision together with
+// the sums, instead of paying a barrier and a memory round trip for each.
+template <int Count, int NT = kBlock>
+__device__ __forceinline__ int combine_partials_bcast(const double* __restrict__ partials,
+                                                      int count, double (&out)[Count],
+                                                      int value0)
+{
+    constexpr int kW = NT / kWave;
+    __shared__ double lds[Count][kW];
+    __shared__ int bc;
+    double acc[Count];
+#pragma unroll
+    for (int c = 0; c < Count; ++c) acc[c] = 0;
+    for (int k = threadIdx.x; k < count; k += NT) {
+#pragma unroll
+        for (int c = 0; c < Count; ++c) acc[c] += partials[static_cast<int64_t>(k) * Count + c];
+    }
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int c = 0; c < Count; ++c) {
+        const double s = wave_sum(acc[c]);
+        if (lane == 0) lds[c][wave] = s;
+    }
+    if (threadIdx.x == 0) bc = value0;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < Count; ++c) {
+        double s = lds[c][0];
+        for (int w = 1; w < kW; ++w) s += lds[c][w];
+        out[c] = s;
+    }
+    const int v = bc;
+    __syncthreads();
+    return v;
+}
+
 // The elementwise kernels (update, split p) run 1024-thread workgroups on a
 // grid of at most kCgUpdGrid: few partials to combine in their consumers,
 // full occupancy for the streaming work.

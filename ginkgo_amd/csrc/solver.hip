@@ -315,13 +315,32 @@ __global__ void __launch_bounds__(kUpdBlock)
 {
     T alpha = T(0);
     bool do_update = false;
+    int64_t i0, i1;
+    block_range(ceildiv(n, kUpdBlock), &i0, &i1);
+    // the loads of this thread's first element, the stop flag and rho go
+    // out before the combine's barrier, so that their latencies overlap
+    // with the partials' (no kernel writes the flag while this one runs)
+    const int64_t ia = i0 * kUpdBlock + threadIdx.x;
+    const bool has_a = i0 < i1 && ia < n;
+    T ra = T(0), xa = T(0), pa = T(0), qa = T(0), da = T(1);
+    if (has_a) {
+        ra = r[ia];
+        if constexpr (Update) {
+            xa = x[ia];
+            pa = p[ia];
+            qa = q[ia];
+        }
+        if constexpr (Jacobi) da = inv_diag[ia];
+    }
     if constexpr (Update) {
-        if (flags->all_stopped) return;
+        const int stopped = flags->all_stopped;
+        const T rho = scalars[par];
         double pq[1];
         combine_partials<1, kUpdBlock>(w.part_dir, count_in, pq);
+        if (stopped) return;
         const T beta = static_cast<T>(pq[0]);
         do_update = beta != T(0);
-        alpha = do_update ? scalars[par] / beta : T(0);
+        alpha = do_update ? rho / beta : T(0);
         if (blockIdx.x == 0 && threadIdx.x == 0) scalars[2] = beta;
     } else {
         double s[2];
@@ -337,19 +356,18 @@ __global__ void __launch_bounds__(kUpdBlock)
             }
         }
     }
-    int64_t i0, i1;
-    block_range(ceildiv(n, kUpdBlock), &i0, &i1);
     T acc[2] = {T(0), T(0)};
     for (int64_t blk = i0; blk < i1; ++blk) {
         const int64_t i = blk * kUpdBlock + threadIdx.x;
         if (i >= n) continue;
-        T ri = r[i];
+        const bool first = blk == i0;
+        T ri = first ? ra : r[i];
         if (do_update) {
-            x[i] = fma(alpha, p[i], x[i]);
-            ri = fma(-alpha, q[i], ri);
+            x[i] = fma(alpha, first ? pa : p[i], first ? xa : x[i]);
+            ri = fma(-alpha, first ? qa : q[i], ri);
             r[i] = ri;
         }
-        const T zi = Jacobi ? ri * inv_diag[i] : ri;
+        const T zi = Jacobi ? ri * (first ? da : inv_diag[i]) : ri;
         z[i] = zi;
         acc[0] += ri * zi;
         acc[1] += ri * ri;
@@ -375,17 +393,16 @@ __global__ void __launch_bounds__(kBlock)
                               cg_work w, int count_in, int64_t iter, int par)
 {
     // workgroup 0 may set the flag during this kernel (when this iteration
-    // stops): read it once per workgroup so the early exit is uniform
-    __shared__ int stopped;
-    if (threadIdx.x == 0) stopped = flags->all_stopped;
-    __syncthreads();
-    if (stopped) return;
-    double s[2];
-    combine_partials<2>(w.part_upd, count_in, s);
-    const T rho = static_cast<T>(s[0]);
-    const T tau = static_cast<T>(sqrt(s[1]));
+    // stops): thread 0 reads it once and the combine's barrier hands it to
+    // the workgroup, so the early exit is uniform; the flag, rho_prev and the
+    // status are loaded together with the partials
+    const int flag0 = threadIdx.x == 0 ? flags->all_stopped : 0;
     const T prev = scalars[par ^ 1];
     uint8_t st = w.status[par ^ 1];
+    double s[2];
+    if (combine_partials_bcast<2>(w.part_upd, count_in, s, flag0)) return;
+    const T rho = static_cast<T>(s[0]);
+    const T tau = static_cast<T>(sqrt(s[1]));
     int all = 0, changed = 0;
     combined_check<T>(crit, iter, 1, &tau, &rho, orig, true, &st, &all, &changed);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -429,16 +446,15 @@ __global__ void __launch_bounds__(kUpdBlock)
                       gkomi_stop_flags* __restrict__ flags, cg_work w, int count_in,
                       int64_t iter, int par)
 {
-    __shared__ int stopped;
-    if (threadIdx.x == 0) stopped = flags->all_stopped;
-    __syncthreads();
-    if (stopped) return;
-    double s[2];
-    combine_partials<2, kUpdBlock>(w.part_upd, count_in, s);
-    const T rho = static_cast<T>(s[0]);
-    const T tau = static_cast<T>(sqrt(s[1]));
+    // as in the direction kernel: flag, rho_prev and status ride with the
+    // partials through one barrier
+    const int flag0 = threadIdx.x == 0 ? flags->all_stopped : 0;
     const T prev = scalars[par ^ 1];
     uint8_t st = w.status[par ^ 1];
+    double s[2];
+    if (combine_partials_bcast<2, kUpdBlock>(w.part_upd, count_in, s, flag0)) return;
+    const T rho = static_cast<T>(s[0]);
+    const T tau = static_cast<T>(sqrt(s[1]));
     int all = 0, changed = 0;
     combined_check<T>(crit, iter, 1, &tau, &rho, orig, true, &st, &all, &changed);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
