@@ -57,16 +57,6 @@ __host__ __device__ inline bcs_work bcs_work_layout(void* base)
 
 constexpr size_t kBcsWorkBytes = (7 * kCgMaxGrid + 2) * sizeof(double);
 
-// Reads a flag once per workgroup, so that an early exit is uniform even
-// when workgroup 0 of the same kernel sets the flag meanwhile.
-__device__ __forceinline__ int workgroup_flag(const int32_t* flag)
-{
-    __shared__ int value;
-    if (threadIdx.x == 0) value = *flag;
-    __syncthreads();
-    return value;
-}
-
 // ------------------------------------------------------------- start ----
 // r = b - A x, rr = r, p = v = 0 (reference/solver/bicgstab_kernels.cpp:
 // 26-58 and core/solver/bicgstab.cpp:139-146); partials of b.b and of
@@ -151,12 +141,30 @@ __global__ void __launch_bounds__(kUpdBlock)
                  uint8_t* __restrict__ status, gkomi_stop_flags* __restrict__ flags, bcs_work w,
                  int count_r, int64_t iter, int par)
 {
-    if (workgroup_flag(&flags->all_stopped)) return;
+    // the stop flag (thread 0; workgroup 0 writes it below), the scalars,
+    // the status and this thread's first element are loaded together with
+    // the partials, and the flag reaches the workgroup through the
+    // combine's barrier
+    int64_t i0, i1;
+    block_range(ceildiv(n, kUpdBlock), &i0, &i1);
+    const int64_t ia = i0 * kUpdBlock + threadIdx.x;
+    const bool has_a = i0 < i1 && ia < n;
+    T ra = T(0), pa = T(0), va = T(0), da = T(1);
+    if (has_a) {
+        ra = r[ia];
+        pa = p[ia];
+        va = v[ia];
+        if constexpr (Jacobi) da = inv_diag[ia];
+    }
+    const int flag0 = threadIdx.x == 0 ? flags->all_stopped : 0;
+    uint8_t st = w.status[par ^ 1];
+    const T prev = scalars[par ^ 1];
+    const T alpha = scalars[2];
+    const T omega = scalars[3];
     double s[2];
-    combine_partials<2, kUpdBlock>(w.part_r, count_r, s);
+    if (combine_partials_bcast<2, kUpdBlock>(w.part_r, count_r, s, flag0)) return;
     const T rho = static_cast<T>(s[0]);
     const T tau = static_cast<T>(sqrt(s[1]));
-    uint8_t st = w.status[par ^ 1];
     int all = 0, changed = 0;
     combined_check<T>(crit, iter, 1, &tau, &rho, orig, true, &st, &all, &changed);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -169,20 +177,16 @@ __global__ void __launch_bounds__(kUpdBlock)
         flags->one_changed = changed;
     }
     if (all) return;
-    const T prev = scalars[par ^ 1];
-    const T alpha = scalars[2];
-    const T omega = scalars[3];
     const bool go = prev * omega != T(0);
     const T tmp = go ? rho / prev * alpha / omega : T(0);
-    int64_t i0, i1;
-    block_range(ceildiv(n, kUpdBlock), &i0, &i1);
     for (int64_t blk = i0; blk < i1; ++blk) {
         const int64_t i = blk * kUpdBlock + threadIdx.x;
         if (i >= n) continue;
-        const T ri = r[i];
-        const T pi = go ? ri + tmp * (p[i] - omega * v[i]) : ri;
+        const bool first = blk == i0;
+        const T ri = first ? ra : r[i];
+        const T pi = go ? ri + tmp * ((first ? pa : p[i]) - omega * (first ? va : v[i])) : ri;
         p[i] = pi;
-        if (Jacobi) y[i] = pi * inv_diag[i];
+        if (Jacobi) y[i] = pi * (first ? da : inv_diag[i]);
     }
 }
 
@@ -219,23 +223,36 @@ __global__ void __launch_bounds__(kUpdBlock)
                  T* __restrict__ scalars, const gkomi_stop_flags* __restrict__ flags,
                  bcs_work w, int count_v, int par)
 {
-    if (flags->all_stopped) return;
-    double bsum[1];
-    combine_partials<1, kUpdBlock>(w.part_v, count_v, bsum);
-    const T beta = static_cast<T>(bsum[0]);
-    const bool go = beta != T(0);
-    const T alpha = go ? scalars[par] / beta : T(0);
-    if (blockIdx.x == 0 && threadIdx.x == 0) scalars[2] = alpha;
+    // no kernel writes the flag while this one runs: every thread reads it,
+    // together with rho and its first element, before the combine
     int64_t i0, i1;
     block_range(ceildiv(n, kUpdBlock), &i0, &i1);
+    const int64_t ia = i0 * kUpdBlock + threadIdx.x;
+    const bool has_a = i0 < i1 && ia < n;
+    T ra = T(0), va = T(0), da = T(1);
+    if (has_a) {
+        ra = r[ia];
+        va = v[ia];
+        if constexpr (Jacobi) da = inv_diag[ia];
+    }
+    const int stopped = flags->all_stopped;
+    const T rho = scalars[par];
+    double bsum[1];
+    combine_partials<1, kUpdBlock>(w.part_v, count_v, bsum);
+    if (stopped) return;
+    const T beta = static_cast<T>(bsum[0]);
+    const bool go = beta != T(0);
+    const T alpha = go ? rho / beta : T(0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) scalars[2] = alpha;
     T acc[1] = {T(0)};
     for (int64_t blk = i0; blk < i1; ++blk) {
         const int64_t i = blk * kUpdBlock + threadIdx.x;
         if (i >= n) continue;
-        const T ri = r[i];
-        const T si = go ? ri - alpha * v[i] : ri;
+        const bool first = blk == i0;
+        const T ri = first ? ra : r[i];
+        const T si = go ? ri - alpha * (first ? va : v[i]) : ri;
         s[i] = si;
-        if (Jacobi) z[i] = si * inv_diag[i];
+        if (Jacobi) z[i] = si * (first ? da : inv_diag[i]);
         acc[0] += si * si;
     }
     store_partials<T, 1, kUpdBlock>(acc, w.part_s);
@@ -253,12 +270,12 @@ __global__ void __launch_bounds__(kBlock)
                  uint8_t* __restrict__ status, gkomi_stop_flags* __restrict__ flags,
                  bcs_work w, int count_s, int64_t iter, int par)
 {
-    if (workgroup_flag(&flags->all_stopped)) return;
-    double ss[1];
-    combine_partials<1>(w.part_s, count_s, ss);
-    const T tau = static_cast<T>(sqrt(ss[0]));
+    const int flag0 = threadIdx.x == 0 ? flags->all_stopped : 0;
     const T rho = scalars[par];
     uint8_t st = w.status[par];
+    double ss[1];
+    if (combine_partials_bcast<1>(w.part_s, count_s, ss, flag0)) return;
+    const T tau = static_cast<T>(sqrt(ss[0]));
     int all = 0, changed = 0;
     combined_check<T>(crit, iter, 1, &tau, &rho, orig, false, &st, &all, &changed);
     if (all) {
@@ -299,15 +316,27 @@ __global__ void __launch_bounds__(kUpdBlock)
                       const gkomi_stop_flags* __restrict__ flags, bcs_work w, int count_t,
                       int64_t iter)
 {
-    __shared__ int mode;  // 0 update, 1 nothing to do, 2 finalize
-    if (threadIdx.x == 0) {
-        mode = flags->all_stopped ? (*w.half_stop == iter ? 2 : 1) : 0;
-    }
-    __syncthreads();
-    if (mode == 1) return;
-    const T alpha = scalars[2];
+    // mode 0 update, 1 nothing to do, 2 finalize: decided by thread 0 and
+    // handed over by the combine's barrier, with alpha and this thread's
+    // first element loaded before it
     int64_t i0, i1;
     block_range(ceildiv(n, kUpdBlock), &i0, &i1);
+    const int64_t ia = i0 * kUpdBlock + threadIdx.x;
+    const bool has_a = i0 < i1 && ia < n;
+    T ya = T(0), za = T(0), sa = T(0), ta = T(0), rra = T(0);
+    if (has_a) {
+        ya = y[ia];
+        za = z[ia];
+        sa = s[ia];
+        ta = t[ia];
+        rra = rr[ia];
+    }
+    int mode0 = 0;
+    if (threadIdx.x == 0) mode0 = flags->all_stopped ? (*w.half_stop == iter ? 2 : 1) : 0;
+    const T alpha = scalars[2];
+    double ts[2];
+    const int mode = combine_partials_bcast<2, kUpdBlock>(w.part_t, count_t, ts, mode0);
+    if (mode == 1) return;
     if (mode == 2) {
         for (int64_t blk = i0; blk < i1; ++blk) {
             const int64_t i = blk * kUpdBlock + threadIdx.x;
@@ -318,8 +347,6 @@ __global__ void __launch_bounds__(kUpdBlock)
         }
         return;
     }
-    double ts[2];
-    combine_partials<2, kUpdBlock>(w.part_t, count_t, ts);
     const T gamma = static_cast<T>(ts[0]);
     const T beta = static_cast<T>(ts[1]);
     const T omega = beta != T(0) ? gamma / beta : T(0);
@@ -328,10 +355,11 @@ __global__ void __launch_bounds__(kUpdBlock)
     for (int64_t blk = i0; blk < i1; ++blk) {
         const int64_t i = blk * kUpdBlock + threadIdx.x;
         if (i >= n) continue;
-        x[i] += alpha * y[i] + omega * z[i];
-        const T ri = s[i] - omega * t[i];
+        const bool first = blk == i0;
+        x[i] += alpha * (first ? ya : y[i]) + omega * (first ? za : z[i]);
+        const T ri = (first ? sa : s[i]) - omega * (first ? ta : t[i]);
         r[i] = ri;
-        acc[0] += rr[i] * ri;
+        acc[0] += (first ? rra : rr[i]) * ri;
         acc[1] += ri * ri;
     }
     store_partials<T, 2, kUpdBlock>(acc, w.part_r);
