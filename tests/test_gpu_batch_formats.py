@@ -294,3 +294,49 @@ def test_descriptor_errors(gko, ex):
         rect = gko.batch.matrix.Dense("f64").from_numpy(ex, np.ones((2, 2, 3)))
         gko.batch.solver.Cg.build("f64").on(ex).generate(rect)
     torch.cuda.synchronize()
+
+
+GOLD_BATCH = {"bicgstab": GOLD["batch_bicgstab"], "cg": GOLD["batch_cg"]}
+
+
+@pytest.mark.parametrize("fmt", ["ell", "dense"])
+@pytest.mark.parametrize("method", ["bicgstab", "cg"])
+def test_reference_stencil_kats_in_ell_and_dense(gko, ex, fmt, method):
+    """The reference's batch solver KATs (SolvesStencilSystem,
+    StencilSystemLoggerLogsIterations of reference/test/solver/
+    batch_{bicgstab,cg}_kernels.cpp) with the 3-point stencil stored as
+    batch::matrix::Ell and Dense (the reference's solver tests run over the
+    same matrix types)."""
+    k = GOLD_BATCH[method][0]
+    n, nb = k["num_rows"], k["num_batch_items"]
+    d = np.diag(np.full(n, k["diag"])) + np.diag(np.full(n - 1, k["off"]), 1) + \
+        np.diag(np.full(n - 1, k["off"]), -1)
+    dense = np.tile(d, (nb, 1, 1))
+    xs = np.full(n, k["exact_sol_fill"])
+    b = np.tile(d @ xs, (nb, 1))
+    A = make(gko, ex, fmt, dense, d != 0)
+    solver_t = gko.batch.solver.Cg if method == "cg" else gko.batch.solver.Bicgstab
+    s = (solver_t.build("f64").with_max_iterations(k["max_iterations"])
+         .with_tolerance(k["tolerance"])
+         .with_tolerance_type(gko.batch.stop.tolerance_type.relative).on(ex).generate(A))
+    x = mv(gko, ex, np.zeros((nb, n)))
+    s.apply(mv(gko, ex, b), x)
+    xh = x.to_numpy().reshape(nb, n)
+    relres_factor = float(k["check_relres_le"].split("*")[0])
+    for i in range(nb):
+        rel = np.linalg.norm(b[i] - d @ xh[i]) / np.linalg.norm(b[i])
+        assert rel <= relres_factor * k["tolerance"]
+        assert np.max(np.abs(xh[i] - xs)) <= 10 * k["tolerance"]
+    # the iteration KAT: tolerance 0 and the limit 5
+    kit = GOLD_BATCH[method][-1]
+    s5 = (solver_t.build("f64").with_max_iterations(kit["max_iterations"]).with_tolerance(0.0)
+          .with_tolerance_type(gko.batch.stop.tolerance_type.relative).on(ex).generate(A))
+    log = gko.batch.log.BatchConvergence("f64").create()
+    s5.add_logger(log)
+    x = mv(gko, ex, np.zeros((nb, n)))
+    s5.apply(mv(gko, ex, b), x)
+    its = np.array(log.num_iterations)
+    if "check_iters_eq" in kit:
+        assert np.all(its == kit["check_iters_eq"])
+    else:
+        assert np.all(its <= kit["check_iters_le"])
