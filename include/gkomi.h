@@ -742,7 +742,9 @@ typedef struct {
 
 /* Zero-filled scratch for the work queues; left zero-filled after every call. */
 size_t gkomi_batch_workspace_bytes(const gkomi_batch_csr* a);
-/* LDS bytes one workgroup needs for a system of `rows` rows and `nnz` entries. */
+/* LDS bytes one workgroup needs for a system of `rows` rows and `nnz` entries
+ * stored as Csr; for Ell and Dense (fewer index bytes) the value is an upper
+ * bound. */
 size_t gkomi_batch_lds_bytes(int32_t value_bytes, int64_t rows, int64_t nnz,
                              int32_t matrix_in_lds);
 /* batch_bicgstab::apply (core/solver/batch_bicgstab_kernels.hpp:177-198;
