@@ -60,8 +60,9 @@ def time_kernel_loop(fn, steps, warmup, stream):
 
 
 def stream_copy_gbps(dev):
-    """Device copy of 1 GiB by the library's 16-byte vector copy kernel
-    (read + write counted), and torch's copy for comparison."""
+    """Device copy of 1 GiB by the library's copy kernel (one 16-byte unit
+    per thread over a full grid, non-temporal loads and stores at this size;
+    read + write counted), and torch's copy for comparison."""
     import torch
     from ginkgo_amd import device as D
     n = 1 << 27  # 1 GiB of doubles

