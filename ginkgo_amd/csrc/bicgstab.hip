@@ -198,17 +198,21 @@ __global__ void __launch_bounds__(kBlock)
                  const T* __restrict__ y, const T* __restrict__ rr, T* __restrict__ v,
                  const gkomi_stop_flags* __restrict__ flags, bcs_work w)
 {
-    if (flags->all_stopped) return;
+    // the first tile's loads go out before the stop flag is known
     int64_t rb0, rb1;
     block_range(ceildiv(n, subwave_tile_rows<SW, RPG>), &rb0, &rb1);
+    subwave_pf<T, RPG> pf;
+    if (rb0 < rb1) subwave_prefetch<T, SW, RPG, Compact>(n, rb0, rp, ci, co, av, pf);
+    if (flags->all_stopped) return;
     T acc[1] = {T(0)};
-    for (int64_t blk = rb0; blk < rb1; ++blk) {
-        subwave_tile<T, SW, RPG, Compact>(
-            n, blk, rp, ci, co, av, [&](int32_t c) { return y[c]; },
-            [&](int64_t row, T sum) {
-                v[row] = sum;
-                acc[0] += rr[row] * sum;
-            });
+    auto gather = [&](int32_t c) { return y[c]; };
+    auto row_op = [&](int64_t row, T sum) {
+        v[row] = sum;
+        acc[0] += rr[row] * sum;
+    };
+    if (rb0 < rb1) subwave_finish<T, SW, RPG, Compact>(n, ci, co, av, pf, gather, row_op);
+    for (int64_t blk = rb0 + 1; blk < rb1; ++blk) {
+        subwave_tile<T, SW, RPG, Compact>(n, blk, rp, ci, co, av, gather, row_op);
     }
     store_partials<T, 1>(acc, w.part_v);
 }
@@ -270,6 +274,10 @@ __global__ void __launch_bounds__(kBlock)
                  uint8_t* __restrict__ status, gkomi_stop_flags* __restrict__ flags,
                  bcs_work w, int count_s, int64_t iter, int par)
 {
+    int64_t rb0, rb1;
+    block_range(ceildiv(n, subwave_tile_rows<SW, RPG>), &rb0, &rb1);
+    subwave_pf<T, RPG> pf;
+    if (rb0 < rb1) subwave_prefetch<T, SW, RPG, Compact>(n, rb0, rp, ci, co, av, pf);
     const int flag0 = threadIdx.x == 0 ? flags->all_stopped : 0;
     const T rho = scalars[par];
     uint8_t st = w.status[par];
@@ -290,17 +298,16 @@ __global__ void __launch_bounds__(kBlock)
         }
         return;
     }
-    int64_t rb0, rb1;
-    block_range(ceildiv(n, subwave_tile_rows<SW, RPG>), &rb0, &rb1);
     T acc[2] = {T(0), T(0)};
-    for (int64_t blk = rb0; blk < rb1; ++blk) {
-        subwave_tile<T, SW, RPG, Compact>(
-            n, blk, rp, ci, co, av, [&](int32_t c) { return z[c]; },
-            [&](int64_t row, T sum) {
-                t[row] = sum;
-                acc[0] += s[row] * sum;
-                acc[1] += sum * sum;
-            });
+    auto gather = [&](int32_t c) { return z[c]; };
+    auto row_op = [&](int64_t row, T sum) {
+        t[row] = sum;
+        acc[0] += s[row] * sum;
+        acc[1] += sum * sum;
+    };
+    if (rb0 < rb1) subwave_finish<T, SW, RPG, Compact>(n, ci, co, av, pf, gather, row_op);
+    for (int64_t blk = rb0 + 1; blk < rb1; ++blk) {
+        subwave_tile<T, SW, RPG, Compact>(n, blk, rp, ci, co, av, gather, row_op);
     }
     store_partials<T, 2>(acc, w.part_t);
 }

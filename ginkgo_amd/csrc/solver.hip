@@ -394,8 +394,13 @@ __global__ void __launch_bounds__(kBlock)
 {
     // workgroup 0 may set the flag during this kernel (when this iteration
     // stops): thread 0 reads it once and the combine's barrier hands it to
-    // the workgroup, so the early exit is uniform; the flag, rho_prev and the
-    // status are loaded together with the partials
+    // the workgroup, so the early exit is uniform; the flag, rho_prev, the
+    // status and the first tile's row pointers, columns and values are
+    // loaded together with the partials
+    int64_t rb0, rb1;
+    block_range(ceildiv(n, subwave_tile_rows<SW, RPG>), &rb0, &rb1);
+    subwave_pf<T, RPG> pf;
+    if (rb0 < rb1) subwave_prefetch<T, SW, RPG, Compact>(n, rb0, rp, ci, co, v, pf);
     const int flag0 = threadIdx.x == 0 ? flags->all_stopped : 0;
     const T prev = scalars[par ^ 1];
     uint8_t st = w.status[par ^ 1];
@@ -417,19 +422,17 @@ __global__ void __launch_bounds__(kBlock)
     if (all) return;
     const bool restart = prev == T(0);
     const T tmp = restart ? T(0) : rho / prev;
-    int64_t rb0, rb1;
-    block_range(ceildiv(n, subwave_tile_rows<SW, RPG>), &rb0, &rb1);
     T acc[1] = {T(0)};
-    for (int64_t blk = rb0; blk < rb1; ++blk) {
-        subwave_tile<T, SW, RPG, Compact>(
-            n, blk, rp, ci, co, v,
-            [&](int32_t c) { return restart ? z[c] : fma(tmp, p_in[c], z[c]); },
-            [&](int64_t row, T sum) {
-                const T pi = restart ? z[row] : fma(tmp, p_in[row], z[row]);
-                p_out[row] = pi;
-                q[row] = sum;
-                acc[0] += pi * sum;
-            });
+    auto gather = [&](int32_t c) { return restart ? z[c] : fma(tmp, p_in[c], z[c]); };
+    auto row_op = [&](int64_t row, T sum) {
+        const T pi = restart ? z[row] : fma(tmp, p_in[row], z[row]);
+        p_out[row] = pi;
+        q[row] = sum;
+        acc[0] += pi * sum;
+    };
+    if (rb0 < rb1) subwave_finish<T, SW, RPG, Compact>(n, ci, co, v, pf, gather, row_op);
+    for (int64_t blk = rb0 + 1; blk < rb1; ++blk) {
+        subwave_tile<T, SW, RPG, Compact>(n, blk, rp, ci, co, v, gather, row_op);
     }
     store_partials<T, 1>(acc, w.part_dir);
 }
@@ -486,17 +489,21 @@ __global__ void __launch_bounds__(kBlock)
                          const T* __restrict__ p, T* __restrict__ q,
                          const gkomi_stop_flags* __restrict__ flags, cg_work w)
 {
-    if (flags->all_stopped) return;
+    // the first tile's loads go out before the stop flag is known
     int64_t rb0, rb1;
     block_range(ceildiv(n, subwave_tile_rows<SW, RPG>), &rb0, &rb1);
+    subwave_pf<T, RPG> pf;
+    if (rb0 < rb1) subwave_prefetch<T, SW, RPG, Compact>(n, rb0, rp, ci, co, v, pf);
+    if (flags->all_stopped) return;
     T acc[1] = {T(0)};
-    for (int64_t blk = rb0; blk < rb1; ++blk) {
-        subwave_tile<T, SW, RPG, Compact>(
-            n, blk, rp, ci, co, v, [&](int32_t c) { return p[c]; },
-            [&](int64_t row, T sum) {
-                q[row] = sum;
-                acc[0] += p[row] * sum;
-            });
+    auto gather = [&](int32_t c) { return p[c]; };
+    auto row_op = [&](int64_t row, T sum) {
+        q[row] = sum;
+        acc[0] += p[row] * sum;
+    };
+    if (rb0 < rb1) subwave_finish<T, SW, RPG, Compact>(n, ci, co, v, pf, gather, row_op);
+    for (int64_t blk = rb0 + 1; blk < rb1; ++blk) {
+        subwave_tile<T, SW, RPG, Compact>(n, blk, rp, ci, co, v, gather, row_op);
     }
     store_partials<T, 1>(acc, w.part_dir);
 }

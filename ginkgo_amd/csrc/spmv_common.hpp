@@ -122,6 +122,124 @@ __device__ __forceinline__ void subwave_tile(int64_t n_rows, int64_t tile,
     }
 }
 
+// subwave_tile split in two, for kernels that have other loads to wait for
+// first (the partial sums of a fused solver): subwave_prefetch issues the
+// tile's row-pointer loads and the first step's column and value loads;
+// subwave_finish gathers, sums and calls row_op as subwave_tile does, in the
+// same order (a row's sum has the same bits either way).
+template <typename T, int RPG>
+struct subwave_pf {
+    int64_t row0;
+    int32_t pos[RPG];
+    int32_t end[RPG];
+    int32_t col[RPG];
+    T val[RPG];
+    bool act[RPG];
+    int32_t iters;
+};
+
+template <typename T, int SW, int RPG, bool Compact>
+__device__ __forceinline__ void subwave_prefetch(int64_t n_rows, int64_t tile,
+                                                 const int32_t* __restrict__ rp,
+                                                 const int32_t* __restrict__ ci,
+                                                 const int16_t* __restrict__ co,
+                                                 const T* __restrict__ v, subwave_pf<T, RPG>& pf)
+{
+    constexpr int kGroups = kWave / SW;
+    constexpr int kRowsPerWave = kGroups * RPG;
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    const int g = lane / SW;
+    const int l = lane % SW;
+    pf.row0 = tile * subwave_tile_rows<SW, RPG> + wave * kRowsPerWave + g;
+    pf.iters = 0;
+#pragma unroll
+    for (int k = 0; k < RPG; ++k) {
+        const int64_t row = pf.row0 + k * kGroups;
+        pf.pos[k] = rp[row < n_rows ? row : n_rows];
+        pf.end[k] = rp[row < n_rows ? row + 1 : n_rows];
+    }
+#pragma unroll
+    for (int k = 0; k < RPG; ++k) {
+        pf.pos[k] += l;
+        const int32_t len = pf.end[k] - pf.pos[k];
+        const int32_t it = len > 0 ? (len + SW - 1) / SW : 0;
+        pf.iters = it > pf.iters ? it : pf.iters;
+    }
+    // a lane with no entry in its rows loads nothing (the matrix may have no
+    // entries at all); the others read entry 0 for their inactive slots, as
+    // subwave_tile does
+    const bool any = pf.iters > 0;
+#pragma unroll
+    for (int k = 0; k < RPG; ++k) {
+        pf.act[k] = pf.pos[k] < pf.end[k];
+        const int32_t p = pf.act[k] ? pf.pos[k] : 0;
+        pf.col[k] = 0;
+        pf.val[k] = T(0);
+        if (any) {
+            if constexpr (Compact) {
+                const int32_t c = static_cast<int32_t>(pf.row0 + k * kGroups) + co[p];
+                pf.col[k] = pf.act[k] ? c : 0;
+            } else {
+                pf.col[k] = ci[p];
+            }
+            pf.val[k] = v[p];
+        }
+        pf.pos[k] = pf.act[k] ? pf.pos[k] + SW : pf.pos[k];
+    }
+}
+
+template <typename T, int SW, int RPG, bool Compact, typename Gather, typename RowOp>
+__device__ __forceinline__ void subwave_finish(int64_t n_rows, const int32_t* __restrict__ ci,
+                                               const int16_t* __restrict__ co,
+                                               const T* __restrict__ v, subwave_pf<T, RPG>& pf,
+                                               Gather gather, RowOp row_op)
+{
+    constexpr int kGroups = kWave / SW;
+    const int l = (threadIdx.x % kWave) % SW;
+    T sum[RPG];
+    {
+        T xv[RPG];
+#pragma unroll
+        for (int k = 0; k < RPG; ++k) xv[k] = gather(pf.col[k]);
+#pragma unroll
+        for (int k = 0; k < RPG; ++k) {
+            sum[k] = fma(pf.act[k] ? pf.val[k] : T(0), pf.act[k] ? xv[k] : T(0), T(0));
+        }
+    }
+    for (int32_t j = 1; j < pf.iters; ++j) {
+        int32_t col[RPG];
+        T val[RPG];
+        bool act[RPG];
+#pragma unroll
+        for (int k = 0; k < RPG; ++k) {
+            act[k] = pf.pos[k] < pf.end[k];
+            const int32_t p = act[k] ? pf.pos[k] : 0;
+            if constexpr (Compact) {
+                const int32_t c = static_cast<int32_t>(pf.row0 + k * kGroups) + co[p];
+                col[k] = act[k] ? c : 0;
+            } else {
+                col[k] = ci[p];
+            }
+            val[k] = v[p];
+            pf.pos[k] = act[k] ? pf.pos[k] + SW : pf.pos[k];
+        }
+        T xv[RPG];
+#pragma unroll
+        for (int k = 0; k < RPG; ++k) xv[k] = gather(col[k]);
+#pragma unroll
+        for (int k = 0; k < RPG; ++k) {
+            sum[k] = fma(act[k] ? val[k] : T(0), act[k] ? xv[k] : T(0), sum[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RPG; ++k) {
+        sum[k] = group_sum<SW>(sum[k]);
+        const int64_t row = pf.row0 + k * kGroups;
+        if (row < n_rows && l == 0) row_op(row, sum[k]);
+    }
+}
+
 // Dispatch over the classical shapes: sw in {1..64}, rpg in {1, 2, 4, 8};
 // CASE(S, R) is expanded for each and must end in `break` or `return`.
 #define GKOMI_SUBWAVE_SWITCH(sh, CASE)                                                  \
