@@ -82,7 +82,9 @@ __device__ __forceinline__ void bcg_store_partials(const double (&acc)[Count], i
 // Single-workgroup column sums of the partials P[g][C] (g < blocks), for the
 // one-workgroup kernels below (kCombineThreads threads): columns in chunks of
 // 64 lanes, the 16 waves take g = w, w + 16, ... and their sums are added in
-// wave order -- a fixed order, and 16 independent streams of coalesced loads.
+// wave order.  Each lane keeps four accumulators (g = w + 64 m + 16 u for
+// u = 0..3), added as (a0 + a1) + (a2 + a3) at the end: a fixed order, and
+// four independent loads in flight per lane instead of one dependent chain.
 constexpr int kCombineThreads = 1024;
 constexpr int kCombineWaves = kCombineThreads / kWave;
 
@@ -96,7 +98,14 @@ __device__ __forceinline__ void combine_columns(const double* __restrict__ P, in
         const int64_t c = c0 + lane;
         double acc = 0;
         if (c < C) {
-            for (int g = wave; g < blocks; g += kCombineWaves) acc += P[g * C + c];
+            double a[4] = {0, 0, 0, 0};
+            int g = wave;
+            for (; g + 3 * kCombineWaves < blocks; g += 4 * kCombineWaves) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) a[u] += P[(g + u * kCombineWaves) * C + c];
+            }
+            for (int u = 0; g < blocks; g += kCombineWaves, ++u) a[u] += P[g * C + c];
+            acc = (a[0] + a[1]) + (a[2] + a[3]);
         }
         lds[wave][lane] = acc;
         __syncthreads();
