@@ -360,8 +360,12 @@ constexpr int kBcgChunkNnz = 2048;
 // and, when they fit, the chunk's column indices and values -- in LDS with
 // coalesced loads, then each lane group takes R rows at a time: entries from
 // LDS, the gathers of all R rows in flight.  The only dependent global round
-// trip per row batch is the gather itself.
-template <typename T, int L>
+// trip per row batch is the gather itself.  With Off32 (n k sizeof(T) below
+// 4 GiB) the gathers use 32-bit byte offsets from the uniform base pointers,
+// which keeps the kernel at 4 waves per SIMD: with 64-bit offsets it needs
+// more than 128 VGPRs, and a 1024-workgroup grid no longer fits the chip at
+// once.
+template <typename T, int L, bool Off32>
 __global__ void __launch_bounds__(kBlock)
     bcg_direction_kernel(int64_t n, int64_t k, const int32_t* __restrict__ rp,
                          const int32_t* __restrict__ ci, const T* __restrict__ v,
@@ -404,6 +408,20 @@ __global__ void __launch_bounds__(kBlock)
             const T r1 = mode == 1 ? zv : fm;
             return mode == 0 ? pv : r1;
         };
+        // p_in and z of row `col`, column j
+        auto gather = [&](int32_t col, T& pv, T& zv) {
+            if constexpr (Off32) {
+                const uint32_t b = (static_cast<uint32_t>(col) * static_cast<uint32_t>(k) +
+                                    static_cast<uint32_t>(j)) *
+                                   static_cast<uint32_t>(sizeof(T));
+                pv = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p_in) + b);
+                zv = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(z) + b);
+            } else {
+                const int64_t o = static_cast<int64_t>(col) * k + j;
+                pv = p_in[o];
+                zv = z[o];
+            }
+        };
         double acc[1] = {0};
         for (int64_t cs = r0; cs < r1; cs += kBcgChunkRows) {
             const int rows = static_cast<int>(min(int64_t{kBcgChunkRows}, r1 - cs));
@@ -445,22 +463,18 @@ __global__ void __launch_bounds__(kBlock)
                         iters = it > iters ? it : iters;
                     }
                     for (int32_t t = 0; t < iters; ++t) {
-                        // three phases -- indices, all gathers, arithmetic --
-                        // so every gather of the batch is in flight at once
-                        int64_t off[R][kRowBatch];
-                        T vv[R][kRowBatch];
-                        bool act[R][kRowBatch];
+                        // three phases -- entry slots, all gathers, arithmetic --
+                        // so every gather of the batch is in flight at once.  A
+                        // slot past its row's end holds -1 and gathers entry 0
+                        // of the chunk (staged chunks with an active row are
+                        // non-empty).
+                        int32_t slot[R][kRowBatch];
 #pragma unroll
                         for (int r = 0; r < R; ++r) {
 #pragma unroll
                             for (int u = 0; u < kRowBatch; ++u) {
                                 const int32_t p = beg[r] + t * kRowBatch + u;
-                                act[r][u] = p < end[r];
-                                // an inactive slot reads entry 0 of the chunk
-                                // (staged chunks with an active row are non-empty)
-                                const int32_t pp = act[r][u] ? p : 0;
-                                vv[r][u] = lv[pp];
-                                off[r][u] = static_cast<int64_t>(lci[pp]) * k + j;
+                                slot[r][u] = p < end[r] ? p : -1;
                             }
                         }
                         T pv[R][kRowBatch];
@@ -469,19 +483,19 @@ __global__ void __launch_bounds__(kBlock)
                         for (int r = 0; r < R; ++r) {
 #pragma unroll
                             for (int u = 0; u < kRowBatch; ++u) {
-                                pv[r][u] = p_in[off[r][u]];
-                                zv[r][u] = z[off[r][u]];
+                                gather(lci[slot[r][u] < 0 ? 0 : slot[r][u]], pv[r][u], zv[r][u]);
                             }
                         }
 #pragma unroll
                         for (int r = 0; r < R; ++r) {
 #pragma unroll
                             for (int u = 0; u < kRowBatch; ++u) {
+                                const bool act = slot[r][u] >= 0;
+                                const T vv = lv[act ? slot[r][u] : 0];
                                 const T fm = fma(f, pv[r][u], zv[r][u]);
                                 const T r1 = mode == 1 ? zv[r][u] : fm;
                                 const T d = mode == 0 ? pv[r][u] : r1;
-                                sum[r] = fma(act[r][u] ? vv[r][u] : T(0), act[r][u] ? d : T(0),
-                                             sum[r]);
+                                sum[r] = fma(act ? vv : T(0), act ? d : T(0), sum[r]);
                             }
                         }
                     }
@@ -624,6 +638,26 @@ int bcg_grid(int64_t n, int lanes)
     return static_cast<int>(std::max<int64_t>(g, 1));
 }
 
+// Grid of the direction kernel: at most the workgroups that are resident on
+// the chip at once.  The kernel needs more registers than the others (about
+// 3 workgroups per CU in FP64), and a larger grid would run a partial second
+// wave of workgroups after the first.
+template <typename T, int L, bool Off32>
+int bcg_direction_grid(int grid)
+{
+    static const int per_cu = [] {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nb, bcg_direction_kernel<T, L, Off32>, kBlock, 0) != hipSuccess ||
+            nb < 1) {
+            (void)hipGetLastError();
+            nb = 1;
+        }
+        return nb;
+    }();
+    return static_cast<int>(std::min<int64_t>(grid, static_cast<int64_t>(num_cus()) * per_cu));
+}
+
 #define GKOMI_BCG_LANES(l, CASE)                                                       \
     switch (l) {                                                                       \
         case 1: CASE(1) case 2: CASE(2) case 4: CASE(4) case 8: CASE(8) case 16: CASE(16) \
@@ -704,6 +738,9 @@ int bcg_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_crite
     T* r = static_cast<T*>(st->r);
     T* z = static_cast<T*>(st->z);
     const T* inv = static_cast<const T*>(st->inv_diag);
+    // byte offsets of p and z entries fit 32 bits
+    const bool off32 = static_cast<uint64_t>(n) * static_cast<uint64_t>(k) * sizeof(T) <
+                       (uint64_t{1} << 32);
     for (int32_t it = 0; it < count; ++it) {
         T* pin = static_cast<T*>(st->p_a);
         T* pout = static_cast<T*>(st->p_b);
@@ -712,14 +749,23 @@ int bcg_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_crite
                                                  st->status, st->flags, st->issued);
         GKOMI_CHECK_LAUNCH("bcg_check_kernel");
 #define GKOMI_BCG_DIR(L)                                                                   \
-    bcg_direction_kernel<T, L><<<grid, kBlock, 0, s>>>(n, k, a->row_ptrs, a->col_idxs, v, z, \
-                                                       pin, pout, static_cast<T*>(st->q),  \
-                                                       sc, st->status, st->flags, w);      \
+    if (off32) {                                                                           \
+        dgrid = bcg_direction_grid<T, L, true>(grid);                                      \
+        bcg_direction_kernel<T, L, true><<<dgrid, kBlock, 0, s>>>(                         \
+            n, k, a->row_ptrs, a->col_idxs, v, z, pin, pout, static_cast<T*>(st->q), sc,   \
+            st->status, st->flags, w);                                                     \
+    } else {                                                                               \
+        dgrid = bcg_direction_grid<T, L, false>(grid);                                     \
+        bcg_direction_kernel<T, L, false><<<dgrid, kBlock, 0, s>>>(                        \
+            n, k, a->row_ptrs, a->col_idxs, v, z, pin, pout, static_cast<T*>(st->q), sc,   \
+            st->status, st->flags, w);                                                     \
+    }                                                                                      \
     break;
+        int dgrid = grid;
         GKOMI_BCG_LANES(lanes, GKOMI_BCG_DIR)
 #undef GKOMI_BCG_DIR
         GKOMI_CHECK_LAUNCH("bcg_direction_kernel");
-        bcg_beta_kernel<T><<<1, kCombineThreads, 0, s>>>(k, grid, w, sc, st->flags);
+        bcg_beta_kernel<T><<<1, kCombineThreads, 0, s>>>(k, dgrid, w, sc, st->flags);
         GKOMI_CHECK_LAUNCH("bcg_beta_kernel");
 #define GKOMI_BCG_UPD(L)                                                                   \
     if (inv) {                                                                             \

@@ -244,6 +244,49 @@ def test_cg_many_rhs_block_path(gko, exec_, k, jacobi, dt):
     assert np.all(res <= tol * (2 if dt == "f64" else 20))
 
 
+def test_cg_many_rhs_block_path_4gib_vectors(gko, exec_):
+    """k-RHS CG whose n x k blocks span 2^32 bytes (2^24 rows, 64 FP32
+    columns), so the direction kernel indexes them with 64-bit offsets.
+    Column j's right-hand side is 2^(j-32) u: every CG operation commutes
+    with a power-of-two scale, so x[:, j] must be exactly 2^(j-32) x[:, 32],
+    which fails if any column reads another's entries.  Column 32 matches
+    the oracle's CG on u (cg.cpp) to FP32 rounding."""
+    n, k, iters = 1 << 24, 64, 3
+    rp = np.zeros(n + 1, np.int32)
+    lens = np.full(n, 3, np.int32)
+    lens[0] = lens[-1] = 2
+    rp[1:] = np.cumsum(lens)
+    rows = np.repeat(np.arange(n, dtype=np.int64), lens)
+    ci = (rows + (np.arange(rp[-1]) - rp[rows]) - (rows > 0)).astype(np.int32)
+    v = np.where(ci == rows, np.float32(3.0), np.float32(-1.0)).astype(np.float32)
+    del rows
+    u = np.random.default_rng(17).uniform(0.5, 1.5, n).astype(np.float32)
+    scales = np.ldexp(np.float32(1), np.arange(k) - 32).astype(np.float32)
+    b_h = u[:, None] * scales[None, :]
+    spec = [{"kind": "iteration", "max_iters": iters},
+            {"kind": "residual_norm", "reduction_factor": 1e-6}]
+    A = gko.csr_from_host(exec_, rp, ci, v)
+    s = gko.solver.Cg.build("f32").with_criteria(*crits(gko, spec, "f32")).on(exec_).generate(A)
+    b = gko.matrix.Dense("f32").from_numpy(exec_, b_h)
+    del b_h
+    x = gko.matrix.Dense("f32").create(exec_, n, k)
+    x.fill(0.0)
+    s.apply(b, x)
+    assert s.last_used_fused_path
+    assert s.last_iterations == iters
+    del b
+    xh = x.to_numpy()
+    del x
+    ref = xh[:, 32].copy()
+    for j in range(k):
+        assert np.array_equal(xh[:, j], ref * scales[j]), j
+    del xh
+    xo, _, io = ora.cg_solve(rp, ci, v.astype(np.float64), u.astype(np.float64)[:, None],
+                             np.zeros((n, 1)), spec)
+    assert io.iterations == iters
+    np.testing.assert_allclose(ref, xo[:, 0], rtol=1e-4, atol=1e-5 * np.abs(xo).max())
+
+
 def test_cg_block_per_column_stopping(gko, exec_):
     """Columns that stop at different iterations keep their p, x and r
     (reference/solver/cg_kernels.cpp:55-95): a zero column stops at
