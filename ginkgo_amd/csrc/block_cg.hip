@@ -240,8 +240,9 @@ __device__ __forceinline__ void rows_product(const int32_t* __restrict__ ci,
     }
 }
 
-// rows per lane group in flight in the direction kernel
-constexpr int kBcgRows = 2;
+// rows per lane group in flight in the direction kernel (one keeps it at
+// about 90 VGPRs in FP64; two needed about 150)
+constexpr int kBcgRows = 1;
 
 // ---------------------------------------------------------------- start ----
 // r = b - A x; p_a = p_b = q = 0; partials of b_j.b_j and r_j.r_j.
