@@ -45,13 +45,17 @@ def loss_fn(x, xt):
     return 0.5 * ((x - xt) ** 2).sum()
 
 
-@pytest.mark.parametrize("method,convection", [("cg", 0.0), ("bicgstab", 10.0)])
-def test_gradients_match_finite_differences_and_dense(dev, method, convection):
+@pytest.mark.parametrize("method,convection,jacobi", [("cg", 0.0, False), ("bicgstab", 10.0, False),
+                                                     ("cg", 0.0, True), ("bicgstab", 10.0, True)])
+def test_gradients_match_finite_differences_and_dense(dev, method, convection, jacobi):
+    """With jacobi the forward and the adjoint solve are both preconditioned
+    (scalar Jacobi of A and of its transpose); the gradient is the same."""
     from ginkgo_amd.autograd import sparse_solve
     rp, ci, theta0, values_of, b, xt, n = problem(12, convection, 3, dev)
     tol = 1e-14
     theta = theta0.clone().requires_grad_(True)
-    x = sparse_solve(values_of(theta), b, rp, ci, method=method, tol=tol, max_iters=5000)
+    x = sparse_solve(values_of(theta), b, rp, ci, method=method, tol=tol, max_iters=5000,
+                     jacobi=jacobi)
     loss = loss_fn(x, xt)
     loss.backward()
     g = theta.grad.detach().cpu().numpy()
@@ -82,9 +86,9 @@ def test_gradients_match_finite_differences_and_dense(dev, method, convection):
         tp[i] += h
         tm[i] -= h
         lp = loss_fn(sparse_solve(values_of(tp), b, rp, ci, method=method, tol=tol,
-                                  max_iters=5000), xt).item()
+                                  max_iters=5000, jacobi=jacobi), xt).item()
         lm = loss_fn(sparse_solve(values_of(tm), b, rp, ci, method=method, tol=tol,
-                                  max_iters=5000), xt).item()
+                                  max_iters=5000, jacobi=jacobi), xt).item()
         fd = (lp - lm) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(abs(fd), 1e-3 * np.max(np.abs(gd))), (i, fd, g[i])
 
