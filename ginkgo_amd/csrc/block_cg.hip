@@ -240,8 +240,7 @@ __device__ __forceinline__ void rows_product(const int32_t* __restrict__ ci,
     }
 }
 
-// rows per lane group in flight in the direction kernel (one keeps it at
-// about 90 VGPRs in FP64; two needed about 150)
+// rows per lane group in flight in the product kernel
 constexpr int kBcgRows = 1;
 
 // ---------------------------------------------------------------- start ----
@@ -357,30 +356,26 @@ __global__ void __launch_bounds__(kCombineThreads)
 constexpr int kBcgChunkRows = 256;
 constexpr int kBcgChunkNnz = 2048;
 
-// The workgroup stages a chunk of up to kBcgChunkRows rows -- row pointers
-// and, when they fit, the chunk's column indices and values -- in LDS with
-// coalesced loads, then each lane group takes R rows at a time: entries from
-// LDS, the gathers of all R rows in flight.  The only dependent global round
-// trip per row batch is the gather itself.  With Off32 (n k sizeof(T) below
-// 4 GiB) the gathers use 32-bit byte offsets from the uniform base pointers,
-// which keeps the kernel at 4 waves per SIMD: with 64-bit offsets it needs
-// more than 128 VGPRs, and a 1024-workgroup grid no longer fits the chip at
-// once.
+// q = A p and the partials of p.q, p being the new directions
+// (bcg_direction_kernel has written them).  The workgroup stages a chunk of
+// up to kBcgChunkRows rows -- row pointers and, when they fit, the chunk's
+// column indices and values -- in LDS with coalesced loads, then each lane
+// group takes R rows at a time: entries from LDS, the gathers of all R rows
+// in flight.  The only dependent global round trip per row batch is the
+// gather itself.  With Off32 (n k sizeof(T) below 4 GiB) the gathers use
+// 32-bit byte offsets from the uniform base pointer.
 template <typename T, int L, bool Off32>
 __global__ void __launch_bounds__(kBlock)
-    bcg_direction_kernel(int64_t n, int64_t k, const int32_t* __restrict__ rp,
-                         const int32_t* __restrict__ ci, const T* __restrict__ v,
-                         const T* __restrict__ z, const T* __restrict__ p_in,
-                         T* __restrict__ p_out, T* __restrict__ q,
-                         const T* __restrict__ scalars, const uint8_t* __restrict__ status,
-                         const gkomi_stop_flags* __restrict__ flags, bcg_work w)
+    bcg_product_kernel(int64_t n, int64_t k, const int32_t* __restrict__ rp,
+                       const int32_t* __restrict__ ci, const T* __restrict__ v,
+                       const T* __restrict__ p, T* __restrict__ q,
+                       const gkomi_stop_flags* __restrict__ flags, bcg_work w)
 {
     if (flags->all_stopped) return;
     __shared__ double lds[1][kBlock];
     __shared__ int32_t lrp[kBcgChunkRows + 1];
     __shared__ int32_t lci[kBcgChunkNnz];
     __shared__ T lv[kBcgChunkNnz];
-    const bcg_scalars<T> sc = bcg_scalar_layout<T>(const_cast<T*>(scalars), k);
     constexpr int R = kBcgRows;
     constexpr int kGroups = kBlock / L;
     const int lane = threadIdx.x % L;
@@ -391,36 +386,16 @@ __global__ void __launch_bounds__(kBlock)
         const bool col_ok = c0 + lane < k;
         // lanes past the last column follow column c0 and store nothing
         const int64_t j = col_ok ? c0 + lane : c0;
-        // per column: keep p (stopped), p = z (restart), or p = z + f p
-        int mode = 0;  // 0 keep, 1 z, 2 z + f p
-        T f = 0;
-        if (!has_stopped(status[j])) {
-            const T prev = sc.prev_used[j];
-            mode = prev == T(0) ? 1 : 2;
-            f = mode == 2 ? sc.rho[j] / prev : T(0);
-        }
-        // both loads unconditionally, then a select: no branch between the
-        // gathers, so all of them stay in flight together
-        auto dir = [&](int64_t row) {
-            const int64_t o = row * k + j;
-            const T pv = p_in[o];
-            const T zv = z[o];
-            const T fm = fma(f, pv, zv);
-            const T r1 = mode == 1 ? zv : fm;
-            return mode == 0 ? pv : r1;
-        };
-        // p_in and z of row `col`, column j
-        auto gather = [&](int32_t col, T& pv, T& zv) {
+        auto at = [&](int64_t row) { return p[row * k + j]; };
+        // p of row `col`, column j
+        auto gather = [&](int32_t col) {
             if constexpr (Off32) {
                 const uint32_t b = (static_cast<uint32_t>(col) * static_cast<uint32_t>(k) +
                                     static_cast<uint32_t>(j)) *
                                    static_cast<uint32_t>(sizeof(T));
-                pv = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p_in) + b);
-                zv = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(z) + b);
+                return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + b);
             } else {
-                const int64_t o = static_cast<int64_t>(col) * k + j;
-                pv = p_in[o];
-                zv = z[o];
+                return p[static_cast<int64_t>(col) * k + j];
             }
         };
         double acc[1] = {0};
@@ -440,14 +415,10 @@ __global__ void __launch_bounds__(kBlock)
             }
             __syncthreads();
             for (int rr = g * R; rr < rows; rr += kGroups * R) {
-                // the rows' own p_in / z, loaded ahead of the gathers
-                T own_p[R], own_z[R];
+                // the rows' own p, loaded ahead of the gathers
+                T own_p[R];
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int64_t o = (cs + (rr + r < rows ? rr + r : 0)) * k + j;
-                    own_p[r] = p_in[o];
-                    own_z[r] = z[o];
-                }
+                for (int r = 0; r < R; ++r) own_p[r] = at(cs + (rr + r < rows ? rr + r : 0));
                 T sum[R];
                 if (staged) {
                     // entries from LDS, kRowBatch at a time for all R rows
@@ -475,19 +446,16 @@ __global__ void __launch_bounds__(kBlock)
                         for (int r = 0; r < R; ++r) {
 #pragma unroll
                             for (int u = 0; u < kRowBatch; ++u) {
-                                const int32_t p = beg[r] + t * kRowBatch + u;
-                                slot[r][u] = p < end[r] ? p : -1;
+                                const int32_t e = beg[r] + t * kRowBatch + u;
+                                slot[r][u] = e < end[r] ? e : -1;
                             }
                         }
                         T pv[R][kRowBatch];
-                        T zv[R][kRowBatch];
 #pragma unroll
                         for (int r = 0; r < R; ++r) {
 #pragma unroll
                             for (int u = 0; u < kRowBatch; ++u) {
-                                pv[r][u] = T(0);
-                                zv[r][u] = T(0);
-                                if (slot[r][u] >= 0) gather(lci[slot[r][u]], pv[r][u], zv[r][u]);
+                                pv[r][u] = slot[r][u] >= 0 ? gather(lci[slot[r][u]]) : T(0);
                             }
                         }
 #pragma unroll
@@ -496,10 +464,7 @@ __global__ void __launch_bounds__(kBlock)
                             for (int u = 0; u < kRowBatch; ++u) {
                                 const bool act = slot[r][u] >= 0;
                                 const T vv = lv[act ? slot[r][u] : 0];
-                                const T fm = fma(f, pv[r][u], zv[r][u]);
-                                const T r1 = mode == 1 ? zv[r][u] : fm;
-                                const T d = mode == 0 ? pv[r][u] : r1;
-                                sum[r] = fma(act ? vv : T(0), act ? d : T(0), sum[r]);
+                                sum[r] = fma(act ? vv : T(0), pv[r][u], sum[r]);
                             }
                         }
                     }
@@ -511,23 +476,62 @@ __global__ void __launch_bounds__(kBlock)
                         beg[r] = lrp[lr];
                         end[r] = lrp[rr + r < rows ? lr + 1 : rows];
                     }
-                    rows_product<T, R, L>(ci, v, beg, end, dir, sum);
+                    rows_product<T, R, L>(ci, v, beg, end, at, sum);
                 }
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int64_t i = cs + rr + r;
                     if (rr + r < rows && col_ok) {
-                        const T fm = fma(f, own_p[r], own_z[r]);
-                        const T r1 = mode == 1 ? own_z[r] : fm;
-                        const T pi = mode == 0 ? own_p[r] : r1;
-                        p_out[i * k + j] = pi;
                         q[i * k + j] = sum[r];
-                        acc[0] += static_cast<double>(pi * sum[r]);
+                        acc[0] += static_cast<double>(own_p[r] * sum[r]);
                     }
                 }
             }
         }
         bcg_store_partials<L, 1>(acc, k, c0, w.part_b, lds);
+    }
+}
+
+// The direction update: p_out = p_in (stopped column), z (restart, the
+// previous rho is zero) or z + (rho / prev_rho) p_in, every row.  Written out
+// here, so the product kernel gathers p only -- one load per entry instead
+// of the two (p_in and z) a product that formed the direction at every
+// gathered entry would need; 3% of the solve on a 5-point stencil with 64
+// columns.
+template <typename T, int L>
+__global__ void __launch_bounds__(kBlock)
+    bcg_direction_kernel(int64_t n, int64_t k, const T* __restrict__ z,
+                         const T* __restrict__ p_in, T* __restrict__ p_out,
+                         const T* __restrict__ scalars, const uint8_t* __restrict__ status,
+                         const gkomi_stop_flags* __restrict__ flags)
+{
+    if (flags->all_stopped) return;
+    const bcg_scalars<T> sc = bcg_scalar_layout<T>(const_cast<T*>(scalars), k);
+    constexpr int kRowsPerBlock = kBlock / L;
+    const int lane = threadIdx.x % L;
+    const int g = threadIdx.x / L;
+    int64_t rb0, rb1;
+    block_range(ceildiv(n, kRowsPerBlock), &rb0, &rb1);
+    for (int64_t c0 = 0; c0 < k; c0 += L) {
+        const int64_t j = c0 + lane;
+        if (j >= k) continue;
+        int mode = 0;  // 0 keep, 1 z, 2 z + f p
+        T f = 0;
+        if (!has_stopped(status[j])) {
+            const T prev = sc.prev_used[j];
+            mode = prev == T(0) ? 1 : 2;
+            f = mode == 2 ? sc.rho[j] / prev : T(0);
+        }
+        for (int64_t blk = rb0; blk < rb1; ++blk) {
+            const int64_t i = blk * kRowsPerBlock + g;
+            if (i >= n) continue;
+            const int64_t o = i * k + j;
+            const T pv = p_in[o];
+            const T zv = z[o];
+            const T fm = fma(f, pv, zv);
+            const T r1 = mode == 1 ? zv : fm;
+            p_out[o] = mode == 0 ? pv : r1;
+        }
     }
 }
 
@@ -646,17 +650,15 @@ int bcg_grid(int64_t n, int lanes)
     return static_cast<int>(std::max<int64_t>(g, 1));
 }
 
-// Grid of the direction kernel: at most the workgroups that are resident on
-// the chip at once.  The kernel needs more registers than the others (about
-// 3 workgroups per CU in FP64), and a larger grid would run a partial second
-// wave of workgroups after the first.
+// Grid of the product kernel: at most the workgroups resident on the chip
+// at once (a larger grid would run a partial second wave of workgroups).
 template <typename T, int L, bool Off32>
-int bcg_direction_grid(int grid)
+int bcg_product_grid(int grid)
 {
     static const int per_cu = [] {
         int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &nb, bcg_direction_kernel<T, L, Off32>, kBlock, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, bcg_product_kernel<T, L, Off32>,
+                                                         kBlock, 0) != hipSuccess ||
             nb < 1) {
             (void)hipGetLastError();
             nb = 1;
@@ -756,23 +758,24 @@ int bcg_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_crite
                                                  static_cast<const T*>(st->orig_tau),
                                                  st->status, st->flags, st->issued);
         GKOMI_CHECK_LAUNCH("bcg_check_kernel");
+#define GKOMI_BCG_PROD(L, OFF)                                                             \
+    dgrid = bcg_product_grid<T, L, OFF>(grid);                                             \
+    bcg_product_kernel<T, L, OFF><<<dgrid, kBlock, 0, s>>>(                                \
+        n, k, a->row_ptrs, a->col_idxs, v, pout, static_cast<T*>(st->q), st->flags, w);
 #define GKOMI_BCG_DIR(L)                                                                   \
+    bcg_direction_kernel<T, L><<<grid, kBlock, 0, s>>>(n, k, z, pin, pout, sc, st->status, \
+                                                       st->flags);                         \
     if (off32) {                                                                           \
-        dgrid = bcg_direction_grid<T, L, true>(grid);                                      \
-        bcg_direction_kernel<T, L, true><<<dgrid, kBlock, 0, s>>>(                         \
-            n, k, a->row_ptrs, a->col_idxs, v, z, pin, pout, static_cast<T*>(st->q), sc,   \
-            st->status, st->flags, w);                                                     \
+        GKOMI_BCG_PROD(L, true)                                                            \
     } else {                                                                               \
-        dgrid = bcg_direction_grid<T, L, false>(grid);                                     \
-        bcg_direction_kernel<T, L, false><<<dgrid, kBlock, 0, s>>>(                        \
-            n, k, a->row_ptrs, a->col_idxs, v, z, pin, pout, static_cast<T*>(st->q), sc,   \
-            st->status, st->flags, w);                                                     \
+        GKOMI_BCG_PROD(L, false)                                                           \
     }                                                                                      \
     break;
         int dgrid = grid;
         GKOMI_BCG_LANES(lanes, GKOMI_BCG_DIR)
 #undef GKOMI_BCG_DIR
-        GKOMI_CHECK_LAUNCH("bcg_direction_kernel");
+#undef GKOMI_BCG_PROD
+        GKOMI_CHECK_LAUNCH("bcg_direction_kernel, bcg_product_kernel");
         bcg_beta_kernel<T><<<1, kCombineThreads, 0, s>>>(k, dgrid, w, sc, st->flags);
         GKOMI_CHECK_LAUNCH("bcg_beta_kernel");
 #define GKOMI_BCG_UPD(L)                                                                   \
