@@ -241,7 +241,7 @@ __device__ __forceinline__ void rows_product(const int32_t* __restrict__ ci,
 }
 
 // rows per lane group in flight in the product kernel
-constexpr int kBcgRows = 1;
+constexpr int kBcgRows = 2;
 
 // ---------------------------------------------------------------- start ----
 // r = b - A x; p_a = p_b = q = 0; partials of b_j.b_j and r_j.r_j.
