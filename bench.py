@@ -431,28 +431,6 @@ def bench_batch_cg(dev, num=20000):
     return out
 
 
-def edge_rank_one(rows_of, vmap, coef, e):
-    """The conductivity theta_e enters A as c * u u^T with u = e_a - e_b
-    (interior edge: entries (a,a,+c), (b,b,+c), (a,b,-c), (b,a,-c)) or
-    u = e_a (edge to the Dirichlet exterior: one entry (a,a,+c)).  Returns
-    (support, signs, c), checking that structure."""
-    import numpy as np
-    ks = np.where(vmap[:, 1] == e)[0]
-    ent = sorted((int(rows_of[vmap[k, 0]]), int(vmap[k, 2]), float(coef[k])) for k in ks)
-    diag = [(r, c, w) for r, c, w in ent if r == c]
-    off = [(r, c, w) for r, c, w in ent if r != c]
-    cval = diag[0][2]
-    if len(ent) == 1 and cval > 0:
-        return [diag[0][0]], [1.0], cval
-    a, b = diag[0][0], diag[1][0]
-    ok = (len(ent) == 4 and len(diag) == 2 and diag[1][2] == cval and
-          sorted((r, c) for r, c, _ in off) == sorted([(a, b), (b, a)]) and
-          all(w == -cval for _, _, w in off))
-    if not ok:
-        raise ValueError(f"theta[{e}] does not enter A as a rank-one term")
-    return [a, b], [1.0, -1.0], cval
-
-
 def bench_adjoint(dev, nx=512, nrhs=64, fd_entries=32, h=1e-6, convection=0.0, method="cg"):
     """512^2 anisotropic diffusion, 64 right-hand sides: forward FP64 solve to
     1e-12 (CG; BiCGSTAB for the nonsymmetric convection-diffusion variant,
@@ -462,86 +440,38 @@ def bench_adjoint(dev, nx=512, nrhs=64, fd_entries=32, h=1e-6, convection=0.0, m
     gradient checked against central differences of the loss on `fd_entries`
     random conductivities with step h.
 
-    Each conductivity enters A as c u u^T (u = e_a - e_b, or e_a at the
-    boundary), so A(theta +- h e_i) = A0 +- h c u u^T and, by the
-    Sherman-Morrison identity, x(theta +- h e_i) = x0 - (+-hc)(u^T x0) w /
-    (1 +- hc s) with w = A0^{-1} u, s = u^T w.  The central difference
-    [L(theta + h e_i) - L(theta - h e_i)] / 2h is evaluated from these exact
-    expressions (the identity needs no symmetry, so it also holds with the
-    constant convection part in A0):
-        (L+ - L-) / 2h = -(c/D) sum_k (u^T x0_k) [w.(x0_k - xt_k)
-                                                 + (hc)^2 s (u^T x0_k) w.w / D],
-        D = 1 - (hcs)^2.
-    Subtracting two separately solved losses instead loses ~eps |L| / h to
-    rounding (|L| ~ 1e10 here), which at h = 1e-6 swamps the difference even
-    with exact solves.  The w are solved forward with A0 (not A0^T), so the
-    check is independent of the adjoint solve."""
+    The central differences are evaluated exactly through the
+    Sherman-Morrison identity (ginkgo_amd.fdcheck), from forward solves with
+    A0, so the check is independent of the adjoint solve."""
     import numpy as np
-    import torch
-    from ginkgo_amd import generators as gen
     from ginkgo_amd.autograd import sparse_solve
-    log(f"[bench] adjoint: generating 512^2 anisotropic diffusion, convection {convection}")
-    rp, ci, vals, theta, vmap, coef = gen.anisotropic2d(nx, seed=6, convection=convection)
-    n, nnz = nx * nx, len(ci)
-    vmap_t = torch.as_tensor(vmap).to(dev)
-    coef_t = torch.as_tensor(coef).to(dev)
-    th0 = torch.as_tensor(theta).to(dev)
-    th = th0.clone().requires_grad_(True)
-    rng = np.random.default_rng(7)
-    b = torch.as_tensor(rng.uniform(-1, 1, (n, nrhs))).to(dev)
-    xt = torch.as_tensor(rng.uniform(-1, 1, (n, nrhs))).to(dev)
-    rp_t = torch.as_tensor(rp).to(dev)
-    ci_t = torch.as_tensor(ci).to(dev)
-
-    def values_of(t, const=None):
-        base = torch.zeros(nnz, dtype=torch.float64, device=dev) if const is None else const
-        return base.index_add(0, vmap_t[:, 0], coef_t * t[vmap_t[:, 1]])
-    # the part of the values that does not depend on theta (the convection)
-    const = torch.as_tensor(vals).to(dev) - values_of(th0)
-    values_of_theta = lambda t: values_of(t, const)
+    from ginkgo_amd.fdcheck import AdjointProblem
+    log(f"[bench] adjoint: generating {nx}^2 anisotropic diffusion, convection {convection}")
+    P = AdjointProblem(nx, nrhs, dev, convection=convection)
+    th = P.theta0.clone().requires_grad_(True)
 
     def run():
         th.grad = None
-        x = sparse_solve(values_of_theta(th), b, rp_t, ci_t, method=method, tol=1e-12,
+        x = sparse_solve(P.values_of(th), P.b, P.rp_t, P.ci_t, method=method, tol=1e-12,
                          max_iters=20000)
-        loss = 0.5 * ((x - xt) ** 2).sum()
+        loss = P.loss(x)
         loss.backward()
         return loss, x
     t, _ = timed(run)
     log(f"[bench] adjoint: first forward + backward {t:.2f} s")
     t2, (loss, x0) = timed(run)
     g = th.grad.detach().clone()
-    out = {"time_s_forward_backward": round(min(t, t2), 4), "n": n, "nnz": nnz, "nrhs": nrhs,
-           "method": method, "convection": convection, "loss": float(loss.item()),
-           "symmetric": bool(torch.all(const == 0).item())}
+    out = {"time_s_forward_backward": round(min(t, t2), 4), "n": P.n, "nnz": P.nnz,
+           "nrhs": nrhs, "method": method, "convection": convection,
+           "loss": float(loss.item()), "symmetric": P.symmetric}
     if fd_entries > 0:
         log(f"[bench] adjoint: central differences on {fd_entries} entries, h = {h}")
         t0 = time.perf_counter()
-        x0 = x0.detach()
-        rows_of = np.repeat(np.arange(n), np.diff(rp))
-        vm = np.stack([vmap[:, 0], vmap[:, 1], ci[vmap[:, 0]]], axis=1)
-        idx = np.random.default_rng(8).choice(theta.size, fd_entries, replace=False)
-        U = torch.zeros((n, fd_entries), dtype=torch.float64, device=dev)
-        shapes = []
-        for m, e in enumerate(idx):
-            sup, sgn, c = edge_rank_one(rows_of, vm, coef, e)
-            for r_, s_ in zip(sup, sgn):
-                U[r_, m] = s_
-            shapes.append((sup, sgn, c))
-        W = sparse_solve(values_of_theta(th0), U, rp_t, ci_t, method=method, tol=1e-12,
-                         max_iters=20000)
-        fd = []
-        for m, (sup, sgn, c) in enumerate(shapes):
-            w = W[:, m]
-            s_ = float(sum(sg * w[r_].item() for r_, sg in zip(sup, sgn)))
-            ux = sum(sg * x0[r_, :] for r_, sg in zip(sup, sgn))        # u^T x0_k
-            wr = (w[:, None] * (x0 - xt)).sum(0)                        # w.(x0_k - xt_k)
-            ww = float((w * w).sum().item())
-            hc = h * c
-            den = 1.0 - (hc * s_) ** 2
-            fd.append(float((-(c / den) * (ux * (wr + hc * hc * s_ * ux * ww / den))).sum()))
-        fd = np.array(fd)
-        gi = g[torch.as_tensor(idx, device=dev)].cpu().numpy()
+        idx = np.random.default_rng(8).choice(P.theta.size, fd_entries, replace=False)
+        fd = P.central_differences(
+            lambda U: sparse_solve(P.values_of(P.theta0), U, P.rp_t, P.ci_t, method=method,
+                                   tol=1e-12, max_iters=20000), x0, idx, h)
+        gi = g[torch_index(idx, dev)].cpu().numpy()
         gmax = float(g.abs().max().item())
         out["fd_check"] = {
             "entries": int(fd_entries), "h": h,
@@ -550,6 +480,11 @@ def bench_adjoint(dev, nx=512, nrhs=64, fd_entries=32, h=1e-6, convection=0.0, m
             "max_err_over_max_grad": float(np.max(np.abs(fd - gi)) / gmax),
             "time_s": round(time.perf_counter() - t0, 2)}
     return out
+
+
+def torch_index(idx, dev):
+    import torch
+    return torch.as_tensor(idx, device=dev)
 
 
 def main():

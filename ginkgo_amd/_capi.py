@@ -18,6 +18,9 @@ ALLOCATION_ERROR = 3
 HIP_ERROR = 4
 DIMENSION_MISMATCH = 5
 
+VALUE_F64 = 0
+VALUE_F32 = 1
+
 CSR_CLASSICAL = 0
 CSR_LOAD_BALANCE = 1
 CSR_MERGE_PATH = 2
@@ -86,7 +89,7 @@ class Csr(ctypes.Structure):
                 ("row_ptrs", ctypes.c_void_p), ("col_idxs", ctypes.c_void_p),
                 ("values", ctypes.c_void_p), ("strategy", ctypes.c_int32),
                 ("subwave", ctypes.c_int32), ("srow", ctypes.c_void_p),
-                ("srow_blocks", ctypes.c_int64), ("col_offsets", ctypes.c_void_p),
+                ("num_srow", ctypes.c_int64), ("col_offsets", ctypes.c_void_p),
                 ("work", ctypes.c_void_p), ("work_bytes", ctypes.c_int64)]
 
 
@@ -218,8 +221,8 @@ _SIGS = {
     "gkomi_convert_f64_to_f32": (ctypes.c_int, [_CTX, _I64, _P, _P]),
     "gkomi_convert_f32_to_f64": (ctypes.c_int, [_CTX, _I64, _P, _P]),
     "gkomi_reduction_workspace_bytes": (_SZ, [_I64, _I64]),
-    "gkomi_csr_lb_chunk": (_I64, [_I32]),
-    "gkomi_csr_lb_partition": (ctypes.c_int, [_CTX, _I64, _P, _I64, _I64, _P, _I64]),
+    "gkomi_csr_lb_clac_size": (_I64, [_I64, _I64, _I32, _I32, _I32]),
+    "gkomi_csr_lb_partition": (ctypes.c_int, [_CTX, _I64, _P, _I64, _I32, _P, _I64]),
     "gkomi_csr_max_row_nnz": (ctypes.c_int, [_CTX, _I64, _P, _P]),
     "gkomi_csr_strategy_work_bytes": (_I64, [_I32, _I64, _I64]),
     "gkomi_csr_col_offsets": (ctypes.c_int, [_CTX, ctypes.POINTER(Csr), _P, _P]),
@@ -236,6 +239,13 @@ _SIGS = {
     "gkomi_batch_lds_bytes": (_SZ, [_I32, _I64, _I64, _I32]),
     "gkomi_batch_cg_lds_bytes": (_SZ, [_I32, _I64, _I64, _I32]),
 }
+
+for _f, _d in (("csr", Csr), ("sellp", Sellp), ("ell", Ell)):
+    _SIGS[f"gkomi_{_f}_spmv_mixed"] = (ctypes.c_int, [_CTX, _I32, ctypes.POINTER(_d), _I64,
+                                                      _I32, _P, _I64, _I32, _P, _I64])
+    _SIGS[f"gkomi_{_f}_advanced_spmv_mixed"] = (ctypes.c_int, [_CTX, _P, _I32,
+                                                               ctypes.POINTER(_d), _I64, _I32,
+                                                               _P, _I64, _P, _I32, _P, _I64])
 
 for _s in ("f64", "f32"):
     _SIGS.update({

@@ -15,8 +15,15 @@ result differentiable with respect to the matrix values and b:
 values is the CSR value array (nnz,), b is (n,) or (n, k).  Gradients flow
 on to whatever produced `values` (e.g. per-edge conductivities scattered
 into the matrix with torch ops).
+
+Both solves must converge: a column whose stopping status lacks the
+converged bit (include/ginkgo/core/stop/stopping_status.hpp:21-100), e.g.
+one stopped by the iteration limit, raises SolveNotConverged (or warns, or
+passes, as `on_not_converged` says), since its x and every gradient through
+it would be wrong.
 """
 import ctypes
+import warnings
 
 import torch
 
@@ -25,6 +32,25 @@ from . import device as D
 from .device import as_2d
 
 _EXECS = {}
+
+
+class SolveNotConverged(C.GkoError):
+    """A forward or adjoint solve stopped without meeting its residual
+    criterion (for instance on the iteration limit)."""
+
+
+def _check_converged(solver, phase, tol, max_iters, policy):
+    status = list(solver.last_status)
+    bad = [j for j, st in enumerate(status) if not (st & C.STATUS_CONVERGED)]
+    if not bad or policy == "ignore":
+        return
+    msg = (f"{phase} solve did not converge to relative residual {tol:g} within "
+           f"{max_iters} iterations ({solver.last_iterations} run) in column(s) "
+           f"{bad[:8]}{'...' if len(bad) > 8 else ''} of {len(status)}")
+    if policy == "warn":
+        warnings.warn(msg, RuntimeWarning, stacklevel=4)
+    else:
+        raise SolveNotConverged(msg)
 
 
 def executor(device=None):
@@ -75,10 +101,12 @@ def sampled_outer_product(row_ptrs, col_idxs, values, lam, x, alpha=-1.0):
 
 class SparseSolve(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, values, b, row_ptrs, col_idxs, method, tol, max_iters, jacobi):
+    def forward(ctx, values, b, row_ptrs, col_idxs, method, tol, max_iters, jacobi, policy):
         from . import gko
         if method not in ("cg", "bicgstab"):
             raise C.NotSupported("method must be 'cg' (SPD) or 'bicgstab'")
+        if policy not in ("raise", "warn", "ignore"):
+            raise C.InvalidArgument("on_not_converged must be 'raise', 'warn' or 'ignore'")
         exec_ = executor(values.device)
         vals = values.detach().contiguous()
         rhs = b.detach().reshape(b.shape[0], -1).contiguous()
@@ -86,8 +114,9 @@ class SparseSolve(torch.autograd.Function):
         ci = col_idxs.to(torch.int32).contiguous()
         A = gko.csr_view(exec_, rp, ci, vals, strategy="classical")
         x, solver = _solve(gko, exec_, A, rhs, method, tol, max_iters, jacobi)
+        _check_converged(solver, "forward", tol, max_iters, policy)
         ctx.save_for_backward(vals, x, rp, ci)
-        ctx.settings = (method, tol, max_iters, jacobi, b.dim())
+        ctx.settings = (method, tol, max_iters, jacobi, b.dim(), policy)
         ctx.A = A
         ctx.iterations = solver.last_iterations
         return x.reshape(b.shape)
@@ -96,20 +125,24 @@ class SparseSolve(torch.autograd.Function):
     def backward(ctx, grad_x):
         from . import gko
         vals, x, rp, ci = ctx.saved_tensors
-        method, tol, max_iters, jacobi, bdim = ctx.settings
+        method, tol, max_iters, jacobi, bdim, policy = ctx.settings
         exec_ = executor(vals.device)
         g = grad_x.reshape(x.shape).contiguous()
         # adjoint system: A is symmetric for CG; otherwise transpose on the device
         At = ctx.A if method == "cg" else ctx.A.transpose()
-        lam, _ = _solve(gko, exec_, At, g, method, tol, max_iters, jacobi)
+        lam, adjoint = _solve(gko, exec_, At, g, method, tol, max_iters, jacobi)
+        _check_converged(adjoint, "adjoint", tol, max_iters, policy)
         grad_vals = None
         if ctx.needs_input_grad[0]:
             grad_vals = sampled_outer_product(rp, ci, vals, lam, x, alpha=-1.0)
         grad_b = lam.reshape(grad_x.shape) if ctx.needs_input_grad[1] else None
-        return grad_vals, grad_b, None, None, None, None, None, None
+        return grad_vals, grad_b, None, None, None, None, None, None, None
 
 
 def sparse_solve(values, b, row_ptrs, col_idxs, method="cg", tol=1e-12, max_iters=10000,
-                 jacobi=False):
-    """Differentiable x = A(values)^{-1} b (see the module docstring)."""
-    return SparseSolve.apply(values, b, row_ptrs, col_idxs, method, tol, max_iters, jacobi)
+                 jacobi=False, on_not_converged="raise"):
+    """Differentiable x = A(values)^{-1} b (see the module docstring).
+    on_not_converged: 'raise' (SolveNotConverged), 'warn' or 'ignore' when a
+    forward or adjoint column stops without meeting the criterion."""
+    return SparseSolve.apply(values, b, row_ptrs, col_idxs, method, tol, max_iters, jacobi,
+                             on_not_converged)

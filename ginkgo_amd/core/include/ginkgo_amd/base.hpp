@@ -267,6 +267,10 @@ public:
     void* get_stream() const { return ctx_.stream; }
     int get_num_multiprocessor() const { return info_.num_cus; }
     int get_warp_size() const { return info_.warp_size; }
+    // HipExecutor::get_num_warps_per_sm / get_num_warps (executor.hpp:1846-1882):
+    // four SIMDs per compute unit on CDNA (hip/base/executor.hip.cpp:234-238)
+    int get_num_warps_per_sm() const { return 4; }
+    int get_num_warps() const { return info_.num_cus * get_num_warps_per_sm(); }
     const gkomi_device_info& get_device_info() const { return info_; }
     static int get_num_devices();
     // bytes currently held in the pool / handed out

@@ -96,4 +96,37 @@ inline void precision_dispatch(Fn fn, const LinOp* alpha, const LinOp* b, const 
     dx.write_back();
 }
 
+// mixed_precision_dispatch (include/ginkgo/core/base/precision_dispatch.hpp:
+// 228-262, the GINKGO_MIXED_PRECISION form): runs fn on the operands as
+// Dense of their own value types, each of V or the other precision, with
+// nothing converted; the kernel computes in the highest precision.
+template <typename V, typename Fn>
+inline void mixed_precision_dispatch(Fn fn, const LinOp* b, LinOp* x)
+{
+    using O = detail::other_t<V>;
+    auto with_out = [&](auto db) {
+        if (auto dx = dynamic_cast<matrix::Dense<V>*>(x)) {
+            fn(db, dx);
+        } else if (auto dx = dynamic_cast<matrix::Dense<O>*>(x)) {
+            fn(db, dx);
+        } else {
+            GKO_NOT_SUPPORTED_MSG("output operand is not a Dense matrix");
+        }
+    };
+    if (auto db = dynamic_cast<const matrix::Dense<V>*>(b)) {
+        with_out(db);
+    } else if (auto db = dynamic_cast<const matrix::Dense<O>*>(b)) {
+        with_out(db);
+    } else {
+        GKO_NOT_SUPPORTED_MSG("input operand is not a Dense matrix");
+    }
+}
+
+// The C ABI's value-type code of T.
+template <typename T>
+constexpr int32_t value_code()
+{
+    return std::is_same<T, double>::value ? GKOMI_VALUE_F64 : GKOMI_VALUE_F32;
+}
+
 }  // namespace gko

@@ -69,11 +69,54 @@ public:
     private:
         int subwave_;
     };
+    // Csr::load_balance (csr.hpp:354-520): the srow partition of
+    // clac_size(nnz) warps, built by process().  The default constructor
+    // takes the executor's warps when the matrix builds its partition (the
+    // reference's default asks CUDA device 0 for them, csr.hpp:359-364).
     class load_balance : public strategy_type {
     public:
         load_balance() = default;
-        explicit load_balance(int64) {}
+        explicit load_balance(std::shared_ptr<const Mi355xExecutor> exec)
+            : load_balance(exec->get_num_warps(), exec->get_warp_size(), false)
+        {}
+        load_balance(int64 nwarps, int warp_size = 32, bool cuda_strategy = true,
+                     std::string strategy_name = "none")
+            : nwarps_{nwarps},
+              warp_size_{warp_size},
+              cuda_strategy_{cuda_strategy},
+              strategy_name_{std::move(strategy_name)}
+        {}
         std::string get_name() const override { return "load_balance"; }
+        // csr.hpp:475-505 (HIP-platform build)
+        int64 clac_size(int64 nnz) const
+        {
+            return gkomi_csr_lb_clac_size(nnz, nwarps_, warp_size_, cuda_strategy_ ? 1 : 0,
+                                          strategy_name_ == "intel" ? 1 : 0);
+        }
+        // csr.hpp:417-468, on the device (bit-identical integers)
+        void process(std::shared_ptr<const Executor> exec, size_type n_rows,
+                     const IndexType* row_ptrs, int64 nnz, array<IndexType>* srow) const
+        {
+            if (srow->get_size() == 0) return;
+            GKO_CALL(gkomi_csr_lb_partition(ctx_of(exec), static_cast<int64>(n_rows), row_ptrs,
+                                            nnz, warp_size_, srow->get_data(),
+                                            static_cast<int64>(srow->get_size())));
+        }
+        bool has_warps() const { return nwarps_ > 0; }
+        // the executor's warps for a default-constructed strategy
+        load_balance resolved(const Executor& exec) const
+        {
+            if (has_warps()) return *this;
+            auto m = dynamic_cast<const Mi355xExecutor*>(&exec);
+            if (!m) return load_balance(1, 64, false);
+            return load_balance(m->get_num_warps(), m->get_warp_size(), false);
+        }
+
+    private:
+        int64 nwarps_{0};
+        int warp_size_{64};
+        bool cuda_strategy_{false};
+        std::string strategy_name_{"none"};
     };
     // merge-path partition of rows and entries (csr.hpp:283-301): every
     // workgroup takes an equal share of n + nnz merge steps
@@ -156,6 +199,7 @@ public:
     std::unique_ptr<Csr> clone() const;
     // host copies of the three arrays
     std::vector<IndexType> row_ptrs_to_host() const { return row_ptrs_.to_host(); }
+    std::vector<IndexType> srow_to_host() const { return srow_.to_host(); }
     std::vector<IndexType> col_idxs_to_host() const { return col_idxs_.to_host(); }
     std::vector<ValueType> values_to_host() const { return values_.to_host(); }
 

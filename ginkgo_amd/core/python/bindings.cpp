@@ -61,6 +61,17 @@ std::shared_ptr<typename matrix::Csr<V>::strategy_type> strategy(const std::stri
     using C = matrix::Csr<V>;
     if (s == "classical") return std::make_shared<typename C::classical>(sw);
     if (s == "load_balance") return std::make_shared<typename C::load_balance>();
+    // load_balance:<nwarps>[:<warp_size>[:cuda|hip]] -- the reference's
+    // load_balance(nwarps, warp_size = 32, cuda_strategy = true) constructor
+    if (s.rfind("load_balance:", 0) == 0) {
+        const auto a = s.find(':');
+        const auto b = s.find(':', a + 1);
+        const int64 nw = std::stoll(s.substr(a + 1, b - a - 1));
+        const auto c = b == std::string::npos ? b : s.find(':', b + 1);
+        const int ws = b == std::string::npos ? 32 : std::stoi(s.substr(b + 1, c - b - 1));
+        const bool cuda = c == std::string::npos || s.substr(c + 1) != "hip";
+        return std::make_shared<typename C::load_balance>(nw, ws, cuda);
+    }
     if (s == "automatical") return std::make_shared<typename C::automatical>();
     if (s == "merge_path") return std::make_shared<typename C::merge_path>();
     if (s == "sparselib") return std::make_shared<typename C::sparselib>();
@@ -348,6 +359,8 @@ void bind_value_type(py::module_& m, const std::string& sfx)
         .def_property_readonly("effective_strategy", &Csr::get_effective_strategy)
         .def_property_readonly("nnz", &Csr::get_num_stored_elements)
         .def("row_ptrs", [](const Csr& a) { return arr(a.row_ptrs_to_host()); })
+        .def("srow", [](const Csr& a) { return arr(a.srow_to_host()); })
+        .def_property_readonly("num_srow_elements", &Csr::get_num_srow_elements)
         .def("col_idxs", [](const Csr& a) { return arr(a.col_idxs_to_host()); })
         .def("values", [](const Csr& a) { return arr(a.values_to_host()); });
 

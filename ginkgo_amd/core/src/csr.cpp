@@ -15,28 +15,16 @@ struct sp;
 
 template <>
 struct sp<double> {
-    static constexpr auto spmv = gkomi_csr_spmv_f64;
-    static constexpr auto adv = gkomi_csr_advanced_spmv_f64;
     static constexpr auto transpose = gkomi_csr_transpose_f64;
     static constexpr auto to_sellp = gkomi_csr_to_sellp_f64;
     static constexpr auto to_ell = gkomi_csr_to_ell_f64;
-    static constexpr auto sellp_spmv = gkomi_sellp_spmv_f64;
-    static constexpr auto sellp_adv = gkomi_sellp_advanced_spmv_f64;
-    static constexpr auto ell_spmv = gkomi_ell_spmv_f64;
-    static constexpr auto ell_adv = gkomi_ell_advanced_spmv_f64;
 };
 
 template <>
 struct sp<float> {
-    static constexpr auto spmv = gkomi_csr_spmv_f32;
-    static constexpr auto adv = gkomi_csr_advanced_spmv_f32;
     static constexpr auto transpose = gkomi_csr_transpose_f32;
     static constexpr auto to_sellp = gkomi_csr_to_sellp_f32;
     static constexpr auto to_ell = gkomi_csr_to_ell_f32;
-    static constexpr auto sellp_spmv = gkomi_sellp_spmv_f32;
-    static constexpr auto sellp_adv = gkomi_sellp_advanced_spmv_f32;
-    static constexpr auto ell_spmv = gkomi_ell_spmv_f32;
-    static constexpr auto ell_adv = gkomi_ell_advanced_spmv_f32;
 };
 
 }  // namespace
@@ -165,13 +153,16 @@ void Csr<V, I>::make_srow()
     }
     effective_ = name;
     if (name == "load_balance") {
-        const int64 chunk = gkomi_csr_lb_chunk(sizeof(V));
+        // an explicit load_balance keeps its warps; automatical resolving to
+        // it uses the executor's, as the reference's automatical does for a
+        // HipExecutor (csr.hpp:606-673)
+        auto lb = std::dynamic_pointer_cast<load_balance>(strategy_);
+        const load_balance strat = lb ? lb->resolved(*this->get_executor())
+                                      : load_balance().resolved(*this->get_executor());
         const int64 nnz = static_cast<int64>(values_.get_size());
-        const int64 blocks = std::max<int64>(1, (nnz + chunk - 1) / chunk);
-        srow_ = array<I>(this->get_executor(), blocks + 1);
-        GKO_CALL(gkomi_csr_lb_partition(ctx_of(this->get_executor()), this->get_size().rows,
-                                        row_ptrs_.get_const_data(), nnz, chunk,
-                                        srow_.get_data(), blocks));
+        srow_ = array<I>(this->get_executor(), static_cast<size_type>(strat.clac_size(nnz)));
+        strat.process(this->get_executor(), this->get_size().rows, row_ptrs_.get_const_data(),
+                      nnz, &srow_);
     } else {
         srow_.clear();
     }
@@ -212,7 +203,7 @@ gkomi_csr Csr<V, I>::descriptor() const
                      : (effective_ == "merge_path" ? GKOMI_CSR_MERGE_PATH : GKOMI_CSR_CLASSICAL);
     d.subwave = subwave_;
     d.srow = srow_.get_const_data();
-    d.srow_blocks = srow_.get_size() ? static_cast<int64>(srow_.get_size()) - 1 : 0;
+    d.num_srow = static_cast<int64>(srow_.get_size());
     d.col_offsets = compact_ ? col_offsets_.get_const_data() : nullptr;
     d.work = strategy_work_.get_data();
     d.work_bytes = static_cast<int64>(strategy_work_.get_size());
@@ -238,15 +229,21 @@ bool Csr<V, I>::compact_indices()
 }
 
 // Csr::apply_impl (core/matrix/csr.cpp:257-315) -> csr::spmv / advanced_spmv
+// over mixed_precision_dispatch: b and x may each be Dense<double> or
+// Dense<float>; alpha is taken in the matrix's precision, beta in x's
+// (make_temporary_conversion, csr.cpp:297-303).
 template <typename V, typename I>
 void Csr<V, I>::apply_impl(const LinOp* b, LinOp* x) const
 {
-    precision_dispatch<V>(
+    mixed_precision_dispatch<V>(
         [this](auto db, auto dx) {
+            using TB = typename std::decay_t<decltype(*db)>::value_type;
+            using TC = typename std::decay_t<decltype(*dx)>::value_type;
             const auto d = descriptor();
-            GKO_CALL(sp<V>::spmv(ctx_of(this->get_executor()), &d, db->get_size().cols,
-                                 db->get_const_values(), db->get_stride(), dx->get_values(),
-                                 dx->get_stride()));
+            GKO_CALL(gkomi_csr_spmv_mixed(ctx_of(this->get_executor()), value_code<V>(), &d,
+                                          db->get_size().cols, value_code<TB>(),
+                                          db->get_const_values(), db->get_stride(),
+                                          value_code<TC>(), dx->get_values(), dx->get_stride()));
         },
         b, x);
 }
@@ -255,14 +252,20 @@ template <typename V, typename I>
 void Csr<V, I>::apply_impl(const LinOp* alpha, const LinOp* b, const LinOp* beta,
                            LinOp* x) const
 {
-    precision_dispatch<V>(
-        [this](auto da, auto db, auto dbeta, auto dx) {
+    mixed_precision_dispatch<V>(
+        [this, alpha, beta](auto db, auto dx) {
+            using TB = typename std::decay_t<decltype(*db)>::value_type;
+            using TC = typename std::decay_t<decltype(*dx)>::value_type;
+            detail::dense_in<V> da(alpha);
+            detail::dense_in<TC> dbeta(beta);
             const auto d = descriptor();
-            GKO_CALL(sp<V>::adv(ctx_of(this->get_executor()), da->get_const_values(), &d,
-                                db->get_size().cols, db->get_const_values(), db->get_stride(),
-                                dbeta->get_const_values(), dx->get_values(), dx->get_stride()));
+            GKO_CALL(gkomi_csr_advanced_spmv_mixed(
+                ctx_of(this->get_executor()), da.ptr->get_const_values(), value_code<V>(), &d,
+                db->get_size().cols, value_code<TB>(), db->get_const_values(), db->get_stride(),
+                dbeta.ptr->get_const_values(), value_code<TC>(), dx->get_values(),
+                dx->get_stride()));
         },
-        alpha, b, beta, x);
+        b, x);
 }
 
 template <typename V, typename I>
@@ -380,15 +383,21 @@ gkomi_sellp Sellp<V, I>::descriptor() const
     return d;
 }
 
+// Sellp::apply_impl (core/matrix/sellp.cpp) over mixed_precision_dispatch,
+// as Csr.
 template <typename V, typename I>
 void Sellp<V, I>::apply_impl(const LinOp* b, LinOp* x) const
 {
-    precision_dispatch<V>(
+    mixed_precision_dispatch<V>(
         [this](auto db, auto dx) {
+            using TB = typename std::decay_t<decltype(*db)>::value_type;
+            using TC = typename std::decay_t<decltype(*dx)>::value_type;
             const auto d = descriptor();
-            GKO_CALL(sp<V>::sellp_spmv(ctx_of(this->get_executor()), &d, db->get_size().cols,
-                                       db->get_const_values(), db->get_stride(),
-                                       dx->get_values(), dx->get_stride()));
+            GKO_CALL(gkomi_sellp_spmv_mixed(ctx_of(this->get_executor()), value_code<V>(), &d,
+                                            db->get_size().cols, value_code<TB>(),
+                                            db->get_const_values(), db->get_stride(),
+                                            value_code<TC>(), dx->get_values(),
+                                            dx->get_stride()));
         },
         b, x);
 }
@@ -397,15 +406,20 @@ template <typename V, typename I>
 void Sellp<V, I>::apply_impl(const LinOp* alpha, const LinOp* b, const LinOp* beta,
                              LinOp* x) const
 {
-    precision_dispatch<V>(
-        [this](auto da, auto db, auto dbeta, auto dx) {
+    mixed_precision_dispatch<V>(
+        [this, alpha, beta](auto db, auto dx) {
+            using TB = typename std::decay_t<decltype(*db)>::value_type;
+            using TC = typename std::decay_t<decltype(*dx)>::value_type;
+            detail::dense_in<V> da(alpha);
+            detail::dense_in<TC> dbeta(beta);
             const auto d = descriptor();
-            GKO_CALL(sp<V>::sellp_adv(ctx_of(this->get_executor()), da->get_const_values(), &d,
-                                      db->get_size().cols, db->get_const_values(),
-                                      db->get_stride(), dbeta->get_const_values(),
-                                      dx->get_values(), dx->get_stride()));
+            GKO_CALL(gkomi_sellp_advanced_spmv_mixed(
+                ctx_of(this->get_executor()), da.ptr->get_const_values(), value_code<V>(), &d,
+                db->get_size().cols, value_code<TB>(), db->get_const_values(), db->get_stride(),
+                dbeta.ptr->get_const_values(), value_code<TC>(), dx->get_values(),
+                dx->get_stride()));
         },
-        alpha, b, beta, x);
+        b, x);
 }
 
 // ---------------------------------------------------------------- Ell ----
@@ -423,15 +437,19 @@ gkomi_ell Ell<V, I>::descriptor() const
     return d;
 }
 
+// Ell::apply_impl (core/matrix/ell.cpp) over mixed_precision_dispatch, as Csr.
 template <typename V, typename I>
 void Ell<V, I>::apply_impl(const LinOp* b, LinOp* x) const
 {
-    precision_dispatch<V>(
+    mixed_precision_dispatch<V>(
         [this](auto db, auto dx) {
+            using TB = typename std::decay_t<decltype(*db)>::value_type;
+            using TC = typename std::decay_t<decltype(*dx)>::value_type;
             const auto d = descriptor();
-            GKO_CALL(sp<V>::ell_spmv(ctx_of(this->get_executor()), &d, db->get_size().cols,
-                                     db->get_const_values(), db->get_stride(), dx->get_values(),
-                                     dx->get_stride()));
+            GKO_CALL(gkomi_ell_spmv_mixed(ctx_of(this->get_executor()), value_code<V>(), &d,
+                                          db->get_size().cols, value_code<TB>(),
+                                          db->get_const_values(), db->get_stride(),
+                                          value_code<TC>(), dx->get_values(), dx->get_stride()));
         },
         b, x);
 }
@@ -440,15 +458,20 @@ template <typename V, typename I>
 void Ell<V, I>::apply_impl(const LinOp* alpha, const LinOp* b, const LinOp* beta,
                            LinOp* x) const
 {
-    precision_dispatch<V>(
-        [this](auto da, auto db, auto dbeta, auto dx) {
+    mixed_precision_dispatch<V>(
+        [this, alpha, beta](auto db, auto dx) {
+            using TB = typename std::decay_t<decltype(*db)>::value_type;
+            using TC = typename std::decay_t<decltype(*dx)>::value_type;
+            detail::dense_in<V> da(alpha);
+            detail::dense_in<TC> dbeta(beta);
             const auto d = descriptor();
-            GKO_CALL(sp<V>::ell_adv(ctx_of(this->get_executor()), da->get_const_values(), &d,
-                                    db->get_size().cols, db->get_const_values(),
-                                    db->get_stride(), dbeta->get_const_values(),
-                                    dx->get_values(), dx->get_stride()));
+            GKO_CALL(gkomi_ell_advanced_spmv_mixed(
+                ctx_of(this->get_executor()), da.ptr->get_const_values(), value_code<V>(), &d,
+                db->get_size().cols, value_code<TB>(), db->get_const_values(), db->get_stride(),
+                dbeta.ptr->get_const_values(), value_code<TC>(), dx->get_values(),
+                dx->get_stride()));
         },
-        alpha, b, beta, x);
+        b, x);
 }
 
 template class Csr<double, int32>;

@@ -1,10 +1,12 @@
 // Shared device/host helpers of the gfx950 kernel library (libgkomi.so).
 //
 // Every kernel in this library is written for CDNA4 (gfx950): wave64,
-// 256-thread workgroups (4 waves) unless stated otherwise, DPP / ds_swizzle
-// cross-lane reductions, and single-pass cross-workgroup combines through a
-// ticket counter (the last workgroup to finish reduces the per-workgroup
-// partials in a fixed order, so every reduction is deterministic).
+// 256-thread workgroups (4 waves) unless stated otherwise, and DPP /
+// ds_swizzle cross-lane reductions.  Cross-workgroup sums never use a
+// device-scope fence or a ticket counter: each workgroup writes one FP64
+// partial, and the consumer (a one-workgroup kernel, or every workgroup of
+// the next kernel of a fused solver) adds the partials in one fixed order,
+// so every reduction is deterministic.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -73,6 +75,23 @@ inline unsigned grid_for(int64_t items, int64_t per_block, int64_t cap_per_cu = 
     if (g > cap) g = cap;
     if (g < 1) g = 1;
     return static_cast<unsigned>(g);
+}
+
+// Calls fn(TA{}, TB{}, TC{}) with the value types (double / float) named by
+// the codes GKOMI_VALUE_F64 / GKOMI_VALUE_F32 of a product's matrix, input
+// and output operands.
+template <typename Fn>
+int dispatch_value_types(int32_t a_type, int32_t b_type, int32_t c_type, Fn fn)
+{
+    const auto valid = [](int32_t t) { return t == GKOMI_VALUE_F64 || t == GKOMI_VALUE_F32; };
+    GKOMI_REQUIRE(valid(a_type) && valid(b_type) && valid(c_type), GKOMI_INVALID_ARGUMENT,
+                  "unknown value type code (GKOMI_VALUE_F64 or GKOMI_VALUE_F32)");
+    const auto with = [](int32_t t, auto f) {
+        return t == GKOMI_VALUE_F64 ? f(double{}) : f(float{});
+    };
+    return with(a_type, [&](auto a) {
+        return with(b_type, [&](auto b) { return with(c_type, [&](auto c) { return fn(a, b, c); }); });
+    });
 }
 
 // --------------------------------------------------------- value traits ----

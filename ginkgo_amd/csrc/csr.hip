@@ -68,16 +68,27 @@ __global__ void __launch_bounds__(kBlock)
     csr_spmv_balanced(int64_t n_rows, const int32_t* __restrict__ rp,
                       const int32_t* __restrict__ ci, const T* __restrict__ v,
                       const T* __restrict__ b, int64_t bs, T* __restrict__ c, int64_t cs,
-                      const int32_t* __restrict__ srow, const T* __restrict__ alpha,
-                      const T* __restrict__ beta)
+                      const int32_t* __restrict__ srow, int64_t num_srow, int64_t group,
+                      const T* __restrict__ alpha, const T* __restrict__ beta)
 {
     __shared__ T prod[kLbLdsNnz];
     __shared__ int32_t lrp[kLbLdsRows + 1];
     __shared__ int32_t long_rows[kLbLdsRows];
     __shared__ int32_t num_long;
+    // workgroup k takes the warps [k * group, (k + 1) * group) of the
+    // partition and owns the rows [srow[k * group], srow[(k + 1) * group]):
+    // row 0 onwards for the first, up to n_rows for the last.  Any
+    // non-decreasing srow gives every row exactly one owner; the clamps keep
+    // a malformed one in bounds.
     const int64_t k = xcd_block(blockIdx.x, gridDim.x);
-    const int32_t rs = srow[k];
-    const int32_t re = srow[k + 1];
+    const int64_t w0 = k * group;
+    const int64_t w1 = w0 + group;
+    int64_t rs64 = k == 0 ? 0 : srow[w0];
+    int64_t re64 = w1 >= num_srow ? n_rows : srow[w1];
+    rs64 = rs64 < 0 ? 0 : (rs64 > n_rows ? n_rows : rs64);
+    re64 = re64 < 0 ? 0 : (re64 > n_rows ? n_rows : re64);
+    const int32_t rs = static_cast<int32_t>(rs64);
+    const int32_t re = static_cast<int32_t>(re64);
     if (rs >= re) {
         return;
     }
@@ -165,30 +176,33 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// The load_balance partition (Csr::load_balance::process,
+// include/ginkgo/core/matrix/csr.hpp:417-468): the reference drops row i into
+// the bucket ceil(L_i * nwarps / D), L_i = ceil(row_ptrs[i + 1] / warp_size),
+// D = ceil(nnz / warp_size) (1 without entries), and prefix-counts the buckets.
+// Since ceil(a / D) <= w exactly when a <= w * D and L_i never decreases,
+// srow[w] is the first row i with L_i * nwarps > w * D (n_rows if none): one
+// binary search per entry, the same integers as the reference's loops.
 __global__ void __launch_bounds__(kBlock)
-    lb_partition_kernel(int64_t n_rows, const int32_t* __restrict__ rp, int64_t chunk,
-                        int32_t* __restrict__ srow, int64_t num_blocks)
+    lb_partition_kernel(int64_t n_rows, const int32_t* __restrict__ rp, int64_t nnz,
+                        int64_t warp_size, int32_t* __restrict__ srow, int64_t nwarps)
 {
+    const int64_t divider = nnz > 0 ? ceildiv(nnz, warp_size) : 1;
     const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
-    for (int64_t k = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; k <= num_blocks;
-         k += stride) {
-        if (k == num_blocks) {
-            srow[k] = static_cast<int32_t>(n_rows);
-            continue;
-        }
-        // first row r in [0, n_rows] with rp[r] >= k * chunk (r = n_rows if none)
-        const int64_t target = k * chunk;
+    for (int64_t w = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; w < nwarps;
+         w += stride) {
+        const int64_t target = w * divider;
         int64_t lo = 0;
         int64_t hi = n_rows;
         while (lo < hi) {
             const int64_t mid = (lo + hi) / 2;
-            if (rp[mid] >= target) {
+            if (ceildiv(rp[mid + 1], warp_size) * nwarps > target) {
                 hi = mid;
             } else {
                 lo = mid + 1;
             }
         }
-        srow[k] = static_cast<int32_t>(lo);
+        srow[w] = static_cast<int32_t>(lo);
     }
 }
 
@@ -288,9 +302,9 @@ int check_csr(const gkomi_csr* a)
     GKOMI_REQUIRE(a->strategy == GKOMI_CSR_CLASSICAL || a->strategy == GKOMI_CSR_LOAD_BALANCE ||
                       a->strategy == GKOMI_CSR_MERGE_PATH,
                   GKOMI_INVALID_ARGUMENT, "CSR: unknown strategy");
-    GKOMI_REQUIRE(a->strategy != GKOMI_CSR_LOAD_BALANCE ||
-                      (a->srow != nullptr && a->srow_blocks >= 1),
-                  GKOMI_INVALID_ARGUMENT, "CSR: load_balance strategy without partition");
+    GKOMI_REQUIRE(a->strategy != GKOMI_CSR_LOAD_BALANCE || a->num_srow == 0 ||
+                      (a->srow != nullptr && a->num_srow > 0),
+                  GKOMI_INVALID_ARGUMENT, "CSR: load_balance strategy with a null partition");
     return GKOMI_SUCCESS;
 }
 
@@ -334,12 +348,18 @@ int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const 
     if (a->strategy == GKOMI_CSR_MERGE_PATH) {
         return csr_spmv_merge_launch<T, Mode>(stream, a, b, bs, c, cs, alpha, beta);
     }
-    if (a->strategy == GKOMI_CSR_LOAD_BALANCE) {
-        GKOMI_REQUIRE(a->srow_blocks < (int64_t{1} << 31), GKOMI_INVALID_ARGUMENT,
+    if (a->strategy == GKOMI_CSR_LOAD_BALANCE && a->num_srow > 0) {
+        // consecutive warps of the partition per workgroup, about kLbChunk
+        // entries each (a partition without elements has no entries: the
+        // classical kernel below then writes the empty rows)
+        const int64_t per_warp = std::max<int64_t>(1, ceildiv(a->nnz, a->num_srow));
+        const int64_t group = std::max<int64_t>(1, kLbChunk / per_warp);
+        const int64_t blocks = ceildiv(a->num_srow, group);
+        GKOMI_REQUIRE(blocks < (int64_t{1} << 31), GKOMI_INVALID_ARGUMENT,
                       "CSR: partition too large");
-        csr_spmv_balanced<T, Mode>
-            <<<static_cast<unsigned>(a->srow_blocks), kBlock, 0, stream>>>(
-                a->n_rows, a->row_ptrs, a->col_idxs, v, b, bs, c, cs, a->srow, alpha, beta);
+        csr_spmv_balanced<T, Mode><<<static_cast<unsigned>(blocks), kBlock, 0, stream>>>(
+            a->n_rows, a->row_ptrs, a->col_idxs, v, b, bs, c, cs, a->srow, a->num_srow, group,
+            alpha, beta);
         GKOMI_CHECK_LAUNCH("csr_spmv_balanced");
         return GKOMI_SUCCESS;
     }
@@ -432,30 +452,50 @@ using namespace gkomi;
 
 extern "C" {
 
-int64_t gkomi_csr_lb_chunk(int32_t value_bytes)
-{
-    (void)value_bytes;
-    return kLbChunk;
-}
-
 int64_t gkomi_csr_strategy_work_bytes(int32_t strategy, int64_t n_rows, int64_t nnz)
 {
     if (strategy != GKOMI_CSR_MERGE_PATH || n_rows < 0 || nnz < 0) return 0;
     return static_cast<int64_t>(merge_work_bytes(n_rows, nnz));
 }
 
+// Csr::load_balance::clac_size (include/ginkgo/core/matrix/csr.hpp:475-505),
+// as the reference's HIP-platform build compiles it.
+int64_t gkomi_csr_lb_clac_size(int64_t nnz, int64_t nwarps, int32_t warp_size,
+                               int32_t cuda_strategy, int32_t intel_strategy)
+{
+    if (warp_size <= 0 || nnz < 0 || nwarps < 0) return 0;
+    int64_t multiple = 8;
+    if (nnz >= 200000000) {
+        multiple = 2048;
+    } else if (nnz >= 20000000) {
+        multiple = 512;
+    } else if (nnz >= 2000000) {
+        multiple = 128;
+    } else if (nnz >= 200000) {
+        multiple = 32;
+    }
+    if (intel_strategy) {
+        multiple = nnz >= 200000000 ? 256 : (nnz >= 20000000 ? 32 : 8);
+    }
+    if (!cuda_strategy) {
+        multiple = nnz >= 10000000 ? 64 : (nnz >= 1000000 ? 16 : 8);
+    }
+    return std::min(ceildiv(nnz, warp_size), nwarps * multiple);
+}
+
 int gkomi_csr_lb_partition(const gkomi_ctx* ctx, int64_t n_rows, const int32_t* row_ptrs,
-                           int64_t nnz, int64_t chunk, int32_t* srow, int64_t num_blocks)
+                           int64_t nnz, int32_t warp_size, int32_t* srow, int64_t nwarps)
 {
     if (int rc = enter(ctx)) return rc;
-    GKOMI_REQUIRE(n_rows >= 0 && nnz >= 0 && chunk > 0, GKOMI_INVALID_ARGUMENT,
-                  "partition: bad size");
-    GKOMI_REQUIRE(num_blocks == std::max<int64_t>(1, ceildiv(nnz, chunk)),
-                  GKOMI_DIMENSION_MISMATCH, "partition: num_blocks != max(1, ceil(nnz/chunk))");
-    GKOMI_REQUIRE(srow != nullptr && (n_rows == 0 || row_ptrs != nullptr),
-                  GKOMI_INVALID_ARGUMENT, "partition: null pointer");
-    lb_partition_kernel<<<grid_for(num_blocks + 1, kBlock), kBlock, 0, stream_of(ctx)>>>(
-        n_rows, row_ptrs, chunk, srow, num_blocks);
+    GKOMI_REQUIRE(n_rows >= 0 && nnz >= 0 && nwarps >= 0 && warp_size > 0,
+                  GKOMI_INVALID_ARGUMENT, "partition: bad size");
+    GKOMI_REQUIRE(n_rows < (int64_t{1} << 31) && nnz < (int64_t{1} << 31),
+                  GKOMI_INVALID_ARGUMENT, "partition: sizes exceed the int32 index range");
+    if (nwarps == 0) return GKOMI_SUCCESS;
+    GKOMI_REQUIRE(srow != nullptr && row_ptrs != nullptr, GKOMI_INVALID_ARGUMENT,
+                  "partition: null pointer");
+    lb_partition_kernel<<<grid_for(nwarps, kBlock), kBlock, 0, stream_of(ctx)>>>(
+        n_rows, row_ptrs, nnz, warp_size, srow, nwarps);
     GKOMI_CHECK_LAUNCH("lb_partition_kernel");
     return GKOMI_SUCCESS;
 }

@@ -171,15 +171,18 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-template <typename T, bool Adv>
+// Matrix values TA, input TB, output TC; the sum is formed in acc_t (the
+// highest precision of the three), as the reference's mixed kernels do.
+template <typename TA, typename TB, typename TC, bool Adv>
 __global__ void __launch_bounds__(kBlock)
     sellp_spmv_kernel(int64_t n_rows, int64_t nrhs, int64_t slice_size,
                       const int64_t* __restrict__ slice_sets,
                       const int64_t* __restrict__ slice_lengths,
-                      const int32_t* __restrict__ s_cols, const T* __restrict__ s_vals,
-                      const T* __restrict__ b, int64_t bs, T* __restrict__ c, int64_t cs,
-                      const T* __restrict__ alpha, const T* __restrict__ beta)
+                      const int32_t* __restrict__ s_cols, const TA* __restrict__ s_vals,
+                      const TB* __restrict__ b, int64_t bs, TC* __restrict__ c, int64_t cs,
+                      const TA* __restrict__ alpha, const TC* __restrict__ beta)
 {
+    using Acc = acc_t<TA, TB, TC>;
     const int64_t nb = gridDim.x;
     const int64_t row_blocks = ceildiv(n_rows, kBlock);
     for (int64_t blk = xcd_block(blockIdx.x, gridDim.x); blk < row_blocks; blk += nb) {
@@ -189,12 +192,13 @@ __global__ void __launch_bounds__(kBlock)
         const int64_t base = slice_sets[s] * slice_size + (r - s * slice_size);
         const int64_t slen = slice_lengths[s];
         for (int64_t j = 0; j < nrhs; ++j) {
-            T sum = 0;
+            Acc sum = 0;
             for (int64_t k = 0; k < slen; ++k) {
                 const int64_t idx = base + k * slice_size;
                 const int32_t col = s_cols[idx];
                 if (col != -1) {
-                    sum += s_vals[idx] * b[static_cast<int64_t>(col) * bs + j];
+                    sum += static_cast<Acc>(s_vals[idx]) *
+                           static_cast<Acc>(b[static_cast<int64_t>(col) * bs + j]);
                 }
             }
             store_result<Adv>(c + r * cs + j, sum, alpha, beta);
@@ -281,24 +285,26 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-template <typename T, bool Adv>
+template <typename TA, typename TB, typename TC, bool Adv>
 __global__ void __launch_bounds__(kBlock)
     ell_spmv_kernel(int64_t n_rows, int64_t nrhs, int64_t width, int64_t stride_e,
-                    const int32_t* __restrict__ e_cols, const T* __restrict__ e_vals,
-                    const T* __restrict__ b, int64_t bs, T* __restrict__ c, int64_t cs,
-                    const T* __restrict__ alpha, const T* __restrict__ beta)
+                    const int32_t* __restrict__ e_cols, const TA* __restrict__ e_vals,
+                    const TB* __restrict__ b, int64_t bs, TC* __restrict__ c, int64_t cs,
+                    const TA* __restrict__ alpha, const TC* __restrict__ beta)
 {
+    using Acc = acc_t<TA, TB, TC>;
     const int64_t nb = gridDim.x;
     const int64_t row_blocks = ceildiv(n_rows, kBlock);
     for (int64_t blk = xcd_block(blockIdx.x, gridDim.x); blk < row_blocks; blk += nb) {
         const int64_t r = blk * kBlock + threadIdx.x;
         if (r >= n_rows) continue;
         for (int64_t j = 0; j < nrhs; ++j) {
-            T sum = 0;
+            Acc sum = 0;
             for (int64_t k = 0; k < width; ++k) {
                 const int32_t col = e_cols[r + k * stride_e];
                 if (col != -1) {
-                    sum += e_vals[r + k * stride_e] * b[static_cast<int64_t>(col) * bs + j];
+                    sum += static_cast<Acc>(e_vals[r + k * stride_e]) *
+                           static_cast<Acc>(b[static_cast<int64_t>(col) * bs + j]);
                 }
             }
             store_result<Adv>(c + r * cs + j, sum, alpha, beta);
@@ -422,9 +428,9 @@ int check_sellp(const gkomi_sellp* a)
     return GKOMI_SUCCESS;
 }
 
-template <typename T, bool Adv>
-int sellp_spmv_impl(const gkomi_ctx* ctx, const T* alpha, const gkomi_sellp* a, int64_t nrhs,
-                    const T* b, int64_t bs, const T* beta, T* c, int64_t cs)
+template <typename TA, typename TB, typename TC, bool Adv>
+int sellp_spmv_impl(const gkomi_ctx* ctx, const TA* alpha, const gkomi_sellp* a, int64_t nrhs,
+                    const TB* b, int64_t bs, const TC* beta, TC* c, int64_t cs)
 {
     if (int rc = enter(ctx)) return rc;
     if (int rc = check_sellp(a)) return rc;
@@ -433,9 +439,10 @@ int sellp_spmv_impl(const gkomi_ctx* ctx, const T* alpha, const gkomi_sellp* a, 
                   "SELL-P spmv: bad strides");
     GKOMI_REQUIRE(b && c, GKOMI_INVALID_ARGUMENT, "SELL-P spmv: null vector");
     GKOMI_REQUIRE(!Adv || (alpha && beta), GKOMI_INVALID_ARGUMENT, "SELL-P: null alpha/beta");
-    sellp_spmv_kernel<T, Adv><<<grid_for(a->n_rows, kBlock, 32), kBlock, 0, stream_of(ctx)>>>(
-        a->n_rows, nrhs, a->slice_size, a->slice_sets, a->slice_lengths, a->col_idxs,
-        static_cast<const T*>(a->values), b, bs, c, cs, alpha, beta);
+    sellp_spmv_kernel<TA, TB, TC, Adv>
+        <<<grid_for(a->n_rows, kBlock, 32), kBlock, 0, stream_of(ctx)>>>(
+            a->n_rows, nrhs, a->slice_size, a->slice_sets, a->slice_lengths, a->col_idxs,
+            static_cast<const TA*>(a->values), b, bs, c, cs, alpha, beta);
     GKOMI_CHECK_LAUNCH("sellp_spmv_kernel");
     return GKOMI_SUCCESS;
 }
@@ -529,9 +536,9 @@ int csr_to_hybrid_impl(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t width, 
     return GKOMI_SUCCESS;
 }
 
-template <typename T, bool Adv>
-int ell_spmv_impl(const gkomi_ctx* ctx, const T* alpha, const gkomi_ell* a, int64_t nrhs,
-                  const T* b, int64_t bs, const T* beta, T* c, int64_t cs)
+template <typename TA, typename TB, typename TC, bool Adv>
+int ell_spmv_impl(const gkomi_ctx* ctx, const TA* alpha, const gkomi_ell* a, int64_t nrhs,
+                  const TB* b, int64_t bs, const TC* beta, TC* c, int64_t cs)
 {
     if (int rc = enter(ctx)) return rc;
     GKOMI_REQUIRE(a != nullptr && a->n_rows >= 0 && a->width >= 0 && a->stride >= a->n_rows,
@@ -542,9 +549,10 @@ int ell_spmv_impl(const gkomi_ctx* ctx, const T* alpha, const gkomi_ell* a, int6
     GKOMI_REQUIRE(c && (a->width == 0 || (b && a->col_idxs && a->values)),
                   GKOMI_INVALID_ARGUMENT, "ELL spmv: null pointer");
     GKOMI_REQUIRE(!Adv || (alpha && beta), GKOMI_INVALID_ARGUMENT, "ELL: null alpha/beta");
-    ell_spmv_kernel<T, Adv><<<grid_for(a->n_rows, kBlock, 32), kBlock, 0, stream_of(ctx)>>>(
-        a->n_rows, nrhs, a->width, a->stride, a->col_idxs, static_cast<const T*>(a->values), b,
-        bs, c, cs, alpha, beta);
+    ell_spmv_kernel<TA, TB, TC, Adv>
+        <<<grid_for(a->n_rows, kBlock, 32), kBlock, 0, stream_of(ctx)>>>(
+            a->n_rows, nrhs, a->width, a->stride, a->col_idxs,
+            static_cast<const TA*>(a->values), b, bs, c, cs, alpha, beta);
     GKOMI_CHECK_LAUNCH("ell_spmv_kernel");
     return GKOMI_SUCCESS;
 }
@@ -659,27 +667,27 @@ int gkomi_csr_to_sellp_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t sli
 int gkomi_sellp_spmv_f64(const gkomi_ctx* ctx, const gkomi_sellp* a, int64_t nrhs,
                          const double* b, int64_t bs, double* c, int64_t cs)
 {
-    return sellp_spmv_impl<double, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
+    return sellp_spmv_impl<double, double, double, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
 }
 
 int gkomi_sellp_spmv_f32(const gkomi_ctx* ctx, const gkomi_sellp* a, int64_t nrhs,
                          const float* b, int64_t bs, float* c, int64_t cs)
 {
-    return sellp_spmv_impl<float, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
+    return sellp_spmv_impl<float, float, float, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
 }
 
 int gkomi_sellp_advanced_spmv_f64(const gkomi_ctx* ctx, const double* alpha,
                                   const gkomi_sellp* a, int64_t nrhs, const double* b,
                                   int64_t bs, const double* beta, double* c, int64_t cs)
 {
-    return sellp_spmv_impl<double, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
+    return sellp_spmv_impl<double, double, double, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
 }
 
 int gkomi_sellp_advanced_spmv_f32(const gkomi_ctx* ctx, const float* alpha,
                                   const gkomi_sellp* a, int64_t nrhs, const float* b,
                                   int64_t bs, const float* beta, float* c, int64_t cs)
 {
-    return sellp_spmv_impl<float, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
+    return sellp_spmv_impl<float, float, float, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
 }
 
 int gkomi_csr_to_ell_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t width,
@@ -697,27 +705,27 @@ int gkomi_csr_to_ell_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t width
 int gkomi_ell_spmv_f64(const gkomi_ctx* ctx, const gkomi_ell* a, int64_t nrhs, const double* b,
                        int64_t bs, double* c, int64_t cs)
 {
-    return ell_spmv_impl<double, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
+    return ell_spmv_impl<double, double, double, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
 }
 
 int gkomi_ell_spmv_f32(const gkomi_ctx* ctx, const gkomi_ell* a, int64_t nrhs, const float* b,
                        int64_t bs, float* c, int64_t cs)
 {
-    return ell_spmv_impl<float, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
+    return ell_spmv_impl<float, float, float, false>(ctx, nullptr, a, nrhs, b, bs, nullptr, c, cs);
 }
 
 int gkomi_ell_advanced_spmv_f64(const gkomi_ctx* ctx, const double* alpha, const gkomi_ell* a,
                                 int64_t nrhs, const double* b, int64_t bs, const double* beta,
                                 double* c, int64_t cs)
 {
-    return ell_spmv_impl<double, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
+    return ell_spmv_impl<double, double, double, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
 }
 
 int gkomi_ell_advanced_spmv_f32(const gkomi_ctx* ctx, const float* alpha, const gkomi_ell* a,
                                 int64_t nrhs, const float* b, int64_t bs, const float* beta,
                                 float* c, int64_t cs)
 {
-    return ell_spmv_impl<float, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
+    return ell_spmv_impl<float, float, float, true>(ctx, alpha, a, nrhs, b, bs, beta, c, cs);
 }
 
 size_t gkomi_csr_transpose_workspace_bytes(int64_t n_rows, int64_t n_cols, int64_t nnz)
@@ -735,6 +743,68 @@ int gkomi_csr_transpose_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int32_t* t
                             int32_t* t_ci, float* t_v, void* ws, size_t wsb)
 {
     return transpose_impl(ctx, a, t_rp, t_ci, t_v, ws, wsb);
+}
+
+
+// Products with mixed value types (reference/matrix/sellp_kernels.cpp:26-95,
+// reference/matrix/ell_kernels.cpp:29-110, instantiated for every mixed
+// value type): a_type, b_type, c_type are GKOMI_VALUE_F64 / _F32.
+int gkomi_sellp_spmv_mixed(const gkomi_ctx* ctx, int32_t a_type, const gkomi_sellp* a,
+                           int64_t nrhs, int32_t b_type, const void* b, int64_t b_stride,
+                           int32_t c_type, void* c, int64_t c_stride)
+{
+    return dispatch_value_types(a_type, b_type, c_type, [&](auto ta, auto tb, auto tc) {
+        using TA = decltype(ta);
+        using TB = decltype(tb);
+        using TC = decltype(tc);
+        return sellp_spmv_impl<TA, TB, TC, false>(ctx, nullptr, a, nrhs,
+                                                  static_cast<const TB*>(b), b_stride, nullptr,
+                                                  static_cast<TC*>(c), c_stride);
+    });
+}
+
+int gkomi_sellp_advanced_spmv_mixed(const gkomi_ctx* ctx, const void* alpha, int32_t a_type,
+                                    const gkomi_sellp* a, int64_t nrhs, int32_t b_type,
+                                    const void* b, int64_t b_stride, const void* beta,
+                                    int32_t c_type, void* c, int64_t c_stride)
+{
+    return dispatch_value_types(a_type, b_type, c_type, [&](auto ta, auto tb, auto tc) {
+        using TA = decltype(ta);
+        using TB = decltype(tb);
+        using TC = decltype(tc);
+        return sellp_spmv_impl<TA, TB, TC, true>(
+            ctx, static_cast<const TA*>(alpha), a, nrhs, static_cast<const TB*>(b), b_stride,
+            static_cast<const TC*>(beta), static_cast<TC*>(c), c_stride);
+    });
+}
+
+int gkomi_ell_spmv_mixed(const gkomi_ctx* ctx, int32_t a_type, const gkomi_ell* a, int64_t nrhs,
+                         int32_t b_type, const void* b, int64_t b_stride, int32_t c_type,
+                         void* c, int64_t c_stride)
+{
+    return dispatch_value_types(a_type, b_type, c_type, [&](auto ta, auto tb, auto tc) {
+        using TA = decltype(ta);
+        using TB = decltype(tb);
+        using TC = decltype(tc);
+        return ell_spmv_impl<TA, TB, TC, false>(ctx, nullptr, a, nrhs,
+                                                static_cast<const TB*>(b), b_stride, nullptr,
+                                                static_cast<TC*>(c), c_stride);
+    });
+}
+
+int gkomi_ell_advanced_spmv_mixed(const gkomi_ctx* ctx, const void* alpha, int32_t a_type,
+                                  const gkomi_ell* a, int64_t nrhs, int32_t b_type,
+                                  const void* b, int64_t b_stride, const void* beta,
+                                  int32_t c_type, void* c, int64_t c_stride)
+{
+    return dispatch_value_types(a_type, b_type, c_type, [&](auto ta, auto tb, auto tc) {
+        using TA = decltype(ta);
+        using TB = decltype(tb);
+        using TC = decltype(tc);
+        return ell_spmv_impl<TA, TB, TC, true>(
+            ctx, static_cast<const TA*>(alpha), a, nrhs, static_cast<const TB*>(b), b_stride,
+            static_cast<const TC*>(beta), static_cast<TC*>(c), c_stride);
+    });
 }
 
 }  // extern "C"

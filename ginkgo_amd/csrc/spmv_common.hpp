@@ -1,6 +1,8 @@
 // Constants and the result store shared by the SpMV kernels.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace gkomi {
@@ -23,19 +25,33 @@ constexpr int kLbShortRow = 32;
 //               coo_kernels.cpp:61-101).
 enum spmv_mode : int { kPlain = 0, kAdvanced = 1, kAccumulate = 2 };
 
-template <int Mode, typename T>
-__device__ __forceinline__ void store_result(T* dst, T sum, const T* alpha, const T* beta)
+// Mixed value types (the reference's arithmetic_type rule,
+// reference/matrix/csr_kernels.cpp:47-121): `sum` is in the highest
+// precision Acc of the operands, alpha has the matrix's type, beta and the
+// destination the output's; the result is rounded to the output type once.
+// With one type throughout this is the plain T arithmetic.
+template <int Mode, typename Acc, typename TC, typename TS>
+__device__ __forceinline__ void store_result(TC* dst, Acc sum, const TS* alpha, const TC* beta)
 {
     if constexpr (Mode == kAdvanced) {
-        const T bt = beta[0];
-        const T r = alpha[0] * sum;
-        *dst = (bt == T(0)) ? r : r + bt * *dst;
+        const Acc bt = static_cast<Acc>(beta[0]);
+        const Acc r = static_cast<Acc>(alpha[0]) * sum;
+        *dst = static_cast<TC>((bt == Acc(0)) ? r : r + bt * static_cast<Acc>(*dst));
     } else if constexpr (Mode == kAccumulate) {
-        *dst += alpha ? alpha[0] * sum : sum;
+        *dst = static_cast<TC>(static_cast<Acc>(*dst) +
+                               (alpha ? static_cast<Acc>(alpha[0]) * sum : sum));
     } else {
-        *dst = sum;
+        *dst = static_cast<TC>(sum);
     }
 }
+
+// The arithmetic type of a product over matrix values TA, input TB and
+// output TC: double when any of them is double.
+template <typename TA, typename TB, typename TC>
+using acc_t = std::conditional_t<std::is_same<TA, double>::value ||
+                                     std::is_same<TB, double>::value ||
+                                     std::is_same<TC, double>::value,
+                                 double, float>;
 
 // Rows per workgroup tile of the classical (sub-wave) kernels.
 template <int SW, int RPG>
@@ -54,12 +70,13 @@ constexpr int64_t subwave_tile_rows = int64_t{kWave / SW} * RPG * kWavesPerBlock
 // column; `row_op(row, sum)` runs in lane 0 of the group for every row below
 // n_rows.  With Compact the columns come from the 2-byte offsets `co`
 // (column = row + co[p]) instead of `ci`.
-template <typename T, int SW, int RPG, bool Compact = false, typename Gather, typename RowOp>
+template <typename T, int SW, int RPG, bool Compact = false, typename Gather, typename RowOp,
+          typename TV>
 __device__ __forceinline__ void subwave_tile(int64_t n_rows, int64_t tile,
                                              const int32_t* __restrict__ rp,
                                              const int32_t* __restrict__ ci,
                                              const int16_t* __restrict__ co,
-                                             const T* __restrict__ v, Gather gather,
+                                             const TV* __restrict__ v, Gather gather,
                                              RowOp row_op)
 {
     constexpr int kGroups = kWave / SW;
@@ -103,7 +120,7 @@ __device__ __forceinline__ void subwave_tile(int64_t n_rows, int64_t tile,
             } else {
                 col[k] = ci[p];
             }
-            val[k] = v[p];
+            val[k] = static_cast<T>(v[p]);
             pos[k] = act[k] ? pos[k] + SW : pos[k];
         }
         T xv[RPG];

@@ -14,6 +14,12 @@ from . import _capi as C
 _VT = {torch.float64: "f64", torch.float32: "f32"}
 
 
+def value_code(dtype):
+    """GKOMI_VALUE_F64 / GKOMI_VALUE_F32 of a torch dtype."""
+    suffix(dtype)
+    return C.VALUE_F64 if dtype == torch.float64 else C.VALUE_F32
+
+
 def suffix(dtype):
     try:
         return _VT[dtype]
@@ -49,6 +55,52 @@ def scalar(value, dtype, device):
     return torch.full((1,), float(value), dtype=dtype, device=device)
 
 
+def check_operand(t, name, dtype=None, device=None, rows=None, cols=None):
+    """Rejects, before any launch, a dense operand the C ABI cannot take as
+    (pointer, row stride): not a CUDA tensor of `device`, another value type
+    than `dtype`, more than two dimensions, columns that are not unit-stride,
+    overlapping rows, or rows/columns other than expected."""
+    if not torch.is_tensor(t):
+        raise C.InvalidArgument(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.device.type != "cuda" or (device is not None and t.device != torch.device(device)):
+        raise C.InvalidArgument(f"{name}: tensor on {t.device}, expected {device}")
+    if dtype is not None and t.dtype != dtype:
+        raise C.NotSupported(f"{name}: value type {t.dtype}, expected {dtype}")
+    if t.dim() not in (1, 2):
+        raise C.DimensionMismatch(f"{name}: expected a vector or a 2-D block, got {t.dim()}-D")
+    if t.dim() == 1 and t.numel() > 1 and t.stride(0) != 1:
+        raise C.NotSupported(f"{name}: a vector must be contiguous (stride {t.stride(0)})")
+    if t.dim() == 2:
+        if t.shape[1] > 1 and t.stride(1) != 1:
+            raise C.NotSupported(f"{name}: columns must be unit-stride (stride {t.stride(1)})")
+        if t.shape[0] > 1 and t.stride(0) < t.shape[1]:
+            raise C.NotSupported(f"{name}: rows overlap (row stride {t.stride(0)})")
+    if rows is not None and t.shape[0] != rows:
+        raise C.DimensionMismatch(f"{name}: {t.shape[0]} rows, expected {rows}")
+    if cols is not None and (t.shape[1] if t.dim() == 2 else 1) != cols:
+        raise C.DimensionMismatch(f"{name}: wrong number of columns, expected {cols}")
+    return t
+
+
+def _cols(t):
+    return t.shape[1] if t.dim() == 2 else 1
+
+
+def _check_scalar(s, name, dtype, device):
+    check_operand(s, name, dtype, device)
+    if s.numel() < 1:
+        raise C.DimensionMismatch(f"{name}: empty scalar")
+    return s
+
+
+def default_lb_warps(device):
+    """The HipExecutor's warp count, compute units x 4 (hip/base/executor.hip.cpp:234-238)."""
+    dev = torch.device(device)
+    info = C.DeviceInfo()
+    C.call("gkomi_get_device_info", dev.index if dev.index is not None else 0, ctypes.byref(info))
+    return int(info.num_cus) * 4
+
+
 class DeviceCsr:
     """Csr<V, int32> on the device with its SpMV strategy.
 
@@ -58,7 +110,8 @@ class DeviceCsr:
     csr.hpp:606-673)."""
 
     def __init__(self, row_ptrs, col_idxs, values, n_cols=None, strategy="automatical",
-                 subwave=0, device="cuda"):
+                 subwave=0, device="cuda", lb_nwarps=None, lb_warp_size=64,
+                 lb_cuda_strategy=False):
         dev = torch.device(device)
         self.row_ptrs = torch.as_tensor(np.asarray(row_ptrs, dtype=np.int32)).to(dev) \
             if not torch.is_tensor(row_ptrs) else row_ptrs.to(dev, torch.int32)
@@ -71,24 +124,32 @@ class DeviceCsr:
         self.n_rows = self.row_ptrs.numel() - 1
         self.n_cols = self.n_rows if n_cols is None else int(n_cols)
         self.nnz = self.col_idxs.numel()
-        self.device = dev
+        self.device = self.values.device
+        dev = self.device
         rp = self.row_ptrs.cpu().numpy() if strategy == "automatical" else None
         if strategy == "automatical":
+            # the C++ Csr's automatical rule: classical up to the reference's
+            # AMD row limit of 768 entries when the longest row is within 8x
+            # the mean plus 64, load_balance otherwise
             lens = np.diff(rp) if self.n_rows > 0 else np.zeros(1)
             mx = int(lens.max()) if lens.size else 0
             mean = float(lens.mean()) if lens.size else 0.0
-            strategy = "classical" if (mx <= 64 and mx <= 4 * mean + 4) else "load_balance"
+            strategy = "classical" if (mx <= 768 and mx <= 8 * mean + 64) else "load_balance"
         self.strategy = strategy
         self.subwave = int(subwave)
         self.srow = None
-        self.srow_blocks = 0
+        self.num_srow = 0
         if strategy == "load_balance":
-            chunk = int(C.lib().gkomi_csr_lb_chunk(self.values.element_size()))
-            self.srow_blocks = max(1, -(-self.nnz // chunk))
-            self.srow = torch.empty(self.srow_blocks + 1, dtype=torch.int32, device=dev)
+            # Csr::load_balance with the executor's warps (HipExecutor form:
+            # warp size 64, no cuda_strategy) unless given
+            nw = default_lb_warps(dev) if lb_nwarps is None else int(lb_nwarps)
+            self.lb = (nw, int(lb_warp_size), bool(lb_cuda_strategy))
+            self.num_srow = int(C.lib().gkomi_csr_lb_clac_size(
+                self.nnz, nw, int(lb_warp_size), int(bool(lb_cuda_strategy)), 0))
+            self.srow = torch.empty(max(self.num_srow, 1), dtype=torch.int32, device=dev)
             c = ctx(dev)
             C.call("gkomi_csr_lb_partition", ctypes.byref(c), self.n_rows, ptr(self.row_ptrs),
-                   self.nnz, chunk, ptr(self.srow), self.srow_blocks)
+                   self.nnz, int(lb_warp_size), ptr(self.srow), self.num_srow)
         elif strategy == "merge_path":
             wb = int(C.lib().gkomi_csr_strategy_work_bytes(C.CSR_MERGE_PATH, self.n_rows,
                                                            self.nnz))
@@ -124,7 +185,7 @@ class DeviceCsr:
                       "merge_path": C.CSR_MERGE_PATH}[self.strategy]
         d.subwave = self.subwave
         d.srow = ptr(self.srow).value if self.srow is not None else None
-        d.srow_blocks = self.srow_blocks
+        d.num_srow = self.num_srow
         d.col_offsets = ptr(self.col_offsets).value if getattr(self, "col_offsets", None) \
             is not None else None
         work = getattr(self, "work", None)
@@ -133,26 +194,49 @@ class DeviceCsr:
         return d
 
     def spmv(self, b, out=None, stream=None):
-        b2 = as_2d(b)
+        """out = A b.  b and out may be float64 or float32 independently of
+        the matrix: a mixed product computes in the highest precision
+        (gkomi_csr_spmv_mixed), as the reference's Csr::apply does."""
+        check_operand(b, "b", None, self.device, rows=self.n_cols)
+        suffix(b.dtype)
         if out is None:
-            out = torch.empty((self.n_rows, b2.shape[1]), dtype=self.dtype, device=self.device)
-        o2 = as_2d(out)
-        if b2.shape[0] != self.n_cols or o2.shape[0] != self.n_rows or o2.shape[1] != b2.shape[1]:
-            raise C.DimensionMismatch("spmv: operand sizes do not conform")
+            out = torch.empty((self.n_rows, _cols(b)), dtype=b.dtype, device=self.device)
+        check_operand(out, "out", None, self.device, rows=self.n_rows, cols=_cols(b))
+        suffix(out.dtype)
+        b2, o2 = as_2d(b), as_2d(out)
         c = ctx(self.device, stream)
-        C.call(f"gkomi_csr_spmv_{suffix(self.dtype)}", ctypes.byref(c), ctypes.byref(self.desc),
-               b2.shape[1], ptr(b2), b2.stride(0), ptr(o2), o2.stride(0))
+        if b.dtype == self.dtype and out.dtype == self.dtype:
+            C.call(f"gkomi_csr_spmv_{suffix(self.dtype)}", ctypes.byref(c),
+                   ctypes.byref(self.desc), b2.shape[1], ptr(b2), b2.stride(0), ptr(o2),
+                   o2.stride(0))
+        else:
+            C.call("gkomi_csr_spmv_mixed", ctypes.byref(c), value_code(self.dtype),
+                   ctypes.byref(self.desc), b2.shape[1], value_code(b.dtype), ptr(b2),
+                   b2.stride(0), value_code(out.dtype), ptr(o2), o2.stride(0))
         return out
 
     def advanced_spmv(self, alpha, b, beta, c_vec, stream=None):
-        b2 = as_2d(b)
-        c2 = as_2d(c_vec)
+        """c = alpha A b + beta c; alpha in the matrix's value type, beta in
+        c's (mixed types as in spmv)."""
+        check_operand(b, "b", None, self.device, rows=self.n_cols)
+        check_operand(c_vec, "c", None, self.device, rows=self.n_rows, cols=_cols(b))
+        suffix(b.dtype)
+        suffix(c_vec.dtype)
         a = alpha if torch.is_tensor(alpha) else scalar(alpha, self.dtype, self.device)
-        bt = beta if torch.is_tensor(beta) else scalar(beta, self.dtype, self.device)
+        bt = beta if torch.is_tensor(beta) else scalar(beta, c_vec.dtype, self.device)
+        _check_scalar(a, "alpha", self.dtype, self.device)
+        _check_scalar(bt, "beta", c_vec.dtype, self.device)
+        b2, c2 = as_2d(b), as_2d(c_vec)
         c = ctx(self.device, stream)
-        C.call(f"gkomi_csr_advanced_spmv_{suffix(self.dtype)}", ctypes.byref(c), ptr(a),
-               ctypes.byref(self.desc), b2.shape[1], ptr(b2), b2.stride(0), ptr(bt), ptr(c2),
-               c2.stride(0))
+        if b.dtype == self.dtype and c_vec.dtype == self.dtype:
+            C.call(f"gkomi_csr_advanced_spmv_{suffix(self.dtype)}", ctypes.byref(c), ptr(a),
+                   ctypes.byref(self.desc), b2.shape[1], ptr(b2), b2.stride(0), ptr(bt),
+                   ptr(c2), c2.stride(0))
+        else:
+            C.call("gkomi_csr_advanced_spmv_mixed", ctypes.byref(c), ptr(a),
+                   value_code(self.dtype), ctypes.byref(self.desc), b2.shape[1],
+                   value_code(b.dtype), ptr(b2), b2.stride(0), ptr(bt),
+                   value_code(c_vec.dtype), ptr(c2), c2.stride(0))
         return c_vec
 
     def max_row_nnz(self):
@@ -215,25 +299,12 @@ class DeviceSellp:
         self.desc = d
 
     def spmv(self, b, out=None, stream=None):
-        b2 = as_2d(b)
-        if out is None:
-            out = torch.empty((self.n_rows, b2.shape[1]), dtype=self.dtype, device=self.device)
-        o2 = as_2d(out)
-        c = ctx(self.device, stream)
-        C.call(f"gkomi_sellp_spmv_{suffix(self.dtype)}", ctypes.byref(c), ctypes.byref(self.desc),
-               b2.shape[1], ptr(b2), b2.stride(0), ptr(o2), o2.stride(0))
-        return out
+        """out = A b, mixed value types as DeviceCsr.spmv."""
+        return _format_spmv(self, "sellp", b, out, stream)
 
-    def advanced_spmv(self, alpha, b, beta, c_vec):
-        b2 = as_2d(b)
-        c2 = as_2d(c_vec)
-        a = scalar(alpha, self.dtype, self.device)
-        bt = scalar(beta, self.dtype, self.device)
-        c = ctx(self.device)
-        C.call(f"gkomi_sellp_advanced_spmv_{suffix(self.dtype)}", ctypes.byref(c), ptr(a),
-               ctypes.byref(self.desc), b2.shape[1], ptr(b2), b2.stride(0), ptr(bt), ptr(c2),
-               c2.stride(0))
-        return c_vec
+    def advanced_spmv(self, alpha, b, beta, c_vec, stream=None):
+        """c = alpha A b + beta c (beta == 0 ignores c, NaN included)."""
+        return _format_advanced_spmv(self, "sellp", alpha, b, beta, c_vec, stream)
 
 
 class DeviceEll:
@@ -257,28 +328,61 @@ class DeviceEll:
         self.desc = d
 
     def spmv(self, b, out=None, stream=None):
-        b2 = as_2d(b)
-        if out is None:
-            out = torch.empty((self.n_rows, b2.shape[1]), dtype=self.dtype, device=self.device)
-        o2 = as_2d(out)
-        c = ctx(self.device, stream)
-        C.call(f"gkomi_ell_spmv_{suffix(self.dtype)}", ctypes.byref(c), ctypes.byref(self.desc),
-               b2.shape[1], ptr(b2), b2.stride(0), ptr(o2), o2.stride(0))
-        return out
+        """out = A b, mixed value types as DeviceCsr.spmv."""
+        return _format_spmv(self, "ell", b, out, stream)
 
-    def advanced_spmv(self, alpha, b, beta, c_vec):
-        b2 = as_2d(b)
-        c2 = as_2d(c_vec)
-        a = scalar(alpha, self.dtype, self.device)
-        bt = scalar(beta, self.dtype, self.device)
-        c = ctx(self.device)
-        C.call(f"gkomi_ell_advanced_spmv_{suffix(self.dtype)}", ctypes.byref(c), ptr(a),
-               ctypes.byref(self.desc), b2.shape[1], ptr(b2), b2.stride(0), ptr(bt), ptr(c2),
+    def advanced_spmv(self, alpha, b, beta, c_vec, stream=None):
+        """c = alpha A b + beta c (beta == 0 ignores c, NaN included)."""
+        return _format_advanced_spmv(self, "ell", alpha, b, beta, c_vec, stream)
+
+
+
+def _format_spmv(m, fmt, b, out, stream):
+    check_operand(b, "b", None, m.device, rows=m.n_cols)
+    suffix(b.dtype)
+    if out is None:
+        out = torch.empty((m.n_rows, _cols(b)), dtype=b.dtype, device=m.device)
+    check_operand(out, "out", None, m.device, rows=m.n_rows, cols=_cols(b))
+    suffix(out.dtype)
+    b2, o2 = as_2d(b), as_2d(out)
+    c = ctx(m.device, stream)
+    if b.dtype == m.dtype and out.dtype == m.dtype:
+        C.call(f"gkomi_{fmt}_spmv_{suffix(m.dtype)}", ctypes.byref(c), ctypes.byref(m.desc),
+               b2.shape[1], ptr(b2), b2.stride(0), ptr(o2), o2.stride(0))
+    else:
+        C.call(f"gkomi_{fmt}_spmv_mixed", ctypes.byref(c), value_code(m.dtype),
+               ctypes.byref(m.desc), b2.shape[1], value_code(b.dtype), ptr(b2), b2.stride(0),
+               value_code(out.dtype), ptr(o2), o2.stride(0))
+    return out
+
+
+def _format_advanced_spmv(m, fmt, alpha, b, beta, c_vec, stream):
+    check_operand(b, "b", None, m.device, rows=m.n_cols)
+    check_operand(c_vec, "c", None, m.device, rows=m.n_rows, cols=_cols(b))
+    suffix(b.dtype)
+    suffix(c_vec.dtype)
+    a = alpha if torch.is_tensor(alpha) else scalar(alpha, m.dtype, m.device)
+    bt = beta if torch.is_tensor(beta) else scalar(beta, c_vec.dtype, m.device)
+    _check_scalar(a, "alpha", m.dtype, m.device)
+    _check_scalar(bt, "beta", c_vec.dtype, m.device)
+    b2, c2 = as_2d(b), as_2d(c_vec)
+    c = ctx(m.device, stream)
+    if b.dtype == m.dtype and c_vec.dtype == m.dtype:
+        C.call(f"gkomi_{fmt}_advanced_spmv_{suffix(m.dtype)}", ctypes.byref(c), ptr(a),
+               ctypes.byref(m.desc), b2.shape[1], ptr(b2), b2.stride(0), ptr(bt), ptr(c2),
                c2.stride(0))
-        return c_vec
+    else:
+        C.call(f"gkomi_{fmt}_advanced_spmv_mixed", ctypes.byref(c), ptr(a),
+               value_code(m.dtype), ctypes.byref(m.desc), b2.shape[1], value_code(b.dtype),
+               ptr(b2), b2.stride(0), ptr(bt), value_code(c_vec.dtype), ptr(c2),
+               c2.stride(0))
+    return c_vec
 
 
 def dot(x, y):
+    check_operand(x, "x")
+    suffix(x.dtype)
+    check_operand(y, "y", x.dtype, x.device, rows=x.shape[0], cols=_cols(x))
     x2 = as_2d(x)
     y2 = as_2d(y)
     out = torch.empty(x2.shape[1], dtype=x.dtype, device=x.device)
@@ -292,6 +396,9 @@ def dot(x, y):
 
 def copy(x, y):
     """y = x for device tensors of equal shape (Dense::copy_from)."""
+    check_operand(x, "x")
+    suffix(x.dtype)
+    check_operand(y, "y", x.dtype, x.device, rows=x.shape[0], cols=_cols(x))
     x2 = as_2d(x)
     y2 = as_2d(y)
     c = ctx(x.device)
@@ -301,6 +408,8 @@ def copy(x, y):
 
 
 def norm2(x):
+    check_operand(x, "x")
+    suffix(x.dtype)
     x2 = as_2d(x)
     out = torch.empty(x2.shape[1], dtype=x.dtype, device=x.device)
     wsb = C.lib().gkomi_reduction_workspace_bytes(x2.shape[0], x2.shape[1])

@@ -149,16 +149,19 @@ int gkomi_dense_copy_f64(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const do
 int gkomi_dense_copy_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const float* x,
                          int64_t x_stride, float* y, int64_t y_stride);
 
+/* Value-type codes of the mixed-precision products (*_mixed). */
+enum { GKOMI_VALUE_F64 = 0, GKOMI_VALUE_F32 = 1 };
+
 /* ------------------------------------------------------------------ CSR */
 
-/* csr::strategy_type (include/ginkgo/core/matrix/csr.hpp:176-699).
+/* Csr strategies (include/ginkgo/core/matrix/csr.hpp:176-699).
  * CLASSICAL: a sub-wavefront of S lanes per row, each lane group keeping R
  * rows in flight; `subwave` = S | (R << 8), S a power of two <= 64, R one of
- * 1, 2, 4, 8, either 0 = chosen from the mean row length.  LOAD_BALANCE: nnz-balanced workgroups over the
- * partition `srow` built by gkomi_csr_lb_partition. */
-/* Csr strategies (csr.hpp:233-699): classical (sub-wave rows), load_balance
- * (nnz-balanced partition `srow`), merge_path (merge-path partition of rows
- * and entries, csr.hpp:283-301). */
+ * 1, 2, 4, 8, either 0 = chosen from the mean row length.
+ * LOAD_BALANCE: the reference's load_balance partition `srow` (built by
+ * gkomi_csr_lb_partition, or by the reference's own load_balance::process);
+ * each workgroup takes consecutive warps of it and owns whole rows.
+ * MERGE_PATH: merge-path partition of rows and entries (csr.hpp:283-301). */
 enum { GKOMI_CSR_CLASSICAL = 0, GKOMI_CSR_LOAD_BALANCE = 1, GKOMI_CSR_MERGE_PATH = 2 };
 
 /* Matrix::Csr<V, int32> storage (csr.hpp:104-1690) plus its strategy data.
@@ -172,8 +175,12 @@ typedef struct {
     const void* values;
     int32_t strategy;
     int32_t subwave;
-    const int32_t* srow;     /* LOAD_BALANCE: srow_blocks + 1 row starts */
-    int64_t srow_blocks;
+    /* LOAD_BALANCE: the partition Csr::get_const_srow() with
+     * Csr::get_num_srow_elements() entries (csr.hpp:1651-1655), any
+     * non-decreasing row indices in [0, n_rows]; num_srow 0 = no partition
+     * (the matrix then has no entries, clac_size(0) = 0). */
+    const int32_t* srow;
+    int64_t num_srow;
     /* Optional (NULL = absent, no reference counterpart): col_idxs[p] - row
      * as int16 for every entry, built by gkomi_csr_col_offsets.  col_idxs
      * stays required; the CLASSICAL single right-hand-side SpMV and the fused
@@ -191,14 +198,19 @@ typedef struct {
  * for an n_rows x ? matrix with nnz entries (0 for CLASSICAL and LOAD_BALANCE). */
 int64_t gkomi_csr_strategy_work_bytes(int32_t strategy, int64_t n_rows, int64_t nnz);
 
-/* nnz handled by one workgroup of the LOAD_BALANCE kernel for value type V. */
-int64_t gkomi_csr_lb_chunk(int32_t value_bytes);
-/* Partition for LOAD_BALANCE (the role of load_balance::process / srow,
- * csr.hpp:417-468): srow[k] = first row whose start is >= k * chunk,
- * srow[num_blocks] = n_rows, num_blocks = max(1, ceil(nnz / chunk)).
- * Integer work, deterministic. */
+/* Csr::load_balance::clac_size (csr.hpp:475-505) as the reference's
+ * HIP-platform build compiles it: the number of srow elements for nnz
+ * entries.  A HipExecutor-built strategy has nwarps = compute units x 4 and
+ * warp_size 64, cuda_strategy 0 (hip/base/executor.hip.cpp:230-238,
+ * csr.hpp:379-382); intel_strategy is the "intel" strategy name. */
+int64_t gkomi_csr_lb_clac_size(int64_t nnz, int64_t nwarps, int32_t warp_size,
+                               int32_t cuda_strategy, int32_t intel_strategy);
+/* Csr::load_balance::process (csr.hpp:417-468) on the device: srow[w] for
+ * w < nwarps, bit-identical to the reference's host loops (srow[w] is the
+ * first row i with ceil(row_ptrs[i+1] / warp_size) * nwarps >
+ * w * ceil(nnz / warp_size)).  Integer work, deterministic. */
 int gkomi_csr_lb_partition(const gkomi_ctx* ctx, int64_t n_rows, const int32_t* row_ptrs,
-                           int64_t nnz, int64_t chunk, int32_t* srow, int64_t num_blocks);
+                           int64_t nnz, int32_t warp_size, int32_t* srow, int64_t nwarps);
 /* Compact column indices (no reference counterpart): out[p] = col_idxs[p] -
  * row(p) as int16 for every entry.  *fits (device int32) becomes 1 when every
  * offset lies in [-32768, 32767], else 0 (then `out` must not be used). */
@@ -223,6 +235,24 @@ int gkomi_csr_advanced_spmv_f64(const gkomi_ctx* ctx, const double* alpha, const
 int gkomi_csr_advanced_spmv_f32(const gkomi_ctx* ctx, const float* alpha, const gkomi_csr* a,
                                 int64_t nrhs, const float* b, int64_t b_stride,
                                 const float* beta, float* c, int64_t c_stride);
+
+/* csr::spmv / advanced_spmv with mixed value types (reference/matrix/
+ * csr_kernels.cpp:47-121, instantiated by GKO_INSTANTIATE_FOR_EACH_MIXED_VALUE_
+ * AND_INDEX_TYPE; reached from Csr::apply_impl through
+ * mixed_precision_dispatch_real_complex, core/matrix/csr.cpp:257-315): the
+ * matrix values (a_type), b (b_type) and c (c_type) are each f64 or f32
+ * (GKOMI_VALUE_*); every product and sum is formed in the highest of the three
+ * precisions and rounded into c once.  alpha has the matrix's type, beta
+ * c's.  Nothing is converted through temporaries.  With one type throughout
+ * this is gkomi_csr_(advanced_)spmv_f64/_f32 with the matrix's strategy; a
+ * mixed product uses the sub-wave kernel for every strategy. */
+int gkomi_csr_spmv_mixed(const gkomi_ctx* ctx, int32_t a_type, const gkomi_csr* a, int64_t nrhs,
+                         int32_t b_type, const void* b, int64_t b_stride, int32_t c_type,
+                         void* c, int64_t c_stride);
+int gkomi_csr_advanced_spmv_mixed(const gkomi_ctx* ctx, const void* alpha, int32_t a_type,
+                                  const gkomi_csr* a, int64_t nrhs, int32_t b_type,
+                                  const void* b, int64_t b_stride, const void* beta,
+                                  int32_t c_type, void* c, int64_t c_stride);
 
 /* csr::transpose (reference/matrix/csr_kernels.cpp:676-731): the CSR of A^T
  * with each row sorted by column, entries in the reference's stable order.
@@ -307,6 +337,25 @@ int gkomi_ell_advanced_spmv_f64(const gkomi_ctx* ctx, const double* alpha, const
 int gkomi_ell_advanced_spmv_f32(const gkomi_ctx* ctx, const float* alpha, const gkomi_ell* a,
                                 int64_t nrhs, const float* b, int64_t b_stride,
                                 const float* beta, float* c, int64_t c_stride);
+/* sellp::spmv / advanced_spmv (reference/matrix/sellp_kernels.cpp:26-95) and
+ * ell::spmv / advanced_spmv (reference/matrix/ell_kernels.cpp:29-110) with
+ * mixed value types, as gkomi_csr_spmv_mixed: computed in the highest
+ * precision of the matrix, b and c (KATs reference/test/matrix/
+ * sellp_kernels.cpp:68-79, 132-145). */
+int gkomi_sellp_spmv_mixed(const gkomi_ctx* ctx, int32_t a_type, const gkomi_sellp* a,
+                           int64_t nrhs, int32_t b_type, const void* b, int64_t b_stride,
+                           int32_t c_type, void* c, int64_t c_stride);
+int gkomi_sellp_advanced_spmv_mixed(const gkomi_ctx* ctx, const void* alpha, int32_t a_type,
+                                    const gkomi_sellp* a, int64_t nrhs, int32_t b_type,
+                                    const void* b, int64_t b_stride, const void* beta,
+                                    int32_t c_type, void* c, int64_t c_stride);
+int gkomi_ell_spmv_mixed(const gkomi_ctx* ctx, int32_t a_type, const gkomi_ell* a, int64_t nrhs,
+                         int32_t b_type, const void* b, int64_t b_stride, int32_t c_type,
+                         void* c, int64_t c_stride);
+int gkomi_ell_advanced_spmv_mixed(const gkomi_ctx* ctx, const void* alpha, int32_t a_type,
+                                  const gkomi_ell* a, int64_t nrhs, int32_t b_type,
+                                  const void* b, int64_t b_stride, const void* beta,
+                                  int32_t c_type, void* c, int64_t c_stride);
 
 /* ---------------------------------------------------------- COO, Hybrid */
 

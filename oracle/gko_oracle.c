@@ -71,6 +71,75 @@ void ora_sellp_compute_slice_sets(int64_t n_rows, const int32_t* row_ptrs,
     slice_sets[num_slices] = run;
 }
 
+/* Csr::load_balance::clac_size (include/ginkgo/core/matrix/csr.hpp:475-505)
+ * as the HIP-platform build compiles it (GINKGO_HIP_PLATFORM_HCC set): the
+ * CUDA multiples, overridden for the "intel" strategy name, overridden again
+ * for a strategy built without cuda_strategy. */
+int64_t ora_csr_lb_clac_size(int64_t nnz, int64_t nwarps, int32_t warp_size,
+                             int32_t cuda_strategy, int32_t intel)
+{
+    if (warp_size <= 0) {
+        return 0;
+    }
+    int64_t multiple = 8;
+    if (nnz >= (int64_t)2e8) {
+        multiple = 2048;
+    } else if (nnz >= (int64_t)2e7) {
+        multiple = 512;
+    } else if (nnz >= (int64_t)2e6) {
+        multiple = 128;
+    } else if (nnz >= (int64_t)2e5) {
+        multiple = 32;
+    }
+    if (intel) {
+        multiple = 8;
+        if (nnz >= (int64_t)2e8) {
+            multiple = 256;
+        } else if (nnz >= (int64_t)2e7) {
+            multiple = 32;
+        }
+    }
+    if (!cuda_strategy) {
+        multiple = 8;
+        if (nnz >= (int64_t)1e7) {
+            multiple = 64;
+        } else if (nnz >= (int64_t)1e6) {
+            multiple = 16;
+        }
+    }
+    const int64_t total = nwarps * multiple;
+    const int64_t lines = (nnz + warp_size - 1) / warp_size;
+    return lines < total ? lines : total;
+}
+
+/* Csr::load_balance::process (include/ginkgo/core/matrix/csr.hpp:417-468):
+ * every row drops into the bucket ceil(ceil(row_end / warp_size) * nwarps /
+ * ceil(nnz / warp_size)); srow is the inclusive prefix count of the buckets
+ * below nwarps.  The loop order is the reference's. */
+void ora_csr_lb_process(int64_t n_rows, const int32_t* row_ptrs, int64_t warp_size,
+                        int32_t* srow, int64_t nwarps)
+{
+    if (nwarps <= 0) {
+        return;
+    }
+    for (int64_t i = 0; i < nwarps; i++) {
+        srow[i] = 0;
+    }
+    const int64_t num_elems = row_ptrs[n_rows];
+    const int64_t bucket_divider =
+        num_elems > 0 ? (num_elems + warp_size - 1) / warp_size : 1;
+    for (int64_t i = 0; i < n_rows; i++) {
+        const int64_t lines = ((int64_t)row_ptrs[i + 1] + warp_size - 1) / warp_size;
+        const int64_t bucket = (lines * nwarps + bucket_divider - 1) / bucket_divider;
+        if (bucket < nwarps) {
+            srow[bucket]++;
+        }
+    }
+    for (int64_t i = 1; i < nwarps; i++) {
+        srow[i] += srow[i - 1];
+    }
+}
+
 /* common/unified/matrix/csr_kernels.cpp (compute_max_row_nnz), used by
  * Csr::convert_to(Ell*) in core/matrix/csr.cpp:512-529. */
 int64_t ora_csr_max_row_nnz(int64_t n_rows, const int32_t* row_ptrs)

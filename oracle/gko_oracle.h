@@ -181,6 +181,11 @@ void ora_sellp_compute_slice_sets(int64_t n_rows, const int32_t* row_ptrs,
                                   int64_t slice_size, int64_t stride_factor,
                                   int64_t* slice_sets, int64_t* slice_lengths);
 int64_t ora_csr_max_row_nnz(int64_t n_rows, const int32_t* row_ptrs);
+/* Csr::load_balance::clac_size and ::process (the srow row partition). */
+int64_t ora_csr_lb_clac_size(int64_t nnz, int64_t nwarps, int32_t warp_size,
+                             int32_t cuda_strategy, int32_t intel);
+void ora_csr_lb_process(int64_t n_rows, const int32_t* row_ptrs, int64_t warp_size,
+                        int32_t* srow, int64_t nwarps);
 
 /* Mixed-precision iterative refinement: solver::Ir<double> around an inner
  * Cg<float> on the float copy of A (core/solver/ir.cpp:193-257,

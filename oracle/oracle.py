@@ -177,6 +177,22 @@ def max_row_nnz(row_ptrs):
     return int(f(ctypes.c_int64(len(row_ptrs) - 1), _p(_c(row_ptrs, np.int32))))
 
 
+def lb_clac_size(nnz, nwarps, warp_size=32, cuda_strategy=True, intel=False):
+    """Csr::load_balance::clac_size (HIP-platform build): srow elements."""
+    f = lib().ora_csr_lb_clac_size
+    f.restype = ctypes.c_int64
+    return int(f(ctypes.c_int64(nnz), ctypes.c_int64(nwarps), ctypes.c_int32(warp_size),
+                 ctypes.c_int32(int(bool(cuda_strategy))), ctypes.c_int32(int(bool(intel)))))
+
+
+def lb_process(row_ptrs, nwarps, warp_size=32):
+    """Csr::load_balance::process: the srow partition of nwarps entries."""
+    srow = np.zeros(max(1, nwarps), dtype=np.int32)
+    lib().ora_csr_lb_process(ctypes.c_int64(len(row_ptrs) - 1), _p(_c(row_ptrs, np.int32)),
+                             ctypes.c_int64(warp_size), _p(srow), ctypes.c_int64(nwarps))
+    return srow[:nwarps]
+
+
 def csr_to_ell(row_ptrs, col_idxs, vals, width=None, stride=None):
     dt = vals.dtype
     n = len(row_ptrs) - 1

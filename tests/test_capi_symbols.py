@@ -32,10 +32,21 @@ def test_kernel_library_reports_errors_without_gpu():
     # workspace sizing is pure host arithmetic
     assert lib.gkomi_reduction_workspace_bytes(1000, 1) > 0
     assert lib.gkomi_csr_transpose_workspace_bytes(10, 10, 30) > 0
-    assert lib.gkomi_csr_lb_chunk(8) > 0
+    # load_balance::clac_size is host arithmetic too; it equals the oracle's
+    # restatement (pinned in test_oracle.py) over the thresholds of both forms
+    import oracle as ora
+    for nnz in (0, 1, 63, 64, 65, 4, 199_999, 200_000, 999_999, 1_000_000, 1_999_999,
+                2_000_000, 9_999_999, 10_000_000, 16_000_000, 20_000_000, 200_000_000):
+        for nwarps, ws, cuda, intel in ((1024, 64, 0, 0), (2, 32, 1, 0), (8192, 32, 1, 0),
+                                        (512, 32, 1, 1), (3, 16, 0, 0)):
+            assert lib.gkomi_csr_lb_clac_size(nnz, nwarps, ws, cuda, intel) == \
+                ora.lb_clac_size(nnz, nwarps, ws, cuda, intel), (nnz, nwarps, ws, cuda, intel)
     # a null descriptor is rejected with a typed error, before any device call
     with pytest.raises(_capi.InvalidArgument):
         _capi.call("gkomi_csr_spmv_f64", None, None, 1, None, 1, None, 1)
+    # an unknown value-type code of a mixed product is rejected the same way
+    with pytest.raises(_capi.InvalidArgument):
+        _capi.call("gkomi_csr_spmv_mixed", None, 7, None, 1, 0, None, 1, 0, None, 1)
 
 
 def test_host_library_has_no_cpu_compute_path():

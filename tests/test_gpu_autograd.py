@@ -118,3 +118,24 @@ def test_sampled_outer_product(dev, nrhs):
     rows = np.repeat(np.arange(n), np.diff(rp))
     ref = -np.einsum("kj,kj->k", lam[rows], x[ci])
     np.testing.assert_allclose(out, ref, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("method,convection", [("cg", 0.0), ("bicgstab", 10.0)])
+def test_unconverged_solves_raise(dev, method, convection):
+    """A forward solve stopped by the iteration limit raises; so does an
+    adjoint solve that cannot converge while the forward one did (b = 0 gives
+    x = 0 at iteration 0, the adjoint right-hand side x - x_t does not)."""
+    from ginkgo_amd.autograd import SolveNotConverged, sparse_solve
+    rp, ci, theta0, values_of, b, xt, n = problem(12, convection, 2, dev)
+    theta = theta0.clone().requires_grad_(True)
+    with pytest.raises(SolveNotConverged, match="forward"):
+        sparse_solve(values_of(theta), b, rp, ci, method=method, tol=1e-12, max_iters=3)
+    x = sparse_solve(values_of(theta), torch.zeros_like(b), rp, ci, method=method, tol=1e-12,
+                     max_iters=3)
+    assert float(x.abs().max()) == 0.0
+    with pytest.raises(SolveNotConverged, match="adjoint"):
+        loss_fn(x, xt).backward()
+    # the policy can downgrade the error to a warning
+    with pytest.warns(RuntimeWarning, match="forward"):
+        sparse_solve(values_of(theta0), b, rp, ci, method=method, tol=1e-12, max_iters=3,
+                     on_not_converged="warn")

@@ -1,0 +1,283 @@
+"""Parity at the full sizes of the five workloads the hot path is measured on.
+
+Each test runs the HIP path through the C ABI (or the gko:: layer over it)
+on the same seeded inputs the bench uses and checks it against the oracle or
+against what the mathematics fixes, in the style of the reference's
+device-vs-reference tests (test/matrix/csr_kernels2.cpp:218-335,
+test/solver/batch_bicgstab_kernels.cpp:60-140):
+
+* SpMV (128^3 7-point Laplacian, 1M-row power-law matrix; FP64 and FP32;
+  CSR classical / load_balance / merge_path, SELL-P, ELL):
+  |y - y_exact|_i <= 4 k eps (|A||x|)_i, with y_exact the oracle's FP64
+  OpenMP product of the widened operands and |A||x| the oracle's product on
+  |values| (plus the FP64 oracle's own bound);
+* 96^3 27-point CG and mixed-precision IR to 1e-12 with b = A x_true: the
+  true relative residual recomputed in FP64, the error against x_true, the
+  iteration counts against the oracle's CG and IR;
+* 20,000 uneven BiCGSTAB systems at 1e-10 (FP64) and 1e-5 (FP32): every
+  system's true relative residual, the iteration counts against the
+  oracle's per-system BiCGSTAB, dense LU error on a sample;
+* 512^2 x 64-RHS adjoint (CG, and BiCGSTAB on the convection-diffusion
+  variant): the gradient against exact central differences (h = 1e-6) on
+  32 conductivities, within 1e-6 relative.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as ora
+from ginkgo_amd import generators as gen
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+EPS = {np.float64: np.finfo(np.float64).eps, np.float32: np.finfo(np.float32).eps}
+TD = {np.float64: torch.float64, np.float32: torch.float32}
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    from ginkgo_amd import _capi
+    _capi.lib()
+    return torch.device("cuda", 0)
+
+
+def D():
+    from ginkgo_amd import device
+    return device
+
+
+@pytest.fixture(scope="module")
+def laplace128():
+    rp, ci, v = gen.laplace3d(128)
+    x = np.random.default_rng(2).uniform(-1, 1, (len(rp) - 1, 1))
+    return rp, ci, v, x
+
+
+@pytest.fixture(scope="module")
+def powerlaw1m():
+    rp, ci, v = gen.powerlaw(1_000_000, seed=3)
+    x = np.random.default_rng(3).uniform(-1, 1, (len(rp) - 1, 1))
+    return rp, ci, v, x
+
+
+def spmv_check(rp, ci, v, x, y, dtype, label):
+    """|y - exact|_i <= 4 k eps_T (|A||x|)_i + the FP64 oracle's own bound."""
+    vw = v.astype(dtype).astype(np.float64)
+    xw = x.astype(dtype).astype(np.float64)
+    exact = ora.csr_spmv(rp, ci, vw, xw, omp=True)
+    ax = ora.csr_spmv(rp, ci, np.abs(vw), np.abs(xw), omp=True)
+    k = np.maximum(np.diff(rp), 1).astype(np.float64)[:, None]
+    bound = 4 * k * (EPS[dtype] + EPS[np.float64]) * ax
+    err = np.abs(y.astype(np.float64) - exact)
+    worst = float(np.max(err / np.maximum(bound, 1e-300)))
+    say(f"{label}: max |y - y_oracle| {err.max():.3e}, worst fraction of the bound {worst:.3f}")
+    assert np.all(err <= bound), label
+
+
+STRATS = ["classical", "load_balance", "merge_path", "sellp", "ell"]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("fmt", STRATS)
+def test_spmv_laplace3d_128(dev, laplace128, fmt, dtype):
+    rp, ci, v, x = laplace128
+    say(f"C2a 128^3 7-pt {fmt} {np.dtype(dtype).name}: n {len(rp) - 1}, nnz {rp[-1]}")
+    vd = v.astype(dtype)
+    xd = torch.as_tensor(x.astype(dtype)).to(dev)
+    A = D().DeviceCsr(rp, ci, vd, strategy=fmt if fmt not in ("sellp", "ell") else "classical",
+                      device=dev)
+    M = D().DeviceSellp(A) if fmt == "sellp" else (D().DeviceEll(A) if fmt == "ell" else A)
+    y = M.spmv(xd).cpu().numpy()
+    spmv_check(rp, ci, v, x, y, dtype, f"C2a {fmt}")
+    # the 7-point closed form: A applied to ones gives 6 minus the neighbours
+    ones = torch.ones((len(rp) - 1, 1), dtype=TD[dtype], device=dev)
+    y1 = M.spmv(ones).cpu().numpy()[:, 0]
+    g = np.indices((128, 128, 128)).reshape(3, -1)
+    nb = sum(((g[d] > 0).astype(int) + (g[d] < 127).astype(int)) for d in range(3))
+    np.testing.assert_array_equal(y1, (6 - nb).astype(dtype))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("fmt", ["classical", "load_balance", "merge_path", "sellp"])
+def test_spmv_powerlaw_1m(dev, powerlaw1m, fmt, dtype):
+    rp, ci, v, x = powerlaw1m
+    say(f"C2b power-law 1M {fmt} {np.dtype(dtype).name}: nnz {rp[-1]}, longest row "
+        f"{np.diff(rp).max()}")
+    vd = v.astype(dtype)
+    xd = torch.as_tensor(x.astype(dtype)).to(dev)
+    A = D().DeviceCsr(rp, ci, vd, strategy="classical" if fmt == "sellp" else fmt, device=dev)
+    M = D().DeviceSellp(A) if fmt == "sellp" else A
+    y = M.spmv(xd).cpu().numpy()
+    spmv_check(rp, ci, v, x, y, dtype, f"C2b {fmt}")
+    if fmt == "sellp":
+        # the conversion at full size is bit-exact too
+        sets, lens, cols, vals = ora.csr_to_sellp(rp, ci, vd, 64, 1)
+        np.testing.assert_array_equal(M.slice_sets.cpu().numpy(), sets)
+        np.testing.assert_array_equal(M.col_idxs.cpu().numpy()[:M.total], cols)
+        np.testing.assert_array_equal(M.values.cpu().numpy()[:M.total], vals)
+    if fmt == "load_balance":
+        # deterministic: a second product is bit-identical
+        np.testing.assert_array_equal(M.spmv(xd).cpu().numpy(), y)
+
+
+# ------------------------------------------------------- 96^3 CG and IR ----
+
+@pytest.fixture(scope="module")
+def diffusion96():
+    rp, ci, v = gen.diffusion27(96)
+    n = len(rp) - 1
+    xt = np.random.default_rng(4).uniform(-1, 1, (n, 1))
+    b = ora.csr_spmv(rp, ci, v, xt, omp=True)
+    return rp, ci, v, xt, b
+
+
+def _crit(gko, dt, tol, it):
+    return (gko.stop.Iteration.build().with_max_iters(it),
+            gko.stop.ResidualNorm.build(dt).with_reduction_factor(tol))
+
+
+def test_cg_and_ir_diffusion27_96(dev, diffusion96):
+    import ginkgo_amd.gko as gko
+    rp, ci, v, xt, b_h = diffusion96
+    n = len(rp) - 1
+    say(f"C3 96^3 27-pt: n {n}, nnz {rp[-1]}; oracle CG and IR (OpenMP)")
+    spec = [{"kind": "iteration", "max_iters": 20000},
+            {"kind": "residual_norm", "reduction_factor": 1e-12}]
+    xo, so, io = ora.cg_solve(rp, ci, v, b_h, np.zeros_like(b_h), spec, omp=True)
+    inner = [{"kind": "iteration", "max_iters": 20000},
+             {"kind": "residual_norm", "reduction_factor": 1e-3}]
+    outer = [{"kind": "iteration", "max_iters": 200},
+             {"kind": "residual_norm", "reduction_factor": 1e-12}]
+    xi, si, ii, inner_total = ora.ir_solve(rp, ci, v, b_h, np.zeros_like(b_h), outer, inner,
+                                           omp=True)
+    ex = gko.Mi355xExecutor.create(0, torch.cuda.current_stream().cuda_stream)
+    A = gko.csr_from_host(ex, rp, ci, v, strategy="classical")
+    b = gko.matrix.Dense("f64").from_numpy(ex, b_h)
+    cg = gko.solver.Cg.build("f64").with_criteria(*_crit(gko, "f64", 1e-12, 20000)) \
+        .on(ex).generate(A)
+    ir = (gko.solver.Ir.build("f64").with_criteria(*_crit(gko, "f64", 1e-12, 200))
+          .with_solver(gko.solver.Cg.build("f32").with_criteria(*_crit(gko, "f32", 1e-3, 20000)))
+          .on(ex).generate(A))
+    bn = np.linalg.norm(b_h)
+    for name, solver, o_iters in (("CG", cg, io.iterations), ("IR", ir, ii.iterations)):
+        say(f"C3 {name} on the device")
+        x = gko.matrix.Dense("f64").create(ex, n, 1)
+        x.fill(0.0)
+        solver.apply(b, x)
+        xh = x.to_numpy()
+        r = b_h - ora.csr_spmv(rp, ci, v, xh, omp=True)
+        relres = np.linalg.norm(r) / bn
+        err = np.max(np.abs(xh - xt))
+        say(f"C3 {name}: {solver.last_iterations} iterations (oracle {o_iters}), true relative "
+            f"residual {relres:.3e}, max |x - x_true| {err:.3e}, rel 2-norm error "
+            f"{np.linalg.norm(xh - xt) / np.linalg.norm(xt):.3e}")
+        assert solver.last_status[0] & 0x80, f"{name} did not converge"
+        assert relres <= 1.05e-12, name
+        # the error against the manufactured solution: within the condition
+        # number times the residual (the oracle's own CG reaches the same)
+        assert err <= 100 * max(np.max(np.abs(xo - xt)), 1e-12), name
+        if name == "CG":
+            assert abs(solver.last_iterations - io.iterations) <= max(3, io.iterations // 100)
+        else:
+            assert abs(solver.last_iterations - ii.iterations) <= 1
+    say(f"C3 oracle IR: {ii.iterations} sweeps, {inner_total} inner FP32 iterations; "
+        f"oracle CG {io.iterations} iterations")
+
+
+# --------------------------------------------- 20,000 uneven BiCGSTAB ----
+
+@pytest.fixture(scope="module")
+def uneven20k():
+    return gen.uneven_batch(20000)
+
+
+def _batch_true_relres(row_off, rp, ci, v, b, x):
+    """Per-system ||b - A x|| / ||b|| in FP64 through the oracle's SpMV over
+    the block diagonal (local columns shifted to global ones)."""
+    sys_of_row = np.repeat(np.arange(len(row_off) - 1), np.diff(row_off))
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    gci = (ci.astype(np.int64) + row_off[sys_of_row[rows]]).astype(np.int32)
+    ax = ora.csr_spmv(rp, gci, v.astype(np.float64), x.astype(np.float64).reshape(-1, 1),
+                      omp=True)[:, 0]
+    r2 = np.add.reduceat((b.astype(np.float64) - ax) ** 2, row_off[:-1])
+    b2 = np.add.reduceat(b.astype(np.float64) ** 2, row_off[:-1])
+    return np.sqrt(r2 / b2)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 1e-5)],
+                         ids=["f64-1e-10", "f32-1e-5"])
+def test_batch_bicgstab_uneven_20000(dev, uneven20k, dtype, tol):
+    """FP32 runs at 1e-5: 1e-10 is below FP32's resolution (eps 1.2e-7), so a
+    relative residual of 1e-10 is not reachable in FP32 arithmetic."""
+    row_off, rp, ci, v, b = uneven20k
+    v = v.astype(dtype)
+    b = b.astype(dtype)
+    ns = len(row_off) - 1
+    say(f"C4 {ns} uneven systems {np.dtype(dtype).name} tol {tol}: rows {row_off[-1]}, "
+        f"nnz {rp[-1]}")
+    A = D().DeviceBatchCsr(row_off, rp, ci, v, device=dev)
+    bd = torch.as_tensor(b).to(dev)
+    st = A.settings(tol=tol, relative=True, max_iters=500, jacobi=True)
+    plan = A.tune(bd, torch.zeros_like(bd), st, A.plan())
+    x = torch.zeros_like(bd)
+    iters, res = A.solve(bd, x, st, plan)
+    it = iters.cpu().numpy()
+    xh = x.cpu().numpy()
+    say("C4 oracle per-system BiCGSTAB (OpenMP)")
+    xo, io, ro = ora.batch_bicgstab_solve(row_off, rp, ci, v, b, np.zeros_like(b),
+                                          use_jacobi=True, relative=True, tol=tol,
+                                          max_iters=500, omp=True)
+    rel = _batch_true_relres(row_off, rp, ci, v, b, xh)
+    diff = np.abs(it - io)
+    say(f"C4: iterations mean {it.mean():.2f} max {it.max()} (oracle mean {io.mean():.2f}); "
+        f"|iterations - oracle| max {diff.max()}, {int((diff > 0).sum())} systems differ; "
+        f"true relative residual max {rel.max():.3e}")
+    assert it.max() < 500
+    assert diff.max() <= (2 if dtype == np.float64 else 4)
+    assert rel.max() <= (3 if dtype == np.float64 else 10) * tol
+    # error against dense FP64 LU on a sample of systems, and against the
+    # oracle's solution everywhere
+    worst = 0.0
+    for s in range(0, ns, 97):
+        r0, r1 = row_off[s], row_off[s + 1]
+        d = gen.csr_to_dense(rp[r0:r1 + 1] - rp[r0], ci[rp[r0]:rp[r1]],
+                             v[rp[r0]:rp[r1]].astype(np.float64))
+        xs = np.linalg.solve(d, b[r0:r1].astype(np.float64))
+        worst = max(worst, float(np.linalg.norm(xh[r0:r1] - xs) / np.linalg.norm(xs)))
+    say(f"C4: max relative error against dense LU (every 97th system) {worst:.3e}; "
+        f"max |x - x_oracle| {np.abs(xh.astype(np.float64) - xo).max():.3e}")
+    assert worst <= (100 if dtype == np.float64 else 1000) * tol
+
+
+# ------------------------------------------------------ 512^2 adjoint ----
+
+@pytest.mark.parametrize("method,convection", [("cg", 0.0), ("bicgstab", 10.0)])
+def test_adjoint_512sq_64rhs_against_central_differences(dev, method, convection):
+    from ginkgo_amd.autograd import sparse_solve
+    from ginkgo_amd.fdcheck import AdjointProblem
+    say(f"C5 512^2 x 64 RHS {method} convection {convection}: forward + adjoint")
+    P = AdjointProblem(512, 64, dev, convection=convection)
+    assert P.symmetric == (convection == 0.0)
+    th = P.theta0.clone().requires_grad_(True)
+    x = sparse_solve(P.values_of(th), P.b, P.rp_t, P.ci_t, method=method, tol=1e-12,
+                     max_iters=20000)
+    loss = P.loss(x)
+    loss.backward()
+    g = th.grad.detach()
+    say("C5 central differences on 32 conductivities (h = 1e-6)")
+    idx = np.random.default_rng(8).choice(P.theta.size, 32, replace=False)
+    fd = P.central_differences(
+        lambda U: sparse_solve(P.values_of(P.theta0), U, P.rp_t, P.ci_t, method=method,
+                               tol=1e-12, max_iters=20000), x, idx, 1e-6)
+    gi = g[torch.as_tensor(idx, device=dev)].cpu().numpy()
+    rel = np.abs(fd - gi) / np.abs(gi)
+    say(f"C5 {method}: loss {float(loss):.6e}, max relative error {rel.max():.3e}, "
+        f"max error / max |g| {np.abs(fd - gi).max() / float(g.abs().max()):.3e}")
+    assert rel.max() <= 1e-6

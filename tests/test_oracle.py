@@ -97,6 +97,80 @@ def test_laplacian_closed_form():
     np.testing.assert_array_equal(y[:, 0], 6 - nb)
 
 
+# ------------------------------------------------- load_balance partition ----
+
+def test_lb_clac_size_known_values():
+    # core/test/matrix/csr.cpp:127-131: load_balance(2) on the 2x3 matrix with
+    # 4 entries has one srow element
+    assert ora.lb_clac_size(4, 2) == 1
+    # hand-computed: min(ceil(nnz / warp_size), nwarps * multiple)
+    # HipExecutor form (1024 warps of 64, no cuda_strategy): multiples 8 / 16 / 64
+    hip = dict(nwarps=1024, warp_size=64, cuda_strategy=False)
+    assert ora.lb_clac_size(100_000, **hip) == 1563          # ceil(1e5/64)
+    assert ora.lb_clac_size(999_999, **hip) == 8192          # 1024 * 8
+    assert ora.lb_clac_size(1_000_000, **hip) == 15625       # ceil(1e6/64) < 1024*16
+    assert ora.lb_clac_size(9_999_999, **hip) == 16384       # 1024 * 16
+    assert ora.lb_clac_size(10_000_000, **hip) == 65536      # 1024 * 64
+    assert ora.lb_clac_size(0, **hip) == 0
+    # CUDA form (32-wide warps): multiples 8 / 32 / 128 / 512 / 2048
+    assert ora.lb_clac_size(199_999, 2, 32) == 16            # 2 * 8
+    assert ora.lb_clac_size(200_000, 2, 32) == 64            # 2 * 32
+    assert ora.lb_clac_size(2_000_000, 2, 32) == 256         # 2 * 128
+    assert ora.lb_clac_size(20_000_000, 2, 32) == 1024       # 2 * 512
+    assert ora.lb_clac_size(200_000_000, 2, 32) == 4096      # 2 * 2048
+    # the "intel" name: 8 / 32 / 256
+    assert ora.lb_clac_size(20_000_000, 2, 32, intel=True) == 64
+    assert ora.lb_clac_size(199_999, 2, 32, intel=True) == 16
+    assert ora.lb_clac_size(100, 2, 0) == 0                  # warp size 0
+
+
+def _lb_by_warp_start(rp, nwarps, ws):
+    """srow from the reference GPU kernel's own warp ranges
+    (common/cuda_hip/matrix/csr_kernels.template.cpp:199-205, 222-228): warp w
+    starts at entry floor(w * ceil(nnz / ws) / nwarps) * ws and begins its
+    walk at srow[w], the row that holds that entry (empty rows ending there
+    skipped).  Brute force over every row, independent of process()."""
+    nnz = int(rp[-1])
+    lines = -(-nnz // ws)
+    out = []
+    for w in range(nwarps):
+        start = (w * lines // nwarps) * ws
+        out.append(next(i for i in range(len(rp) - 1) if rp[i + 1] > start))
+    return np.array(out, np.int32)
+
+
+@pytest.mark.parametrize("ws", [32, 64])
+def test_lb_process_matches_warp_starts(ws):
+    rng = np.random.default_rng(9)
+    # empty rows at the start, inside and at the end, one very long row
+    lens = rng.integers(0, 40, 700)
+    lens[:5] = 0
+    lens[100:130] = 0
+    lens[-7:] = 0
+    lens[300] = 5000
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    for nwarps in (1, 2, 7, 64, 333, ora.lb_clac_size(int(rp[-1]), 1024, ws, False)):
+        s = ora.lb_process(rp, nwarps, ws)
+        np.testing.assert_array_equal(s, _lb_by_warp_start(rp, nwarps, ws))
+        assert np.all(np.diff(s) >= 0) and s[0] == 5 and s[-1] < 700
+
+
+def test_lb_process_kat_and_dropped_term_fails():
+    # core/test/matrix/csr.cpp:20-45, 127-131: 2x3 matrix, load_balance(2)
+    np.testing.assert_array_equal(ora.lb_process(np.array([0, 3, 4], np.int32), 1), [0])
+    rp = np.concatenate([[0], np.cumsum(np.arange(1, 200))]).astype(np.int32)
+    s = ora.lb_process(rp, 50, 32)
+    # a partition that forgot the ceil of the bucket (floor instead) differs
+    nnz = int(rp[-1])
+    div = -(-nnz // 32)
+    floor_b = np.zeros(50, np.int64)
+    for i in range(199):
+        bkt = (-(-int(rp[i + 1]) // 32)) * 50 // div
+        if bkt < 50:
+            floor_b[bkt] += 1
+    assert not np.array_equal(np.cumsum(floor_b), s)
+
+
 # ---------------------------------------------------------- SELL-P / ELL ----
 
 @pytest.mark.parametrize("dtype", DTYPES)
@@ -146,6 +220,78 @@ def test_sellp_roundtrip_and_spmv(slice_size, stride_factor):
     for s in range(len(lens)):
         m = rl[s * slice_size:(s + 1) * slice_size].max()
         assert lens[s] == -(-m // stride_factor) * stride_factor
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("case", KATS["sellp_advanced"], ids=lambda c: c["name"])
+def test_sellp_advanced_kats(case, dtype):
+    rp, ci, v = mat(case["matrix"], dtype)
+    sets, lens, cols, vals = ora.csr_to_sellp(rp, ci, v, case["slice_size"],
+                                              case["stride_factor"])
+    y = ora.sellp_advanced_spmv(case["alpha"], 2, case["slice_size"], sets, lens, cols, vals,
+                                arr(case["x"], dtype), case["beta"], arr(case["y"], dtype))
+    np.testing.assert_array_equal(y, arr(case["expected"], dtype))
+
+
+@pytest.mark.parametrize("slice_size,stride_factor", [(64, 1), (8, 3), (1, 1)])
+def test_sellp_advanced_and_omp_against_dense(slice_size, stride_factor):
+    rng = np.random.default_rng(17)
+    rp, ci, v = gen.powerlaw(901, mean=7, seed=21)
+    sets, lens, cols, vals = ora.csr_to_sellp(rp, ci, v, slice_size, stride_factor)
+    x = rng.uniform(-1, 1, (901, 3))
+    c0 = rng.uniform(-1, 1, (901, 3))
+    dense = gen.csr_to_dense(rp, ci, v)
+    ref = -0.75 * (dense @ x) + 1.5 * c0
+    ya = ora.sellp_advanced_spmv(-0.75, 901, slice_size, sets, lens, cols, vals, x, 1.5, c0)
+    # within the bound of the k-term sums (k = the longest row), and never a
+    # sign or operand swap: alpha and beta exchanged must fail
+    k = int(np.diff(rp).max())
+    bound = 4 * (k + 2) * np.finfo(np.float64).eps * (np.abs(dense) @ np.abs(x) * 0.75 +
+                                                      1.5 * np.abs(c0))
+    assert np.all(np.abs(ya - ref) <= bound)
+    swapped = 1.5 * (dense @ x) - 0.75 * c0
+    assert np.max(np.abs(ya - swapped)) > 1e-3
+    # beta = 0 ignores c, NaN included
+    nan_c = np.full((901, 3), np.nan)
+    y0 = ora.sellp_advanced_spmv(2.0, 901, slice_size, sets, lens, cols, vals, x, 0.0, nan_c)
+    np.testing.assert_array_equal(y0, 2.0 * ora.sellp_spmv(901, slice_size, sets, lens, cols,
+                                                           vals, x))
+    # the OpenMP form sums each row in the same order: bit-identical
+    np.testing.assert_array_equal(ora.sellp_spmv(901, slice_size, sets, lens, cols, vals, x,
+                                                 omp=True),
+                                  ora.sellp_spmv(901, slice_size, sets, lens, cols, vals, x))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("case", KATS["ell_advanced"], ids=lambda c: c["name"])
+def test_ell_advanced_kats(case, dtype):
+    rp, ci, v = mat(case["matrix"], dtype)
+    w, s, cols, vals = ora.csr_to_ell(rp, ci, v, stride=case["stride"])
+    y = ora.ell_advanced_spmv(case["alpha"], 2, w, s, cols, vals, arr(case["x"], dtype),
+                              case["beta"], arr(case["y"], dtype))
+    np.testing.assert_array_equal(y, arr(case["expected"], dtype))
+
+
+def test_mixed_apply_kats_through_the_widening_rule():
+    """A mixed product computes in the highest precision of its operands
+    (reference/matrix/csr_kernels.cpp:47-121): with float and double
+    operands that is the double product of the exactly widened inputs,
+    rounded to the output type once.  The reference's mixed KATs hold for
+    every (matrix, input, output) combination this way."""
+    k = KATS["mixed_apply"]
+    for mdt in DTYPES:
+        rp, ci, v = mat(k["matrix"], mdt)
+        for bt, ct in k["combos"]:
+            bdt = mdt if bt == "same" else (np.float32 if mdt == np.float64 else np.float64)
+            cdt = mdt if ct == "same" else (np.float32 if mdt == np.float64 else np.float64)
+            xb = arr(k["apply"]["x"], bdt)
+            y = ora.csr_spmv(rp, ci, v.astype(np.float64), xb.astype(np.float64)).astype(cdt)
+            np.testing.assert_array_equal(y, arr(k["apply"]["expected"], cdt))
+            a = k["advanced"]
+            ya = ora.csr_advanced_spmv(a["alpha"], rp, ci, v.astype(np.float64),
+                                       arr(a["x"], bdt).astype(np.float64), a["beta"],
+                                       arr(a["y"], cdt).astype(np.float64)).astype(cdt)
+            np.testing.assert_array_equal(ya, arr(a["expected"], cdt))
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
