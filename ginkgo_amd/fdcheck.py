@@ -69,7 +69,13 @@ class AdjointProblem:
         self.xt = torch.as_tensor(rng.uniform(-1, 1, (self.n, nrhs))).to(device)
         self.rp_t = torch.as_tensor(rp).to(device)
         self.ci_t = torch.as_tensor(ci).to(device)
+        # the part of the values that theta does not scale (the convection);
+        # without convection it is zero (the generator's values differ from
+        # the scatter only by the rounding of another summation order), so A
+        # stays exactly symmetric
         self.const = torch.as_tensor(vals).to(device) - self._scatter(self.theta0, None)
+        if convection == 0.0:
+            self.const.zero_()
 
     def _scatter(self, t, base):
         z = torch.zeros(self.nnz, dtype=torch.float64, device=self.device) if base is None \

@@ -357,4 +357,4 @@ def test_operand_checks_reject_before_launch(dev):
     # the valid calls still work
     assert torch.allclose(D().norm2(good), torch.tensor([10.0], dtype=torch.float64, device=dev))
     y = A.spmv(good)
-    assert y.shape == (100,)
+    assert y.shape == (100, 1)
