@@ -240,6 +240,10 @@ _SIGS = {
     "gkomi_batch_cg_lds_bytes": (_SZ, [_I32, _I64, _I64, _I32]),
 }
 
+_SIGS["gkomi_batch_bicgstab_solve_noalloc_f64"] = (
+    ctypes.c_int, [_CTX, ctypes.POINTER(BatchCsr), ctypes.POINTER(BatchSettings),
+                   ctypes.POINTER(BatchPlan), _P, _P, _P, _P])
+
 for _f, _d in (("csr", Csr), ("sellp", Sellp), ("ell", Ell)):
     _SIGS[f"gkomi_{_f}_spmv_mixed"] = (ctypes.c_int, [_CTX, _I32, ctypes.POINTER(_d), _I64,
                                                       _I32, _P, _I64, _I32, _P, _I64])

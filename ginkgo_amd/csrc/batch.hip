@@ -22,6 +22,9 @@
 //  * uniform Dense: batch::matrix::Dense (batch_dense.hpp): row-major values
 //    per item (staged transposed in LDS, so that the lanes of a wave read
 //    consecutive words).
+#include <mutex>
+#include <vector>
+
 #include "common.hpp"
 
 namespace gkomi {
@@ -991,5 +994,65 @@ int gkomi_batch_spmv_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* a, const f
 GKOMI_BATCH_CG_ENTRY(double, f64)
 GKOMI_BATCH_CG_ENTRY(float, f32)
 #undef GKOMI_BATCH_CG_ENTRY
+
+
+// ------------------------------------------------ allocation-free solve ----
+// gkomi_batch_bicgstab_solve_noalloc_f64 keeps the work-queue counters of
+// every size bin in this file-scope device array instead of a workspace
+// argument.  Each GPU holds its own copy for the life of the process.  In
+// host code the array's name is only a host-side stand-in: the device
+// address is resolved once per device by hipGetSymbolAddress (with that
+// device current) and kept here; only that address reaches kernels.
+}  // extern "C"
+
+namespace gkomi {
+
+constexpr size_t kBatchScratchBytes = 4096;
+static __device__ __align__(256) unsigned char scratch[kBatchScratchBytes];
+
+// The current device's address of `scratch`, resolved on the first call on
+// that device.
+int batch_scratch_address(void** out)
+{
+    static std::mutex mtx;
+    static std::vector<void*> addresses;
+    int dev = 0;
+    GKOMI_CHECK_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> guard(mtx);
+    if (static_cast<int>(addresses.size()) <= dev) addresses.resize(dev + 1, nullptr);
+    if (addresses[dev] == nullptr) {
+        void* ptr = nullptr;
+        GKOMI_CHECK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        GKOMI_REQUIRE(ptr != nullptr, GKOMI_HIP_ERROR,
+                      "batch: no device address for the allocation-free scratch");
+        addresses[dev] = ptr;
+    }
+    *out = addresses[dev];
+    return GKOMI_SUCCESS;
+}
+
+}  // namespace gkomi
+
+extern "C" {
+
+int gkomi_batch_bicgstab_solve_noalloc_f64(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                                           const gkomi_batch_settings* st,
+                                           const gkomi_batch_plan* plan, const double* b,
+                                           double* x, int32_t* iters, double* res)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_batch(a, st)) return rc;
+    const size_t need = gkomi_batch_workspace_bytes(a);
+    GKOMI_REQUIRE(need <= kBatchScratchBytes, GKOMI_NOT_SUPPORTED,
+                  "batch (allocation-free): the work queues need more than the 4 KiB of "
+                  "device scratch; use gkomi_batch_bicgstab_solve_f64 with a workspace");
+    void* ws = nullptr;
+    if (int rc = batch_scratch_address(&ws)) return rc;
+    // the queues start at zero: cleared by a kernel (no memset node under
+    // graph capture), whatever an earlier call left
+    if (int rc = gkomi_fill_zero(ctx, ws, need)) return rc;
+    return batch_solve_impl<double, kBatchBicgstab>(ctx, a, st, plan, b, x, iters, res, ws,
+                                                    need);
+}
 
 }  // extern "C"

@@ -810,6 +810,19 @@ int gkomi_batch_bicgstab_solve_f32(const gkomi_ctx* ctx, const gkomi_batch_csr* 
                                    const gkomi_batch_plan* plan, const float* b, float* x,
                                    int32_t* iterations, float* residual_norms, void* workspace,
                                    size_t workspace_bytes);
+/* No counterpart in the reference: gkomi_batch_bicgstab_solve_f64 without the
+ * workspace argument, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It allocates nothing: the work-queue counters live in a 4 KiB
+ * device array of the library, one per GPU, which the call clears with a
+ * kernel first.  A batch whose queues would need more than 4 KiB is
+ * rejected with GKOMI_NOT_SUPPORTED.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (on different streams) race. */
+int gkomi_batch_bicgstab_solve_noalloc_f64(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
+                                           const gkomi_batch_settings* settings,
+                                           const gkomi_batch_plan* plan, const double* b,
+                                           double* x, int32_t* iterations,
+                                           double* residual_norms);
 /* Autotuner: for every bin of `plan`, times the candidate workgroup sizes
  * (64..512) and LDS / global matrix residency on that bin's systems (solving
  * into x_scratch from the initial guess x, which is not modified) and
