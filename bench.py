@@ -13,10 +13,15 @@ The line also carries
     stream it runs on, against MI355X's 8 TB/s HBM peak;
   * stream_copy_GBps: a device-to-device copy measured in the same run;
   * cpu_baseline: the oracle's OpenMP CSR SpMV (restated from
-    omp/matrix/csr_kernels.cpp) on the host cores, same matrix;
+    omp/matrix/csr_kernels.cpp) on the host cores, same matrix, with the
+    host CPU model; its "configs" time the oracle on the other workloads
+    (256^2 CG + Jacobi, 96^3 CG and IR, the 20,000-system batch uneven and
+    padded, a bounded sample of the 512^2 x 64 adjoint solve).  This leg is
+    the only place the bench runs the oracle;
   * secondary: further measurements of the path (FP32, SELL-P, power-law and
-    27-point SpMV; CG + Jacobi on the 256^2 Poisson problem beside the host
-    solves; FP64 CG vs mixed IR; the uneven batch; the adjoint solves).
+    27-point SpMV; CG + Jacobi on the 256^2 Poisson problem; FP64 CG vs mixed
+    IR; the uneven batch; the adjoint solves), each with its result checked
+    in FP64 on the host (scipy's CSR product for the residuals).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
 """
@@ -76,29 +81,46 @@ def stream_copy_gbps(dev):
     return 2 * n * 8 / (ms_own * 1e-3) / 1e9, 2 * n * 8 / (ms_torch * 1e-3) / 1e9
 
 
-def cpu_spmv_baseline(rp, ci, v, x, budget_s=10.0):
-    """Oracle OpenMP SpMV on the host (the CPU baseline, not a target)."""
+def host_spmv(rp, ci, v, x):
+    """FP64 A x on the host (scipy's CSR product), for checking results."""
     import numpy as np
-    import oracle as ora
-    ora.csr_spmv(rp, ci, v, x, omp=True)
-    reps = 0
-    t0 = time.perf_counter()
-    while True:
-        ora.csr_spmv(rp, ci, v, x, omp=True)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el > budget_s:
-            break
-    per = el / reps
+    import scipy.sparse as sp
     n = len(rp) - 1
-    return {
-        "value": round(spmv_bytes(n, int(rp[-1]), v.itemsize) / per / 1e9, 3),
-        "unit": "GB/s",
-        "cores": ora.omp_threads(),
-        "kind": "port",
-        "sample": f"{reps} OpenMP CSR SpMVs of the same 128^3 Laplacian "
-                  f"({per * 1e3:.2f} ms each, {el:.1f} s total)",
-    }
+    ncols = max(n, int(ci.max()) + 1 if len(ci) else n)
+    A = sp.csr_matrix((np.asarray(v, np.float64), np.asarray(ci), np.asarray(rp)),
+                      shape=(n, ncols))
+    return A @ np.asarray(x, np.float64)
+
+
+def host_relres(rp, ci, v, b, x):
+    import numpy as np
+    r = np.asarray(b, np.float64) - host_spmv(rp, ci, v, x)
+    return float(np.linalg.norm(r) / np.linalg.norm(np.asarray(b, np.float64)))
+
+
+def batch_relres(row_off, rp, ci, v, b, x):
+    """Per-system ||b - A x|| / ||b|| of a block-diagonal batch (local
+    column indices), in FP64 through scipy's CSR product."""
+    import numpy as np
+    sys_of_row = np.repeat(np.arange(len(row_off) - 1), np.diff(row_off))
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    gci = ci.astype(np.int64) + row_off[sys_of_row[rows]]
+    ax = host_spmv(rp, gci, v, np.asarray(x, np.float64).reshape(-1))
+    bb = np.asarray(b, np.float64)
+    r2 = np.add.reduceat((bb - ax) ** 2, row_off[:-1])
+    return np.sqrt(r2 / np.add.reduceat(bb ** 2, row_off[:-1]))
+
+
+def cpu_model():
+    """The host CPU model name (lscpu's "Model name")."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def timed(fn):
@@ -113,18 +135,16 @@ def timed(fn):
 def bench_poisson_cg(dev, nx=256):
     """2-D 5-point Poisson 256^2 (diagonal 4, off-diagonals -1), x_true ~
     U[-1,1] (seed 1), b = A x_true: CG with scalar Jacobi to 1e-10 relative
-    residual (rhs_norm), at most 2000 iterations.  The device solve goes
-    through the gko:: C++ solver; the same solve runs on the host through the
-    oracle's sequential (reference executor) and OpenMP restatements."""
+    residual (rhs_norm), at most 2000 iterations, through the gko:: C++
+    solver.  The host solves of the same system run in the CPU leg."""
     import numpy as np
     import torch
     import ginkgo_amd.gko as gko
-    import oracle as ora
     from ginkgo_amd import generators as gen
     rp, ci, v = gen.poisson2d(nx)
     n, nnz = len(rp) - 1, int(rp[-1])
     x_true = np.random.default_rng(1).uniform(-1, 1, (n, 1))
-    b_h = ora.csr_spmv(rp, ci, v, x_true)
+    b_h = host_spmv(rp, ci, v, x_true)
     exec_ = gko.Mi355xExecutor.create(0, torch.cuda.current_stream(dev).cuda_stream)
     A = gko.csr_from_host(exec_, rp, ci, v, strategy="classical")
     b = gko.matrix.Dense("f64").from_numpy(exec_, b_h)
@@ -140,27 +160,14 @@ def bench_poisson_cg(dev, nx=256):
         t, _ = timed(lambda: s.apply(b, x))
         best = t if best is None else min(best, t)
     xs = x.to_numpy()
-    r = b_h - ora.csr_spmv(rp, ci, v, xs)
     its = int(s.last_iterations)
     b_it = nnz * 12 + 4 * (n + 1) + 18 * n * 8 + n * 8
     out = {"n": n, "nnz": nnz,
            "gpu": {"time_s": round(best, 6), "iterations": its,
-                   "true_relres": float(np.linalg.norm(r) / np.linalg.norm(b_h)),
+                   "true_relres": host_relres(rp, ci, v, b_h, xs),
                    "max_abs_err_vs_x_true": float(np.max(np.abs(xs - x_true))),
                    "GBps_per_iteration_model": round(its * b_it / best / 1e9, 1)}}
-    crit = [{"kind": "iteration", "max_iters": 2000},
-            {"kind": "residual_norm", "reduction_factor": 1e-10}]
-    inv_diag = ora.jacobi_inverse_diagonal(rp, ci, v)
-    for name, omp in (("cpu_reference_sequential", False), ("cpu_omp", True)):
-        t0 = time.perf_counter()
-        xo, so, io = ora.cg_solve(rp, ci, v, b_h, np.zeros_like(b_h), crit, inv_diag=inv_diag,
-                                  omp=omp)
-        t = time.perf_counter() - t0
-        out[name] = {"time_s": round(t, 4), "iterations": int(io.iterations),
-                     "cores": ora.omp_threads() if omp else 1,
-                     "max_abs_err_vs_x_true": float(np.max(np.abs(xo - x_true)))}
-    out["speedup_gpu_over_cpu_omp"] = round(out["cpu_omp"]["time_s"] / best, 2)
-    return out
+    return out, (rp, ci, v, b_h, x_true)
 
 
 # Rows of the fork's published FP64 CG table on an NVIDIA A100
@@ -238,7 +245,6 @@ def bench_bicgstab(dev, nx=512):
     import numpy as np
     import torch
     import ginkgo_amd.gko as gko
-    import oracle as ora
     from ginkgo_amd import generators as gen
     rp, ci, v, *_ = gen.anisotropic2d(nx, seed=6, convection=10.0)
     n, nnz = len(rp) - 1, int(rp[-1])
@@ -262,11 +268,10 @@ def bench_bicgstab(dev, nx=512):
             t, _ = timed(lambda: s.apply(b, x))
             best = t if best is None else min(best, t)
         xs = x.to_numpy()
-        r = b_h - ora.csr_spmv(rp, ci, v, xs)
         its = int(s.last_iterations)
         out[name] = {"time_s": round(best, 5), "iterations": its,
                      "used_fused_path": bool(s.last_used_fused_path),
-                     "true_relres": float(np.linalg.norm(r) / np.linalg.norm(b_h)),
+                     "true_relres": host_relres(rp, ci, v, b_h, xs),
                      "GBps_reference_model": round(its * b_it / best / 1e9, 1)}
     out["speedup_fused_over_generic"] = round(out["generic"]["time_s"] / out["fused"]["time_s"],
                                               2)
@@ -274,9 +279,11 @@ def bench_bicgstab(dev, nx=512):
 
 
 def bench_cg_vs_ir(dev, nx=96):
-    """96^3 27-pt variable-coefficient diffusion: plain FP64 CG to 1e-12 vs
+    """96^3 27-pt variable-coefficient diffusion with a manufactured solution
+    (x_true ~ U[-1,1], seed 4; b = A x_true): plain FP64 CG to 1e-12 vs
     mixed-precision IR (FP64 outer to 1e-12, FP32 CG inner reducing the
-    residual by 1e-3 per sweep), both through the gko:: C++ solvers."""
+    residual by 1e-3 per sweep), both through the gko:: C++ solvers.  The
+    error is stated against x_true, the solution of the FP64 system."""
     import numpy as np
     import torch
     import ginkgo_amd.gko as gko
@@ -286,7 +293,8 @@ def bench_cg_vs_ir(dev, nx=96):
     n, nnz = len(rp) - 1, int(rp[-1])
     exec_ = gko.Mi355xExecutor.create(0, torch.cuda.current_stream(dev).cuda_stream)
     A = gko.csr_from_host(exec_, rp, ci, v, strategy="classical")
-    b_h = np.random.default_rng(4).uniform(-1, 1, (n, 1))
+    x_true = np.random.default_rng(4).uniform(-1, 1, (n, 1))
+    b_h = host_spmv(rp, ci, v, x_true)
     b = gko.matrix.Dense("f64").from_numpy(exec_, b_h)
     crit = lambda dt, tol, it: (gko.stop.Iteration.build().with_max_iters(it),
                                 gko.stop.ResidualNorm.build(dt).with_reduction_factor(tol))
@@ -307,15 +315,14 @@ def bench_cg_vs_ir(dev, nx=96):
             x.fill(0.0)
             t, _ = timed(lambda: solver.apply(b, x))
             best = t if best is None else min(best, t)
-        sols[name] = x.to_numpy()[:, 0]
+        sols[name] = x.to_numpy()
         res[name] = {"time_s": round(best, 5), "iterations": int(solver.last_iterations),
                      "status": solver.last_status}
-    # true relative residuals and the distance between the two solutions
-    d_rp, d_ci = rp, ci
-    import oracle as ora
     for name, xs in sols.items():
-        r = b_h[:, 0] - ora.csr_spmv(d_rp, d_ci, v, xs.reshape(-1, 1))[:, 0]
-        res[name]["true_relres"] = float(np.linalg.norm(r) / np.linalg.norm(b_h))
+        res[name]["true_relres"] = host_relres(rp, ci, v, b_h, xs)
+        res[name]["max_abs_err_vs_x_true"] = float(np.max(np.abs(xs - x_true)))
+        res[name]["rel_err_vs_x_true"] = float(np.linalg.norm(xs - x_true) /
+                                               np.linalg.norm(x_true))
     res["ir_vs_cg_rel_diff"] = float(np.linalg.norm(sols["ir_f64_f32"] - sols["cg_f64"]) /
                                      np.linalg.norm(sols["cg_f64"]))
     res["speedup_ir_over_cg"] = round(res["cg_f64"]["time_s"] / res["ir_f64_f32"]["time_s"], 3)
@@ -324,9 +331,8 @@ def bench_cg_vs_ir(dev, nx=96):
     b_it64 = nnz * 12 + 4 * (n + 1) + 18 * n * 8
     res["cg_f64"]["achieved_GBps"] = round(
         res["cg_f64"]["iterations"] * b_it64 / res["cg_f64"]["time_s"] / 1e9, 1)
-    res["problem"] = {"n": n, "nnz": nnz, "reference_note": "x is compared with the FP64 CG "
-                      "solution; the direct FP64 solve of 884,736 unknowns is not run in the bench"}
-    return res, (rp, ci, v, b_h)
+    res["problem"] = {"n": n, "nnz": nnz, "b": "A x_true, x_true ~ U[-1,1] (seed 4)"}
+    return res, (rp, ci, v, b_h, x_true)
 
 
 def bench_batch(dev, num=20000):
@@ -352,6 +358,7 @@ def bench_batch(dev, num=20000):
             t, (it, rs) = timed(lambda: A.solve(bd, x, st, plan))
             times.append(t)
         key = "f64" if dt == np.float64 else "f32"
+        rel = batch_relres(row_off, rp, ci, v.astype(dt), b.astype(dt), x.cpu().numpy())
         # the same solve as one captured HIP graph (x reset inside the graph)
         xg = torch.zeros_like(bd)
         graph, _, _ = A.capture(bd, xg, st, plan)
@@ -365,8 +372,14 @@ def bench_batch(dev, num=20000):
                                 "solves_per_s": round(num / min(times), 1),
                                 "max_iterations": int(it.max().item()),
                                 "mean_iterations": round(float(it.float().mean().item()), 2),
+                                "tolerance": tol,
+                                "max_true_relres_all_systems": float(rel.max()),
                                 "plan_block_sizes": [plan.block_size[i] for i in range(plan.num_bins)],
                                 "plan_matrix_in_lds": [plan.matrix_in_lds[i] for i in range(plan.num_bins)]}
+        if dt == np.float32:
+            out[f"uneven_{key}"]["tolerance_note"] = (
+                "FP32 runs at 1e-5: a relative residual of 1e-10 is below FP32 resolution "
+                "(eps 1.2e-7), so the configuration's 1e-10 applies to FP64 only")
         del A
     log("[bench] padding the batch to n = 512 (reference-compatible uniform batch)")
     prp, pci, pv, pb = gen.pad_to_uniform(row_off, rp, ci, v, b, 512)
@@ -412,18 +425,13 @@ def bench_batch_cg(dev, num=20000):
             x = torch.zeros_like(bd)
             t, (it, rs) = timed(lambda: A.solve(bd, x, st, plan, method="cg"))
             times.append(t)
-        xh = x.cpu().numpy().astype(np.float64)
-        worst = 0.0
-        for s in range(0, num, max(1, num // 40)):
-            r0, r1 = row_off[s], row_off[s + 1]
-            d = gen.csr_to_dense(rp[r0:r1 + 1] - rp[r0], ci[rp[r0]:rp[r1]], v[rp[r0]:rp[r1]])
-            bs = b[r0:r1]
-            worst = max(worst, float(np.linalg.norm(bs - d @ xh[r0:r1]) / np.linalg.norm(bs)))
+        rel = batch_relres(row_off, rp, ci, v.astype(dt), b.astype(dt), x.cpu().numpy())
         key = "f64" if dt == np.float64 else "f32"
         out[key] = {"time_s": round(min(times), 5), "solves_per_s": round(num / min(times), 1),
                     "max_iterations": int(it.max().item()),
                     "mean_iterations": round(float(it.float().mean().item()), 2),
-                    "sampled_max_true_relres": worst,
+                    "tolerance": tol,
+                    "max_true_relres_all_systems": float(rel.max()),
                     "plan_block_sizes": [plan.block_size[i] for i in range(plan.num_bins)],
                     "plan_matrix_in_lds": [plan.matrix_in_lds[i] for i in range(plan.num_bins)]}
         del A
@@ -460,10 +468,14 @@ def bench_adjoint(dev, nx=512, nrhs=64, fd_entries=32, h=1e-6, convection=0.0, m
     t, _ = timed(run)
     log(f"[bench] adjoint: first forward + backward {t:.2f} s")
     t2, (loss, x0) = timed(run)
+    from ginkgo_amd import autograd as ag
+    iters = ag.last_solve_info.get("forward_iterations", 0) + \
+        ag.last_solve_info.get("adjoint_iterations", 0)
     g = th.grad.detach().clone()
     out = {"time_s_forward_backward": round(min(t, t2), 4), "n": P.n, "nnz": P.nnz,
            "nrhs": nrhs, "method": method, "convection": convection,
-           "loss": float(loss.item()), "symmetric": P.symmetric}
+           "loss": float(loss.item()), "symmetric": P.symmetric,
+           "iterations_forward_plus_adjoint": iters}
     if fd_entries > 0:
         log(f"[bench] adjoint: central differences on {fd_entries} entries, h = {h}")
         t0 = time.perf_counter()
@@ -479,12 +491,127 @@ def bench_adjoint(dev, nx=512, nrhs=64, fd_entries=32, h=1e-6, convection=0.0, m
             "max_rel_err": float(np.max(np.abs(fd - gi) / np.abs(gi))),
             "max_err_over_max_grad": float(np.max(np.abs(fd - gi)) / gmax),
             "time_s": round(time.perf_counter() - t0, 2)}
-    return out
+    host = (P.rp, P.ci, P.values_of(P.theta0).detach().cpu().numpy(), P.b.cpu().numpy(), iters)
+    return out, host
 
 
 def torch_index(idx, dev):
     import torch
     return torch.as_tensor(idx, device=dev)
+
+
+def cpu_baseline_leg(headline, work, budget_s=10.0):
+    """The CPU baseline: the oracle (a restatement of the reference's
+    sequential and OpenMP executors) timed on the host cores.  The headline
+    SpMV gets `value`; `configs` time the other workloads on the same inputs
+    as the GPU sections (bounded samples where a full run would take
+    minutes).  The oracle is loaded here and nowhere else in the bench."""
+    import numpy as np
+    import oracle as ora
+    rp, ci, v, x = headline
+    ora.csr_spmv(rp, ci, v, x, omp=True)
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        ora.csr_spmv(rp, ci, v, x, omp=True)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s:
+            break
+    per = el / reps
+    n = len(rp) - 1
+    out = {
+        "value": round(spmv_bytes(n, int(rp[-1]), v.itemsize) / per / 1e9, 3),
+        "unit": "GB/s",
+        "cores": ora.omp_threads(),
+        "kind": "port",
+        "cpu_model": cpu_model(),
+        "sample": f"{reps} OpenMP CSR SpMVs of the same 128^3 Laplacian "
+                  f"({per * 1e3:.2f} ms each, {el:.1f} s total)",
+    }
+    cfg = {}
+    crit = lambda tol, it: [{"kind": "iteration", "max_iters": it},
+                            {"kind": "residual_norm", "reduction_factor": tol}]
+    if "poisson" in work:
+        prp, pci, pv, pb, pxt = work["poisson"]
+        inv = ora.jacobi_inverse_diagonal(prp, pci, pv)
+        c = {}
+        for name, omp in (("sequential", False), ("omp", True)):
+            t0 = time.perf_counter()
+            xo, so, io = ora.cg_solve(prp, pci, pv, pb, np.zeros_like(pb), crit(1e-10, 2000),
+                                      inv_diag=inv, omp=omp)
+            c[name] = {"time_s": round(time.perf_counter() - t0, 4),
+                       "iterations": int(io.iterations),
+                       "cores": ora.omp_threads() if omp else 1,
+                       "max_abs_err_vs_x_true": float(np.max(np.abs(xo - pxt)))}
+        cfg["cg_jacobi_poisson256sq"] = c
+    if "cg_ir" in work:
+        drp, dci, dv, db, dxt = work["cg_ir"]
+        c = {"cores": ora.omp_threads()}
+        t0 = time.perf_counter()
+        xo, so, io = ora.cg_solve(drp, dci, dv, db, np.zeros_like(db), crit(1e-12, 20000),
+                                  omp=True)
+        c["cg_f64"] = {"time_s": round(time.perf_counter() - t0, 3),
+                       "iterations": int(io.iterations),
+                       "max_abs_err_vs_x_true": float(np.max(np.abs(xo - dxt)))}
+        t0 = time.perf_counter()
+        xi, si, ii, inner_total = ora.ir_solve(drp, dci, dv, db, np.zeros_like(db),
+                                               crit(1e-12, 200), crit(1e-3, 20000), omp=True)
+        c["ir_f64_f32"] = {"time_s": round(time.perf_counter() - t0, 3),
+                           "outer_iterations": int(ii.iterations),
+                           "inner_iterations_total": int(inner_total),
+                           "max_abs_err_vs_x_true": float(np.max(np.abs(
+                               xi.reshape(-1, 1) - dxt)))}
+        cfg["cg_vs_ir_96cubed"] = c
+    if "batch" in work:
+        row_off, brp, bci, bv, bb = work["batch"]
+        c = {"cores": ora.omp_threads()}
+        for dt, tol in ((np.float64, 1e-10), (np.float32, 1e-5)):
+            t0 = time.perf_counter()
+            xo, io, ro = ora.batch_bicgstab_solve(row_off, brp, bci, bv.astype(dt),
+                                                  bb.astype(dt), np.zeros_like(bb, dtype=dt),
+                                                  use_jacobi=True, relative=True, tol=tol,
+                                                  max_iters=500, omp=True)
+            t = time.perf_counter() - t0
+            key = "uneven_f64" if dt == np.float64 else "uneven_f32"
+            c[key] = {"time_s": round(t, 4), "solves_per_s": round((len(row_off) - 1) / t, 1),
+                      "mean_iterations": round(float(io.mean()), 2)}
+        # the padded uniform form on a sample of 1,000 systems (each padded to
+        # 512 rows with the union pattern of the sample), scaled to the batch
+        from ginkgo_amd import generators as gen
+        k = 1000
+        r1 = int(row_off[k])
+        prp, pci, pv, pb = gen.pad_to_uniform(row_off[:k + 1], brp[:r1 + 1], bci[:brp[r1]],
+                                              bv[:brp[r1]], bb[:r1], 512)
+        n_pad = len(prp) - 1
+        p_off = np.arange(k + 1, dtype=np.int64) * n_pad
+        # the k padded systems as one block-diagonal batch (shared pattern)
+        g_rp = (np.concatenate([[0]] + [prp[1:].astype(np.int64) + i * int(prp[-1])
+                                        for i in range(k)]).astype(np.int32))
+        g_ci = np.tile(pci, k)
+        t0 = time.perf_counter()
+        ora.batch_bicgstab_solve(p_off, g_rp, g_ci, pv, pb, np.zeros_like(pb), use_jacobi=True,
+                                 relative=True, tol=1e-10, max_iters=500, omp=True)
+        t = time.perf_counter() - t0
+        c["padded_uniform_f64"] = {"time_s_scaled": round(t * (len(row_off) - 1) / k, 3),
+                                   "sample": f"{k} of {len(row_off) - 1} systems, "
+                                             f"{t:.2f} s, scaled"}
+        cfg["batched_bicgstab_20000"] = c
+    if "adjoint" in work:
+        arp, aci, av, ab, gpu_iters = work["adjoint"]
+        its = 100
+        t0 = time.perf_counter()
+        ora.cg_solve(arp, aci, av, ab, np.zeros_like(ab),
+                     [{"kind": "iteration", "max_iters": its}], omp=True)
+        t = time.perf_counter() - t0
+        cfg["adjoint_512sq_64rhs"] = {
+            "cores": ora.omp_threads(),
+            "seconds_per_cg_iteration_64rhs": round(t / its, 5),
+            "projected_forward_plus_backward_s": round(t / its * gpu_iters, 2),
+            "sample": f"{its} CG iterations with 64 right-hand sides ({t:.2f} s), projected to "
+                      f"the {gpu_iters} iterations of the device's forward and adjoint solves"}
+    out["configs"] = cfg
+    return out
 
 
 def main():
@@ -528,8 +655,22 @@ def main():
     y = torch.empty(n, dtype=torch.float64, device=dev)
     log(f"[bench] n={n} nnz={nnz} strategy={A.strategy}")
 
-    # correctness guard on the measured kernel
+    # correctness guard on the measured kernel: the product against scipy's
+    # FP64 CSR product within 4 k eps (|A||x|)_i, and the 7-point closed
+    # form A 1 = 6 - (number of neighbours) exactly
     A.spmv(x, y)
+    ref = host_spmv(rp, ci, v, x_h)
+    bound = 4 * 7 * np.finfo(np.float64).eps * host_spmv(rp, ci, np.abs(v), np.abs(x_h))
+    if not np.all(np.abs(y.cpu().numpy() - ref) <= 2 * bound):
+        print(json.dumps({"error": "headline SpMV outside the FP64 error bound"}))
+        return 1
+    ones = torch.ones(n, dtype=torch.float64, device=dev)
+    g3 = np.indices((128, 128, 128)).reshape(3, -1)
+    nb = sum(((g3[d] > 0).astype(int) + (g3[d] < 127).astype(int)) for d in range(3))
+    if not np.array_equal(A.spmv(ones).cpu().numpy().reshape(-1), (6 - nb).astype(np.float64)):
+        print(json.dumps({"error": "headline SpMV differs from the 7-point closed form"}))
+        return 1
+    del ref, bound, ones, g3, nb
     torch.cuda.synchronize()
 
     log("[bench] timing FP64 CSR SpMV")
@@ -585,7 +726,11 @@ def main():
         S = D.DeviceSellp(A, 64, 1)
         _, ms = time_kernel_loop(lambda: S.spmv(x, y), args.steps, args.warmup, stream)
         secondary["sellp_f64_laplace128_GBps"] = round(b64 / (ms * 1e-3) / 1e9, 1)
-        del S, A32
+        S32 = D.DeviceSellp(A32, 64, 1)
+        _, ms = time_kernel_loop(lambda: S32.spmv(x32, y32), args.steps, args.warmup, stream)
+        secondary["sellp_f32_laplace128_GBps"] = round(spmv_bytes(n, nnz, 4) / (ms * 1e-3) / 1e9,
+                                                       1)
+        del S, S32, A32
         prp, pci, pv = gen.powerlaw(1_000_000, seed=3)
         pn, pnnz = len(prp) - 1, int(prp[-1])
         P = D.DeviceCsr(prp, pci, pv, strategy="load_balance", device=dev)
@@ -598,6 +743,15 @@ def main():
         _, ms = time_kernel_loop(lambda: P32.spmv(px32, py32), args.steps, args.warmup, stream)
         secondary["csr_f32_powerlaw1M_GBps"] = round(spmv_bytes(pn, pnnz, 4) / (ms * 1e-3) / 1e9, 1)
         secondary["powerlaw_nnz"] = pnnz
+        # SELL-P of the power-law matrix (slice 64, padded to each slice's
+        # longest row; the padding is not credited)
+        for tag, M, xx, yy, vb in (("f64", P, px, py, 8), ("f32", P32, px32, py32, 4)):
+            SP = D.DeviceSellp(M, 64, 1)
+            _, ms = time_kernel_loop(lambda: SP.spmv(xx, yy), args.steps, args.warmup, stream)
+            secondary[f"sellp_{tag}_powerlaw1M_GBps"] = round(
+                spmv_bytes(pn, pnnz, vb) / (ms * 1e-3) / 1e9, 1)
+            secondary["sellp_powerlaw1M_stored_over_nnz"] = round(SP.total / pnnz, 3)
+            del SP
         # the merge_path strategy on the same two matrices
         PM = D.DeviceCsr(prp, pci, pv, strategy="merge_path", device=dev)
         _, ms = time_kernel_loop(lambda: PM.spmv(px, py), args.steps, args.warmup, stream)
@@ -630,12 +784,13 @@ def main():
         secondary["csr_f64_diffusion27_96_GBps"] = round(
             spmv_bytes(dn, dnnz, 8) / (ms * 1e-3) / 1e9, 1)
         del Dm
+    work = {}  # host inputs the CPU leg times the oracle on
     for key, name, fn in (
             ("poisson", "cg_jacobi_poisson256sq", lambda: bench_poisson_cg(dev)),
             ("cg_a100_sizes", "cg_at_fork_a100_table_sizes", lambda: bench_cg_a100_sizes(dev)),
             ("bicgstab", "bicgstab_jacobi_convection512sq", lambda: bench_bicgstab(dev)),
-            ("cg_ir", "cg_vs_ir_96cubed", lambda: bench_cg_vs_ir(dev)[0]),
-            ("batch", "batched_bicgstab_20000", lambda: bench_batch(dev)[0]),
+            ("cg_ir", "cg_vs_ir_96cubed", lambda: bench_cg_vs_ir(dev)),
+            ("batch", "batched_bicgstab_20000", lambda: bench_batch(dev)),
             ("batch_cg", "batched_cg_20000_spd", lambda: bench_batch_cg(dev)),
             ("adjoint", "adjoint_512sq_64rhs",
              lambda: bench_adjoint(dev, fd_entries=args.fd_entries)),
@@ -646,7 +801,10 @@ def main():
             continue
         t0 = time.perf_counter()
         try:
-            secondary[name] = fn()
+            r = fn()
+            if isinstance(r, tuple):
+                r, work[key] = r
+            secondary[name] = r
         except Exception as e:  # a secondary failure must not hide the headline
             secondary[name] = {"error": f"{type(e).__name__}: {e}"}
         log(f"[bench] {name}: {time.perf_counter() - t0:.1f} s")
@@ -655,8 +813,10 @@ def main():
         result["secondary"] = secondary
 
     if not args.no_cpu:
-        log("[bench] CPU baseline (oracle OpenMP SpMV)")
-        result["cpu_baseline"] = cpu_spmv_baseline(rp, ci, v, x_h.reshape(-1, 1))
+        log("[bench] CPU baseline (the oracle on the host cores)")
+        t0 = time.perf_counter()
+        result["cpu_baseline"] = cpu_baseline_leg((rp, ci, v, x_h.reshape(-1, 1)), work)
+        log(f"[bench] CPU baseline: {time.perf_counter() - t0:.1f} s")
     print(json.dumps(result), flush=True)
     return 0
 

@@ -33,6 +33,9 @@ from .device import as_2d
 
 _EXECS = {}
 
+# iteration counts of the most recent forward and adjoint solves
+last_solve_info = {}
+
 
 class SolveNotConverged(C.GkoError):
     """A forward or adjoint solve stopped without meeting its residual
@@ -115,6 +118,7 @@ class SparseSolve(torch.autograd.Function):
         A = gko.csr_view(exec_, rp, ci, vals, strategy="classical")
         x, solver = _solve(gko, exec_, A, rhs, method, tol, max_iters, jacobi)
         _check_converged(solver, "forward", tol, max_iters, policy)
+        last_solve_info["forward_iterations"] = int(solver.last_iterations)
         ctx.save_for_backward(vals, x, rp, ci)
         ctx.settings = (method, tol, max_iters, jacobi, b.dim(), policy)
         ctx.A = A
@@ -132,6 +136,7 @@ class SparseSolve(torch.autograd.Function):
         At = ctx.A if method == "cg" else ctx.A.transpose()
         lam, adjoint = _solve(gko, exec_, At, g, method, tol, max_iters, jacobi)
         _check_converged(adjoint, "adjoint", tol, max_iters, policy)
+        last_solve_info["adjoint_iterations"] = int(adjoint.last_iterations)
         grad_vals = None
         if ctx.needs_input_grad[0]:
             grad_vals = sampled_outer_product(rp, ci, vals, lam, x, alpha=-1.0)

@@ -206,6 +206,126 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// SELL-P product for slices of 64 rows and one right-hand side: one wave per
+// slice, lane = row.  A row's stored entries precede its padding (the
+// layout the conversions produce), so a lane stops at its row's first
+// padding slot and never reads the padded tail of a slice -- which on
+// matrices with a few long rows is most of the storage.
+//  * Phase 1: each lane walks its own row over the first kSellpLanePhase
+//    slots, four slots in flight at once.
+//  * Phase 2: rows not finished by then (found by a ballot) are spread over
+//    the wave, G = 64 / pow2ceil(count) lanes each, striding along the row
+//    four slots at a time; a lane group sum joins the row's phase-1 sum.
+// Every row sums in one fixed order: deterministic.
+constexpr int kSellpLanePhase = 32;
+
+template <typename TA, typename TB, typename TC, bool Adv>
+__global__ void __launch_bounds__(kBlock)
+    sellp_spmv_wave_kernel(int64_t n_rows, int64_t num_slices,
+                           const int64_t* __restrict__ slice_sets,
+                           const int64_t* __restrict__ slice_lengths,
+                           const int32_t* __restrict__ s_cols, const TA* __restrict__ s_vals,
+                           const TB* __restrict__ b, int64_t bs, TC* __restrict__ c, int64_t cs,
+                           const TA* __restrict__ alpha, const TC* __restrict__ beta)
+{
+    using Acc = acc_t<TA, TB, TC>;
+    constexpr int U = 4;
+    const int lane = threadIdx.x % kWave;
+    const int64_t num_waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+    for (int64_t s = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kWavesPerBlock +
+                     threadIdx.x / kWave;
+         s < num_slices; s += num_waves) {
+        const int64_t set = slice_sets[s];
+        const int64_t len = slice_lengths[s];
+        const int64_t row = s * kWave + lane;
+        const int64_t base = set * kWave + lane;
+        const int64_t k1 = len < kSellpLanePhase ? len : kSellpLanePhase;
+        Acc sum = 0;
+        bool done = row >= n_rows;
+        for (int64_t k = 0; k < k1; k += U) {
+            if (__ballot(!done) == 0) break;
+            int32_t col[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                col[u] = (!done && k + u < k1) ? s_cols[base + (k + u) * kWave] : -2;
+            }
+            bool ok[U];
+            bool go = !done;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ok[u] = go && col[u] >= 0;
+                go = ok[u];
+                // padding ends the row; the end of phase 1 (-2) does not
+                done = done || col[u] == -1;
+            }
+            TA val[U];
+            TB xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                val[u] = s_vals[ok[u] ? base + (k + u) * kWave : base];
+                xv[u] = b[static_cast<int64_t>(ok[u] ? col[u] : col[0] >= 0 ? col[0] : 0) * bs];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                sum = fma(ok[u] ? static_cast<Acc>(val[u]) : Acc(0),
+                          ok[u] ? static_cast<Acc>(xv[u]) : Acc(0), sum);
+            }
+        }
+        const bool pending = !done && len > kSellpLanePhase;
+        const unsigned long long mask = __ballot(pending);
+        if (mask != 0ull) {
+            const int cnt = __popcll(mask);
+            int gsz = kWave;
+            while (gsz > 1 && gsz * cnt > kWave) gsz /= 2;
+            const int g = lane / gsz;
+            const int li = lane % gsz;
+            Acc part = 0;
+            if (g < cnt) {
+                unsigned long long m = mask;
+                for (int i = 0; i < g; ++i) m &= m - 1;
+                const int prow = __ffsll(static_cast<long long>(m)) - 1;
+                const int64_t pbase = set * kWave + prow;
+                bool pdone = false;
+                for (int64_t k = kSellpLanePhase + li; k < len && !pdone;
+                     k += static_cast<int64_t>(U) * gsz) {
+                    int32_t col[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int64_t kk = k + static_cast<int64_t>(u) * gsz;
+                        col[u] = kk < len ? s_cols[pbase + kk * kWave] : -1;
+                    }
+                    bool ok[U];
+                    bool go = true;
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        ok[u] = go && col[u] >= 0;
+                        go = ok[u];
+                    }
+                    pdone = !go;
+                    TA val[U];
+                    TB xv[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int64_t kk = k + static_cast<int64_t>(u) * gsz;
+                        val[u] = s_vals[ok[u] ? pbase + kk * kWave : pbase];
+                        xv[u] = b[static_cast<int64_t>(ok[u] ? col[u] : 0) * bs];
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        part = fma(ok[u] ? static_cast<Acc>(val[u]) : Acc(0),
+                                   ok[u] ? static_cast<Acc>(xv[u]) : Acc(0), part);
+                    }
+                }
+            }
+            for (int off = gsz / 2; off >= 1; off /= 2) part += __shfl_xor(part, off, kWave);
+            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            const Acc add = __shfl(part, (rank * gsz) & (kWave - 1), kWave);
+            if (pending) sum += add;
+        }
+        if (row < n_rows) store_result<Adv>(c + row * cs, sum, alpha, beta);
+    }
+}
+
 // ------------------------------------------------------------------ ELL ----
 
 template <typename T>
@@ -439,6 +559,15 @@ int sellp_spmv_impl(const gkomi_ctx* ctx, const TA* alpha, const gkomi_sellp* a,
                   "SELL-P spmv: bad strides");
     GKOMI_REQUIRE(b && c, GKOMI_INVALID_ARGUMENT, "SELL-P spmv: null vector");
     GKOMI_REQUIRE(!Adv || (alpha && beta), GKOMI_INVALID_ARGUMENT, "SELL-P: null alpha/beta");
+    if (a->slice_size == kWave && nrhs == 1) {
+        const int64_t num_slices = ceildiv(a->n_rows, kWave);
+        sellp_spmv_wave_kernel<TA, TB, TC, Adv>
+            <<<grid_for(num_slices, kWavesPerBlock, 32), kBlock, 0, stream_of(ctx)>>>(
+                a->n_rows, num_slices, a->slice_sets, a->slice_lengths, a->col_idxs,
+                static_cast<const TA*>(a->values), b, bs, c, cs, alpha, beta);
+        GKOMI_CHECK_LAUNCH("sellp_spmv_wave_kernel");
+        return GKOMI_SUCCESS;
+    }
     sellp_spmv_kernel<TA, TB, TC, Adv>
         <<<grid_for(a->n_rows, kBlock, 32), kBlock, 0, stream_of(ctx)>>>(
             a->n_rows, nrhs, a->slice_size, a->slice_sets, a->slice_lengths, a->col_idxs,
