@@ -375,7 +375,8 @@ def bench_batch(dev, num=20000):
                                 "tolerance": tol,
                                 "max_true_relres_all_systems": float(rel.max()),
                                 "plan_block_sizes": [plan.block_size[i] for i in range(plan.num_bins)],
-                                "plan_matrix_in_lds": [plan.matrix_in_lds[i] for i in range(plan.num_bins)]}
+                                "plan_matrix_in_lds": [plan.matrix_in_lds[i] for i in range(plan.num_bins)],
+                                "plan_vectors": [plan.vectors[i] for i in range(plan.num_bins)]}
         if dt == np.float32:
             out[f"uneven_{key}"]["tolerance_note"] = (
                 "FP32 runs at 1e-5: a relative residual of 1e-10 is below FP32 resolution "
@@ -433,7 +434,8 @@ def bench_batch_cg(dev, num=20000):
                     "tolerance": tol,
                     "max_true_relres_all_systems": float(rel.max()),
                     "plan_block_sizes": [plan.block_size[i] for i in range(plan.num_bins)],
-                    "plan_matrix_in_lds": [plan.matrix_in_lds[i] for i in range(plan.num_bins)]}
+                    "plan_matrix_in_lds": [plan.matrix_in_lds[i] for i in range(plan.num_bins)],
+                    "plan_vectors": [plan.vectors[i] for i in range(plan.num_bins)]}
         del A
     torch.cuda.empty_cache()
     return out

@@ -429,7 +429,8 @@ __global__ void __launch_bounds__(NThreads)
             omega = ts / tt;
             T r2 = 0, rhr = 0;
             for (int row = tid; row < n; row += NThreads) {
-                xl[row] += alpha * ph[row] + omega * sh[row];
+                // one explicit contraction, the same in both kernels
+                xl[row] += fma(alpha, ph[row], omega * sh[row]);
                 const T ri = s[row] - omega * t[row];
                 r[row] = ri;
                 r2 += ri * ri;
@@ -443,6 +444,272 @@ __global__ void __launch_bounds__(NThreads)
         }
         __syncthreads();
         for (int row = tid; row < n; row += NThreads) gx[row] = xl[row];
+        if (tid == 0) {
+            iters_out[sys] = iter;
+            res_out[sys] = res;
+        }
+    }
+    // the last workgroup out resets the queue for the next launch
+    if (tid == 0) {
+        const unsigned d = atomicAdd(queue + 1, 1u);
+        if (d == gridDim.x - 1) {
+            queue[0] = 0;
+            queue[1] = 0;
+        }
+    }
+}
+
+// The same solvers with the work vectors in registers (Csr formats).  Thread
+// t owns rows t, t + NT, ..., t + (R - 1) NT of the system and keeps every
+// work vector's entries of those rows in registers; only the vectors the
+// products gather -- p_hat and s_hat for BiCGSTAB, p for CG -- go to LDS,
+// beside the system's matrix (values, 16-bit local columns, row pointers).
+// At about 10 bytes per stored entry plus 16 (FP64) or 8 (FP32) bytes per
+// row, a 512-row system of 9 entries per row takes 54 KB instead of the
+// 86 KB of the LDS-vector kernel, and the vector updates touch no LDS.  A
+// product sums each row by one thread in the stored order, as the
+// reference's batch Csr product does (common/cuda_hip/matrix/
+// batch_csr_kernels.hpp).  The iteration is the LDS-vector kernel's: the
+// reference's batch BiCGSTAB (no breakdown guards, the half-step exit) or
+// batch CG (implicit norm sqrt|r.z|, iterations counted from 1).
+//
+// LDS layout per system slot (n = lds_rows, z = lds_nnz):
+//   T in0[n], T in1[n] (BiCGSTAB only), T vals[z], int32 rp[n+1], uint16 ci[z]
+template <typename T, int NT, int R, int Fmt, int Method>
+__global__ void __launch_bounds__(NT)
+    batch_reg_kernel(int64_t num_systems, const int32_t* __restrict__ order,
+                     const int64_t* __restrict__ row_off, const int32_t* __restrict__ rp,
+                     const int32_t* __restrict__ ci, const T* __restrict__ vals, int64_t uni_n,
+                     int64_t uni_nnz, const T* __restrict__ b, T* __restrict__ x,
+                     int32_t max_iters, int32_t relative, double tol_d, int32_t jacobi,
+                     int32_t* __restrict__ iters_out, T* __restrict__ res_out,
+                     unsigned* __restrict__ queue, int64_t lds_rows, int64_t lds_nnz)
+{
+    static_assert(Fmt == kFmtUneven || Fmt == kFmtCsr, "register kernel: Csr formats");
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ T red2[4 * (NT / kWave)];
+    int red_turn = 0;
+    auto red = [&]() {
+        red_turn ^= 1;
+        return red2 + red_turn * 2 * (NT / kWave);
+    };
+    __shared__ int64_t slot;
+    const int tid = threadIdx.x;
+    const int64_t L = lds_rows;
+    constexpr int kIn = Method == kBatchCg ? 1 : 2;
+    T* in0 = reinterpret_cast<T*>(smem);
+    T* in1 = in0 + L;
+    T* lv = in0 + kIn * L;
+    int32_t* lrp = reinterpret_cast<int32_t*>(lv + lds_nnz);
+    uint16_t* lci = reinterpret_cast<uint16_t*>(lrp + L + 1);
+    const T tol = static_cast<T>(tol_d);
+
+    while (true) {
+        __syncthreads();
+        if (tid == 0) slot = static_cast<int64_t>(atomicAdd(queue, 1u));
+        __syncthreads();
+        const int64_t k = slot;
+        if (k >= num_systems) break;
+        const int64_t sys = order ? order[k] : k;
+        int32_t n;
+        int64_t row0, nnz0;
+        const int32_t* srp;
+        int32_t rp_base, nnz;
+        if constexpr (Fmt == kFmtCsr) {
+            n = static_cast<int32_t>(uni_n);
+            nnz = static_cast<int32_t>(uni_nnz);
+            row0 = sys * uni_n;
+            nnz0 = sys * uni_nnz;
+            srp = rp;
+            rp_base = 0;
+        } else {
+            row0 = row_off[sys];
+            n = static_cast<int32_t>(row_off[sys + 1] - row0);
+            srp = rp + row0;
+            rp_base = srp[0];
+            nnz = srp[n] - rp_base;
+            nnz0 = rp_base;
+        }
+        const T* gv = vals + nnz0;
+        const int32_t* gci = ci + (Fmt == kFmtCsr ? 0 : nnz0);
+        const T* gb = b + row0;
+        T* gx = x + row0;
+        for (int i = tid; i < nnz; i += NT) {
+            lv[i] = gv[i];
+            lci[i] = static_cast<uint16_t>(gci[i]);
+        }
+        for (int i = tid; i <= n; i += NT) lrp[i] = srp[i] - rp_base;
+        // own rows: prec, x, b
+        T prec[R], xr[R], r[R];
+        T bb = 0, unused = 0;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int row = tid + j * NT;
+            const bool own = row < n;
+            xr[j] = own ? gx[row] : T(0);
+            r[j] = own ? gb[row] : T(0);
+            bb += r[j] * r[j];
+            if (own) in0[row] = xr[j];
+        }
+        __syncthreads();
+        // the scalar Jacobi of the reference's batch Csr (1 for a zero or
+        // missing diagonal), or the identity
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int row = tid + j * NT;
+            T d = T(1);
+            if (jacobi && row < n) {
+                const int32_t e1 = lrp[row + 1];
+                for (int32_t e = lrp[row]; e < e1; ++e) {
+                    if (lci[e] == row) {
+                        if (lv[e] != T(0)) d = T(1) / lv[e];
+                        break;
+                    }
+                }
+            }
+            prec[j] = d;
+        }
+        // out = A in at the own rows, each row summed in stored order
+        auto spmv = [&](const T* in, T out[R]) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const int row = tid + j * NT;
+                T sum = 0;
+                if (row < n) {
+                    const int32_t e1 = lrp[row + 1];
+                    for (int32_t e = lrp[row]; e < e1; ++e) sum += lv[e] * in[lci[e]];
+                }
+                out[j] = sum;
+            }
+        };
+        block_sum2<T, NT>(bb, unused, red());
+        const T rhs_norm = sqrt(bb);
+        const T limit = relative ? tol * rhs_norm : tol;
+        int32_t iter = 0;
+        T res = 0;
+        if constexpr (Method == kBatchCg) {
+            T z[R], p[R], ap[R];
+            spmv(in0, ap);
+            T rz = 0;
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                r[j] -= ap[j];
+                p[j] = T(0);
+                z[j] = prec[j] * r[j];
+                rz += r[j] * z[j];
+            }
+            T rho_old = 1;
+            while (true) {
+                // the first block sum also orders the reads of x in in0
+                // (the product above) before p overwrites it
+                const T rho = block_sum1<T, NT>(rz, red());
+                ++iter;
+                res = sqrt(rho < T(0) ? -rho : rho);
+                if (iter >= max_iters || res <= limit) break;
+                const bool restart = rho_old == T(0);
+                const T beta = restart ? T(0) : rho / rho_old;
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const int row = tid + j * NT;
+                    p[j] = restart ? z[j] : z[j] + beta * p[j];
+                    if (row < n) in0[row] = p[j];
+                }
+                __syncthreads();
+                spmv(in0, ap);
+                T pap = 0;
+#pragma unroll
+                for (int j = 0; j < R; ++j) pap += p[j] * ap[j];
+                pap = block_sum1<T, NT>(pap, red());
+                const T temp = rho / pap;
+                rz = 0;
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    xr[j] += temp * p[j];
+                    r[j] -= temp * ap[j];
+                    z[j] = prec[j] * r[j];
+                    rz += r[j] * z[j];
+                }
+                rho_old = rho;
+            }
+        } else {
+            T rh[R], p[R], ph[R], v[R], s[R], sh[R], t[R];
+            spmv(in0, v);
+            T rr = 0;
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                r[j] -= v[j];
+                rh[j] = r[j];
+                p[j] = T(0);
+                v[j] = T(0);
+                rr += r[j] * r[j];
+            }
+            T rho_new = rr;
+            // also orders the reads of x in in0 before p_hat overwrites it
+            block_sum2<T, NT>(rr, rho_new, red());
+            res = sqrt(rr);
+            T rho_old = 1, omega = 1, alpha = 1;
+            for (; iter < max_iters; ++iter) {
+                if (res <= limit) break;
+                const T beta = (rho_new / rho_old) * (alpha / omega);
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const int row = tid + j * NT;
+                    p[j] = r[j] + beta * (p[j] - omega * v[j]);
+                    ph[j] = prec[j] * p[j];
+                    if (row < n) in0[row] = ph[j];
+                }
+                __syncthreads();
+                spmv(in0, v);
+                T rv = 0;
+#pragma unroll
+                for (int j = 0; j < R; ++j) rv += rh[j] * v[j];
+                rv = block_sum1<T, NT>(rv, red());
+                alpha = rho_new / rv;
+                T ss = 0;
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const int row = tid + j * NT;
+                    s[j] = r[j] - alpha * v[j];
+                    sh[j] = prec[j] * s[j];
+                    if (row < n) in1[row] = sh[j];
+                    ss += s[j] * s[j];
+                }
+                // publishes s_hat for the product below as well
+                ss = block_sum1<T, NT>(ss, red());
+                res = sqrt(ss);
+                if (res <= limit) {
+#pragma unroll
+                    for (int j = 0; j < R; ++j) xr[j] += alpha * ph[j];
+                    break;
+                }
+                spmv(in1, t);
+                T ts = 0, tt = 0;
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    ts += t[j] * s[j];
+                    tt += t[j] * t[j];
+                }
+                block_sum2<T, NT>(ts, tt, red());
+                omega = ts / tt;
+                T r2 = 0, rhr = 0;
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    xr[j] += fma(alpha, ph[j], omega * sh[j]);
+                    r[j] = s[j] - omega * t[j];
+                    r2 += r[j] * r[j];
+                    rhr += rh[j] * r[j];
+                }
+                block_sum2<T, NT>(r2, rhr, red());
+                res = sqrt(r2);
+                rho_old = rho_new;
+                rho_new = rhr;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int row = tid + j * NT;
+            if (row < n) gx[row] = xr[j];
+        }
         if (tid == 0) {
             iters_out[sys] = iter;
             res_out[sys] = res;
@@ -486,6 +753,15 @@ int32_t batch_subwave(int32_t requested, int64_t rows, int64_t nnz)
     int32_t sw = 1;
     while (sw < 16 && int64_t{8} * sw < avg) sw *= 2;
     return sw;
+}
+
+// LDS of the register-vector kernel: the gathered vectors (two for
+// BiCGSTAB, one for CG), values, row pointers and 16-bit columns.
+size_t batch_reg_lds_bytes(int vbytes, int64_t rows, int64_t nnz, int method)
+{
+    const int vecs = method == kBatchCg ? 1 : 2;
+    const size_t b = static_cast<size_t>(vbytes) * (vecs * rows + nnz) + 4 * (rows + 1) + 2 * nnz;
+    return (b + 15) / 16 * 16;
 }
 
 int fmt_of(const gkomi_batch_csr* a)
@@ -607,6 +883,56 @@ int check_batch(const gkomi_batch_csr* a, const gkomi_batch_settings* st)
     return GKOMI_SUCCESS;
 }
 
+template <typename T, int NT, int R, int Fmt, int Method>
+int launch_reg(hipStream_t stream, int grid, size_t lds, int64_t count, const int32_t* order,
+               const gkomi_batch_csr* a, const T* b, T* x, const gkomi_batch_settings* st,
+               int32_t* iters, T* res, unsigned* queue, int64_t lds_rows, int64_t lds_nnz)
+{
+    auto kern = batch_reg_kernel<T, NT, R, Fmt, Method>;
+    if (lds > 64 * 1024) {
+        GKOMI_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            static_cast<int>(lds)));
+    }
+    kern<<<grid, NT, lds, stream>>>(count, order, a->row_offsets, a->row_ptrs, a->col_idxs,
+                                    static_cast<const T*>(a->values), a->uniform_rows,
+                                    a->uniform_nnz, b, x, st->max_iterations, st->tolerance_type,
+                                    st->tolerance, st->preconditioner, iters, res, queue,
+                                    lds_rows, lds_nnz);
+    GKOMI_CHECK_LAUNCH("batch_reg_kernel");
+    return GKOMI_SUCCESS;
+}
+
+// Rows per thread of the register kernel for a bin: 1 or 2 (0 = does not apply).
+int reg_rows_per_thread(int nt, int64_t rows)
+{
+    if (rows <= nt) return 1;
+    if (rows <= 2 * int64_t{nt}) return 2;
+    return 0;
+}
+
+template <typename T, int Fmt, int Method>
+int launch_reg_shape(int nt, int rpt, hipStream_t s, int grid, size_t lds, int64_t cnt,
+                     const int32_t* order, const gkomi_batch_csr* a, const T* b, T* x,
+                     const gkomi_batch_settings* st, int32_t* iters, T* res, unsigned* q,
+                     int64_t rows, int64_t nnz)
+{
+#define GKOMI_REG_CASE(NTV, RV)                                                               \
+    if (nt == NTV && rpt == RV)                                                               \
+        return launch_reg<T, NTV, RV, Fmt, Method>(s, grid, lds, cnt, order, a, b, x, st, iters, \
+                                                   res, q, rows, nnz);
+    GKOMI_REG_CASE(64, 1)
+    GKOMI_REG_CASE(64, 2)
+    GKOMI_REG_CASE(128, 1)
+    GKOMI_REG_CASE(128, 2)
+    GKOMI_REG_CASE(256, 1)
+    GKOMI_REG_CASE(256, 2)
+    GKOMI_REG_CASE(512, 1)
+    GKOMI_REG_CASE(512, 2)
+#undef GKOMI_REG_CASE
+    return fail(GKOMI_INVALID_ARGUMENT, "batch: block size must be 64, 128, 256 or 512");
+}
+
 template <typename T, bool MatLds, int Method>
 int launch_batch_fmt(int fmt, int nt, hipStream_t s, int grid, size_t lds, int64_t cnt,
                      const int32_t* order, const gkomi_batch_csr* a, const T* b, T* x,
@@ -668,6 +994,41 @@ int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
         const int64_t nnz = plan->bin_max_nnz[bin];
         GKOMI_REQUIRE(rows <= a->max_rows && nnz <= a->max_nnz, GKOMI_INVALID_ARGUMENT,
                       "batch: bin maxima exceed the batch maxima");
+        const int32_t* order_bin = plan->order ? plan->order + beg : nullptr;
+        GKOMI_REQUIRE(plan->order != nullptr || plan->num_bins == 1, GKOMI_INVALID_ARGUMENT,
+                      "batch: several bins need an order");
+        unsigned* qb = queues + bin * (kBatchQueueBytes / sizeof(unsigned));
+        {
+            // work vectors in registers (Csr formats, up to two rows per
+            // thread, the matrix in LDS): the default where it applies
+            const int want = plan->vectors[bin];
+            GKOMI_REQUIRE(want >= 0 && want <= 2, GKOMI_INVALID_ARGUMENT,
+                          "batch: vectors must be 0 (automatic), 1 (LDS) or 2 (registers)");
+            const int nt = plan->block_size[bin];
+            const int rpt = reg_rows_per_thread(nt, rows);
+            const size_t rlds = batch_reg_lds_bytes(sizeof(T), rows, nnz, Method);
+            const bool applies = (fmt == kFmtUneven || fmt == kFmtCsr) && rpt > 0 &&
+                                 rlds <= static_cast<size_t>(lds_limit);
+            GKOMI_REQUIRE(want != 2 || applies, GKOMI_NOT_SUPPORTED,
+                          "batch: register work vectors need a Csr batch, at most two rows "
+                          "per thread and the matrix in LDS");
+            if (want == 2 || (want == 0 && applies)) {
+                int bpc = plan->blocks_per_cu[bin];
+                if (bpc <= 0) {
+                    bpc = static_cast<int>(std::max<size_t>(
+                        1, std::min<size_t>(2048 / nt, (160 * 1024) / std::max<size_t>(rlds, 1))));
+                }
+                const int grid = static_cast<int>(std::max<int64_t>(
+                    1, std::min<int64_t>(cnt, static_cast<int64_t>(num_cus()) * bpc)));
+                const int rc = fmt == kFmtUneven
+                    ? launch_reg_shape<T, kFmtUneven, Method>(nt, rpt, s, grid, rlds, cnt, order_bin,
+                                                              a, b, x, st, iters, res, qb, rows, nnz)
+                    : launch_reg_shape<T, kFmtCsr, Method>(nt, rpt, s, grid, rlds, cnt, order_bin,
+                                                           a, b, x, st, iters, res, qb, rows, nnz);
+                if (rc) return rc;
+                continue;
+            }
+        }
         int mat = plan->matrix_in_lds[bin];
         if (mat < 0) {
             mat = batch_lds_bytes(sizeof(T), rows, nnz, true, Method, fmt) <=
@@ -867,14 +1228,28 @@ int batch_tune_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
         gkomi_batch_csr sub = *a;
         sub.num_systems = cnt;
         float best = 1e30f;
-        int best_nt = 256, best_mat = 0;
+        int best_nt = 256, best_mat = 0, best_vec = 1;
         const bool fits = batch_lds_bytes(sizeof(T), one.bin_max_rows[0], one.bin_max_nnz[0],
                                           true, Method, fmt_of(a)) <=
                           static_cast<size_t>(lds_limit);
+        const int fmt = fmt_of(a);
+        const bool reg_fits = (fmt == kFmtUneven || fmt == kFmtCsr) &&
+                              batch_reg_lds_bytes(sizeof(T), one.bin_max_rows[0],
+                                                  one.bin_max_nnz[0], Method) <=
+                                  static_cast<size_t>(lds_limit);
         for (int nt : {64, 128, 256, 512}) {
-            for (int mat = fits ? 1 : 0; mat >= 0; --mat) {
+            // candidates: register work vectors (matrix in LDS), then LDS
+            // work vectors with the matrix in LDS and in global memory
+            for (int cand = 2; cand >= 0; --cand) {
+                const int vec = cand == 2 ? 2 : 1;
+                const int mat = cand == 2 ? 1 : cand;
+                if (vec == 2 && !(reg_fits && reg_rows_per_thread(nt, one.bin_max_rows[0]) > 0)) {
+                    continue;
+                }
+                if (vec == 1 && mat == 1 && !fits) continue;
                 one.block_size[0] = nt;
                 one.matrix_in_lds[0] = mat;
+                one.vectors[0] = vec;
                 // sub-batch of this bin only: order is rebased, so the
                 // descriptor keeps the full arrays but counts cnt systems
                 GKOMI_CHECK_HIP(hipMemcpyAsync(xs, x, sizeof(T) * rows_all,
@@ -891,12 +1266,14 @@ int batch_tune_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
                     best = ms;
                     best_nt = nt;
                     best_mat = mat;
+                    best_vec = vec;
                 }
             }
             if (rc) break;
         }
         plan->block_size[bin] = best_nt;
         plan->matrix_in_lds[bin] = best_mat;
+        plan->vectors[bin] = best_vec;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

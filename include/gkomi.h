@@ -772,10 +772,14 @@ typedef struct {
 /* Launch plan: systems in `order` (device, largest first) split into bins of
  * similar size; each bin is one persistent launch with its own workgroup
  * size, LDS residency of the matrix (1 = LDS, 0 = global, -1 = LDS if it
- * fits), workgroups per CU (0 = from the LDS footprint) and, for the Csr
- * formats, lanes per row of the product (1, 2, 4, 8 or 16; 0 = from the
- * bin's entries per row).  No reference counterpart: the reference launches
- * one uniform kernel per batch (hip/solver/batch_bicgstab_kernels.hip.cpp). */
+ * fits), workgroups per CU (0 = from the LDS footprint), for the Csr
+ * formats lanes per row of the product (1, 2, 4, 8 or 16; 0 = from the
+ * bin's entries per row), and where the work vectors live: `vectors`
+ * 2 = registers (Csr formats, up to two rows per thread; only the vectors
+ * the products gather go to LDS, beside the matrix), 1 = LDS, 0 = registers
+ * where that applies and fits, LDS otherwise.  No reference counterpart: the
+ * reference launches one uniform kernel per batch
+ * (hip/solver/batch_bicgstab_kernels.hip.cpp). */
 typedef struct {
     int32_t num_bins;
     int32_t reserved;
@@ -787,6 +791,7 @@ typedef struct {
     int32_t matrix_in_lds[GKOMI_BATCH_MAX_BINS];
     int32_t blocks_per_cu[GKOMI_BATCH_MAX_BINS];
     int32_t subwave[GKOMI_BATCH_MAX_BINS];
+    int32_t vectors[GKOMI_BATCH_MAX_BINS];
 } gkomi_batch_plan;
 
 /* Zero-filled scratch for the work queues; left zero-filled after every call. */

@@ -112,6 +112,7 @@ def test_lds_and_global_residency_agree(dev):
             plan = A.plan(block_size=bs)
             for i in range(plan.num_bins):
                 plan.matrix_in_lds[i] = mat
+                plan.vectors[i] = 1    # the LDS work-vector kernel
             x = torch.zeros_like(bd)
             iters, res = A.solve(bd, x, st, plan)
             outs.append((x.cpu().numpy(), iters.cpu().numpy()))
@@ -120,6 +121,71 @@ def test_lds_and_global_residency_agree(dev):
     np.testing.assert_array_equal(outs[2][0], outs[5][0])
     for xo, it in outs[1:]:
         np.testing.assert_allclose(xo, outs[0][0], rtol=0, atol=1e-8)
+
+
+@pytest.mark.parametrize("method", ["bicgstab", "cg"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("uniform", [False, True])
+def test_register_and_lds_work_vectors_agree(dev, method, dtype, uniform):
+    """The register work-vector kernel against the LDS work-vector kernel
+    with one lane per row: both sum every row in stored order by one thread
+    and reduce in the same order.  CG agrees bit for bit; BiCGSTAB in the
+    iteration counts, with solutions and residual norms within a few units
+    in the last place (the compiler fuses the BiCGSTAB updates' products
+    differently in the two kernels).  One and two rows per thread."""
+    if uniform:
+        k = KATS["batch_bicgstab"][0]
+        _, rp, ci, v, _, _, n = stencil_batch(dict(k, num_batch_items=64, num_rows=300))
+        A = D().DeviceBatchCsr.uniform(64, rp[: n + 1], ci[: rp[n]],
+                                       np.tile(v[: rp[n]], 64).astype(dtype), device=dev)
+        b = np.random.default_rng(3).uniform(-1, 1, 64 * n)
+    else:
+        maker = gen.uneven_spd_batch if method == "cg" else gen.uneven_batch
+        row_off, rp, ci, v, b = maker(400, n_min=16, n_max=512, seed=17)
+        A = D().DeviceBatchCsr(row_off, rp, ci, v.astype(dtype), device=dev)
+    bd = torch.as_tensor(b.astype(dtype)).to(dev)
+    tol = 1e-10 if dtype == np.float64 else 1e-5
+    st = A.settings(tol=tol, relative=True, max_iters=500, jacobi=True)
+    for bs in (256, 512):
+        outs = []
+        for vec in (1, 2):
+            plan = A.plan(block_size=bs)
+            for i in range(plan.num_bins):
+                plan.vectors[i] = vec
+                plan.matrix_in_lds[i] = 1
+                plan.subwave[i] = 1
+            x = torch.zeros_like(bd)
+            it, res = A.solve(bd, x, st, plan, method=method)
+            outs.append((x.cpu().numpy(), it.cpu().numpy(), res.cpu().numpy()))
+        np.testing.assert_array_equal(outs[0][1], outs[1][1])
+        if method == "cg":
+            np.testing.assert_array_equal(outs[0][0], outs[1][0])
+            np.testing.assert_array_equal(outs[0][2], outs[1][2])
+        else:
+            eps = np.finfo(dtype).eps
+            scale = float(np.abs(outs[0][0]).max())
+            np.testing.assert_allclose(outs[1][0], outs[0][0], rtol=16 * eps,
+                                       atol=4 * eps * scale)
+            # the final residual norms are small differences of O(1) terms
+            np.testing.assert_allclose(outs[1][2], outs[0][2], rtol=1e-2 if
+                                       dtype == np.float32 else 1e-6, atol=0)
+        assert int(outs[1][1].max()) < 500
+
+
+def test_register_vectors_refused_where_they_do_not_apply(dev):
+    from ginkgo_amd import _capi as C
+    row_off, rp, ci, v, b = gen.uneven_batch(20, n_min=300, n_max=512, seed=2)
+    A = D().DeviceBatchCsr(row_off, rp, ci, v, device=dev)
+    bd = torch.as_tensor(b).to(dev)
+    plan = A.plan(block_size=64)           # 512 rows need more than two rows per thread
+    for i in range(plan.num_bins):
+        plan.vectors[i] = 2
+    with pytest.raises(C.NotSupported):
+        A.solve(bd, torch.zeros_like(bd), A.settings(), plan)
+    for i in range(plan.num_bins):
+        plan.vectors[i] = 0                # automatic: the LDS-vector kernel runs
+    it, _ = A.solve(bd, torch.zeros_like(bd), A.settings(tol=1e-10, max_iters=300), plan)
+    assert int(it.max()) < 300
 
 
 def test_tuned_plan_and_padded_uniform_agree(dev):
