@@ -185,7 +185,8 @@ class BatchPlan(ctypes.Structure):
                 ("matrix_in_lds", ctypes.c_int32 * BATCH_MAX_BINS),
                 ("blocks_per_cu", ctypes.c_int32 * BATCH_MAX_BINS),
                 ("subwave", ctypes.c_int32 * BATCH_MAX_BINS),
-                ("vectors", ctypes.c_int32 * BATCH_MAX_BINS)]
+                ("vectors", ctypes.c_int32 * BATCH_MAX_BINS),
+                ("bin_max_row_nnz", ctypes.c_int64 * BATCH_MAX_BINS)]
 
 
 _P = ctypes.c_void_p

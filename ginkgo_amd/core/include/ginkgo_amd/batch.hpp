@@ -110,6 +110,11 @@ public:
     {
         return std::vector<int64>(size_.num_batch_items, descriptor().uniform_nnz);
     }
+    // longest row (stored entries) of each item, 0 where not known
+    virtual std::vector<int64> get_item_max_row_nnz() const
+    {
+        return std::vector<int64>(size_.num_batch_items, 0);
+    }
     virtual size_type get_total_rows() const
     {
         return size_.num_batch_items * size_.common_size.rows;
@@ -229,6 +234,7 @@ public:
         const std::vector<ValueType>& values);
     std::vector<int64> get_item_rows() const override { return item_rows_; }
     std::vector<int64> get_item_nnz() const override { return item_nnz_; }
+    std::vector<int64> get_item_max_row_nnz() const override { return item_max_row_nnz_; }
     size_type get_total_rows() const override
     {
         return row_offsets_host_.empty() ? 0 : static_cast<size_type>(row_offsets_host_.back());
@@ -249,7 +255,7 @@ private:
     array<IndexType> row_ptrs_;
     array<IndexType> col_idxs_;
     array<ValueType> values_;
-    std::vector<int64> row_offsets_host_, item_rows_, item_nnz_;
+    std::vector<int64> row_offsets_host_, item_rows_, item_nnz_, item_max_row_nnz_;
 };
 
 }  // namespace matrix

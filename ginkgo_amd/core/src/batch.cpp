@@ -280,13 +280,14 @@ std::unique_ptr<BlockDiagCsr<V, I>> BlockDiagCsr<V, I>::create_from_host(
         GKO_DIMENSION_MISMATCH_MSG("batch::matrix::BlockDiagCsr: inconsistent arrays");
     }
     const size_type num = row_offsets.size() - 1;
-    std::vector<int64> rows(num), nnz(num);
+    std::vector<int64> rows(num), nnz(num), max_row(num, 0);
     int64 max_rows = 0;
     for (size_type k = 0; k < num; ++k) {
         rows[k] = row_offsets[k + 1] - row_offsets[k];
         if (rows[k] < 0) GKO_DIMENSION_MISMATCH_MSG("BlockDiagCsr: decreasing row offsets");
         nnz[k] = row_ptrs[row_offsets[k + 1]] - row_ptrs[row_offsets[k]];
         for (int64 r = row_offsets[k]; r < row_offsets[k + 1]; ++r) {
+            max_row[k] = std::max<int64>(max_row[k], row_ptrs[r + 1] - row_ptrs[r]);
             for (I p = row_ptrs[r]; p < row_ptrs[r + 1]; ++p) {
                 if (col_idxs[p] < 0 || col_idxs[p] >= rows[k]) {
                     GKO_DIMENSION_MISMATCH_MSG(
@@ -306,6 +307,7 @@ std::unique_ptr<BlockDiagCsr<V, I>> BlockDiagCsr<V, I>::create_from_host(
     m->row_offsets_host_ = row_offsets;
     m->item_rows_ = std::move(rows);
     m->item_nnz_ = std::move(nnz);
+    m->item_max_row_nnz_ = std::move(max_row);
     return m;
 }
 
@@ -397,7 +399,7 @@ BatchKrylov<V, M>::BatchKrylov(std::shared_ptr<const Executor> exec, parameters_
     if (system_->get_common_size().rows != system_->get_common_size().cols) {
         GKO_DIMENSION_MISMATCH_MSG(what_of(who, "items must be square"));
     }
-    std::vector<int64> rows, nnz;
+    std::vector<int64> rows, nnz, max_row;
     const size_type num = system_->get_num_batch_items();
     auto mat = dynamic_cast<const matrix::BatchMatrix<V>*>(system_.get());
     if (!mat) {
@@ -407,6 +409,7 @@ BatchKrylov<V, M>::BatchKrylov(std::shared_ptr<const Executor> exec, parameters_
     desc_ = mat->descriptor();
     rows = mat->get_item_rows();
     nnz = mat->get_item_nnz();
+    max_row = mat->get_item_max_row_nnz();
     total_rows_ = mat->get_total_rows();
     // plan: items sorted by size (largest first), split into bins
     std::vector<int32> order(num);
@@ -422,14 +425,18 @@ BatchKrylov<V, M>::BatchKrylov(std::shared_ptr<const Executor> exec, parameters_
         size_type end = begin;
         while (end < num && rows[order[end]] > kBinEdges[e + 1]) ++end;
         if (end > begin) {
-            int64 mr = 0, mz = 0;
+            int64 mr = 0, mz = 0, ml = 0;
+            bool known = true;
             for (size_type k = begin; k < end; ++k) {
                 mr = std::max(mr, rows[order[k]]);
                 mz = std::max(mz, nnz[order[k]]);
+                ml = std::max(ml, max_row[order[k]]);
+                known = known && (max_row[order[k]] > 0 || nnz[order[k]] == 0);
             }
             plan_.bin_begin[nb] = static_cast<int64_t>(begin);
             plan_.bin_max_rows[nb] = mr;
             plan_.bin_max_nnz[nb] = mz;
+            plan_.bin_max_row_nnz[nb] = known ? ml : 0;
             plan_.block_size[nb] = mr > 256 ? 256 : (mr > 64 ? 128 : 64);
             plan_.matrix_in_lds[nb] = -1;
             plan_.blocks_per_cu[nb] = 0;

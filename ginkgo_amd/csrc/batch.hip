@@ -473,9 +473,15 @@ __global__ void __launch_bounds__(NThreads)
 // reference's batch BiCGSTAB (no breakdown guards, the half-step exit) or
 // batch CG (implicit norm sqrt|r.z|, iterations counted from 1).
 //
+// With K > 0 the matrix rows stay in registers as well: the thread holding
+// row i keeps its first K stored entries (values and columns), reads any
+// further ones from global memory in the products, and LDS holds only the
+// gathered vectors -- one LDS access per entry in a product instead of three.
+//
 // LDS layout per system slot (n = lds_rows, z = lds_nnz):
-//   T in0[n], T in1[n] (BiCGSTAB only), T vals[z], int32 rp[n+1], uint16 ci[z]
-template <typename T, int NT, int R, int Fmt, int Method>
+//   T in0[n], T in1[n] (BiCGSTAB only), then for K == 0
+//   T vals[z], int32 rp[n+1], uint16 ci[z]
+template <typename T, int NT, int R, int K, int Fmt, int Method>
 __global__ void __launch_bounds__(NT)
     batch_reg_kernel(int64_t num_systems, const int32_t* __restrict__ order,
                      const int64_t* __restrict__ row_off, const int32_t* __restrict__ rp,
@@ -534,11 +540,31 @@ __global__ void __launch_bounds__(NT)
         const int32_t* gci = ci + (Fmt == kFmtCsr ? 0 : nnz0);
         const T* gb = b + row0;
         T* gx = x + row0;
-        for (int i = tid; i < nnz; i += NT) {
-            lv[i] = gv[i];
-            lci[i] = static_cast<uint16_t>(gci[i]);
+        // K > 0: the own rows' first K entries in registers
+        constexpr int KR = K > 0 ? K : 1;
+        T mv[R][KR];
+        int32_t mc[R][KR];
+        int32_t mb[R], ml[R];
+        if constexpr (K > 0) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const int row = tid + j * NT;
+                mb[j] = row < n ? srp[row] - rp_base : 0;
+                ml[j] = row < n ? srp[row + 1] - rp_base - mb[j] : 0;
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    const bool in_row = q < ml[j];
+                    mv[j][q] = in_row ? gv[mb[j] + q] : T(0);
+                    mc[j][q] = in_row ? gci[mb[j] + q] : 0;
+                }
+            }
+        } else {
+            for (int i = tid; i < nnz; i += NT) {
+                lv[i] = gv[i];
+                lci[i] = static_cast<uint16_t>(gci[i]);
+            }
+            for (int i = tid; i <= n; i += NT) lrp[i] = srp[i] - rp_base;
         }
-        for (int i = tid; i <= n; i += NT) lrp[i] = srp[i] - rp_base;
         // own rows: prec, x, b
         T prec[R], xr[R], r[R];
         T bb = 0, unused = 0;
@@ -559,11 +585,29 @@ __global__ void __launch_bounds__(NT)
             const int row = tid + j * NT;
             T d = T(1);
             if (jacobi && row < n) {
-                const int32_t e1 = lrp[row + 1];
-                for (int32_t e = lrp[row]; e < e1; ++e) {
-                    if (lci[e] == row) {
-                        if (lv[e] != T(0)) d = T(1) / lv[e];
-                        break;
+                if constexpr (K > 0) {
+                    // the first stored diagonal entry, as the stored-order scan
+                    bool found = false;
+#pragma unroll
+                    for (int q = 0; q < K; ++q) {
+                        if (!found && q < ml[j] && mc[j][q] == row) {
+                            found = true;
+                            if (mv[j][q] != T(0)) d = T(1) / mv[j][q];
+                        }
+                    }
+                    for (int32_t e = mb[j] + K; !found && e < mb[j] + ml[j]; ++e) {
+                        if (gci[e] == row) {
+                            found = true;
+                            if (gv[e] != T(0)) d = T(1) / gv[e];
+                        }
+                    }
+                } else {
+                    const int32_t e1 = lrp[row + 1];
+                    for (int32_t e = lrp[row]; e < e1; ++e) {
+                        if (lci[e] == row) {
+                            if (lv[e] != T(0)) d = T(1) / lv[e];
+                            break;
+                        }
                     }
                 }
             }
@@ -575,7 +619,13 @@ __global__ void __launch_bounds__(NT)
             for (int j = 0; j < R; ++j) {
                 const int row = tid + j * NT;
                 T sum = 0;
-                if (row < n) {
+                if constexpr (K > 0) {
+#pragma unroll
+                    for (int q = 0; q < K; ++q) {
+                        if (q < ml[j]) sum += mv[j][q] * in[mc[j][q]];
+                    }
+                    for (int32_t e = mb[j] + K; e < mb[j] + ml[j]; ++e) sum += gv[e] * in[gci[e]];
+                } else if (row < n) {
                     const int32_t e1 = lrp[row + 1];
                     for (int32_t e = lrp[row]; e < e1; ++e) sum += lv[e] * in[lci[e]];
                 }
@@ -883,12 +933,12 @@ int check_batch(const gkomi_batch_csr* a, const gkomi_batch_settings* st)
     return GKOMI_SUCCESS;
 }
 
-template <typename T, int NT, int R, int Fmt, int Method>
+template <typename T, int NT, int R, int K, int Fmt, int Method>
 int launch_reg(hipStream_t stream, int grid, size_t lds, int64_t count, const int32_t* order,
                const gkomi_batch_csr* a, const T* b, T* x, const gkomi_batch_settings* st,
                int32_t* iters, T* res, unsigned* queue, int64_t lds_rows, int64_t lds_nnz)
 {
-    auto kern = batch_reg_kernel<T, NT, R, Fmt, Method>;
+    auto kern = batch_reg_kernel<T, NT, R, K, Fmt, Method>;
     if (lds > 64 * 1024) {
         GKOMI_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                             hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -911,26 +961,51 @@ int reg_rows_per_thread(int nt, int64_t rows)
     return 0;
 }
 
+// Entries per row the register kernel keeps in registers (16 or 32; 0 =
+// the matrix in LDS) for `nt` threads with `rpt` rows each, a bin whose
+// longest row has `max_row_nnz` entries (0 = unknown) and the plan's
+// matrix_in_lds (1 = LDS; 0 = registers, longer rows finishing from global
+// memory; -1 = registers when every row fits).  Two rows per thread keep at
+// most 16 entries each, and only with 256 or 512 threads.
+int reg_row_entries(int nt, int rpt, int64_t max_row_nnz, int mat)
+{
+    if (mat == 1 || !(rpt == 1 || (rpt == 2 && nt >= 256))) return 0;
+    const int cap = rpt == 1 ? 32 : 16;
+    if (max_row_nnz > 0 && max_row_nnz <= 16) return 16;
+    if (max_row_nnz > 0 && max_row_nnz <= cap) return cap;
+    return mat == 0 ? cap : 0;
+}
+
 template <typename T, int Fmt, int Method>
-int launch_reg_shape(int nt, int rpt, hipStream_t s, int grid, size_t lds, int64_t cnt,
+int launch_reg_shape(int nt, int rpt, int kreg, hipStream_t s, int grid, size_t lds, int64_t cnt,
                      const int32_t* order, const gkomi_batch_csr* a, const T* b, T* x,
                      const gkomi_batch_settings* st, int32_t* iters, T* res, unsigned* q,
                      int64_t rows, int64_t nnz)
 {
-#define GKOMI_REG_CASE(NTV, RV)                                                               \
-    if (nt == NTV && rpt == RV)                                                               \
-        return launch_reg<T, NTV, RV, Fmt, Method>(s, grid, lds, cnt, order, a, b, x, st, iters, \
-                                                   res, q, rows, nnz);
-    GKOMI_REG_CASE(64, 1)
-    GKOMI_REG_CASE(64, 2)
-    GKOMI_REG_CASE(128, 1)
-    GKOMI_REG_CASE(128, 2)
-    GKOMI_REG_CASE(256, 1)
-    GKOMI_REG_CASE(256, 2)
-    GKOMI_REG_CASE(512, 1)
-    GKOMI_REG_CASE(512, 2)
+#define GKOMI_REG_CASE(NTV, RV, KV)                                                            \
+    if (nt == NTV && rpt == RV && kreg == KV)                                                  \
+        return launch_reg<T, NTV, RV, KV, Fmt, Method>(s, grid, lds, cnt, order, a, b, x, st,   \
+                                                       iters, res, q, rows, nnz);
+#define GKOMI_REG_SHAPE(KV)                                                                    \
+    GKOMI_REG_CASE(64, 1, KV)                                                                  \
+    GKOMI_REG_CASE(128, 1, KV)                                                                 \
+    GKOMI_REG_CASE(256, 1, KV)                                                                 \
+    GKOMI_REG_CASE(512, 1, KV)
+    GKOMI_REG_SHAPE(0)
+    GKOMI_REG_CASE(64, 2, 0)
+    GKOMI_REG_CASE(128, 2, 0)
+    GKOMI_REG_CASE(256, 2, 0)
+    GKOMI_REG_CASE(512, 2, 0)
+    if constexpr (Fmt == kFmtUneven) {
+        GKOMI_REG_SHAPE(16)
+        GKOMI_REG_SHAPE(32)
+        GKOMI_REG_CASE(256, 2, 16)
+        GKOMI_REG_CASE(512, 2, 16)
+    }
+#undef GKOMI_REG_SHAPE
 #undef GKOMI_REG_CASE
-    return fail(GKOMI_INVALID_ARGUMENT, "batch: block size must be 64, 128, 256 or 512");
+    return fail(GKOMI_INVALID_ARGUMENT,
+                "batch: no register kernel for this block size, rows per thread and row length");
 }
 
 template <typename T, bool MatLds, int Method>
@@ -1006,7 +1081,14 @@ int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
                           "batch: vectors must be 0 (automatic), 1 (LDS) or 2 (registers)");
             const int nt = plan->block_size[bin];
             const int rpt = reg_rows_per_thread(nt, rows);
-            const size_t rlds = batch_reg_lds_bytes(sizeof(T), rows, nnz, Method);
+            // matrix rows in registers (uneven Csr): matrix_in_lds 0, or -1
+            // when the bin's longest row is known and short enough
+            const int kreg = fmt == kFmtUneven
+                ? reg_row_entries(nt, rpt, plan->bin_max_row_nnz[bin], plan->matrix_in_lds[bin])
+                : 0;
+            const size_t rlds = kreg > 0
+                ? batch_reg_lds_bytes(sizeof(T), rows, 0, Method) - 4 * (rows + 1)
+                : batch_reg_lds_bytes(sizeof(T), rows, nnz, Method);
             const bool applies = (fmt == kFmtUneven || fmt == kFmtCsr) && rpt > 0 &&
                                  rlds <= static_cast<size_t>(lds_limit);
             GKOMI_REQUIRE(want != 2 || applies, GKOMI_NOT_SUPPORTED,
@@ -1021,10 +1103,12 @@ int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
                 const int grid = static_cast<int>(std::max<int64_t>(
                     1, std::min<int64_t>(cnt, static_cast<int64_t>(num_cus()) * bpc)));
                 const int rc = fmt == kFmtUneven
-                    ? launch_reg_shape<T, kFmtUneven, Method>(nt, rpt, s, grid, rlds, cnt, order_bin,
-                                                              a, b, x, st, iters, res, qb, rows, nnz)
-                    : launch_reg_shape<T, kFmtCsr, Method>(nt, rpt, s, grid, rlds, cnt, order_bin,
-                                                           a, b, x, st, iters, res, qb, rows, nnz);
+                    ? launch_reg_shape<T, kFmtUneven, Method>(nt, rpt, kreg, s, grid, rlds, cnt,
+                                                              order_bin, a, b, x, st, iters, res,
+                                                              qb, rows, nnz)
+                    : launch_reg_shape<T, kFmtCsr, Method>(nt, rpt, kreg, s, grid, rlds, cnt,
+                                                           order_bin, a, b, x, st, iters, res, qb,
+                                                           rows, nnz);
                 if (rc) return rc;
                 continue;
             }
@@ -1237,15 +1321,20 @@ int batch_tune_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
                               batch_reg_lds_bytes(sizeof(T), one.bin_max_rows[0],
                                                   one.bin_max_nnz[0], Method) <=
                                   static_cast<size_t>(lds_limit);
+        one.bin_max_row_nnz[0] = plan->bin_max_row_nnz[bin];
         for (int nt : {64, 128, 256, 512}) {
-            // candidates: register work vectors (matrix in LDS), then LDS
-            // work vectors with the matrix in LDS and in global memory
-            for (int cand = 2; cand >= 0; --cand) {
-                const int vec = cand == 2 ? 2 : 1;
-                const int mat = cand == 2 ? 1 : cand;
-                if (vec == 2 && !(reg_fits && reg_rows_per_thread(nt, one.bin_max_rows[0]) > 0)) {
+            // candidates: register work vectors with the matrix rows in
+            // registers (uneven Csr, rows of known length) and in LDS, then
+            // LDS work vectors with the matrix in LDS and in global memory
+            for (int cand = 3; cand >= 0; --cand) {
+                const int vec = cand >= 2 ? 2 : 1;
+                const int mat = cand == 3 ? 0 : (cand == 2 ? 1 : cand);
+                const int rpt = reg_rows_per_thread(nt, one.bin_max_rows[0]);
+                if (cand == 3 && !(fmt == kFmtUneven && rpt > 0 &&
+                                   reg_row_entries(nt, rpt, one.bin_max_row_nnz[0], -1) > 0)) {
                     continue;
                 }
+                if (cand == 2 && !(reg_fits && rpt > 0)) continue;
                 if (vec == 1 && mat == 1 && !fits) continue;
                 one.block_size[0] = nt;
                 one.matrix_in_lds[0] = mat;

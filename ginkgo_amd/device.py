@@ -551,6 +551,12 @@ class DeviceBatchCsr:
         sizes = np.diff(row_off)
         nnzs = rp[row_off[1:]] - rp[row_off[:-1]]
         self.sizes, self.nnzs = sizes, nnzs
+        # longest row of each system
+        lens = np.diff(rp).astype(np.int64)
+        self.max_row_nnz = np.zeros(len(sizes), dtype=np.int64)
+        nonempty = sizes > 0
+        if lens.size and nonempty.any():
+            self.max_row_nnz[nonempty] = np.maximum.reduceat(lens, row_off[:-1][nonempty])
         self.num_systems = len(sizes)
         self.total_rows = int(row_off[-1])
         self.row_off = torch.as_tensor(row_off).to(dev)
@@ -572,6 +578,8 @@ class DeviceBatchCsr:
         nnz = int(rp[-1])
         self.sizes = np.full(num_systems, n, dtype=np.int64)
         self.nnzs = np.full(num_systems, nnz, dtype=np.int64)
+        self.max_row_nnz = np.full(num_systems, int(np.diff(rp).max()) if n else 0,
+                                   dtype=np.int64)
         self.num_systems = num_systems
         self.total_rows = num_systems * n
         self.row_off = None
@@ -614,6 +622,7 @@ class DeviceBatchCsr:
                 p.bin_begin[nb] = begin
                 p.bin_max_rows[nb] = int(ss[begin:end].max())
                 p.bin_max_nnz[nb] = int(nz[begin:end].max())
+                p.bin_max_row_nnz[nb] = int(self.max_row_nnz[order[begin:end]].max())
                 r = p.bin_max_rows[nb]
                 p.block_size[nb] = block_size or (256 if r > 256 else (128 if r > 64 else 64))
                 p.matrix_in_lds[nb] = -1

@@ -792,6 +792,10 @@ typedef struct {
     int32_t blocks_per_cu[GKOMI_BATCH_MAX_BINS];
     int32_t subwave[GKOMI_BATCH_MAX_BINS];
     int32_t vectors[GKOMI_BATCH_MAX_BINS];
+    /* longest row (stored entries) of any system in the bin, 0 = unknown;
+     * with register work vectors on an uneven batch, rows of up to 32
+     * entries then stay in registers too (matrix_in_lds -1 or 0) */
+    int64_t bin_max_row_nnz[GKOMI_BATCH_MAX_BINS];
 } gkomi_batch_plan;
 
 /* Zero-filled scratch for the work queues; left zero-filled after every call. */

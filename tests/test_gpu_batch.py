@@ -146,30 +146,34 @@ def test_register_and_lds_work_vectors_agree(dev, method, dtype, uniform):
     bd = torch.as_tensor(b.astype(dtype)).to(dev)
     tol = 1e-10 if dtype == np.float64 else 1e-5
     st = A.settings(tol=tol, relative=True, max_iters=500, jacobi=True)
+    # (vectors, matrix_in_lds): LDS work vectors; register work vectors with
+    # the matrix in LDS; and (uneven) with the matrix rows in registers
+    variants = [(1, 1), (2, 1)] + ([] if uniform else [(2, 0)])
     for bs in (256, 512):
         outs = []
-        for vec in (1, 2):
+        for vec, mat in variants:
             plan = A.plan(block_size=bs)
             for i in range(plan.num_bins):
                 plan.vectors[i] = vec
-                plan.matrix_in_lds[i] = 1
+                plan.matrix_in_lds[i] = mat
                 plan.subwave[i] = 1
             x = torch.zeros_like(bd)
             it, res = A.solve(bd, x, st, plan, method=method)
             outs.append((x.cpu().numpy(), it.cpu().numpy(), res.cpu().numpy()))
-        np.testing.assert_array_equal(outs[0][1], outs[1][1])
-        if method == "cg":
-            np.testing.assert_array_equal(outs[0][0], outs[1][0])
-            np.testing.assert_array_equal(outs[0][2], outs[1][2])
-        else:
-            eps = np.finfo(dtype).eps
-            scale = float(np.abs(outs[0][0]).max())
-            np.testing.assert_allclose(outs[1][0], outs[0][0], rtol=16 * eps,
-                                       atol=4 * eps * scale)
-            # the final residual norms are small differences of O(1) terms
-            np.testing.assert_allclose(outs[1][2], outs[0][2], rtol=1e-2 if
-                                       dtype == np.float32 else 1e-6, atol=0)
-        assert int(outs[1][1].max()) < 500
+        for o in outs[1:]:
+            np.testing.assert_array_equal(outs[0][1], o[1])
+            if method == "cg":
+                np.testing.assert_array_equal(outs[0][0], o[0])
+                np.testing.assert_array_equal(outs[0][2], o[2])
+            else:
+                eps = np.finfo(dtype).eps
+                scale = float(np.abs(outs[0][0]).max())
+                np.testing.assert_allclose(o[0], outs[0][0], rtol=16 * eps,
+                                           atol=4 * eps * scale)
+                # the final residual norms are small differences of O(1) terms
+                np.testing.assert_allclose(o[2], outs[0][2], rtol=1e-2 if
+                                           dtype == np.float32 else 1e-6, atol=0)
+            assert int(o[1].max()) < 500
 
 
 def test_register_vectors_refused_where_they_do_not_apply(dev):
