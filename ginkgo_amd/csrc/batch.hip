@@ -489,7 +489,8 @@ __global__ void __launch_bounds__(NT)
                      int64_t uni_nnz, const T* __restrict__ b, T* __restrict__ x,
                      int32_t max_iters, int32_t relative, double tol_d, int32_t jacobi,
                      int32_t* __restrict__ iters_out, T* __restrict__ res_out,
-                     unsigned* __restrict__ queue, int64_t lds_rows, int64_t lds_nnz)
+                     unsigned* __restrict__ queue, int64_t lds_rows, int64_t lds_nnz,
+                     int32_t dynamic)
 {
     static_assert(Fmt == kFmtUneven || Fmt == kFmtCsr, "register kernel: Csr formats");
     extern __shared__ __align__(16) unsigned char smem[];
@@ -510,9 +511,20 @@ __global__ void __launch_bounds__(NT)
     uint16_t* lci = reinterpret_cast<uint16_t*>(lrp + L + 1);
     const T tol = static_cast<T>(tol_d);
 
+    // the first system of a workgroup is its block index; further ones come
+    // from the queue (numbered after the grid) only when the launch has more
+    // systems than workgroups, so a launch with one system per workgroup
+    // touches no atomic counter
+    bool first = true;
     while (true) {
         __syncthreads();
-        if (tid == 0) slot = static_cast<int64_t>(atomicAdd(queue, 1u));
+        if (tid == 0) {
+            slot = first ? static_cast<int64_t>(blockIdx.x)
+                         : (dynamic ? static_cast<int64_t>(gridDim.x) +
+                                          static_cast<int64_t>(atomicAdd(queue, 1u))
+                                    : num_systems);
+        }
+        first = false;
         __syncthreads();
         const int64_t k = slot;
         if (k >= num_systems) break;
@@ -766,7 +778,7 @@ __global__ void __launch_bounds__(NT)
         }
     }
     // the last workgroup out resets the queue for the next launch
-    if (tid == 0) {
+    if (dynamic && tid == 0) {
         const unsigned d = atomicAdd(queue + 1, 1u);
         if (d == gridDim.x - 1) {
             queue[0] = 0;
@@ -933,6 +945,26 @@ int check_batch(const gkomi_batch_csr* a, const gkomi_batch_settings* st)
     return GKOMI_SUCCESS;
 }
 
+// Workgroups of `kern` resident per CU with `lds` bytes of dynamic LDS
+// (cached per kernel and LDS size).
+int resident_per_cu(const void* kern, int nt, size_t lds)
+{
+    static std::mutex mtx;
+    static std::vector<std::pair<std::pair<const void*, size_t>, int>> cache;
+    std::lock_guard<std::mutex> guard(mtx);
+    for (const auto& e : cache) {
+        if (e.first.first == kern && e.first.second == lds) return e.second;
+    }
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, nt, lds) != hipSuccess ||
+        nb < 1) {
+        (void)hipGetLastError();
+        nb = 1;
+    }
+    cache.push_back({{kern, lds}, nb});
+    return nb;
+}
+
 template <typename T, int NT, int R, int K, int Fmt, int Method>
 int launch_reg(hipStream_t stream, int grid, size_t lds, int64_t count, const int32_t* order,
                const gkomi_batch_csr* a, const T* b, T* x, const gkomi_batch_settings* st,
@@ -944,11 +976,19 @@ int launch_reg(hipStream_t stream, int grid, size_t lds, int64_t count, const in
                                             hipFuncAttributeMaxDynamicSharedMemorySize,
                                             static_cast<int>(lds)));
     }
+    // grid = 0: every workgroup resident at once (occupancy of this kernel)
+    if (grid <= 0) {
+        const int64_t res_wg =
+            static_cast<int64_t>(num_cus()) *
+            resident_per_cu(reinterpret_cast<const void*>(kern), NT, lds);
+        grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(count, res_wg)));
+    }
+    const int32_t dynamic = count > grid ? 1 : 0;
     kern<<<grid, NT, lds, stream>>>(count, order, a->row_offsets, a->row_ptrs, a->col_idxs,
                                     static_cast<const T*>(a->values), a->uniform_rows,
                                     a->uniform_nnz, b, x, st->max_iterations, st->tolerance_type,
                                     st->tolerance, st->preconditioner, iters, res, queue,
-                                    lds_rows, lds_nnz);
+                                    lds_rows, lds_nnz, dynamic);
     GKOMI_CHECK_LAUNCH("batch_reg_kernel");
     return GKOMI_SUCCESS;
 }
@@ -1095,13 +1135,12 @@ int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
                           "batch: register work vectors need a Csr batch, at most two rows "
                           "per thread and the matrix in LDS");
             if (want == 2 || (want == 0 && applies)) {
-                int bpc = plan->blocks_per_cu[bin];
-                if (bpc <= 0) {
-                    bpc = static_cast<int>(std::max<size_t>(
-                        1, std::min<size_t>(2048 / nt, (160 * 1024) / std::max<size_t>(rlds, 1))));
-                }
-                const int grid = static_cast<int>(std::max<int64_t>(
-                    1, std::min<int64_t>(cnt, static_cast<int64_t>(num_cus()) * bpc)));
+                // the plan's workgroups per CU, or 0: as many as are resident
+                const int bpc = plan->blocks_per_cu[bin];
+                const int grid = bpc > 0 ? static_cast<int>(std::max<int64_t>(
+                                               1, std::min<int64_t>(
+                                                      cnt, static_cast<int64_t>(num_cus()) * bpc)))
+                                         : 0;
                 const int rc = fmt == kFmtUneven
                     ? launch_reg_shape<T, kFmtUneven, Method>(nt, rpt, kreg, s, grid, rlds, cnt,
                                                               order_bin, a, b, x, st, iters, res,
