@@ -745,6 +745,22 @@ def main():
         _, ms = time_kernel_loop(lambda: P32.spmv(px32, py32), args.steps, args.warmup, stream)
         secondary["csr_f32_powerlaw1M_GBps"] = round(spmv_bytes(pn, pnnz, 4) / (ms * 1e-3) / 1e9, 1)
         secondary["powerlaw_nnz"] = pnnz
+        # the same products over 4 column panels (one 2 MB FP64 window of x
+        # per XCD pair); credited with the unpanelled algorithmic bytes
+        for tag, M, xx, yy, vb in (("f64", P, px, py, 8), ("f32", P32, px32, py32, 4)):
+            M.spmv(xx, yy)
+            y0 = yy.clone()
+            t0 = time.perf_counter()
+            M.build_panels(4)
+            torch.cuda.synchronize()
+            build_ms = (time.perf_counter() - t0) * 1e3
+            _, ms = time_kernel_loop(lambda: M.spmv(xx, yy), args.steps, args.warmup, stream)
+            secondary[f"csr_{tag}_powerlaw1M_panels4_GBps"] = round(
+                spmv_bytes(pn, pnnz, vb) / (ms * 1e-3) / 1e9, 1)
+            secondary[f"csr_{tag}_powerlaw1M_panels4_build_ms"] = round(build_ms, 3)
+            secondary[f"csr_{tag}_powerlaw1M_panels4_max_diff_vs_unpanelled"] = float(
+                (yy - y0).abs().max().item())
+            M.build_panels(0)
         # SELL-P of the power-law matrix (slice 64, padded to each slice's
         # longest row; the padding is not credited)
         for tag, M, xx, yy, vb in (("f64", P, px, py, 8), ("f32", P32, px32, py32, 4)):

@@ -90,7 +90,9 @@ class Csr(ctypes.Structure):
                 ("values", ctypes.c_void_p), ("strategy", ctypes.c_int32),
                 ("subwave", ctypes.c_int32), ("srow", ctypes.c_void_p),
                 ("num_srow", ctypes.c_int64), ("col_offsets", ctypes.c_void_p),
-                ("work", ctypes.c_void_p), ("work_bytes", ctypes.c_int64)]
+                ("work", ctypes.c_void_p), ("work_bytes", ctypes.c_int64),
+                ("panels", ctypes.c_void_p), ("panels_bytes", ctypes.c_int64),
+                ("num_panels", ctypes.c_int32)]
 
 
 class Sellp(ctypes.Structure):
@@ -228,6 +230,9 @@ _SIGS = {
     "gkomi_csr_max_row_nnz": (ctypes.c_int, [_CTX, _I64, _P, _P]),
     "gkomi_csr_strategy_work_bytes": (_I64, [_I32, _I64, _I64]),
     "gkomi_csr_col_offsets": (ctypes.c_int, [_CTX, ctypes.POINTER(Csr), _P, _P]),
+    "gkomi_csr_panels_bytes": (_I64, [_I64, _I64, _I32, _I32]),
+    "gkomi_csr_build_panels_f64": (ctypes.c_int, [_CTX, ctypes.POINTER(Csr), _I32, _P, _I64]),
+    "gkomi_csr_build_panels_f32": (ctypes.c_int, [_CTX, ctypes.POINTER(Csr), _I32, _P, _I64]),
     "gkomi_csr_transpose_workspace_bytes": (_SZ, [_I64, _I64, _I64]),
     "gkomi_scan_workspace_bytes": (_SZ, [_I64]),
     "gkomi_sellp_compute_slice_sets": (ctypes.c_int, [_CTX, _I64, _P, _I64, _I64, _P, _P, _P,

@@ -1,5 +1,5 @@
 // CSR SpMV for gfx950 plus the CSR-side index work (load-balance partition,
-// longest row) and the scalar Jacobi generate / apply.
+// column panels, longest row) and the scalar Jacobi generate / apply.
 //
 // Kernels:
 //  * csr_spmv_subwave<SW, RPG>: SW lanes of a wave64 per row, each lane group
@@ -9,14 +9,19 @@
 //  * csr_spmv_balanced: workgroups own contiguous nnz ranges of the
 //    partition `srow`; the block's row pointers and the products a_ij * x_j
 //    are staged in LDS, then rows are reduced thread-per-row (short) or
-//    wave-per-row (long).  Blocks that do not fit LDS walk their rows
-//    wave-per-row from global memory.  Fixed summation order: deterministic.
+//    wave-per-row (long).  Ranges that do not fit LDS take several passes;
+//    a row beyond the LDS capacity is summed by the whole workgroup from
+//    memory.  Fixed summation order: deterministic.
+//  * csr_spmv_panels + panel_combine_kernel: the same row reduction over a
+//    column-panel copy of the matrix, each XCD on one panel, then the panel
+//    sums of a row added in panel order.
 //  * csr_spmm_rowwave: one wave per row, lanes over right-hand sides (the
 //    many-RHS case).
 #include <algorithm>
 #include <cmath>
 
 #include "common.hpp"
+#include "scan.hpp"
 #include "spmv_common.hpp"
 
 namespace gkomi {
@@ -63,6 +68,122 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// One LDS pass over the rows [rs, re) of a CSR product, which hold at most
+// kLbLdsRows rows and kLbLdsNnz entries: the row pointers and the products
+// a_ij * x_j are staged in LDS, then rows up to kLbShortRow entries are
+// summed thread-per-row (sequentially) and longer ones wave-per-row.
+template <typename T, int Mode>
+__device__ __forceinline__ void csr_rows_lds_pass(int32_t rs, int32_t re,
+                                                  const int32_t* __restrict__ rp,
+                                                  const int32_t* __restrict__ ci,
+                                                  const T* __restrict__ v,
+                                                  const T* __restrict__ b, int64_t bs,
+                                                  T* __restrict__ c, int64_t cs,
+                                                  const T* __restrict__ alpha,
+                                                  const T* __restrict__ beta, T* prod,
+                                                  int32_t* lrp, int32_t* long_rows,
+                                                  int32_t* num_long)
+{
+    const int32_t ps = rp[rs];
+    const int32_t nrows = re - rs;
+    const int32_t nnzb = rp[re] - ps;
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    if (threadIdx.x == 0) *num_long = 0;
+    for (int i = threadIdx.x; i <= nrows; i += kBlock) {
+        lrp[i] = rp[rs + i] - ps;
+    }
+    for (int i = threadIdx.x; i < nnzb; i += kBlock) {
+        prod[i] = v[ps + i] * b[static_cast<int64_t>(ci[ps + i]) * bs];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nrows; i += kBlock) {
+        const int32_t s = lrp[i];
+        const int32_t e = lrp[i + 1];
+        if (e - s <= kLbShortRow) {
+            T sum = 0;
+            for (int32_t j = s; j < e; ++j) sum += prod[j];
+            store_result<Mode>(c + static_cast<int64_t>(rs + i) * cs, sum, alpha, beta);
+        } else {
+            long_rows[atomicAdd(num_long, 1)] = i;
+        }
+    }
+    __syncthreads();
+    const int nl = *num_long;
+    for (int t = wave; t < nl; t += kWavesPerBlock) {
+        const int i = long_rows[t];
+        const int32_t s = lrp[i];
+        const int32_t e = lrp[i + 1];
+        T sum = 0;
+        for (int32_t j = s + lane; j < e; j += kWave) sum += prod[j];
+        sum = wave_sum(sum);
+        if (lane == 0) {
+            store_result<Mode>(c + static_cast<int64_t>(rs + i) * cs, sum, alpha, beta);
+        }
+    }
+}
+
+// The rows [rs, re) of a CSR product, reduced by one workgroup.  rp holds
+// absolute positions into ci / v.  A range that fits LDS takes one pass;
+// a larger one is cut into consecutive passes that fit, and a row longer
+// than the LDS capacity is summed by the whole workgroup from memory.  The
+// summation order depends only on the range: deterministic.
+template <typename T, int Mode>
+__device__ __forceinline__ void csr_rows_staged(int32_t rs, int32_t re,
+                                                const int32_t* __restrict__ rp,
+                                                const int32_t* __restrict__ ci,
+                                                const T* __restrict__ v,
+                                                const T* __restrict__ b, int64_t bs,
+                                                T* __restrict__ c, int64_t cs,
+                                                const T* __restrict__ alpha,
+                                                const T* __restrict__ beta)
+{
+    __shared__ T prod[kLbLdsNnz];
+    __shared__ int32_t lrp[kLbLdsRows + 1];
+    __shared__ int32_t long_rows[kLbLdsRows];
+    __shared__ int32_t num_long;
+    __shared__ T red[kWavesPerBlock];
+    if (re - rs <= kLbLdsRows && rp[re] - rp[rs] <= kLbLdsNnz) {
+        csr_rows_lds_pass<T, Mode>(rs, re, rp, ci, v, b, bs, c, cs, alpha, beta, prod, lrp,
+                                   long_rows, &num_long);
+        return;
+    }
+    // every thread takes the same steps (uniform control flow)
+    int32_t cur = rs;
+    while (cur < re) {
+        const int32_t base = rp[cur];
+        // the longest run [cur, end) with at most kLbLdsRows rows and
+        // kLbLdsNnz entries
+        int32_t lo = cur;
+        int32_t hi = min(re, cur + kLbLdsRows);
+        while (lo < hi) {
+            const int32_t mid = lo + (hi - lo + 1) / 2;
+            if (rp[mid] - base <= kLbLdsNnz) {
+                lo = mid;
+            } else {
+                hi = mid - 1;
+            }
+        }
+        if (lo == cur) {
+            const int32_t e = rp[cur + 1];
+            T sum = 0;
+            for (int32_t j = base + threadIdx.x; j < e; j += kBlock) {
+                sum += v[j] * b[static_cast<int64_t>(ci[j]) * bs];
+            }
+            sum = block_sum(sum, red);
+            if (threadIdx.x == 0) {
+                store_result<Mode>(c + static_cast<int64_t>(cur) * cs, sum, alpha, beta);
+            }
+            cur += 1;
+        } else {
+            csr_rows_lds_pass<T, Mode>(cur, lo, rp, ci, v, b, bs, c, cs, alpha, beta, prod, lrp,
+                                       long_rows, &num_long);
+            __syncthreads();
+            cur = lo;
+        }
+    }
+}
+
 template <typename T, int Mode>
 __global__ void __launch_bounds__(kBlock)
     csr_spmv_balanced(int64_t n_rows, const int32_t* __restrict__ rp,
@@ -71,10 +192,6 @@ __global__ void __launch_bounds__(kBlock)
                       const int32_t* __restrict__ srow, int64_t num_srow, int64_t group,
                       const T* __restrict__ alpha, const T* __restrict__ beta)
 {
-    __shared__ T prod[kLbLdsNnz];
-    __shared__ int32_t lrp[kLbLdsRows + 1];
-    __shared__ int32_t long_rows[kLbLdsRows];
-    __shared__ int32_t num_long;
     // workgroup k takes the warps [k * group, (k + 1) * group) of the
     // partition and owns the rows [srow[k * group], srow[(k + 1) * group]):
     // row 0 onwards for the first, up to n_rows for the last.  Any
@@ -87,65 +204,11 @@ __global__ void __launch_bounds__(kBlock)
     int64_t re64 = w1 >= num_srow ? n_rows : srow[w1];
     rs64 = rs64 < 0 ? 0 : (rs64 > n_rows ? n_rows : rs64);
     re64 = re64 < 0 ? 0 : (re64 > n_rows ? n_rows : re64);
-    const int32_t rs = static_cast<int32_t>(rs64);
-    const int32_t re = static_cast<int32_t>(re64);
-    if (rs >= re) {
+    if (rs64 >= re64) {
         return;
     }
-    const int32_t ps = rp[rs];
-    const int32_t pe = rp[re];
-    const int32_t nrows = re - rs;
-    const int32_t nnzb = pe - ps;
-    const int lane = threadIdx.x % kWave;
-    const int wave = threadIdx.x / kWave;
-    if (nnzb <= kLbLdsNnz && nrows <= kLbLdsRows) {
-        if (threadIdx.x == 0) num_long = 0;
-        for (int i = threadIdx.x; i <= nrows; i += kBlock) {
-            lrp[i] = rp[rs + i] - ps;
-        }
-        for (int i = threadIdx.x; i < nnzb; i += kBlock) {
-            prod[i] = v[ps + i] * b[static_cast<int64_t>(ci[ps + i]) * bs];
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < nrows; i += kBlock) {
-            const int32_t s = lrp[i];
-            const int32_t e = lrp[i + 1];
-            if (e - s <= kLbShortRow) {
-                T sum = 0;
-                for (int32_t j = s; j < e; ++j) sum += prod[j];
-                store_result<Mode>(c + static_cast<int64_t>(rs + i) * cs, sum, alpha, beta);
-            } else {
-                long_rows[atomicAdd(&num_long, 1)] = i;
-            }
-        }
-        __syncthreads();
-        const int nl = num_long;
-        for (int t = wave; t < nl; t += kWavesPerBlock) {
-            const int i = long_rows[t];
-            const int32_t s = lrp[i];
-            const int32_t e = lrp[i + 1];
-            T sum = 0;
-            for (int32_t j = s + lane; j < e; j += kWave) sum += prod[j];
-            sum = wave_sum(sum);
-            if (lane == 0) {
-                store_result<Mode>(c + static_cast<int64_t>(rs + i) * cs, sum, alpha, beta);
-            }
-        }
-    } else {
-        // rows that do not fit LDS: one wave per row straight from memory
-        for (int32_t i = wave; i < nrows; i += kWavesPerBlock) {
-            const int64_t row = rs + i;
-            const int32_t e = rp[row + 1];
-            T sum = 0;
-            for (int32_t j = rp[row] + lane; j < e; j += kWave) {
-                sum += v[j] * b[static_cast<int64_t>(ci[j]) * bs];
-            }
-            sum = wave_sum(sum);
-            if (lane == 0) {
-                store_result<Mode>(c + row * cs, sum, alpha, beta);
-            }
-        }
-    }
+    csr_rows_staged<T, Mode>(static_cast<int32_t>(rs64), static_cast<int32_t>(re64), rp, ci, v,
+                             b, bs, c, cs, alpha, beta);
 }
 
 template <typename T, int Mode>
@@ -203,6 +266,207 @@ __global__ void __launch_bounds__(kBlock)
             }
         }
         srow[w] = static_cast<int32_t>(lo);
+    }
+}
+
+// ---------------------------------------------------------- column panels ----
+// The columns split into P equal panels (P = 2, 4 or 8, width W =
+// ceil(n_cols / P)); panel p keeps its own CSR copy of the entries with
+// columns in [p W, (p + 1) W), rows in the same order and the entries of a row
+// in their original order.  Workgroup b of a product works on panel b % P;
+// the hardware places workgroup b on XCD b % 8, so each XCD gathers from one
+// window of x of n_cols / P values, which for P = 8 and an 8 MB FP64 x stays
+// in that XCD's 4 MB L2.  Panel p writes its row sums to partial[p][row];
+// a second pass adds the P partial sums of a row in panel order.
+//
+// Panel storage in one device block (each part 256-byte aligned):
+//   row_ptrs  int32 [P n_rows + 1]: panel p's row i starts at
+//             row_ptrs[p n_rows + i]; one exclusive scan over the (panel,
+//             row) counts, so panel p ends where panel p + 1 starts
+//   col_idxs  int32 [nnz], values T [nnz]
+//   bounds    int32 [P (C + 1)]: chunk k of panel p owns the rows
+//             [bounds[p (C + 1) + k], bounds[p (C + 1) + k + 1])
+//   partial   T [P n_rows] (also the scan's workspace while building)
+constexpr int kMaxPanels = 8;
+// rows plus entries per chunk of an evenly filled panel (a chunk beyond the
+// LDS capacity takes several passes); on the 1M-row power-law matrix in 4
+// panels 768 is 3% slower, 384 35% slower
+constexpr int64_t kPanelChunk = 1536;
+
+struct panel_layout {
+    int64_t P, C, n, nnz;
+    size_t row_ptrs, col_idxs, values, bounds, partial, total;
+};
+
+inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// Chunks per panel: at most kPanelChunk rows plus entries each for an
+// evenly filled panel.
+inline int64_t panel_chunks(int64_t n_rows, int64_t nnz, int64_t P)
+{
+    return std::max<int64_t>(1, ceildiv(n_rows + ceildiv(nnz, P), kPanelChunk));
+}
+
+inline panel_layout make_panel_layout(int64_t n_rows, int64_t nnz, int64_t P, size_t vb)
+{
+    panel_layout L{};
+    L.P = P;
+    L.n = n_rows;
+    L.nnz = nnz;
+    L.C = panel_chunks(n_rows, nnz, P);
+    size_t off = 0;
+    L.row_ptrs = off;
+    off += align256(sizeof(int32_t) * static_cast<size_t>(P * n_rows + 1));
+    L.col_idxs = off;
+    off += align256(sizeof(int32_t) * static_cast<size_t>(nnz));
+    L.values = off;
+    off += align256(vb * static_cast<size_t>(nnz));
+    L.bounds = off;
+    off += align256(sizeof(int32_t) * static_cast<size_t>(P * (L.C + 1)));
+    L.partial = off;
+    off += align256(std::max(vb * static_cast<size_t>(P * n_rows),
+                             scan_ws_bytes(P * n_rows, sizeof(int32_t))));
+    L.total = off;
+    return L;
+}
+
+__device__ __forceinline__ int panel_of(int32_t col, int64_t width, int P)
+{
+    const int64_t q = col < 0 ? 0 : col / width;
+    return q >= P ? P - 1 : static_cast<int>(q);
+}
+
+// entries of each row per panel: counts[p n_rows + row], one wave per row
+__global__ void __launch_bounds__(kBlock)
+    panel_count_kernel(int64_t n_rows, const int32_t* __restrict__ rp,
+                       const int32_t* __restrict__ ci, int64_t width, int P,
+                       int32_t* __restrict__ counts)
+{
+    const int lane = threadIdx.x % kWave;
+    const int64_t num_waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+    for (int64_t row = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+         row < n_rows; row += num_waves) {
+        const int32_t e = rp[row + 1];
+        int32_t mine = 0;  // lane q < P counts panel q
+        for (int32_t j0 = rp[row]; j0 < e; j0 += kWave) {
+            const int32_t j = j0 + lane;
+            const int p = j < e ? panel_of(ci[j], width, P) : -1;
+#pragma unroll
+            for (int q = 0; q < kMaxPanels; ++q) {
+                const uint64_t m = __ballot(p == q);
+                if (lane == q) mine += __popcll(m);
+            }
+        }
+        if (lane < P) counts[lane * n_rows + row] = mine;
+    }
+}
+
+// copies every entry to its panel, keeping the order inside a row: the
+// position is the panel row's start plus the entries of that row and panel
+// in earlier 64-entry steps plus those of lower lanes in this step
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    panel_scatter_kernel(int64_t n_rows, const int32_t* __restrict__ rp,
+                         const int32_t* __restrict__ ci, const T* __restrict__ v,
+                         int64_t width, int P, const int32_t* __restrict__ prp,
+                         int32_t* __restrict__ pci, T* __restrict__ pv)
+{
+    const int lane = threadIdx.x % kWave;
+    const uint64_t below = (uint64_t{1} << lane) - 1;
+    const int64_t num_waves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+    for (int64_t row = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+         row < n_rows; row += num_waves) {
+        int32_t next[kMaxPanels];
+#pragma unroll
+        for (int q = 0; q < kMaxPanels; ++q) next[q] = q < P ? prp[q * n_rows + row] : 0;
+        const int32_t e = rp[row + 1];
+        for (int32_t j0 = rp[row]; j0 < e; j0 += kWave) {
+            const int32_t j = j0 + lane;
+            const int32_t col = j < e ? ci[j] : 0;
+            const int p = j < e ? panel_of(col, width, P) : -1;
+            int32_t pos = 0;
+#pragma unroll
+            for (int q = 0; q < kMaxPanels; ++q) {
+                const uint64_t m = __ballot(p == q);
+                if (p == q) pos = next[q] + __popcll(m & below);
+                next[q] += __popcll(m);
+            }
+            if (p >= 0) {
+                pci[pos] = col;
+                pv[pos] = v[j];
+            }
+        }
+    }
+}
+
+// chunk bounds of every panel: chunk k of panel p starts at the first row i
+// whose rows-plus-entries count through row i, (i + 1) + (entries of panel
+// p in rows 0..i), exceeds k (n_rows + nnz_p) / C; chunk 0 starts at row 0,
+// chunk C at n_rows
+__global__ void __launch_bounds__(kBlock)
+    panel_bounds_kernel(int64_t n_rows, int P, int64_t C, const int32_t* __restrict__ prp,
+                        int32_t* __restrict__ bounds)
+{
+    const int64_t total_items = P * (C + 1);
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t idx = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+         idx < total_items; idx += stride) {
+        const int64_t p = idx / (C + 1);
+        const int64_t k = idx - p * (C + 1);
+        const int32_t* rp = prp + p * n_rows;
+        const int64_t base = rp[0];
+        const int64_t total = n_rows + (static_cast<int64_t>(rp[n_rows]) - base);
+        const int64_t target = k * total / C;
+        int64_t lo = 0;
+        int64_t hi = n_rows;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) / 2;
+            if (mid + 1 + (static_cast<int64_t>(rp[mid + 1]) - base) > target) {
+                hi = mid;
+            } else {
+                lo = mid + 1;
+            }
+        }
+        bounds[idx] = static_cast<int32_t>(lo);
+    }
+}
+
+// partial[p][row] for every panel: workgroup b takes chunk b / P of panel
+// b % P (grid P C)
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    csr_spmv_panels(int64_t n_rows, int P, int64_t C, const int32_t* __restrict__ prp,
+                    const int32_t* __restrict__ pci, const T* __restrict__ pv,
+                    const int32_t* __restrict__ bounds, const T* __restrict__ b, int64_t bs,
+                    T* __restrict__ partial)
+{
+    const int64_t p = blockIdx.x % P;
+    const int64_t k = blockIdx.x / P;
+    const int32_t* bnd = bounds + p * (C + 1) + k;
+    int32_t rs = bnd[0];
+    int32_t re = bnd[1];
+    rs = rs < 0 ? 0 : (rs > n_rows ? static_cast<int32_t>(n_rows) : rs);
+    re = re < rs ? rs : (re > n_rows ? static_cast<int32_t>(n_rows) : re);
+    if (rs >= re) {
+        return;
+    }
+    csr_rows_staged<T, kPlain>(rs, re, prp + p * n_rows, pci, pv, b, bs, partial + p * n_rows,
+                               1, static_cast<const T*>(nullptr), static_cast<const T*>(nullptr));
+}
+
+// c[row] <- sum over panels of partial[p][row], in panel order
+template <typename T, int Mode>
+__global__ void __launch_bounds__(kBlock)
+    panel_combine_kernel(int64_t n_rows, int P, const T* __restrict__ partial,
+                         T* __restrict__ c, int64_t cs, const T* __restrict__ alpha,
+                         const T* __restrict__ beta)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t row = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; row < n_rows;
+         row += stride) {
+        T sum = partial[row];
+        for (int p = 1; p < P; ++p) sum += partial[p * n_rows + row];
+        store_result<Mode>(c + row * cs, sum, alpha, beta);
     }
 }
 
@@ -332,6 +596,74 @@ subwave_shape pick_subwave(const gkomi_csr* a, int value_bytes)
     return sh;
 }
 
+int check_panels(const gkomi_csr* a, size_t vb, panel_layout* L)
+{
+    GKOMI_REQUIRE(a->num_panels == 2 || a->num_panels == 4 || a->num_panels == 8,
+                  GKOMI_INVALID_ARGUMENT, "CSR: column panels: num_panels must be 2, 4 or 8");
+    *L = make_panel_layout(a->n_rows, a->nnz, a->num_panels, vb);
+    GKOMI_REQUIRE(a->panels_bytes >= static_cast<int64_t>(L->total), GKOMI_INVALID_ARGUMENT,
+                  "CSR: column panels smaller than gkomi_csr_panels_bytes");
+    GKOMI_REQUIRE(L->P * L->C < (int64_t{1} << 31) && L->P * a->n_rows < (int64_t{1} << 31),
+                  GKOMI_INVALID_ARGUMENT, "CSR: column panels too large");
+    return GKOMI_SUCCESS;
+}
+
+template <typename T, int Mode>
+int csr_spmv_panels_launch(hipStream_t stream, const gkomi_csr* a, const T* b, int64_t bs, T* c,
+                           int64_t cs, const T* alpha, const T* beta)
+{
+    panel_layout L;
+    if (int rc = check_panels(a, sizeof(T), &L)) return rc;
+    char* base = static_cast<char*>(a->panels);
+    T* partial = reinterpret_cast<T*>(base + L.partial);
+    csr_spmv_panels<T><<<static_cast<unsigned>(L.P * L.C), kBlock, 0, stream>>>(
+        a->n_rows, static_cast<int>(L.P), L.C, reinterpret_cast<const int32_t*>(base + L.row_ptrs),
+        reinterpret_cast<const int32_t*>(base + L.col_idxs),
+        reinterpret_cast<const T*>(base + L.values),
+        reinterpret_cast<const int32_t*>(base + L.bounds), b, bs, partial);
+    GKOMI_CHECK_LAUNCH("csr_spmv_panels");
+    panel_combine_kernel<T, Mode><<<grid_for(a->n_rows, kBlock, 16), kBlock, 0, stream>>>(
+        a->n_rows, static_cast<int>(L.P), partial, c, cs, alpha, beta);
+    GKOMI_CHECK_LAUNCH("panel_combine_kernel");
+    return GKOMI_SUCCESS;
+}
+
+template <typename T>
+int csr_build_panels_impl(const gkomi_ctx* ctx, const gkomi_csr* a, int32_t num_panels,
+                          void* panels, int64_t panels_bytes)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_csr(a)) return rc;
+    gkomi_csr d = *a;
+    d.panels = panels;
+    d.panels_bytes = panels_bytes;
+    d.num_panels = num_panels;
+    panel_layout L;
+    if (int rc = check_panels(&d, sizeof(T), &L)) return rc;
+    if (a->n_rows == 0) return GKOMI_SUCCESS;
+    GKOMI_REQUIRE(panels != nullptr, GKOMI_INVALID_ARGUMENT, "CSR: null column panels");
+    const hipStream_t stream = stream_of(ctx);
+    char* base = static_cast<char*>(panels);
+    int32_t* prp = reinterpret_cast<int32_t*>(base + L.row_ptrs);
+    const int P = num_panels;
+    const int64_t width = std::max<int64_t>(1, ceildiv(a->n_cols, P));
+    const unsigned g = grid_for(a->n_rows, kWavesPerBlock, 32);
+    panel_count_kernel<<<g, kBlock, 0, stream>>>(a->n_rows, a->row_ptrs, a->col_idxs, width, P,
+                                                 prp);
+    GKOMI_CHECK_LAUNCH("panel_count_kernel");
+    if (int rc = exclusive_scan<int32_t>(stream, L.P * a->n_rows, prp, prp, base + L.partial)) {
+        return rc;
+    }
+    panel_scatter_kernel<T><<<g, kBlock, 0, stream>>>(
+        a->n_rows, a->row_ptrs, a->col_idxs, static_cast<const T*>(a->values), width, P, prp,
+        reinterpret_cast<int32_t*>(base + L.col_idxs), reinterpret_cast<T*>(base + L.values));
+    GKOMI_CHECK_LAUNCH("panel_scatter_kernel");
+    panel_bounds_kernel<<<grid_for(L.P * (L.C + 1), kBlock), kBlock, 0, stream>>>(
+        a->n_rows, P, L.C, prp, reinterpret_cast<int32_t*>(base + L.bounds));
+    GKOMI_CHECK_LAUNCH("panel_bounds_kernel");
+    return GKOMI_SUCCESS;
+}
+
 template <typename T, int Mode>
 int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const T* b, int64_t bs,
                     T* c, int64_t cs, const T* alpha, const T* beta)
@@ -344,6 +676,9 @@ int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const 
                                                           beta);
         GKOMI_CHECK_LAUNCH("csr_spmm_rowwave");
         return GKOMI_SUCCESS;
+    }
+    if (a->panels != nullptr && a->num_panels > 0) {
+        return csr_spmv_panels_launch<T, Mode>(stream, a, b, bs, c, cs, alpha, beta);
     }
     if (a->strategy == GKOMI_CSR_MERGE_PATH) {
         return csr_spmv_merge_launch<T, Mode>(stream, a, b, bs, c, cs, alpha, beta);
@@ -481,6 +816,29 @@ int64_t gkomi_csr_lb_clac_size(int64_t nnz, int64_t nwarps, int32_t warp_size,
         multiple = nnz >= 10000000 ? 64 : (nnz >= 1000000 ? 16 : 8);
     }
     return std::min(ceildiv(nnz, warp_size), nwarps * multiple);
+}
+
+int64_t gkomi_csr_panels_bytes(int64_t n_rows, int64_t nnz, int32_t num_panels,
+                               int32_t value_type)
+{
+    if (n_rows < 0 || nnz < 0 || (num_panels != 2 && num_panels != 4 && num_panels != 8) ||
+        (value_type != GKOMI_VALUE_F64 && value_type != GKOMI_VALUE_F32)) {
+        return -1;
+    }
+    const size_t vb = value_type == GKOMI_VALUE_F64 ? sizeof(double) : sizeof(float);
+    return static_cast<int64_t>(make_panel_layout(n_rows, nnz, num_panels, vb).total);
+}
+
+int gkomi_csr_build_panels_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int32_t num_panels,
+                               void* panels, int64_t panels_bytes)
+{
+    return csr_build_panels_impl<double>(ctx, a, num_panels, panels, panels_bytes);
+}
+
+int gkomi_csr_build_panels_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int32_t num_panels,
+                               void* panels, int64_t panels_bytes)
+{
+    return csr_build_panels_impl<float>(ctx, a, num_panels, panels, panels_bytes);
 }
 
 int gkomi_csr_lb_partition(const gkomi_ctx* ctx, int64_t n_rows, const int32_t* row_ptrs,

@@ -175,6 +175,31 @@ class DeviceCsr:
             return True
         return False
 
+    def build_panels(self, num_panels=8):
+        """Builds the column panels (gkomi_csr_build_panels_*): a copy of the
+        matrix in num_panels (2, 4 or 8) column panels, one per XCD group, so
+        that the gathers from b of a single right-hand-side SpMV stay within
+        one XCD's L2 window; every such SpMV then uses them.  The copy is
+        taken now: call again after changing the values.  Results differ
+        from the unpanelled product only in the order of a row's sum (panel by
+        panel), and are deterministic.  num_panels 0 removes them."""
+        num_panels = int(num_panels)
+        self.panel_buf, self.num_panels = None, 0
+        if num_panels == 0:
+            self.desc = self._desc()
+            return
+        nb = int(C.lib().gkomi_csr_panels_bytes(self.n_rows, self.nnz, num_panels,
+                                                value_code(self.dtype)))
+        if nb < 0:
+            raise C.InvalidArgument(f"column panels: num_panels {num_panels} not 2, 4 or 8")
+        buf = workspace(nb, self.device)
+        c = ctx(self.device)
+        d = self._desc()
+        C.call(f"gkomi_csr_build_panels_{suffix(self.dtype)}", ctypes.byref(c), ctypes.byref(d),
+               num_panels, ptr(buf), nb)
+        self.panel_buf, self.num_panels = buf, num_panels
+        self.desc = self._desc()
+
     def _desc(self):
         d = C.Csr()
         d.n_rows, d.n_cols, d.nnz = self.n_rows, self.n_cols, self.nnz
@@ -191,6 +216,10 @@ class DeviceCsr:
         work = getattr(self, "work", None)
         d.work = ptr(work).value if work is not None else None
         d.work_bytes = work.numel() if work is not None else 0
+        buf = getattr(self, "panel_buf", None)
+        d.panels = ptr(buf).value if buf is not None else None
+        d.panels_bytes = buf.numel() if buf is not None else 0
+        d.num_panels = getattr(self, "num_panels", 0) if buf is not None else 0
         return d
 
     def spmv(self, b, out=None, stream=None):

@@ -192,6 +192,16 @@ typedef struct {
      * descriptor writes it, so SpMVs of one matrix must not overlap. */
     void* work;
     int64_t work_bytes;
+    /* Optional column panels (NULL = absent, no reference counterpart),
+     * built by gkomi_csr_build_panels_*: a copy of the matrix split into
+     * num_panels (2, 4 or 8) column panels, so that each XCD gathers from one
+     * panel's window of b.  When present, every single right-hand-side SpMV
+     * through the descriptor (any strategy, same value type as the build)
+     * uses them; they also hold the panels' partial sums, so SpMVs of one
+     * matrix must not overlap.  Rebuild after changing the matrix. */
+    void* panels;
+    int64_t panels_bytes;
+    int32_t num_panels;
 } gkomi_csr;
 
 /* Bytes of strategy scratch (gkomi_csr::work) an SpMV with `strategy` needs
@@ -211,6 +221,21 @@ int64_t gkomi_csr_lb_clac_size(int64_t nnz, int64_t nwarps, int32_t warp_size,
  * w * ceil(nnz / warp_size)).  Integer work, deterministic. */
 int gkomi_csr_lb_partition(const gkomi_ctx* ctx, int64_t n_rows, const int32_t* row_ptrs,
                            int64_t nnz, int32_t warp_size, int32_t* srow, int64_t nwarps);
+/* Column panels (no reference counterpart): bytes of device memory that
+ * gkomi_csr_build_panels_* needs for an n_rows-row matrix with nnz entries
+ * in num_panels panels (2, 4 or 8) of value_type (GKOMI_VALUE_*); -1 for
+ * invalid arguments.  Panel p holds the entries with columns in
+ * [p W, (p + 1) W), W = ceil(n_cols / num_panels), in their row order; one
+ * SpMV then reads the matrix once, writes and reads num_panels partial
+ * sums per row, and gathers b within one panel per XCD.  The build copies
+ * a's row pointers, indices and values (integer work, deterministic); the
+ * product sums each row panel by panel, in panel order (deterministic). */
+int64_t gkomi_csr_panels_bytes(int64_t n_rows, int64_t nnz, int32_t num_panels,
+                               int32_t value_type);
+int gkomi_csr_build_panels_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int32_t num_panels,
+                               void* panels, int64_t panels_bytes);
+int gkomi_csr_build_panels_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int32_t num_panels,
+                               void* panels, int64_t panels_bytes);
 /* Compact column indices (no reference counterpart): out[p] = col_idxs[p] -
  * row(p) as int16 for every entry.  *fits (device int32) becomes 1 when every
  * offset lies in [-32768, 32767], else 0 (then `out` must not be used). */
