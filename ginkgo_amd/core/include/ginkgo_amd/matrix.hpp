@@ -172,11 +172,25 @@ public:
                                                   const std::vector<std::vector<ValueType>>& rows,
                                                   std::shared_ptr<strategy_type> strategy = nullptr);
 
-    ValueType* get_values() noexcept { return values_.get_data(); }
+    // the mutable accessors drop the column panels (their copy of the
+    // matrix could go stale)
+    ValueType* get_values() noexcept
+    {
+        panels_valid_ = false;
+        return values_.get_data();
+    }
     const ValueType* get_const_values() const noexcept { return values_.get_const_data(); }
-    IndexType* get_col_idxs() noexcept { return col_idxs_.get_data(); }
+    IndexType* get_col_idxs() noexcept
+    {
+        panels_valid_ = false;
+        return col_idxs_.get_data();
+    }
     const IndexType* get_const_col_idxs() const noexcept { return col_idxs_.get_const_data(); }
-    IndexType* get_row_ptrs() noexcept { return row_ptrs_.get_data(); }
+    IndexType* get_row_ptrs() noexcept
+    {
+        panels_valid_ = false;
+        return row_ptrs_.get_data();
+    }
     const IndexType* get_const_row_ptrs() const noexcept { return row_ptrs_.get_const_data(); }
     const IndexType* get_const_srow() const noexcept { return srow_.get_const_data(); }
     size_type get_num_srow_elements() const noexcept { return srow_.get_size(); }
@@ -211,6 +225,18 @@ public:
     bool compact_indices();
     bool has_compact_indices() const noexcept { return compact_; }
 
+    // Column panels (gkomi_csr_build_panels_*, no reference counterpart): a
+    // copy of the matrix split into num_panels (2, 4 or 8) column panels, so
+    // that each XCD gathers the input vector from one panel's window; every
+    // single right-hand-side apply then uses them, with any strategy.  Pays
+    // off when the input vector exceeds one XCD's 4 MB L2 and the columns
+    // are scattered (the 1M-row power-law matrix: 1.8x); for banded matrices
+    // the extra row visits cost more than they save.  The copy is taken now:
+    // get_values() / get_col_idxs() / get_row_ptrs() drop it, and writes
+    // through a pointer obtained earlier need another call.  0 removes them.
+    void build_column_panels(int num_panels = 4);
+    int get_num_column_panels() const noexcept { return panels_valid_ ? num_panels_ : 0; }
+
     // the plain descriptor the C ABI takes
     gkomi_csr descriptor() const;
     // longest row (cached after the first query)
@@ -226,6 +252,7 @@ protected:
           srow_(exec),
           strategy_work_(exec),
           col_offsets_(exec),
+          panels_(exec),
           strategy_{strategy ? std::move(strategy) : std::make_shared<automatical>()}
     {
         if (size.rows == 0 && nnz == 0) {
@@ -245,6 +272,9 @@ private:
     mutable array<char> strategy_work_;  // merge_path carries
     array<int16> col_offsets_;
     bool compact_{false};
+    mutable array<char> panels_;  // column panels (also written: partial sums)
+    int num_panels_{0};
+    bool panels_valid_{false};
     std::shared_ptr<strategy_type> strategy_;
     std::string effective_{"classical"};
     int subwave_{0};

@@ -355,6 +355,8 @@ void bind_value_type(py::module_& m, const std::string& sfx)
         })
         .def("max_row_nnz", &Csr::max_row_nnz)
         .def("compact_indices", &Csr::compact_indices)
+        .def("build_column_panels", &Csr::build_column_panels, py::arg("num_panels") = 4)
+        .def_property_readonly("num_column_panels", &Csr::get_num_column_panels)
         .def_property_readonly("has_compact_indices", &Csr::has_compact_indices)
         .def_property_readonly("effective_strategy", &Csr::get_effective_strategy)
         .def_property_readonly("nnz", &Csr::get_num_stored_elements)
@@ -362,7 +364,12 @@ void bind_value_type(py::module_& m, const std::string& sfx)
         .def("srow", [](const Csr& a) { return arr(a.srow_to_host()); })
         .def_property_readonly("num_srow_elements", &Csr::get_num_srow_elements)
         .def("col_idxs", [](const Csr& a) { return arr(a.col_idxs_to_host()); })
-        .def("values", [](const Csr& a) { return arr(a.values_to_host()); });
+        .def("values", [](const Csr& a) { return arr(a.values_to_host()); })
+        // device address of the values for writing in place (get_values():
+        // drops the column panels)
+        .def_property_readonly("values_data_ptr", [](Csr& a) {
+            return reinterpret_cast<std::uintptr_t>(a.get_values());
+        });
 
     py::class_<Sellp, LinOp, std::shared_ptr<Sellp>>(m, ("Sellp" + sfx).c_str())
         .def_property_readonly("slice_size", &Sellp::get_slice_size)

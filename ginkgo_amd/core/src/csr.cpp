@@ -207,7 +207,40 @@ gkomi_csr Csr<V, I>::descriptor() const
     d.col_offsets = compact_ ? col_offsets_.get_const_data() : nullptr;
     d.work = strategy_work_.get_data();
     d.work_bytes = static_cast<int64>(strategy_work_.get_size());
+    if (panels_valid_ && num_panels_ > 0) {
+        d.panels = panels_.get_data();
+        d.panels_bytes = static_cast<int64>(panels_.get_size());
+        d.num_panels = num_panels_;
+    }
     return d;
+}
+
+template <typename V, typename I>
+void Csr<V, I>::build_column_panels(int num_panels)
+{
+    panels_valid_ = false;
+    num_panels_ = 0;
+    if (num_panels == 0) {
+        panels_.clear();
+        return;
+    }
+    const int64 nb = gkomi_csr_panels_bytes(static_cast<int64>(this->get_size().rows),
+                                            static_cast<int64>(values_.get_size()), num_panels,
+                                            value_code<V>());
+    if (nb < 0) {
+        throw Error(__FILE__, __LINE__, "column panels: num_panels must be 2, 4 or 8");
+    }
+    auto exec = this->get_executor();
+    if (exec->is_host()) {
+        GKO_NOT_SUPPORTED_MSG("column panels live on a device executor");
+    }
+    panels_ = array<char>(exec, static_cast<size_type>(nb));
+    auto d = descriptor();
+    const auto build = std::is_same<V, double>::value ? gkomi_csr_build_panels_f64
+                                                      : gkomi_csr_build_panels_f32;
+    GKO_CALL(build(ctx_of(exec), &d, num_panels, panels_.get_data(), nb));
+    num_panels_ = num_panels;
+    panels_valid_ = true;
 }
 
 template <typename V, typename I>

@@ -265,3 +265,40 @@ def test_panel_powerlaw_1m(dev, P):
     hp, hci, hv, hb, _, _ = host_panels(rp, ci, v, n, P)
     np.testing.assert_array_equal(prp, hp)
     np.testing.assert_array_equal(bnd, hb)
+
+
+def test_panels_through_the_cpp_api(dev):
+    """Csr::build_column_panels: apply and advanced apply through the gko::
+    surface equal the Python path's panel product bit for bit; a mutable
+    accessor drops the panels (the plain product again)."""
+    import ginkgo_amd.gko as gko
+    ex = gko.Mi355xExecutor.create(0, torch.cuda.current_stream().cuda_stream)
+    name, n_cols, rp, ci, v = CASES[0]
+    n = len(rp) - 1
+    x = np.random.default_rng(9).uniform(-1, 1, (n_cols, 1))
+    A = gko.csr_from_host(ex, rp, ci, v, n_cols, "load_balance")
+    b = gko.matrix.Dense("f64").from_numpy(ex, x)
+    out = gko.matrix.Dense("f64").create(ex, n, 1)
+    A.apply(b, out)
+    plain = out.to_numpy()
+    A.build_column_panels(4)
+    assert A.num_column_panels == 4
+    A.apply(b, out)
+    y = out.to_numpy()
+    P4 = D().DeviceCsr(rp, ci, v, n_cols=n_cols, strategy="load_balance", device=dev)
+    P4.build_panels(4)
+    ref = P4.spmv(torch.as_tensor(x).to(dev)).cpu().numpy()
+    np.testing.assert_array_equal(y, ref)
+    np.testing.assert_array_equal(y[:, 0], sequential_panel_product(rp, ci, v, x[:, 0], n_cols,
+                                                                    4, np.float64))
+    alpha = gko.matrix.Dense("f64").from_numpy(ex, np.array([[1.0]]))
+    beta = gko.matrix.Dense("f64").from_numpy(ex, np.array([[0.0]]))
+    cn = gko.matrix.Dense("f64").from_numpy(ex, np.full((n, 1), np.nan))
+    A.apply(alpha, b, beta, cn)
+    np.testing.assert_array_equal(cn.to_numpy(), ref)
+    _ = A.values_data_ptr
+    assert A.num_column_panels == 0
+    A.apply(b, out)
+    np.testing.assert_array_equal(out.to_numpy(), plain)
+    with pytest.raises(Exception):
+        A.build_column_panels(5)
