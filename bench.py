@@ -323,6 +323,35 @@ def bench_cg_vs_ir(dev, nx=96):
         res[name]["max_abs_err_vs_x_true"] = float(np.max(np.abs(xs - x_true)))
         res[name]["rel_err_vs_x_true"] = float(np.linalg.norm(xs - x_true) /
                                                np.linalg.norm(x_true))
+    res["ir_f64_f32"]["inner_iterations_per_sweep"] = [int(k) for k in ir.last_inner_iterations]
+    res["ir_f64_f32"]["inner_iterations_total"] = int(sum(ir.last_inner_iterations))
+    # cost of one CG iteration in each precision: 500 iterations, no
+    # residual test (the FP32 matrix is the IR's inner one: 2-byte offsets)
+    A32 = A.to_f32()
+    A32.compact_indices()
+    b32 = gko.matrix.Dense("f32").from_numpy(exec_, b_h.astype(np.float32))
+    it_us = {}
+    for name, M, dt, rhs in (("cg_f64", A, "f64", b), ("cg_f64_compact_offsets", Ac, "f64", b),
+                             ("cg_f32_compact_offsets", A32, "f32", b32)):
+        fixed = gko.solver.Cg.build(dt).with_criteria(
+            gko.stop.Iteration.build().with_max_iters(500)).on(exec_).generate(M)
+        best = None
+        for rep in range(2):
+            x = gko.matrix.Dense(dt).create(exec_, n, 1)
+            x.fill(0.0)
+            t, _ = timed(lambda: fixed.apply(rhs, x))
+            best = t if best is None else min(best, t)
+        it_us[name] = round(best / 500 * 1e6, 2)
+    it_us["f32_over_f64"] = round(it_us["cg_f32_compact_offsets"] / it_us["cg_f64"], 3)
+    it_us["f32_over_f64_compact_offsets"] = round(
+        it_us["cg_f32_compact_offsets"] / it_us["cg_f64_compact_offsets"], 3)
+    res["iteration_time_us"] = it_us
+    # the most IR can gain with this many inner iterations: plain CG's
+    # iterations over the inner ones, times the per-iteration cost ratio
+    # (outer residuals and conversions ignored)
+    res["ir_speedup_ceiling_over_cg_compact_offsets"] = round(
+        res["cg_f64_compact_offsets"]["iterations"] /
+        (res["ir_f64_f32"]["inner_iterations_total"] * it_us["f32_over_f64_compact_offsets"]), 3)
     res["ir_vs_cg_rel_diff"] = float(np.linalg.norm(sols["ir_f64_f32"] - sols["cg_f64"]) /
                                      np.linalg.norm(sols["cg_f64"]))
     res["speedup_ir_over_cg"] = round(res["cg_f64"]["time_s"] / res["ir_f64_f32"]["time_s"], 3)

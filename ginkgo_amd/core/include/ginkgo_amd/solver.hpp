@@ -111,6 +111,8 @@ struct solve_stats {
     int64 iterations{};
     std::vector<stopping_status> status;
     bool used_fused_path{};
+    // Ir: the inner solver's iterations in every sweep (Krylov inner solvers)
+    std::vector<int64> inner_iterations;
 };
 
 template <typename ValueType>

@@ -502,6 +502,8 @@ void bind_value_type(py::module_& m, const std::string& sfx)
     py::class_<I, LinOp, std::shared_ptr<I>>(m, ("Ir" + sfx).c_str())
         .def_property_readonly("last_iterations",
                                [](const I& s) { return s.get_last_stats().iterations; })
+        .def_property_readonly("last_inner_iterations",
+                               [](const I& s) { return s.get_last_stats().inner_iterations; })
         .def_property_readonly("last_status", [](const I& s) {
             std::vector<int> out;
             for (auto st : s.get_last_stats().status) out.push_back(st.raw());

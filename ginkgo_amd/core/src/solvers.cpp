@@ -710,6 +710,7 @@ void Ir<V>::apply_impl(const LinOp* b, LinOp* x) const
             system_matrix_->apply(neg.get(), dx, one.get(), residual.get());
             auto orig = baselines<V>(exec, crit_, db, residual.get());
             int64 iter = -1;
+            stats_.inner_iterations.clear();
             while (true) {
                 ++iter;
                 set_iteration(exec, dflags, iter);
@@ -739,6 +740,11 @@ void Ir<V>::apply_impl(const LinOp* b, LinOp* x) const
                     inner_solution->copy_from(residual.get());
                     inner_->apply(residual.get(), inner_solution.get());
                     dx->add_scaled(relax.get(), inner_solution.get());
+                    if (auto k = dynamic_cast<const KrylovBase<double>*>(inner_.get())) {
+                        stats_.inner_iterations.push_back(k->get_last_stats().iterations);
+                    } else if (auto k = dynamic_cast<const KrylovBase<float>*>(inner_.get())) {
+                        stats_.inner_iterations.push_back(k->get_last_stats().iterations);
+                    }
                 } else {
                     dx->add_scaled(relax.get(), residual.get());
                 }

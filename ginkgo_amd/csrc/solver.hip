@@ -9,6 +9,8 @@
 //    the axpy updates, the Jacobi scaling, both dots and ||r|| into the SpMV
 //    and update passes, and evaluate stop::Combined on the device so the host
 //    never waits inside the loop.
+#include <cmath>
+
 #include "common.hpp"
 #include "reduce.hpp"
 #include "fused.hpp"
@@ -529,11 +531,29 @@ struct cg_grids {
     int update;  // update kernels
 };
 
+// The split form's automatic shape: about 16 bytes of matrix values per
+// lane and row (lanes per row: the power of two nearest to mean row length
+// x value bytes / 16) and two rows in flight per lane group.  On the
+// 27-point 96^3 matrix (mean 26.4) that is 16 lanes for FP64 and 8 for FP32,
+// the fastest CG iterations of sw 1..32 x rpg 1..8 in both precisions
+// (90.5 and 61.8 us against 92.3 and 67.6 us with the SpMV's own (16, 4)).
+subwave_shape split_shape(double mean, int value_bytes)
+{
+    const double target = mean * value_bytes / 16.0;
+    int sw = 1;
+    while (sw < 64 && std::abs(2 * sw - target) < std::abs(sw - target)) sw *= 2;
+    return subwave_shape{sw, 2};
+}
+
 template <typename T>
 cg_grids cg_grids_for(const gkomi_csr* a, int64_t n)
 {
     cg_grids g;
     g.sh = pick_subwave(a, sizeof(T));
+    const double mean = n > 0 ? static_cast<double>(a->nnz) / static_cast<double>(n) : 0.0;
+    if (a->subwave == 0 && mean >= kCgSplitMeanRow) {
+        g.sh = split_shape(mean, sizeof(T));
+    }
     const int64_t tile_rows = int64_t{kBlock} / g.sh.sw * g.sh.rpg;
     g.spmv = fused_grid(ceildiv(std::max<int64_t>(n, 1), tile_rows));
     g.update = static_cast<int>(

@@ -163,7 +163,8 @@ int ora_ir_solve(int64_t n, const int32_t* rp, const int32_t* ci, const double* 
                  const double* b, double* x, const ora_criterion* outer,
                  int32_t n_outer, const ora_criterion* inner, int32_t n_inner,
                  double relaxation, int inner_cg, int use_omp, uint8_t* status,
-                 ora_solve_info* info, int64_t* inner_iterations_total)
+                 ora_solve_info* info, int64_t* inner_iterations_total,
+                 int64_t* inner_per_sweep, int64_t sweep_capacity)
 {
     const int64_t nnz = rp[n];
     float* vf = (float*)malloc(sizeof(float) * (size_t)(nnz + 1));
@@ -222,6 +223,9 @@ int ora_ir_solve(int64_t n, const int32_t* rp, const int32_t* ci, const double* 
         ora_solve_info inner_info;
         ora_cg_solve_s(n, 1, rp, ci, vf, NULL, bf, xf, inner, n_inner, use_omp,
                        inner_status, &inner_info);
+        if (inner_per_sweep && iter < sweep_capacity) {
+            inner_per_sweep[iter] = inner_info.iterations;
+        }
         inner_total += inner_info.iterations;
         spmvs += inner_info.spmv_count;
         for (int64_t i = 0; i < n; ++i) {

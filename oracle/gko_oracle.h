@@ -191,13 +191,16 @@ void ora_csr_lb_process(int64_t n_rows, const int32_t* row_ptrs, int64_t warp_si
  * Cg<float> on the float copy of A (core/solver/ir.cpp:193-257,
  * examples/mixed-precision-ir/mixed-precision-ir.cpp:96-143).  With
  * inner_cg == 0 the inner solver is the Identity (Richardson), the
- * reference's default when no inner solver is given. */
+ * reference's default when no inner solver is given.  inner_per_sweep
+ * (NULL = not wanted) receives the inner iterations of the first
+ * sweep_capacity sweeps. */
 int ora_ir_solve(int64_t n, const int32_t* row_ptrs, const int32_t* col_idxs,
                  const double* vals, const double* b, double* x,
                  const ora_criterion* outer, int32_t n_outer,
                  const ora_criterion* inner, int32_t n_inner, double relaxation,
                  int inner_cg, int use_omp, uint8_t* status, ora_solve_info* info,
-                 int64_t* inner_iterations_total);
+                 int64_t* inner_iterations_total, int64_t* inner_per_sweep,
+                 int64_t sweep_capacity);
 
 /* Number of OpenMP threads the _omp variants use. */
 int ora_omp_threads(void);

@@ -314,7 +314,9 @@ def bicgstab_solve(row_ptrs, col_idxs, vals, b, x0, crit, inv_diag=None, omp=Fal
 
 
 def ir_solve(row_ptrs, col_idxs, vals, b, x0, outer, inner=None, relaxation=1.0,
-             inner_cg=True, omp=False):
+             inner_cg=True, omp=False, per_sweep=False):
+    """(x, status, info, inner iterations in total); with per_sweep also the
+    inner iterations of every sweep (numpy int64 array)."""
     vals = _c(vals, np.float64)
     b = _c(b, np.float64).reshape(-1)
     x = _c(x0, np.float64).reshape(-1).copy()
@@ -324,14 +326,18 @@ def ir_solve(row_ptrs, col_idxs, vals, b, x0, outer, inner=None, relaxation=1.0,
     status = np.zeros(1, dtype=np.uint8)
     info = SolveInfo()
     inner_total = ctypes.c_int64(0)
+    cap = max(1024, max((int(c.get("max_iters", 0)) for c in outer), default=0) + 1)
+    sweeps = np.full(cap, -1, dtype=np.int64)
     rc = lib().ora_ir_solve(ctypes.c_int64(n), _p(_c(row_ptrs, np.int32)),
                             _p(_c(col_idxs, np.int32)), _p(vals), _p(b), _p(x), oarr,
                             ctypes.c_int32(no), iarr, ctypes.c_int32(ni),
                             ctypes.c_double(relaxation), ctypes.c_int(1 if inner_cg else 0),
                             ctypes.c_int(1 if omp else 0), _p(status), ctypes.byref(info),
-                            ctypes.byref(inner_total))
+                            ctypes.byref(inner_total), _p(sweeps), ctypes.c_int64(cap))
     if rc != 0:
         raise MemoryError("oracle IR failed to allocate")
+    if per_sweep:
+        return x, status, info, inner_total.value, sweeps[:max(0, int(info.iterations))]
     return x, status, info, inner_total.value
 
 

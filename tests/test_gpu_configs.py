@@ -155,8 +155,8 @@ def test_cg_and_ir_diffusion27_96(dev, diffusion96):
              {"kind": "residual_norm", "reduction_factor": 1e-3}]
     outer = [{"kind": "iteration", "max_iters": 200},
              {"kind": "residual_norm", "reduction_factor": 1e-12}]
-    xi, si, ii, inner_total = ora.ir_solve(rp, ci, v, b_h, np.zeros_like(b_h), outer, inner,
-                                           omp=True)
+    xi, si, ii, inner_total, sweeps = ora.ir_solve(rp, ci, v, b_h, np.zeros_like(b_h), outer,
+                                                   inner, omp=True, per_sweep=True)
     ex = gko.Mi355xExecutor.create(0, torch.cuda.current_stream().cuda_stream)
     A = gko.csr_from_host(ex, rp, ci, v, strategy="classical")
     b = gko.matrix.Dense("f64").from_numpy(ex, b_h)
@@ -187,6 +187,18 @@ def test_cg_and_ir_diffusion27_96(dev, diffusion96):
             assert abs(solver.last_iterations - io.iterations) <= max(3, io.iterations // 100)
         else:
             assert abs(solver.last_iterations - ii.iterations) <= 1
+            dev_sweeps = list(solver.last_inner_iterations)
+            say(f"C3 IR inner FP32 iterations per sweep: device {dev_sweeps} = "
+                f"{sum(dev_sweeps)}, oracle {[int(k) for k in sweeps]} = {inner_total}")
+            assert len(dev_sweeps) == solver.last_iterations
+            # the sweeps before the last agree to 5%; the last ones end at
+            # the FP32 accuracy floor, where the device's dots (FP32 per
+            # wave, combined in FP64) and the host's (summed in FP32) round
+            # differently and one more sweep may be needed on either side
+            for k in range(min(len(dev_sweeps), len(sweeps)) - 1):
+                assert abs(dev_sweeps[k] - sweeps[k]) <= max(5, sweeps[k] // 20), k
+            # the restart penalty: more inner FP32 iterations than plain CG's
+            assert sum(dev_sweeps) > cg.last_iterations
     say(f"C3 oracle IR: {ii.iterations} sweeps, {inner_total} inner FP32 iterations; "
         f"oracle CG {io.iterations} iterations")
 

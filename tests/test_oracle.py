@@ -586,6 +586,37 @@ def test_ir_mixed_precision_reaches_fp64_accuracy():
     assert info.iterations >= 3 and inner_total > info.iterations
 
 
+def test_ir_inner_iterations_per_sweep():
+    """Why mixed-precision IR needs more inner FP32 iterations than plain FP64
+    CG needs in total: every sweep restarts CG with the residual as initial
+    guess (core/solver/ir.cpp:242-250, examples/mixed-precision-ir/
+    mixed-precision-ir.cpp:124-130), so the Krylov space is rebuilt and the
+    first inner residual b - A b exceeds the baseline ||b||.  27-point 32^3
+    diffusion with b = A x_true: the per-sweep counts add up to the total, the
+    first sweep is short, and the inner total exceeds plain CG's count."""
+    rp, ci, v = gen.diffusion27(32)
+    n = len(rp) - 1
+    xt = np.random.default_rng(4).uniform(-1, 1, (n, 1))
+    b = ora.csr_spmv(rp, ci, v, xt, omp=True)
+    cg_crit = [{"kind": "iteration", "max_iters": 20000},
+               {"kind": "residual_norm", "reduction_factor": 1e-12}]
+    _, st, info = ora.cg_solve(rp, ci, v, b, np.zeros((n, 1)), cg_crit, omp=True)
+    outer = [{"kind": "iteration", "max_iters": 200},
+             {"kind": "residual_norm", "reduction_factor": 1e-12}]
+    inner = [{"kind": "iteration", "max_iters": 20000},
+             {"kind": "residual_norm", "reduction_factor": 1e-3}]
+    x, sti, ii, total, sweeps = ora.ir_solve(rp, ci, v, b[:, 0], np.zeros(n), outer, inner,
+                                             omp=True, per_sweep=True)
+    print(f"32^3: plain CG {info.iterations} iterations; IR {ii.iterations} sweeps, "
+          f"inner {[int(s) for s in sweeps]} = {total}")
+    assert sti[0] & ora.STATUS_CONVERGED and st[0] & ora.STATUS_CONVERGED
+    assert len(sweeps) == ii.iterations and int(sweeps.sum()) == total
+    assert sweeps[0] < min(sweeps[1:])
+    assert total > info.iterations
+    relres = np.linalg.norm(b[:, 0] - ora.csr_spmv(rp, ci, v, x.reshape(-1, 1))[:, 0])
+    assert relres <= 1e-12 * np.linalg.norm(b)
+
+
 def test_batch_bicgstab_reference_stencil_kat():
     k = KATS["batch_bicgstab"][0]
     row_off, rp, ci, v = gen.stencil3pt_batch(k["num_batch_items"], k["num_rows"])
