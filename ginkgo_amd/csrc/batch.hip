@@ -1,13 +1,20 @@
 // Batched BiCGSTAB and CG for gfx950: thousands of small independent systems.
 //
-// One workgroup solves one system at a time and pulls the next system from
-// a work queue (an atomic ticket over a size-sorted order, largest first),
-// so uneven sizes balance themselves.  The system's matrix (values, 16-bit
-// local column indices, row pointers) and all work vectors live in LDS
-// when they fit; otherwise the matrix is read from global memory (L2
-// resident for the working set of the active workgroups).  The whole
-// iteration runs inside the kernel, with the reference's per-system
-// semantics and logging of iterations and residual norm: BiCGSTAB
+// One workgroup solves one system at a time; the systems are size-sorted
+// (largest first) and cut into size bins, one launch per bin.  Two kernel
+// families:
+//  * batch_solver_kernel: every work vector in LDS, the matrix (values,
+//    16-bit local column indices, row pointers) in LDS when it fits, else
+//    read from global memory (L2 resident for the working set of the active
+//    workgroups); each workgroup pulls its systems from an atomic queue, so
+//    uneven sizes balance themselves;
+//  * batch_reg_kernel (Csr formats): the work vectors in registers, one or
+//    two rows per thread, only the gathered vectors in LDS beside the matrix
+//    or with the matrix rows in registers too; workgroup b takes system b of
+//    the bin first and uses the queue only when the bin holds more systems
+//    than the grid has workgroups.
+// The whole iteration runs inside the kernel, with the reference's
+// per-system semantics and logging of iterations and residual norm: BiCGSTAB
 // (reference/solver/batch_bicgstab_kernels.hpp:180-354: no breakdown guards,
 // the half-step convergence exit) or CG (reference/solver/
 // batch_cg_kernels.hpp:31-221: the implicit residual norm sqrt|r.z|).
