@@ -415,6 +415,25 @@ inline const gkomi_ctx* ctx_of(const std::shared_ptr<const Executor>& exec)
     return exec->get_ctx();
 }
 
+// A recorded sequence of device work (a HIP graph; no reference
+// counterpart), replayed on its executor's stream.  The buffers the recorded
+// calls named must outlive it.
+class Graph {
+public:
+    Graph(std::shared_ptr<const Executor> exec, gkomi_graph* g)
+        : exec_{std::move(exec)}, graph_{g}
+    {}
+    Graph(const Graph&) = delete;
+    Graph& operator=(const Graph&) = delete;
+    ~Graph() { gkomi_graph_destroy(graph_); }
+    void replay() const { GKO_CALL(gkomi_graph_launch(ctx_of(exec_), graph_)); }
+    std::shared_ptr<const Executor> get_executor() const { return exec_; }
+
+private:
+    std::shared_ptr<const Executor> exec_;
+    gkomi_graph* graph_;
+};
+
 }  // namespace gko
 
 namespace gko {

@@ -384,6 +384,15 @@ public:
 
     // x holds the initial guess and receives the solution
     void apply(const MultiVector<ValueType>* b, MultiVector<ValueType>* x) const;
+    // apply(b, x) recorded as one HIP graph (extension, no reference
+    // counterpart): with zero_initial_guess, x is first set to zero by a fill
+    // kernel inside the graph; otherwise every replay starts from x as it
+    // is.  b, x and this solver must outlive the graph.  Replays write the
+    // per-item iterations and residual norms but call no logger;
+    // record_logs() hands the last ones to the loggers.
+    std::unique_ptr<Graph> capture(const MultiVector<ValueType>* b, MultiVector<ValueType>* x,
+                                   bool zero_initial_guess = true) const;
+    void record_logs() const;
     // times the launch candidates of every size bin on (b, x) and keeps the
     // fastest (x is not modified); extension, no reference counterpart
     void tune(const MultiVector<ValueType>* b, const MultiVector<ValueType>* x);

@@ -105,6 +105,15 @@ void bind_batch_solver(py::module_& m, const std::string& name, const std::strin
     py::class_<BS, std::shared_ptr<BS>>(m, (name + sfx).c_str())
         .def("apply", [](const BS& s, const MV* b, MV* x) { s.apply(b, x); })
         .def("tune", [](BS& s, const MV* b, const MV* x) { s.tune(b, x); })
+        .def(
+            "capture",
+            [](const BS& s, const MV* b, MV* x, bool zero) {
+                return std::shared_ptr<Graph>(s.capture(b, x, zero));
+            },
+            py::arg("b"), py::arg("x"), py::arg("zero_initial_guess") = true,
+            // the graph keeps the solver, b and x alive
+            py::keep_alive<0, 1>(), py::keep_alive<0, 2>(), py::keep_alive<0, 3>())
+        .def("record_logs", &BS::record_logs)
         .def("add_logger", [](BS& s, std::shared_ptr<const BL> l) { s.add_logger(l); })
         .def_property_readonly("block_sizes", &BS::get_block_sizes)
         .def_property_readonly("matrix_in_lds", &BS::get_matrix_in_lds);
@@ -747,6 +756,9 @@ PYBIND11_MODULE(_gko, m)
         .def("should_propagate_log", &Executor::should_propagate_log);
     py::class_<HostExecutor, Executor, std::shared_ptr<HostExecutor>>(m, "HostExecutor")
         .def_static("create", &HostExecutor::create);
+    py::class_<Graph, std::shared_ptr<Graph>>(m, "Graph")
+        .def("replay", &Graph::replay)
+        .def_property_readonly("executor", &Graph::get_executor);
     py::class_<Mi355xExecutor, Executor, std::shared_ptr<Mi355xExecutor>>(m, "Mi355xExecutor")
         .def_static(
             "create",

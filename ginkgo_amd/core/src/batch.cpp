@@ -490,11 +490,50 @@ void BatchKrylov<V, M>::apply(const MultiVector<V>* b, MultiVector<V>* x) const
     GKO_CALL(K::solve(ctx_of(exec_), &desc_, &s, &plan_, b->get_const_values(), x->get_values(),
                       iterations_.get_data(), residuals_.get_data(), workspace_.get_data(),
                       workspace_.get_size()));
+    record_logs();
+}
+
+template <typename V, batch_method M>
+void BatchKrylov<V, M>::record_logs() const
+{
     if (!loggers_.empty()) {
         auto it = iterations_.to_host();
         auto res = residuals_.to_host();
         for (auto& l : loggers_) l->record(it, res);
     }
+}
+
+template <typename V, batch_method M>
+std::unique_ptr<Graph> BatchKrylov<V, M>::capture(const MultiVector<V>* b, MultiVector<V>* x,
+                                                  bool zero_initial_guess) const
+{
+    check_vectors(b, x);
+    const auto s = settings();
+    using K = bk<V, M>;
+    gkomi_ctx cap{};
+    GKO_CALL(gkomi_graph_begin(ctx_of(exec_), &cap));
+    int rc = GKOMI_SUCCESS;
+    if (zero_initial_guess) {
+        if constexpr (std::is_same<V, double>::value) {
+            rc = gkomi_fill_f64(&cap, static_cast<int64>(total_rows_), 0.0, x->get_values());
+        } else {
+            rc = gkomi_fill_f32(&cap, static_cast<int64>(total_rows_), 0.0f, x->get_values());
+        }
+    }
+    if (rc == GKOMI_SUCCESS) {
+        rc = K::solve(&cap, &desc_, &s, &plan_, b->get_const_values(), x->get_values(),
+                      iterations_.get_data(), residuals_.get_data(), workspace_.get_data(),
+                      workspace_.get_size());
+    }
+    // the capture ends in every case (its stream is destroyed there)
+    gkomi_graph* g = nullptr;
+    const int end_rc = gkomi_graph_end(&cap, &g);
+    if (rc != GKOMI_SUCCESS) {
+        gkomi_graph_destroy(g);
+        GKO_CALL(rc);
+    }
+    GKO_CALL(end_rc);
+    return std::unique_ptr<Graph>(new Graph(exec_, g));
 }
 
 template <typename V, batch_method M>

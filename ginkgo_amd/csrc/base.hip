@@ -269,6 +269,13 @@ int reduce_impl(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, const T* x, int64
 
 using namespace gkomi;
 
+// A captured and instantiated HIP graph (the opaque gkomi_graph of the C ABI).
+struct gkomi_graph {
+    int32_t device;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
 extern "C" {
 
 const char* gkomi_last_error(void) { return g_last_error.c_str(); }
@@ -428,6 +435,72 @@ int gkomi_event_elapsed_ms(void* start, void* stop, float* ms)
     GKOMI_REQUIRE(ms != nullptr, GKOMI_INVALID_ARGUMENT, "null output");
     GKOMI_CHECK_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start),
                                         static_cast<hipEvent_t>(stop)));
+    return GKOMI_SUCCESS;
+}
+
+int gkomi_graph_begin(const gkomi_ctx* ctx, gkomi_ctx* capture)
+{
+    if (int rc = enter(ctx)) return rc;
+    GKOMI_REQUIRE(capture != nullptr, GKOMI_INVALID_ARGUMENT, "graph: null capture context");
+    hipStream_t s = nullptr;
+    // default priority, as every stream of this library
+    GKOMI_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    const hipError_t err = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+    if (err != hipSuccess) {
+        (void)hipStreamDestroy(s);
+        set_error("hipStreamBeginCapture: %s", hipGetErrorString(err));
+        return GKOMI_HIP_ERROR;
+    }
+    capture->device = ctx->device;
+    capture->reserved = 0;
+    capture->stream = s;
+    return GKOMI_SUCCESS;
+}
+
+int gkomi_graph_end(gkomi_ctx* capture, gkomi_graph** graph)
+{
+    GKOMI_REQUIRE(capture != nullptr && capture->stream != nullptr && graph != nullptr,
+                  GKOMI_INVALID_ARGUMENT, "graph: no capture in progress");
+    if (int rc = enter(capture)) return rc;
+    hipStream_t s = static_cast<hipStream_t>(capture->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t err = hipStreamEndCapture(s, &g);
+    (void)hipStreamDestroy(s);
+    capture->stream = nullptr;
+    if (err != hipSuccess || g == nullptr) {
+        set_error("hipStreamEndCapture: %s", hipGetErrorString(err));
+        return GKOMI_HIP_ERROR;
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ierr = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (ierr != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        set_error("hipGraphInstantiate: %s", hipGetErrorString(ierr));
+        return GKOMI_HIP_ERROR;
+    }
+    *graph = new gkomi_graph{capture->device, g, ex};
+    return GKOMI_SUCCESS;
+}
+
+int gkomi_graph_launch(const gkomi_ctx* ctx, const gkomi_graph* graph)
+{
+    if (int rc = enter(ctx)) return rc;
+    GKOMI_REQUIRE(graph != nullptr, GKOMI_INVALID_ARGUMENT, "graph: null graph");
+    GKOMI_REQUIRE(ctx->device == graph->device, GKOMI_INVALID_ARGUMENT,
+                  "graph: launched on another device than it was captured on");
+    GKOMI_CHECK_HIP(hipGraphLaunch(graph->exec, stream_of(ctx)));
+    return GKOMI_SUCCESS;
+}
+
+int gkomi_graph_destroy(gkomi_graph* graph)
+{
+    if (graph == nullptr) return GKOMI_SUCCESS;
+    GKOMI_CHECK_HIP(hipSetDevice(graph->device));
+    const hipError_t e1 = hipGraphExecDestroy(graph->exec);
+    const hipError_t e2 = hipGraphDestroy(graph->graph);
+    delete graph;
+    GKOMI_CHECK_HIP(e1);
+    GKOMI_CHECK_HIP(e2);
     return GKOMI_SUCCESS;
 }
 

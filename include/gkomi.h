@@ -53,6 +53,20 @@ typedef struct {
     void* stream;
 } gkomi_ctx;
 
+/* HIP graphs (no reference counterpart; HipExecutor has none): work enqueued
+ * on `capture` between gkomi_graph_begin and gkomi_graph_end is recorded,
+ * not run, then instantiated; gkomi_graph_launch replays it on ctx->stream.
+ * gkomi_graph_begin fills *capture with ctx's device and a new stream of its
+ * own in relaxed capture mode; gkomi_graph_end destroys that stream.  A
+ * replay runs on the buffers the recorded calls named.  The library's
+ * kernels that may be recorded clear memory with fill kernels, never with
+ * memset nodes. */
+typedef struct gkomi_graph gkomi_graph;
+int gkomi_graph_begin(const gkomi_ctx* ctx, gkomi_ctx* capture);
+int gkomi_graph_end(gkomi_ctx* capture, gkomi_graph** graph);
+int gkomi_graph_launch(const gkomi_ctx* ctx, const gkomi_graph* graph);
+int gkomi_graph_destroy(gkomi_graph* graph);
+
 /* Thread-local description of the last failure. */
 const char* gkomi_last_error(void);
 /* Library version string. */

@@ -287,6 +287,50 @@ def test_api_uneven_block_diagonal_against_oracle(gko_exec):
     assert np.max(np.abs(logger.num_iterations - io)) <= 2
 
 
+@pytest.mark.parametrize("method", ["Bicgstab", "Cg"])
+def test_api_graph_capture_replays(gko_exec, method):
+    """batch::solver::{Bicgstab, Cg}::capture: the whole solve (x = 0 by a
+    fill kernel, then every bin's launch) as one HIP graph.  Three replays,
+    with eager work on the same buffers and host synchronisations between
+    them, each equal bit for bit to the eager solve."""
+    gko, ex = gko_exec
+    row_off, rp, ci, v, b = gen.uneven_batch(150, n_min=16, n_max=300, seed=9)
+    if method == "Cg":
+        row_off, rp, ci, v, b = gen.uneven_spd_batch(150, n_min=16, n_max=300, seed=9)
+    A = gko.batch.matrix.BlockDiagCsr("f64").from_numpy(ex, row_off, rp, ci, v)
+    solver = (getattr(gko.batch.solver, method).build("f64").with_max_iterations(500)
+              .with_tolerance(1e-10)
+              .with_tolerance_type(gko.batch.stop.tolerance_type.relative)
+              .with_preconditioner(gko.batch.preconditioner.Jacobi.build("f64"))
+              .on(ex).generate(A))
+    logger = gko.batch.log.BatchConvergence("f64").create()
+    solver.add_logger(logger)
+    bb = gko.batch.MultiVector("f64").uneven(ex, row_off, b)
+    x = gko.batch.MultiVector("f64").uneven(ex, row_off, np.zeros_like(b))
+    solver.apply(bb, x)
+    eager = x.to_numpy()
+    eager_iters = np.array(logger.num_iterations)
+    graph = solver.capture(bb, x)
+    for rep in range(3):
+        x.fill(7.0)                       # eager work on the graph's buffers
+        solver.apply(bb, x)               # and an eager solve in between
+        ex.synchronize()
+        x.fill(-1.0)
+        graph.replay()
+        ex.synchronize()
+        np.testing.assert_array_equal(x.to_numpy(), eager, err_msg=f"replay {rep}")
+        solver.record_logs()
+        np.testing.assert_array_equal(np.array(logger.num_iterations), eager_iters)
+    # without the zero fill a replay starts from x as it is: from the
+    # solution itself the iterations stop at once
+    g2 = solver.capture(bb, x, zero_initial_guess=False)
+    g2.replay()
+    ex.synchronize()
+    solver.record_logs()
+    assert np.max(np.array(logger.num_iterations)) <= 2
+    del graph, g2
+
+
 def test_api_errors(gko_exec):
     gko, ex = gko_exec
     with pytest.raises(gko.NotSupported):
