@@ -165,7 +165,8 @@ class BatchCsr(ctypes.Structure):
                 ("row_ptrs", ctypes.c_void_p), ("col_idxs", ctypes.c_void_p),
                 ("values", ctypes.c_void_p), ("uniform_rows", ctypes.c_int64),
                 ("uniform_nnz", ctypes.c_int64), ("format", ctypes.c_int32),
-                ("ell_width", ctypes.c_int32), ("uniform_cols", ctypes.c_int64)]
+                ("ell_width", ctypes.c_int32), ("uniform_cols", ctypes.c_int64),
+                ("item_offsets", ctypes.c_void_p)]
 
 
 BATCH_CSR, BATCH_ELL, BATCH_DENSE = 0, 1, 2

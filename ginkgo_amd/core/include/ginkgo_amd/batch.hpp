@@ -258,6 +258,82 @@ private:
     std::vector<int64> row_offsets_host_, item_rows_, item_nnz_, item_max_row_nnz_;
 };
 
+// Extension (no reference counterpart; the reference's batch Ell and Dense
+// have one size): Ell and Dense items of different sizes.  Item k owns rows
+// [row_offsets[k], row_offsets[k+1]) of the vectors (n_k rows, square) and
+// its stored entries from item_offsets[k] on.  BlockDiagEll: n_k * w_k slots
+// column-major (slot j of row i at i + j * n_k), w_k the item's longest row,
+// the item's own local column indices at the same positions, -1 after a
+// row's last entry (the reference's Ell layout per item).  BlockDiagDense:
+// n_k * n_k values, row-major.
+template <typename ValueType>
+class BlockDiagBase : public BatchMatrix<ValueType> {
+public:
+    std::vector<int64> get_item_rows() const override { return item_rows_; }
+    std::vector<int64> get_item_nnz() const override { return item_nnz_; }
+    size_type get_total_rows() const override
+    {
+        return row_offsets_host_.empty() ? 0 : static_cast<size_type>(row_offsets_host_.back());
+    }
+    size_type get_total_cols() const override { return get_total_rows(); }
+    const std::vector<int64>& get_row_offsets() const { return row_offsets_host_; }
+    const std::vector<int64>& get_item_offsets() const { return item_offsets_host_; }
+    std::vector<ValueType> values_to_host() const { return values_.to_host(); }
+
+protected:
+    BlockDiagBase(std::shared_ptr<const Executor> exec, const std::vector<int64>& row_offsets,
+                  const std::vector<int64>& item_offsets, const std::vector<ValueType>& values);
+    gkomi_batch_csr base_descriptor(int32_t format) const;
+    array<int64> row_offsets_;
+    array<int64> item_offsets_;
+    array<ValueType> values_;
+    std::vector<int64> row_offsets_host_, item_offsets_host_, item_rows_, item_nnz_;
+};
+
+template <typename ValueType, typename IndexType = int32>
+class BlockDiagEll : public BlockDiagBase<ValueType> {
+public:
+    using value_type = ValueType;
+    using index_type = IndexType;
+    // from a block-diagonal CSR on the host (global row pointers, local
+    // column indices); each row keeps the order of its entries
+    static std::unique_ptr<BlockDiagEll> create_from_block_diag_csr_host(
+        std::shared_ptr<const Executor> exec, const std::vector<int64>& row_offsets,
+        const std::vector<IndexType>& row_ptrs, const std::vector<IndexType>& col_idxs,
+        const std::vector<ValueType>& values);
+    std::vector<int64> get_item_max_row_nnz() const override { return item_width_; }
+    const std::vector<int64>& get_item_widths() const { return item_width_; }
+    std::vector<IndexType> col_idxs_to_host() const { return col_idxs_.to_host(); }
+    gkomi_batch_csr descriptor() const override;
+
+private:
+    BlockDiagEll(std::shared_ptr<const Executor> exec, const std::vector<int64>& row_offsets,
+                 const std::vector<int64>& item_offsets, const std::vector<IndexType>& col_idxs,
+                 const std::vector<ValueType>& values, std::vector<int64> widths);
+    array<IndexType> col_idxs_;
+    std::vector<int64> item_width_;
+};
+
+template <typename ValueType>
+class BlockDiagDense : public BlockDiagBase<ValueType> {
+public:
+    using value_type = ValueType;
+    // item k's n_k x n_k values, row-major, one item after another
+    static std::unique_ptr<BlockDiagDense> create_from_host(
+        std::shared_ptr<const Executor> exec, const std::vector<int64>& row_offsets,
+        const std::vector<ValueType>& values);
+    // from a block-diagonal CSR on the host (duplicate entries are summed)
+    template <typename IndexType>
+    static std::unique_ptr<BlockDiagDense> create_from_block_diag_csr_host(
+        std::shared_ptr<const Executor> exec, const std::vector<int64>& row_offsets,
+        const std::vector<IndexType>& row_ptrs, const std::vector<IndexType>& col_idxs,
+        const std::vector<ValueType>& values);
+    gkomi_batch_csr descriptor() const override;
+
+private:
+    using BlockDiagBase<ValueType>::BlockDiagBase;
+};
+
 }  // namespace matrix
 
 namespace preconditioner {

@@ -241,6 +241,44 @@ void bind_batch(py::module_& m, const std::string& sfx)
                         return std::shared_ptr<BDiag>(BDiag::create_from_host(
                             e, vec<int64>(off), vec<int32>(rp), vec<int32>(ci), vec<V>(v)));
                     });
+    using BDEll = batch::matrix::BlockDiagEll<V, int32>;
+    using BDDense = batch::matrix::BlockDiagDense<V>;
+    py::class_<BDEll, BM, std::shared_ptr<BDEll>>(m, ("BatchBlockDiagEll" + sfx).c_str())
+        .def_static("from_numpy",
+                    [](std::shared_ptr<const Executor> e,
+                       py::array_t<int64, py::array::c_style | py::array::forcecast> off,
+                       py::array_t<int32, py::array::c_style | py::array::forcecast> rp,
+                       py::array_t<int32, py::array::c_style | py::array::forcecast> ci,
+                       py::array_t<V, py::array::c_style | py::array::forcecast> v) {
+                        return std::shared_ptr<BDEll>(BDEll::create_from_block_diag_csr_host(
+                            e, vec<int64>(off), vec<int32>(rp), vec<int32>(ci), vec<V>(v)));
+                    })
+        .def_property_readonly("item_offsets",
+                               [](const BDEll& a) { return arr(a.get_item_offsets()); })
+        .def_property_readonly("item_widths", [](const BDEll& a) { return arr(a.get_item_widths()); })
+        .def("col_idxs", [](const BDEll& a) { return arr(a.col_idxs_to_host()); })
+        .def("values", [](const BDEll& a) { return arr(a.values_to_host()); });
+    py::class_<BDDense, BM, std::shared_ptr<BDDense>>(m, ("BatchBlockDiagDense" + sfx).c_str())
+        .def_static("from_numpy",
+                    [](std::shared_ptr<const Executor> e,
+                       py::array_t<int64, py::array::c_style | py::array::forcecast> off,
+                       py::array_t<V, py::array::c_style | py::array::forcecast> v) {
+                        return std::shared_ptr<BDDense>(
+                            BDDense::create_from_host(e, vec<int64>(off), vec<V>(v)));
+                    })
+        .def_static("from_block_diag_csr",
+                    [](std::shared_ptr<const Executor> e,
+                       py::array_t<int64, py::array::c_style | py::array::forcecast> off,
+                       py::array_t<int32, py::array::c_style | py::array::forcecast> rp,
+                       py::array_t<int32, py::array::c_style | py::array::forcecast> ci,
+                       py::array_t<V, py::array::c_style | py::array::forcecast> v) {
+                        return std::shared_ptr<BDDense>(
+                            BDDense::template create_from_block_diag_csr_host<int32>(
+                                e, vec<int64>(off), vec<int32>(rp), vec<int32>(ci), vec<V>(v)));
+                    })
+        .def_property_readonly("item_offsets",
+                               [](const BDDense& a) { return arr(a.get_item_offsets()); })
+        .def("values", [](const BDDense& a) { return arr(a.values_to_host()); });
     py::class_<typename BJ::parameters_type>(m, ("BatchJacobiParams" + sfx).c_str())
         .def(py::init<>())
         .def("with_max_block_size", [](typename BJ::parameters_type p, unsigned b) {

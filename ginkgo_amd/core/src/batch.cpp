@@ -327,6 +327,202 @@ gkomi_batch_csr BlockDiagCsr<V, I>::descriptor() const
     return d;
 }
 
+// ------------------------------------------- BlockDiagEll / BlockDiagDense ----
+
+namespace {
+
+// item k's rows from row offsets, checked
+std::vector<int64> item_rows_of(const std::vector<int64>& row_offsets, const char* who)
+{
+    if (row_offsets.empty() || row_offsets.front() != 0) {
+        GKO_DIMENSION_MISMATCH_MSG(std::string(who) + ": row offsets must start at 0");
+    }
+    std::vector<int64> rows(row_offsets.size() - 1);
+    for (size_type k = 0; k + 1 < row_offsets.size(); ++k) {
+        rows[k] = row_offsets[k + 1] - row_offsets[k];
+        if (rows[k] < 0) GKO_DIMENSION_MISMATCH_MSG(std::string(who) + ": decreasing row offsets");
+    }
+    return rows;
+}
+
+template <typename I>
+void check_block_diag_csr(const std::vector<int64>& row_offsets, const std::vector<I>& row_ptrs,
+                          const std::vector<I>& col_idxs, size_type num_values, const char* who)
+{
+    if (row_offsets.empty() || row_ptrs.size() != static_cast<size_type>(row_offsets.back()) + 1 ||
+        static_cast<size_type>(row_ptrs.back()) != col_idxs.size() ||
+        num_values != col_idxs.size()) {
+        GKO_DIMENSION_MISMATCH_MSG(std::string(who) + ": inconsistent arrays");
+    }
+    for (size_type k = 0; k + 1 < row_offsets.size(); ++k) {
+        const int64 n = row_offsets[k + 1] - row_offsets[k];
+        for (int64 r = row_offsets[k]; r < row_offsets[k + 1]; ++r) {
+            if (row_ptrs[r + 1] < row_ptrs[r]) {
+                GKO_DIMENSION_MISMATCH_MSG(std::string(who) + ": decreasing row pointers");
+            }
+            for (I p = row_ptrs[r]; p < row_ptrs[r + 1]; ++p) {
+                if (col_idxs[p] < 0 || col_idxs[p] >= n) {
+                    GKO_DIMENSION_MISMATCH_MSG(std::string(who) +
+                                               ": column index outside its item (local indices)");
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+template <typename V>
+BlockDiagBase<V>::BlockDiagBase(std::shared_ptr<const Executor> exec,
+                                const std::vector<int64>& row_offsets,
+                                const std::vector<int64>& item_offsets,
+                                const std::vector<V>& values)
+    : BatchMatrix<V>(exec, batch_dim<2>{}),
+      row_offsets_(exec, row_offsets),
+      item_offsets_(exec, item_offsets),
+      values_(exec, values),
+      row_offsets_host_(row_offsets),
+      item_offsets_host_(item_offsets)
+{
+    item_rows_ = item_rows_of(row_offsets, "batch BlockDiag matrix");
+    const size_type num = item_rows_.size();
+    item_nnz_.resize(num);
+    int64 max_rows = 0;
+    for (size_type k = 0; k < num; ++k) {
+        item_nnz_[k] = item_offsets[k + 1] - item_offsets[k];
+        max_rows = std::max(max_rows, item_rows_[k]);
+    }
+    this->size_ = batch_dim<2>{num, dim<2>{static_cast<size_type>(max_rows),
+                                           static_cast<size_type>(max_rows)}};
+}
+
+template <typename V>
+gkomi_batch_csr BlockDiagBase<V>::base_descriptor(int32_t format) const
+{
+    gkomi_batch_csr d{};
+    d.num_systems = static_cast<int64_t>(this->size_.num_batch_items);
+    d.max_rows = static_cast<int64_t>(this->size_.common_size.rows);
+    d.max_nnz = item_nnz_.empty() ? 0 : *std::max_element(item_nnz_.begin(), item_nnz_.end());
+    d.row_offsets = row_offsets_.get_const_data();
+    d.item_offsets = item_offsets_.get_const_data();
+    d.values = values_.get_const_data();
+    d.format = format;
+    return d;
+}
+
+template <typename V, typename I>
+BlockDiagEll<V, I>::BlockDiagEll(std::shared_ptr<const Executor> exec,
+                                 const std::vector<int64>& row_offsets,
+                                 const std::vector<int64>& item_offsets,
+                                 const std::vector<I>& col_idxs, const std::vector<V>& values,
+                                 std::vector<int64> widths)
+    : BlockDiagBase<V>(exec, row_offsets, item_offsets, values),
+      col_idxs_(exec, col_idxs),
+      item_width_(std::move(widths))
+{}
+
+template <typename V, typename I>
+std::unique_ptr<BlockDiagEll<V, I>> BlockDiagEll<V, I>::create_from_block_diag_csr_host(
+    std::shared_ptr<const Executor> exec, const std::vector<int64>& row_offsets,
+    const std::vector<I>& row_ptrs, const std::vector<I>& col_idxs, const std::vector<V>& values)
+{
+    const char* who = "batch::matrix::BlockDiagEll";
+    const auto rows = item_rows_of(row_offsets, who);
+    check_block_diag_csr(row_offsets, row_ptrs, col_idxs, values.size(), who);
+    const size_type num = rows.size();
+    std::vector<int64> widths(num, 0), offsets(num + 1, 0);
+    for (size_type k = 0; k < num; ++k) {
+        for (int64 r = row_offsets[k]; r < row_offsets[k + 1]; ++r) {
+            widths[k] = std::max<int64>(widths[k], row_ptrs[r + 1] - row_ptrs[r]);
+        }
+        offsets[k + 1] = offsets[k] + rows[k] * widths[k];
+    }
+    if (offsets.back() >= (int64{1} << 31)) {
+        GKO_NOT_SUPPORTED_MSG("batch::matrix::BlockDiagEll: more than 2^31 - 1 slots");
+    }
+    std::vector<I> ci(static_cast<size_type>(offsets.back()), I{-1});
+    std::vector<V> v(static_cast<size_type>(offsets.back()), V{0});
+    for (size_type k = 0; k < num; ++k) {
+        for (int64 i = 0; i < rows[k]; ++i) {
+            const int64 r = row_offsets[k] + i;
+            for (I p = row_ptrs[r]; p < row_ptrs[r + 1]; ++p) {
+                const int64 slot = offsets[k] + i + (p - row_ptrs[r]) * rows[k];
+                ci[slot] = col_idxs[p];
+                v[slot] = values[p];
+            }
+        }
+    }
+    return std::unique_ptr<BlockDiagEll>(
+        new BlockDiagEll(exec, row_offsets, offsets, ci, v, std::move(widths)));
+}
+
+template <typename V, typename I>
+gkomi_batch_csr BlockDiagEll<V, I>::descriptor() const
+{
+    auto d = this->base_descriptor(GKOMI_BATCH_ELL);
+    d.col_idxs = col_idxs_.get_const_data();
+    return d;
+}
+
+template <typename V>
+std::unique_ptr<BlockDiagDense<V>> BlockDiagDense<V>::create_from_host(
+    std::shared_ptr<const Executor> exec, const std::vector<int64>& row_offsets,
+    const std::vector<V>& values)
+{
+    const auto rows = item_rows_of(row_offsets, "batch::matrix::BlockDiagDense");
+    std::vector<int64> offsets(rows.size() + 1, 0);
+    for (size_type k = 0; k < rows.size(); ++k) offsets[k + 1] = offsets[k] + rows[k] * rows[k];
+    if (static_cast<size_type>(offsets.back()) != values.size()) {
+        GKO_DIMENSION_MISMATCH_MSG(
+            "batch::matrix::BlockDiagDense: values must hold n_k * n_k entries per item");
+    }
+    return std::unique_ptr<BlockDiagDense>(new BlockDiagDense(exec, row_offsets, offsets, values));
+}
+
+template <typename V>
+template <typename I>
+std::unique_ptr<BlockDiagDense<V>> BlockDiagDense<V>::create_from_block_diag_csr_host(
+    std::shared_ptr<const Executor> exec, const std::vector<int64>& row_offsets,
+    const std::vector<I>& row_ptrs, const std::vector<I>& col_idxs, const std::vector<V>& values)
+{
+    const char* who = "batch::matrix::BlockDiagDense";
+    const auto rows = item_rows_of(row_offsets, who);
+    check_block_diag_csr(row_offsets, row_ptrs, col_idxs, values.size(), who);
+    std::vector<int64> offsets(rows.size() + 1, 0);
+    for (size_type k = 0; k < rows.size(); ++k) offsets[k + 1] = offsets[k] + rows[k] * rows[k];
+    std::vector<V> d(static_cast<size_type>(offsets.back()), V{0});
+    for (size_type k = 0; k < rows.size(); ++k) {
+        for (int64 i = 0; i < rows[k]; ++i) {
+            const int64 r = row_offsets[k] + i;
+            for (I p = row_ptrs[r]; p < row_ptrs[r + 1]; ++p) {
+                d[static_cast<size_type>(offsets[k] + i * rows[k] + col_idxs[p])] += values[p];
+            }
+        }
+    }
+    return std::unique_ptr<BlockDiagDense>(new BlockDiagDense(exec, row_offsets, offsets, d));
+}
+
+template <typename V>
+gkomi_batch_csr BlockDiagDense<V>::descriptor() const
+{
+    return this->base_descriptor(GKOMI_BATCH_DENSE);
+}
+
+template class BlockDiagBase<double>;
+template class BlockDiagBase<float>;
+template class BlockDiagEll<double, int32>;
+template class BlockDiagEll<float, int32>;
+template class BlockDiagDense<double>;
+template class BlockDiagDense<float>;
+template std::unique_ptr<BlockDiagDense<double>>
+BlockDiagDense<double>::create_from_block_diag_csr_host<int32>(
+    std::shared_ptr<const Executor>, const std::vector<int64>&, const std::vector<int32>&,
+    const std::vector<int32>&, const std::vector<double>&);
+template std::unique_ptr<BlockDiagDense<float>>
+BlockDiagDense<float>::create_from_block_diag_csr_host<int32>(
+    std::shared_ptr<const Executor>, const std::vector<int64>&, const std::vector<int32>&,
+    const std::vector<int32>&, const std::vector<float>&);
+
 }  // namespace matrix
 
 // ------------------------------------------------------ Bicgstab and Cg ----
@@ -404,7 +600,8 @@ BatchKrylov<V, M>::BatchKrylov(std::shared_ptr<const Executor> exec, parameters_
     auto mat = dynamic_cast<const matrix::BatchMatrix<V>*>(system_.get());
     if (!mat) {
         GKO_NOT_SUPPORTED_MSG(
-            what_of(who, "system must be batch::matrix::Csr, Ell, Dense or BlockDiagCsr"));
+            what_of(who, "system must be a batch::matrix (Csr, Ell, Dense, BlockDiagCsr, "
+                         "BlockDiagEll or BlockDiagDense)"));
     }
     desc_ = mat->descriptor();
     rows = mat->get_item_rows();

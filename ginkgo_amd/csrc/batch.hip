@@ -120,7 +120,8 @@ __host__ __device__ constexpr int batch_vectors(int method)
 template <typename T, int NThreads, bool MatLds, int Fmt, int Method>
 __global__ void __launch_bounds__(NThreads)
     batch_solver_kernel(int64_t num_systems, const int32_t* __restrict__ order,
-                          const int64_t* __restrict__ row_off, const int32_t* __restrict__ rp,
+                          const int64_t* __restrict__ row_off,
+                          const int64_t* __restrict__ item_off, const int32_t* __restrict__ rp,
                           const int32_t* __restrict__ ci, const T* __restrict__ vals,
                           int64_t uni_n, int64_t uni_nnz, int32_t ell_width,
                           const T* __restrict__ b, T* __restrict__ x, int32_t max_iters,
@@ -159,7 +160,30 @@ __global__ void __launch_bounds__(NThreads)
         if (k >= num_systems) break;
         const int64_t sys = order ? order[k] : k;
         sys_view sv;
-        if constexpr (Fmt != kFmtUneven) {
+        // Ell slots per row of this system
+        int32_t ew = ell_width;
+        if constexpr (Fmt == kFmtEll || Fmt == kFmtDense) {
+            if (row_off != nullptr) {
+                // uneven Ell / Dense: the system's rows from row_off, its
+                // stored entries (and Ell column indices) from item_off
+                sv.row0 = row_off[sys];
+                sv.n = static_cast<int32_t>(row_off[sys + 1] - sv.row0);
+                sv.nnz0 = item_off[sys];
+                sv.nnz = static_cast<int32_t>(item_off[sys + 1] - sv.nnz0);
+                sv.rp = rp;
+                sv.ci = Fmt == kFmtEll ? ci + sv.nnz0 : ci;
+                sv.rp_base = 0;
+                ew = sv.n > 0 ? sv.nnz / sv.n : 0;
+            } else {
+                sv.n = static_cast<int32_t>(uni_n);
+                sv.nnz = static_cast<int32_t>(uni_nnz);
+                sv.row0 = sys * uni_n;
+                sv.nnz0 = sys * uni_nnz;
+                sv.rp = rp;
+                sv.ci = ci;
+                sv.rp_base = 0;
+            }
+        } else if constexpr (Fmt != kFmtUneven) {
             sv.n = static_cast<int32_t>(uni_n);
             sv.nnz = static_cast<int32_t>(uni_nnz);
             sv.row0 = sys * uni_n;
@@ -236,7 +260,7 @@ __global__ void __launch_bounds__(NThreads)
                     sum += a * in[j];
                 }
             } else if constexpr (Fmt == kFmtEll) {
-                for (int j = 0; j < ell_width; ++j) {
+                for (int j = 0; j < ew; ++j) {
                     const int64_t e = row + static_cast<int64_t>(j) * n;
                     const int32_t c = col(e);
                     if (c < 0) break;
@@ -291,7 +315,7 @@ __global__ void __launch_bounds__(NThreads)
                                    : gv[static_cast<int64_t>(row) * n + row];
                 if (a != T(0)) d = T(1) / a;
             } else if constexpr (Fmt == kFmtEll) {
-                for (int j = 0; j < ell_width; ++j) {
+                for (int j = 0; j < ew; ++j) {
                     const int64_t e = row + static_cast<int64_t>(j) * n;
                     const int32_t c = col(e);
                     if (c < j) break;
@@ -852,7 +876,8 @@ int launch_batch(hipStream_t stream, int grid, size_t lds, int64_t count, const 
                                             hipFuncAttributeMaxDynamicSharedMemorySize,
                                             static_cast<int>(lds)));
     }
-    kern<<<grid, NT, lds, stream>>>(count, order, a->row_offsets, a->row_ptrs, a->col_idxs,
+    kern<<<grid, NT, lds, stream>>>(count, order, a->row_offsets, a->item_offsets, a->row_ptrs,
+                                    a->col_idxs,
                                     static_cast<const T*>(a->values), a->uniform_rows,
                                     a->uniform_nnz, a->ell_width, b, x, st->max_iterations,
                                     st->tolerance_type,
@@ -909,8 +934,9 @@ int check_batch_matrix(const gkomi_batch_csr* a)
                       a->format == GKOMI_BATCH_DENSE,
                   GKOMI_INVALID_ARGUMENT, "batch: format must be CSR, ELL or DENSE");
     const bool uniform = a->row_offsets == nullptr;
-    GKOMI_REQUIRE(uniform || a->format == GKOMI_BATCH_CSR, GKOMI_NOT_SUPPORTED,
-                  "batch: Ell and Dense batches have one common size (row_offsets NULL)");
+    GKOMI_REQUIRE(uniform || a->format == GKOMI_BATCH_CSR || a->item_offsets != nullptr,
+                  GKOMI_INVALID_ARGUMENT,
+                  "batch: uneven Ell and Dense batches need item_offsets");
     GKOMI_REQUIRE(!uniform || (a->uniform_rows == a->max_rows && a->uniform_nnz == a->max_nnz),
                   GKOMI_INVALID_ARGUMENT, "batch: uniform batch needs max_rows/max_nnz = n/nnz");
     GKOMI_REQUIRE(uniform || a->uniform_cols == 0, GKOMI_INVALID_ARGUMENT,
@@ -920,7 +946,11 @@ int check_batch_matrix(const gkomi_batch_csr* a)
     GKOMI_REQUIRE(a->max_nnz < (int64_t{1} << 31), GKOMI_NOT_SUPPORTED,
                   "batch: at most 2^31 - 1 stored values per system");
     const int64_t cols = a->uniform_cols ? a->uniform_cols : a->uniform_rows;
-    if (a->format == GKOMI_BATCH_ELL) {
+    if (!uniform && a->format != GKOMI_BATCH_CSR) {
+        GKOMI_REQUIRE(a->num_systems == 0 || a->max_nnz == 0 ||
+                          (a->values && (a->format == GKOMI_BATCH_DENSE || a->col_idxs)),
+                      GKOMI_INVALID_ARGUMENT, "batch: null column indices or values");
+    } else if (a->format == GKOMI_BATCH_ELL) {
         GKOMI_REQUIRE(a->ell_width >= 0 && a->uniform_nnz == a->uniform_rows * a->ell_width,
                       GKOMI_INVALID_ARGUMENT,
                       "batch Ell: uniform_nnz must be uniform_rows * ell_width");
@@ -1203,6 +1233,7 @@ int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
 template <typename T, int Fmt, bool Adv>
 __global__ void __launch_bounds__(kBlock)
     batch_spmv_kernel(int64_t num_systems, const int64_t* __restrict__ row_off,
+                      const int64_t* __restrict__ item_off,
                       const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                       const T* __restrict__ vals, int64_t uni_rows, int64_t uni_cols,
                       int64_t uni_nnz, int32_t ell_width, const T* __restrict__ alpha,
@@ -1213,7 +1244,16 @@ __global__ void __launch_bounds__(kBlock)
         const int32_t* irp = rp;
         const int32_t* ici = ci;
         int32_t base = 0;
-        if constexpr (Fmt == kFmtUneven) {
+        int32_t ew = ell_width;
+        if ((Fmt == kFmtEll || Fmt == kFmtDense) && row_off != nullptr) {
+            row0 = row_off[sys];
+            rows = row_off[sys + 1] - row0;
+            cols = rows;
+            col0 = row0;
+            nnz0 = item_off[sys];
+            if (Fmt == kFmtEll) ici = ci + nnz0;
+            ew = rows > 0 ? static_cast<int32_t>((item_off[sys + 1] - nnz0) / rows) : 0;
+        } else if constexpr (Fmt == kFmtUneven) {
             row0 = row_off[sys];
             rows = row_off[sys + 1] - row0;
             cols = rows;
@@ -1252,7 +1292,7 @@ __global__ void __launch_bounds__(kBlock)
             for (int64_t row = threadIdx.x; row < rows; row += kBlock) {
                 T sum = 0;
                 if constexpr (Fmt == kFmtEll) {
-                    for (int j = 0; j < ell_width; ++j) {
+                    for (int j = 0; j < ew; ++j) {
                         const int64_t e = row + static_cast<int64_t>(j) * rows;
                         const int32_t c = ici[e];
                         if (c < 0) break;
@@ -1288,11 +1328,13 @@ int batch_spmv_launch(hipStream_t s, const gkomi_batch_csr* a, const T* alpha, c
     const T* v = static_cast<const T*>(a->values);
     if (alpha) {
         batch_spmv_kernel<T, Fmt, true><<<grid, kBlock, 0, s>>>(
-            a->num_systems, a->row_offsets, a->row_ptrs, a->col_idxs, v, a->uniform_rows, cols,
+            a->num_systems, a->row_offsets, a->item_offsets, a->row_ptrs, a->col_idxs, v,
+            a->uniform_rows, cols,
             a->uniform_nnz, a->ell_width, alpha, b, beta, x);
     } else {
         batch_spmv_kernel<T, Fmt, false><<<grid, kBlock, 0, s>>>(
-            a->num_systems, a->row_offsets, a->row_ptrs, a->col_idxs, v, a->uniform_rows, cols,
+            a->num_systems, a->row_offsets, a->item_offsets, a->row_ptrs, a->col_idxs, v,
+            a->uniform_rows, cols,
             a->uniform_nnz, a->ell_width, alpha, b, beta, x);
     }
     GKOMI_CHECK_LAUNCH("batch_spmv_kernel");

@@ -72,14 +72,17 @@ Timer = _g.Timer
 CpuTimer = _g.CpuTimer
 Mi355xTimer = _g.Mi355xTimer
 
-# batch module: batch.matrix.Csr (shared pattern), Ell, Dense, BlockDiagCsr
+# batch module: batch.matrix.Csr (shared pattern), Ell, Dense, BlockDiagCsr /
+# BlockDiagEll / BlockDiagDense
 # (items of different sizes), batch.MultiVector, batch.solver.Bicgstab and Cg,
 # batch.preconditioner.Jacobi, batch.log.BatchConvergence, batch.stop.tolerance_type
 batch = types.SimpleNamespace(
     MultiVector=_typed("BatchMultiVector"),
     matrix=types.SimpleNamespace(Csr=_typed("BatchCsr"), Ell=_typed("BatchEll"),
                                  Dense=_typed("BatchDense"),
-                                 BlockDiagCsr=_typed("BatchBlockDiagCsr")),
+                                 BlockDiagCsr=_typed("BatchBlockDiagCsr"),
+                                 BlockDiagEll=_typed("BatchBlockDiagEll"),
+                                 BlockDiagDense=_typed("BatchBlockDiagDense")),
     stop=types.SimpleNamespace(tolerance_type=_g.batch_tolerance_type),
     preconditioner=types.SimpleNamespace(
         Jacobi=types.SimpleNamespace(build=lambda dtype="f64": _typed("BatchJacobiParams")(dtype)())),

@@ -754,13 +754,21 @@ int gkomi_bicgstab_finalize_f32(const gkomi_ctx* ctx, int64_t n, int64_t nrhs, f
  *   uniform (row_offsets == NULL): every system has uniform_rows rows and the
  *   shared pattern row_ptrs[uniform_rows + 1] / col_idxs[uniform_nnz];
  *   system k's values start at k * uniform_nnz.
- *  format GKOMI_BATCH_ELL (uniform only; batch_ell.hpp): col_idxs[uniform_rows *
- *   ell_width], shared, column-major (slot j of row i at i + j * uniform_rows),
- *   -1 after the last entry of a row; system k's values, in the same layout,
- *   start at k * uniform_nnz with uniform_nnz = uniform_rows * ell_width.
- *  format GKOMI_BATCH_DENSE (uniform only; batch_dense.hpp): system k's
- *   values, row-major with stride uniform_cols, start at k * uniform_nnz with
+ *  format GKOMI_BATCH_ELL (batch_ell.hpp), uniform (row_offsets == NULL):
+ *   col_idxs[uniform_rows * ell_width], shared, column-major (slot j of row i
+ *   at i + j * uniform_rows), -1 after the last entry of a row; system k's
+ *   values, in the same layout, start at k * uniform_nnz with uniform_nnz =
+ *   uniform_rows * ell_width.
+ *  format GKOMI_BATCH_DENSE (batch_dense.hpp), uniform: system k's values,
+ *   row-major with stride uniform_cols, start at k * uniform_nnz with
  *   uniform_nnz = uniform_rows * uniform_cols; row_ptrs and col_idxs unused.
+ *  ELL and DENSE, uneven (row_offsets != NULL; no reference counterpart,
+ *   the reference's Ell and Dense batches have one size): system k has
+ *   n_k = row_offsets[k+1] - row_offsets[k] rows (square) and its stored
+ *   entries at [item_offsets[k], item_offsets[k+1]) -- ELL: n_k * w_k slots,
+ *   column-major within the system (slot j of row i at i + j * n_k), its
+ *   own local column indices at the same positions of col_idxs, -1 after a
+ *   row's last entry; DENSE: n_k * n_k values, row-major.
  * uniform_cols == 0 means uniform_cols = uniform_rows; the solvers need square
  * systems, gkomi_batch_spmv_* takes rectangular ones.  A descriptor with
  * format, ell_width and uniform_cols zero is the Csr form above. */
@@ -781,6 +789,7 @@ typedef struct {
     int32_t format;
     int32_t ell_width;
     int64_t uniform_cols;
+    const int64_t* item_offsets; /* uneven ELL / DENSE only, num_systems + 1 */
 } gkomi_batch_csr;
 
 /* BatchLinOp::apply of batch::matrix::{Csr, Ell, Dense}

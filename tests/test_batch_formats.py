@@ -119,3 +119,70 @@ def test_ell_and_dense_reject_bad_input():
                                               np.ones(3))
     with pytest.raises(gko.DimensionMismatch):
         gko.batch.matrix.Dense("f64").from_numpy(host, np.ones((2, 3)))
+
+
+def block_diag_csr_of(blocks):
+    """Block-diagonal CSR (global row pointers, local columns) of dense
+    blocks of different sizes."""
+    row_off, rp, ci, v = [0], [0], [], []
+    for d in blocks:
+        for i in range(d.shape[0]):
+            cols = np.nonzero(d[i])[0]
+            ci.extend(cols)
+            v.extend(d[i, cols])
+            rp.append(len(ci))
+        row_off.append(row_off[-1] + d.shape[0])
+    return (np.array(row_off, np.int64), np.array(rp, np.int32), np.array(ci, np.int32),
+            np.array(v))
+
+
+def uneven_blocks(seed=3, num=7):
+    rng = np.random.default_rng(seed)
+    blocks = []
+    for k in range(num):
+        n = int(rng.integers(1, 9))
+        d = rng.uniform(-1, 1, (n, n)) * (rng.random((n, n)) < 0.4)
+        d[np.arange(n), np.arange(n)] = 4.0 + k
+        blocks.append(d)
+    return blocks
+
+
+def test_block_diag_ell_and_dense_layout():
+    """The uneven Ell / Dense batches (extension: items of different sizes)
+    built on the host executor: per item n_k x w_k column-major slots with
+    the item's own local columns and -1 after a row's end, w_k the item's
+    longest row; Dense n_k x n_k row-major; item offsets their prefix sums."""
+    import ginkgo_amd.gko as gko
+    host = gko.HostExecutor.create()
+    blocks = uneven_blocks()
+    row_off, rp, ci, v = block_diag_csr_of(blocks)
+    E = gko.batch.matrix.BlockDiagEll("f64").from_numpy(host, row_off, rp, ci, v)
+    widths = [max(int(np.count_nonzero(d[i])) for i in range(d.shape[0])) for d in blocks]
+    offs = np.concatenate([[0], np.cumsum([d.shape[0] * w for d, w in zip(blocks, widths)])])
+    np.testing.assert_array_equal(E.item_widths, widths)
+    np.testing.assert_array_equal(E.item_offsets, offs)
+    eci, ev = E.col_idxs(), E.values()
+    for k, d in enumerate(blocks):
+        n, w = d.shape[0], widths[k]
+        for i in range(n):
+            cols = np.nonzero(d[i])[0]
+            for s in range(w):
+                slot = offs[k] + i + s * n
+                if s < len(cols):
+                    assert eci[slot] == cols[s] and ev[slot] == d[i, cols[s]]
+                else:
+                    assert eci[slot] == -1 and ev[slot] == 0.0
+    D1 = gko.batch.matrix.BlockDiagDense("f64").from_block_diag_csr(host, row_off, rp, ci, v)
+    D2 = gko.batch.matrix.BlockDiagDense("f64").from_numpy(
+        host, row_off, np.concatenate([d.reshape(-1) for d in blocks]))
+    want = np.concatenate([d.reshape(-1) for d in blocks])
+    np.testing.assert_array_equal(D1.values(), want)
+    np.testing.assert_array_equal(D2.values(), want)
+    np.testing.assert_array_equal(D1.item_offsets,
+                                  np.concatenate([[0], np.cumsum([d.size for d in blocks])]))
+    with pytest.raises(gko.DimensionMismatch):
+        gko.batch.matrix.BlockDiagDense("f64").from_numpy(host, row_off, np.ones(3))
+    with pytest.raises(gko.DimensionMismatch):   # a column outside its item
+        gko.batch.matrix.BlockDiagEll("f64").from_numpy(
+            host, np.array([0, 1]), np.array([0, 1], np.int32), np.array([1], np.int32),
+            np.ones(1))

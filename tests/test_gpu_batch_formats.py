@@ -273,8 +273,8 @@ def test_ell_jacobi_missing_diagonal_is_one(gko, ex):
 
 
 def test_descriptor_errors(gko, ex):
-    """The C ABI rejects an uneven Ell, a rectangular solve and a half-given
-    alpha/beta pair."""
+    """The C ABI rejects an uneven Ell without item offsets, a rectangular
+    solve and a half-given alpha/beta pair."""
     from ginkgo_amd import _capi as C, device as D
     x = torch.zeros(8, dtype=torch.float64, device="cuda")
     ctx = D.ctx(x.device)
@@ -282,7 +282,7 @@ def test_descriptor_errors(gko, ex):
     ci = torch.zeros(8, dtype=torch.int32, device="cuda")
     d = C.BatchCsr(num_systems=2, max_rows=4, max_nnz=8, row_offsets=D.ptr(off),
                    col_idxs=D.ptr(ci), values=D.ptr(x), format=C.BATCH_ELL, ell_width=2)
-    with pytest.raises(C.NotSupported):
+    with pytest.raises(C.InvalidArgument):
         C.call("gkomi_batch_spmv_f64", ctypes.byref(ctx), ctypes.byref(d), None, D.ptr(x), None,
                D.ptr(x))
     d = C.BatchCsr(num_systems=2, max_rows=2, max_nnz=4, values=D.ptr(x), uniform_rows=2,
@@ -340,3 +340,91 @@ def test_reference_stencil_kats_in_ell_and_dense(gko, ex, fmt, method):
         assert np.all(its == kit["check_iters_eq"])
     else:
         assert np.all(its <= kit["check_iters_le"])
+
+
+# ------------------------------------------- uneven Ell / Dense batches ----
+
+def uneven_dense_blocks(num, n_min, n_max, seed, spd):
+    """Dense blocks of different sizes (the diagonal plus 4 random columns
+    per row; SPD ones symmetric), diagonally dominant."""
+    rng = np.random.default_rng(seed)
+    blocks = []
+    for k in range(num):
+        n = int(rng.integers(n_min, n_max + 1))
+        d, _ = random_items(1, n, n, 4, seed=seed * 1000 + k, spd=spd)
+        blocks.append(d[0])
+    return blocks
+
+
+def block_diag(blocks):
+    row_off, rp, ci, v = [0], [0], [], []
+    for d in blocks:
+        for i in range(d.shape[0]):
+            cols = np.nonzero(d[i])[0]
+            ci.extend(cols)
+            v.extend(d[i, cols])
+            rp.append(len(ci))
+        row_off.append(row_off[-1] + d.shape[0])
+    return (np.array(row_off, np.int64), np.array(rp, np.int32), np.array(ci, np.int32),
+            np.array(v))
+
+
+def test_apply_uneven_ell_and_dense(gko, ex):
+    """BlockDiagEll and BlockDiagDense products: every item against its dense
+    block; the Ell product sums a row in the stored order, as the
+    BlockDiagCsr product does, so the two agree bit for bit."""
+    blocks = uneven_dense_blocks(150, 1, 70, seed=4, spd=False)
+    row_off, rp, ci, v = block_diag(blocks)
+    b = np.random.default_rng(8).uniform(-1, 1, row_off[-1])
+    out = {}
+    for name, A in (("csr", gko.batch.matrix.BlockDiagCsr("f64").from_numpy(ex, row_off, rp, ci, v)),
+                    ("ell", gko.batch.matrix.BlockDiagEll("f64").from_numpy(ex, row_off, rp, ci, v)),
+                    ("dense", gko.batch.matrix.BlockDiagDense("f64").from_block_diag_csr(
+                        ex, row_off, rp, ci, v))):
+        x = gko.batch.MultiVector("f64").uneven(ex, row_off, np.zeros(row_off[-1]))
+        A.apply(gko.batch.MultiVector("f64").uneven(ex, row_off, b), x)
+        out[name] = x.to_numpy()
+    for k, d in enumerate(blocks):
+        r0, r1 = row_off[k], row_off[k + 1]
+        ref = d @ b[r0:r1]
+        tol = 4 * d.shape[0] * np.finfo(np.float64).eps * (np.abs(d) @ np.abs(b[r0:r1])) + 1e-300
+        for name in out:
+            assert np.all(np.abs(out[name][r0:r1] - ref) <= tol), (name, k)
+    np.testing.assert_array_equal(out["ell"], out["csr"])
+
+
+@pytest.mark.parametrize("method", ["bicgstab", "cg"])
+def test_solvers_uneven_ell_and_dense(gko, ex, method):
+    """Batched BiCGSTAB and CG over BlockDiagEll and BlockDiagDense items of
+    16 to 64 rows: iteration counts within one of the oracle's (the
+    reference's per-system loop on the Csr form of each item), every
+    system's true relative residual at the tolerance, and the Ell and Dense
+    solves agreeing (Dense adds each row's zeros, 0 * x_j = 0)."""
+    blocks = uneven_dense_blocks(200, 16, 64, seed=7, spd=(method == "cg"))
+    row_off, rp, ci, v = block_diag(blocks)
+    b = np.random.default_rng(9).uniform(-1, 1, row_off[-1])
+    solver_t = gko.batch.solver.Cg if method == "cg" else gko.batch.solver.Bicgstab
+    got = {}
+    for name, A in (("ell", gko.batch.matrix.BlockDiagEll("f64").from_numpy(ex, row_off, rp, ci, v)),
+                    ("dense", gko.batch.matrix.BlockDiagDense("f64").from_block_diag_csr(
+                        ex, row_off, rp, ci, v))):
+        s = (solver_t.build("f64").with_max_iterations(300).with_tolerance(1e-10)
+             .with_tolerance_type(gko.batch.stop.tolerance_type.relative)
+             .with_preconditioner(gko.batch.preconditioner.Jacobi.build("f64"))
+             .on(ex).generate(A))
+        log = gko.batch.log.BatchConvergence("f64").create()
+        s.add_logger(log)
+        x = gko.batch.MultiVector("f64").uneven(ex, row_off, np.zeros(row_off[-1]))
+        s.apply(gko.batch.MultiVector("f64").uneven(ex, row_off, b), x)
+        got[name] = (x.to_numpy(), np.array(log.num_iterations))
+    oracle_solve = ora.batch_cg_solve if method == "cg" else ora.batch_bicgstab_solve
+    xo, io, ro = oracle_solve(row_off, rp, ci, v, b, np.zeros_like(b), use_jacobi=True,
+                              relative=True, tol=1e-10, max_iters=300)
+    for name, (xh, it) in got.items():
+        assert np.max(np.abs(it - io)) <= 1, name
+        for k, d in enumerate(blocks):
+            r0, r1 = row_off[k], row_off[k + 1]
+            res = np.linalg.norm(b[r0:r1] - d @ xh[r0:r1]) / np.linalg.norm(b[r0:r1])
+            assert res <= (1e-8 if method == "cg" else 1e-9), (name, k, res)
+    assert np.max(np.abs(got["ell"][1] - got["dense"][1])) <= 1
+    np.testing.assert_allclose(got["ell"][0], got["dense"][0], rtol=1e-7, atol=1e-9)
