@@ -92,7 +92,9 @@ class Csr(ctypes.Structure):
                 ("num_srow", ctypes.c_int64), ("col_offsets", ctypes.c_void_p),
                 ("work", ctypes.c_void_p), ("work_bytes", ctypes.c_int64),
                 ("panels", ctypes.c_void_p), ("panels_bytes", ctypes.c_int64),
-                ("num_panels", ctypes.c_int32)]
+                ("num_panels", ctypes.c_int32), ("slice_sets", ctypes.c_void_p),
+                ("slice_cols", ctypes.c_void_p), ("slice_offsets", ctypes.c_void_p),
+                ("slice_values", ctypes.c_void_p)]
 
 
 class Sellp(ctypes.Structure):
@@ -232,6 +234,7 @@ _SIGS = {
     "gkomi_csr_strategy_work_bytes": (_I64, [_I32, _I64, _I64]),
     "gkomi_csr_col_offsets": (ctypes.c_int, [_CTX, ctypes.POINTER(Csr), _P, _P]),
     "gkomi_csr_panels_bytes": (_I64, [_I64, _I64, _I32, _I32]),
+    "gkomi_sellp_col_offsets": (ctypes.c_int, [_CTX, ctypes.POINTER(Sellp), _P, _P]),
     "gkomi_csr_build_panels_f64": (ctypes.c_int, [_CTX, ctypes.POINTER(Csr), _I32, _P, _I64]),
     "gkomi_csr_build_panels_f32": (ctypes.c_int, [_CTX, ctypes.POINTER(Csr), _I32, _P, _I64]),
     "gkomi_csr_transpose_workspace_bytes": (_SZ, [_I64, _I64, _I64]),

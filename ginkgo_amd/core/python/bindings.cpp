@@ -403,6 +403,8 @@ void bind_value_type(py::module_& m, const std::string& sfx)
         .def("max_row_nnz", &Csr::max_row_nnz)
         .def("compact_indices", &Csr::compact_indices)
         .def("build_column_panels", &Csr::build_column_panels, py::arg("num_panels") = 4)
+        .def("build_slices", &Csr::build_slices, py::arg("compact") = true)
+        .def_property_readonly("has_slices", &Csr::has_slices)
         .def_property_readonly("num_column_panels", &Csr::get_num_column_panels)
         .def_property_readonly("has_compact_indices", &Csr::has_compact_indices)
         .def_property_readonly("effective_strategy", &Csr::get_effective_strategy)

@@ -212,7 +212,39 @@ gkomi_csr Csr<V, I>::descriptor() const
         d.panels_bytes = static_cast<int64>(panels_.get_size());
         d.num_panels = num_panels_;
     }
+    if (slices_valid_ && slices_) {
+        d.slice_sets = slices_->get_const_slice_sets();
+        d.slice_cols = slices_->get_const_col_idxs();
+        d.slice_values = slices_->get_const_values();
+        d.slice_offsets = slice_offsets_valid_ ? slice_offsets_.get_const_data() : nullptr;
+    }
     return d;
+}
+
+template <typename V, typename I>
+void Csr<V, I>::build_slices(bool compact)
+{
+    slices_valid_ = false;
+    slice_offsets_valid_ = false;
+    slices_.reset();
+    slice_offsets_.clear();
+    auto exec = this->get_executor();
+    if (exec->is_host()) {
+        GKO_NOT_SUPPORTED_MSG("the sliced copy lives on a device executor");
+    }
+    std::shared_ptr<const Sellp<V, I>> s(to_sellp(64, 1));
+    if (compact && s->get_num_stored_elements() > 0) {
+        array<int16> off(exec, s->get_num_stored_elements());
+        array<int32> fits(exec, 1);
+        const auto sd = s->descriptor();
+        GKO_CALL(gkomi_sellp_col_offsets(ctx_of(exec), &sd, off.get_data(), fits.get_data()));
+        if (exec->copy_val_to_host(fits.get_const_data()) == 1) {
+            slice_offsets_ = std::move(off);
+            slice_offsets_valid_ = true;
+        }
+    }
+    slices_ = std::move(s);
+    slices_valid_ = true;
 }
 
 template <typename V, typename I>

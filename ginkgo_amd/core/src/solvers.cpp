@@ -680,6 +680,14 @@ Ir<V>::Ir(std::shared_ptr<const Executor> exec, const parameters_type& p,
                 // entry, bit-identical results)
                 auto lower = csr->template convert_precision<Other>();
                 lower->compact_indices();
+                // rows of nearly equal length also get the sliced copy
+                // (SELL-P layout: coalesced loads, a lane per row)
+                const auto n = lower->get_size().rows;
+                const double mean =
+                    n ? double(lower->get_num_stored_elements()) / double(n) : 0.0;
+                if (n > 0 && double(lower->max_row_nnz()) <= 1.25 * mean + 4.0) {
+                    lower->build_slices(true);
+                }
                 inner_a = std::shared_ptr<const LinOp>(std::move(lower));
             }
         }

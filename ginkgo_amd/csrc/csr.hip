@@ -211,6 +211,28 @@ __global__ void __launch_bounds__(kBlock)
                              b, bs, c, cs, alpha, beta);
 }
 
+// The CLASSICAL single right-hand-side product over the sliced copy: one
+// wave per 64-row slice, slices walked in XCD-contiguous ranges.
+template <typename T, int Mode, bool Compact>
+__global__ void __launch_bounds__(kBlock)
+    csr_spmv_sliced(int64_t n_rows, const int32_t* __restrict__ rp,
+                    const int64_t* __restrict__ sets, const int32_t* __restrict__ cols,
+                    const int16_t* __restrict__ offs, const T* __restrict__ vals,
+                    const T* __restrict__ b, int64_t bs, T* __restrict__ c, int64_t cs,
+                    const T* __restrict__ alpha, const T* __restrict__ beta)
+{
+    const int64_t slices = ceildiv(n_rows, kWave);
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+    for (int64_t sl = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kWavesPerBlock +
+                      threadIdx.x / kWave;
+         sl < slices; sl += nwaves) {
+        sliced_slice<T, Compact>(
+            n_rows, sl, rp, sets, cols, offs, vals,
+            [&](int32_t col) { return b[static_cast<int64_t>(col) * bs]; },
+            [&](int64_t row, T sum) { store_result<Mode>(c + row * cs, sum, alpha, beta); });
+    }
+}
+
 template <typename T, int Mode>
 __global__ void __launch_bounds__(kBlock)
     csr_spmm_rowwave(int64_t n_rows, int64_t nrhs, const int32_t* __restrict__ rp,
@@ -696,6 +718,22 @@ int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const 
             a->n_rows, a->row_ptrs, a->col_idxs, v, b, bs, c, cs, a->srow, a->num_srow, group,
             alpha, beta);
         GKOMI_CHECK_LAUNCH("csr_spmv_balanced");
+        return GKOMI_SUCCESS;
+    }
+    if (a->slice_sets != nullptr && a->slice_values != nullptr &&
+        (a->slice_offsets != nullptr || a->slice_cols != nullptr)) {
+        const unsigned g = grid_for(ceildiv(a->n_rows, kWave), kWavesPerBlock, 64);
+        const T* sv = static_cast<const T*>(a->slice_values);
+        if (a->slice_offsets) {
+            csr_spmv_sliced<T, Mode, true><<<g, kBlock, 0, stream>>>(
+                a->n_rows, a->row_ptrs, a->slice_sets, a->slice_cols, a->slice_offsets, sv, b, bs,
+                c, cs, alpha, beta);
+        } else {
+            csr_spmv_sliced<T, Mode, false><<<g, kBlock, 0, stream>>>(
+                a->n_rows, a->row_ptrs, a->slice_sets, a->slice_cols, a->slice_offsets, sv, b, bs,
+                c, cs, alpha, beta);
+        }
+        GKOMI_CHECK_LAUNCH("csr_spmv_sliced");
         return GKOMI_SUCCESS;
     }
     const subwave_shape sh = pick_subwave(a, sizeof(T));

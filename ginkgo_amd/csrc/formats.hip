@@ -433,6 +433,31 @@ size_t transpose_ws_layout(int64_t n_rows, int64_t n_cols, int64_t nnz, void* ba
 
 // --------------------------------------------------------------- drivers ----
 
+// col - row of every slot of a SELL-P matrix with 64-row slices (0 for
+// padding), one wave per slice; a slot that needs a wider offset clears *fits
+__global__ void __launch_bounds__(kBlock)
+    sellp_col_offsets_kernel(int64_t n_rows, int64_t num_slices, const int64_t* __restrict__ sets,
+                             const int32_t* __restrict__ cols, int16_t* __restrict__ out,
+                             int32_t* __restrict__ fits)
+{
+    const int lane = threadIdx.x % kWave;
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+    bool ok = true;
+    for (int64_t sl = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+         sl < num_slices; sl += nwaves) {
+        const int64_t row = sl * kWave + lane;
+        const int64_t e = sets[sl + 1];
+        for (int64_t k = sets[sl]; k < e; ++k) {
+            const int64_t idx = k * kWave + lane;
+            const int32_t c = cols[idx];
+            const int64_t d = c < 0 ? 0 : static_cast<int64_t>(c) - row;
+            ok = ok && d >= -32768 && d <= 32767;
+            out[idx] = static_cast<int16_t>(d);
+        }
+    }
+    if (!ok) atomicMin(fits, 0);
+}
+
 int check_sellp(const gkomi_sellp* a)
 {
     GKOMI_REQUIRE(a != nullptr, GKOMI_INVALID_ARGUMENT, "null SELL-P descriptor");
@@ -686,6 +711,30 @@ int gkomi_csr_to_sellp_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t sli
                            int32_t* s_cols, float* s_vals)
 {
     return csr_to_sellp_impl(ctx, a, slice_size, slice_sets, slice_lengths, s_cols, s_vals);
+}
+
+int gkomi_sellp_col_offsets(const gkomi_ctx* ctx, const gkomi_sellp* a, int16_t* out,
+                            int32_t* fits)
+{
+    if (int rc = enter(ctx)) return rc;
+    if (int rc = check_sellp(a)) return rc;
+    GKOMI_REQUIRE(a->slice_size == kWave && a->stride_factor == 1, GKOMI_NOT_SUPPORTED,
+                  "sellp col_offsets: needs slice size 64 and stride factor 1");
+    GKOMI_REQUIRE(fits != nullptr, GKOMI_INVALID_ARGUMENT, "sellp col_offsets: null output");
+    static_assert(sizeof(int32_t) == sizeof(float), "fill by bit pattern");
+    if (int rc = gkomi_fill_f32(ctx, 1, __builtin_bit_cast(float, int32_t{1}),
+                                reinterpret_cast<float*>(fits))) {
+        return rc;
+    }
+    if (a->n_rows == 0) return GKOMI_SUCCESS;
+    GKOMI_REQUIRE(out != nullptr && a->col_idxs != nullptr, GKOMI_INVALID_ARGUMENT,
+                  "sellp col_offsets: null pointer");
+    const int64_t num_slices = ceildiv(a->n_rows, kWave);
+    sellp_col_offsets_kernel<<<grid_for(num_slices, kWavesPerBlock, 16), kBlock, 0,
+                               stream_of(ctx)>>>(a->n_rows, num_slices, a->slice_sets,
+                                                 a->col_idxs, out, fits);
+    GKOMI_CHECK_LAUNCH("sellp_col_offsets_kernel");
+    return GKOMI_SUCCESS;
 }
 
 int gkomi_sellp_spmv_f64(const gkomi_ctx* ctx, const gkomi_sellp* a, int64_t nrhs,

@@ -300,6 +300,58 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// The start's kernel 1 over the matrix's sliced copy: one wave per 64-row
+// slice, a workgroup's tiles of four slices in a contiguous range.
+constexpr int64_t kSlicedTileRows = int64_t{kWave} * kWavesPerBlock;
+
+template <typename T, bool Compact>
+__global__ void __launch_bounds__(kBlock)
+    cg_fused_init_sliced_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                const int64_t* __restrict__ sets,
+                                const int32_t* __restrict__ cols,
+                                const int16_t* __restrict__ offs, const T* __restrict__ v,
+                                const T* __restrict__ b, const T* __restrict__ x,
+                                T* __restrict__ r, T* __restrict__ pa, T* __restrict__ pb,
+                                T* __restrict__ q, T* __restrict__ scalars,
+                                uint8_t* __restrict__ status, gkomi_stop_flags* __restrict__ flags,
+                                cg_work w)
+{
+    int64_t rb0, rb1;
+    block_range(ceildiv(n, kSlicedTileRows), &rb0, &rb1);
+    const int64_t slices = ceildiv(n, kWave);
+    T acc[2] = {T(0), T(0)};
+    for (int64_t blk = rb0; blk < rb1; ++blk) {
+        const int64_t sl = blk * kWavesPerBlock + threadIdx.x / kWave;
+        if (sl >= slices) continue;
+        sliced_slice<T, Compact>(
+            n, sl, rp, sets, cols, offs, v, [&](int32_t c) { return x[c]; },
+            [&](int64_t row, T sum) {
+                const T bi = b[row];
+                const T ri = bi - sum;
+                r[row] = ri;
+                pa[row] = T(0);
+                pb[row] = T(0);
+                q[row] = T(0);
+                acc[0] += bi * bi;
+                acc[1] += ri * ri;
+            });
+    }
+    store_partials<T, 2>(acc, w.part_init);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scalars[0] = T(0);
+        scalars[1] = T(1);  // prev_rho of iteration 0
+        scalars[2] = T(0);
+        scalars[3] = T(0);
+        scalars[4] = T(0);
+        status[0] = 0;
+        w.status[0] = 0;
+        w.status[1] = 0;
+        flags->iteration = -1;
+        flags->all_stopped = 0;
+        flags->one_changed = 0;
+    }
+}
+
 // Kernel 2 of an iteration (Update) and of the start (!Update):
 //   Update: beta = p.q from the direction partials; x += rho/beta p,
 //           r -= rho/beta q (skipped when beta == 0,
@@ -510,6 +562,40 @@ __global__ void __launch_bounds__(kBlock)
     store_partials<T, 1>(acc, w.part_dir);
 }
 
+// (b) over the matrix's sliced copy: q = A p and the partials of p.q.
+template <typename T, bool Compact>
+__global__ void __launch_bounds__(kBlock)
+    cg_split_spmv_sliced_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                const int64_t* __restrict__ sets,
+                                const int32_t* __restrict__ cols,
+                                const int16_t* __restrict__ offs, const T* __restrict__ v,
+                                const T* __restrict__ p, T* __restrict__ q,
+                                const gkomi_stop_flags* __restrict__ flags, cg_work w)
+{
+    if (flags->all_stopped) return;
+    int64_t rb0, rb1;
+    block_range(ceildiv(n, kSlicedTileRows), &rb0, &rb1);
+    const int64_t slices = ceildiv(n, kWave);
+    T acc[1] = {T(0)};
+    for (int64_t blk = rb0; blk < rb1; ++blk) {
+        const int64_t sl = blk * kWavesPerBlock + threadIdx.x / kWave;
+        if (sl >= slices) continue;
+        sliced_slice<T, Compact>(n, sl, rp, sets, cols, offs, v,
+                                 [&](int32_t c) { return p[c]; },
+                                 [&](int64_t row, T sum) {
+                                     q[row] = sum;
+                                     acc[0] += p[row] * sum;
+                                 });
+    }
+    store_partials<T, 1>(acc, w.part_dir);
+}
+
+bool has_slices(const gkomi_csr* a)
+{
+    return a->slice_sets != nullptr && a->slice_values != nullptr &&
+           (a->slice_offsets != nullptr || a->slice_cols != nullptr);
+}
+
 // Rows of at least this mean length take the split form of kernel 1.
 constexpr double kCgSplitMeanRow = 16.0;
 
@@ -554,7 +640,8 @@ cg_grids cg_grids_for(const gkomi_csr* a, int64_t n)
     if (a->subwave == 0 && mean >= kCgSplitMeanRow) {
         g.sh = split_shape(mean, sizeof(T));
     }
-    const int64_t tile_rows = int64_t{kBlock} / g.sh.sw * g.sh.rpg;
+    const int64_t tile_rows =
+        has_slices(a) ? kSlicedTileRows : int64_t{kBlock} / g.sh.sw * g.sh.rpg;
     g.spmv = fused_grid(ceildiv(std::max<int64_t>(n, 1), tile_rows));
     g.update = static_cast<int>(
         std::max<int64_t>(1, std::min<int64_t>(ceildiv(std::max<int64_t>(n, 1), kUpdBlock),
@@ -580,6 +667,22 @@ int cg_fused_start_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_cr
     T* sc = static_cast<T*>(st->scalars);
     const int64_t n = st->n;
     const cg_grids g = cg_grids_for<T>(a, n);
+    if (has_slices(a)) {
+        const T* sv = static_cast<const T*>(a->slice_values);
+        if (a->slice_offsets) {
+            cg_fused_init_sliced_kernel<T, true><<<g.spmv, kBlock, 0, s>>>(
+                n, a->row_ptrs, a->slice_sets, a->slice_cols, a->slice_offsets, sv,
+                static_cast<const T*>(st->b), static_cast<const T*>(st->x),
+                static_cast<T*>(st->r), static_cast<T*>(st->p_a), static_cast<T*>(st->p_b),
+                static_cast<T*>(st->q), sc, st->status, st->flags, w);
+        } else {
+            cg_fused_init_sliced_kernel<T, false><<<g.spmv, kBlock, 0, s>>>(
+                n, a->row_ptrs, a->slice_sets, a->slice_cols, a->slice_offsets, sv,
+                static_cast<const T*>(st->b), static_cast<const T*>(st->x),
+                static_cast<T*>(st->r), static_cast<T*>(st->p_a), static_cast<T*>(st->p_b),
+                static_cast<T*>(st->q), sc, st->status, st->flags, w);
+        }
+    } else {
 #define GKOMI_INIT_CASE_C(S, R, C)                                                           \
     cg_fused_init_kernel<T, S, R, C><<<g.spmv, kBlock, 0, s>>>(                              \
         n, a->row_ptrs, a->col_idxs, a->col_offsets, v, static_cast<const T*>(st->b),        \
@@ -596,6 +699,7 @@ int cg_fused_start_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_cr
 #undef GKOMI_INIT_CASE
 #undef GKOMI_INIT_CASE_COMPACT
 #undef GKOMI_INIT_CASE_C
+    }
     GKOMI_CHECK_LAUNCH("cg_fused_init_kernel");
     if (st->inv_diag) {
         cg_fused_update_kernel<T, false, true><<<g.update, kUpdBlock, 0, s>>>(
@@ -635,8 +739,9 @@ int cg_fused_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_
     T* sc = static_cast<T*>(st->scalars);
     const int64_t n = st->n;
     const cg_grids g = cg_grids_for<T>(a, n);
-    const bool split =
-        n > 0 && static_cast<double>(a->nnz) / static_cast<double>(n) >= kCgSplitMeanRow;
+    const bool sliced = has_slices(a);
+    const bool split = sliced || (n > 0 && static_cast<double>(a->nnz) / static_cast<double>(n) >=
+                                               kCgSplitMeanRow);
     for (int32_t it = 0; it < count; ++it) {
         T* pin = static_cast<T*>(st->p_a);
         T* pout = static_cast<T*>(st->p_b);
@@ -648,6 +753,19 @@ int cg_fused_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_
                 static_cast<const T*>(st->orig_tau), st->status, st->flags, w, g.update, iter,
                 par);
             GKOMI_CHECK_LAUNCH("cg_split_p_kernel");
+            if (sliced) {
+                const T* sv = static_cast<const T*>(a->slice_values);
+                if (a->slice_offsets) {
+                    cg_split_spmv_sliced_kernel<T, true><<<g.spmv, kBlock, 0, s>>>(
+                        n, a->row_ptrs, a->slice_sets, a->slice_cols, a->slice_offsets, sv, pout,
+                        static_cast<T*>(st->q), st->flags, w);
+                } else {
+                    cg_split_spmv_sliced_kernel<T, false><<<g.spmv, kBlock, 0, s>>>(
+                        n, a->row_ptrs, a->slice_sets, a->slice_cols, a->slice_offsets, sv, pout,
+                        static_cast<T*>(st->q), st->flags, w);
+                }
+                GKOMI_CHECK_LAUNCH("cg_split_spmv_sliced_kernel");
+            } else {
 #define GKOMI_SPLIT_CASE_C(S, R, C)                                                        \
     cg_split_spmv_kernel<T, S, R, C><<<g.spmv, kBlock, 0, s>>>(                            \
         n, a->row_ptrs, a->col_idxs, a->col_offsets, v, pout, static_cast<T*>(st->q),      \
@@ -664,6 +782,7 @@ int cg_fused_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_
 #undef GKOMI_SPLIT_CASE_COMPACT
 #undef GKOMI_SPLIT_CASE_C
             GKOMI_CHECK_LAUNCH("cg_split_spmv_kernel");
+            }
         } else {
 #define GKOMI_DIR_CASE_C(S, R, C)                                                          \
     cg_fused_direction_kernel<T, S, R, C><<<g.spmv, kBlock, 0, s>>>(                       \

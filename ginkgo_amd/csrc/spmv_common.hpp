@@ -257,6 +257,64 @@ __device__ __forceinline__ void subwave_finish(int64_t n_rows, const int32_t* __
     }
 }
 
+// One 64-row slice of a CSR matrix's sliced copy (SELL-P layout, slice 64,
+// stride factor 1: slot k of row r at (sets[r / 64] + k) * 64 + r % 64), run
+// by one wave with lane = row.  Each lane walks its own row's slots in order
+// and stops at the row's length (rp[r + 1] - rp[r]), so the padding of a
+// slice is never read; four slots are loaded before they are used.  With
+// Compact the column is row + offs[slot], else cols[slot].  `gather(col)`
+// supplies the vector entry; `row_op(row, sum)` runs for every row below
+// n_rows.  The products and the row sum are formed in double for either
+// value type (a product of two floats is exact in double), so an FP32 row
+// sum is rounded once, when the caller stores it.
+template <typename T, bool Compact, typename TV, typename Gather, typename RowOp>
+__device__ __forceinline__ void sliced_slice(int64_t n_rows, int64_t s,
+                                             const int32_t* __restrict__ rp,
+                                             const int64_t* __restrict__ sets,
+                                             const int32_t* __restrict__ cols,
+                                             const int16_t* __restrict__ offs,
+                                             const TV* __restrict__ vals, Gather gather,
+                                             RowOp row_op)
+{
+    const int lane = threadIdx.x % kWave;
+    const int64_t row = s * kWave + lane;
+    const bool own = row < n_rows;
+    const int32_t len = own ? rp[row + 1] - rp[row] : 0;
+    const int64_t base = sets[s] * kWave + lane;
+    const int32_t r32 = static_cast<int32_t>(row);
+    auto col_of = [&](int64_t idx) -> int32_t {
+        if constexpr (Compact) {
+            return r32 + offs[idx];
+        } else {
+            return cols[idx];
+        }
+    };
+    double sum = 0;
+    int32_t k = 0;
+    for (; k + 4 <= len; k += 4) {
+        const int64_t i0 = base + static_cast<int64_t>(k) * kWave;
+        int32_t c[4];
+        TV v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            c[u] = col_of(i0 + u * kWave);
+            v[u] = vals[i0 + u * kWave];
+        }
+        T x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = gather(c[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            sum = fma(static_cast<double>(v[u]), static_cast<double>(x[u]), sum);
+        }
+    }
+    for (; k < len; ++k) {
+        const int64_t i = base + static_cast<int64_t>(k) * kWave;
+        sum = fma(static_cast<double>(vals[i]), static_cast<double>(gather(col_of(i))), sum);
+    }
+    if (own) row_op(row, static_cast<T>(sum));
+}
+
 // Dispatch over the classical shapes: sw in {1..64}, rpg in {1, 2, 4, 8};
 // CASE(S, R) is expanded for each and must end in `break` or `return`.
 #define GKOMI_SUBWAVE_SWITCH(sh, CASE)                                                  \

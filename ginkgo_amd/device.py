@@ -200,6 +200,29 @@ class DeviceCsr:
         self.panel_buf, self.num_panels = buf, num_panels
         self.desc = self._desc()
 
+    def build_slices(self, compact=True):
+        """Builds the sliced copy (SELL-P layout, 64-row slices; no reference
+        counterpart) that the classical single right-hand-side SpMV and the
+        fused CG then read: one lane per row, a row's entries summed in their
+        stored order.  With compact, the slot columns are kept as 2-byte
+        offsets col - row when they fit.  Returns whether they did.  The copy
+        is taken now: call again after changing the values."""
+        self.slices = None
+        self.desc = self._desc()
+        S = DeviceSellp(self, 64, 1)
+        offs = None
+        if compact and S.total > 0:
+            o = torch.empty(S.total, dtype=torch.int16, device=self.device)
+            fits = torch.zeros(1, dtype=torch.int32, device=self.device)
+            c = ctx(self.device)
+            C.call("gkomi_sellp_col_offsets", ctypes.byref(c), ctypes.byref(S.desc), ptr(o),
+                   ptr(fits))
+            if int(fits.item()) == 1:
+                offs = o
+        self.slices = (S, offs)
+        self.desc = self._desc()
+        return offs is not None
+
     def _desc(self):
         d = C.Csr()
         d.n_rows, d.n_cols, d.nnz = self.n_rows, self.n_cols, self.nnz
@@ -220,6 +243,13 @@ class DeviceCsr:
         d.panels = ptr(buf).value if buf is not None else None
         d.panels_bytes = buf.numel() if buf is not None else 0
         d.num_panels = getattr(self, "num_panels", 0) if buf is not None else 0
+        sl = getattr(self, "slices", None)
+        if sl is not None:
+            S, offs = sl
+            d.slice_sets = ptr(S.slice_sets).value
+            d.slice_values = ptr(S.values).value
+            d.slice_cols = ptr(S.col_idxs).value
+            d.slice_offsets = ptr(offs).value if offs is not None else None
         return d
 
     def spmv(self, b, out=None, stream=None):

@@ -216,6 +216,22 @@ typedef struct {
     void* panels;
     int64_t panels_bytes;
     int32_t num_panels;
+    /* Optional sliced copy (NULL = absent, no reference counterpart): the
+     * same matrix in SELL-P form, 64-row slices and stride factor 1, as
+     * gkomi_sellp_compute_slice_sets / gkomi_csr_to_sellp_* build it:
+     * slice_sets (num_slices + 1), slice_values, and each slot's column as
+     * int32 (slice_cols) or, when slice_offsets is set, as the 2-byte offset
+     * col - row (gkomi_sellp_col_offsets).  One lane per row walks its row's
+     * slots and stops at the row's length (from row_ptrs), so padding is
+     * never read; a row sums its entries in their stored order, in double
+     * for either value type (rounded once to the value type).  When
+     * present, the single right-hand-side CLASSICAL SpMV and the fused CG
+     * read it (same value type as the matrix).  Rebuild after changing the
+     * matrix. */
+    const int64_t* slice_sets;
+    const int32_t* slice_cols;
+    const int16_t* slice_offsets;
+    const void* slice_values;
 } gkomi_csr;
 
 /* Bytes of strategy scratch (gkomi_csr::work) an SpMV with `strategy` needs
@@ -347,6 +363,12 @@ int gkomi_csr_to_sellp_f64(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t sli
 int gkomi_csr_to_sellp_f32(const gkomi_ctx* ctx, const gkomi_csr* a, int64_t slice_size,
                            const int64_t* slice_sets, const int64_t* slice_lengths,
                            int32_t* s_cols, float* s_vals);
+/* Compact slot columns of a SELL-P matrix with 64-row slices (no reference
+ * counterpart): out[slot] = col - row as int16 for every slot, 0 for padding.
+ * *fits (device int32) becomes 1 when every offset lies in [-32768, 32767],
+ * else 0 (then `out` must not be used). */
+int gkomi_sellp_col_offsets(const gkomi_ctx* ctx, const gkomi_sellp* a, int16_t* out,
+                            int32_t* fits);
 /* sellp::spmv / advanced_spmv (reference/matrix/sellp_kernels.cpp:26-103). */
 int gkomi_sellp_spmv_f64(const gkomi_ctx* ctx, const gkomi_sellp* a, int64_t nrhs,
                          const double* b, int64_t b_stride, double* c, int64_t c_stride);
