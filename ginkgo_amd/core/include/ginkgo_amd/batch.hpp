@@ -148,6 +148,13 @@ public:
                                                  const std::vector<IndexType>& row_ptrs,
                                                  const std::vector<IndexType>& col_idxs,
                                                  const std::vector<ValueType>& values);
+    // items of size.rows x size.cols (include/ginkgo/core/matrix/batch_csr.hpp:
+    // batch_dim<2> of any shape); the column indices must be below size.cols
+    static std::unique_ptr<Csr> create_from_host(std::shared_ptr<const Executor> exec,
+                                                 size_type num_items, dim<2> size,
+                                                 const std::vector<IndexType>& row_ptrs,
+                                                 const std::vector<IndexType>& col_idxs,
+                                                 const std::vector<ValueType>& values);
     size_type get_num_elements_per_item() const { return col_idxs_.get_size(); }
     const IndexType* get_const_row_ptrs() const { return row_ptrs_.get_const_data(); }
     const IndexType* get_const_col_idxs() const { return col_idxs_.get_const_data(); }

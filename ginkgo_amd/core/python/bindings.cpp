@@ -187,10 +187,14 @@ void bind_batch(py::module_& m, const std::string& sfx)
                     [](std::shared_ptr<const Executor> e, size_type num, size_type n,
                        py::array_t<int32, py::array::c_style | py::array::forcecast> rp,
                        py::array_t<int32, py::array::c_style | py::array::forcecast> ci,
-                       py::array_t<V, py::array::c_style | py::array::forcecast> v) {
+                       py::array_t<V, py::array::c_style | py::array::forcecast> v,
+                       py::object cols) {
+                        const size_type nc = cols.is_none() ? n : cols.cast<size_type>();
                         return std::shared_ptr<BCsr>(BCsr::create_from_host(
-                            e, num, n, vec<int32>(rp), vec<int32>(ci), vec<V>(v)));
-                    });
+                            e, num, dim<2>{n, nc}, vec<int32>(rp), vec<int32>(ci), vec<V>(v)));
+                    },
+                    py::arg("exec"), py::arg("num_items"), py::arg("rows"), py::arg("row_ptrs"),
+                    py::arg("col_idxs"), py::arg("values"), py::arg("cols") = py::none());
     py::class_<BEll, BM, std::shared_ptr<BEll>>(m, ("BatchEll" + sfx).c_str())
         .def_static("from_numpy",
                     [](std::shared_ptr<const Executor> e, size_type num, size_type rows,

@@ -77,11 +77,33 @@ std::unique_ptr<Csr<V, I>> Csr<V, I>::create_from_host(std::shared_ptr<const Exe
                                                        const std::vector<I>& col_idxs,
                                                        const std::vector<V>& values)
 {
-    if (row_ptrs.size() != n + 1 || static_cast<size_type>(row_ptrs.back()) != col_idxs.size() ||
+    return create_from_host(std::move(exec), num_items, dim<2>{n, n}, row_ptrs, col_idxs,
+                            values);
+}
+
+template <typename V, typename I>
+std::unique_ptr<Csr<V, I>> Csr<V, I>::create_from_host(std::shared_ptr<const Executor> exec,
+                                                       size_type num_items, dim<2> size,
+                                                       const std::vector<I>& row_ptrs,
+                                                       const std::vector<I>& col_idxs,
+                                                       const std::vector<V>& values)
+{
+    if (row_ptrs.size() != size.rows + 1 || row_ptrs.front() != 0 ||
+        static_cast<size_type>(row_ptrs.back()) != col_idxs.size() ||
         values.size() != num_items * col_idxs.size()) {
         GKO_DIMENSION_MISMATCH_MSG("batch::matrix::Csr: inconsistent pattern or values");
     }
-    std::unique_ptr<Csr> m(new Csr(exec, batch_dim<2>{num_items, dim<2>{n, n}}));
+    for (size_type i = 0; i < size.rows; ++i) {
+        if (row_ptrs[i + 1] < row_ptrs[i]) {
+            GKO_DIMENSION_MISMATCH_MSG("batch::matrix::Csr: row pointers must not decrease");
+        }
+    }
+    for (const auto c : col_idxs) {
+        if (c < 0 || static_cast<size_type>(c) >= size.cols) {
+            GKO_DIMENSION_MISMATCH_MSG("batch::matrix::Csr: column index outside the item");
+        }
+    }
+    std::unique_ptr<Csr> m(new Csr(exec, batch_dim<2>{num_items, size}));
     m->row_ptrs_ = array<I>(exec, row_ptrs);
     m->col_idxs_ = array<I>(exec, col_idxs);
     m->values_ = array<V>(exec, values);
@@ -101,6 +123,7 @@ gkomi_batch_csr Csr<V, I>::descriptor() const
     d.values = values_.get_const_data();
     d.uniform_rows = d.max_rows;
     d.uniform_nnz = d.max_nnz;
+    d.uniform_cols = static_cast<int64_t>(this->size_.common_size.cols);
     return d;
 }
 

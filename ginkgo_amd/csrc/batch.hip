@@ -46,11 +46,11 @@ template <typename T, int NThreads>
 __device__ __forceinline__ void block_sum2(T& a, T& b, T* lds)
 {
     constexpr int kW = NThreads / kWave;
-    a = wave_sum(a);
-    b = wave_sum(b);
+    a = wave_total_last(a);
+    b = wave_total_last(b);
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
-    if (lane == 0) {
+    if (lane == kWave - 1) {
         lds[wave] = a;
         lds[kW + wave] = b;
     }
@@ -68,9 +68,14 @@ __device__ __forceinline__ void block_sum2(T& a, T& b, T* lds)
 template <typename T, int NThreads>
 __device__ __forceinline__ T block_sum1(T a, T* lds)
 {
-    T dummy = 0;
-    block_sum2<T, NThreads>(a, dummy, lds);
-    return a;
+    constexpr int kW = NThreads / kWave;
+    a = wave_total_last(a);
+    if (threadIdx.x % kWave == kWave - 1) lds[threadIdx.x / kWave] = a;
+    __syncthreads();
+    T s = lds[0];
+#pragma unroll
+    for (int w = 1; w < kW; ++w) s += lds[w];
+    return s;
 }
 
 // group_sum over aligned groups of g lanes, g a power of two <= 64 known
@@ -583,11 +588,17 @@ __global__ void __launch_bounds__(NT)
         const int32_t* gci = ci + (Fmt == kFmtCsr ? 0 : nnz0);
         const T* gb = b + row0;
         T* gx = x + row0;
-        // K > 0: the own rows' first K entries in registers
+        // K > 0: the own rows' first K entries in registers, the local
+        // columns (below 2^16: a system has at most 2 NT rows) two per
+        // register, entry q in bits 16 (q & 1) .. 16 (q & 1) + 15 of mc[q / 2]
         constexpr int KR = K > 0 ? K : 1;
+        constexpr int KC = (KR + 1) / 2;
         T mv[R][KR];
-        int32_t mc[R][KR];
+        uint32_t mc[R][KC];
         int32_t mb[R], ml[R];
+        auto col_of = [&](int j, int q) -> int {
+            return static_cast<int>((mc[j][q / 2] >> (16 * (q & 1))) & 0xFFFFu);
+        };
         if constexpr (K > 0) {
 #pragma unroll
             for (int j = 0; j < R; ++j) {
@@ -595,10 +606,13 @@ __global__ void __launch_bounds__(NT)
                 mb[j] = row < n ? srp[row] - rp_base : 0;
                 ml[j] = row < n ? srp[row + 1] - rp_base - mb[j] : 0;
 #pragma unroll
+                for (int q = 0; q < KC; ++q) mc[j][q] = 0;
+#pragma unroll
                 for (int q = 0; q < K; ++q) {
                     const bool in_row = q < ml[j];
                     mv[j][q] = in_row ? gv[mb[j] + q] : T(0);
-                    mc[j][q] = in_row ? gci[mb[j] + q] : 0;
+                    const uint32_t c = in_row ? static_cast<uint32_t>(gci[mb[j] + q]) : 0u;
+                    mc[j][q / 2] |= c << (16 * (q & 1));
                 }
             }
         } else {
@@ -633,7 +647,7 @@ __global__ void __launch_bounds__(NT)
                     bool found = false;
 #pragma unroll
                     for (int q = 0; q < K; ++q) {
-                        if (!found && q < ml[j] && mc[j][q] == row) {
+                        if (!found && q < ml[j] && col_of(j, q) == row) {
                             found = true;
                             if (mv[j][q] != T(0)) d = T(1) / mv[j][q];
                         }
@@ -664,8 +678,23 @@ __global__ void __launch_bounds__(NT)
                 T sum = 0;
                 if constexpr (K > 0) {
 #pragma unroll
-                    for (int q = 0; q < K; ++q) {
-                        if (q < ml[j]) sum += mv[j][q] * in[mc[j][q]];
+                    for (int qq = 0; qq < KC; ++qq) {
+                        // the empty asm keeps the packed word opaque, so the
+                        // compiler decodes it here instead of holding every
+                        // entry's LDS address in a register of its own
+                        uint32_t w = mc[j][qq];
+                        asm volatile("" : "+v"(w));
+                        const int q0 = 2 * qq;
+                        // branch-free: a slot past the row's end holds column
+                        // 0 (a valid word of `in`) and its product is not
+                        // added (a select, so the row sum is the stored-order
+                        // sum of the row's own entries)
+                        const T p0 = mv[j][q0] * in[w & 0xFFFFu];
+                        sum = q0 < ml[j] ? sum + p0 : sum;
+                        if (q0 + 1 < K) {
+                            const T p1 = mv[j][q0 + 1] * in[w >> 16];
+                            sum = q0 + 1 < ml[j] ? sum + p1 : sum;
+                        }
                     }
                     for (int32_t e = mb[j] + K; e < mb[j] + ml[j]; ++e) sum += gv[e] * in[gci[e]];
                 } else if (row < n) {
@@ -725,7 +754,10 @@ __global__ void __launch_bounds__(NT)
                 rho_old = rho;
             }
         } else {
-            T rh[R], p[R], ph[R], v[R], s[R], sh[R], t[R];
+            // p_hat = M p and s_hat = M s are formed again where x is
+            // updated (the same products, so the same values) instead of
+            // being held in registers across the products
+            T rh[R], p[R], v[R], s[R], t[R];
             spmv(in0, v);
             T rr = 0;
 #pragma unroll
@@ -748,8 +780,7 @@ __global__ void __launch_bounds__(NT)
                 for (int j = 0; j < R; ++j) {
                     const int row = tid + j * NT;
                     p[j] = r[j] + beta * (p[j] - omega * v[j]);
-                    ph[j] = prec[j] * p[j];
-                    if (row < n) in0[row] = ph[j];
+                    if (row < n) in0[row] = prec[j] * p[j];
                 }
                 __syncthreads();
                 spmv(in0, v);
@@ -763,8 +794,7 @@ __global__ void __launch_bounds__(NT)
                 for (int j = 0; j < R; ++j) {
                     const int row = tid + j * NT;
                     s[j] = r[j] - alpha * v[j];
-                    sh[j] = prec[j] * s[j];
-                    if (row < n) in1[row] = sh[j];
+                    if (row < n) in1[row] = prec[j] * s[j];
                     ss += s[j] * s[j];
                 }
                 // publishes s_hat for the product below as well
@@ -772,7 +802,7 @@ __global__ void __launch_bounds__(NT)
                 res = sqrt(ss);
                 if (res <= limit) {
 #pragma unroll
-                    for (int j = 0; j < R; ++j) xr[j] += alpha * ph[j];
+                    for (int j = 0; j < R; ++j) xr[j] += alpha * (prec[j] * p[j]);
                     break;
                 }
                 spmv(in1, t);
@@ -787,7 +817,7 @@ __global__ void __launch_bounds__(NT)
                 T r2 = 0, rhr = 0;
 #pragma unroll
                 for (int j = 0; j < R; ++j) {
-                    xr[j] += fma(alpha, ph[j], omega * sh[j]);
+                    xr[j] += fma(alpha, prec[j] * p[j], omega * (prec[j] * s[j]));
                     r[j] = s[j] - omega * t[j];
                     r2 += r[j] * r[j];
                     rhr += rh[j] * r[j];
@@ -1038,16 +1068,19 @@ int reg_rows_per_thread(int nt, int64_t rows)
     return 0;
 }
 
-// Entries per row the register kernel keeps in registers (16 or 32; 0 =
-// the matrix in LDS) for `nt` threads with `rpt` rows each, a bin whose
+// Entries per row the register kernel keeps in registers (12, 16 or 32;
+// 0 = the matrix in LDS) for `nt` threads with `rpt` rows each, a bin whose
 // longest row has `max_row_nnz` entries (0 = unknown) and the plan's
 // matrix_in_lds (1 = LDS; 0 = registers, longer rows finishing from global
 // memory; -1 = registers when every row fits).  Two rows per thread keep at
-// most 16 entries each, and only with 256 or 512 threads.
+// most 16 entries each, and only with 256 or 512 threads.  The smallest
+// shape that holds the longest row leaves the most registers free, so the
+// most workgroups resident per CU.
 int reg_row_entries(int nt, int rpt, int64_t max_row_nnz, int mat)
 {
     if (mat == 1 || !(rpt == 1 || (rpt == 2 && nt >= 256))) return 0;
     const int cap = rpt == 1 ? 32 : 16;
+    if (max_row_nnz > 0 && max_row_nnz <= 12) return 12;
     if (max_row_nnz > 0 && max_row_nnz <= 16) return 16;
     if (max_row_nnz > 0 && max_row_nnz <= cap) return cap;
     return mat == 0 ? cap : 0;
@@ -1074,8 +1107,11 @@ int launch_reg_shape(int nt, int rpt, int kreg, hipStream_t s, int grid, size_t 
     GKOMI_REG_CASE(256, 2, 0)
     GKOMI_REG_CASE(512, 2, 0)
     if constexpr (Fmt == kFmtUneven) {
+        GKOMI_REG_SHAPE(12)
         GKOMI_REG_SHAPE(16)
         GKOMI_REG_SHAPE(32)
+        GKOMI_REG_CASE(256, 2, 12)
+        GKOMI_REG_CASE(512, 2, 12)
         GKOMI_REG_CASE(256, 2, 16)
         GKOMI_REG_CASE(512, 2, 16)
     }

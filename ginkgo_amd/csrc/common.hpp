@@ -170,6 +170,42 @@ __device__ __forceinline__ T wave_sum(T v)
     return group_sum<64>(v);
 }
 
+// DPP move restricted to the rows (16-lane groups) in RowMask; the other rows
+// receive 0.
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ float dpp_rows(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), Ctrl, RowMask, 0xF,
+                                                      false));
+}
+
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ double dpp_rows(double v)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(bits & 0xffffffffll), Ctrl,
+                                               RowMask, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(bits >> 32), Ctrl, RowMask,
+                                               0xF, false);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                                (static_cast<unsigned int>(lo)));
+}
+
+// Sum of the 64 lanes of a wave, valid in lane 63 only, without LDS traffic.
+// Each row of 16 lanes is reduced by the butterfly of group_sum; DPP
+// row_bcast:15 (0x142) then adds lane 15's row total into row 1 and lane
+// 47's into row 3, and row_bcast:31 (0x143) adds lane 31 (rows 0 + 1) into
+// row 3.  Lane 63 ends with (r3 + r2) + (r1 + r0): the same pairs as
+// wave_sum's (r0 + r1) + (r2 + r3), so the same bits.
+template <typename T>
+__device__ __forceinline__ T wave_total_last(T v)
+{
+    v = group_sum<16>(v);
+    v += dpp_rows<0x142, 0xA>(v);
+    v += dpp_rows<0x143, 0xC>(v);
+    return v;
+}
+
 // Workgroup sum over blockDim.x == 256 threads; result valid in all threads.
 // `lds` must hold kWavesPerBlock values.
 template <typename T>

@@ -121,6 +121,24 @@ def test_ell_and_dense_reject_bad_input():
         gko.batch.matrix.Dense("f64").from_numpy(host, np.ones((2, 3)))
 
 
+def test_batch_csr_shapes_and_bad_input():
+    """batch::matrix::Csr items of any shape (include/ginkgo/core/matrix/
+    batch_csr.hpp: a batch_dim<2>); patterns that do not fit the item raise."""
+    import ginkgo_amd.gko as gko
+    host = gko.HostExecutor.create()
+    rp = np.array([0, 2, 3], np.int32)
+    ci = np.array([0, 4, 2], np.int32)
+    m = gko.batch.matrix.Csr("f64").from_numpy(host, 3, 2, rp, ci, np.ones(9), cols=5)
+    assert m.size == (3, 2, 5)
+    with pytest.raises(gko.DimensionMismatch):  # column 4 outside a 2 x 2 item
+        gko.batch.matrix.Csr("f64").from_numpy(host, 3, 2, rp, ci, np.ones(9))
+    with pytest.raises(gko.DimensionMismatch):  # values of 2 items, 3 declared
+        gko.batch.matrix.Csr("f64").from_numpy(host, 3, 2, rp, ci, np.ones(6), cols=5)
+    with pytest.raises(gko.DimensionMismatch):  # decreasing row pointers
+        gko.batch.matrix.Csr("f64").from_numpy(host, 1, 2, np.array([0, 3, 2], np.int32),
+                                               np.array([0, 1], np.int32), np.ones(2), cols=5)
+
+
 def block_diag_csr_of(blocks):
     """Block-diagonal CSR (global row pointers, local columns) of dense
     blocks of different sizes."""

@@ -86,7 +86,8 @@ def make(gko, ex, fmt, dense, mask):
     if fmt == "dense":
         return gko.batch.matrix.Dense(dt).from_numpy(ex, dense)
     rp, ci = csr_pattern(mask)
-    return gko.batch.matrix.Csr(dt).from_numpy(ex, num, rows, rp, ci, csr_values(dense, rp, ci))
+    return gko.batch.matrix.Csr(dt).from_numpy(ex, num, rows, rp, ci, csr_values(dense, rp, ci),
+                                               cols=cols)
 
 
 def mv(gko, ex, a):
@@ -123,8 +124,6 @@ def test_apply_reference_fixture(gko, ex, fmt, advanced):
                          ids=["40x40", "200x200", "30x50"])
 def test_apply_random_against_oracle(gko, ex, dtype, fmt, shape):
     num, rows, cols, per_row = shape
-    if fmt == "csr" and rows != cols:
-        pytest.skip("batch::matrix::Csr items are square here")
     dense, mask = random_items(num, rows, cols, per_row, seed=rows + cols, dtype=dtype)
     rng = np.random.default_rng(1)
     b = rng.uniform(-1, 1, size=(num, cols)).astype(dtype)
