@@ -395,11 +395,21 @@ def bench_batch(dev, num=20000):
         x0 = torch.zeros_like(bd)
         st = A.settings(tol=tol, relative=True, max_iters=500, jacobi=True)
         plan = A.tune(bd, x0, st, A.plan())
-        times = []
-        for _ in range(3):
-            x = torch.zeros_like(bd)
-            t, (it, rs) = timed(lambda: A.solve(bd, x, st, plan))
-            times.append(t)
+        chosen = plan.single_launch
+        # both launch forms, the tuner's choice last (its x is checked below)
+        forms = {}
+        for single in (1 - chosen, chosen):
+            plan.single_launch = single
+            times = []
+            for _ in range(3):
+                x = torch.zeros_like(bd)
+                try:
+                    t, (it, rs) = timed(lambda: A.solve(bd, x, st, plan))
+                except Exception:      # one launch does not apply
+                    break
+                times.append(t)
+            if times:
+                forms["one_launch" if single else "per_bin_launches"] = round(min(times), 5)
         key = "f64" if dt == np.float64 else "f32"
         rel = batch_relres(row_off, rp, ci, v.astype(dt), b.astype(dt), x.cpu().numpy())
         # the same solve as one captured HIP graph (x reset inside the graph)
@@ -419,7 +429,11 @@ def bench_batch(dev, num=20000):
                                 "max_true_relres_all_systems": float(rel.max()),
                                 "plan_block_sizes": [plan.block_size[i] for i in range(plan.num_bins)],
                                 "plan_matrix_in_lds": [plan.matrix_in_lds[i] for i in range(plan.num_bins)],
-                                "plan_vectors": [plan.vectors[i] for i in range(plan.num_bins)]}
+                                "plan_vectors": [plan.vectors[i] for i in range(plan.num_bins)],
+                                "plan_single_launch": int(plan.single_launch),
+                                "kernel_launches_per_solve": 1 if plan.single_launch else
+                                int(plan.num_bins),
+                                "time_s_by_launch_form": forms}
         if dt == np.float32:
             out[f"uneven_{key}"]["tolerance_note"] = (
                 "FP32 runs at 1e-5: a relative residual of 1e-10 is below FP32 resolution "

@@ -181,7 +181,7 @@ class BatchSettings(ctypes.Structure):
 
 
 class BatchPlan(ctypes.Structure):
-    _fields_ = [("num_bins", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    _fields_ = [("num_bins", ctypes.c_int32), ("single_launch", ctypes.c_int32),
                 ("order", ctypes.c_void_p),
                 ("bin_begin", ctypes.c_int64 * (BATCH_MAX_BINS + 1)),
                 ("bin_max_rows", ctypes.c_int64 * BATCH_MAX_BINS),

@@ -1,7 +1,8 @@
 // Batched BiCGSTAB and CG for gfx950: thousands of small independent systems.
 //
-// One workgroup solves one system at a time; the systems are size-sorted
-// (largest first) and cut into size bins, one launch per bin.  Two kernel
+// One workgroup solves one system at a time, or several small ones side by
+// side; the systems are size-sorted (largest first) and cut into size bins,
+// one launch per bin or one launch for the whole batch.  Three kernel
 // families:
 //  * batch_solver_kernel: every work vector in LDS, the matrix (values,
 //    16-bit local column indices, row pointers) in LDS when it fits, else
@@ -12,7 +13,11 @@
 //    two rows per thread, only the gathered vectors in LDS beside the matrix
 //    or with the matrix rows in registers too; workgroup b takes system b of
 //    the bin first and uses the queue only when the bin holds more systems
-//    than the grid has workgroups.
+//    than the grid has workgroups;
+//  * batch_packed_kernel (uneven Csr, systems of up to 512 rows, rows of up
+//    to 16 entries): the whole batch in one launch of 256-thread workgroups
+//    over one queue of work items, largest first, an item being one system
+//    of more than 256 rows or 2, 4 or 8 smaller ones side by side.
 // The whole iteration runs inside the kernel, with the reference's
 // per-system semantics and logging of iterations and residual norm: BiCGSTAB
 // (reference/solver/batch_bicgstab_kernels.hpp:180-354: no breakdown guards,
@@ -848,6 +853,429 @@ __global__ void __launch_bounds__(NT)
     }
 }
 
+// ----------------------------------------------------- one-launch kernel ----
+
+// Work items of the one-launch kernel.  Bin q of the plan holds the items
+// [item_begin[q], item_begin[q+1]); item i of bin q is the systems
+// k = sys_begin[q] + (i - item_begin[q]) * (256 >> shift[q]) + g of the size
+// order, g = 0 .. (256 >> shift[q]) - 1, those with k < sys_begin[q+1], one
+// per group of 2^shift[q] threads (each system has at most 2 * 2^shift[q]
+// rows).
+struct batch_items {
+    int32_t num_bins;
+    int32_t shift[GKOMI_BATCH_MAX_BINS];
+    int64_t item_begin[GKOMI_BATCH_MAX_BINS + 1];
+    int64_t sys_begin[GKOMI_BATCH_MAX_BINS + 1];
+};
+
+// Sums a and b over each group of L = 2^shift threads (L = 32, 64, 128 or
+// 256); every thread receives its group's sums.  A wave's rows of 16 lanes
+// are reduced by DPP, then row_bcast:15 leaves each half-wave's total in
+// lanes 31 and 63 and, for groups of a wave or more, row_bcast:31 the wave's
+// total in lane 63; those lanes write one partial each, and after one barrier
+// a group adds its partials in order (for L = 256 the order of block_sum2).
+// The caller alternates between two `lds` buffers of 16 values.
+template <typename T>
+__device__ __forceinline__ void group_sum2_lds(T& a, T& b, T* lds, int shift)
+{
+    const int tid = threadIdx.x;
+    a = group_sum<16>(a);
+    b = group_sum<16>(b);
+    a += dpp_rows<0x142, 0xA>(a);
+    b += dpp_rows<0x142, 0xA>(b);
+    const int pshift = shift < 6 ? 5 : 6;  // threads per partial: 32 or 64
+    if (pshift == 6) {
+        a += dpp_rows<0x143, 0xC>(a);
+        b += dpp_rows<0x143, 0xC>(b);
+    }
+    if ((tid & ((1 << pshift) - 1)) == (1 << pshift) - 1) {
+        lds[tid >> pshift] = a;
+        lds[8 + (tid >> pshift)] = b;
+    }
+    __syncthreads();
+    const int per = 1 << (shift - pshift);
+    const int first = (tid >> shift) * per;
+    T s0 = lds[first], s1 = lds[8 + first];
+    for (int w = 1; w < per; ++w) {
+        s0 += lds[first + w];
+        s1 += lds[8 + first + w];
+    }
+    a = s0;
+    b = s1;
+}
+
+template <typename T>
+__device__ __forceinline__ T group_sum1_lds(T a, T* lds, int shift)
+{
+    const int tid = threadIdx.x;
+    a = group_sum<16>(a);
+    a += dpp_rows<0x142, 0xA>(a);
+    const int pshift = shift < 6 ? 5 : 6;
+    if (pshift == 6) a += dpp_rows<0x143, 0xC>(a);
+    if ((tid & ((1 << pshift) - 1)) == (1 << pshift) - 1) lds[tid >> pshift] = a;
+    __syncthreads();
+    const int per = 1 << (shift - pshift);
+    const int first = (tid >> shift) * per;
+    T s = lds[first];
+    for (int w = 1; w < per; ++w) s += lds[first + w];
+    return s;
+}
+
+// The whole uneven Csr batch in one persistent launch: 256 threads, two rows
+// per thread, the matrix rows (up to K entries, local columns packed two per
+// register) and the work vectors in registers, only the gathered vectors in
+// LDS.  A workgroup takes its first work item by block index and further
+// ones from one queue ordered largest first, so the last work of the launch
+// is its smallest systems and no size bin waits for another's tail.  An item
+// is one system of 257..512 rows or 256 / L systems of up to 2 L rows (L =
+// 128, 64, 32 threads each); the systems of one item iterate side by side,
+// every group stepping its own system with the semantics of the register
+// kernel (BiCGSTAB or CG, iteration counts and residual norms logged per
+// system) and idling once it has stopped, until every group of the item has.
+template <typename T, int K, int Method>
+__global__ void __launch_bounds__(256)
+    batch_packed_kernel(int64_t num_items, batch_items items, const int32_t* __restrict__ order,
+                        const int64_t* __restrict__ row_off, const int32_t* __restrict__ rp,
+                        const int32_t* __restrict__ ci, const T* __restrict__ vals,
+                        const T* __restrict__ b, T* __restrict__ x, int32_t max_iters,
+                        int32_t relative, double tol_d, int32_t jacobi,
+                        int32_t* __restrict__ iters_out, T* __restrict__ res_out,
+                        unsigned* __restrict__ queue, int32_t dynamic)
+{
+    constexpr int NT = 256;
+    constexpr int R = 2;
+    constexpr int KC = (K + 1) / 2;
+    constexpr int kIn = Method == kBatchCg ? 1 : 2;
+    // the gathered vectors (p_hat, s_hat; CG p), and in FP32 also x, prec
+    // and (BiCGSTAB) r_hat, p and v: a thread reads back only its own rows of
+    // those, so they take no barrier, and the registers they free raise the
+    // FP32 kernel from four to five waves per SIMD.  FP64 keeps them in
+    // registers (measured faster at its three waves per SIMD).
+    constexpr bool kLdsVec = sizeof(T) == 4;
+    constexpr int kVec = kIn + (kLdsVec ? (Method == kBatchCg ? 2 : 5) : 0);
+    __shared__ T in_s[kVec][R * NT];
+    __shared__ T red2[2 * 16];
+    __shared__ int32_t go_flag[8];
+    __shared__ int64_t slot;
+    int red_turn = 0;
+    auto red = [&]() {
+        red_turn ^= 1;
+        return red2 + red_turn * 16;
+    };
+    const int tid = threadIdx.x;
+    const T tol = static_cast<T>(tol_d);
+
+    bool first = true;
+    while (true) {
+        __syncthreads();
+        if (tid == 0) {
+            slot = first ? static_cast<int64_t>(blockIdx.x)
+                         : (dynamic ? static_cast<int64_t>(gridDim.x) +
+                                          static_cast<int64_t>(atomicAdd(queue, 1u))
+                                    : num_items);
+        }
+        first = false;
+        __syncthreads();
+        const int64_t item = slot;
+        if (item >= num_items) break;
+        int bin = 0;
+        while (bin + 1 < items.num_bins && item >= items.item_begin[bin + 1]) ++bin;
+        const int shift = items.shift[bin];
+        const int L = 1 << shift;
+        const int G = NT >> shift;
+        const int g = tid >> shift;
+        const int lt = tid & (L - 1);
+        const int64_t k =
+            items.sys_begin[bin] + (item - items.item_begin[bin]) * G + g;
+        const bool valid = k < items.sys_begin[bin + 1];
+        const int64_t sys = valid ? (order ? static_cast<int64_t>(order[k]) : k) : 0;
+        const int64_t row0 = valid ? row_off[sys] : 0;
+        const int32_t n = valid ? static_cast<int32_t>(row_off[sys + 1] - row0) : 0;
+        const int32_t* srp = rp + row0;
+        const int32_t rp_base = valid ? srp[0] : 0;
+        const T* gv = vals + rp_base;
+        const int32_t* gci = ci + rp_base;
+        const T* gb = b + row0;
+        T* gx = x + row0;
+        T* in0 = in_s[0] + g * (R * L);
+        T* in1 = in_s[kIn - 1] + g * (R * L);
+        // own-row vectors: LDS rows (kLdsVec) or registers
+        T* lx = in_s[kLdsVec ? kIn : 0] + g * (R * L);
+        T* lprec = in_s[kLdsVec ? kIn + 1 : 0] + g * (R * L);
+        T* lrh = in_s[kLdsVec ? kVec - 1 : 0] + g * (R * L);
+        T* lp = in_s[kLdsVec && kIn + 2 < kVec ? kIn + 2 : 0] + g * (R * L);
+        T* lv = in_s[kLdsVec && kIn + 3 < kVec ? kIn + 3 : 0] + g * (R * L);
+        T x_r[R], prec_r[R], rh_r[R], p_r[R], v_r[R];
+        auto own_row = [&](T* lds_rows, T* regs, int j) -> T& {
+            if constexpr (kLdsVec) {
+                return lds_rows[lt + j * L];
+            } else {
+                return regs[j];
+            }
+        };
+        auto xr = [&](int j) -> T& { return own_row(lx, x_r, j); };
+        auto prec = [&](int j) -> T& { return own_row(lprec, prec_r, j); };
+
+        T mv[R][K];
+        uint32_t mc[R][KC];
+        int32_t mb[R], ml[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int row = lt + j * L;
+            mb[j] = row < n ? srp[row] - rp_base : 0;
+            ml[j] = row < n ? srp[row + 1] - rp_base - mb[j] : 0;
+#pragma unroll
+            for (int q = 0; q < KC; ++q) mc[j][q] = 0;
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                const bool in_row = q < ml[j];
+                mv[j][q] = in_row ? gv[mb[j] + q] : T(0);
+                const uint32_t c = in_row ? static_cast<uint32_t>(gci[mb[j] + q]) : 0u;
+                mc[j][q / 2] |= c << (16 * (q & 1));
+            }
+        }
+        T r[R];
+        T bb = 0;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int row = lt + j * L;
+            const bool own = row < n;
+            const T xj = own ? gx[row] : T(0);
+            r[j] = own ? gb[row] : T(0);
+            bb += r[j] * r[j];
+            xr(j) = xj;
+            if (own) in0[row] = xj;
+        }
+        __syncthreads();
+        // the scalar Jacobi of the reference's batch Csr: 1 / the first
+        // stored diagonal entry, 1 for a zero or missing one
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int row = lt + j * L;
+            T d = T(1);
+            if (jacobi && row < n) {
+                bool found = false;
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    const int c = static_cast<int>((mc[j][q / 2] >> (16 * (q & 1))) & 0xFFFFu);
+                    if (!found && q < ml[j] && c == row) {
+                        found = true;
+                        if (mv[j][q] != T(0)) d = T(1) / mv[j][q];
+                    }
+                }
+                for (int32_t e = mb[j] + K; !found && e < mb[j] + ml[j]; ++e) {
+                    if (gci[e] == row) {
+                        found = true;
+                        if (gv[e] != T(0)) d = T(1) / gv[e];
+                    }
+                }
+            }
+            prec(j) = d;
+        }
+        // out = A in at the own rows, each row summed in stored order
+        auto spmv = [&](const T* in, T out[R]) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                T sum = 0;
+#pragma unroll
+                for (int qq = 0; qq < KC; ++qq) {
+                    uint32_t w = mc[j][qq];
+                    asm volatile("" : "+v"(w));
+                    const int q0 = 2 * qq;
+                    const T p0 = mv[j][q0] * in[w & 0xFFFFu];
+                    sum = q0 < ml[j] ? sum + p0 : sum;
+                    if (q0 + 1 < K) {
+                        const T p1 = mv[j][q0 + 1] * in[w >> 16];
+                        sum = q0 + 1 < ml[j] ? sum + p1 : sum;
+                    }
+                }
+                for (int32_t e = mb[j] + K; e < mb[j] + ml[j]; ++e) sum += gv[e] * in[gci[e]];
+                out[j] = sum;
+            }
+        };
+        // every group's flag, published by the barrier that follows; true
+        // while any group of the item still iterates
+        auto any_going = [&](bool go) {
+            if (lt == 0) go_flag[g] = go ? 1 : 0;
+            __syncthreads();
+            int32_t any = 0;
+            for (int h = 0; h < G; ++h) any |= go_flag[h];
+            return any != 0;
+        };
+        bb = group_sum1_lds(bb, red(), shift);
+        const T rhs_norm = sqrt(bb);
+        const T limit = relative ? tol * rhs_norm : tol;
+        int32_t iter = 0;
+        T res = 0;
+        bool go = valid;
+        if constexpr (Method == kBatchCg) {
+            T z[R], p[R], ap[R];
+            spmv(in0, ap);
+            T rz = 0;
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                r[j] -= ap[j];
+                p[j] = T(0);
+                z[j] = prec(j) * r[j];
+                rz += r[j] * z[j];
+            }
+            T rho_old = 1;
+            while (true) {
+                // also orders the reads of x in in0 before p overwrites it
+                const T rho = group_sum1_lds(go ? rz : T(0), red(), shift);
+                if (go) {
+                    ++iter;
+                    res = sqrt(rho < T(0) ? -rho : rho);
+                    if (iter >= max_iters || res <= limit) go = false;
+                }
+                if (go) {
+                    const bool restart = rho_old == T(0);
+                    const T beta = restart ? T(0) : rho / rho_old;
+#pragma unroll
+                    for (int j = 0; j < R; ++j) {
+                        const int row = lt + j * L;
+                        p[j] = restart ? z[j] : z[j] + beta * p[j];
+                        if (row < n) in0[row] = p[j];
+                    }
+                }
+                if (!any_going(go)) break;
+                T pap = 0;
+                if (go) {
+                    spmv(in0, ap);
+#pragma unroll
+                    for (int j = 0; j < R; ++j) pap += p[j] * ap[j];
+                }
+                pap = group_sum1_lds(pap, red(), shift);
+                if (go) {
+                    const T temp = rho / pap;
+                    rz = 0;
+#pragma unroll
+                    for (int j = 0; j < R; ++j) {
+                        xr(j) += temp * p[j];
+                        r[j] -= temp * ap[j];
+                        z[j] = prec(j) * r[j];
+                        rz += r[j] * z[j];
+                    }
+                    rho_old = rho;
+                }
+            }
+        } else {
+            T s[R], t[R];
+            auto rh = [&](int j) -> T& { return own_row(lrh, rh_r, j); };
+            auto p = [&](int j) -> T& { return own_row(lp, p_r, j); };
+            auto v = [&](int j) -> T& { return own_row(lv, v_r, j); };
+            spmv(in0, t);
+            T rr = 0;
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                r[j] -= t[j];
+                rh(j) = r[j];
+                p(j) = T(0);
+                v(j) = T(0);
+                s[j] = T(0);
+                rr += r[j] * r[j];
+            }
+            // also orders the reads of x in in0 before p_hat overwrites it
+            rr = group_sum1_lds(rr, red(), shift);
+            T rho_new = rr;
+            res = sqrt(rr);
+            T rho_old = 1, omega = 1, alpha = 1;
+            while (true) {
+                if (go && (iter >= max_iters || res <= limit)) go = false;
+                if (go) {
+                    const T beta = (rho_new / rho_old) * (alpha / omega);
+#pragma unroll
+                    for (int j = 0; j < R; ++j) {
+                        const int row = lt + j * L;
+                        p(j) = r[j] + beta * (p(j) - omega * v(j));
+                        if (row < n) in0[row] = prec(j) * p(j);
+                    }
+                }
+                if (!any_going(go)) break;
+                T rv = 0;
+                if (go) {
+                    spmv(in0, t);
+#pragma unroll
+                    for (int j = 0; j < R; ++j) {
+                        v(j) = t[j];
+                        rv += rh(j) * t[j];
+                    }
+                }
+                rv = group_sum1_lds(rv, red(), shift);
+                T ss = 0;
+                if (go) {
+                    alpha = rho_new / rv;
+#pragma unroll
+                    for (int j = 0; j < R; ++j) {
+                        const int row = lt + j * L;
+                        s[j] = r[j] - alpha * v(j);
+                        if (row < n) in1[row] = prec(j) * s[j];
+                        ss += s[j] * s[j];
+                    }
+                }
+                // publishes s_hat for the product below as well
+                ss = group_sum1_lds(ss, red(), shift);
+                if (go) {
+                    res = sqrt(ss);
+                    if (res <= limit) {
+                        // the half-step exit: x += alpha p_hat, the
+                        // iteration not counted
+#pragma unroll
+                        for (int j = 0; j < R; ++j) xr(j) += alpha * (prec(j) * p(j));
+                        go = false;
+                    }
+                }
+                T ts = 0, tt = 0;
+                if (go) {
+                    spmv(in1, t);
+#pragma unroll
+                    for (int j = 0; j < R; ++j) {
+                        ts += t[j] * s[j];
+                        tt += t[j] * t[j];
+                    }
+                }
+                group_sum2_lds(ts, tt, red(), shift);
+                T r2 = 0, rhr = 0;
+                if (go) {
+                    omega = ts / tt;
+#pragma unroll
+                    for (int j = 0; j < R; ++j) {
+                        const T pj = prec(j);
+                        xr(j) += fma(alpha, pj * p(j), omega * (pj * s[j]));
+                        r[j] = s[j] - omega * t[j];
+                        r2 += r[j] * r[j];
+                        rhr += rh(j) * r[j];
+                    }
+                }
+                group_sum2_lds(r2, rhr, red(), shift);
+                if (go) {
+                    res = sqrt(r2);
+                    rho_old = rho_new;
+                    rho_new = rhr;
+                    ++iter;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int row = lt + j * L;
+            if (row < n) gx[row] = xr(j);
+        }
+        if (valid && lt == 0) {
+            iters_out[sys] = iter;
+            res_out[sys] = res;
+        }
+    }
+    // the last workgroup out resets the queue for the next launch
+    if (dynamic && tid == 0) {
+        const unsigned d = atomicAdd(queue + 1, 1u);
+        if (d == gridDim.x - 1) {
+            queue[0] = 0;
+            queue[1] = 0;
+        }
+    }
+}
+
 // ------------------------------------------------------------ launcher ----
 
 constexpr int kBatchMaxBins = GKOMI_BATCH_MAX_BINS;
@@ -1143,6 +1571,71 @@ int launch_batch_fmt(int fmt, int nt, hipStream_t s, int grid, size_t lds, int64
     }
 }
 
+// The one-launch form (batch_packed_kernel) applies to an uneven Csr batch of
+// systems of at most 512 rows whose plan knows every bin's longest row and no
+// row is longer than 16 entries; *kreg gets the register entries per row.
+bool packed_applies(const gkomi_batch_csr* a, const gkomi_batch_plan* plan, int* kreg)
+{
+    if (fmt_of(a) != kFmtUneven || a->max_rows > 512) return false;
+    if (plan->order == nullptr && plan->num_bins > 1) return false;
+    int64_t longest = 0;
+    for (int q = 0; q < plan->num_bins; ++q) {
+        if (plan->bin_begin[q + 1] == plan->bin_begin[q]) continue;
+        if (plan->bin_max_row_nnz[q] <= 0 || plan->bin_max_rows[q] > 512) return false;
+        longest = std::max(longest, plan->bin_max_row_nnz[q]);
+    }
+    if (longest > 16) return false;
+    *kreg = longest <= 12 ? 12 : 16;
+    return true;
+}
+
+template <typename T, int K, int Method>
+int launch_packed_k(hipStream_t s, const gkomi_batch_csr* a, const gkomi_batch_settings* st,
+                    const gkomi_batch_plan* plan, const T* b, T* x, int32_t* iters, T* res,
+                    unsigned* queue)
+{
+    batch_items it{};
+    it.num_bins = plan->num_bins;
+    int64_t total = 0;
+    for (int q = 0; q < plan->num_bins; ++q) {
+        const int64_t cnt = plan->bin_begin[q + 1] - plan->bin_begin[q];
+        const int64_t half = (plan->bin_max_rows[q] + 1) / 2;
+        int shift = 5;
+        while ((int64_t{1} << shift) < half) ++shift;
+        GKOMI_REQUIRE(shift <= 8, GKOMI_INVALID_ARGUMENT,
+                      "batch (one launch): a bin's systems exceed 512 rows");
+        it.shift[q] = shift;
+        it.item_begin[q] = total;
+        it.sys_begin[q] = plan->bin_begin[q];
+        const int64_t per_item = 256 >> shift;
+        total += (cnt + per_item - 1) / per_item;
+    }
+    it.item_begin[plan->num_bins] = total;
+    it.sys_begin[plan->num_bins] = plan->bin_begin[plan->num_bins];
+    if (total == 0) return GKOMI_SUCCESS;
+    auto kern = batch_packed_kernel<T, K, Method>;
+    const int64_t resident =
+        static_cast<int64_t>(num_cus()) * resident_per_cu(reinterpret_cast<const void*>(kern), 256, 0);
+    const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(total, resident)));
+    const int32_t dynamic = total > grid ? 1 : 0;
+    kern<<<grid, 256, 0, s>>>(total, it, plan->order, a->row_offsets, a->row_ptrs, a->col_idxs,
+                              static_cast<const T*>(a->values), b, x, st->max_iterations,
+                              st->tolerance_type, st->tolerance, st->preconditioner, iters, res,
+                              queue, dynamic);
+    GKOMI_CHECK_LAUNCH("batch_packed_kernel");
+    return GKOMI_SUCCESS;
+}
+
+template <typename T, int Method>
+int launch_packed(hipStream_t s, int kreg, const gkomi_batch_csr* a,
+                  const gkomi_batch_settings* st, const gkomi_batch_plan* plan, const T* b, T* x,
+                  int32_t* iters, T* res, unsigned* queue)
+{
+    return kreg == 12
+        ? launch_packed_k<T, 12, Method>(s, a, st, plan, b, x, iters, res, queue)
+        : launch_packed_k<T, 16, Method>(s, a, st, plan, b, x, iters, res, queue);
+}
+
 template <typename T, int Method>
 int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
                      const gkomi_batch_settings* st, const gkomi_batch_plan* plan, const T* b,
@@ -1174,6 +1667,15 @@ int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
     const int lds_limit = device_lds_limit();
     hipStream_t s = stream_of(ctx);
     unsigned* queues = static_cast<unsigned*>(ws);
+    GKOMI_REQUIRE(plan->single_launch == 0 || plan->single_launch == 1, GKOMI_INVALID_ARGUMENT,
+                  "batch: single_launch must be 0 or 1");
+    if (plan->single_launch == 1) {
+        int kreg = 0;
+        GKOMI_REQUIRE(packed_applies(a, plan, &kreg), GKOMI_NOT_SUPPORTED,
+                      "batch: one launch needs an uneven Csr batch of systems of at most 512 "
+                      "rows, an order, and every bin's longest row known and at most 16 entries");
+        return launch_packed<T, Method>(s, kreg, a, st, plan, b, x, iters, res, queues);
+    }
     for (int bin = 0; bin < plan->num_bins; ++bin) {
         const int64_t beg = plan->bin_begin[bin];
         const int64_t cnt = plan->bin_begin[bin + 1] - beg;
@@ -1425,6 +1927,7 @@ int batch_tune_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
     for (int bin = 0; bin < plan->num_bins && rc == GKOMI_SUCCESS; ++bin) {
         gkomi_batch_plan one = *plan;
         one.num_bins = 1;
+        one.single_launch = 0;
         one.order = plan->order ? plan->order + plan->bin_begin[bin] : nullptr;
         const int64_t cnt = plan->bin_begin[bin + 1] - plan->bin_begin[bin];
         if (cnt <= 0) continue;
@@ -1487,6 +1990,31 @@ int batch_tune_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
         plan->block_size[bin] = best_nt;
         plan->matrix_in_lds[bin] = best_mat;
         plan->vectors[bin] = best_vec;
+    }
+    // the tuned per-bin launches against one launch over the whole batch
+    int kreg = 0;
+    plan->single_launch = 0;
+    if (rc == GKOMI_SUCCESS && packed_applies(a, plan, &kreg)) {
+        float best = 1e30f;
+        int best_single = 0;
+        for (int single : {0, 1}) {
+            gkomi_batch_plan trial = *plan;
+            trial.single_launch = single;
+            GKOMI_CHECK_HIP(hipMemcpyAsync(xs, x, sizeof(T) * rows_all,
+                                           hipMemcpyDeviceToDevice, s));
+            GKOMI_CHECK_HIP(hipEventRecord(e0, s));
+            rc = batch_solve_impl<T, Method>(ctx, a, st, &trial, b, xs, iters, res, ws, wsb);
+            if (rc) break;
+            GKOMI_CHECK_HIP(hipEventRecord(e1, s));
+            GKOMI_CHECK_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            GKOMI_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                best_single = single;
+            }
+        }
+        plan->single_launch = best_single;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -852,7 +852,14 @@ typedef struct {
  * (hip/solver/batch_bicgstab_kernels.hip.cpp). */
 typedef struct {
     int32_t num_bins;
-    int32_t reserved;
+    /* 1: the whole batch in one persistent launch (uneven Csr, systems of at
+     * most 512 rows, every bin's longest row known and at most 16 entries):
+     * one queue of work items ordered largest first, each item one system of
+     * more than 256 rows or up to eight smaller ones side by side in one
+     * 256-thread workgroup; block_size, matrix_in_lds, blocks_per_cu, subwave
+     * and vectors are then unused.  0: one launch per bin.  The autotuner
+     * sets it. */
+    int32_t single_launch;
     const int32_t* order;
     int64_t bin_begin[GKOMI_BATCH_MAX_BINS + 1];
     int64_t bin_max_rows[GKOMI_BATCH_MAX_BINS];
