@@ -18,6 +18,7 @@
 //  * csr_spmm_rowwave: one wave per row, lanes over right-hand sides (the
 //    many-RHS case).
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 
 #include "common.hpp"
@@ -43,6 +44,97 @@ __device__ __forceinline__ void csr_subwave_body(int64_t n_rows, const int32_t* 
             n_rows, t, rp, ci, co, v,
             [&](int32_t col) { return Strided ? b[static_cast<int64_t>(col) * bs] : b[col]; },
             [&](int64_t row, T sum) { store_result<Mode>(c + row * cs, sum, alpha, beta); });
+    }
+}
+
+// One lane per row with staged, coalesced matrix loads (rows of nearly
+// equal, short length: stencils).  Wave w takes the window of rows
+// [64 w, 64 w + 64); the window's entries are copied into the wave's LDS
+// staging area of C entries in runs of 64 consecutive entries (every load
+// one contiguous run of values and of column indices -- or, with Compact,
+// of the 2-byte offsets column - row -- all issued before the first LDS
+// write), then lane l sums row 64 w + l from LDS in stored order by fused
+// multiply-add, gathering b: consecutive lanes are consecutive rows, so a
+// stencil's gathers read consecutive words.  A window of more than C entries
+// is staged in chunks; a row that crosses a chunk boundary continues its sum
+// in the next chunk, so every row is the sequential stored-order sum (the
+// order of the one-lane subwave shape, bit for bit).  Only the wave itself
+// reads its staging area: LDS accesses of one wave complete in order, and
+// the wave barriers keep the compiler from moving them across the staging.
+template <typename T, int C, int Mode, bool Compact>
+__global__ void __launch_bounds__(kBlock)
+    csr_spmv_lanerow(int64_t n_rows, const int32_t* __restrict__ rp,
+                     const int32_t* __restrict__ ci, const int16_t* __restrict__ co,
+                     const T* __restrict__ v, const T* __restrict__ b, T* __restrict__ c,
+                     int64_t cs, const T* __restrict__ alpha, const T* __restrict__ beta)
+{
+    using Idx = std::conditional_t<Compact, int16_t, int32_t>;
+    constexpr int kSteps = C / kWave;
+    constexpr int U = 4;
+    __shared__ T sv[kWavesPerBlock][C];
+    __shared__ Idx sc[kWavesPerBlock][C];
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    T* lv = sv[wave];
+    Idx* lc = sc[wave];
+    const int64_t windows = ceildiv(n_rows, kWave);
+    const int64_t nw = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+    for (int64_t w = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kWavesPerBlock +
+                     wave;
+         w < windows; w += nw) {
+        const int64_t row = w * kWave + lane;
+        const int32_t rs = rp[row < n_rows ? row : n_rows];
+        const int32_t re = rp[row < n_rows ? row + 1 : n_rows];
+        const int32_t ws = __shfl(rs, 0, kWave);
+        const int32_t we = __shfl(re, kWave - 1, kWave);
+        T sum = 0;
+        for (int32_t c0 = ws; c0 < we; c0 += C) {
+            const int32_t c1 = c0 + C < we ? c0 + C : we;
+            T tv[kSteps];
+            Idx tc[kSteps];
+#pragma unroll
+            for (int u = 0; u < kSteps; ++u) {
+                const int32_t e = c0 + u * kWave + lane;
+                const int32_t ee = e < c1 ? e : c0;  // c0 < we <= nnz: a valid entry
+                tv[u] = v[ee];
+                if constexpr (Compact) {
+                    tc[u] = co[ee];
+                } else {
+                    tc[u] = ci[ee];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int u = 0; u < kSteps; ++u) {
+                lv[u * kWave + lane] = tv[u];
+                lc[u * kWave + lane] = tc[u];
+            }
+            __builtin_amdgcn_wave_barrier();
+            const int32_t a = (rs > c0 ? rs : c0) - c0;
+            const int32_t z = (re < c1 ? re : c1) - c0;
+            for (int32_t k = a; __ballot(k < z) != 0; k += U) {
+                int32_t col[U];
+                T val[U];
+                bool ok[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    ok[u] = k + u < z;
+                    const int32_t idx = ok[u] ? k + u : 0;
+                    col[u] = Compact ? static_cast<int32_t>(row) + lc[idx] : lc[idx];
+                    val[u] = lv[idx];
+                }
+                T xv[U];
+                // a slot past the row's end gathers column 0 (always valid)
+#pragma unroll
+                for (int u = 0; u < U; ++u) xv[u] = b[ok[u] ? col[u] : 0];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    sum = fma(ok[u] ? val[u] : T(0), ok[u] ? xv[u] : T(0), sum);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (row < n_rows) store_result<Mode>(c + row * cs, sum, alpha, beta);
     }
 }
 
@@ -594,6 +686,18 @@ int check_csr(const gkomi_csr* a)
     return GKOMI_SUCCESS;
 }
 
+// The staged one-lane-per-row kernel: asked for (GKOMI_CSR_LANE_ROWS), or
+// automatic (subwave 0) when a window of 64 rows holds on average at most
+// one staging chunk (mean row length <= kLanerowStage / 64 = 8 entries).
+// Measured against the best subwave shape: 128^3 7-point FP64 30.7 against
+// 37.8 us, 1448^2 5-point FP64 26.1 against 30.9 us; 27-point rows (four
+// chunks per window) stay on the subwave kernel (94.8 against 70.6 us).
+bool use_lanerow(const gkomi_csr* a)
+{
+    if (a->subwave == GKOMI_CSR_LANE_ROWS) return true;
+    return a->subwave == 0 && a->n_rows > 0 && a->nnz <= (kLanerowStage / kWave) * a->n_rows;
+}
+
 subwave_shape pick_subwave(const gkomi_csr* a, int value_bytes)
 {
     subwave_shape sh{a->subwave & 0xff, (a->subwave >> 8) & 0xff};
@@ -734,6 +838,18 @@ int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const 
                 c, cs, alpha, beta);
         }
         GKOMI_CHECK_LAUNCH("csr_spmv_sliced");
+        return GKOMI_SUCCESS;
+    }
+    if (bs == 1 && use_lanerow(a)) {
+        const unsigned g = grid_for(ceildiv(a->n_rows, kWave), kWavesPerBlock, 64);
+        if (a->col_offsets != nullptr) {
+            csr_spmv_lanerow<T, kLanerowStage, Mode, true><<<g, kBlock, 0, stream>>>(
+                a->n_rows, a->row_ptrs, a->col_idxs, a->col_offsets, v, b, c, cs, alpha, beta);
+        } else {
+            csr_spmv_lanerow<T, kLanerowStage, Mode, false><<<g, kBlock, 0, stream>>>(
+                a->n_rows, a->row_ptrs, a->col_idxs, a->col_offsets, v, b, c, cs, alpha, beta);
+        }
+        GKOMI_CHECK_LAUNCH("csr_spmv_lanerow");
         return GKOMI_SUCCESS;
     }
     const subwave_shape sh = pick_subwave(a, sizeof(T));

@@ -342,6 +342,11 @@ struct subwave_shape {
 
 subwave_shape pick_subwave(const gkomi_csr* a, int value_bytes);
 
+// Entries of a wave's LDS staging area in the one-lane-per-row kernel
+// (csr_spmv_lanerow): 64 rows of up to 8 entries in one chunk.
+constexpr int kLanerowStage = 512;
+bool use_lanerow(const gkomi_csr* a);
+
 
 template <typename T, int Mode>
 int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const T* b, int64_t bs,

@@ -171,12 +171,20 @@ enum { GKOMI_VALUE_F64 = 0, GKOMI_VALUE_F32 = 1 };
 /* Csr strategies (include/ginkgo/core/matrix/csr.hpp:176-699).
  * CLASSICAL: a sub-wavefront of S lanes per row, each lane group keeping R
  * rows in flight; `subwave` = S | (R << 8), S a power of two <= 64, R one of
- * 1, 2, 4, 8, either 0 = chosen from the mean row length.
+ * 1, 2, 4, 8, either 0 = chosen from the mean row length; or `subwave` =
+ * GKOMI_CSR_LANE_ROWS (below).
  * LOAD_BALANCE: the reference's load_balance partition `srow` (built by
  * gkomi_csr_lb_partition, or by the reference's own load_balance::process);
  * each workgroup takes consecutive warps of it and owns whole rows.
  * MERGE_PATH: merge-path partition of rows and entries (csr.hpp:283-301). */
 enum { GKOMI_CSR_CLASSICAL = 0, GKOMI_CSR_LOAD_BALANCE = 1, GKOMI_CSR_MERGE_PATH = 2 };
+/* CLASSICAL `subwave` value: one lane per row, each wave staging its 64 rows'
+ * entries in LDS with contiguous loads (every row the sequential sum in
+ * stored order, as S = 1).  `subwave` = 0 takes this kernel for a single
+ * right-hand side with unit stride when the mean row holds at most 8 entries
+ * (stencils); other right-hand-side shapes fall back to the sub-wavefront
+ * kernel. */
+enum { GKOMI_CSR_LANE_ROWS = 0x10000 };
 
 /* Matrix::Csr<V, int32> storage (csr.hpp:104-1690) plus its strategy data.
  * `values` has the value type of the entry point it is passed to. */

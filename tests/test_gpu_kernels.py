@@ -109,6 +109,55 @@ def test_spmv_classical_every_subwave(dev, dtype, subwave):
         np.testing.assert_array_equal(y3, y)
 
 
+LANE_ROWS = 0x10000     # GKOMI_CSR_LANE_ROWS
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("case", ["irregular", "stencil", "tiny"])
+def test_spmv_lane_rows_staged(dev, dtype, case):
+    """The staged one-lane-per-row kernel: windows of 64 rows whose entries
+    exceed the 512-entry staging chunk (rows of 100..3000 entries cross
+    chunks), empty rows, a last partial window; every row is the sequential
+    stored-order sum, so the product equals the one-lane subwave shape bit
+    for bit, with the indices or with the 2-byte offsets; and it is the
+    automatic classical choice for short rows."""
+    if case == "irregular":
+        rp, ci, v = irregular(3000, seed=7)
+    elif case == "stencil":
+        rp, ci, v = gen.laplace3d(21)
+    else:
+        rp, ci, v = irregular(37, seed=2)
+    n = len(rp) - 1
+    v = v.astype(dtype)
+    x = np.random.default_rng(1).uniform(-1, 1, (n, 1)).astype(dtype)
+    xd = to_dev(x, dev)
+    one = D().DeviceCsr(rp, ci, v, strategy="classical", subwave=1 | (1 << 8), device=dev)
+    y1 = one.spmv(xd).cpu().numpy()
+    check_spmv(rp, ci, v, x, y1, dtype)
+    A = D().DeviceCsr(rp, ci, v, strategy="classical", subwave=LANE_ROWS, device=dev)
+    np.testing.assert_array_equal(A.spmv(xd).cpu().numpy(), y1)
+    if A.compact():
+        np.testing.assert_array_equal(A.spmv(xd).cpu().numpy(), y1)
+    if rp[-1] <= 8 * n:                    # mean row <= 8: the automatic choice
+        auto = D().DeviceCsr(rp, ci, v, strategy="classical", device=dev)
+        np.testing.assert_array_equal(auto.spmv(xd).cpu().numpy(), y1)
+    # advanced: y = alpha A x + beta y, beta = 0 ignores NaN in y
+    y0 = np.random.default_rng(3).uniform(-1, 1, (n, 1)).astype(dtype)
+    yd = to_dev(y0, dev)
+    A.advanced_spmv(-2.0, xd, 0.5, yd)
+    ref = ora.csr_advanced_spmv(-2.0, rp, ci, v, x, 0.5, y0.copy()).astype(np.float64)
+    tol = 2 * spmv_bound(rp, ci, v, x, dtype) + 4 * np.finfo(dtype).eps * np.abs(y0)
+    assert np.all(np.abs(yd.cpu().numpy().astype(np.float64) - ref) <= tol)
+    nan = to_dev(np.full((n, 1), np.nan, dtype), dev)
+    A.advanced_spmv(1.0, xd, 0.0, nan)
+    np.testing.assert_array_equal(nan.cpu().numpy(), y1)
+    # a strided right-hand side falls back to the subwave kernel
+    X = np.random.default_rng(4).uniform(-1, 1, (n, 3)).astype(dtype)
+    Xd = to_dev(X, dev)
+    ys = A.spmv(Xd[:, 1:2]).cpu().numpy()
+    check_spmv(rp, ci, v, X[:, 1:2].copy(), ys, dtype)
+
+
 def test_compact_offsets_refuse_wide_rows(dev):
     n = 40000
     rp = np.arange(n + 1, dtype=np.int32)
