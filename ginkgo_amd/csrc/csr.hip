@@ -47,47 +47,53 @@ __device__ __forceinline__ void csr_subwave_body(int64_t n_rows, const int32_t* 
     }
 }
 
-// One lane per row with staged, coalesced matrix loads (rows of nearly
-// equal, short length: stencils).  Wave w takes the window of rows
-// [64 w, 64 w + 64); the window's entries are copied into the wave's LDS
+// G lanes per row with staged, coalesced matrix loads (rows of nearly equal
+// length: stencils).  Wave w takes the window of R = 64 / G consecutive rows
+// [R w, R w + R); the window's entries are copied into the wave's LDS
 // staging area of C entries in runs of 64 consecutive entries (every load
 // one contiguous run of values and of column indices -- or, with Compact,
 // of the 2-byte offsets column - row -- all issued before the first LDS
-// write), then lane l sums row 64 w + l from LDS in stored order by fused
-// multiply-add, gathering b: consecutive lanes are consecutive rows, so a
-// stencil's gathers read consecutive words.  A window of more than C entries
-// is staged in chunks; a row that crosses a chunk boundary continues its sum
-// in the next chunk, so every row is the sequential stored-order sum (the
-// order of the one-lane subwave shape, bit for bit).  Only the wave itself
-// reads its staging area: LDS accesses of one wave complete in order, and
-// the wave barriers keep the compiler from moving them across the staging.
-template <typename T, int C, int Mode, bool Compact>
+// write).  Then lane l of a row's group adds the row's entries l, l + G,
+// l + 2G, ... from LDS by fused multiply-add, gathering b, and the group's
+// partial sums are added by the DPP tree of group_sum<G>: the order of the
+// sub-wave kernel with G lanes per row, bit for bit.  For G = 1 consecutive
+// lanes are consecutive rows, so a stencil's gathers read consecutive words;
+// for larger G each gather instruction covers G offsets of R consecutive
+// rows.  A window of more than C entries is staged in chunks; a lane's
+// entries continue in the next chunk in the same order.  Only the wave
+// itself reads its staging area: LDS accesses of one wave complete in order,
+// and the wave barriers keep the compiler from moving them across the
+// staging.
+template <typename T, int G, int C, int Mode, bool Compact>
 __global__ void __launch_bounds__(kBlock)
-    csr_spmv_lanerow(int64_t n_rows, const int32_t* __restrict__ rp,
-                     const int32_t* __restrict__ ci, const int16_t* __restrict__ co,
-                     const T* __restrict__ v, const T* __restrict__ b, T* __restrict__ c,
-                     int64_t cs, const T* __restrict__ alpha, const T* __restrict__ beta)
+    csr_spmv_staged(int64_t n_rows, const int32_t* __restrict__ rp,
+                    const int32_t* __restrict__ ci, const int16_t* __restrict__ co,
+                    const T* __restrict__ v, const T* __restrict__ b, T* __restrict__ c,
+                    int64_t cs, const T* __restrict__ alpha, const T* __restrict__ beta)
 {
     using Idx = std::conditional_t<Compact, int16_t, int32_t>;
     constexpr int kSteps = C / kWave;
+    constexpr int kRows = kWave / G;
     constexpr int U = 4;
     __shared__ T sv[kWavesPerBlock][C];
     __shared__ Idx sc[kWavesPerBlock][C];
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
+    const int l = lane % G;
     T* lv = sv[wave];
     Idx* lc = sc[wave];
-    const int64_t windows = ceildiv(n_rows, kWave);
+    const int64_t windows = ceildiv(n_rows, kRows);
     const int64_t nw = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
     for (int64_t w = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kWavesPerBlock +
                      wave;
          w < windows; w += nw) {
-        const int64_t row = w * kWave + lane;
+        const int64_t row = w * kRows + lane / G;
         const int32_t rs = rp[row < n_rows ? row : n_rows];
         const int32_t re = rp[row < n_rows ? row + 1 : n_rows];
         const int32_t ws = __shfl(rs, 0, kWave);
         const int32_t we = __shfl(re, kWave - 1, kWave);
         T sum = 0;
+        int32_t k = rs + l;  // the lane's next entry (absolute position)
         for (int32_t c0 = ws; c0 < we; c0 += C) {
             const int32_t c1 = c0 + C < we ? c0 + C : we;
             T tv[kSteps];
@@ -110,16 +116,18 @@ __global__ void __launch_bounds__(kBlock)
                 lc[u * kWave + lane] = tc[u];
             }
             __builtin_amdgcn_wave_barrier();
-            const int32_t a = (rs > c0 ? rs : c0) - c0;
-            const int32_t z = (re < c1 ? re : c1) - c0;
-            for (int32_t k = a; __ballot(k < z) != 0; k += U) {
+            // the lane's entries k, k + G, ... below the row's end and the
+            // chunk's end (k never falls below c0: earlier chunks took them)
+            const int32_t z = re < c1 ? re : c1;
+            while (__ballot(k < z) != 0) {
                 int32_t col[U];
                 T val[U];
                 bool ok[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    ok[u] = k + u < z;
-                    const int32_t idx = ok[u] ? k + u : 0;
+                    const int32_t kk = k + u * G;
+                    ok[u] = kk < z;
+                    const int32_t idx = ok[u] ? kk - c0 : 0;
                     col[u] = Compact ? static_cast<int32_t>(row) + lc[idx] : lc[idx];
                     val[u] = lv[idx];
                 }
@@ -131,10 +139,15 @@ __global__ void __launch_bounds__(kBlock)
                 for (int u = 0; u < U; ++u) {
                     sum = fma(ok[u] ? val[u] : T(0), ok[u] ? xv[u] : T(0), sum);
                 }
+                // advance past the entries taken (at most to the first one
+                // not taken, which may lie in the next chunk)
+#pragma unroll
+                for (int u = 0; u < U; ++u) k = ok[u] ? k + G : k;
             }
             __builtin_amdgcn_wave_barrier();
         }
-        if (row < n_rows) store_result<Mode>(c + row * cs, sum, alpha, beta);
+        sum = group_sum<G>(sum);
+        if (row < n_rows && l == 0) store_result<Mode>(c + row * cs, sum, alpha, beta);
     }
 }
 
@@ -686,16 +699,28 @@ int check_csr(const gkomi_csr* a)
     return GKOMI_SUCCESS;
 }
 
-// The staged one-lane-per-row kernel: asked for (GKOMI_CSR_LANE_ROWS), or
-// automatic (subwave 0) when a window of 64 rows holds on average at most
-// one staging chunk (mean row length <= kLanerowStage / 64 = 8 entries).
-// Measured against the best subwave shape: 128^3 7-point FP64 30.7 against
-// 37.8 us, 1448^2 5-point FP64 26.1 against 30.9 us; 27-point rows (four
-// chunks per window) stay on the subwave kernel (94.8 against 70.6 us).
-bool use_lanerow(const gkomi_csr* a)
+// Lanes per row of the staged kernel (csr_spmv_staged), 0 = the sub-wave
+// kernel, -1 = an invalid request: GKOMI_CSR_STAGED | G asks for it;
+// automatic (subwave 0) takes the smallest power of two G with
+// (64 / G) * mean row <= kStagedChunk (a window of rows holds on average at
+// most one staging chunk), if G <= 4, or G = 8 in FP32.  Measured against the
+// sub-wave kernel's automatic shape (us per product): 128^3 7-point FP64 G = 1
+// 30.7 / 37.7, FP32 22.3 / 22.8; 1448^2 5-point FP64 25.8 / 31.0; 27-point
+// 96^3 FP64 G = 4 61.5 / 70.0, FP32 31.1 / 45.3; a box stencil of 34.6
+// entries per row (505,521 rows) G = 8 FP64 43.2 / 38.7 (slower: not taken),
+// FP32 29.5 / 32.1.
+int staged_lanes(const gkomi_csr* a, int value_bytes)
 {
-    if (a->subwave == GKOMI_CSR_LANE_ROWS) return true;
-    return a->subwave == 0 && a->n_rows > 0 && a->nnz <= (kLanerowStage / kWave) * a->n_rows;
+    if ((a->subwave & ~0xff) == GKOMI_CSR_STAGED) {
+        const int G = a->subwave & 0xff;
+        return (G == 1 || G == 2 || G == 4 || G == 8 || G == 16) ? G : -1;
+    }
+    if (a->subwave != 0 || a->n_rows <= 0) return 0;
+    const int limit = value_bytes >= 8 ? 4 : 8;
+    for (int G = 1; G <= limit; G *= 2) {
+        if ((kWave / G) * a->nnz <= int64_t{kStagedChunk} * a->n_rows) return G;
+    }
+    return 0;
 }
 
 subwave_shape pick_subwave(const gkomi_csr* a, int value_bytes)
@@ -840,17 +865,34 @@ int csr_spmv_launch(hipStream_t stream, const gkomi_csr* a, int64_t nrhs, const 
         GKOMI_CHECK_LAUNCH("csr_spmv_sliced");
         return GKOMI_SUCCESS;
     }
-    if (bs == 1 && use_lanerow(a)) {
-        const unsigned g = grid_for(ceildiv(a->n_rows, kWave), kWavesPerBlock, 64);
-        if (a->col_offsets != nullptr) {
-            csr_spmv_lanerow<T, kLanerowStage, Mode, true><<<g, kBlock, 0, stream>>>(
-                a->n_rows, a->row_ptrs, a->col_idxs, a->col_offsets, v, b, c, cs, alpha, beta);
-        } else {
-            csr_spmv_lanerow<T, kLanerowStage, Mode, false><<<g, kBlock, 0, stream>>>(
-                a->n_rows, a->row_ptrs, a->col_idxs, a->col_offsets, v, b, c, cs, alpha, beta);
+    if (bs == 1) {
+        const int G = staged_lanes(a, sizeof(T));
+        GKOMI_REQUIRE(G >= 0, GKOMI_INVALID_ARGUMENT,
+                      "CSR: GKOMI_CSR_STAGED needs 1, 2, 4, 8 or 16 lanes per row");
+        if (G > 0) {
+            const unsigned g =
+                grid_for(ceildiv(a->n_rows, int64_t{kWave} / G), kWavesPerBlock, 64);
+#define GKOMI_STAGED_CASE(GV)                                                                \
+    if (G == GV) {                                                                          \
+        if (a->col_offsets != nullptr) {                                                    \
+            csr_spmv_staged<T, GV, kStagedChunk, Mode, true><<<g, kBlock, 0, stream>>>(      \
+                a->n_rows, a->row_ptrs, a->col_idxs, a->col_offsets, v, b, c, cs, alpha,     \
+                beta);                                                                      \
+        } else {                                                                            \
+            csr_spmv_staged<T, GV, kStagedChunk, Mode, false><<<g, kBlock, 0, stream>>>(     \
+                a->n_rows, a->row_ptrs, a->col_idxs, a->col_offsets, v, b, c, cs, alpha,     \
+                beta);                                                                      \
+        }                                                                                   \
+        GKOMI_CHECK_LAUNCH("csr_spmv_staged");                                              \
+        return GKOMI_SUCCESS;                                                               \
+    }
+            GKOMI_STAGED_CASE(1)
+            GKOMI_STAGED_CASE(2)
+            GKOMI_STAGED_CASE(4)
+            GKOMI_STAGED_CASE(8)
+            GKOMI_STAGED_CASE(16)
+#undef GKOMI_STAGED_CASE
         }
-        GKOMI_CHECK_LAUNCH("csr_spmv_lanerow");
-        return GKOMI_SUCCESS;
     }
     const subwave_shape sh = pick_subwave(a, sizeof(T));
     const int64_t tile_rows = int64_t{kBlock} / sh.sw * sh.rpg;

@@ -342,10 +342,10 @@ struct subwave_shape {
 
 subwave_shape pick_subwave(const gkomi_csr* a, int value_bytes);
 
-// Entries of a wave's LDS staging area in the one-lane-per-row kernel
-// (csr_spmv_lanerow): 64 rows of up to 8 entries in one chunk.
-constexpr int kLanerowStage = 512;
-bool use_lanerow(const gkomi_csr* a);
+// Entries of a wave's LDS staging area in the staged kernel
+// (csr_spmv_staged): 64 / G rows of up to 8 G entries in one chunk.
+constexpr int kStagedChunk = 512;
+int staged_lanes(const gkomi_csr* a, int value_bytes);
 
 
 template <typename T, int Mode>

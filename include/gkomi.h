@@ -178,13 +178,15 @@ enum { GKOMI_VALUE_F64 = 0, GKOMI_VALUE_F32 = 1 };
  * each workgroup takes consecutive warps of it and owns whole rows.
  * MERGE_PATH: merge-path partition of rows and entries (csr.hpp:283-301). */
 enum { GKOMI_CSR_CLASSICAL = 0, GKOMI_CSR_LOAD_BALANCE = 1, GKOMI_CSR_MERGE_PATH = 2 };
-/* CLASSICAL `subwave` value: one lane per row, each wave staging its 64 rows'
- * entries in LDS with contiguous loads (every row the sequential sum in
- * stored order, as S = 1).  `subwave` = 0 takes this kernel for a single
- * right-hand side with unit stride when the mean row holds at most 8 entries
- * (stencils); other right-hand-side shapes fall back to the sub-wavefront
- * kernel. */
-enum { GKOMI_CSR_LANE_ROWS = 0x10000 };
+/* CLASSICAL `subwave` = GKOMI_CSR_STAGED | G (G = 1, 2, 4, 8 or 16): G lanes
+ * per row, each wave staging its 64 / G rows' entries in LDS with contiguous
+ * loads (the per-row summation order of S = G, bit for bit).  `subwave` = 0
+ * takes it for a single right-hand side with unit stride with the smallest G
+ * for which 64 / G mean rows hold at most 512 entries, if G <= 4 (FP64) or
+ * G <= 8 (FP32): rows of up to 32 / 64 entries on average.  Other
+ * right-hand-side shapes use the sub-wavefront kernel; a G outside the list
+ * is GKOMI_INVALID_ARGUMENT. */
+enum { GKOMI_CSR_STAGED = 0x10000 };
 
 /* Matrix::Csr<V, int32> storage (csr.hpp:104-1690) plus its strategy data.
  * `values` has the value type of the entry point it is passed to. */

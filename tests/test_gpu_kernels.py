@@ -109,39 +109,55 @@ def test_spmv_classical_every_subwave(dev, dtype, subwave):
         np.testing.assert_array_equal(y3, y)
 
 
-LANE_ROWS = 0x10000     # GKOMI_CSR_LANE_ROWS
+STAGED = 0x10000     # GKOMI_CSR_STAGED
+
+
+def auto_staged_lanes(rp, dtype):
+    n, nnz = len(rp) - 1, int(rp[-1])
+    for g in ((1, 2, 4) if dtype == np.float64 else (1, 2, 4, 8)):
+        if (64 // g) * nnz <= 512 * n:
+            return g
+    return 0
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-@pytest.mark.parametrize("case", ["irregular", "stencil", "tiny"])
-def test_spmv_lane_rows_staged(dev, dtype, case):
-    """The staged one-lane-per-row kernel: windows of 64 rows whose entries
-    exceed the 512-entry staging chunk (rows of 100..3000 entries cross
-    chunks), empty rows, a last partial window; every row is the sequential
-    stored-order sum, so the product equals the one-lane subwave shape bit
-    for bit, with the indices or with the 2-byte offsets; and it is the
-    automatic classical choice for short rows."""
+@pytest.mark.parametrize("case", ["irregular", "stencil7", "stencil27", "tiny"])
+def test_spmv_staged_every_group(dev, dtype, case):
+    """The staged kernel (windows of 64 / G rows, their entries copied to LDS
+    with contiguous loads, G lanes per row): windows whose entries exceed the
+    512-entry staging chunk (rows of 100..3000 entries cross chunks), empty
+    rows, a last partial window.  Lane l of a row adds entries l, l + G, ...
+    and the group tree adds the lanes, so the product equals the sub-wave
+    shape with G lanes per row bit for bit, with the indices or the 2-byte
+    offsets; the automatic classical choice for short rows is that kernel."""
     if case == "irregular":
         rp, ci, v = irregular(3000, seed=7)
-    elif case == "stencil":
+    elif case == "stencil7":
         rp, ci, v = gen.laplace3d(21)
+    elif case == "stencil27":
+        rp, ci, v = gen.diffusion27(14)
     else:
         rp, ci, v = irregular(37, seed=2)
     n = len(rp) - 1
     v = v.astype(dtype)
     x = np.random.default_rng(1).uniform(-1, 1, (n, 1)).astype(dtype)
     xd = to_dev(x, dev)
-    one = D().DeviceCsr(rp, ci, v, strategy="classical", subwave=1 | (1 << 8), device=dev)
-    y1 = one.spmv(xd).cpu().numpy()
-    check_spmv(rp, ci, v, x, y1, dtype)
-    A = D().DeviceCsr(rp, ci, v, strategy="classical", subwave=LANE_ROWS, device=dev)
-    np.testing.assert_array_equal(A.spmv(xd).cpu().numpy(), y1)
-    if A.compact():
-        np.testing.assert_array_equal(A.spmv(xd).cpu().numpy(), y1)
-    if rp[-1] <= 8 * n:                    # mean row <= 8: the automatic choice
+    outs = {}
+    for g in (1, 2, 4, 8, 16):
+        sub = D().DeviceCsr(rp, ci, v, strategy="classical", subwave=g | (1 << 8), device=dev)
+        y = sub.spmv(xd).cpu().numpy()
+        check_spmv(rp, ci, v, x, y, dtype)
+        A = D().DeviceCsr(rp, ci, v, strategy="classical", subwave=STAGED | g, device=dev)
+        np.testing.assert_array_equal(A.spmv(xd).cpu().numpy(), y)
+        if A.compact():
+            np.testing.assert_array_equal(A.spmv(xd).cpu().numpy(), y)
+        outs[g] = y
+    g = auto_staged_lanes(rp, dtype)
+    if g:
         auto = D().DeviceCsr(rp, ci, v, strategy="classical", device=dev)
-        np.testing.assert_array_equal(auto.spmv(xd).cpu().numpy(), y1)
-    # advanced: y = alpha A x + beta y, beta = 0 ignores NaN in y
+        np.testing.assert_array_equal(auto.spmv(xd).cpu().numpy(), outs[g])
+    # advanced: y = alpha A x + beta y; beta = 0 ignores NaN in y
+    A = D().DeviceCsr(rp, ci, v, strategy="classical", subwave=STAGED | 4, device=dev)
     y0 = np.random.default_rng(3).uniform(-1, 1, (n, 1)).astype(dtype)
     yd = to_dev(y0, dev)
     A.advanced_spmv(-2.0, xd, 0.5, yd)
@@ -150,8 +166,12 @@ def test_spmv_lane_rows_staged(dev, dtype, case):
     assert np.all(np.abs(yd.cpu().numpy().astype(np.float64) - ref) <= tol)
     nan = to_dev(np.full((n, 1), np.nan, dtype), dev)
     A.advanced_spmv(1.0, xd, 0.0, nan)
-    np.testing.assert_array_equal(nan.cpu().numpy(), y1)
-    # a strided right-hand side falls back to the subwave kernel
+    np.testing.assert_array_equal(nan.cpu().numpy(), outs[4])
+    # a lane count outside 1..16 is refused
+    from ginkgo_amd import _capi as C
+    with pytest.raises(C.InvalidArgument):
+        D().DeviceCsr(rp, ci, v, strategy="classical", subwave=STAGED | 3, device=dev).spmv(xd)
+    # a strided right-hand side takes the sub-wave kernel
     X = np.random.default_rng(4).uniform(-1, 1, (n, 3)).astype(dtype)
     Xd = to_dev(X, dev)
     ys = A.spmv(Xd[:, 1:2]).cpu().numpy()
