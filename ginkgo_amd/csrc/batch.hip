@@ -545,12 +545,23 @@ __global__ void __launch_bounds__(NT)
     const int tid = threadIdx.x;
     const int64_t L = lds_rows;
     constexpr int kIn = Method == kBatchCg ? 1 : 2;
+    // K > 0: each gathered vector has one more slot, index L, that stays
+    // zero; a register slot past its row's end points there with value 0,
+    // so the product is one fused multiply-add per slot, without a compare
+    // or a select (0 * 0 adds nothing)
+    constexpr int kPad = K > 0 ? 1 : 0;
     T* in0 = reinterpret_cast<T*>(smem);
-    T* in1 = in0 + L;
+    T* in1 = in0 + L + kPad;
     T* lv = in0 + kIn * L;
     int32_t* lrp = reinterpret_cast<int32_t*>(lv + lds_nnz);
     uint16_t* lci = reinterpret_cast<uint16_t*>(lrp + L + 1);
     const T tol = static_cast<T>(tol_d);
+    if constexpr (K > 0) {
+        if (tid == 0) {
+            in0[L] = T(0);
+            in1[L] = T(0);
+        }
+    }
 
     // the first system of a workgroup is its block index; further ones come
     // from the queue (numbered after the grid) only when the launch has more
@@ -616,7 +627,8 @@ __global__ void __launch_bounds__(NT)
                 for (int q = 0; q < K; ++q) {
                     const bool in_row = q < ml[j];
                     mv[j][q] = in_row ? gv[mb[j] + q] : T(0);
-                    const uint32_t c = in_row ? static_cast<uint32_t>(gci[mb[j] + q]) : 0u;
+                    const uint32_t c = in_row ? static_cast<uint32_t>(gci[mb[j] + q])
+                                              : static_cast<uint32_t>(L);
                     mc[j][q / 2] |= c << (16 * (q & 1));
                 }
             }
@@ -690,16 +702,11 @@ __global__ void __launch_bounds__(NT)
                         uint32_t w = mc[j][qq];
                         asm volatile("" : "+v"(w));
                         const int q0 = 2 * qq;
-                        // branch-free: a slot past the row's end holds column
-                        // 0 (a valid word of `in`) and its product is not
-                        // added (a select, so the row sum is the stored-order
-                        // sum of the row's own entries)
-                        const T p0 = mv[j][q0] * in[w & 0xFFFFu];
-                        sum = q0 < ml[j] ? sum + p0 : sum;
-                        if (q0 + 1 < K) {
-                            const T p1 = mv[j][q0 + 1] * in[w >> 16];
-                            sum = q0 + 1 < ml[j] ? sum + p1 : sum;
-                        }
+                        // a slot past the row's end reads the zero slot with
+                        // value 0: the row sum is the stored-order sum of the
+                        // row's own entries
+                        sum = fma(mv[j][q0], in[w & 0xFFFFu], sum);
+                        if (q0 + 1 < K) sum = fma(mv[j][q0 + 1], in[w >> 16], sum);
                     }
                     for (int32_t e = mb[j] + K; e < mb[j] + ml[j]; ++e) sum += gv[e] * in[gci[e]];
                 } else if (row < n) {
@@ -953,7 +960,10 @@ __global__ void __launch_bounds__(256)
     // registers (measured faster at its three waves per SIMD).
     constexpr bool kLdsVec = sizeof(T) == 4;
     constexpr int kVec = kIn + (kLdsVec ? (Method == kBatchCg ? 2 : 5) : 0);
-    __shared__ T in_s[kVec][R * NT];
+    // one more slot per vector, index R * NT, that stays zero: a register slot
+    // past its row's end points there with value 0, so the product is one
+    // fused multiply-add per slot, without a compare or a select
+    __shared__ T in_s[kVec][R * NT + 1];
     __shared__ T red2[2 * 16];
     __shared__ int32_t go_flag[8];
     __shared__ int64_t slot;
@@ -964,6 +974,10 @@ __global__ void __launch_bounds__(256)
     };
     const int tid = threadIdx.x;
     const T tol = static_cast<T>(tol_d);
+    if (tid == 0) {
+        in_s[0][R * NT] = T(0);
+        in_s[kIn - 1][R * NT] = T(0);
+    }
 
     bool first = true;
     while (true) {
@@ -1019,6 +1033,8 @@ __global__ void __launch_bounds__(256)
         T mv[R][K];
         uint32_t mc[R][KC];
         int32_t mb[R], ml[R];
+        // the zero slot, relative to the group's vectors (at most 512)
+        const uint32_t zero_col = static_cast<uint32_t>(R * NT - g * (R * L));
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             const int row = lt + j * L;
@@ -1030,7 +1046,7 @@ __global__ void __launch_bounds__(256)
             for (int q = 0; q < K; ++q) {
                 const bool in_row = q < ml[j];
                 mv[j][q] = in_row ? gv[mb[j] + q] : T(0);
-                const uint32_t c = in_row ? static_cast<uint32_t>(gci[mb[j] + q]) : 0u;
+                const uint32_t c = in_row ? static_cast<uint32_t>(gci[mb[j] + q]) : zero_col;
                 mc[j][q / 2] |= c << (16 * (q & 1));
             }
         }
@@ -1082,12 +1098,8 @@ __global__ void __launch_bounds__(256)
                     uint32_t w = mc[j][qq];
                     asm volatile("" : "+v"(w));
                     const int q0 = 2 * qq;
-                    const T p0 = mv[j][q0] * in[w & 0xFFFFu];
-                    sum = q0 < ml[j] ? sum + p0 : sum;
-                    if (q0 + 1 < K) {
-                        const T p1 = mv[j][q0 + 1] * in[w >> 16];
-                        sum = q0 + 1 < ml[j] ? sum + p1 : sum;
-                    }
+                    sum = fma(mv[j][q0], in[w & 0xFFFFu], sum);
+                    if (q0 + 1 < K) sum = fma(mv[j][q0 + 1], in[w >> 16], sum);
                 }
                 for (int32_t e = mb[j] + K; e < mb[j] + ml[j]; ++e) sum += gv[e] * in[gci[e]];
                 out[j] = sum;
@@ -1496,7 +1508,7 @@ int reg_rows_per_thread(int nt, int64_t rows)
     return 0;
 }
 
-// Entries per row the register kernel keeps in registers (12, 16 or 32;
+// Entries per row the register kernel keeps in registers (10, 12, 16 or 32;
 // 0 = the matrix in LDS) for `nt` threads with `rpt` rows each, a bin whose
 // longest row has `max_row_nnz` entries (0 = unknown) and the plan's
 // matrix_in_lds (1 = LDS; 0 = registers, longer rows finishing from global
@@ -1508,6 +1520,7 @@ int reg_row_entries(int nt, int rpt, int64_t max_row_nnz, int mat)
 {
     if (mat == 1 || !(rpt == 1 || (rpt == 2 && nt >= 256))) return 0;
     const int cap = rpt == 1 ? 32 : 16;
+    if (max_row_nnz > 0 && max_row_nnz <= 10) return 10;
     if (max_row_nnz > 0 && max_row_nnz <= 12) return 12;
     if (max_row_nnz > 0 && max_row_nnz <= 16) return 16;
     if (max_row_nnz > 0 && max_row_nnz <= cap) return cap;
@@ -1535,9 +1548,12 @@ int launch_reg_shape(int nt, int rpt, int kreg, hipStream_t s, int grid, size_t 
     GKOMI_REG_CASE(256, 2, 0)
     GKOMI_REG_CASE(512, 2, 0)
     if constexpr (Fmt == kFmtUneven) {
+        GKOMI_REG_SHAPE(10)
         GKOMI_REG_SHAPE(12)
         GKOMI_REG_SHAPE(16)
         GKOMI_REG_SHAPE(32)
+        GKOMI_REG_CASE(256, 2, 10)
+        GKOMI_REG_CASE(512, 2, 10)
         GKOMI_REG_CASE(256, 2, 12)
         GKOMI_REG_CASE(512, 2, 12)
         GKOMI_REG_CASE(256, 2, 16)
@@ -1585,7 +1601,7 @@ bool packed_applies(const gkomi_batch_csr* a, const gkomi_batch_plan* plan, int*
         longest = std::max(longest, plan->bin_max_row_nnz[q]);
     }
     if (longest > 16) return false;
-    *kreg = longest <= 12 ? 12 : 16;
+    *kreg = longest <= 10 ? 10 : (longest <= 12 ? 12 : 16);
     return true;
 }
 
@@ -1631,9 +1647,14 @@ int launch_packed(hipStream_t s, int kreg, const gkomi_batch_csr* a,
                   const gkomi_batch_settings* st, const gkomi_batch_plan* plan, const T* b, T* x,
                   int32_t* iters, T* res, unsigned* queue)
 {
-    return kreg == 12
-        ? launch_packed_k<T, 12, Method>(s, a, st, plan, b, x, iters, res, queue)
-        : launch_packed_k<T, 16, Method>(s, a, st, plan, b, x, iters, res, queue);
+    switch (kreg) {
+    case 10:
+        return launch_packed_k<T, 10, Method>(s, a, st, plan, b, x, iters, res, queue);
+    case 12:
+        return launch_packed_k<T, 12, Method>(s, a, st, plan, b, x, iters, res, queue);
+    default:
+        return launch_packed_k<T, 16, Method>(s, a, st, plan, b, x, iters, res, queue);
+    }
 }
 
 template <typename T, int Method>
@@ -1702,7 +1723,7 @@ int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
                 ? reg_row_entries(nt, rpt, plan->bin_max_row_nnz[bin], plan->matrix_in_lds[bin])
                 : 0;
             const size_t rlds = kreg > 0
-                ? batch_reg_lds_bytes(sizeof(T), rows, 0, Method) - 4 * (rows + 1)
+                ? (sizeof(T) * (Method == kBatchCg ? 1 : 2) * (rows + 1) + 15) / 16 * 16
                 : batch_reg_lds_bytes(sizeof(T), rows, nnz, Method);
             const bool applies = (fmt == kFmtUneven || fmt == kFmtCsr) && rpt > 0 &&
                                  rlds <= static_cast<size_t>(lds_limit);
@@ -1991,30 +2012,31 @@ int batch_tune_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
         plan->matrix_in_lds[bin] = best_mat;
         plan->vectors[bin] = best_vec;
     }
-    // the tuned per-bin launches against one launch over the whole batch
+    // the tuned per-bin launches against one launch over the whole batch:
+    // the two forms alternate three times and each keeps its best time (the
+    // two are often within a few percent)
     int kreg = 0;
     plan->single_launch = 0;
     if (rc == GKOMI_SUCCESS && packed_applies(a, plan, &kreg)) {
-        float best = 1e30f;
-        int best_single = 0;
-        for (int single : {0, 1}) {
-            gkomi_batch_plan trial = *plan;
-            trial.single_launch = single;
-            GKOMI_CHECK_HIP(hipMemcpyAsync(xs, x, sizeof(T) * rows_all,
-                                           hipMemcpyDeviceToDevice, s));
-            GKOMI_CHECK_HIP(hipEventRecord(e0, s));
-            rc = batch_solve_impl<T, Method>(ctx, a, st, &trial, b, xs, iters, res, ws, wsb);
-            if (rc) break;
-            GKOMI_CHECK_HIP(hipEventRecord(e1, s));
-            GKOMI_CHECK_HIP(hipEventSynchronize(e1));
-            float ms = 0;
-            GKOMI_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best) {
-                best = ms;
-                best_single = single;
+        float best[2] = {1e30f, 1e30f};
+        for (int rep = 0; rep < 3 && rc == GKOMI_SUCCESS; ++rep) {
+            for (int single : {0, 1}) {
+                gkomi_batch_plan trial = *plan;
+                trial.single_launch = single;
+                GKOMI_CHECK_HIP(hipMemcpyAsync(xs, x, sizeof(T) * rows_all,
+                                               hipMemcpyDeviceToDevice, s));
+                GKOMI_CHECK_HIP(hipEventRecord(e0, s));
+                rc = batch_solve_impl<T, Method>(ctx, a, st, &trial, b, xs, iters, res, ws,
+                                                 wsb);
+                if (rc) break;
+                GKOMI_CHECK_HIP(hipEventRecord(e1, s));
+                GKOMI_CHECK_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                GKOMI_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+                best[single] = std::min(best[single], ms);
             }
         }
-        plan->single_launch = best_single;
+        plan->single_launch = best[1] < best[0] ? 1 : 0;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
