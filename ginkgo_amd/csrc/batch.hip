@@ -545,21 +545,30 @@ __global__ void __launch_bounds__(NT)
     const int tid = threadIdx.x;
     const int64_t L = lds_rows;
     constexpr int kIn = Method == kBatchCg ? 1 : 2;
-    // K > 0: each gathered vector has one more slot, index L, that stays
-    // zero; a register slot past its row's end points there with value 0,
-    // so the product is one fused multiply-add per slot, without a compare
-    // or a select (0 * 0 adds nothing)
-    constexpr int kPad = K > 0 ? 1 : 0;
-    T* in0 = reinterpret_cast<T*>(smem);
-    T* in1 = in0 + L + kPad;
+    // K > 0: each gathered vector has one more slot that stays zero (index
+    // kZero); a register slot past its row's end points there with value 0,
+    // so a product slot is one fused multiply-add, without a compare or a
+    // select.  In FP64 the gathered vectors sit in static LDS at a fixed
+    // stride and a register slot holds its column's byte offset in the
+    // first vector: the same register addresses the second through the LDS
+    // instruction's immediate offset, so a slot needs no decoding.  In FP32
+    // (measured faster that way) they sit in dynamic LDS, L + 1 values
+    // each, and the columns are packed two per register.
+    constexpr bool kOff = sizeof(T) >= 8;
+    constexpr bool kStatic = K > 0 && kOff;
+    constexpr int kStride = R * NT + 1;
+    __shared__ __align__(16) T stat_in[kStatic ? kIn : 1][kStatic ? kStride : 1];
+    T* in0 = kStatic ? &stat_in[0][0] : reinterpret_cast<T*>(smem);
+    T* in1 = kStatic ? &stat_in[kIn - 1][0] : in0 + L + (K > 0 ? 1 : 0);
+    const int32_t kZero = kStatic ? R * NT : static_cast<int32_t>(L);
     T* lv = in0 + kIn * L;
     int32_t* lrp = reinterpret_cast<int32_t*>(lv + lds_nnz);
     uint16_t* lci = reinterpret_cast<uint16_t*>(lrp + L + 1);
     const T tol = static_cast<T>(tol_d);
     if constexpr (K > 0) {
         if (tid == 0) {
-            in0[L] = T(0);
-            in1[L] = T(0);
+            in0[kZero] = T(0);
+            in1[kZero] = T(0);
         }
     }
 
@@ -604,16 +613,21 @@ __global__ void __launch_bounds__(NT)
         const int32_t* gci = ci + (Fmt == kFmtCsr ? 0 : nnz0);
         const T* gb = b + row0;
         T* gx = x + row0;
-        // K > 0: the own rows' first K entries in registers, the local
-        // columns (below 2^16: a system has at most 2 NT rows) two per
-        // register, entry q in bits 16 (q & 1) .. 16 (q & 1) + 15 of mc[q / 2]
+        // K > 0: the own rows' first K entries in registers; in FP64 each
+        // column as its byte offset in a gathered vector (no decoding in the
+        // products), in FP32 the columns packed two per register (entry q in
+        // bits 16 (q & 1) ..; fewer registers, which keep the occupancy)
         constexpr int KR = K > 0 ? K : 1;
-        constexpr int KC = (KR + 1) / 2;
+        constexpr int KM = kOff ? KR : (KR + 1) / 2;
         T mv[R][KR];
-        uint32_t mc[R][KC];
+        uint32_t mc[R][KM];
         int32_t mb[R], ml[R];
-        auto col_of = [&](int j, int q) -> int {
-            return static_cast<int>((mc[j][q / 2] >> (16 * (q & 1))) & 0xFFFFu);
+        auto col_is = [&](int j, int q, int col) {
+            if constexpr (kOff) {
+                return mc[j][q] == static_cast<uint32_t>(col * static_cast<int>(sizeof(T)));
+            } else {
+                return static_cast<int>((mc[j][q / 2] >> (16 * (q & 1))) & 0xFFFFu) == col;
+            }
         };
         if constexpr (K > 0) {
 #pragma unroll
@@ -622,14 +636,17 @@ __global__ void __launch_bounds__(NT)
                 mb[j] = row < n ? srp[row] - rp_base : 0;
                 ml[j] = row < n ? srp[row + 1] - rp_base - mb[j] : 0;
 #pragma unroll
-                for (int q = 0; q < KC; ++q) mc[j][q] = 0;
+                for (int q = 0; q < KM; ++q) mc[j][q] = 0;
 #pragma unroll
                 for (int q = 0; q < K; ++q) {
                     const bool in_row = q < ml[j];
                     mv[j][q] = in_row ? gv[mb[j] + q] : T(0);
-                    const uint32_t c = in_row ? static_cast<uint32_t>(gci[mb[j] + q])
-                                              : static_cast<uint32_t>(L);
-                    mc[j][q / 2] |= c << (16 * (q & 1));
+                    const uint32_t c = static_cast<uint32_t>(in_row ? gci[mb[j] + q] : kZero);
+                    if constexpr (kOff) {
+                        mc[j][q] = c * static_cast<uint32_t>(sizeof(T));
+                    } else {
+                        mc[j][q / 2] |= c << (16 * (q & 1));
+                    }
                 }
             }
         } else {
@@ -664,7 +681,7 @@ __global__ void __launch_bounds__(NT)
                     bool found = false;
 #pragma unroll
                     for (int q = 0; q < K; ++q) {
-                        if (!found && q < ml[j] && col_of(j, q) == row) {
+                        if (!found && q < ml[j] && col_is(j, q, row)) {
                             found = true;
                             if (mv[j][q] != T(0)) d = T(1) / mv[j][q];
                         }
@@ -694,19 +711,27 @@ __global__ void __launch_bounds__(NT)
                 const int row = tid + j * NT;
                 T sum = 0;
                 if constexpr (K > 0) {
+                    // a slot past the row's end reads the zero slot with
+                    // value 0: the row sum is the stored-order sum of the
+                    // row's own entries
+                    if constexpr (kOff) {
+                        const char* base = reinterpret_cast<const char*>(in);
 #pragma unroll
-                    for (int qq = 0; qq < KC; ++qq) {
-                        // the empty asm keeps the packed word opaque, so the
-                        // compiler decodes it here instead of holding every
-                        // entry's LDS address in a register of its own
-                        uint32_t w = mc[j][qq];
-                        asm volatile("" : "+v"(w));
-                        const int q0 = 2 * qq;
-                        // a slot past the row's end reads the zero slot with
-                        // value 0: the row sum is the stored-order sum of the
-                        // row's own entries
-                        sum = fma(mv[j][q0], in[w & 0xFFFFu], sum);
-                        if (q0 + 1 < K) sum = fma(mv[j][q0 + 1], in[w >> 16], sum);
+                        for (int q = 0; q < K; ++q) {
+                            sum = fma(mv[j][q], *reinterpret_cast<const T*>(base + mc[j][q]),
+                                      sum);
+                        }
+                    } else {
+#pragma unroll
+                        for (int qq = 0; qq < KM; ++qq) {
+                            // the empty asm keeps the packed word opaque, so
+                            // the compiler decodes it here instead of holding
+                            // every entry's LDS address in a register
+                            uint32_t w = mc[j][qq];
+                            asm volatile("" : "+v"(w));
+                            sum = fma(mv[j][2 * qq], in[w & 0xFFFFu], sum);
+                            if (2 * qq + 1 < K) sum = fma(mv[j][2 * qq + 1], in[w >> 16], sum);
+                        }
                     }
                     for (int32_t e = mb[j] + K; e < mb[j] + ml[j]; ++e) sum += gv[e] * in[gci[e]];
                 } else if (row < n) {
@@ -951,7 +976,6 @@ __global__ void __launch_bounds__(256)
 {
     constexpr int NT = 256;
     constexpr int R = 2;
-    constexpr int KC = (K + 1) / 2;
     constexpr int kIn = Method == kBatchCg ? 1 : 2;
     // the gathered vectors (p_hat, s_hat; CG p), and in FP32 also x, prec
     // and (BiCGSTAB) r_hat, p and v: a thread reads back only its own rows of
@@ -962,8 +986,11 @@ __global__ void __launch_bounds__(256)
     constexpr int kVec = kIn + (kLdsVec ? (Method == kBatchCg ? 2 : 5) : 0);
     // one more slot per vector, index R * NT, that stays zero: a register slot
     // past its row's end points there with value 0, so the product is one
-    // fused multiply-add per slot, without a compare or a select
-    __shared__ T in_s[kVec][R * NT + 1];
+    // fused multiply-add per slot, without a compare or a select.  A register
+    // slot holds its column's byte offset in the first vector (the group's
+    // rows included); the second vector is read through the LDS
+    // instruction's immediate offset.
+    __shared__ __align__(16) T in_s[kVec][R * NT + 1];
     __shared__ T red2[2 * 16];
     __shared__ int32_t go_flag[8];
     __shared__ int64_t slot;
@@ -1030,24 +1057,44 @@ __global__ void __launch_bounds__(256)
         auto xr = [&](int j) -> T& { return own_row(lx, x_r, j); };
         auto prec = [&](int j) -> T& { return own_row(lprec, prec_r, j); };
 
+        // the own rows' first K entries: in FP64 each column as its byte
+        // offset from the start of a vector of in_s (the group's rows start
+        // at first_col); in FP32 the group-local columns packed two per
+        // register, read through the group's vector pointers (measured
+        // faster in FP32)
+        constexpr bool kOff = sizeof(T) >= 8;
+        constexpr int KM = kOff ? K : (K + 1) / 2;
         T mv[R][K];
-        uint32_t mc[R][KC];
+        uint32_t mc[R][KM];
         int32_t mb[R], ml[R];
-        // the zero slot, relative to the group's vectors (at most 512)
-        const uint32_t zero_col = static_cast<uint32_t>(R * NT - g * (R * L));
+        const int32_t first_col = kOff ? g * (R * L) : 0;
+        // the zero slot: index R * NT of in_s, relative to the columns' origin
+        const int32_t zero_col = R * NT - (kOff ? 0 : g * (R * L));
+        auto col_is = [&](int j, int q, int col) {
+            if constexpr (kOff) {
+                return mc[j][q] == static_cast<uint32_t>(col * static_cast<int>(sizeof(T)));
+            } else {
+                return static_cast<int>((mc[j][q / 2] >> (16 * (q & 1))) & 0xFFFFu) == col;
+            }
+        };
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             const int row = lt + j * L;
             mb[j] = row < n ? srp[row] - rp_base : 0;
             ml[j] = row < n ? srp[row + 1] - rp_base - mb[j] : 0;
 #pragma unroll
-            for (int q = 0; q < KC; ++q) mc[j][q] = 0;
+            for (int q = 0; q < KM; ++q) mc[j][q] = 0;
 #pragma unroll
             for (int q = 0; q < K; ++q) {
                 const bool in_row = q < ml[j];
                 mv[j][q] = in_row ? gv[mb[j] + q] : T(0);
-                const uint32_t c = in_row ? static_cast<uint32_t>(gci[mb[j] + q]) : zero_col;
-                mc[j][q / 2] |= c << (16 * (q & 1));
+                const uint32_t c =
+                    static_cast<uint32_t>(in_row ? first_col + gci[mb[j] + q] : zero_col);
+                if constexpr (kOff) {
+                    mc[j][q] = c * static_cast<uint32_t>(sizeof(T));
+                } else {
+                    mc[j][q / 2] |= c << (16 * (q & 1));
+                }
             }
         }
         T r[R];
@@ -1073,8 +1120,7 @@ __global__ void __launch_bounds__(256)
                 bool found = false;
 #pragma unroll
                 for (int q = 0; q < K; ++q) {
-                    const int c = static_cast<int>((mc[j][q / 2] >> (16 * (q & 1))) & 0xFFFFu);
-                    if (!found && q < ml[j] && c == row) {
+                    if (!found && q < ml[j] && col_is(j, q, first_col + row)) {
                         found = true;
                         if (mv[j][q] != T(0)) d = T(1) / mv[j][q];
                     }
@@ -1088,18 +1134,28 @@ __global__ void __launch_bounds__(256)
             }
             prec(j) = d;
         }
-        // out = A in at the own rows, each row summed in stored order
-        auto spmv = [&](const T* in, T out[R]) {
+        // out = A in at the own rows, each row summed in stored order, for
+        // in = vector `which` of in_s (0 or kIn - 1)
+        auto spmv = [&](int which, T out[R]) {
+            const T* vec = in_s[which];
+            const char* base = reinterpret_cast<const char*>(vec);
+            const T* in = vec + g * (R * L);
 #pragma unroll
             for (int j = 0; j < R; ++j) {
                 T sum = 0;
+                if constexpr (kOff) {
 #pragma unroll
-                for (int qq = 0; qq < KC; ++qq) {
-                    uint32_t w = mc[j][qq];
-                    asm volatile("" : "+v"(w));
-                    const int q0 = 2 * qq;
-                    sum = fma(mv[j][q0], in[w & 0xFFFFu], sum);
-                    if (q0 + 1 < K) sum = fma(mv[j][q0 + 1], in[w >> 16], sum);
+                    for (int q = 0; q < K; ++q) {
+                        sum = fma(mv[j][q], *reinterpret_cast<const T*>(base + mc[j][q]), sum);
+                    }
+                } else {
+#pragma unroll
+                    for (int qq = 0; qq < KM; ++qq) {
+                        uint32_t w = mc[j][qq];
+                        asm volatile("" : "+v"(w));
+                        sum = fma(mv[j][2 * qq], in[w & 0xFFFFu], sum);
+                        if (2 * qq + 1 < K) sum = fma(mv[j][2 * qq + 1], in[w >> 16], sum);
+                    }
                 }
                 for (int32_t e = mb[j] + K; e < mb[j] + ml[j]; ++e) sum += gv[e] * in[gci[e]];
                 out[j] = sum;
@@ -1122,7 +1178,7 @@ __global__ void __launch_bounds__(256)
         bool go = valid;
         if constexpr (Method == kBatchCg) {
             T z[R], p[R], ap[R];
-            spmv(in0, ap);
+            spmv(0, ap);
             T rz = 0;
 #pragma unroll
             for (int j = 0; j < R; ++j) {
@@ -1153,7 +1209,7 @@ __global__ void __launch_bounds__(256)
                 if (!any_going(go)) break;
                 T pap = 0;
                 if (go) {
-                    spmv(in0, ap);
+                    spmv(0, ap);
 #pragma unroll
                     for (int j = 0; j < R; ++j) pap += p[j] * ap[j];
                 }
@@ -1176,7 +1232,7 @@ __global__ void __launch_bounds__(256)
             auto rh = [&](int j) -> T& { return own_row(lrh, rh_r, j); };
             auto p = [&](int j) -> T& { return own_row(lp, p_r, j); };
             auto v = [&](int j) -> T& { return own_row(lv, v_r, j); };
-            spmv(in0, t);
+            spmv(0, t);
             T rr = 0;
 #pragma unroll
             for (int j = 0; j < R; ++j) {
@@ -1206,7 +1262,7 @@ __global__ void __launch_bounds__(256)
                 if (!any_going(go)) break;
                 T rv = 0;
                 if (go) {
-                    spmv(in0, t);
+                    spmv(0, t);
 #pragma unroll
                     for (int j = 0; j < R; ++j) {
                         v(j) = t[j];
@@ -1239,7 +1295,7 @@ __global__ void __launch_bounds__(256)
                 }
                 T ts = 0, tt = 0;
                 if (go) {
-                    spmv(in1, t);
+                    spmv(kIn - 1, t);
 #pragma unroll
                     for (int j = 0; j < R; ++j) {
                         ts += t[j] * s[j];
@@ -1722,9 +1778,13 @@ int batch_solve_impl(const gkomi_ctx* ctx, const gkomi_batch_csr* a,
             const int kreg = fmt == kFmtUneven
                 ? reg_row_entries(nt, rpt, plan->bin_max_row_nnz[bin], plan->matrix_in_lds[bin])
                 : 0;
-            const size_t rlds = kreg > 0
-                ? (sizeof(T) * (Method == kBatchCg ? 1 : 2) * (rows + 1) + 15) / 16 * 16
-                : batch_reg_lds_bytes(sizeof(T), rows, nnz, Method);
+            // the rows in registers: the gathered vectors, each with its zero
+            // slot (static LDS in FP64)
+            const size_t rlds =
+                kreg > 0 ? (sizeof(T) >= 8 ? 0
+                                           : (sizeof(T) * (Method == kBatchCg ? 1 : 2) * (rows + 1) +
+                                              15) / 16 * 16)
+                         : batch_reg_lds_bytes(sizeof(T), rows, nnz, Method);
             const bool applies = (fmt == kFmtUneven || fmt == kFmtCsr) && rpt > 0 &&
                                  rlds <= static_cast<size_t>(lds_limit);
             GKOMI_REQUIRE(want != 2 || applies, GKOMI_NOT_SUPPORTED,
