@@ -48,22 +48,11 @@ __device__ __forceinline__ void csr_subwave_body(int64_t n_rows, const int32_t* 
 }
 
 // G lanes per row with staged, coalesced matrix loads (rows of nearly equal
-// length: stencils).  Wave w takes the window of R = 64 / G consecutive rows
-// [R w, R w + R); the window's entries are copied into the wave's LDS
-// staging area of C entries in runs of 64 consecutive entries (every load
-// one contiguous run of values and of column indices -- or, with Compact,
-// of the 2-byte offsets column - row -- all issued before the first LDS
-// write).  Then lane l of a row's group adds the row's entries l, l + G,
-// l + 2G, ... from LDS by fused multiply-add, gathering b, and the group's
-// partial sums are added by the DPP tree of group_sum<G>: the order of the
-// sub-wave kernel with G lanes per row, bit for bit.  For G = 1 consecutive
-// lanes are consecutive rows, so a stencil's gathers read consecutive words;
-// for larger G each gather instruction covers G offsets of R consecutive
-// rows.  A window of more than C entries is staged in chunks; a lane's
-// entries continue in the next chunk in the same order.  Only the wave
-// itself reads its staging area: LDS accesses of one wave complete in order,
-// and the wave barriers keep the compiler from moving them across the
-// staging.
+// length: stencils): every wave walks windows of 64 / G consecutive rows with
+// staged_window (spmv_common.hpp), which copies a window's entries to the
+// wave's LDS in contiguous 64-entry runs and then sums each row in the order
+// of the sub-wave kernel with G lanes per row.  For G = 1 consecutive lanes
+// are consecutive rows, so a stencil's gathers read consecutive words.
 template <typename T, int G, int C, int Mode, bool Compact>
 __global__ void __launch_bounds__(kBlock)
     csr_spmv_staged(int64_t n_rows, const int32_t* __restrict__ rp,
@@ -72,82 +61,17 @@ __global__ void __launch_bounds__(kBlock)
                     int64_t cs, const T* __restrict__ alpha, const T* __restrict__ beta)
 {
     using Idx = std::conditional_t<Compact, int16_t, int32_t>;
-    constexpr int kSteps = C / kWave;
-    constexpr int kRows = kWave / G;
-    constexpr int U = 4;
     __shared__ T sv[kWavesPerBlock][C];
     __shared__ Idx sc[kWavesPerBlock][C];
-    const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
-    const int l = lane % G;
-    T* lv = sv[wave];
-    Idx* lc = sc[wave];
-    const int64_t windows = ceildiv(n_rows, kRows);
+    const int64_t windows = ceildiv(n_rows, kWave / G);
     const int64_t nw = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
     for (int64_t w = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * kWavesPerBlock +
                      wave;
          w < windows; w += nw) {
-        const int64_t row = w * kRows + lane / G;
-        const int32_t rs = rp[row < n_rows ? row : n_rows];
-        const int32_t re = rp[row < n_rows ? row + 1 : n_rows];
-        const int32_t ws = __shfl(rs, 0, kWave);
-        const int32_t we = __shfl(re, kWave - 1, kWave);
-        T sum = 0;
-        int32_t k = rs + l;  // the lane's next entry (absolute position)
-        for (int32_t c0 = ws; c0 < we; c0 += C) {
-            const int32_t c1 = c0 + C < we ? c0 + C : we;
-            T tv[kSteps];
-            Idx tc[kSteps];
-#pragma unroll
-            for (int u = 0; u < kSteps; ++u) {
-                const int32_t e = c0 + u * kWave + lane;
-                const int32_t ee = e < c1 ? e : c0;  // c0 < we <= nnz: a valid entry
-                tv[u] = v[ee];
-                if constexpr (Compact) {
-                    tc[u] = co[ee];
-                } else {
-                    tc[u] = ci[ee];
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-#pragma unroll
-            for (int u = 0; u < kSteps; ++u) {
-                lv[u * kWave + lane] = tv[u];
-                lc[u * kWave + lane] = tc[u];
-            }
-            __builtin_amdgcn_wave_barrier();
-            // the lane's entries k, k + G, ... below the row's end and the
-            // chunk's end (k never falls below c0: earlier chunks took them)
-            const int32_t z = re < c1 ? re : c1;
-            while (__ballot(k < z) != 0) {
-                int32_t col[U];
-                T val[U];
-                bool ok[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int32_t kk = k + u * G;
-                    ok[u] = kk < z;
-                    const int32_t idx = ok[u] ? kk - c0 : 0;
-                    col[u] = Compact ? static_cast<int32_t>(row) + lc[idx] : lc[idx];
-                    val[u] = lv[idx];
-                }
-                T xv[U];
-                // a slot past the row's end gathers column 0 (always valid)
-#pragma unroll
-                for (int u = 0; u < U; ++u) xv[u] = b[ok[u] ? col[u] : 0];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    sum = fma(ok[u] ? val[u] : T(0), ok[u] ? xv[u] : T(0), sum);
-                }
-                // advance past the entries taken (at most to the first one
-                // not taken, which may lie in the next chunk)
-#pragma unroll
-                for (int u = 0; u < U; ++u) k = ok[u] ? k + G : k;
-            }
-            __builtin_amdgcn_wave_barrier();
-        }
-        sum = group_sum<G>(sum);
-        if (row < n_rows && l == 0) store_result<Mode>(c + row * cs, sum, alpha, beta);
+        staged_window<T, G, C, Compact>(
+            n_rows, w, rp, ci, co, v, sv[wave], sc[wave], [&](int32_t col) { return b[col]; },
+            [&](int64_t row, T sum) { store_result<Mode>(c + row * cs, sum, alpha, beta); });
     }
 }
 

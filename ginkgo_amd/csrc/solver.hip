@@ -10,6 +10,7 @@
 //    and update passes, and evaluate stop::Combined on the device so the host
 //    never waits inside the loop.
 #include <cmath>
+#include <type_traits>
 
 #include "common.hpp"
 #include "reduce.hpp"
@@ -562,6 +563,94 @@ __global__ void __launch_bounds__(kBlock)
     store_partials<T, 1>(acc, w.part_dir);
 }
 
+// Kernel 1 over the staged product (staged_window, G lanes per row; rows
+// of stencil length): the check and p_out = z + rho/prev_rho p_in as in
+// cg_fused_direction_kernel, q = A p_out with p_out recomputed at the
+// gathered columns, and partials of p.q.  Workgroup tiles of one window
+// per wave.
+template <typename T, int G, bool Compact>
+__global__ void __launch_bounds__(kBlock)
+    cg_fused_direction_staged_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                     const int32_t* __restrict__ ci,
+                                     const int16_t* __restrict__ co, const T* __restrict__ v,
+                                     const T* __restrict__ z, const T* __restrict__ p_in,
+                                     T* __restrict__ p_out, T* __restrict__ q,
+                                     gkomi_criteria crit, T* __restrict__ scalars,
+                                     const T* __restrict__ orig, uint8_t* __restrict__ status,
+                                     gkomi_stop_flags* __restrict__ flags, cg_work w,
+                                     int count_in, int64_t iter, int par)
+{
+    using Idx = std::conditional_t<Compact, int16_t, int32_t>;
+    __shared__ T sv[kWavesPerBlock][kStagedChunk];
+    __shared__ Idx sc[kWavesPerBlock][kStagedChunk];
+    const int flag0 = threadIdx.x == 0 ? flags->all_stopped : 0;
+    const T prev = scalars[par ^ 1];
+    uint8_t st = w.status[par ^ 1];
+    double s[2];
+    if (combine_partials_bcast<2>(w.part_upd, count_in, s, flag0)) return;
+    const T rho = static_cast<T>(s[0]);
+    const T tau = static_cast<T>(sqrt(s[1]));
+    int all = 0, changed = 0;
+    combined_check<T>(crit, iter, 1, &tau, &rho, orig, true, &st, &all, &changed);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scalars[par] = rho;
+        scalars[3 + par] = tau;
+        w.status[par] = st;
+        status[0] = st;
+        flags->iteration = iter;
+        flags->all_stopped = all;
+        flags->one_changed = changed;
+    }
+    if (all) return;
+    const bool restart = prev == T(0);
+    const T tmp = restart ? T(0) : rho / prev;
+    T acc[1] = {T(0)};
+    auto gather = [&](int32_t c) { return restart ? z[c] : fma(tmp, p_in[c], z[c]); };
+    auto row_op = [&](int64_t row, T sum) {
+        const T pi = restart ? z[row] : fma(tmp, p_in[row], z[row]);
+        p_out[row] = pi;
+        q[row] = sum;
+        acc[0] += pi * sum;
+    };
+    const int wave = threadIdx.x / kWave;
+    int64_t rb0, rb1;
+    block_range(ceildiv(n, staged_tile_rows<G>), &rb0, &rb1);
+    for (int64_t blk = rb0; blk < rb1; ++blk) {
+        staged_window<T, G, kStagedChunk, Compact>(n, blk * kWavesPerBlock + wave, rp, ci, co,
+                                                   v, sv[wave], sc[wave], gather, row_op);
+    }
+    store_partials<T, 1>(acc, w.part_dir);
+}
+
+// (b) over the staged product: q = A p and the partials of p.q.
+template <typename T, int G, bool Compact>
+__global__ void __launch_bounds__(kBlock)
+    cg_split_spmv_staged_kernel(int64_t n, const int32_t* __restrict__ rp,
+                                const int32_t* __restrict__ ci, const int16_t* __restrict__ co,
+                                const T* __restrict__ v, const T* __restrict__ p,
+                                T* __restrict__ q, const gkomi_stop_flags* __restrict__ flags,
+                                cg_work w)
+{
+    using Idx = std::conditional_t<Compact, int16_t, int32_t>;
+    __shared__ T sv[kWavesPerBlock][kStagedChunk];
+    __shared__ Idx sc[kWavesPerBlock][kStagedChunk];
+    if (flags->all_stopped) return;
+    T acc[1] = {T(0)};
+    const int wave = threadIdx.x / kWave;
+    int64_t rb0, rb1;
+    block_range(ceildiv(n, staged_tile_rows<G>), &rb0, &rb1);
+    for (int64_t blk = rb0; blk < rb1; ++blk) {
+        staged_window<T, G, kStagedChunk, Compact>(
+            n, blk * kWavesPerBlock + wave, rp, ci, co, v, sv[wave], sc[wave],
+            [&](int32_t c) { return p[c]; },
+            [&](int64_t row, T sum) {
+                q[row] = sum;
+                acc[0] += p[row] * sum;
+            });
+    }
+    store_partials<T, 1>(acc, w.part_dir);
+}
+
 // (b) over the matrix's sliced copy: q = A p and the partials of p.q.
 template <typename T, bool Compact>
 __global__ void __launch_bounds__(kBlock)
@@ -613,6 +702,7 @@ int check_cg_state(const gkomi_cg_state* st)
 // array must know how many workgroups produced it).
 struct cg_grids {
     subwave_shape sh;
+    int staged;  // lanes per row of the staged product kernels, 0 = sub-wave
     int spmv;    // init and direction kernels
     int update;  // update kernels
 };
@@ -640,9 +730,26 @@ cg_grids cg_grids_for(const gkomi_csr* a, int64_t n)
     if (a->subwave == 0 && mean >= kCgSplitMeanRow) {
         g.sh = split_shape(mean, sizeof(T));
     }
-    const int64_t tile_rows =
-        has_slices(a) ? kSlicedTileRows : int64_t{kBlock} / g.sh.sw * g.sh.rpg;
+    // the staged product where the classical SpMV takes it (rows of
+    // stencil length), unless the matrix carries its sliced copy
+    g.staged = has_slices(a) ? 0 : std::max(0, staged_lanes(a, sizeof(T)));
+    const int64_t tile_rows = has_slices(a) ? kSlicedTileRows
+                              : g.staged > 0 ? int64_t{kWave} / g.staged * kWavesPerBlock
+                                             : int64_t{kBlock} / g.sh.sw * g.sh.rpg;
     g.spmv = fused_grid(ceildiv(std::max<int64_t>(n, 1), tile_rows));
+    if (g.staged > 0) {
+        // no more workgroups than are resident at once: each holds four
+        // staging areas (24 KB in FP64 with 4-byte indices, so 6 per CU of
+        // 160 KB), and a partial second round of workgroups left the CUs of
+        // the first idle at the end (measured on 7-entry rows of 0.5-1.2M
+        // rows: 8 per CU 37.8 us, resident only 31-32 us per iteration)
+        const int64_t per_wg =
+            int64_t{kWavesPerBlock} * kStagedChunk *
+                (static_cast<int64_t>(sizeof(T)) + (a->col_offsets != nullptr ? 2 : 4)) +
+            2048;
+        const int64_t resident = std::max<int64_t>(1, std::min<int64_t>(8, (160 * 1024) / per_wg));
+        g.spmv = static_cast<int>(std::min<int64_t>(g.spmv, resident * num_cus()));
+    }
     g.update = static_cast<int>(
         std::max<int64_t>(1, std::min<int64_t>(ceildiv(std::max<int64_t>(n, 1), kUpdBlock),
                                                kCgUpdGrid)));
@@ -765,6 +872,24 @@ int cg_fused_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_
                         static_cast<T*>(st->q), st->flags, w);
                 }
                 GKOMI_CHECK_LAUNCH("cg_split_spmv_sliced_kernel");
+            } else if (g.staged > 0) {
+#define GKOMI_STAGED_SPLIT(GV, C)                                                          \
+    if (g.staged == GV) {                                                                  \
+        cg_split_spmv_staged_kernel<T, GV, C><<<g.spmv, kBlock, 0, s>>>(                   \
+            n, a->row_ptrs, a->col_idxs, a->col_offsets, v, pout, static_cast<T*>(st->q),  \
+            st->flags, w);                                                                 \
+    }
+                if (a->col_offsets) {
+                    GKOMI_STAGED_SPLIT(1, true) GKOMI_STAGED_SPLIT(2, true)
+                    GKOMI_STAGED_SPLIT(4, true) GKOMI_STAGED_SPLIT(8, true)
+                    GKOMI_STAGED_SPLIT(16, true)
+                } else {
+                    GKOMI_STAGED_SPLIT(1, false) GKOMI_STAGED_SPLIT(2, false)
+                    GKOMI_STAGED_SPLIT(4, false) GKOMI_STAGED_SPLIT(8, false)
+                    GKOMI_STAGED_SPLIT(16, false)
+                }
+#undef GKOMI_STAGED_SPLIT
+                GKOMI_CHECK_LAUNCH("cg_split_spmv_staged_kernel");
             } else {
 #define GKOMI_SPLIT_CASE_C(S, R, C)                                                        \
     cg_split_spmv_kernel<T, S, R, C><<<g.spmv, kBlock, 0, s>>>(                            \
@@ -783,6 +908,24 @@ int cg_fused_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_
 #undef GKOMI_SPLIT_CASE_C
             GKOMI_CHECK_LAUNCH("cg_split_spmv_kernel");
             }
+        } else if (g.staged > 0) {
+#define GKOMI_STAGED_DIR(GV, C)                                                            \
+    if (g.staged == GV) {                                                                  \
+        cg_fused_direction_staged_kernel<T, GV, C><<<g.spmv, kBlock, 0, s>>>(              \
+            n, a->row_ptrs, a->col_idxs, a->col_offsets, v, static_cast<const T*>(st->z),  \
+            pin, pout, static_cast<T*>(st->q), *crit, sc,                                  \
+            static_cast<const T*>(st->orig_tau), st->status, st->flags, w, g.update, iter, \
+            par);                                                                          \
+    }
+            if (a->col_offsets) {
+                GKOMI_STAGED_DIR(1, true) GKOMI_STAGED_DIR(2, true) GKOMI_STAGED_DIR(4, true)
+                GKOMI_STAGED_DIR(8, true) GKOMI_STAGED_DIR(16, true)
+            } else {
+                GKOMI_STAGED_DIR(1, false) GKOMI_STAGED_DIR(2, false) GKOMI_STAGED_DIR(4, false)
+                GKOMI_STAGED_DIR(8, false) GKOMI_STAGED_DIR(16, false)
+            }
+#undef GKOMI_STAGED_DIR
+            GKOMI_CHECK_LAUNCH("cg_fused_direction_staged_kernel");
         } else {
 #define GKOMI_DIR_CASE_C(S, R, C)                                                          \
     cg_fused_direction_kernel<T, S, R, C><<<g.spmv, kBlock, 0, s>>>(                       \

@@ -139,6 +139,100 @@ __device__ __forceinline__ void subwave_tile(int64_t n_rows, int64_t tile,
     }
 }
 
+// One window of the staged CSR product, run by one wave: the R = 64 / G
+// consecutive rows [R w, R w + R).  The window's entries are copied into the
+// wave's LDS staging area (lv, lc: C values and C column indices, or with
+// Compact the 2-byte offsets column - row) in runs of 64 consecutive entries
+// (every load one contiguous run, all issued before the first LDS write),
+// then lane l of a row's group adds the row's entries l, l + G, ... from LDS
+// by fused multiply-add with gather(col), and the group tree adds the lanes:
+// the per-row order of subwave_tile with SW = G, bit for bit.  A window of
+// more than C entries is staged in chunks; a lane's entries continue in the
+// next chunk in the same order.  `row_op(row, sum)` runs in lane 0 of the
+// group for every row below n_rows.  Only the wave itself reads its staging
+// area: LDS accesses of one wave complete in order, and the wave barriers
+// keep the compiler from moving them across the staging.
+template <typename T, int G, int C, bool Compact, typename Idx, typename Gather,
+          typename RowOp, typename TV>
+__device__ __forceinline__ void staged_window(int64_t n_rows, int64_t w,
+                                              const int32_t* __restrict__ rp,
+                                              const int32_t* __restrict__ ci,
+                                              const int16_t* __restrict__ co,
+                                              const TV* __restrict__ v, TV* lv, Idx* lc,
+                                              Gather gather, RowOp row_op)
+{
+    constexpr int kSteps = C / kWave;
+    constexpr int kRows = kWave / G;
+    constexpr int U = 4;
+    const int lane = threadIdx.x % kWave;
+    const int l = lane % G;
+    const int64_t row = w * kRows + lane / G;
+    const int32_t rs = rp[row < n_rows ? row : n_rows];
+    const int32_t re = rp[row < n_rows ? row + 1 : n_rows];
+    const int32_t ws = __shfl(rs, 0, kWave);
+    const int32_t we = __shfl(re, kWave - 1, kWave);
+    T sum = 0;
+    int32_t k = rs + l;  // the lane's next entry (absolute position)
+    for (int32_t c0 = ws; c0 < we; c0 += C) {
+        const int32_t c1 = c0 + C < we ? c0 + C : we;
+        TV tv[kSteps];
+        Idx tc[kSteps];
+#pragma unroll
+        for (int u = 0; u < kSteps; ++u) {
+            const int32_t e = c0 + u * kWave + lane;
+            const int32_t ee = e < c1 ? e : c0;  // c0 < we <= nnz: a valid entry
+            tv[u] = v[ee];
+            if constexpr (Compact) {
+                tc[u] = co[ee];
+            } else {
+                tc[u] = ci[ee];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int u = 0; u < kSteps; ++u) {
+            lv[u * kWave + lane] = tv[u];
+            lc[u * kWave + lane] = tc[u];
+        }
+        __builtin_amdgcn_wave_barrier();
+        // the lane's entries k, k + G, ... below the row's end and the
+        // chunk's end (k never falls below c0: earlier chunks took them)
+        const int32_t z = re < c1 ? re : c1;
+        while (__ballot(k < z) != 0) {
+            int32_t col[U];
+            T val[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int32_t kk = k + u * G;
+                ok[u] = kk < z;
+                const int32_t idx = ok[u] ? kk - c0 : 0;
+                col[u] = Compact ? static_cast<int32_t>(row) + lc[idx] : lc[idx];
+                val[u] = static_cast<T>(lv[idx]);
+            }
+            T xv[U];
+            // a slot past the row's end gathers column 0 (always valid)
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = gather(ok[u] ? col[u] : 0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                sum = fma(ok[u] ? val[u] : T(0), ok[u] ? xv[u] : T(0), sum);
+            }
+            // past the entries taken (at most to the first one not taken,
+            // which may lie in the next chunk)
+#pragma unroll
+            for (int u = 0; u < U; ++u) k = ok[u] ? k + G : k;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    sum = group_sum<G>(sum);
+    if (row < n_rows && l == 0) row_op(row, sum);
+}
+
+// Rows of one workgroup tile of the staged product (one window per wave).
+template <int G>
+constexpr int64_t staged_tile_rows = int64_t{kWave / G} * kWavesPerBlock;
+
 // subwave_tile split in two, for kernels that have other loads to wait for
 // first (the partial sums of a fused solver): subwave_prefetch issues the
 // tile's row-pointer loads and the first step's column and value loads;
