@@ -22,6 +22,8 @@
 // as prev_rho = 1), [2] alpha, [3] omega, [4], [5] ||r|| of even / odd
 // iterations, [6] ||s|| of the half step that stopped the solve.  No
 // workgroup reads a value that another workgroup of the same kernel writes.
+#include <type_traits>
+
 #include "common.hpp"
 #include "fused.hpp"
 #include "spmv_common.hpp"
@@ -312,6 +314,87 @@ __global__ void __launch_bounds__(kBlock)
     store_partials<T, 2>(acc, w.part_t);
 }
 
+// K2 and K4 over the staged product (staged_window, G lanes per row; rows
+// of stencil length), one window per wave: the same sums, checks and
+// partials as bcs_v_kernel and bcs_t_kernel.
+template <typename T, int G, bool Compact>
+__global__ void __launch_bounds__(kBlock)
+    bcs_v_staged_kernel(int64_t n, const int32_t* __restrict__ rp,
+                        const int32_t* __restrict__ ci, const int16_t* __restrict__ co,
+                        const T* __restrict__ av, const T* __restrict__ y,
+                        const T* __restrict__ rr, T* __restrict__ v,
+                        const gkomi_stop_flags* __restrict__ flags, bcs_work w)
+{
+    using Idx = std::conditional_t<Compact, int16_t, int32_t>;
+    __shared__ T sv[kWavesPerBlock][kStagedChunk];
+    __shared__ Idx sc[kWavesPerBlock][kStagedChunk];
+    if (flags->all_stopped) return;
+    T acc[1] = {T(0)};
+    const int wave = threadIdx.x / kWave;
+    int64_t rb0, rb1;
+    block_range(ceildiv(n, staged_tile_rows<G>), &rb0, &rb1);
+    for (int64_t blk = rb0; blk < rb1; ++blk) {
+        staged_window<T, G, kStagedChunk, Compact>(
+            n, blk * kWavesPerBlock + wave, rp, ci, co, av, sv[wave], sc[wave],
+            [&](int32_t c) { return y[c]; },
+            [&](int64_t row, T sum) {
+                v[row] = sum;
+                acc[0] += rr[row] * sum;
+            });
+    }
+    store_partials<T, 1>(acc, w.part_v);
+}
+
+template <typename T, int G, bool Compact>
+__global__ void __launch_bounds__(kBlock)
+    bcs_t_staged_kernel(int64_t n, const int32_t* __restrict__ rp,
+                        const int32_t* __restrict__ ci, const int16_t* __restrict__ co,
+                        const T* __restrict__ av, const T* __restrict__ z,
+                        const T* __restrict__ s, T* __restrict__ t, gkomi_criteria crit,
+                        T* __restrict__ scalars, const T* __restrict__ orig,
+                        uint8_t* __restrict__ status, gkomi_stop_flags* __restrict__ flags,
+                        bcs_work w, int count_s, int64_t iter, int par)
+{
+    using Idx = std::conditional_t<Compact, int16_t, int32_t>;
+    __shared__ T sv[kWavesPerBlock][kStagedChunk];
+    __shared__ Idx sc[kWavesPerBlock][kStagedChunk];
+    const int flag0 = threadIdx.x == 0 ? flags->all_stopped : 0;
+    const T rho = scalars[par];
+    uint8_t st = w.status[par];
+    double ss[1];
+    if (combine_partials_bcast<1>(w.part_s, count_s, ss, flag0)) return;
+    const T tau = static_cast<T>(sqrt(ss[0]));
+    int all = 0, changed = 0;
+    combined_check<T>(crit, iter, 1, &tau, &rho, orig, false, &st, &all, &changed);
+    if (all) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            scalars[6] = tau;
+            w.status[2] = st;
+            status[0] = st;
+            *w.half_stop = iter;
+            flags->iteration = iter;
+            flags->one_changed = changed;
+            flags->all_stopped = 1;
+        }
+        return;
+    }
+    T acc[2] = {T(0), T(0)};
+    const int wave = threadIdx.x / kWave;
+    int64_t rb0, rb1;
+    block_range(ceildiv(n, staged_tile_rows<G>), &rb0, &rb1);
+    for (int64_t blk = rb0; blk < rb1; ++blk) {
+        staged_window<T, G, kStagedChunk, Compact>(
+            n, blk * kWavesPerBlock + wave, rp, ci, co, av, sv[wave], sc[wave],
+            [&](int32_t c) { return z[c]; },
+            [&](int64_t row, T sum) {
+                t[row] = sum;
+                acc[0] += s[row] * sum;
+                acc[1] += sum * sum;
+            });
+    }
+    store_partials<T, 2>(acc, w.part_t);
+}
+
 // K5 (step_3, reference/solver/bicgstab_kernels.cpp:122-152; finalize,
 // :157-172, after a half-step stop).
 template <typename T>
@@ -402,6 +485,7 @@ int check_bcs_call(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_criteri
 
 struct bcs_grids {
     subwave_shape sh;
+    int staged;  // lanes per row of the staged K2 / K4, 0 = sub-wave
     int spmv;    // init, K2, K4
     int update;  // K1, K3, K5
 };
@@ -411,8 +495,22 @@ bcs_grids bcs_grids_for(const gkomi_csr* a, int64_t n)
 {
     bcs_grids g;
     g.sh = pick_subwave(a, sizeof(T));
-    const int64_t tile_rows = int64_t{kBlock} / g.sh.sw * g.sh.rpg;
+    // the staged product where the classical SpMV takes it (rows of
+    // stencil length)
+    g.staged = std::max(0, staged_lanes(a, sizeof(T)));
+    const int64_t tile_rows = g.staged > 0 ? int64_t{kWave} / g.staged * kWavesPerBlock
+                                           : int64_t{kBlock} / g.sh.sw * g.sh.rpg;
     g.spmv = fused_grid(ceildiv(std::max<int64_t>(n, 1), tile_rows));
+    if (g.staged > 0) {
+        // no more workgroups than are resident at once (four staging areas
+        // per workgroup), as the fused CG
+        const int64_t per_wg =
+            int64_t{kWavesPerBlock} * kStagedChunk *
+                (static_cast<int64_t>(sizeof(T)) + (a->col_offsets != nullptr ? 2 : 4)) +
+            2048;
+        const int64_t resident = std::max<int64_t>(1, std::min<int64_t>(8, (160 * 1024) / per_wg));
+        g.spmv = static_cast<int>(std::min<int64_t>(g.spmv, resident * num_cus()));
+    }
     g.update = static_cast<int>(std::max<int64_t>(
         1, std::min<int64_t>(ceildiv(std::max<int64_t>(n, 1), kUpdBlock), kCgUpdGrid)));
     return g;
@@ -494,6 +592,23 @@ int bcs_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_crite
                 iter, par);
         }
         GKOMI_CHECK_LAUNCH("bcs_p_kernel");
+        if (g.staged > 0) {
+#define GKOMI_BCS_V_STAGED(GV, C)                                                            \
+    if (g.staged == GV) {                                                                    \
+        bcs_v_staged_kernel<T, GV, C><<<g.spmv, kBlock, 0, s>>>(                             \
+            n, a->row_ptrs, a->col_idxs, a->col_offsets, av, y, rr, v, st->flags, w);        \
+    }
+            if (a->col_offsets) {
+                GKOMI_BCS_V_STAGED(1, true) GKOMI_BCS_V_STAGED(2, true)
+                GKOMI_BCS_V_STAGED(4, true) GKOMI_BCS_V_STAGED(8, true)
+                GKOMI_BCS_V_STAGED(16, true)
+            } else {
+                GKOMI_BCS_V_STAGED(1, false) GKOMI_BCS_V_STAGED(2, false)
+                GKOMI_BCS_V_STAGED(4, false) GKOMI_BCS_V_STAGED(8, false)
+                GKOMI_BCS_V_STAGED(16, false)
+            }
+#undef GKOMI_BCS_V_STAGED
+        } else {
 #define GKOMI_BCS_V_C(S, R, C)                                                               \
     bcs_v_kernel<T, S, R, C><<<g.spmv, kBlock, 0, s>>>(n, a->row_ptrs, a->col_idxs,          \
                                                        a->col_offsets, av, y, rr, v,         \
@@ -509,6 +624,7 @@ int bcs_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_crite
 #undef GKOMI_BCS_V
 #undef GKOMI_BCS_V_COMPACT
 #undef GKOMI_BCS_V_C
+        }
         GKOMI_CHECK_LAUNCH("bcs_v_kernel");
         if (jac) {
             bcs_s_kernel<T, true><<<g.update, kUpdBlock, 0, s>>>(n, r, v, sv, z, dinv, sc,
@@ -518,6 +634,24 @@ int bcs_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_crite
                                                                  st->flags, w, g.spmv, par);
         }
         GKOMI_CHECK_LAUNCH("bcs_s_kernel");
+        if (g.staged > 0) {
+#define GKOMI_BCS_T_STAGED(GV, C)                                                            \
+    if (g.staged == GV) {                                                                    \
+        bcs_t_staged_kernel<T, GV, C><<<g.spmv, kBlock, 0, s>>>(                             \
+            n, a->row_ptrs, a->col_idxs, a->col_offsets, av, z, sv, t, *crit, sc, orig,      \
+            st->status, st->flags, w, g.update, iter, par);                                  \
+    }
+            if (a->col_offsets) {
+                GKOMI_BCS_T_STAGED(1, true) GKOMI_BCS_T_STAGED(2, true)
+                GKOMI_BCS_T_STAGED(4, true) GKOMI_BCS_T_STAGED(8, true)
+                GKOMI_BCS_T_STAGED(16, true)
+            } else {
+                GKOMI_BCS_T_STAGED(1, false) GKOMI_BCS_T_STAGED(2, false)
+                GKOMI_BCS_T_STAGED(4, false) GKOMI_BCS_T_STAGED(8, false)
+                GKOMI_BCS_T_STAGED(16, false)
+            }
+#undef GKOMI_BCS_T_STAGED
+        } else {
 #define GKOMI_BCS_T_C(S, R, C)                                                               \
     bcs_t_kernel<T, S, R, C><<<g.spmv, kBlock, 0, s>>>(                                      \
         n, a->row_ptrs, a->col_idxs, a->col_offsets, av, z, sv, t, *crit, sc, orig,          \
@@ -533,6 +667,7 @@ int bcs_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_crite
 #undef GKOMI_BCS_T
 #undef GKOMI_BCS_T_COMPACT
 #undef GKOMI_BCS_T_C
+        }
         GKOMI_CHECK_LAUNCH("bcs_t_kernel");
         bcs_update_kernel<T><<<g.update, kUpdBlock, 0, s>>>(n, x, r, rr, sv, t, y, z, sc,
                                                             st->status, st->flags, w, g.spmv,
