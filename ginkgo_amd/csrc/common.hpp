@@ -206,6 +206,68 @@ __device__ __forceinline__ T wave_total_last(T v)
     return v;
 }
 
+// v[lane] + v[lane ^ M] in every lane, for M = 4, 8, 16 or 32.  M = 32 and
+// 16 use the gfx950 lane swaps with the value as both operands: one result
+// holds the lower partner of every pair in both lanes, the other the upper
+// one, whichever way the instruction swaps, so their sum is the pair's sum.
+// M = 8 is DPP row_ror:8 (lane i of a row reads lane (i + 8) mod 16), M = 4
+// a ds_swizzle in xor mode.
+template <int M>
+__device__ __forceinline__ void xor_pair(uint32_t v, uint32_t* a, uint32_t* b)
+{
+    static_assert(M == 4 || M == 8 || M == 16 || M == 32, "xor_pair: M");
+    if constexpr (M == 32) {
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        *a = r[0];
+        *b = r[1];
+    } else if constexpr (M == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        *a = r[0];
+        *b = r[1];
+    } else if constexpr (M == 8) {
+        *a = v;
+        *b = static_cast<uint32_t>(dpp_i32<0x128>(static_cast<int>(v)));
+    } else {
+        *a = v;
+        *b = static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(v), 0x101F));
+    }
+}
+
+template <int M>
+__device__ __forceinline__ float xor_pair_sum(float v)
+{
+    uint32_t a, b;
+    xor_pair<M>(__float_as_uint(v), &a, &b);
+    return __uint_as_float(a) + __uint_as_float(b);
+}
+
+template <int M>
+__device__ __forceinline__ double xor_pair_sum(double v)
+{
+    const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(v));
+    uint32_t alo, blo, ahi, bhi;
+    xor_pair<M>(static_cast<uint32_t>(bits), &alo, &blo);
+    xor_pair<M>(static_cast<uint32_t>(bits >> 32), &ahi, &bhi);
+    const double a = __longlong_as_double(static_cast<long long>((uint64_t{ahi} << 32) | alo));
+    const double b = __longlong_as_double(static_cast<long long>((uint64_t{bhi} << 32) | blo));
+    return a + b;
+}
+
+// Sum over the G lanes r, r + 64/G, r + 2*64/G, ... (G a power of two <= 16),
+// in every one of them: the butterfly of group_sum<G> with lane strides
+// 64/G, 2*64/G, ..., so sub-lane s of a row (lane r + s*64/G) takes the place
+// of lane s of group_sum's group and the sum has the same bits.
+template <int G, typename T>
+__device__ __forceinline__ T strided_sum(T v)
+{
+    constexpr int R = kWave / G;
+    if constexpr (G >= 16) v = xor_pair_sum<R>(v);
+    if constexpr (G >= 8) v = xor_pair_sum<R * (G >= 16 ? 2 : 1)>(v);
+    if constexpr (G >= 4) v = xor_pair_sum<16>(v);
+    if constexpr (G >= 2) v = xor_pair_sum<32>(v);
+    return v;
+}
+
 // Workgroup sum over blockDim.x == 256 threads; result valid in all threads.
 // `lds` must hold kWavesPerBlock values.
 template <typename T>

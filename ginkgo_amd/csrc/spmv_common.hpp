@@ -143,15 +143,20 @@ __device__ __forceinline__ void subwave_tile(int64_t n_rows, int64_t tile,
 // consecutive rows [R w, R w + R).  The window's entries are copied into the
 // wave's LDS staging area (lv, lc: C values and C column indices, or with
 // Compact the 2-byte offsets column - row) in runs of 64 consecutive entries
-// (every load one contiguous run, all issued before the first LDS write),
-// then lane l of a row's group adds the row's entries l, l + G, ... from LDS
-// by fused multiply-add with gather(col), and the group tree adds the lanes:
-// the per-row order of subwave_tile with SW = G, bit for bit.  A window of
-// more than C entries is staged in chunks; a lane's entries continue in the
-// next chunk in the same order.  `row_op(row, sum)` runs in lane 0 of the
-// group for every row below n_rows.  Only the wave itself reads its staging
-// area: LDS accesses of one wave complete in order, and the wave barriers
-// keep the compiler from moving them across the staging.
+// (every load one contiguous run, all issued before the first LDS write).
+// Row r of the window belongs to lanes r, r + R, ..., r + (G - 1) R: its
+// sub-lane s (lane r + s R) adds the row's entries s, s + G, ... from LDS by
+// fused multiply-add with gather(col), and the strided butterfly adds the
+// sub-lanes: the per-row order of subwave_tile with SW = G, bit for bit.
+// Lanes of one sub-lane index hold R consecutive rows, so each gather
+// instruction reads R consecutive rows at G stencil offsets (R contiguous
+// values per offset) rather than 64 / G rows at G interleaved offsets.  A
+// window of more than C entries is staged in chunks; a lane's entries
+// continue in the next chunk in the same order.  `row_op(row, sum)` runs in
+// sub-lane 0 (lanes 0 .. R - 1) for every row below n_rows.  Only the wave
+// itself reads its staging area: LDS accesses of one wave complete in
+// order, and the wave barriers keep the compiler from moving them across
+// the staging.
 template <typename T, int G, int C, bool Compact, typename Idx, typename Gather,
           typename RowOp, typename TV>
 __device__ __forceinline__ void staged_window(int64_t n_rows, int64_t w,
@@ -165,8 +170,8 @@ __device__ __forceinline__ void staged_window(int64_t n_rows, int64_t w,
     constexpr int kRows = kWave / G;
     constexpr int U = 4;
     const int lane = threadIdx.x % kWave;
-    const int l = lane % G;
-    const int64_t row = w * kRows + lane / G;
+    const int l = lane / kRows;  // sub-lane
+    const int64_t row = w * kRows + lane % kRows;
     const int32_t rs = rp[row < n_rows ? row : n_rows];
     const int32_t re = rp[row < n_rows ? row + 1 : n_rows];
     const int32_t ws = __shfl(rs, 0, kWave);
@@ -225,7 +230,7 @@ __device__ __forceinline__ void staged_window(int64_t n_rows, int64_t w,
         }
         __builtin_amdgcn_wave_barrier();
     }
-    sum = group_sum<G>(sum);
+    sum = strided_sum<G>(sum);
     if (row < n_rows && l == 0) row_op(row, sum);
 }
 
