@@ -567,8 +567,12 @@ __global__ void __launch_bounds__(kBlock)
 // of stencil length): the check and p_out = z + rho/prev_rho p_in as in
 // cg_fused_direction_kernel, q = A p_out with p_out recomputed at the
 // gathered columns, and partials of p.q.  Workgroup tiles of one window
-// per wave.
-template <typename T, int G, bool Compact>
+// per wave.  With Pre (small systems: the launch gives every workgroup at
+// most one tile) the wave's window is staged and the lane's first entries'
+// z and p_in gathered, and its row's z and p_in loaded, before the wait for
+// the partials, so that only the arithmetic and the stores follow it; the
+// sums are the same.
+template <typename T, int G, bool Compact, bool Pre>
 __global__ void __launch_bounds__(kBlock)
     cg_fused_direction_staged_kernel(int64_t n, const int32_t* __restrict__ rp,
                                      const int32_t* __restrict__ ci,
@@ -581,11 +585,28 @@ __global__ void __launch_bounds__(kBlock)
                                      int count_in, int64_t iter, int par)
 {
     using Idx = std::conditional_t<Compact, int16_t, int32_t>;
+    constexpr int UB = 8;
     __shared__ T sv[kWavesPerBlock][kStagedChunk];
     __shared__ Idx sc[kWavesPerBlock][kStagedChunk];
     const int flag0 = threadIdx.x == 0 ? flags->all_stopped : 0;
     const T prev = scalars[par ^ 1];
     uint8_t st = w.status[par ^ 1];
+    const int wave = threadIdx.x / kWave;
+    int64_t rb0, rb1;
+    block_range(ceildiv(n, staged_tile_rows<G>), &rb0, &rb1);
+    staged_pf<T, UB> pf;
+    T z_row = T(0), p_row = T(0);
+    if constexpr (Pre) {
+        if (rb0 < rb1) {
+            staged_prefetch<T, G, kStagedChunk, Compact, UB>(
+                n, rb0 * kWavesPerBlock + wave, rp, ci, co, v, sv[wave], sc[wave],
+                [&](int32_t c) { return z[c]; }, [&](int32_t c) { return p_in[c]; }, pf);
+            if (pf.row < n) {
+                z_row = z[pf.row];
+                p_row = p_in[pf.row];
+            }
+        }
+    }
     double s[2];
     if (combine_partials_bcast<2>(w.part_upd, count_in, s, flag0)) return;
     const T rho = static_cast<T>(s[0]);
@@ -605,17 +626,29 @@ __global__ void __launch_bounds__(kBlock)
     const bool restart = prev == T(0);
     const T tmp = restart ? T(0) : rho / prev;
     T acc[1] = {T(0)};
-    auto gather = [&](int32_t c) { return restart ? z[c] : fma(tmp, p_in[c], z[c]); };
+    const auto dir = [&](T zc, T pc) { return restart ? zc : fma(tmp, pc, zc); };
+    auto gather = [&](int32_t c) { return dir(z[c], p_in[c]); };
     auto row_op = [&](int64_t row, T sum) {
-        const T pi = restart ? z[row] : fma(tmp, p_in[row], z[row]);
+        const T pi = dir(z[row], p_in[row]);
         p_out[row] = pi;
         q[row] = sum;
         acc[0] += pi * sum;
     };
-    const int wave = threadIdx.x / kWave;
-    int64_t rb0, rb1;
-    block_range(ceildiv(n, staged_tile_rows<G>), &rb0, &rb1);
-    for (int64_t blk = rb0; blk < rb1; ++blk) {
+    int64_t blk = rb0;
+    if constexpr (Pre) {
+        if (rb0 < rb1) {
+            staged_finish<T, G, kStagedChunk, Compact, UB>(
+                n, ci, co, v, sv[wave], sc[wave], pf, dir, gather,
+                [&](int64_t row, T sum) {
+                    const T pi = dir(z_row, p_row);
+                    p_out[row] = pi;
+                    q[row] = sum;
+                    acc[0] += pi * sum;
+                });
+            ++blk;
+        }
+    }
+    for (; blk < rb1; ++blk) {
         staged_window<T, G, kStagedChunk, Compact>(n, blk * kWavesPerBlock + wave, rp, ci, co,
                                                    v, sv[wave], sc[wave], gather, row_op);
     }
@@ -909,13 +942,21 @@ int cg_fused_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_
             GKOMI_CHECK_LAUNCH("cg_split_spmv_kernel");
             }
         } else if (g.staged > 0) {
+            // at most one tile per workgroup: the variant that stages its
+            // window before waiting for the partials
+            const bool pre = ceildiv(n, int64_t{kWave} / g.staged * kWavesPerBlock) <= g.spmv;
+#define GKOMI_STAGED_DIR_P(GV, C, P)                                                       \
+    cg_fused_direction_staged_kernel<T, GV, C, P><<<g.spmv, kBlock, 0, s>>>(               \
+        n, a->row_ptrs, a->col_idxs, a->col_offsets, v, static_cast<const T*>(st->z), pin, \
+        pout, static_cast<T*>(st->q), *crit, sc, static_cast<const T*>(st->orig_tau),      \
+        st->status, st->flags, w, g.update, iter, par);
 #define GKOMI_STAGED_DIR(GV, C)                                                            \
     if (g.staged == GV) {                                                                  \
-        cg_fused_direction_staged_kernel<T, GV, C><<<g.spmv, kBlock, 0, s>>>(              \
-            n, a->row_ptrs, a->col_idxs, a->col_offsets, v, static_cast<const T*>(st->z),  \
-            pin, pout, static_cast<T*>(st->q), *crit, sc,                                  \
-            static_cast<const T*>(st->orig_tau), st->status, st->flags, w, g.update, iter, \
-            par);                                                                          \
+        if (pre) {                                                                         \
+            GKOMI_STAGED_DIR_P(GV, C, true)                                                \
+        } else {                                                                           \
+            GKOMI_STAGED_DIR_P(GV, C, false)                                               \
+        }                                                                                  \
     }
             if (a->col_offsets) {
                 GKOMI_STAGED_DIR(1, true) GKOMI_STAGED_DIR(2, true) GKOMI_STAGED_DIR(4, true)
@@ -925,6 +966,7 @@ int cg_fused_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_
                 GKOMI_STAGED_DIR(8, false) GKOMI_STAGED_DIR(16, false)
             }
 #undef GKOMI_STAGED_DIR
+#undef GKOMI_STAGED_DIR_P
             GKOMI_CHECK_LAUNCH("cg_fused_direction_staged_kernel");
         } else {
 #define GKOMI_DIR_CASE_C(S, R, C)                                                          \

@@ -157,6 +157,74 @@ __device__ __forceinline__ void subwave_tile(int64_t n_rows, int64_t tile,
 // itself reads its staging area: LDS accesses of one wave complete in
 // order, and the wave barriers keep the compiler from moving them across
 // the staging.
+// Copies the entries [c0, c1) (c0 < c1 <= nnz, c1 - c0 <= C) into the
+// wave's staging area: every load one contiguous run of 64 entries, all
+// issued before the first LDS write.
+template <int C, bool Compact, typename Idx, typename TV>
+__device__ __forceinline__ void staged_copy(int32_t c0, int32_t c1,
+                                            const int32_t* __restrict__ ci,
+                                            const int16_t* __restrict__ co,
+                                            const TV* __restrict__ v, TV* lv, Idx* lc)
+{
+    constexpr int kSteps = C / kWave;
+    const int lane = threadIdx.x % kWave;
+    TV tv[kSteps];
+    Idx tc[kSteps];
+#pragma unroll
+    for (int u = 0; u < kSteps; ++u) {
+        const int32_t e = c0 + u * kWave + lane;
+        const int32_t ee = e < c1 ? e : c0;  // c0 < c1 <= nnz: a valid entry
+        tv[u] = v[ee];
+        if constexpr (Compact) {
+            tc[u] = co[ee];
+        } else {
+            tc[u] = ci[ee];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < kSteps; ++u) {
+        lv[u * kWave + lane] = tv[u];
+        lc[u * kWave + lane] = tc[u];
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Adds the lane's entries k, k + G, ... below z (the row's end or the
+// chunk's end) from the chunk staged at c0 to `sum`, in order, and advances
+// k past them (at most to the first one not taken, which may lie in the
+// next chunk; k never falls below c0: earlier chunks took those).
+template <typename T, int G, bool Compact, typename Idx, typename TV, typename Gather>
+__device__ __forceinline__ void staged_accumulate(int64_t row, int32_t c0, int32_t z,
+                                                  int32_t& k, T& sum, const TV* lv,
+                                                  const Idx* lc, Gather gather)
+{
+    constexpr int U = 4;
+    while (__ballot(k < z) != 0) {
+        int32_t col[U];
+        T val[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int32_t kk = k + u * G;
+            ok[u] = kk < z;
+            const int32_t idx = ok[u] ? kk - c0 : 0;
+            col[u] = Compact ? static_cast<int32_t>(row) + lc[idx] : lc[idx];
+            val[u] = static_cast<T>(lv[idx]);
+        }
+        T xv[U];
+        // a slot past the row's end gathers column 0 (always valid)
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = gather(ok[u] ? col[u] : 0);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            sum = fma(ok[u] ? val[u] : T(0), ok[u] ? xv[u] : T(0), sum);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) k = ok[u] ? k + G : k;
+    }
+}
+
 template <typename T, int G, int C, bool Compact, typename Idx, typename Gather,
           typename RowOp, typename TV>
 __device__ __forceinline__ void staged_window(int64_t n_rows, int64_t w,
@@ -166,9 +234,7 @@ __device__ __forceinline__ void staged_window(int64_t n_rows, int64_t w,
                                               const TV* __restrict__ v, TV* lv, Idx* lc,
                                               Gather gather, RowOp row_op)
 {
-    constexpr int kSteps = C / kWave;
     constexpr int kRows = kWave / G;
-    constexpr int U = 4;
     const int lane = threadIdx.x % kWave;
     const int l = lane / kRows;  // sub-lane
     const int64_t row = w * kRows + lane % kRows;
@@ -180,58 +246,107 @@ __device__ __forceinline__ void staged_window(int64_t n_rows, int64_t w,
     int32_t k = rs + l;  // the lane's next entry (absolute position)
     for (int32_t c0 = ws; c0 < we; c0 += C) {
         const int32_t c1 = c0 + C < we ? c0 + C : we;
-        TV tv[kSteps];
-        Idx tc[kSteps];
-#pragma unroll
-        for (int u = 0; u < kSteps; ++u) {
-            const int32_t e = c0 + u * kWave + lane;
-            const int32_t ee = e < c1 ? e : c0;  // c0 < we <= nnz: a valid entry
-            tv[u] = v[ee];
-            if constexpr (Compact) {
-                tc[u] = co[ee];
-            } else {
-                tc[u] = ci[ee];
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int u = 0; u < kSteps; ++u) {
-            lv[u * kWave + lane] = tv[u];
-            lc[u * kWave + lane] = tc[u];
-        }
-        __builtin_amdgcn_wave_barrier();
-        // the lane's entries k, k + G, ... below the row's end and the
-        // chunk's end (k never falls below c0: earlier chunks took them)
-        const int32_t z = re < c1 ? re : c1;
-        while (__ballot(k < z) != 0) {
-            int32_t col[U];
-            T val[U];
-            bool ok[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int32_t kk = k + u * G;
-                ok[u] = kk < z;
-                const int32_t idx = ok[u] ? kk - c0 : 0;
-                col[u] = Compact ? static_cast<int32_t>(row) + lc[idx] : lc[idx];
-                val[u] = static_cast<T>(lv[idx]);
-            }
-            T xv[U];
-            // a slot past the row's end gathers column 0 (always valid)
-#pragma unroll
-            for (int u = 0; u < U; ++u) xv[u] = gather(ok[u] ? col[u] : 0);
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                sum = fma(ok[u] ? val[u] : T(0), ok[u] ? xv[u] : T(0), sum);
-            }
-            // past the entries taken (at most to the first one not taken,
-            // which may lie in the next chunk)
-#pragma unroll
-            for (int u = 0; u < U; ++u) k = ok[u] ? k + G : k;
-        }
+        staged_copy<C, Compact>(c0, c1, ci, co, v, lv, lc);
+        staged_accumulate<T, G, Compact>(row, c0, re < c1 ? re : c1, k, sum, lv, lc, gather);
         __builtin_amdgcn_wave_barrier();
     }
     sum = strided_sum<G>(sum);
     if (row < n_rows && l == 0) row_op(row, sum);
+}
+
+// staged_window split in two around a wait for other data (the partial sums
+// of a fused solver), for a wave's first window.  staged_prefetch loads the
+// window's row pointers and, when the window fits one chunk, stages it and
+// gathers both operands of the lane's first UB entries with load_a(col) and
+// load_b(col); staged_finish adds fin(a, b) for those entries and then the
+// rest as staged_window does, in the same order: fin(load_a(c), load_b(c))
+// must equal gather(c), and the sums have the bits of staged_window's.
+template <typename T, int UB>
+struct staged_pf {
+    int64_t row;
+    int32_t re, ws, we, k;
+    bool one;
+    bool ok[UB];
+    T val[UB], a[UB], b[UB];
+};
+
+template <typename T, int G, int C, bool Compact, int UB, typename Idx, typename TV,
+          typename LoadA, typename LoadB>
+__device__ __forceinline__ void staged_prefetch(int64_t n_rows, int64_t w,
+                                                const int32_t* __restrict__ rp,
+                                                const int32_t* __restrict__ ci,
+                                                const int16_t* __restrict__ co,
+                                                const TV* __restrict__ v, TV* lv, Idx* lc,
+                                                LoadA load_a, LoadB load_b, staged_pf<T, UB>& pf)
+{
+    constexpr int kRows = kWave / G;
+    const int lane = threadIdx.x % kWave;
+    const int l = lane / kRows;
+    pf.row = w * kRows + lane % kRows;
+    const int32_t rs = rp[pf.row < n_rows ? pf.row : n_rows];
+    pf.re = rp[pf.row < n_rows ? pf.row + 1 : n_rows];
+    pf.ws = __shfl(rs, 0, kWave);
+    pf.we = __shfl(pf.re, kWave - 1, kWave);
+    pf.k = rs + l;
+    pf.one = pf.we - pf.ws <= C;
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+        pf.ok[u] = false;
+        pf.val[u] = pf.a[u] = pf.b[u] = T(0);
+    }
+    if (pf.one && pf.ws < pf.we) {
+        staged_copy<C, Compact>(pf.ws, pf.we, ci, co, v, lv, lc);
+        int32_t col[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int32_t kk = pf.k + u * G;
+            pf.ok[u] = kk < pf.re;
+            const int32_t idx = pf.ok[u] ? kk - pf.ws : 0;
+            col[u] = Compact ? static_cast<int32_t>(pf.row) + lc[idx] : lc[idx];
+            pf.val[u] = static_cast<T>(lv[idx]);
+        }
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int32_t c = pf.ok[u] ? col[u] : 0;
+            pf.a[u] = load_a(c);
+            pf.b[u] = load_b(c);
+        }
+    }
+}
+
+template <typename T, int G, int C, bool Compact, int UB, typename Idx, typename TV,
+          typename Fin, typename Gather, typename RowOp>
+__device__ __forceinline__ void staged_finish(int64_t n_rows, const int32_t* __restrict__ ci,
+                                              const int16_t* __restrict__ co,
+                                              const TV* __restrict__ v, TV* lv, Idx* lc,
+                                              staged_pf<T, UB>& pf, Fin fin, Gather gather,
+                                              RowOp row_op)
+{
+    constexpr int kRows = kWave / G;
+    const int l = (threadIdx.x % kWave) / kRows;
+    T sum = 0;
+    int32_t k = pf.k;
+    if (pf.one) {
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            sum = fma(pf.ok[u] ? pf.val[u] : T(0), pf.ok[u] ? fin(pf.a[u], pf.b[u]) : T(0), sum);
+        }
+#pragma unroll
+        for (int u = 0; u < UB; ++u) k = pf.ok[u] ? k + G : k;
+        // entries past the first UB, from the staged window
+        staged_accumulate<T, G, Compact>(pf.row, pf.ws, pf.re, k, sum, lv, lc, gather);
+        __builtin_amdgcn_wave_barrier();
+    } else {
+        for (int32_t c0 = pf.ws; c0 < pf.we; c0 += C) {
+            const int32_t c1 = c0 + C < pf.we ? c0 + C : pf.we;
+            staged_copy<C, Compact>(c0, c1, ci, co, v, lv, lc);
+            staged_accumulate<T, G, Compact>(pf.row, c0, pf.re < c1 ? pf.re : c1, k, sum, lv, lc,
+                                             gather);
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    sum = strided_sum<G>(sum);
+    if (pf.row < n_rows && l == 0) row_op(pf.row, sum);
 }
 
 // Rows of one workgroup tile of the staged product (one window per wave).
