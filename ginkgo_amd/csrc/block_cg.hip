@@ -118,11 +118,67 @@ __device__ __forceinline__ void combine_columns(const double* __restrict__ P, in
     }
 }
 
+// Column sums of the partials P[g][C] (g < blocks) spread over workgroups of
+// kColSumThreads threads: workgroup b sums the columns [8 b, 8 b + 8), its
+// thread t column 8 b + t % 8 over the rows g = t / 8 + 32 (4 m + u) into
+// accumulator u, (a0 + a1) + (a2 + a3), then the 32 row groups in order
+// through LDS: a fixed order.  One workgroup reading all the partials was
+// bound by that CU's bandwidth (1024 x 128 partials, written from every XCD:
+// 26 us); here each reads 1/16 of them.  With `beta` the sums are also the
+// step_1 factors (the block CG's beta kernel).
+constexpr int kColSumThreads = 256;
+constexpr int kColSumCols = 8;
+
+template <typename T>
+__global__ void __launch_bounds__(kColSumThreads)
+    bcg_colsum_kernel(const double* __restrict__ P, int blocks, int64_t C,
+                      double* __restrict__ out, T* __restrict__ beta,
+                      const gkomi_stop_flags* __restrict__ flags)
+{
+    if (flags->all_stopped) return;
+    constexpr int kRowGroups = kColSumThreads / kColSumCols;
+    __shared__ double lds[kColSumThreads];
+    const int cl = threadIdx.x % kColSumCols;
+    const int g0 = threadIdx.x / kColSumCols;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kColSumCols + cl;
+    double acc = 0;
+    if (c < C) {
+        double a[4] = {0, 0, 0, 0};
+        for (int g = g0; g < blocks; g += 4 * kRowGroups) {
+            // all four loads issued before the adds; a slot past the last
+            // block rereads row g and adds +0 (the accumulators start at +0,
+            // so they never hold -0 and +0 leaves them be)
+            double t[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int gg = g + u * kRowGroups;
+                t[u] = P[static_cast<int64_t>(gg < blocks ? gg : g) * C + c];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] += g + u * kRowGroups < blocks ? t[u] : 0.0;
+        }
+        acc = (a[0] + a[1]) + (a[2] + a[3]);
+    }
+    lds[threadIdx.x] = acc;
+    __syncthreads();
+    if (g0 == 0 && c < C) {
+        double sum = lds[cl];
+        for (int r = 1; r < kRowGroups; ++r) sum += lds[r * kColSumCols + cl];
+        out[c] = sum;
+        if (beta != nullptr) beta[c] = static_cast<T>(sum);
+    }
+}
+
+inline unsigned colsum_grid(int64_t C)
+{
+    return static_cast<unsigned>(std::max<int64_t>(1, ceildiv(C, kColSumCols)));
+}
+
 // Row i of A times a gathered vector, kRowBatch entries at a time with all
 // loads of a batch issued before their use (lanes past the row end reuse the
 // row's first entry and add nothing).  Entry order within the row is the
 // CSR order, one fused multiply-add each.
-constexpr int kRowBatch = 8;
+constexpr int kRowBatch = 6;
 
 template <typename T, typename Gather>
 __device__ __forceinline__ T row_product(const int32_t* __restrict__ ci,
@@ -240,8 +296,10 @@ __device__ __forceinline__ void rows_product(const int32_t* __restrict__ ci,
     }
 }
 
-// rows per lane group in flight in the product kernel
-constexpr int kBcgRows = 2;
+// rows per lane group in flight in the product kernel: 3 rows of 6 slots
+// (18 gathers in flight per wave on 5-point rows at 115 VGPRs, against 10
+// with 2 rows of 8 at 122: 512^2, 64 columns 119.6 -> 110.0 us)
+constexpr int kBcgRows = 3;
 
 // ---------------------------------------------------------------- start ----
 // r = b - A x; p_a = p_b = q = 0; partials of b_j.b_j and r_j.r_j.
@@ -318,19 +376,17 @@ __global__ void __launch_bounds__(kCombineThreads)
 }
 
 // ----------------------------------------------------------- iteration ----
-// One workgroup: sums of the update partials, the check of iteration `iter`,
-// and the step_1 factor.
+// One workgroup: the check of iteration `iter` from the column sums of the
+// update partials (bcg_colsum_kernel has formed them), and the step_1
+// factor.
 template <typename T>
 __global__ void __launch_bounds__(kCombineThreads)
-    bcg_check_kernel(int64_t k, int blocks, gkomi_criteria crit, bcg_work w,
-                     T* __restrict__ scalars, const T* __restrict__ orig,
-                     uint8_t* __restrict__ status, gkomi_stop_flags* __restrict__ flags,
-                     int64_t iter)
+    bcg_check_kernel(int64_t k, gkomi_criteria crit, bcg_work w, T* __restrict__ scalars,
+                     const T* __restrict__ orig, uint8_t* __restrict__ status,
+                     gkomi_stop_flags* __restrict__ flags, int64_t iter)
 {
     if (flags->all_stopped) return;
-    __shared__ double lds[kCombineWaves][kWave];
     const bcg_scalars<T> sc = bcg_scalar_layout<T>(scalars, k);
-    combine_columns(w.part_a, blocks, 2 * k, w.sums, lds);
     for (int64_t j = threadIdx.x; j < k; j += kCombineThreads) {
         sc.rho[j] = static_cast<T>(w.sums[j]);
         sc.tau[j] = static_cast<T>(sqrt(w.sums[k + j]));
@@ -532,20 +588,6 @@ __global__ void __launch_bounds__(kBlock)
             const T r1 = mode == 1 ? zv : fm;
             p_out[o] = mode == 0 ? pv : r1;
         }
-    }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(kCombineThreads)
-    bcg_beta_kernel(int64_t k, int blocks, bcg_work w, T* __restrict__ scalars,
-                    const gkomi_stop_flags* __restrict__ flags)
-{
-    if (flags->all_stopped) return;
-    __shared__ double lds[kCombineWaves][kWave];
-    const bcg_scalars<T> sc = bcg_scalar_layout<T>(scalars, k);
-    combine_columns(w.part_b, blocks, k, w.sums, lds);
-    for (int64_t j = threadIdx.x; j < k; j += kCombineThreads) {
-        sc.beta[j] = static_cast<T>(w.sums[j]);
     }
 }
 
@@ -754,10 +796,12 @@ int bcg_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_crite
     for (int32_t it = 0; it < count; ++it) {
         T* pin = static_cast<T*>(st->p_a);
         T* pout = static_cast<T*>(st->p_b);
-        bcg_check_kernel<T><<<1, kCombineThreads, 0, s>>>(k, grid, *crit, w, sc,
+        bcg_colsum_kernel<T><<<colsum_grid(2 * k), kColSumThreads, 0, s>>>(
+            w.part_a, grid, 2 * k, w.sums, nullptr, st->flags);
+        bcg_check_kernel<T><<<1, kCombineThreads, 0, s>>>(k, *crit, w, sc,
                                                  static_cast<const T*>(st->orig_tau),
                                                  st->status, st->flags, st->issued);
-        GKOMI_CHECK_LAUNCH("bcg_check_kernel");
+        GKOMI_CHECK_LAUNCH("bcg_colsum_kernel, bcg_check_kernel");
 #define GKOMI_BCG_PROD(L, OFF)                                                             \
     dgrid = bcg_product_grid<T, L, OFF>(grid);                                             \
     bcg_product_kernel<T, L, OFF><<<dgrid, kBlock, 0, s>>>(                                \
@@ -776,8 +820,9 @@ int bcg_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_crite
 #undef GKOMI_BCG_DIR
 #undef GKOMI_BCG_PROD
         GKOMI_CHECK_LAUNCH("bcg_direction_kernel, bcg_product_kernel");
-        bcg_beta_kernel<T><<<1, kCombineThreads, 0, s>>>(k, dgrid, w, sc, st->flags);
-        GKOMI_CHECK_LAUNCH("bcg_beta_kernel");
+        bcg_colsum_kernel<T><<<colsum_grid(k), kColSumThreads, 0, s>>>(
+            w.part_b, dgrid, k, w.sums, bcg_scalar_layout<T>(sc, k).beta, st->flags);
+        GKOMI_CHECK_LAUNCH("bcg_colsum_kernel (beta)");
 #define GKOMI_BCG_UPD(L)                                                                   \
     if (inv) {                                                                             \
         bcg_update_kernel<T, L, true, true><<<grid, kBlock, 0, s>>>(                       \

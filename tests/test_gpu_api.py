@@ -248,6 +248,28 @@ def test_cg_many_rhs_block_path(gko, exec_, k, jacobi, dt):
     assert np.all(res <= tol * (2 if dt == "f64" else 20))
 
 
+def test_cg_many_rhs_block_path_long_rows(gko, exec_):
+    """36 right-hand sides (one wave per row in the product) on a 27-point
+    matrix: rows of 8..27 entries take several rounds of gathers per row.
+    Iterations, per-column status and x against the oracle's cg.cpp."""
+    rp, ci, v = gen.diffusion27(9)
+    n, k = len(rp) - 1, 36
+    rng = np.random.default_rng(36)
+    b_h = rng.uniform(-1, 1, (n, k))
+    spec = [{"kind": "iteration", "max_iters": 1000},
+            {"kind": "residual_norm", "reduction_factor": 1e-10}]
+    xo, so, io = ora.cg_solve(rp, ci, v, b_h, np.zeros_like(b_h), spec)
+    A = gko.csr_from_host(exec_, rp, ci, v)
+    s = gko.solver.Cg.build("f64").with_criteria(*crits(gko, spec, "f64")).on(exec_).generate(A)
+    x = gko.matrix.Dense("f64").create(exec_, n, k)
+    x.fill(0.0)
+    s.apply(dense(gko, exec_, b_h, "f64"), x)
+    assert s.last_used_fused_path
+    assert abs(s.last_iterations - io.iterations) <= 3
+    assert s.last_status == [int(c) for c in so]
+    np.testing.assert_allclose(x.to_numpy(), xo, atol=1e-8)
+
+
 def test_cg_many_rhs_block_path_4gib_vectors(gko, exec_):
     """k-RHS CG whose n x k blocks span 2^32 bytes (2^24 rows, 64 FP32
     columns), so the direction kernel indexes them with 64-bit offsets.
