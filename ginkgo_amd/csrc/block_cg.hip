@@ -392,9 +392,10 @@ __global__ void __launch_bounds__(kCombineThreads)
         sc.tau[j] = static_cast<T>(sqrt(w.sums[k + j]));
     }
     __syncthreads();
+    int all = 0, changed = 0;
+    combined_check_block<kCombineThreads, T>(crit, iter, k, sc.tau, sc.rho, orig, true, status,
+                                             &all, &changed);
     if (threadIdx.x == 0) {
-        int all = 0, changed = 0;
-        combined_check<T>(crit, iter, k, sc.tau, sc.rho, orig, true, status, &all, &changed);
         flags->iteration = iter;
         flags->all_stopped = all;
         flags->one_changed = changed;
