@@ -579,17 +579,38 @@ __global__ void __launch_bounds__(kBlock)
             mode = prev == T(0) ? 1 : 2;
             f = mode == 2 ? sc.rho[j] / prev : T(0);
         }
-        for (int64_t blk = rb0; blk < rb1; ++blk) {
-            const int64_t i = blk * kRowsPerBlock + g;
-            if (i >= n) continue;
-            const int64_t o = i * k + j;
-            const T pv = p_in[o];
-            const T zv = z[o];
-            const T fm = fma(f, pv, zv);
-            const T r1 = mode == 1 ? zv : fm;
-            p_out[o] = mode == 0 ? pv : r1;
+        // four rows per step, all eight loads issued first (a row past the
+        // end reads row 0 and stores nothing)
+        constexpr int U = 4;
+        for (int64_t blk = rb0; blk < rb1; blk += U) {
+            int64_t o[U];
+            bool ok[U];
+            T pv[U], zv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t i = (blk + u) * kRowsPerBlock + g;
+                ok[u] = blk + u < rb1 && i < n;
+                o[u] = ok[u] ? i * k + j : j;
+                pv[u] = __builtin_nontemporal_load(p_in + o[u]);
+                zv[u] = __builtin_nontemporal_load(z + o[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const T fm = fma(f, pv[u], zv[u]);
+                const T r1 = mode == 1 ? zv[u] : fm;
+                if (ok[u]) __builtin_nontemporal_store(mode == 0 ? pv[u] : r1, p_out + o[u]);
+            }
         }
     }
+}
+
+// Grid of the direction kernel (no partials, so not bound by their
+// capacity): up to 8 workgroups per CU.
+inline int bcg_direction_grid(int64_t n, int lanes)
+{
+    const int64_t tiles = ceildiv(std::max<int64_t>(n, 1), kBlock / lanes);
+    const int64_t g = std::min<int64_t>(tiles, static_cast<int64_t>(num_cus()) * 8);
+    return static_cast<int>(std::max<int64_t>(g, 1));
 }
 
 // Update (and, with Update = false, the tail of the start): x, r update of
@@ -808,8 +829,8 @@ int bcg_iterate_impl(const gkomi_ctx* ctx, const gkomi_csr* a, const gkomi_crite
     bcg_product_kernel<T, L, OFF><<<dgrid, kBlock, 0, s>>>(                                \
         n, k, a->row_ptrs, a->col_idxs, v, pout, static_cast<T*>(st->q), st->flags, w);
 #define GKOMI_BCG_DIR(L)                                                                   \
-    bcg_direction_kernel<T, L><<<grid, kBlock, 0, s>>>(n, k, z, pin, pout, sc, st->status, \
-                                                       st->flags);                         \
+    bcg_direction_kernel<T, L><<<bcg_direction_grid(n, L), kBlock, 0, s>>>(                 \
+        n, k, z, pin, pout, sc, st->status, st->flags);                                    \
     if (off32) {                                                                           \
         GKOMI_BCG_PROD(L, true)                                                            \
     } else {                                                                               \
