@@ -505,10 +505,19 @@ __global__ void __launch_bounds__(kUpdBlock)
                       int64_t iter, int par)
 {
     // as in the direction kernel: flag, rho_prev and status ride with the
-    // partials through one barrier
+    // partials through one barrier, and so do the loads of this thread's
+    // first z and p_in (p_in is never used on a restart, whatever it holds)
     const int flag0 = threadIdx.x == 0 ? flags->all_stopped : 0;
     const T prev = scalars[par ^ 1];
     uint8_t st = w.status[par ^ 1];
+    int64_t i0, i1;
+    block_range(ceildiv(n, kUpdBlock), &i0, &i1);
+    const int64_t ia = i0 * kUpdBlock + threadIdx.x;
+    T za = T(0), pa = T(0);
+    if (i0 < i1 && ia < n) {
+        za = z[ia];
+        pa = p_in[ia];
+    }
     double s[2];
     if (combine_partials_bcast<2, kUpdBlock>(w.part_upd, count_in, s, flag0)) return;
     const T rho = static_cast<T>(s[0]);
@@ -527,11 +536,12 @@ __global__ void __launch_bounds__(kUpdBlock)
     if (all) return;
     const bool restart = prev == T(0);
     const T tmp = restart ? T(0) : rho / prev;
-    int64_t i0, i1;
-    block_range(ceildiv(n, kUpdBlock), &i0, &i1);
     for (int64_t blk = i0; blk < i1; ++blk) {
         const int64_t i = blk * kUpdBlock + threadIdx.x;
-        if (i < n) p_out[i] = restart ? z[i] : fma(tmp, p_in[i], z[i]);
+        if (i >= n) continue;
+        const bool first = blk == i0;
+        const T zi = first ? za : z[i];
+        p_out[i] = restart ? zi : fma(tmp, first ? pa : p_in[i], zi);
     }
 }
 
