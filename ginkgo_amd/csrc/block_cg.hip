@@ -18,6 +18,7 @@
 // Sums are FP64 partials per workgroup added in fixed order by a
 // one-workgroup kernel: deterministic and free of device-scope fences.
 #include "common.hpp"
+#include "reduce.hpp"
 #include "spmv_common.hpp"
 #include "stop.hpp"
 
@@ -118,17 +119,9 @@ __device__ __forceinline__ void combine_columns(const double* __restrict__ P, in
     }
 }
 
-// Column sums of the partials P[g][C] (g < blocks) spread over workgroups of
-// kColSumThreads threads: workgroup b sums the columns [8 b, 8 b + 8), its
-// thread t column 8 b + t % 8 over the rows g = t / 8 + 32 (4 m + u) into
-// accumulator u, (a0 + a1) + (a2 + a3), then the 32 row groups in order
-// through LDS: a fixed order.  One workgroup reading all the partials was
-// bound by that CU's bandwidth (1024 x 128 partials, written from every XCD:
-// 26 us); here each reads 1/16 of them.  With `beta` the sums are also the
-// step_1 factors (the block CG's beta kernel).
-constexpr int kColSumThreads = 256;
-constexpr int kColSumCols = 8;
-
+// Column sums of the partials (column_sums_block, reduce.hpp) for the
+// one-workgroup check: a fixed order, one workgroup per 8 columns.  With
+// `beta` the sums are also the step_1 factors (p.q per column).
 template <typename T>
 __global__ void __launch_bounds__(kColSumThreads)
     bcg_colsum_kernel(const double* __restrict__ P, int blocks, int64_t C,
@@ -136,42 +129,10 @@ __global__ void __launch_bounds__(kColSumThreads)
                       const gkomi_stop_flags* __restrict__ flags)
 {
     if (flags->all_stopped) return;
-    constexpr int kRowGroups = kColSumThreads / kColSumCols;
-    __shared__ double lds[kColSumThreads];
-    const int cl = threadIdx.x % kColSumCols;
-    const int g0 = threadIdx.x / kColSumCols;
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * kColSumCols + cl;
-    double acc = 0;
-    if (c < C) {
-        double a[4] = {0, 0, 0, 0};
-        for (int g = g0; g < blocks; g += 4 * kRowGroups) {
-            // all four loads issued before the adds; a slot past the last
-            // block rereads row g and adds +0 (the accumulators start at +0,
-            // so they never hold -0 and +0 leaves them be)
-            double t[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int gg = g + u * kRowGroups;
-                t[u] = P[static_cast<int64_t>(gg < blocks ? gg : g) * C + c];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) a[u] += g + u * kRowGroups < blocks ? t[u] : 0.0;
-        }
-        acc = (a[0] + a[1]) + (a[2] + a[3]);
-    }
-    lds[threadIdx.x] = acc;
-    __syncthreads();
-    if (g0 == 0 && c < C) {
-        double sum = lds[cl];
-        for (int r = 1; r < kRowGroups; ++r) sum += lds[r * kColSumCols + cl];
+    column_sums_block(P, blocks, C, [&](int64_t c, double sum) {
         out[c] = sum;
         if (beta != nullptr) beta[c] = static_cast<T>(sum);
-    }
-}
-
-inline unsigned colsum_grid(int64_t C)
-{
-    return static_cast<unsigned>(std::max<int64_t>(1, ceildiv(C, kColSumCols)));
+    });
 }
 
 // Row i of A times a gathered vector, kRowBatch entries at a time with all

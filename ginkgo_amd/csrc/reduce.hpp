@@ -8,6 +8,8 @@
 // cost more than the reduction; the kernel boundary orders the two steps.
 #pragma once
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace gkomi {
@@ -67,8 +69,73 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// Column sums of partials P[g][C] (g < blocks) spread over workgroups of
+// kColSumThreads threads: workgroup b sums the columns [8 b, 8 b + 8), its
+// thread t column 8 b + t % 8 over the rows g = t / 8 + 32 (4 m + u) into
+// accumulator u, (a0 + a1) + (a2 + a3), then the 32 row groups in order
+// through LDS: a fixed order.  fn(c, sum) runs once per column c < C.  One
+// workgroup reading all the partials is bound by that CU's bandwidth (the
+// partials come from every XCD: 1024 x 64 took 42 us); here each workgroup
+// reads 1/(C/8) of them.  Every thread of the workgroup must call it.
+constexpr int kColSumThreads = 256;
+constexpr int kColSumCols = 8;
+
+template <typename Fn>
+__device__ __forceinline__ void column_sums_block(const double* __restrict__ P, int64_t blocks,
+                                                  int64_t C, Fn fn)
+{
+    constexpr int kRowGroups = kColSumThreads / kColSumCols;
+    __shared__ double lds[kColSumThreads];
+    const int cl = threadIdx.x % kColSumCols;
+    const int g0 = threadIdx.x / kColSumCols;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kColSumCols + cl;
+    double acc = 0;
+    if (c < C) {
+        double a[4] = {0, 0, 0, 0};
+        for (int64_t g = g0; g < blocks; g += 4 * kRowGroups) {
+            // all four loads issued before the adds; a slot past the last
+            // row rereads row g and adds +0 (the accumulators start at +0,
+            // so they never hold -0 and +0 leaves them be)
+            double t[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t gg = g + u * kRowGroups;
+                t[u] = P[(gg < blocks ? gg : g) * C + c];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] += g + u * kRowGroups < blocks ? t[u] : 0.0;
+        }
+        acc = (a[0] + a[1]) + (a[2] + a[3]);
+    }
+    lds[threadIdx.x] = acc;
+    __syncthreads();
+    if (g0 == 0 && c < C) {
+        double sum = lds[cl];
+        for (int r = 1; r < kRowGroups; ++r) sum += lds[r * kColSumCols + cl];
+        fn(c, sum);
+    }
+}
+
+inline unsigned colsum_grid(int64_t C)
+{
+    return static_cast<unsigned>(std::max<int64_t>(1, ceildiv(C, kColSumCols)));
+}
+
+// result[c] = sum of column c of the partials (sqrt for norms), several
+// columns: column_sums_block.
+template <typename T, bool Norm>
+__global__ void __launch_bounds__(kColSumThreads)
+    reduce_colsum_kernel(int64_t blocks, int64_t nrhs, const double* __restrict__ partials,
+                         T* __restrict__ result)
+{
+    column_sums_block(partials, blocks, nrhs, [&](int64_t c, double s) {
+        result[c] = static_cast<T>(Norm ? sqrt(s) : s);
+    });
+}
+
 // One workgroup: result[c] = sum over `blocks` partials of column c, in
-// workgroup order per wave, then the waves in order (sqrt for norms).
+// workgroup order per wave, then the waves in order (sqrt for norms); the
+// single-column reductions.
 template <typename T, bool Norm>
 __global__ void __launch_bounds__(kBlock)
     reduce_combine_kernel(int64_t blocks, int64_t nrhs, const double* __restrict__ partials,
@@ -127,7 +194,12 @@ int launch_column_reduction(hipStream_t stream, int64_t n, int64_t nrhs, const T
             n, nrhs, x, xs, y, ys, partials);
     }
     GKOMI_CHECK_LAUNCH("column reduction");
-    reduce_combine_kernel<T, Norm><<<1, kBlock, 0, stream>>>(g, nrhs, partials, result);
+    if (nrhs == 1) {
+        reduce_combine_kernel<T, Norm><<<1, kBlock, 0, stream>>>(g, nrhs, partials, result);
+    } else {
+        reduce_colsum_kernel<T, Norm><<<colsum_grid(nrhs), kColSumThreads, 0, stream>>>(
+            g, nrhs, partials, result);
+    }
     GKOMI_CHECK_LAUNCH("column reduction combine");
     return GKOMI_SUCCESS;
 }
