@@ -284,9 +284,24 @@ __global__ void __launch_bounds__(kBlock)
         for (int64_t j0 = 0; j0 < nrhs; j0 += kWave) {
             const int64_t j = j0 + lane;
             if (j < nrhs) {
+                // entries in batches of eight, every load of a batch issued
+                // before its first product (the batch bounds are the same in
+                // the whole wave); the products are added in stored order
+                constexpr int B = 8;
                 T sum = 0;
-                for (int32_t k = s; k < e; ++k) {
-                    sum += v[k] * b[static_cast<int64_t>(ci[k]) * bs + j];
+                for (int32_t k0 = s; k0 < e; k0 += B) {
+                    T vv[B], bv[B];
+#pragma unroll
+                    for (int u = 0; u < B; ++u) {
+                        if (k0 + u < e) {
+                            vv[u] = v[k0 + u];
+                            bv[u] = b[static_cast<int64_t>(ci[k0 + u]) * bs + j];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < B; ++u) {
+                        if (k0 + u < e) sum += vv[u] * bv[u];
+                    }
                 }
                 store_result<Mode>(c + row * cs + j, sum, alpha, beta);
             }
