@@ -253,6 +253,28 @@ def test_spmv_many_rhs_and_advanced(dev, nrhs):
     np.testing.assert_allclose(c.cpu().numpy(), ref, rtol=0, atol=1e-13)
 
 
+def test_spmm_long_rows_and_strided_input(dev):
+    """36 right-hand sides (one wave per row, lane = column) on rows of 0 to
+    3000 entries, empty rows included: rows longer than one batch of eight
+    gathers take several batches; the input block is a column slice of a
+    wider array (row stride 40)."""
+    rp, ci, v = irregular(3000, seed=11)
+    n = len(rp) - 1
+    rng = np.random.default_rng(36)
+    xw = rng.uniform(-1, 1, (n, 40))
+    xd = to_dev(xw, dev)[:, 2:38]
+    x = xw[:, 2:38].copy()
+    A = D().DeviceCsr(rp, ci, v, device=dev)
+    y = A.spmv(xd).cpu().numpy()
+    check_spmv(rp, ci, v, x, y, np.float64)
+    c0 = rng.uniform(-1, 1, (n, 36))
+    c = to_dev(c0, dev)
+    A.advanced_spmv(1.5, xd, -1.0, c)
+    ref = ora.csr_advanced_spmv(1.5, rp, ci, v, x, -1.0, c0)
+    tol = 2 * spmv_bound(rp, ci, v, x, np.float64) * 1.5 + 4 * np.finfo(np.float64).eps * np.abs(c0)
+    assert np.all(np.abs(c.cpu().numpy() - ref) <= tol)
+
+
 def test_spmv_rectangular_and_strided(dev):
     rng = np.random.default_rng(8)
     d = rng.uniform(-1, 1, (300, 170)) * (rng.random((300, 170)) < 0.05)
