@@ -126,10 +126,12 @@ def test_spmv_staged_every_group(dev, dtype, case):
     """The staged kernel (windows of 64 / G rows, their entries copied to LDS
     with contiguous loads, G lanes per row): windows whose entries exceed the
     512-entry staging chunk (rows of 100..3000 entries cross chunks), empty
-    rows, a last partial window.  Lane l of a row adds entries l, l + G, ...
-    and the group tree adds the lanes, so the product equals the sub-wave
-    shape with G lanes per row bit for bit, with the indices or the 2-byte
-    offsets; the automatic classical choice for short rows is that kernel."""
+    rows, a last partial window.  Sub-lane s of a row (lane r + s 64/G)
+    adds entries s, s + G, ... and a butterfly with lane strides 64/G, ...
+    adds the sub-lanes as the group tree pairs lanes, so the product equals
+    the sub-wave shape with G lanes per row bit for bit, with the indices or
+    the 2-byte offsets; the automatic classical choice for short rows is that
+    kernel."""
     if case == "irregular":
         rp, ci, v = irregular(3000, seed=7)
     elif case == "stencil7":
