@@ -680,9 +680,10 @@ __global__ void __launch_bounds__(kBlock)
     if (flags->all_stopped) return;
     T acc[1] = {T(0)};
     const int wave = threadIdx.x / kWave;
-    int64_t rb0, rb1;
-    block_range(ceildiv(n, staged_tile_rows<G>), &rb0, &rb1);
-    for (int64_t blk = rb0; blk < rb1; ++blk) {
+    // tiles in grid-stride order (XCD-aware start): the resident waves move
+    // through the matrix together
+    const int64_t tiles = ceildiv(n, staged_tile_rows<G>);
+    for (int64_t blk = xcd_block(blockIdx.x, gridDim.x); blk < tiles; blk += gridDim.x) {
         staged_window<T, G, kStagedChunk, Compact>(
             n, blk * kWavesPerBlock + wave, rp, ci, co, v, sv[wave], sc[wave],
             [&](int32_t c) { return p[c]; },
