@@ -591,8 +591,6 @@ __global__ void __launch_bounds__(kBlock)
     constexpr int kRowsPerBlock = kBlock / L;
     const int lane = threadIdx.x % L;
     const int g = threadIdx.x / L;
-    int64_t rb0, rb1;
-    block_range(ceildiv(n, kRowsPerBlock), &rb0, &rb1);
     for (int64_t c0 = 0; c0 < k; c0 += L) {
         const int64_t j = c0 + lane;
         const bool col_ok = j < k;
@@ -611,14 +609,19 @@ __global__ void __launch_bounds__(kBlock)
         // solve on a 512^2 stencil with 64 columns); r and z stay cached for
         // the direction kernel (non-temporal stores of them cost it 29%).
         constexpr int U = 4;
-        for (int64_t blk = rb0; blk < rb1; blk += U) {
+        // row tiles in grid-stride order (XCD-aware start), U at a time:
+        // 125.5 -> 120.2 us on the 512^2 adjoint against a contiguous range
+        // of tiles per workgroup (the direction kernel measured no gain)
+        const int64_t tiles = ceildiv(n, kRowsPerBlock);
+        const int64_t nb = gridDim.x;
+        for (int64_t blk = xcd_block(blockIdx.x, gridDim.x); blk < tiles; blk += U * nb) {
             int64_t o[U], ox[U], id[U];
             bool ok[U];
             T rv[U], pv[U], xv[U], qv[U], dv[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int64_t i = (blk + u) * kRowsPerBlock + g;
-                ok[u] = blk + u < rb1 && i < n && col_ok;
+                const int64_t i = (blk + u * nb) * kRowsPerBlock + g;
+                ok[u] = blk + u * nb < tiles && i < n && col_ok;
                 o[u] = ok[u] ? i * k + j : 0;
                 ox[u] = ok[u] ? i * xs + j : 0;
                 id[u] = ok[u] ? i : 0;
