@@ -558,7 +558,7 @@ subwave_shape pick_subwave(const gkomi_csr* a, int value_bytes);
 
 // Entries of a wave's LDS staging area in the staged kernel
 // (csr_spmv_staged): 64 / G rows of up to 8 G entries in one chunk.
-constexpr int kStagedChunk = 512;
+constexpr int kStagedChunk = 448;
 int staged_lanes(const gkomi_csr* a, int value_bytes);
 
 

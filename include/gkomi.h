@@ -182,8 +182,8 @@ enum { GKOMI_CSR_CLASSICAL = 0, GKOMI_CSR_LOAD_BALANCE = 1, GKOMI_CSR_MERGE_PATH
  * per row, each wave staging its 64 / G rows' entries in LDS with contiguous
  * loads (the per-row summation order of S = G, bit for bit).  `subwave` = 0
  * takes it for a single right-hand side with unit stride with the smallest G
- * for which 64 / G mean rows hold at most 512 entries, if G <= 4 (FP64) or
- * G <= 8 (FP32): rows of up to 32 / 64 entries on average.  Other
+ * for which 64 / G mean rows hold at most 448 entries (the staging chunk), if
+ * G <= 4 (FP64) or G <= 8 (FP32): rows of up to 28 / 56 entries on average.  Other
  * right-hand-side shapes use the sub-wavefront kernel; a G outside the list
  * is GKOMI_INVALID_ARGUMENT. */
 enum { GKOMI_CSR_STAGED = 0x10000 };

@@ -115,7 +115,7 @@ STAGED = 0x10000     # GKOMI_CSR_STAGED
 def auto_staged_lanes(rp, dtype):
     n, nnz = len(rp) - 1, int(rp[-1])
     for g in ((1, 2, 4) if dtype == np.float64 else (1, 2, 4, 8)):
-        if (64 // g) * nnz <= 512 * n:
+        if (64 // g) * nnz <= 448 * n:
             return g
     return 0
 
@@ -125,7 +125,7 @@ def auto_staged_lanes(rp, dtype):
 def test_spmv_staged_every_group(dev, dtype, case):
     """The staged kernel (windows of 64 / G rows, their entries copied to LDS
     with contiguous loads, G lanes per row): windows whose entries exceed the
-    512-entry staging chunk (rows of 100..3000 entries cross chunks), empty
+    448-entry staging chunk (rows of 100..3000 entries cross chunks), empty
     rows, a last partial window.  Sub-lane s of a row (lane r + s 64/G)
     adds entries s, s + G, ... and a butterfly with lane strides 64/G, ...
     adds the sub-lanes as the group tree pairs lanes, so the product equals
